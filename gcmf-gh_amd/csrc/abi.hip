@@ -1,0 +1,318 @@
+// abi.hip -- extern "C" boundary of libgcmf_hip.so (include/gcmf_hip.h):
+// contexts, device memory helpers, kernel timers, and the batched join / clip
+// entry points.  Library exceptions become negative status codes here.
+#include <cstring>
+#include <string>
+
+#include "../../include/gcmf_hip.h"
+#include "gcmf_internal.hpp"
+#include "gcmf_join.hpp"
+
+struct gcmf_ctx {
+  gcmf::Context c;
+  bool join_valid = false;
+  gcmf::JoinInputs jin;
+  long K = 0;
+  int *pairs = nullptr, *counts = nullptr;
+  gcmf::MbrSortView sv;
+  gcmf::JoinRefine jr;
+  bool clip_valid = false;
+  gcmf::ClipResult cr;
+  gcmf::Phase1View p1;
+};
+
+namespace {
+
+template <typename F>
+int guarded(gcmf_ctx *ctx, F &&f) {
+  if (!ctx) return GCMF_ERR_ARG;
+  try {
+    HIP_CHECK(hipSetDevice(ctx->c.device));
+    f();
+    return GCMF_OK;
+  } catch (const gcmf::Error &e) {
+    ctx->c.last_error = e.what;
+    return e.code;
+  } catch (const std::exception &e) {
+    ctx->c.last_error = e.what();
+    return GCMF_ERR_ALLOC;
+  }
+}
+
+gcmf::JoinInputs to_inputs(const gcmf_dlayer *b, const gcmf_dlayer *o) {
+  gcmf::JoinInputs in;
+  in.B = b->npoly;
+  in.O = o->npoly;
+  in.bCoords = b->d_coords;
+  in.oCoords = o->d_coords;
+  in.bVNum = b->d_vnum;
+  in.oVNum = o->d_vnum;
+  in.bVPSNum = b->d_vpsnum;
+  in.oVPSNum = o->d_vpsnum;
+  in.bMBR = b->d_mbr;
+  in.oMBR = o->d_mbr;
+  in.bMBR2 = b->d_mbr2;
+  in.oMBR2 = o->d_mbr2;
+  return in;
+}
+
+void run_join(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+              gcmf_join_stats *st) {
+  if (!base || !over) throw gcmf::Error{GCMF_ERR_ARG, "null layer"};
+  ctx->join_valid = false;
+  ctx->jin = to_inputs(base, over);
+  ctx->K = gcmf::mbr_filter(ctx->c, base->npoly, over->npoly, base->d_mbr, over->d_mbr,
+                            &ctx->counts, &ctx->pairs, &ctx->sv);
+  gcmf::join_refine(ctx->c, ctx->jin, ctx->K, ctx->pairs, ctx->jr);
+  ctx->c.flush_timers();
+  ctx->join_valid = true;
+  if (st) {
+    st->candidates = ctx->K;
+    st->pip_candidates = ctx->jr.pipNum;
+    st->within = ctx->jr.within;
+    st->cmf_candidates = ctx->jr.cmfCand;
+    st->survivors = ctx->jr.E;
+    st->b_edges = ctx->jr.nbEdges;
+    st->o_edges = ctx->jr.noEdges;
+    st->intersecting = ctx->jr.hits;
+    st->joined = ctx->jr.nJoined;
+  }
+}
+
+void fill_clip_stats(gcmf_ctx *ctx, gcmf_clip_stats *st) {
+  if (!st) return;
+  std::memset(st, 0, sizeof(*st));
+  const long M = ctx->cr.M;
+  st->pairs = M;
+  st->total_comps = ctx->cr.totalComps;
+  st->total_verts = ctx->cr.totalVerts;
+  st->edges_p = ctx->p1.edgesP;
+  st->edges_q = ctx->p1.edgesQ;
+  st->slots_p = ctx->p1.slotsP;
+  st->slots_q = ctx->p1.slotsQ;
+  st->n_eval = ctx->cr.nEvalP;
+  st->n_intersections = ctx->cr.totalIntersections;
+  if (M > 0) {
+    std::string buf;
+    buf.resize(sizeof(int) * (size_t)M);
+    HIP_CHECK(hipMemcpy(&buf[0], ctx->cr.status, sizeof(int) * (size_t)M, hipMemcpyDeviceToHost));
+    const int *s = (const int *)buf.data();
+    for (long k = 0; k < M; k++) {
+      if (s[k] == gcmf::GH_OK) st->ok++;
+      else if (s[k] == gcmf::GH_INVALID) st->invalid++;
+      else if (s[k] == gcmf::GH_DISJOINT) st->disjoint++;
+      else st->failed++;
+    }
+  }
+}
+
+void run_clip(gcmf_ctx *ctx, const gcmf::JoinInputs &in, const int *d_pairs, long M, int flags,
+              gcmf_clip_stats *st) {
+  ctx->clip_valid = false;
+  gcmf::ClipOptions opt;
+  opt.normalize = (flags & GCMF_CLIP_NORMALIZE) ? 1 : 0;
+  opt.cleanup = (flags & GCMF_CLIP_NO_CLEANUP) ? 0 : 1;
+  gcmf::clip_pairs(ctx->c, in, d_pairs, M, opt, ctx->cr, &ctx->p1);
+  ctx->c.flush_timers();
+  ctx->clip_valid = true;
+  fill_clip_stats(ctx, st);
+}
+
+void copy_out(gcmf::Context &c, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return;
+  if (!src) throw gcmf::Error{GCMF_ERR_ARG, "field not available"};
+  HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+int gcmf_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int gcmf_init(int device, gcmf_ctx **out) {
+  if (!out) return GCMF_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GCMF_ERR_NODEV;
+  if (device < 0 || device >= n) return GCMF_ERR_ARG;
+  gcmf_ctx *ctx = new gcmf_ctx();
+  ctx->c.device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->c.stream) != hipSuccess) {
+    delete ctx;
+    return GCMF_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+    ctx->c.cu_count = prop.multiProcessorCount;
+  *out = ctx;
+  return GCMF_OK;
+}
+
+void gcmf_shutdown(gcmf_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  (void)hipStreamSynchronize(ctx->c.stream);
+  ctx->c.ws.release_all();
+  for (auto &p : ctx->c.pending) {
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  for (auto e : ctx->c.event_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+const char *gcmf_last_error(const gcmf_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : ""; }
+
+int gcmf_malloc(gcmf_ctx *ctx, void **d_ptr, size_t bytes) {
+  return guarded(ctx, [&] { HIP_CHECK(hipMalloc(d_ptr, bytes ? bytes : 1)); });
+}
+
+int gcmf_free(gcmf_ctx *ctx, void *d_ptr) {
+  return guarded(ctx, [&] { HIP_CHECK(hipFree(d_ptr)); });
+}
+
+int gcmf_memcpy_h2d(gcmf_ctx *ctx, void *d_dst, const void *src, size_t bytes) {
+  return guarded(ctx, [&] {
+    HIP_CHECK(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int gcmf_memcpy_d2h(gcmf_ctx *ctx, void *dst, const void *d_src, size_t bytes) {
+  return guarded(ctx, [&] { copy_out(ctx->c, dst, d_src, bytes); });
+}
+
+int gcmf_synchronize(gcmf_ctx *ctx) {
+  return guarded(ctx, [&] { HIP_CHECK(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+int gcmf_set_timing(gcmf_ctx *ctx, int on) {
+  return guarded(ctx, [&] {
+    ctx->c.flush_timers();
+    ctx->c.timing = on != 0;
+  });
+}
+
+int gcmf_timer_reset(gcmf_ctx *ctx) {
+  return guarded(ctx, [&] {
+    ctx->c.flush_timers();
+    ctx->c.timers.clear();
+  });
+}
+
+int gcmf_timer_get(gcmf_ctx *ctx, const char *kernel, double *total_ms, long *launches) {
+  return guarded(ctx, [&] {
+    ctx->c.flush_timers();
+    auto it = ctx->c.timers.find(kernel ? kernel : "");
+    if (total_ms) *total_ms = it == ctx->c.timers.end() ? 0.0 : it->second.total_ms;
+    if (launches) *launches = it == ctx->c.timers.end() ? 0 : it->second.launches;
+  });
+}
+
+int gcmf_join(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+              gcmf_join_stats *stats) {
+  return guarded(ctx, [&] { run_join(ctx, base, over, stats); });
+}
+
+int gcmf_join_get(gcmf_ctx *ctx, int field, void *dst, long count) {
+  return guarded(ctx, [&] {
+    if (!ctx->join_valid) throw gcmf::Error{GCMF_ERR_ARG, "no join result on this context"};
+    if (count < 0) throw gcmf::Error{GCMF_ERR_ARG, "negative count"};
+    const long K = ctx->K, N = ctx->jin.B + ctx->jin.O;
+    const void *src = nullptr;
+    size_t elem = 4;
+    long maxc = 0;
+    auto &r = ctx->jr;
+    switch (field) {
+      case GCMF_J_PAIRS: src = ctx->pairs; maxc = 2 * K; break;
+      case GCMF_J_COUNTS: src = ctx->counts; maxc = N; break;
+      case GCMF_J_PERM: src = ctx->sv.perm; maxc = 2 * N; break;
+      case GCMF_J_RANK: src = ctx->sv.rank; maxc = 2 * N; break;
+      case GCMF_J_CMBR: src = r.cmbr; maxc = 4 * K; elem = 8; break;
+      case GCMF_J_PIPTYPE: src = r.pipType; maxc = K; elem = 1; break;
+      case GCMF_J_PIPFLAG: src = r.pipFlag; maxc = K; break;
+      case GCMF_J_JOINFLAG: src = r.joinFlag; maxc = K; break;
+      case GCMF_J_BCOUNT: src = r.bCount; maxc = K; break;
+      case GCMF_J_OCOUNT: src = r.oCount; maxc = K; break;
+      case GCMF_J_BEDGE: src = r.bEdge; maxc = r.nbEdges; break;
+      case GCMF_J_OEDGE: src = r.oEdge; maxc = r.noEdges; break;
+      case GCMF_J_SEGFLAG: src = r.segFlag; maxc = K; break;
+      case GCMF_J_JOINED: src = r.joinedPairs; maxc = 2 * r.nJoined; break;
+      default: throw gcmf::Error{GCMF_ERR_ARG, "unknown join field"};
+    }
+    if (count > maxc) throw gcmf::Error{GCMF_ERR_ARG, "count exceeds field size"};
+    copy_out(ctx->c, dst, src, elem * (size_t)count);
+  });
+}
+
+int gcmf_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+              const int *d_pairs, long M, int flags, gcmf_clip_stats *stats) {
+  return guarded(ctx, [&] {
+    if (!base || !over || (M > 0 && !d_pairs) || M < 0)
+      throw gcmf::Error{GCMF_ERR_ARG, "bad clip arguments"};
+    run_clip(ctx, to_inputs(base, over), d_pairs, M, flags, stats);
+  });
+}
+
+int gcmf_join_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over, int flags,
+                   gcmf_join_stats *jstats, gcmf_clip_stats *cstats) {
+  return guarded(ctx, [&] {
+    run_join(ctx, base, over, jstats);
+    run_clip(ctx, ctx->jin, ctx->jr.joinedPairs, ctx->jr.nJoined, flags, cstats);
+  });
+}
+
+int gcmf_clip_get(gcmf_ctx *ctx, int field, void *dst, long count) {
+  return guarded(ctx, [&] {
+    if (!ctx->clip_valid) throw gcmf::Error{GCMF_ERR_ARG, "no clip result on this context"};
+    if (count < 0) throw gcmf::Error{GCMF_ERR_ARG, "negative count"};
+    const auto &r = ctx->cr;
+    const auto &p = ctx->p1;
+    const long M = r.M;
+    const void *src = nullptr;
+    size_t elem = 4;
+    long maxc = 0;
+    switch (field) {
+      case GCMF_C_STATUS: src = r.status; maxc = M; break;
+      case GCMF_C_SWAPPED: src = r.swapped; maxc = M; break;
+      case GCMF_C_NCOMP: src = r.ncomp; maxc = M; break;
+      case GCMF_C_NVERTS: src = r.nverts; maxc = M; break;
+      case GCMF_C_COMPOFF: src = r.compOff; maxc = M + 1; elem = 8; break;
+      case GCMF_C_VERTOFF: src = r.vertOff; maxc = M + 1; elem = 8; break;
+      case GCMF_C_COMPSIZES: src = r.compSizes; maxc = r.totalComps; break;
+      case GCMF_C_XY: src = r.xy; maxc = 2 * r.totalVerts; elem = 8; break;
+      case GCMF_C_OFFP: src = p.offP; maxc = M + 1; elem = 8; break;
+      case GCMF_C_OFFQ: src = p.offQ; maxc = M + 1; elem = 8; break;
+      case GCMF_C_PSP1: src = p.psP1; maxc = p.edgesP + 1; elem = 8; break;
+      case GCMF_C_PSP2: src = p.psP2; maxc = p.edgesP + 1; elem = 8; break;
+      case GCMF_C_PSQ1: src = p.psQ1; maxc = p.edgesQ + 1; elem = 8; break;
+      case GCMF_C_PSQ2: src = p.psQ2; maxc = p.edgesQ + 1; elem = 8; break;
+      case GCMF_C_BOOLP: src = p.boolP; maxc = p.edgesP; break;
+      case GCMF_C_BOOLQ: src = p.boolQ; maxc = p.edgesQ; break;
+      case GCMF_C_NMAPQ: src = p.nmapQ; maxc = p.slotsQ; break;
+      case GCMF_C_IP: src = p.iP; maxc = 2 * p.slotsP; elem = 8; break;
+      case GCMF_C_IQ: src = p.iQ; maxc = 2 * p.slotsQ; elem = 8; break;
+      case GCMF_C_OP: src = p.oP; maxc = 2 * p.slotsP; elem = 8; break;
+      case GCMF_C_OQ: src = p.oQ; maxc = 2 * p.slotsQ; elem = 8; break;
+      case GCMF_C_AP: src = p.aP; maxc = p.slotsP; break;
+      case GCMF_C_AQ: src = p.aQ; maxc = p.slotsQ; break;
+      case GCMF_C_LP: src = p.lP; maxc = p.slotsP; break;
+      case GCMF_C_LQ: src = p.lQ; maxc = p.slotsQ; break;
+      case GCMF_C_NBP: src = p.nbP; maxc = p.slotsP; break;
+      case GCMF_C_NBQ: src = p.nbQ; maxc = p.slotsQ; break;
+      case GCMF_C_NORM: src = p.norm; maxc = 3 * M; elem = 8; break;
+      default: throw gcmf::Error{GCMF_ERR_ARG, "unknown clip field"};
+    }
+    if (count > maxc) throw gcmf::Error{GCMF_ERR_ARG, "count exceeds field size"};
+    copy_out(ctx->c, dst, src, elem * (size_t)count);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+// gcmf_internal.hpp -- host-side internals shared by the HIP translation
+// units of libgcmf_hip.so: the context (stream, workspace arena, kernel
+// timers) and the device primitives' host drivers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#define GCMF_OK 0
+#define GCMF_ERR_HIP -1
+#define GCMF_ERR_ALLOC -2
+#define GCMF_ERR_ARG -3
+#define GCMF_ERR_NODEV -4
+#define GCMF_ERR_OVERFLOW -5
+
+namespace gcmf {
+
+// Raised by HIP_CHECK inside library code; converted to a status code at the
+// C-ABI boundary.
+struct Error {
+  int code;
+  std::string what;
+};
+
+#define HIP_CHECK(expr)                                                        \
+  do {                                                                         \
+    hipError_t _e = (expr);                                                    \
+    if (_e != hipSuccess)                                                      \
+      throw ::gcmf::Error{_e == hipErrorOutOfMemory ? GCMF_ERR_ALLOC           \
+                                                    : GCMF_ERR_HIP,            \
+                          std::string(#expr) + ": " + hipGetErrorString(_e)};  \
+  } while (0)
+
+// Grow-only device workspace: named slots are reused across calls so the
+// steady-state path performs no hipMalloc.
+struct Workspace {
+  struct Slot {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<std::string, Slot> slots;
+  void *get(const std::string &name, size_t bytes);
+  void release_all();
+};
+
+struct KernelTimer {
+  double total_ms = 0.0;
+  long launches = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Workspace ws;
+  bool timing = false;
+  std::map<std::string, KernelTimer> timers;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::string last_error;
+  int cu_count = 256;
+
+  hipEvent_t take_event();
+  void flush_timers();  // synchronises on pending events and accumulates
+};
+
+// RAII timer around one launch (records events on ctx.stream when enabled).
+struct LaunchTimer {
+  Context &c;
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  LaunchTimer(Context &ctx, const char *n);
+  ~LaunchTimer();
+};
+
+// ---------------------------------------------------------------- prims ----
+// Exclusive scan of n int32 values into out[0..n] (out[n] = total).
+void scan_i32_to_i64(Context &c, const int *in, long long *out, long n);
+void scan_i32_to_i32(Context &c, const int *in, int *out, long n);
+// out[k] = i for the k-th i with flag[i] != 0; returns the count (synchronises).
+long compact_flags(Context &c, const int *flag, long n, int *out_idx, const char *tag);
+// Stable LSD radix sort of keys (u64) carrying int32 values; bits = number of
+// significant low bits of every key.  Results land in keys_out / vals_out.
+void radix_sort_u64(Context &c, const uint64_t *keys_in, const int *vals_in, uint64_t *keys_out,
+                    int *vals_out, long n, int bits);
+// fills
+void fill_i32(Context &c, int *p, int v, long n);
+// copies one value back to the host (synchronises)
+template <typename T>
+T read_scalar(Context &c, const T *dptr) {
+  T v;
+  HIP_CHECK(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  return v;
+}
+
+inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace gcmf

@@ -1,0 +1,1362 @@
+// gh.hip -- batched Greiner-Hormann (Foster-Hormann-Popa) clipping of many
+// polygon pairs in one pass on gfx950.
+//
+// Phase 1 reproduces the reference GPU phase (gppolyclip_mDIV512.cu:382-1223)
+// for every pair of the batch at once: all edges of all pairs form one index
+// space (edge -> pair map), counts of both views are scanned globally so each
+// pair's slot range falls out of the scan, and the intersection records,
+// neighbour links, per-edge alpha order and initial labels are written at
+// those global slots.  The reference's host round trip (D2H counts, host
+// thrust scans, H2D) and its per-pair allocations disappear.
+//
+// Phases 2-3 (labelling, tracing, clean-up; polyclip.h:335-807 with the
+// commented step 1 restored, fed as ghcuda.cpp:183-339 feeds it) run on the
+// GPU, one thread per pair over array-linked node lists.
+//
+// Every predicate runs on per-pair normalised coordinates
+// ((x-cx)*s, (y-cy)*s), (cx,cy) = CMBR corner, s = 1/max CMBR extent, and
+// every output vertex is produced in original coordinates from the same alpha.
+#include "gcmf_device.hpp"
+#include "gcmf_internal.hpp"
+#include "gcmf_join.hpp"
+
+namespace gcmf {
+
+__device__ __forceinline__ pt ld(const double *c, long i) { return pt{c[2 * i], c[2 * i + 1]}; }
+__device__ __forceinline__ void st(double *c, long i, pt p) {
+  c[2 * i] = p.x;
+  c[2 * i + 1] = p.y;
+}
+
+__global__ void k_fill_u64_pair(unsigned long long *p) {
+  if (threadIdx.x == 0) {
+    p[0] = 0ULL;
+    p[1] = 0ULL;
+  }
+}
+
+// ------------------------------------------------------------- setup ----
+struct PairMeta {
+  long *startP, *startQ;  // ring start (vertex index) inside the source layer
+  int *nP, *nQ, *swapped, *status;
+  double *norm;   // [3M] cx, cy, s
+  double *cmbrN;  // [4M] normalised CMBR
+};
+
+// one wave per pair: sizes, swap (|P| >= |Q|, ghcuda.cpp:64-91), ring MBRs
+// (readShapefile.h:304-334), CMBR and disjointness (ghcuda.cpp:94-125)
+__global__ void __launch_bounds__(256)
+    k_pair_setup(long M, const int *pairs, const double *bCoords, const int *bVNum,
+                 const long *bVPSNum, const double *oCoords, const int *oVNum, const long *oVPSNum,
+                 int normalize, int noswap, PairMeta pm, int *eP, int *eQ) {
+  const long k = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= M) return;
+  const int lane = __lane_id();
+  const int b = pairs[2 * k], o = pairs[2 * k + 1];
+  const int nb = bVNum[b] - 1, no = oVNum[o] - 1;
+  const long sb = bVPSNum[b] - bVNum[b], so = oVPSNum[o] - oVNum[o];
+  const int sw = noswap ? 0 : (nb < no);
+  const int nP = sw ? no : nb, nQ = sw ? nb : no;
+  const long sP = sw ? so : sb, sQ = sw ? sb : so;
+  const double *cP = sw ? oCoords : bCoords, *cQ = sw ? bCoords : oCoords;
+  double m[2][4];
+  for (int side = 0; side < 2; side++) {
+    const double *c = side == 0 ? cP + 2 * sP : cQ + 2 * sQ;
+    const int n = side == 0 ? nP : nQ;
+    double mnx = c[0], mny = c[1], mxx = c[0], mxy = c[1];
+    for (int i = lane; i < n; i += 64) {
+      double x = c[2 * i], y = c[2 * i + 1];
+      mnx = fmin(mnx, x);
+      mny = fmin(mny, y);
+      mxx = fmax(mxx, x);
+      mxy = fmax(mxy, y);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      mnx = fmin(mnx, __shfl_xor(mnx, off, 64));
+      mny = fmin(mny, __shfl_xor(mny, off, 64));
+      mxx = fmax(mxx, __shfl_xor(mxx, off, 64));
+      mxy = fmax(mxy, __shfl_xor(mxy, off, 64));
+    }
+    m[side][0] = mnx;
+    m[side][1] = mny;
+    m[side][2] = mxx;
+    m[side][3] = mxy;
+  }
+  if (lane != 0) return;
+  int status = GH_OK;
+  if (nP < 1 || nQ < 1) status = GH_DISJOINT;
+  if (m[0][0] > m[1][2] || m[0][2] < m[1][0]) status = GH_DISJOINT;
+  if (m[0][1] > m[1][3] || m[0][3] < m[1][1]) status = GH_DISJOINT;
+  double cm[4];
+  cm[0] = m[0][0] < m[1][0] ? m[1][0] : m[0][0];
+  cm[1] = m[0][1] < m[1][1] ? m[1][1] : m[0][1];
+  cm[2] = m[1][2] < m[0][2] ? m[1][2] : m[0][2];
+  cm[3] = m[1][3] < m[0][3] ? m[1][3] : m[0][3];
+  double cx = 0.0, cy = 0.0, s = 1.0;
+  if (normalize) {
+    double w = cm[2] - cm[0], h = cm[3] - cm[1];
+    double e = w > h ? w : h;
+    cx = cm[0];
+    cy = cm[1];
+    s = e > 0.0 ? 1.0 / e : 1.0;
+  }
+  pm.startP[k] = sP;
+  pm.startQ[k] = sQ;
+  pm.nP[k] = nP;
+  pm.nQ[k] = nQ;
+  pm.swapped[k] = sw;
+  pm.status[k] = status;
+  pm.norm[3 * k] = cx;
+  pm.norm[3 * k + 1] = cy;
+  pm.norm[3 * k + 2] = s;
+  pm.cmbrN[4 * k] = (cm[0] - cx) * s;
+  pm.cmbrN[4 * k + 1] = (cm[1] - cy) * s;
+  pm.cmbrN[4 * k + 2] = (cm[2] - cx) * s;
+  pm.cmbrN[4 * k + 3] = (cm[3] - cy) * s;
+  eP[k] = status == GH_OK ? nP : 0;
+  eQ[k] = status == GH_OK ? nQ : 0;
+}
+
+// one wave per pair: edge -> pair maps and normalised rings
+__global__ void __launch_bounds__(256)
+    k_pair_fill(long M, const int *pairs, const double *bCoords, const double *oCoords,
+                PairMeta pm, const long long *offP, const long long *offQ, int *mapP, int *mapQ,
+                double *nP_, double *nQ_) {
+  const long k = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= M) return;
+  const int lane = __lane_id();
+  const int sw = pm.swapped[k];
+  const double cx = pm.norm[3 * k], cy = pm.norm[3 * k + 1], s = pm.norm[3 * k + 2];
+  for (int side = 0; side < 2; side++) {
+    const long long base = side == 0 ? offP[k] : offQ[k];
+    const long long n = (side == 0 ? offP[k + 1] : offQ[k + 1]) - base;
+    const double *c = (side == 0) == (sw == 0) ? bCoords : oCoords;
+    const long start = side == 0 ? pm.startP[k] : pm.startQ[k];
+    int *map = side == 0 ? mapP : mapQ;
+    double *dst = side == 0 ? nP_ : nQ_;
+    for (long long i = lane; i < n; i += 64) {
+      double x = c[2 * (start + i)], y = c[2 * (start + i) + 1];
+      dst[2 * (base + i)] = (x - cx) * s;
+      dst[2 * (base + i) + 1] = (y - cy) * s;
+      map[base + i] = (int)k;
+    }
+  }
+}
+
+// ------------------------------------------------------------- count ----
+// gpuCMBRFilter (:382-415) + gpuCountIntersections (:463-548), both views in
+// one launch: threads [0, edgesP) count P edges against Q, the rest count Q
+// edges against P.
+__global__ void __launch_bounds__(256)
+    k_count(long edgesP, long edgesQ, const int *mapP, const int *mapQ, const long long *offP,
+            const long long *offQ, const double *ringP, const double *ringQ,
+            const double *cmbrN, int *cnt1P, int *cnt2P, int *cnt1Q, int *cnt2Q, int *boolP,
+            int *boolQ, unsigned long long *nEval) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edgesP + edgesQ) return;
+  const bool sideP = g < edgesP;
+  const long ga = sideP ? g : g - edgesP;
+  const int k = sideP ? mapP[ga] : mapQ[ga];
+  const long long aBase = sideP ? offP[k] : offQ[k];
+  const long long aN = (sideP ? offP[k + 1] : offQ[k + 1]) - aBase;
+  const long long bBase = sideP ? offQ[k] : offP[k];
+  const long long bN = (sideP ? offQ[k + 1] : offP[k + 1]) - bBase;
+  const double *A = sideP ? ringP : ringQ;
+  const double *Bv = (sideP ? ringQ : ringP) + 2 * bBase;
+  const long long i = ga - aBase;
+  const pt A1 = ld(A, ga), A2 = ld(A, i == aN - 1 ? aBase : ga + 1);
+  const double cx1 = cmbrN[4 * k], cy1 = cmbrN[4 * k + 1], cx2 = cmbrN[4 * k + 2],
+               cy2 = cmbrN[4 * k + 3];
+  const double mnX = fmin(A1.x, A2.x), mxX = fmax(A1.x, A2.x);
+  const double mnY = fmin(A1.y, A2.y), mxY = fmax(A1.y, A2.y);
+  bool flag = !(mnX > cx2 || mxX < cx1) && !(mnY > cy2 || mxY < cy1);
+  int c1 = 0, c2 = 0;
+  unsigned long long ev = 0;
+  if (flag) {
+    pt B1 = ld(Bv, 0);
+    for (long long q = 0; q < bN; q++) {
+      pt B2 = ld(Bv, q == bN - 1 ? 0 : q + 1);
+      if (lsmf(A1, A2, B1, B2)) {
+        double al, be;
+        ev++;
+        int t = itype(A1, A2, B1, B2, al, be);
+        c1 += t != 0;
+        c2 += adds_vertex(t);
+      }
+      B1 = B2;
+    }
+  }
+  if (sideP) {
+    cnt1P[ga] = c1;
+    cnt2P[ga] = c2 + 1;
+    boolP[ga] = flag;
+  } else {
+    cnt1Q[ga] = c1;
+    cnt2Q[ga] = c2 + 1;
+    boolQ[ga] = flag;
+  }
+  // one atomic per wave for the evaluation statistic
+  for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off, 64);
+  if (__lane_id() == 0 && ev) atomicAdd(nEval, ev);
+}
+
+// per pair: invalid check (:1460-1476) and slot bases
+__global__ void k_pair_check(long M, const long long *offP, const long long *offQ,
+                             const long long *psP1, const long long *psQ1,
+                             const long long *psP2, const long long *psQ2, int *status,
+                             unsigned long long *nInter) {
+  long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  if (status[k] != GH_OK) return;
+  long long c1P = psP1[offP[k + 1]] - psP1[offP[k]];
+  long long c1Q = psQ1[offQ[k + 1]] - psQ1[offQ[k]];
+  if (c1P != c1Q) status[k] = GH_INVALID;
+  else atomicAdd(nInter, (unsigned long long)c1P);
+}
+
+// ----------------------------------------------------- neighbour map ----
+// gpuNeighborMap (:549-624): for each Q edge, the P edge of its k-th
+// vertex-adding intersection (degenerate types use the parent slot).
+__global__ void __launch_bounds__(256)
+    k_nmap(long edgesQ, const int *mapQ, const long long *offP, const long long *offQ,
+           const double *ringP, const double *ringQ, const long long *psP1,
+           const long long *psQ1, const long long *psQ2, const int *status, int *nmap) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edgesQ) return;
+  const int k = mapQ[g];
+  if (status[k] != GH_OK) return;
+  if (psQ1[g + 1] == psQ1[g]) return;
+  const long long qBase = offQ[k], qN = offQ[k + 1] - qBase;
+  const long long pBase = offP[k], pN = offP[k + 1] - pBase;
+  const long long i = g - qBase;
+  const pt A1 = ld(ringQ, g), A2 = ld(ringQ, i == qN - 1 ? qBase : g + 1);
+  const long long slot = psQ2[g];
+  int c2 = 0, nd = 0;
+  for (long long p = 0; p < pN; p++) {
+    const long long pg = pBase + p;
+    if (psP1[pg + 1] == psP1[pg]) continue;
+    pt B1 = ld(ringP, pg), B2 = ld(ringP, p == pN - 1 ? pBase : pg + 1);
+    if (!lsmf(A1, A2, B1, B2)) continue;
+    double al, be;
+    int t = itype(A1, A2, B1, B2, al, be);
+    if (t) {
+      if (adds_vertex(t)) c2 = ++nd;
+      else c2 = 0;
+      nmap[slot + c2] = (int)p;
+    }
+  }
+}
+
+// ----------------------------------------------------- intersections ----
+struct SlotArrays {
+  double *iP, *iQ, *oP, *oQ;      // normalised / original coordinates
+  double *iP2, *iQ2, *oP2, *oQ2;  // unsorted copies (sort sources)
+  int *aP, *aQ, *aP2, *aQ2;       // trunc(1e7 alpha), -100 none
+  int *nbP, *nbQ, *nbP2, *nbQ2;   // global slot + 1, 0 none
+  int *lP, *lQ;                   // initial labels
+};
+
+__global__ void k_init_slots(long slotsP, long slotsQ, SlotArrays s, int *nmap) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < slotsP) {
+    s.aP[i] = kNone;
+    s.aP2[i] = kNone;
+    s.nbP[i] = 0;
+    s.nbP2[i] = 0;
+    s.lP[i] = kNone;
+  }
+  if (i < slotsQ) {
+    s.aQ[i] = kNone;
+    s.aQ2[i] = kNone;
+    s.nbQ[i] = 0;
+    s.nbQ2[i] = 0;
+    s.lQ[i] = kNone;
+    nmap[i] = kNone;
+  }
+}
+
+// Q parents: slot psQ2[g] holds Q vertex g (:930-935)
+__global__ void k_q_parents(long edgesQ, const int *mapQ, const long long *offQ,
+                            const double *ringQ, const long long *psQ2, const int *status,
+                            PairMeta pm, const double *bCoords, const double *oCoords,
+                            SlotArrays s) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edgesQ) return;
+  const int k = mapQ[g];
+  if (status[k] != GH_OK) return;
+  const long long i = g - offQ[k];
+  const double *cQ = pm.swapped[k] ? bCoords : oCoords;
+  const pt o = ld(cQ, pm.startQ[k] + i);
+  const pt n = ld(ringQ, g);
+  const long long sl = psQ2[g];
+  st(s.iQ, sl, n);
+  st(s.iQ2, sl, n);
+  st(s.oQ, sl, o);
+  st(s.oQ2, sl, o);
+}
+
+// Stable order of keys[lo..hi) by value: dest[j] = #{i : k_i < k_j} +
+// #{i < j : k_i == k_j}, which equals the reference's 7-digit decimal LSD
+// counting sort (:263-305) on these non-negative keys.
+__device__ __forceinline__ long long stable_dest(const int *keys, long long lo, long long hi,
+                                                 long long j) {
+  const int kj = keys[j];
+  long long d = 0;
+  for (long long i = lo; i < hi; i++) {
+    int ki = keys[i];
+    d += (ki < kj) || (ki == kj && i < j);
+  }
+  return d;
+}
+
+// Several P edges may write the same Q slot (degenerate configurations, and
+// the reference's neighbour index that persists across Q edges when a search
+// fails).  The reference resolves these races arbitrarily; here the writes of
+// the highest P edge win, which is the order of a sequential loop over P
+// edges.  OWN pass: record that writer per Q slot and field group with
+// atomicMax; WRITE pass: only the recorded writer stores.
+struct Owners {
+  int *a;  // aQ / aQ2
+  int *c;  // iQ, oQ and copies
+  int *n;  // nbQ / nbQ2
+};
+
+// gpuCalculateIntersections (:897-1123) including the per-edge sort
+template <bool OWN>
+__global__ void __launch_bounds__(256)
+    k_calc(long edgesP, const int *mapP, const long long *offP, const long long *offQ,
+           const double *ringP, const double *ringQ, const long long *psP1,
+           const long long *psP2, const long long *psQ1, const long long *psQ2,
+           const int *status, PairMeta pm, const double *bCoords, const double *oCoords,
+           const int *nmap, SlotArrays s, Owners own) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edgesP) return;
+  const int k = mapP[g];
+  if (status[k] != GH_OK) return;
+  const long long pBase = offP[k], pN = offP[k + 1] - pBase;
+  const long long qBase = offQ[k], qN = offQ[k + 1] - qBase;
+  const int pid = (int)(g - pBase);
+  const int sw = pm.swapped[k];
+  const double *cP = sw ? oCoords : bCoords, *cQ = sw ? bCoords : oCoords;
+  const long sP = pm.startP[k], sQ = pm.startQ[k];
+  const long long base = psP2[g];
+  const pt P1 = ld(ringP, g), P2 = ld(ringP, pid == pN - 1 ? pBase : g + 1);
+  const pt P1o = ld(cP, sP + pid), P2o = ld(cP, sP + (pid == pN - 1 ? 0 : pid + 1));
+  if (!OWN) {
+    st(s.iP, base, P1);
+    st(s.iP2, base, P1);
+    st(s.oP, base, P1o);
+    st(s.oP2, base, P1o);
+  }
+  if (psP1[g + 1] == psP1[g]) return;
+  // Q-slot stores (conditional on ownership in the WRITE pass)
+  auto setA = [&](long long sl, int v) {
+    if (OWN) atomicMax(&own.a[sl], pid);
+    else if (own.a[sl] == pid) {
+      s.aQ[sl] = v;
+      s.aQ2[sl] = v;
+    }
+  };
+  auto setC = [&](long long sl, pt v, pt vo) {
+    if (OWN) atomicMax(&own.c[sl], pid);
+    else if (own.c[sl] == pid) {
+      st(s.iQ, sl, v);
+      st(s.iQ2, sl, v);
+      st(s.oQ, sl, vo);
+      st(s.oQ2, sl, vo);
+    }
+  };
+  auto setN = [&](long long sl, int v) {
+    if (OWN) atomicMax(&own.n[sl], pid);
+    else if (own.n[sl] == pid) {
+      s.nbQ[sl] = v;
+      s.nbQ2[sl] = v;
+    }
+  };
+  auto setP = [&](long long sl, int nb) {
+    if (!OWN) {
+      s.nbP[sl] = nb;
+      s.nbP2[sl] = nb;
+    }
+  };
+  int c2 = 0, nd = 0;
+  long long nQId = -1;
+  for (long long q = 0; q < qN; q++) {
+    const long long qg = qBase + q;
+    if (psQ1[qg + 1] == psQ1[qg]) continue;
+    const pt Q1 = ld(ringQ, qg), Q2 = ld(ringQ, q == qN - 1 ? qBase : qg + 1);
+    if (!lsmf(P1, P2, Q1, Q2)) continue;
+    double alpha, beta;
+    const int t = itype(P1, P2, Q1, Q2, alpha, beta);
+    if (!t) continue;
+    if (adds_vertex(t)) c2 = ++nd;
+    else c2 = 0;
+    const long long start = psQ2[qg], end = psQ2[qg + 1];
+    const long long ps = base + c2;
+    if (t != 5) {
+      for (long long l = start; l < end; l++) {
+        if (nmap[l] == pid) {
+          nQId = l;
+          setP(ps, (int)(l + 1));
+          setN(l, (int)(ps + 1));
+          break;
+        }
+      }
+    } else {
+      nQId = start;
+      setP(ps, (int)(start + 1));
+      setN(start, (int)(ps + 1));
+      for (long long l = start; l < end; l++) {
+        if (nmap[l] == pid) {
+          nQId = l;
+          setP(base, (int)(l + 1));
+          setN(l, (int)(base + 1));
+          break;
+        }
+      }
+    }
+    const int ka = trunc_sat((double)kAlphaScale * alpha);
+    const int kb = trunc_sat((double)kAlphaScale * beta);
+    const pt Q1o = ld(cQ, sQ + q);
+    // P-side values of this record
+    bool pAdds = t == 1 || t == 5 || t == 3 || t == 7;
+    if (!OWN) {
+      if (pAdds) {
+        pt v = Q1, vo = Q1o;
+        if (t == 1) {
+          v = pt{(1.0 - alpha) * P1.x + alpha * P2.x, (1.0 - alpha) * P1.y + alpha * P2.y};
+          vo = pt{(1.0 - alpha) * P1o.x + alpha * P2o.x, (1.0 - alpha) * P1o.y + alpha * P2o.y};
+        }
+        st(s.iP, ps, v);
+        st(s.iP2, ps, v);
+        st(s.oP, ps, vo);
+        st(s.oP2, ps, vo);
+        s.aP[ps] = ka;
+        s.aP2[ps] = ka;
+      } else {
+        s.aP[base] = ka;
+        s.aP2[base] = ka;
+      }
+    }
+    // Q-side values
+    if (t == 1 || t == 5 || t == 2 || t == 6) {
+      if (nQId >= 0) {
+        if (t == 1) {
+          const pt I = {(1.0 - alpha) * P1.x + alpha * P2.x, (1.0 - alpha) * P1.y + alpha * P2.y};
+          const pt Io = {(1.0 - alpha) * P1o.x + alpha * P2o.x,
+                         (1.0 - alpha) * P1o.y + alpha * P2o.y};
+          setC(nQId, I, Io);
+        } else {
+          setC(nQId, P1, P1o);
+        }
+        setA(nQId, kb);
+      }
+    } else {
+      setA(start, kb);
+    }
+  }
+  // per-edge alpha sort (:1103-1120)
+  const long long lo = base + 1, hi = psP2[g + 1];
+  if (hi - lo >= 2) {
+    for (long long j = lo; j < hi; j++) {
+      const int nb = OWN ? 0 : s.nbP2[j];
+      if (OWN) continue;
+      const long long d = lo + stable_dest(s.aP2, lo, hi, j);
+      s.aP[d] = s.aP2[j];
+      st(s.iP, d, ld(s.iP2, j));
+      st(s.oP, d, ld(s.oP2, j));
+      s.nbP[d] = nb;
+      if (nb > 0) setN(nb - 1, (int)(d + 1));
+    }
+  }
+}
+
+// gpuSortPolyQ (:1132-1160).  OWN pass: the highest Q edge writing a P
+// slot's back link owns it (sequential loop order over Q edges).
+template <bool OWN>
+__global__ void k_sort_q(long edgesQ, const int *mapQ, const long long *psQ2, const int *status,
+                         SlotArrays s, int *ownP) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edgesQ) return;
+  if (status[mapQ[g]] != GH_OK) return;
+  const long long lo = psQ2[g] + 1, hi = psQ2[g + 1];
+  if (hi - lo < 2) return;
+  for (long long j = lo; j < hi; j++) {
+    const int nb = s.nbQ2[j];
+    if (OWN) {
+      if (nb > 0) atomicMax(&ownP[nb - 1], (int)g);
+      continue;
+    }
+    const long long d = lo + stable_dest(s.aQ2, lo, hi, j);
+    s.aQ[d] = s.aQ2[j];
+    st(s.iQ, d, ld(s.iQ2, j));
+    st(s.oQ, d, ld(s.oQ2, j));
+    s.nbQ[d] = nb;
+    if (nb > 0 && ownP[nb - 1] == (int)g) s.nbP[nb - 1] = (int)(d + 1);
+  }
+}
+
+// gpuCalculateInitLabel (:1171-1223); circular ids within the pair's slots.
+// OWN pass: the highest P slot writing a Q label owns it.
+template <bool OWN>
+__global__ void k_init_label(long edgesP, const int *mapP, const long long *offP,
+                             const long long *offQ, const long long *psP2,
+                             const long long *psQ2, const int *status, SlotArrays s,
+                             int *ownL) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edgesP) return;
+  const int k = mapP[g];
+  if (status[k] != GH_OK) return;
+  const long long sbP = psP2[offP[k]], cP = psP2[offP[k + 1]] - sbP;
+  const long long sbQ = psQ2[offQ[k]], cQ = psQ2[offQ[k + 1]] - sbQ;
+  for (long long i = psP2[g]; i < psP2[g + 1]; i++) {
+    if (!OWN) s.lP[i] = kNone;
+    if (s.aP[i] == kNone) continue;
+    const long long nId = (long long)s.nbP[i] - 1;
+    if (nId < sbQ || nId >= sbQ + cQ) continue;
+    if (OWN) {
+      atomicMax(&ownL[nId], (int)i);
+      continue;
+    }
+    const pt cur = ld(s.iP, i);
+    long long t = i - 1 < sbP ? sbP + cP - 1 : i - 1;
+    const pt pM = ld(s.iP, t);
+    const long long pMN = s.aP[t] != kNone ? (long long)s.nbP[t] - 1 : -100;
+    t = i + 1 == sbP + cP ? sbP : i + 1;
+    const pt pP = ld(s.iP, t);
+    const long long pPN = s.aP[t] != kNone ? (long long)s.nbP[t] - 1 : -100;
+    t = nId - 1 < sbQ ? sbQ + cQ - 1 : nId - 1;
+    const int tM = rel_pos(pMN != -100 && pMN == t, pPN != -100 && pPN == t, ld(s.iQ, t), pM,
+                           cur, pP);
+    t = nId + 1 == sbQ + cQ ? sbQ : nId + 1;
+    const int tP = rel_pos(pMN != -100 && pMN == t, pPN != -100 && pPN == t, ld(s.iQ, t), pM,
+                           cur, pP);
+    const int lab = initial_label(tM, tP);
+    s.lP[i] = lab;
+    if (ownL[nId] == (int)i) s.lQ[nId] = lab;
+  }
+}
+
+// ------------------------------------------------ labelling and tracing ----
+enum { L_NONE = 0, L_CROSSING, L_BOUNCING, L_LEFT_ON, L_RIGHT_ON, L_ON_ON, L_ON_LEFT,
+       L_ON_RIGHT, L_DELAYED_CROSSING, L_DELAYED_BOUNCING };
+enum { EXIT = 0, ENTRY = 1, NEITHER = 2 };
+enum { F_SRC = 1, F_INT = 2, F_SPLIT = 4, F_CROSS = 8 };
+
+struct Nodes {  // one pair's node region, local indices
+  double *pn, *po;
+  int *next, *prev, *nb;
+  unsigned char *fl;
+  signed char *lab, *enex;
+  int n, cap;
+};
+
+__device__ __forceinline__ bool isint(const Nodes &N, int v) { return N.fl[v] & F_INT; }
+__device__ __forceinline__ pt P_(const Nodes &N, int v) { return ld(N.pn, v); }
+
+__device__ int node_new(Nodes &N, pt p, pt o) {
+  if (N.n >= N.cap) return -1;
+  int v = N.n++;
+  st(N.pn, v, p);
+  st(N.po, v, o);
+  N.next[v] = N.prev[v] = N.nb[v] = -1;
+  N.fl[v] = 0;
+  N.lab[v] = L_NONE;
+  N.enex[v] = NEITHER;
+  return v;
+}
+
+// insertVertex(V, curr) without alpha (polygon.h:101-113): after curr
+__device__ void node_insert_after(Nodes &N, int V, int curr) {
+  int c = N.next[curr];
+  N.next[N.prev[c]] = V;
+  N.prev[V] = N.prev[c];
+  N.next[V] = c;
+  N.prev[c] = V;
+}
+
+__device__ void node_link(Nodes &N, int a, int b) {
+  N.nb[a] = b;
+  N.nb[b] = a;
+  N.fl[a] |= F_INT;
+  N.fl[b] |= F_INT;
+}
+
+// polygon::pointInPoly over edges(ALL), polygon.h:427-448
+__device__ bool node_pip(const Nodes &N, int root, pt R) {
+  int w = 0, a = root;
+  do {
+    int b = N.next[a];
+    pt P0 = P_(N, a), P1 = P_(N, b);
+    if ((P0.y < R.y) != (P1.y < R.y)) {
+      if (P0.x >= R.x) {
+        if (P1.x > R.x) w = w + 2 * (P1.y > P0.y) - 1;
+        else if ((triA(P0, P1, R) > 0) == (P1.y > P0.y)) w = w + 2 * (P1.y > P0.y) - 1;
+      } else if (P1.x > R.x) {
+        if ((triA(P0, P1, R) > 0) == (P1.y > P0.y)) w = w + 2 * (P1.y > P0.y) - 1;
+      }
+    }
+    a = b;
+  } while (a != root);
+  return (w % 2) != 0;
+}
+
+__device__ __forceinline__ void toggle(int &s) {
+  if (s == ENTRY) s = EXIT;
+  else if (s == EXIT) s = ENTRY;
+}
+
+__device__ int node_rel(const Nodes &N, int Q, int P1, int P2, int P3) {
+  return rel_pos(isint(N, P1) && N.nb[P1] == Q, isint(N, P3) && N.nb[P3] == Q, P_(N, Q),
+                 P_(N, P1), P_(N, P2), P_(N, P3));
+}
+
+struct OutBuf {  // one pair's result region
+  double *pn, *po;
+  int *next, *prev;
+  int n, cap;
+  int *compRoot;  // local roots
+  int ncomp, capComp;
+};
+
+__device__ int out_new(OutBuf &O, pt p, pt o) {
+  if (O.n >= O.cap) return -1;
+  int v = O.n++;
+  st(O.pn, v, p);
+  st(O.po, v, o);
+  O.next[v] = O.prev[v] = v;
+  return v;
+}
+
+__device__ void out_append(OutBuf &O, int root, int v) {
+  O.prev[v] = O.prev[root];
+  O.next[v] = root;
+  O.next[O.prev[root]] = v;
+  O.prev[root] = v;
+}
+
+// copies a whole node ring into the output as one component
+__device__ bool out_copy_ring(OutBuf &O, const Nodes &N, int root) {
+  if (O.ncomp >= O.capComp) return false;
+  int r = -1, v = root;
+  do {
+    int w = out_new(O, P_(N, v), ld(N.po, v));
+    if (w < 0) return false;
+    if (r < 0) r = w;
+    else out_append(O, r, w);
+    v = N.next[v];
+  } while (v != root);
+  O.compRoot[O.ncomp++] = r;
+  return true;
+}
+
+// steps 1..6 of labelIntersections, createResult; returns status
+__device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
+  // step 1 (polyclip.h:343-397, restored) then step 3 (copy labels to Q)
+  {
+    int I = rootP;
+    do {
+      if (isint(N, I) && N.nb[I] >= 0) {
+        int Pm = N.prev[I], Pp = N.next[I], Qm = N.prev[N.nb[I]], Qp = N.next[N.nb[I]];
+        int tm = node_rel(N, Qm, Pm, I, Pp), tp = node_rel(N, Qp, Pm, I, Pp);
+        int l = N.lab[I];
+        if ((tm == 0 && tp == 1) || (tm == 1 && tp == 0)) l = L_CROSSING;
+        if ((tm == 0 && tp == 0) || (tm == 1 && tp == 1)) l = L_BOUNCING;
+        if ((tp == 3 && tm == 1) || (tm == 3 && tp == 1)) l = L_LEFT_ON;
+        if ((tp == 3 && tm == 0) || (tm == 3 && tp == 0)) l = L_RIGHT_ON;
+        if ((tp == 3 && tm == 2) || (tm == 3 && tp == 2)) l = L_ON_ON;
+        if ((tm == 2 && tp == 1) || (tp == 2 && tm == 1)) l = L_ON_LEFT;
+        if ((tm == 2 && tp == 0) || (tp == 2 && tm == 0)) l = L_ON_RIGHT;
+        N.lab[I] = (signed char)l;
+      }
+      I = N.next[I];
+    } while (I != rootP);
+    I = rootP;
+    do {
+      if (isint(N, I) && N.nb[I] >= 0) N.lab[N.nb[I]] = N.lab[I];
+      I = N.next[I];
+    } while (I != rootP);
+  }
+  // step 2: intersection chains (:398-451)
+  {
+    int it = rootP;
+    do {
+      if (isint(N, it) && (N.lab[it] == L_LEFT_ON || N.lab[it] == L_RIGHT_ON)) {
+        int I = it;
+        int x = N.lab[I] == L_LEFT_ON ? 0 : 1;
+        int X = I;
+        do {
+          N.lab[I] = L_NONE;
+          if (N.nb[I] >= 0) N.lab[N.nb[I]] = L_NONE;
+          I = N.next[I];
+        } while (N.lab[I] == L_ON_ON);
+        int y = N.lab[I] == L_ON_LEFT ? 0 : 1;
+        signed char ct = (x != y) ? L_DELAYED_CROSSING : L_DELAYED_BOUNCING;
+        N.lab[X] = ct;
+        N.lab[I] = ct;
+        if (N.nb[X] >= 0) N.lab[N.nb[X]] = ct;
+        if (N.nb[I] >= 0) N.lab[N.nb[I]] = ct;
+      }
+      it = N.next[it];
+    } while (it != rootP);
+  }
+  // step 3.5: components without crossings (:474-543)
+  bool noInter[2] = {false, false}, ident[2] = {false, false};
+  for (int i = 0; i < 2; i++) {
+    int Pr = i == 0 ? rootP : rootQ, Qr = i == 0 ? rootQ : rootP;
+    bool nc = true;
+    int V = Pr;
+    do {
+      if (isint(N, V) && (N.lab[V] == L_CROSSING || N.lab[V] == L_DELAYED_CROSSING)) {
+        nc = false;
+        break;
+      }
+      V = N.next[V];
+    } while (V != Pr);
+    if (!nc) continue;
+    noInter[i] = true;
+    bool oo = true;
+    V = Pr;
+    do {
+      if (N.lab[V] != L_ON_ON) {
+        oo = false;
+        break;
+      }
+      V = N.next[V];
+    } while (V != Pr);
+    if (oo) {
+      ident[i] = true;
+      continue;
+    }
+    // getNonIntersectionPoint (polygon.h:480-491)
+    pt p;
+    bool have = false;
+    V = Pr;
+    do {
+      if (!isint(N, V)) {
+        p = P_(N, V);
+        have = true;
+        break;
+      }
+      V = N.next[V];
+    } while (V != Pr);
+    if (!have) {
+      V = Pr;
+      do {
+        int n = N.next[V], nb = N.nb[V], nnb = N.nb[n];
+        if (nb >= 0 && nnb != N.prev[nb] && nnb != N.next[nb]) {
+          pt a = P_(N, V), b = P_(N, n);
+          pt sm = {a.x + b.x, a.y + b.y};
+          p = pt{0.5 * sm.x, 0.5 * sm.y};
+          have = true;
+          break;
+        }
+        V = n;
+      } while (V != Pr);
+    }
+    if (!have) return GH_TRACE_FAIL;
+    if (node_pip(N, Qr, p))
+      if (!out_copy_ring(O, N, Pr)) return GH_OVERFLOW;
+  }
+  if (ident[0] && ident[1]) {
+    int V = rootQ, target = N.nb[rootP];
+    bool found = false;
+    do {
+      if (V == target) {
+        found = true;
+        break;
+      }
+      V = N.next[V];
+    } while (V != rootQ);
+    if (found && !out_copy_ring(O, N, rootP)) return GH_OVERFLOW;
+  }
+  // step 4: entry/exit (:549-655)
+  for (int i = 0; i < 2; i++) {
+    if (noInter[i]) continue;
+    int Pr = i == 0 ? rootP : rootQ, Qr = i == 0 ? rootQ : rootP;
+    int V = -1;
+    {
+      int W = Pr;
+      do {
+        if (!isint(N, W)) {
+          V = W;
+          break;
+        }
+        W = N.next[W];
+      } while (W != Pr);
+      if (V < 0) {  // getNonIntersectionVertex: temporary midpoint vertex
+        W = Pr;
+        do {
+          int n = N.next[W], nb = N.nb[W], nnb = N.nb[n];
+          if (nb >= 0 && nnb != N.prev[nb] && nnb != N.next[nb]) {
+            pt a = P_(N, W), b = P_(N, n), ao = ld(N.po, W), bo = ld(N.po, n);
+            pt sm = {a.x + b.x, a.y + b.y}, so = {ao.x + bo.x, ao.y + bo.y};
+            int T = node_new(N, pt{0.5 * sm.x, 0.5 * sm.y}, pt{0.5 * so.x, 0.5 * so.y});
+            if (T < 0) return GH_OVERFLOW;
+            node_insert_after(N, T, W);
+            V = T;
+            break;
+          }
+          W = n;
+        } while (W != Pr);
+      }
+      if (V < 0) return GH_TRACE_FAIL;
+    }
+    int status = ENTRY;
+    if (node_pip(N, Qr, P_(N, V))) toggle(status);
+    bool first = true;
+    int I = V;
+    do {
+      I = N.next[I];
+    } while (!isint(N, I) && I != V);
+    while (I != V) {
+      int l = N.lab[I];
+      if (l == L_CROSSING) {
+        N.enex[I] = (signed char)status;
+        toggle(status);
+      }
+      if (N.lab[I] == L_BOUNCING && status == EXIT) N.fl[I] |= F_SPLIT;
+      if (N.lab[I] == L_DELAYED_CROSSING) {
+        N.enex[I] = (signed char)status;
+        if (first) {
+          if (status == EXIT) N.lab[I] = L_CROSSING;
+          first = false;
+        } else {
+          if (status == ENTRY) N.lab[I] = L_CROSSING;
+          first = true;
+          toggle(status);
+        }
+      }
+      if (N.lab[I] == L_DELAYED_BOUNCING) {
+        N.enex[I] = (signed char)status;
+        if (first) {
+          if (status == EXIT) N.fl[I] |= F_CROSS;
+          first = false;
+        } else {
+          if (status == ENTRY) N.fl[I] |= F_CROSS;
+          first = true;
+        }
+        toggle(status);
+      }
+      do {
+        I = N.next[I];
+      } while (!isint(N, I) && I != V);
+    }
+  }
+  // step 5: split bouncing vertex pairs (:661-714); step 6 (:722-734)
+  {
+    const int nNodes = N.n;
+    int I = rootP;
+    do {
+      int nextI = N.next[I];
+      if ((N.fl[I] & F_SPLIT) && N.nb[I] >= 0 && (N.fl[N.nb[I]] & F_SPLIT)) {
+        int IQ = N.nb[I];
+        int VP = node_new(N, P_(N, I), ld(N.po, I));
+        int VQ = node_new(N, P_(N, IQ), ld(N.po, IQ));
+        if (VP < 0 || VQ < 0) return GH_OVERFLOW;
+        double sP = triA(P_(N, N.prev[I]), P_(N, I), P_(N, N.next[I]));
+        double sQ = triA(P_(N, N.prev[IQ]), P_(N, IQ), P_(N, N.next[IQ]));
+        if (sP * sQ > 0) {
+          node_link(N, I, VQ);
+          node_link(N, IQ, VP);
+        } else {
+          node_link(N, VP, VQ);
+        }
+        node_insert_after(N, VP, I);
+        node_insert_after(N, VQ, IQ);
+        N.enex[I] = EXIT;
+        N.enex[VP] = ENTRY;
+        N.enex[IQ] = EXIT;
+        N.enex[VQ] = ENTRY;
+        N.lab[I] = N.lab[VP] = N.lab[IQ] = N.lab[VQ] = L_CROSSING;
+        N.fl[I] &= ~F_SPLIT;  // handled
+      }
+      I = nextI;
+    } while (I != rootP);
+    (void)nNodes;
+    I = rootP;
+    do {
+      if ((N.fl[I] & F_CROSS) && N.nb[I] >= 0 && (N.fl[N.nb[I]] & F_CROSS)) {
+        N.lab[I] = L_CROSSING;
+        N.lab[N.nb[I]] = L_CROSSING;
+      }
+      I = N.next[I];
+    } while (I != rootP);
+  }
+  // createResult (:743-783)
+  {
+    const long long guard = 4LL * N.n + 64;
+#define IS_CI(v) (isint(N, (v)) && N.lab[(v)] == L_CROSSING)
+    int I = rootP;
+    bool have = IS_CI(I);
+    if (!have) {
+      do {
+        I = N.next[I];
+      } while (!IS_CI(I) && I != rootP);
+      have = I != rootP;
+    }
+    while (have) {
+      if (O.ncomp >= O.capComp) return GH_OVERFLOW;
+      int R = -1, V = I;
+      N.fl[V] &= ~F_INT;
+      long long steps = 0;
+      do {
+        int status = N.enex[V];
+        toggle(status);
+        do {
+          V = status == EXIT ? N.next[V] : N.prev[V];
+          N.fl[V] &= ~F_INT;
+          int w = out_new(O, P_(N, V), ld(N.po, V));
+          if (w < 0) return GH_OVERFLOW;
+          if (R < 0) R = w;
+          else out_append(O, R, w);
+          if (++steps > guard) return GH_TRACE_FAIL;
+        } while (!(N.enex[V] == status) && V != I);
+        if (V != I) {
+          V = N.nb[V];
+          if (V < 0) return GH_TRACE_FAIL;
+          N.fl[V] &= ~F_INT;
+        }
+      } while (V != I);
+      O.compRoot[O.ncomp++] = R;
+      do {
+        I = N.next[I];
+      } while (!IS_CI(I) && I != rootP);
+      have = I != rootP;
+    }
+#undef IS_CI
+  }
+  return GH_OK;
+}
+
+// cleanUpResult (:791-807) on one output component; returns the new root
+__device__ int out_cleanup(OutBuf &O, int r) {
+  while (r >= 0 &&
+         fabs(triA(ld(O.pn, O.prev[r]), ld(O.pn, r), ld(O.pn, O.next[r]))) < kGhEps) {
+    int V = r;
+    r = O.next[V];
+    if (O.next[r] == r) r = -1;
+    O.next[O.prev[V]] = O.next[V];
+    O.prev[O.next[V]] = O.prev[V];
+  }
+  if (r < 0) return -1;
+  int it = r;
+  do {
+    int nx = O.next[it];
+    if (fabs(triA(ld(O.pn, O.prev[it]), ld(O.pn, it), ld(O.pn, O.next[it]))) < kGhEps) {
+      O.next[O.prev[it]] = O.next[it];
+      O.prev[O.next[it]] = O.prev[it];
+    }
+    it = nx;
+  } while (it != r);
+  return r;
+}
+
+struct TraceArgs {
+  const long long *offP, *offQ, *psP2, *psQ2;
+  const long long *nodeOff;  // [M+1] node / output capacity prefix
+  const long long *compOff;  // [M+1] component capacity prefix
+  double *npn, *npo;
+  int *nnext, *nprev, *nnb;
+  unsigned char *nfl;
+  signed char *nlab, *nenex;
+  double *opn, *opo;
+  int *onext, *oprev;
+  int *ocompRoot;
+  int *outNComp, *outNVerts;
+  int *compLen;  // [comp capacity] final component sizes
+  double *xyTmp; // [node capacity] compacted result vertices per pair
+  int cleanup;
+};
+
+__global__ void __launch_bounds__(64)
+    k_label_trace(long M, int *status, SlotArrays s, TraceArgs a) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  a.outNComp[k] = 0;
+  a.outNVerts[k] = 0;
+  if (status[k] != GH_OK) return;
+  const long long sbP = a.psP2[a.offP[k]], cP = a.psP2[a.offP[k + 1]] - sbP;
+  const long long sbQ = a.psQ2[a.offQ[k]], cQ = a.psQ2[a.offQ[k + 1]] - sbQ;
+  const long long nb0 = a.nodeOff[k], cap = a.nodeOff[k + 1] - nb0;
+  Nodes N;
+  N.pn = a.npn + 2 * nb0;
+  N.po = a.npo + 2 * nb0;
+  N.next = a.nnext + nb0;
+  N.prev = a.nprev + nb0;
+  N.nb = a.nnb + nb0;
+  N.fl = a.nfl + nb0;
+  N.lab = a.nlab + nb0;
+  N.enex = a.nenex + nb0;
+  N.n = 0;
+  N.cap = (int)cap;
+  // P list from the P slots, parents flagged by their alpha (ghcuda.cpp:246-273)
+  const long long pE0 = a.offP[k], pE1 = a.offP[k + 1];
+  long long pe = pE0;
+  for (long long j = 0; j < cP; j++) {
+    const long long sl = sbP + j;
+    int v = node_new(N, ld(s.iP, sl), ld(s.oP, sl));
+    N.prev[v] = v == 0 ? (int)(cP - 1) : v - 1;
+    N.next[v] = v == cP - 1 ? 0 : v + 1;
+    if (pe < pE1 && a.psP2[pe] == sl) {
+      N.fl[v] = F_SRC | (s.aP[sl] != kNone ? F_INT : 0);
+      pe++;
+    } else {
+      N.fl[v] = F_INT;
+    }
+  }
+  // Q list; parents are linked when phase 1 recorded a neighbour
+  // (ghcuda.cpp:277-322, with X-overlap parents linked as link() does)
+  const long long qE0 = a.offQ[k], qE1 = a.offQ[k + 1];
+  long long qe = qE0;
+  for (long long j = 0; j < cQ; j++) {
+    const long long sl = sbQ + j;
+    int v = node_new(N, ld(s.iQ, sl), ld(s.oQ, sl));
+    N.prev[v] = j == 0 ? (int)(cP + cQ - 1) : v - 1;
+    N.next[v] = j == cQ - 1 ? (int)cP : v + 1;
+    bool linked;
+    if (qe < qE1 && a.psQ2[qe] == sl) {
+      N.fl[v] = F_SRC;
+      linked = s.aQ[sl] != kNone || s.nbQ[sl] > 0;
+      qe++;
+    } else {
+      N.fl[v] = 0;
+      linked = true;
+    }
+    if (linked) {
+      long long kp = (long long)s.nbQ[sl] - 1 - sbP;
+      if (kp >= 0 && kp < cP) node_link(N, v, (int)kp);
+      else N.fl[v] |= F_INT;
+    }
+  }
+  OutBuf O;
+  O.pn = a.opn + 2 * nb0;
+  O.po = a.opo + 2 * nb0;
+  O.next = a.onext + nb0;
+  O.prev = a.oprev + nb0;
+  O.n = 0;
+  O.cap = (int)cap;
+  O.compRoot = a.ocompRoot + a.compOff[k];
+  O.ncomp = 0;
+  O.capComp = (int)(a.compOff[k + 1] - a.compOff[k]);
+  int rc = label_and_trace(N, 0, (int)cP, O);
+  if (rc != GH_OK) {
+    status[k] = rc;
+    return;
+  }
+  // clean-up and compaction into xyTmp / compLen (per pair regions)
+  int nc = 0, nv = 0;
+  double *xy = a.xyTmp + 2 * nb0;
+  int *clen = a.compLen + a.compOff[k];
+  for (int c = 0; c < O.ncomp; c++) {
+    int r = O.compRoot[c];
+    if (a.cleanup) r = out_cleanup(O, r);
+    if (r < 0) continue;
+    int len = 0, v = r;
+    do {
+      st(xy, nv + len, ld(O.po, v));
+      len++;
+      v = O.next[v];
+    } while (v != r);
+    clen[nc++] = len;
+    nv += len;
+  }
+  a.outNComp[k] = nc;
+  a.outNVerts[k] = nv;
+}
+
+// node / component capacity per pair
+__global__ void k_capacity(long M, const long long *offP, const long long *offQ,
+                           const long long *psP2, const long long *psQ2, const int *status,
+                           int *nodeCap, int *compCap) {
+  long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  if (status[k] != GH_OK) {
+    nodeCap[k] = 0;
+    compCap[k] = 0;
+    return;
+  }
+  long long cP = psP2[offP[k + 1]] - psP2[offP[k]];
+  long long cQ = psQ2[offQ[k + 1]] - psQ2[offQ[k]];
+  long long n = 2 * (cP + cQ) + 4;
+  nodeCap[k] = (int)(n > 0x7fffffff ? 0x7fffffff : n);
+  compCap[k] = (int)(n / 2 + 4);
+}
+
+// final dense output: result vertices and component sizes in pair order
+__global__ void __launch_bounds__(256)
+    k_gather_out(long M, const long long *nodeOff, const long long *compCapOff,
+                 const long long *vOff, const long long *cOff, const double *xyTmp,
+                 const int *compLen, double *xy, int *compSizes) {
+  const long k = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= M) return;
+  const int lane = __lane_id();
+  const long long nv = vOff[k + 1] - vOff[k], nc = cOff[k + 1] - cOff[k];
+  for (long long i = lane; i < nv; i += 64) {
+    xy[2 * (vOff[k] + i)] = xyTmp[2 * (nodeOff[k] + i)];
+    xy[2 * (vOff[k] + i) + 1] = xyTmp[2 * (nodeOff[k] + i) + 1];
+  }
+  for (long long i = lane; i < nc; i += 64) compSizes[cOff[k] + i] = compLen[compCapOff[k] + i];
+}
+
+template <typename T>
+static T *wsT(Context &c, const char *name, long long count) {
+  return (T *)c.ws.get(name, sizeof(T) * (size_t)(count > 0 ? count : 1));
+}
+
+void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
+                const ClipOptions &opt, ClipResult &res, Phase1View *p1) {
+  res = ClipResult();
+  res.M = M;
+  if (M <= 0) return;
+  PairMeta pm;
+  pm.startP = wsT<long>(c, "gh_startP", M);
+  pm.startQ = wsT<long>(c, "gh_startQ", M);
+  pm.nP = wsT<int>(c, "gh_nP", M);
+  pm.nQ = wsT<int>(c, "gh_nQ", M);
+  pm.swapped = wsT<int>(c, "gh_swapped", M);
+  pm.status = wsT<int>(c, "gh_status", M);
+  pm.norm = wsT<double>(c, "gh_norm", 3 * M);
+  pm.cmbrN = wsT<double>(c, "gh_cmbrN", 4 * M);
+  int *eP = wsT<int>(c, "gh_eP", M), *eQ = wsT<int>(c, "gh_eQ", M);
+  long long *offP = wsT<long long>(c, "gh_offP", M + 1);
+  long long *offQ = wsT<long long>(c, "gh_offQ", M + 1);
+  {
+    LaunchTimer t(c, "gh_pair_setup");
+    hipLaunchKernelGGL(k_pair_setup, dim3(grid_for(M, 4)), dim3(256), 0, c.stream, M, pairs,
+                       in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum, in.oVPSNum,
+                       opt.normalize, opt.noswap, pm, eP, eQ);
+    HIP_CHECK(hipGetLastError());
+  }
+  scan_i32_to_i64(c, eP, offP, M);
+  scan_i32_to_i64(c, eQ, offQ, M);
+  const long edgesP = (long)read_scalar(c, offP + M);
+  const long edgesQ = (long)read_scalar(c, offQ + M);
+  int *mapP = wsT<int>(c, "gh_mapP", edgesP), *mapQ = wsT<int>(c, "gh_mapQ", edgesQ);
+  double *ringP = wsT<double>(c, "gh_ringP", 2 * edgesP);
+  double *ringQ = wsT<double>(c, "gh_ringQ", 2 * edgesQ);
+  {
+    LaunchTimer t(c, "gh_pair_fill");
+    hipLaunchKernelGGL(k_pair_fill, dim3(grid_for(M, 4)), dim3(256), 0, c.stream, M, pairs,
+                       in.bCoords, in.oCoords, pm, (const long long *)offP,
+                       (const long long *)offQ, mapP, mapQ, ringP, ringQ);
+    HIP_CHECK(hipGetLastError());
+  }
+  int *cnt1P = wsT<int>(c, "gh_cnt1P", edgesP), *cnt2P = wsT<int>(c, "gh_cnt2P", edgesP);
+  int *cnt1Q = wsT<int>(c, "gh_cnt1Q", edgesQ), *cnt2Q = wsT<int>(c, "gh_cnt2Q", edgesQ);
+  int *boolP = wsT<int>(c, "gh_boolP", edgesP), *boolQ = wsT<int>(c, "gh_boolQ", edgesQ);
+  unsigned long long *stats = wsT<unsigned long long>(c, "gh_stats", 2);
+  hipLaunchKernelGGL(k_fill_u64_pair, dim3(1), dim3(64), 0, c.stream, stats);
+  if (edgesP + edgesQ > 0) {
+    LaunchTimer t(c, "gh_count");
+    hipLaunchKernelGGL(k_count, dim3(grid_for(edgesP + edgesQ, 256)), dim3(256), 0, c.stream,
+                       edgesP, edgesQ, mapP, mapQ, (const long long *)offP,
+                       (const long long *)offQ, ringP, ringQ, pm.cmbrN, cnt1P, cnt2P, cnt1Q,
+                       cnt2Q, boolP, boolQ, stats);
+    HIP_CHECK(hipGetLastError());
+  }
+  long long *psP1 = wsT<long long>(c, "gh_psP1", edgesP + 1);
+  long long *psP2 = wsT<long long>(c, "gh_psP2", edgesP + 1);
+  long long *psQ1 = wsT<long long>(c, "gh_psQ1", edgesQ + 1);
+  long long *psQ2 = wsT<long long>(c, "gh_psQ2", edgesQ + 1);
+  scan_i32_to_i64(c, cnt1P, psP1, edgesP);
+  scan_i32_to_i64(c, cnt2P, psP2, edgesP);
+  scan_i32_to_i64(c, cnt1Q, psQ1, edgesQ);
+  scan_i32_to_i64(c, cnt2Q, psQ2, edgesQ);
+  hipLaunchKernelGGL(k_pair_check, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
+                     (const long long *)offP, (const long long *)offQ, (const long long *)psP1,
+                     (const long long *)psQ1, (const long long *)psP2, (const long long *)psQ2,
+                     pm.status, stats + 1);
+  HIP_CHECK(hipGetLastError());
+  const long slotsP = (long)read_scalar(c, psP2 + edgesP);
+  const long slotsQ = (long)read_scalar(c, psQ2 + edgesQ);
+  if (slotsP >= 0x7fffffffL || slotsQ >= 0x7fffffffL)
+    throw Error{GCMF_ERR_OVERFLOW, "clip batch exceeds 2^31 intersection slots"};
+  SlotArrays s;
+  s.iP = wsT<double>(c, "gh_iP", 2 * slotsP);
+  s.iP2 = wsT<double>(c, "gh_iP2", 2 * slotsP);
+  s.oP = wsT<double>(c, "gh_oP", 2 * slotsP);
+  s.oP2 = wsT<double>(c, "gh_oP2", 2 * slotsP);
+  s.iQ = wsT<double>(c, "gh_iQ", 2 * slotsQ);
+  s.iQ2 = wsT<double>(c, "gh_iQ2", 2 * slotsQ);
+  s.oQ = wsT<double>(c, "gh_oQ", 2 * slotsQ);
+  s.oQ2 = wsT<double>(c, "gh_oQ2", 2 * slotsQ);
+  s.aP = wsT<int>(c, "gh_aP", slotsP);
+  s.aP2 = wsT<int>(c, "gh_aP2", slotsP);
+  s.aQ = wsT<int>(c, "gh_aQ", slotsQ);
+  s.aQ2 = wsT<int>(c, "gh_aQ2", slotsQ);
+  s.nbP = wsT<int>(c, "gh_nbP", slotsP);
+  s.nbP2 = wsT<int>(c, "gh_nbP2", slotsP);
+  s.nbQ = wsT<int>(c, "gh_nbQ", slotsQ);
+  s.nbQ2 = wsT<int>(c, "gh_nbQ2", slotsQ);
+  s.lP = wsT<int>(c, "gh_lP", slotsP);
+  s.lQ = wsT<int>(c, "gh_lQ", slotsQ);
+  int *nmap = wsT<int>(c, "gh_nmap", slotsQ);
+  const long slotsMax = slotsP > slotsQ ? slotsP : slotsQ;
+  if (slotsMax > 0) {
+    hipLaunchKernelGGL(k_init_slots, dim3(grid_for(slotsMax, 256)), dim3(256), 0, c.stream,
+                       slotsP, slotsQ, s, nmap);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (edgesQ > 0) {
+    {
+      LaunchTimer t(c, "gh_nmap");
+      hipLaunchKernelGGL(k_nmap, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream, edgesQ,
+                         mapQ, (const long long *)offP, (const long long *)offQ, ringP, ringQ,
+                         (const long long *)psP1, (const long long *)psQ1,
+                         (const long long *)psQ2, pm.status, nmap);
+      HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_q_parents, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream, edgesQ,
+                       mapQ, (const long long *)offQ, ringQ, (const long long *)psQ2, pm.status,
+                       pm, in.bCoords, in.oCoords, s);
+    HIP_CHECK(hipGetLastError());
+  }
+  Owners own;
+  own.a = wsT<int>(c, "gh_ownA", slotsQ);
+  own.c = wsT<int>(c, "gh_ownC", slotsQ);
+  own.n = wsT<int>(c, "gh_ownN", slotsQ);
+  int *ownP = wsT<int>(c, "gh_ownP", slotsP);
+  int *ownL = wsT<int>(c, "gh_ownL", slotsQ);
+  fill_i32(c, own.a, -1, slotsQ);
+  fill_i32(c, own.c, -1, slotsQ);
+  fill_i32(c, own.n, -1, slotsQ);
+  fill_i32(c, ownP, -1, slotsP);
+  fill_i32(c, ownL, -1, slotsQ);
+  if (edgesP > 0) {
+    {
+      LaunchTimer t(c, "gh_calc_own");
+      hipLaunchKernelGGL(k_calc<true>, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
+                         edgesP, mapP, (const long long *)offP, (const long long *)offQ, ringP,
+                         ringQ, (const long long *)psP1, (const long long *)psP2,
+                         (const long long *)psQ1, (const long long *)psQ2, pm.status, pm,
+                         in.bCoords, in.oCoords, nmap, s, own);
+      HIP_CHECK(hipGetLastError());
+    }
+    LaunchTimer t(c, "gh_calc");
+    hipLaunchKernelGGL(k_calc<false>, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
+                       edgesP, mapP, (const long long *)offP, (const long long *)offQ, ringP,
+                       ringQ, (const long long *)psP1, (const long long *)psP2,
+                       (const long long *)psQ1, (const long long *)psQ2, pm.status, pm,
+                       in.bCoords, in.oCoords, nmap, s, own);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (edgesQ > 0) {
+    hipLaunchKernelGGL(k_sort_q<true>, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
+                       edgesQ, mapQ, (const long long *)psQ2, pm.status, s, ownP);
+    HIP_CHECK(hipGetLastError());
+    LaunchTimer t(c, "gh_sort_q");
+    hipLaunchKernelGGL(k_sort_q<false>, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
+                       edgesQ, mapQ, (const long long *)psQ2, pm.status, s, ownP);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (edgesP > 0) {
+    hipLaunchKernelGGL(k_init_label<true>, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
+                       edgesP, mapP, (const long long *)offP, (const long long *)offQ,
+                       (const long long *)psP2, (const long long *)psQ2, pm.status, s, ownL);
+    HIP_CHECK(hipGetLastError());
+    LaunchTimer t(c, "gh_init_label");
+    hipLaunchKernelGGL(k_init_label<false>, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
+                       edgesP, mapP, (const long long *)offP, (const long long *)offQ,
+                       (const long long *)psP2, (const long long *)psQ2, pm.status, s, ownL);
+    HIP_CHECK(hipGetLastError());
+  }
+  // capacities of the per-pair node, output and component regions
+  int *nodeCap = wsT<int>(c, "gh_nodeCap", M), *compCap = wsT<int>(c, "gh_compCap", M);
+  long long *nodeOff = wsT<long long>(c, "gh_nodeOff", M + 1);
+  long long *compCapOff = wsT<long long>(c, "gh_compCapOff", M + 1);
+  hipLaunchKernelGGL(k_capacity, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
+                     (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
+                     (const long long *)psQ2, pm.status, nodeCap, compCap);
+  HIP_CHECK(hipGetLastError());
+  scan_i32_to_i64(c, nodeCap, nodeOff, M);
+  scan_i32_to_i64(c, compCap, compCapOff, M);
+  const long nodes = (long)read_scalar(c, nodeOff + M);
+  const long comps = (long)read_scalar(c, compCapOff + M);
+  TraceArgs ta;
+  ta.offP = offP;
+  ta.offQ = offQ;
+  ta.psP2 = psP2;
+  ta.psQ2 = psQ2;
+  ta.nodeOff = nodeOff;
+  ta.compOff = compCapOff;
+  ta.npn = wsT<double>(c, "gh_npn", 2 * nodes);
+  ta.npo = wsT<double>(c, "gh_npo", 2 * nodes);
+  ta.nnext = wsT<int>(c, "gh_nnext", nodes);
+  ta.nprev = wsT<int>(c, "gh_nprev", nodes);
+  ta.nnb = wsT<int>(c, "gh_nnb", nodes);
+  ta.nfl = wsT<unsigned char>(c, "gh_nfl", nodes);
+  ta.nlab = wsT<signed char>(c, "gh_nlab", nodes);
+  ta.nenex = wsT<signed char>(c, "gh_nenex", nodes);
+  ta.opn = wsT<double>(c, "gh_opn", 2 * nodes);
+  ta.opo = wsT<double>(c, "gh_opo", 2 * nodes);
+  ta.onext = wsT<int>(c, "gh_onext", nodes);
+  ta.oprev = wsT<int>(c, "gh_oprev", nodes);
+  ta.ocompRoot = wsT<int>(c, "gh_ocompRoot", comps);
+  ta.outNComp = wsT<int>(c, "gh_outNComp", M);
+  ta.outNVerts = wsT<int>(c, "gh_outNVerts", M);
+  ta.compLen = wsT<int>(c, "gh_compLen", comps);
+  ta.xyTmp = wsT<double>(c, "gh_xyTmp", 2 * nodes);
+  ta.cleanup = opt.cleanup;
+  {
+    LaunchTimer t(c, "gh_label_trace");
+    hipLaunchKernelGGL(k_label_trace, dim3(grid_for(M, 64)), dim3(64), 0, c.stream, M,
+                       pm.status, s, ta);
+    HIP_CHECK(hipGetLastError());
+  }
+  res.vertOff = wsT<long long>(c, "gh_vOff", M + 1);
+  res.compOff = wsT<long long>(c, "gh_cOff", M + 1);
+  scan_i32_to_i64(c, ta.outNVerts, res.vertOff, M);
+  scan_i32_to_i64(c, ta.outNComp, res.compOff, M);
+  res.totalVerts = (long)read_scalar(c, res.vertOff + M);
+  res.totalComps = (long)read_scalar(c, res.compOff + M);
+  res.xy = wsT<double>(c, "gh_out_xy", 2 * res.totalVerts);
+  res.compSizes = wsT<int>(c, "gh_out_comp", res.totalComps);
+  {
+    LaunchTimer t(c, "gh_gather_out");
+    hipLaunchKernelGGL(k_gather_out, dim3(grid_for(M, 4)), dim3(256), 0, c.stream, M,
+                       (const long long *)nodeOff, (const long long *)compCapOff,
+                       (const long long *)res.vertOff, (const long long *)res.compOff, ta.xyTmp,
+                       ta.compLen, res.xy, res.compSizes);
+    HIP_CHECK(hipGetLastError());
+  }
+  res.status = pm.status;
+  res.swapped = pm.swapped;
+  res.ncomp = ta.outNComp;
+  res.nverts = ta.outNVerts;
+  unsigned long long hs[2];
+  HIP_CHECK(hipMemcpyAsync(hs, stats, 16, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  res.nEvalP = (long long)hs[0];
+  res.totalIntersections = (long long)hs[1];
+  if (p1) {
+    p1->M = M;
+    p1->edgesP = edgesP;
+    p1->edgesQ = edgesQ;
+    p1->slotsP = slotsP;
+    p1->slotsQ = slotsQ;
+    p1->offP = offP;
+    p1->offQ = offQ;
+    p1->psP1 = psP1;
+    p1->psP2 = psP2;
+    p1->psQ1 = psQ1;
+    p1->psQ2 = psQ2;
+    p1->boolP = boolP;
+    p1->boolQ = boolQ;
+    p1->nmapQ = nmap;
+    p1->iP = s.iP;
+    p1->iQ = s.iQ;
+    p1->oP = s.oP;
+    p1->oQ = s.oQ;
+    p1->aP = s.aP;
+    p1->aQ = s.aQ;
+    p1->lP = s.lP;
+    p1->lQ = s.lQ;
+    p1->nbP = s.nbP;
+    p1->nbQ = s.nbQ;
+    p1->status = pm.status;
+    p1->norm = pm.norm;
+  }
+}
+
+}  // namespace gcmf

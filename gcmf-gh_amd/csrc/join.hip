@@ -1,0 +1,562 @@
+// join.hip -- the GCMF filter-and-refine spatial join on gfx950.
+//
+// Stages (reference semantics in brackets):
+//   1. MBR filter as sort-and-sweep [Join.h:5-153, GPU_MBR.h:339-539]: the
+//      2(B+O) x-endpoint keys (mod 1e16, the reference sorts 16 decimal
+//      digits) are radix sorted with the endpoint index as a stable
+//      tie-break; each polygon then scans the contiguous rank range between
+//      its own two endpoints in a structure-of-arrays record stream
+//      (endpoint id, partner y-interval), one wave64 per polygon, ballots
+//      for counting and ordered output.
+//   2. CMBR + PiP candidate flags [Join.h:397-432, GPU_Join.h:461-508].
+//   3. PiP "within" refine, one wave per candidate, crossing parity from
+//      ballot popcounts [Join.h:437-467, GPU_Join.h:839-943].
+//   4. CMF edge filter count + apply, one wave per surviving pair, ordered
+//      edge lists from ballot ranks [Join.h:157-287, GPU_Join.h:511-735].
+//   5. Segment-intersection refine, one wave per CMF survivor, wave-uniform
+//      early exit [Join.h:356-393, GPU_Join.h:326-456].
+#include "gcmf_device.hpp"
+#include "gcmf_internal.hpp"
+#include "gcmf_join.hpp"
+
+namespace gcmf {
+
+constexpr uint64_t kKeyMod = 10000000000000000ULL;  // 16 decimal digits
+
+// ------------------------------------------------------------ stage 1 ----
+__global__ void k_endpoints(long B, long O, const uint64_t *bMBR, const uint64_t *oMBR,
+                            uint64_t *keys, int *idx, uint64_t *Y, unsigned long long *mn,
+                            unsigned long long *mx) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t lo = ~0ULL, hi = 0;
+  if (e < B + O) {
+    const uint64_t *m = e < B ? bMBR + 4 * e : oMBR + 4 * (e - B);
+    uint64_t x1 = m[0] % kKeyMod, x2 = m[2] % kKeyMod;
+    keys[2 * e] = x1;
+    keys[2 * e + 1] = x2;
+    idx[2 * e] = (int)(2 * e);
+    idx[2 * e + 1] = (int)(2 * e + 1);
+    Y[2 * e] = m[1];
+    Y[2 * e + 1] = m[3];
+    lo = x1 < x2 ? x1 : x2;
+    hi = x1 < x2 ? x2 : x1;
+  }
+  // wave min/max, then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
+  }
+  if (__lane_id() == 0) {
+    atomicMin(mn, (unsigned long long)lo);
+    atomicMax(mx, (unsigned long long)hi);
+  }
+}
+
+__global__ void k_init_minmax(unsigned long long *mn, unsigned long long *mx) {
+  if (threadIdx.x == 0) {
+    *mn = ~0ULL;
+    *mx = 0ULL;
+  }
+}
+
+__global__ void k_sub_key(uint64_t *keys, long n, const unsigned long long *mn) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] -= *mn;
+}
+
+// rank[perm[i]] = i; sweep records in sorted order
+__global__ void k_sweep_prep(const int *perm, long n, const uint64_t *Y, int *rank, int *rec_id,
+                             uint64_t *rec_ylo, uint64_t *rec_yhi) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int e = perm[i];
+  rank[e] = (int)i;
+  rec_id[i] = e;
+  int m = e & ~1;
+  rec_ylo[i] = Y[m];
+  rec_yhi[i] = Y[m + 1];
+}
+
+// one wave per polygon element; write == nullptr -> count only
+__global__ void __launch_bounds__(256)
+    k_sweep(long B, long O, const int *rank, const int *rec_id, const uint64_t *rec_ylo,
+            const uint64_t *rec_yhi, const uint64_t *Y, int *counts, const long long *offs,
+            int *pairs) {
+  const long e = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= B + O) return;
+  const int lane = __lane_id();
+  const int lo = rank[2 * e], hi = rank[2 * e + 1];
+  const uint64_t ey1 = Y[2 * e], ey2 = Y[2 * e + 1];
+  const bool overlay = e >= B;
+  const int lim = (int)(2 * B);
+  long long pos = pairs ? offs[e] : 0;
+  int cnt = 0;
+  for (int i0 = lo + 1; i0 < hi; i0 += 64) {
+    int i = i0 + lane;
+    bool ok = false;
+    int m = 0;
+    if (i < hi) {
+      m = rec_id[i];
+      bool other = overlay ? (m < lim) : (m >= lim);
+      ok = other && !(m & 1) && !(ey1 > rec_yhi[i]) && !(rec_ylo[i] > ey2);
+    }
+    unsigned long long msk = __ballot(ok);
+    if (pairs && ok) {
+      long long p = pos + __popcll(msk & ((1ULL << lane) - 1ULL));
+      if (!overlay) {
+        pairs[2 * p] = (int)e;
+        pairs[2 * p + 1] = m / 2 - (int)B;
+      } else {
+        pairs[2 * p] = m / 2;
+        pairs[2 * p + 1] = (int)(e - B);
+      }
+    }
+    pos += __popcll(msk);
+    cnt += __popcll(msk);
+  }
+  if (!pairs && lane == 0) counts[e] = cnt;
+}
+
+long mbr_filter(Context &c, long B, long O, const uint64_t *bMBR, const uint64_t *oMBR,
+                int **counts_out, int **pairs_out, MbrSortView *sv) {
+  const long N = B + O, n = 2 * N;
+  if (N <= 0) return 0;
+  uint64_t *keys = (uint64_t *)c.ws.get("mbr_keys", 8 * (size_t)n);
+  int *idx = (int *)c.ws.get("mbr_idx", 4 * (size_t)n);
+  uint64_t *Y = (uint64_t *)c.ws.get("mbr_Y", 8 * (size_t)n);
+  unsigned long long *mm = (unsigned long long *)c.ws.get("mbr_minmax", 16);
+  uint64_t *skeys = (uint64_t *)c.ws.get("mbr_skeys", 8 * (size_t)n);
+  int *perm = (int *)c.ws.get("mbr_perm", 4 * (size_t)n);
+  int *rank = (int *)c.ws.get("mbr_rank", 4 * (size_t)n);
+  int *rec_id = (int *)c.ws.get("mbr_rec_id", 4 * (size_t)n);
+  uint64_t *rec_ylo = (uint64_t *)c.ws.get("mbr_rec_ylo", 8 * (size_t)n);
+  uint64_t *rec_yhi = (uint64_t *)c.ws.get("mbr_rec_yhi", 8 * (size_t)n);
+  int *counts = (int *)c.ws.get("mbr_counts", 4 * (size_t)N);
+  long long *offs = (long long *)c.ws.get("mbr_offs", 8 * (size_t)(N + 1));
+  hipLaunchKernelGGL(k_init_minmax, dim3(1), dim3(64), 0, c.stream, mm, mm + 1);
+  {
+    LaunchTimer t(c, "mbr_endpoints");
+    hipLaunchKernelGGL(k_endpoints, dim3(grid_for(N, 256)), dim3(256), 0, c.stream, B, O, bMBR,
+                       oMBR, keys, idx, Y, mm, mm + 1);
+    HIP_CHECK(hipGetLastError());
+  }
+  unsigned long long h[2];
+  HIP_CHECK(hipMemcpyAsync(h, mm, 16, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  unsigned long long range = h[1] - h[0];
+  int bits = 0;
+  while (bits < 64 && (range >> bits) != 0) bits++;
+  hipLaunchKernelGGL(k_sub_key, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, keys, n, mm);
+  HIP_CHECK(hipGetLastError());
+  radix_sort_u64(c, keys, idx, skeys, perm, n, bits);
+  {
+    LaunchTimer t(c, "mbr_sweep_prep");
+    hipLaunchKernelGGL(k_sweep_prep, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, perm, n, Y,
+                       rank, rec_id, rec_ylo, rec_yhi);
+    HIP_CHECK(hipGetLastError());
+  }
+  {
+    LaunchTimer t(c, "mbr_sweep_count");
+    hipLaunchKernelGGL(k_sweep, dim3(grid_for(N, 4)), dim3(256), 0, c.stream, B, O, rank, rec_id,
+                       rec_ylo, rec_yhi, Y, counts, (const long long *)nullptr, (int *)nullptr);
+    HIP_CHECK(hipGetLastError());
+  }
+  scan_i32_to_i64(c, counts, offs, N);
+  long K = (long)read_scalar(c, offs + N);
+  int *pairs = (int *)c.ws.get("mbr_pairs", 8 * (size_t)(K > 0 ? K : 1));
+  if (K > 0) {
+    LaunchTimer t(c, "mbr_sweep_write");
+    hipLaunchKernelGGL(k_sweep, dim3(grid_for(N, 4)), dim3(256), 0, c.stream, B, O, rank, rec_id,
+                       rec_ylo, rec_yhi, Y, counts, (const long long *)offs, pairs);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (counts_out) *counts_out = counts;
+  if (pairs_out) *pairs_out = pairs;
+  if (sv) {
+    sv->perm = perm;
+    sv->rank = rank;
+    sv->n = n;
+  }
+  return K;
+}
+
+// ------------------------------------------------------------ stage 2 ----
+__global__ void k_get_cmbr(long K, const int *pairs, const double *bMBR2, const double *oMBR2,
+                           double *cmbr, int *pipFlag, signed char *pipType, int *joinFlag) {
+  long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  int b = pairs[2 * k], o = pairs[2 * k + 1];
+  double x1 = bMBR2[4 * b], x2 = bMBR2[4 * b + 2], y1 = bMBR2[4 * b + 1], y2 = bMBR2[4 * b + 3];
+  double a1 = oMBR2[4 * o], a2 = oMBR2[4 * o + 2], b1 = oMBR2[4 * o + 1], b2 = oMBR2[4 * o + 3];
+  double cX1 = a1 > x1 ? a1 : x1, cX2 = a2 < x2 ? a2 : x2;
+  double cY1 = b1 > y1 ? b1 : y1, cY2 = b2 < y2 ? b2 : y2;
+  cmbr[4 * k] = cX1;
+  cmbr[4 * k + 1] = cY1;
+  cmbr[4 * k + 2] = cX2;
+  cmbr[4 * k + 3] = cY2;
+  joinFlag[k] = 1;
+  int f = 0, t = 0;
+  if (x1 == cX1 && x2 == cX2 && y1 == cY1 && y2 == cY2) {
+    f = 1;
+    t = 1;
+  } else if (a1 == cX1 && a2 == cX2 && b1 == cY1 && b2 == cY2) {
+    f = 1;
+    t = 2;
+  }
+  pipFlag[k] = f;
+  pipType[k] = (signed char)t;
+}
+
+// ------------------------------------------------------------ stage 3 ----
+__global__ void __launch_bounds__(256)
+    k_pip(long nCand, const int *cand, const int *pairs, const signed char *pipType,
+          const double *bCoords, const long *bVPSNum, const double *oCoords, const long *oVPSNum,
+          int *pipFlag, int *joinFlag) {
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nCand) return;
+  const int lane = __lane_id();
+  const int k = cand[w];
+  const int b = pairs[2 * k], o = pairs[2 * k + 1];
+  const long bS = b > 0 ? bVPSNum[b - 1] : 0, oS = o > 0 ? oVPSNum[o - 1] : 0;
+  double pX, pY;
+  const double *poly;
+  long vNum;
+  if (pipType[k] == 1) {
+    pX = bCoords[2 * bS];
+    pY = bCoords[2 * bS + 1];
+    vNum = oVPSNum[o] - oS;
+    poly = oCoords + 2 * oS;
+  } else {
+    pX = oCoords[2 * oS];
+    pY = oCoords[2 * oS + 1];
+    vNum = bVPSNum[b] - bS;
+    poly = bCoords + 2 * bS;
+  }
+  int parity = 0;
+  for (long i0 = 0; i0 < vNum; i0 += 64) {
+    long i = i0 + lane;
+    bool tog = false;
+    if (i < vNum) {
+      long j = i == 0 ? vNum - 1 : i - 1;
+      double X1 = poly[2 * i], Y1 = poly[2 * i + 1], X2 = poly[2 * j], Y2 = poly[2 * j + 1];
+      if ((Y1 > pY) != (Y2 > pY)) tog = pX < (X2 - X1) * (pY - Y1) / (Y2 - Y1) + X1;
+    }
+    parity ^= __popcll(__ballot(tog)) & 1;
+  }
+  if (lane == 0) {
+    pipFlag[k] = parity;
+    if (parity == 1) joinFlag[k] = 0;
+  }
+}
+
+// ------------------------------------------------------------ stage 4 ----
+__device__ __forceinline__ bool cmf_edge(const double *c, int vNum, int i, double cX1, double cY1,
+                                         double cX2, double cY2) {
+  double x1 = c[2 * i], y1 = c[2 * i + 1], x2, y2;
+  if (i < vNum - 1) {
+    x2 = c[2 * (i + 1)];
+    y2 = c[2 * (i + 1) + 1];
+  } else {
+    x2 = c[0];
+    y2 = c[1];
+  }
+  if ((cX1 > x1 && cX1 > x2) || (cX2 < x1 && cX2 < x2)) return false;
+  if ((cY1 > y1 && cY1 > y2) || (cY2 < y1 && cY2 < y2)) return false;
+  return true;
+}
+
+__device__ __forceinline__ long long list_base(const long long *off, int k, bool inclusive) {
+  return inclusive ? (k > 0 ? off[k - 1] : 0) : off[k];
+}
+
+// count mode (bEdge == nullptr): counts and clears joinFlag when a side has
+// no edge; apply mode: writes the ordered edge lists at the pair offsets
+// (exclusive offsets, or the reference's inclusive prefix when inclusive).
+template <typename EdgeT>
+__global__ void __launch_bounds__(256)
+    k_cmf(long nList, const int *list, const int *pairs, const double *cmbr,
+          const double *bCoords, const int *bVNum, const long *bVPSNum, const double *oCoords,
+          const int *oVNum, const long *oVPSNum, int *joinFlag, int *bCount, int *oCount,
+          const long long *bOff, const long long *oOff, EdgeT *bEdge, EdgeT *oEdge,
+          bool inclusive) {
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nList) return;
+  const int lane = __lane_id();
+  const int k = list[w];
+  const int b = pairs[2 * k], o = pairs[2 * k + 1];
+  const double cX1 = cmbr[4 * k], cY1 = cmbr[4 * k + 1], cX2 = cmbr[4 * k + 2],
+               cY2 = cmbr[4 * k + 3];
+  int cnt[2];
+  for (int side = 0; side < 2; side++) {
+    const double *c = side == 0 ? bCoords + 2 * (b > 0 ? bVPSNum[b - 1] : 0)
+                                : oCoords + 2 * (o > 0 ? oVPSNum[o - 1] : 0);
+    const int vNum = side == 0 ? bVNum[b] : oVNum[o];
+    EdgeT *outE = side == 0 ? bEdge : oEdge;
+    long long pos = bEdge ? list_base(side == 0 ? bOff : oOff, k, inclusive) : 0;
+    int n = 0;
+    for (int i0 = 0; i0 < vNum; i0 += 64) {
+      int i = i0 + lane;
+      bool ok = i < vNum && cmf_edge(c, vNum, i, cX1, cY1, cX2, cY2);
+      unsigned long long m = __ballot(ok);
+      if (outE && ok) outE[pos + n + __popcll(m & ((1ULL << lane) - 1ULL))] = (EdgeT)i;
+      n += __popcll(m);
+    }
+    cnt[side] = n;
+  }
+  if (!bEdge && lane == 0) {
+    if (cnt[0] == 0 || cnt[1] == 0) {
+      joinFlag[k] = 0;
+      bCount[k] = 0;
+      oCount[k] = 0;
+    } else {
+      bCount[k] = cnt[0];
+      oCount[k] = cnt[1];
+    }
+  }
+}
+
+// ------------------------------------------------------------ stage 5 ----
+__device__ __forceinline__ bool seg_hit(double bX1, double bY1, double bX2, double bY2,
+                                        double oX1, double oY1, double oX2, double oY2) {
+  if (((bX2 < oX1) && (bX2 < oX2)) && ((bX1 < oX1) && (bX1 < oX2))) return false;
+  if (((bX2 > oX1) && (bX2 > oX2)) && ((bX1 > oX1) && (bX1 > oX2))) return false;
+  if (((bY2 < oY1) && (bY2 < oY2)) && ((bY1 < oY1) && (bY1 < oY2))) return false;
+  if (((bY2 > oY1) && (bY2 > oY2)) && ((bY1 > oY1) && (bY1 > oY2))) return false;
+  double denom = ((bX1 - bX2) * (oY1 - oY2)) - ((bY1 - bY2) * (oX1 - oX2));
+  if (denom == 0) {
+    if ((oX1 == oX2 && oX1 == bX1) || (bX1 == bX2 && bX1 == oX1)) return true;
+    double oM = (oY2 - oY1) / (oX2 - oX1);
+    double bM = (bY2 - bY1) / (bX2 - bX1);
+    double oB = oY1 - oM * oX1;
+    double bB = bY1 - bM * bX1;
+    return oB == bB;
+  }
+  double nomX = (((bX1 * bY2) - (bY1 * bX2)) * (oX1 - oX2)) -
+                ((bX1 - bX2) * ((oX1 * oY2) - (oY1 * oX2)));
+  double nomY = ((((bX1 * bY2) - (bY1 * bX2)) * (oY1 - oY2)) -
+                 (((bY1 - bY2) * ((oX1 * oY2) - (oY1 * oX2)))));
+  double x0 = nomX / denom, y0 = nomY / denom;
+  if ((x0 > bX1 && x0 > bX2) || (x0 < bX1 && x0 < bX2)) return false;
+  if ((x0 > oX1 && x0 > oX2) || (x0 < oX1 && x0 < oX2)) return false;
+  if ((y0 > bY1 && y0 > bY2) || (y0 < bY1 && y0 < bY2)) return false;
+  if ((y0 > oY1 && y0 > oY2) || (y0 < oY1 && y0 < oY2)) return false;
+  return true;
+}
+
+// counts come from bCount/oCount, or (when null) from the offsets; the flag
+// goes to segFlag[k] or, with flagByList, to segFlag[w] (the reference's
+// per-survivor layout).
+template <typename EdgeT>
+__global__ void __launch_bounds__(256)
+    k_segjoin(long nList, const int *list, const int *pairs, const double *bCoords,
+              const long *bVPSNum, const double *oCoords, const long *oVPSNum, const int *bCount,
+              const int *oCount, const long long *bOff, const long long *oOff,
+              const EdgeT *bEdge, const EdgeT *oEdge, int *segFlag, bool inclusive,
+              bool flagByList) {
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nList) return;
+  const int lane = __lane_id();
+  const int k = list[w];
+  const int b = pairs[2 * k], o = pairs[2 * k + 1];
+  const long bBase = b > 0 ? bVPSNum[b - 1] : 0, oBase = o > 0 ? oVPSNum[o - 1] : 0;
+  const int bV = (int)(bVPSNum[b] - bBase), oV = (int)(oVPSNum[o] - oBase);
+  const long long b0 = list_base(bOff, k, inclusive), o0 = list_base(oOff, k, inclusive);
+  const int bE = bCount ? bCount[k] : (int)(bOff[k] - b0);
+  const int oE = oCount ? oCount[k] : (int)(oOff[k] - o0);
+  const EdgeT *be = bEdge + b0, *oe = oEdge + o0;
+  const double *bc = bCoords + 2 * bBase, *oc = oCoords + 2 * oBase;
+  const long long total = (long long)bE * oE;
+  int flag = 0;
+  for (long long t0 = 0; t0 < total && !flag; t0 += 64) {
+    long long t = t0 + lane;
+    bool hit = false;
+    if (t < total) {
+      int bi = be[t / oE], oi = oe[t % oE];
+      double bX1 = bc[2 * bi], bY1 = bc[2 * bi + 1], bX2, bY2;
+      if (bi < bV - 1) {
+        bX2 = bc[2 * (bi + 1)];
+        bY2 = bc[2 * (bi + 1) + 1];
+      } else {
+        bX2 = bc[0];
+        bY2 = bc[1];
+      }
+      if (!(bX1 == bX2 && bY1 == bY2)) {
+        double oX1 = oc[2 * oi], oY1 = oc[2 * oi + 1], oX2, oY2;
+        if (oi < oV - 1) {
+          oX2 = oc[2 * (oi + 1)];
+          oY2 = oc[2 * (oi + 1) + 1];
+        } else {
+          oX2 = oc[0];
+          oY2 = oc[1];
+        }
+        hit = seg_hit(bX1, bY1, bX2, bY2, oX1, oY1, oX2, oY2);
+      }
+    }
+    flag = __ballot(hit) != 0ULL;
+  }
+  if (lane == 0) segFlag[flagByList ? w : k] = flag;
+}
+
+__global__ void k_join_final(long K, const int *pipFlag, const int *segFlag, int *joined) {
+  long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < K) joined[k] = (pipFlag[k] == 1 || segFlag[k] == 1) ? 1 : 0;
+}
+
+__global__ void k_gather_pairs(long n, const int *idx, const int *pairs, int *out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int k = idx[i];
+  out[2 * i] = pairs[2 * k];
+  out[2 * i + 1] = pairs[2 * k + 1];
+}
+
+void join_refine(Context &c, const JoinInputs &in, long K, const int *pairs, JoinRefine &r) {
+  r.K = K;
+  long Kc = K > 0 ? K : 1;
+  r.cmbr = (double *)c.ws.get("jr_cmbr", 32 * (size_t)Kc);
+  r.pipFlag = (int *)c.ws.get("jr_pipFlag", 4 * (size_t)Kc);
+  r.pipType = (signed char *)c.ws.get("jr_pipType", (size_t)Kc);
+  r.joinFlag = (int *)c.ws.get("jr_joinFlag", 4 * (size_t)Kc);
+  r.pipCand = (int *)c.ws.get("jr_pipCand", 4 * (size_t)Kc);
+  r.bCount = (int *)c.ws.get("jr_bCount", 4 * (size_t)Kc);
+  r.oCount = (int *)c.ws.get("jr_oCount", 4 * (size_t)Kc);
+  r.bOff = (long long *)c.ws.get("jr_bOff", 8 * (size_t)(Kc + 1));
+  r.oOff = (long long *)c.ws.get("jr_oOff", 8 * (size_t)(Kc + 1));
+  r.segFlag = (int *)c.ws.get("jr_segFlag", 4 * (size_t)Kc);
+  r.list = (int *)c.ws.get("jr_list", 4 * (size_t)Kc);
+  r.joined = (int *)c.ws.get("jr_joined", 4 * (size_t)Kc);
+  if (K <= 0) {
+    r.pipNum = r.within = r.cmfCand = r.E = r.hits = r.nJoined = 0;
+    r.joinedPairs = (int *)c.ws.get("jr_joinedPairs", 8);
+    return;
+  }
+  {
+    LaunchTimer t(c, "get_cmbr");
+    hipLaunchKernelGGL(k_get_cmbr, dim3(grid_for(K, 256)), dim3(256), 0, c.stream, K, pairs,
+                       in.bMBR2, in.oMBR2, r.cmbr, r.pipFlag, r.pipType, r.joinFlag);
+    HIP_CHECK(hipGetLastError());
+  }
+  r.pipNum = compact_flags(c, r.pipFlag, K, r.pipCand, "pip");
+  if (r.pipNum > 0) {
+    LaunchTimer t(c, "pip");
+    hipLaunchKernelGGL(k_pip, dim3(grid_for(r.pipNum, 4)), dim3(256), 0, c.stream, r.pipNum,
+                       r.pipCand, pairs, r.pipType, in.bCoords, in.bVPSNum, in.oCoords,
+                       in.oVPSNum, r.pipFlag, r.joinFlag);
+    HIP_CHECK(hipGetLastError());
+  }
+  // within count = number of pipFlag==1 after the test
+  fill_i32(c, r.bCount, 0, K);
+  fill_i32(c, r.oCount, 0, K);
+  fill_i32(c, r.segFlag, 0, K);
+  r.within = compact_flags(c, r.pipFlag, K, r.list, "within");
+  r.cmfCand = compact_flags(c, r.joinFlag, K, r.list, "cmf");
+  if (r.cmfCand > 0) {
+    LaunchTimer t(c, "cmf_count");
+    hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(r.cmfCand, 4)), dim3(256), 0, c.stream,
+                       r.cmfCand, r.list, pairs, r.cmbr, in.bCoords, in.bVNum, in.bVPSNum,
+                       in.oCoords, in.oVNum, in.oVPSNum, r.joinFlag, r.bCount, r.oCount,
+                       (const long long *)nullptr, (const long long *)nullptr, (int *)nullptr,
+                       (int *)nullptr, false);
+    HIP_CHECK(hipGetLastError());
+  }
+  scan_i32_to_i64(c, r.bCount, r.bOff, K);
+  scan_i32_to_i64(c, r.oCount, r.oOff, K);
+  r.nbEdges = (long)read_scalar(c, r.bOff + K);
+  r.noEdges = (long)read_scalar(c, r.oOff + K);
+  r.bEdge = (int *)c.ws.get("jr_bEdge", 4 * (size_t)(r.nbEdges + 1));
+  r.oEdge = (int *)c.ws.get("jr_oEdge", 4 * (size_t)(r.noEdges + 1));
+  r.E = compact_flags(c, r.joinFlag, K, r.list, "cmf2");
+  if (r.E > 0) {
+    {
+      LaunchTimer t(c, "cmf_apply");
+      hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(r.E, 4)), dim3(256), 0, c.stream, r.E,
+                         r.list, pairs, r.cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords,
+                         in.oVNum, in.oVPSNum, r.joinFlag, r.bCount, r.oCount,
+                         (const long long *)r.bOff, (const long long *)r.oOff, r.bEdge, r.oEdge,
+                         false);
+      HIP_CHECK(hipGetLastError());
+    }
+    {
+      LaunchTimer t(c, "segment_join");
+      hipLaunchKernelGGL(k_segjoin<int>, dim3(grid_for(r.E, 4)), dim3(256), 0, c.stream, r.E,
+                         r.list, pairs, in.bCoords, in.bVPSNum, in.oCoords, in.oVPSNum, r.bCount,
+                         r.oCount, (const long long *)r.bOff, (const long long *)r.oOff, r.bEdge,
+                         r.oEdge, r.segFlag, false, false);
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+  int *tmp = (int *)c.ws.get("jr_hits_idx", 4 * (size_t)Kc);
+  r.hits = compact_flags(c, r.segFlag, K, tmp, "hits");
+  hipLaunchKernelGGL(k_join_final, dim3(grid_for(K, 256)), dim3(256), 0, c.stream, K, r.pipFlag,
+                     r.segFlag, r.joined);
+  HIP_CHECK(hipGetLastError());
+  int *jidx = (int *)c.ws.get("jr_joined_idx", 4 * (size_t)Kc);
+  r.nJoined = compact_flags(c, r.joined, K, jidx, "joined");
+  r.joinedIdx = jidx;
+  r.joinedPairs = (int *)c.ws.get("jr_joinedPairs", 8 * (size_t)(r.nJoined > 0 ? r.nJoined : 1));
+  if (r.nJoined > 0) {
+    hipLaunchKernelGGL(k_gather_pairs, dim3(grid_for(r.nJoined, 256)), dim3(256), 0, c.stream,
+                       r.nJoined, jidx, pairs, r.joinedPairs);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+// ------------------------------------------- single-stage launchers ----
+void launch_get_cmbr(Context &c, long K, const int *pairs, const double *bMBR2,
+                     const double *oMBR2, double *cmbr, int *pipFlag, signed char *pipType,
+                     int *joinFlag) {
+  if (K <= 0) return;
+  LaunchTimer t(c, "get_cmbr");
+  hipLaunchKernelGGL(k_get_cmbr, dim3(grid_for(K, 256)), dim3(256), 0, c.stream, K, pairs, bMBR2,
+                     oMBR2, cmbr, pipFlag, pipType, joinFlag);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_pip(Context &c, long nCand, const int *cand, const int *pairs,
+                const signed char *pipType, const double *bCoords, const long *bVPSNum,
+                const double *oCoords, const long *oVPSNum, int *pipFlag, int *joinFlag) {
+  if (nCand <= 0) return;
+  LaunchTimer t(c, "pip");
+  hipLaunchKernelGGL(k_pip, dim3(grid_for(nCand, 4)), dim3(256), 0, c.stream, nCand, cand, pairs,
+                     pipType, bCoords, bVPSNum, oCoords, oVPSNum, pipFlag, joinFlag);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_cmf_count(Context &c, long nList, const int *list, const int *pairs,
+                      const double *cmbr, const JoinInputs &in, int *joinFlag, int *bCount,
+                      int *oCount) {
+  if (nList <= 0) return;
+  LaunchTimer t(c, "cmf_count");
+  hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(nList, 4)), dim3(256), 0, c.stream, nList, list,
+                     pairs, cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum,
+                     in.oVPSNum, joinFlag, bCount, oCount, (const long long *)nullptr,
+                     (const long long *)nullptr, (int *)nullptr, (int *)nullptr, false);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_cmf_apply_short(Context &c, long nList, const int *list, const int *pairs,
+                            const double *cmbr, const JoinInputs &in, const long long *bIncl,
+                            const long long *oIncl, short *bEdge, short *oEdge) {
+  if (nList <= 0) return;
+  LaunchTimer t(c, "cmf_apply");
+  hipLaunchKernelGGL(k_cmf<short>, dim3(grid_for(nList, 4)), dim3(256), 0, c.stream, nList, list,
+                     pairs, cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum,
+                     in.oVPSNum, (int *)nullptr, (int *)nullptr, (int *)nullptr, bIncl, oIncl,
+                     bEdge, oEdge, true);
+  HIP_CHECK(hipGetLastError());
+}
+
+void launch_segjoin_short(Context &c, long nList, const int *list, const int *pairs,
+                          const JoinInputs &in, const long long *bIncl, const long long *oIncl,
+                          const short *bEdge, const short *oEdge, int *flagByList) {
+  if (nList <= 0) return;
+  LaunchTimer t(c, "segment_join");
+  hipLaunchKernelGGL(k_segjoin<short>, dim3(grid_for(nList, 4)), dim3(256), 0, c.stream, nList,
+                     list, pairs, in.bCoords, in.bVPSNum, in.oCoords, in.oVPSNum,
+                     (const int *)nullptr, (const int *)nullptr, bIncl, oIncl, bEdge, oEdge,
+                     flagByList, true, true);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace gcmf

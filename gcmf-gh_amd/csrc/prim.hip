@@ -1,0 +1,395 @@
+// prim.hip -- device primitives for gfx950: exclusive scan, stream
+// compaction and a stable LSD radix sort.  These replace the reference's
+// multi-level Hillis-Steele scans (GPU_PrefixSum.h, GPU_Utility.h:62-219),
+// Make_Filtered_Index_List (GPU_Vector.h:108-120) and the one-hot decimal
+// radix sort (GPU_RadixSort.h, GPU_Utility.h:224-289).
+//
+// Scan: reduce-then-scan over 4096-element tiles (256 threads x 16 items),
+// wave64 shuffles inside a wave, LDS across the 4 waves of a block.
+// Radix sort: 8-bit digits, LDS digit histograms per 4096-key tile, a
+// digit-major scan of the histograms, and a stable scatter ranked with
+// 8 wave ballots per 64 keys (peer mask = keys with the same digit).
+#include "gcmf_device.hpp"
+#include "gcmf_internal.hpp"
+
+namespace gcmf {
+
+// ------------------------------------------------------------ context ----
+void *Workspace::get(const std::string &name, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  Slot &s = slots[name];
+  if (s.bytes < bytes) {
+    if (s.ptr) HIP_CHECK(hipFree(s.ptr));
+    s.ptr = nullptr;
+    size_t want = bytes + bytes / 4;  // headroom against small growth
+    HIP_CHECK(hipMalloc(&s.ptr, want));
+    s.bytes = want;
+  }
+  return s.ptr;
+}
+
+void Workspace::release_all() {
+  for (auto &kv : slots)
+    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  slots.clear();
+}
+
+hipEvent_t Context::take_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+void Context::flush_timers() {
+  if (pending.empty()) return;
+  HIP_CHECK(hipStreamSynchronize(stream));
+  for (auto &p : pending) {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    KernelTimer &t = timers[p.first];
+    t.total_ms += ms;
+    t.launches += 1;
+    event_pool.push_back(p.second.first);
+    event_pool.push_back(p.second.second);
+  }
+  pending.clear();
+}
+
+LaunchTimer::LaunchTimer(Context &ctx, const char *n) : c(ctx), name(n) {
+  if (c.timing) {
+    a = c.take_event();
+    b = c.take_event();
+    HIP_CHECK(hipEventRecord(a, c.stream));
+  }
+}
+
+LaunchTimer::~LaunchTimer() {
+  if (c.timing && a) {
+    if (hipEventRecord(b, c.stream) == hipSuccess) c.pending.push_back({name, {a, b}});
+  }
+}
+
+// --------------------------------------------------------------- fill ----
+__global__ void k_fill_i32(int *p, int v, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void fill_i32(Context &c, int *p, int v, long n) {
+  if (n <= 0) return;
+  LaunchTimer t(c, "fill_i32");
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, p, v, n);
+  HIP_CHECK(hipGetLastError());
+}
+
+// --------------------------------------------------------------- scan ----
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    T u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T *lds_waves, T &total) {
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  T inc = wave_incl_scan(v);
+  if (lane == 63) lds_waves[wave] = inc;
+  __syncthreads();
+  T wbase = 0, tot = 0;
+  const int nw = blockDim.x >> 6;
+  for (int w = 0; w < nw; w++) {
+    T s = lds_waves[w];
+    if (w < wave) wbase += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return wbase + inc - v;
+}
+
+template <typename TI, typename TA>
+__global__ void __launch_bounds__(kScanThreads) k_tile_reduce(const TI *in, long n, TA *partials) {
+  __shared__ TA lds[kScanThreads / 64];
+  const long base = (long)blockIdx.x * kScanTile;
+  TA s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    long i = base + (long)k * kScanThreads + threadIdx.x;
+    if (i < n) s += (TA)in[i];
+  }
+  TA tot;
+  block_excl_scan<TA>(s, lds, tot);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// Scans one tile; adds offsets[blockIdx.x] (if given).  When total_out is
+// given the block writes its grand total there (single-tile scans).
+template <typename TI, typename TA, typename TO>
+__global__ void __launch_bounds__(kScanThreads) k_tile_scan(const TI *in, long n, TO *out,
+                                                            const TA *offsets, TO *total_out) {
+  __shared__ TA tile[kScanTile];
+  __shared__ TA lds[kScanThreads / 64];
+  const long base = (long)blockIdx.x * kScanTile;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    int j = k * kScanThreads + threadIdx.x;
+    long i = base + j;
+    tile[j] = i < n ? (TA)in[i] : (TA)0;
+  }
+  __syncthreads();
+  TA local[kScanItems];
+  TA s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    local[k] = tile[threadIdx.x * kScanItems + k];
+    s += local[k];
+  }
+  TA tot;
+  TA run = block_excl_scan<TA>(s, lds, tot);
+  TA off = offsets ? offsets[blockIdx.x] : (TA)0;
+  run += off;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    tile[threadIdx.x * kScanItems + k] = run;
+    run += local[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    int j = k * kScanThreads + threadIdx.x;
+    long i = base + j;
+    if (i < n) out[i] = (TO)tile[j];
+  }
+  if (total_out && threadIdx.x == 0) *total_out = (TO)(off + tot);
+}
+
+template <typename TO>
+__global__ void k_copy_total(TO *dst, const long long *src) {
+  if (threadIdx.x == 0) *dst = (TO)*src;
+}
+
+template <typename TI, typename TO>
+static void scan_impl(Context &c, const TI *in, TO *out, long n, int depth) {
+  // out has n+1 entries; out[n] = total.
+  if (n <= kScanTile) {
+    LaunchTimer t(c, "scan_tile");
+    hipLaunchKernelGGL((k_tile_scan<TI, long long, TO>), dim3(1), dim3(kScanThreads), 0,
+                       c.stream, in, n, out, (const long long *)nullptr, out + n);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  long tiles = (n + kScanTile - 1) / kScanTile;
+  std::string tag = "scan_partials_" + std::to_string(depth);
+  long long *partials = (long long *)c.ws.get(tag, sizeof(long long) * (size_t)(tiles + 1));
+  {
+    LaunchTimer t(c, "scan_reduce");
+    hipLaunchKernelGGL((k_tile_reduce<TI, long long>), dim3((unsigned)tiles), dim3(kScanThreads),
+                       0, c.stream, in, n, partials);
+    HIP_CHECK(hipGetLastError());
+  }
+  scan_impl<long long, long long>(c, partials, partials, tiles, depth + 1);
+  {
+    LaunchTimer t(c, "scan_downsweep");
+    hipLaunchKernelGGL((k_tile_scan<TI, long long, TO>), dim3((unsigned)tiles),
+                       dim3(kScanThreads), 0, c.stream, in, n, out, (const long long *)partials,
+                       (TO *)nullptr);
+    HIP_CHECK(hipGetLastError());
+  }
+  // out[n] = partials[tiles] (the grand total)
+  hipLaunchKernelGGL((k_copy_total<TO>), dim3(1), dim3(64), 0, c.stream, out + n,
+                     (const long long *)(partials + tiles));
+  HIP_CHECK(hipGetLastError());
+}
+
+// The in-place recursive call (partials -> partials) is safe because every
+// tile is read into LDS before any element of it is written.
+
+__global__ void k_zero_total(void *p, int bytes) {
+  if (threadIdx.x == 0) {
+    if (bytes == 8) *(long long *)p = 0;
+    else *(int *)p = 0;
+  }
+}
+
+void scan_i32_to_i64(Context &c, const int *in, long long *out, long n) {
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_zero_total, dim3(1), dim3(64), 0, c.stream, (void *)out, 8);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  scan_impl<int, long long>(c, in, out, n, 0);
+}
+
+void scan_i32_to_i32(Context &c, const int *in, int *out, long n) {
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_zero_total, dim3(1), dim3(64), 0, c.stream, (void *)out, 4);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  scan_impl<int, int>(c, in, out, n, 0);
+}
+
+// --------------------------------------------------------- compaction ----
+__global__ void k_flags01(const int *flag, int *f01, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f01[i] = flag[i] != 0;
+}
+
+__global__ void k_compact(const int *flag, const long long *pos, long n, int *out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i] != 0) out[pos[i]] = (int)i;
+}
+
+long compact_flags(Context &c, const int *flag, long n, int *out_idx, const char *tag) {
+  if (n <= 0) return 0;
+  std::string t = std::string("compact_") + tag;
+  int *f01 = (int *)c.ws.get(t + "_f", sizeof(int) * (size_t)n);
+  long long *pos = (long long *)c.ws.get(t + "_p", sizeof(long long) * (size_t)(n + 1));
+  hipLaunchKernelGGL(k_flags01, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, flag, f01, n);
+  HIP_CHECK(hipGetLastError());
+  scan_i32_to_i64(c, f01, pos, n);
+  hipLaunchKernelGGL(k_compact, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, flag, pos, n,
+                     out_idx);
+  HIP_CHECK(hipGetLastError());
+  return (long)read_scalar(c, pos + n);
+}
+
+// --------------------------------------------------------- radix sort ----
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;  // per lane
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kWaveKeys = 64 * kSortItems;
+
+// Per-tile digit histogram, written digit-major: hist[d * tiles + tile].
+__global__ void __launch_bounds__(kSortThreads)
+    k_sort_hist(const uint64_t *keys, long n, int shift, int *hist, long tiles) {
+  __shared__ int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const long base = (long)blockIdx.x * kSortTile;
+#pragma unroll 4
+  for (int k = 0; k < kSortItems; k++) {
+    long i = base + (long)k * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & 255)], 1);
+  }
+  __syncthreads();
+  hist[(long)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: wave w ranks its 1024 keys in 16 rounds of 64 in order;
+// position = tile digit offset + earlier waves' counts + rank within wave.
+__global__ void __launch_bounds__(kSortThreads)
+    k_sort_scatter(const uint64_t *keys, const int *vals, long n, int shift,
+                   const long long *off, long tiles, uint64_t *keys_out, int *vals_out) {
+  __shared__ int wcnt[kSortThreads / 64][256];
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  for (int d = threadIdx.x; d < 256 * (kSortThreads / 64); d += kSortThreads)
+    (&wcnt[0][0])[d] = 0;
+  __syncthreads();
+  const long wbase = (long)blockIdx.x * kSortTile + (long)wave * kWaveKeys;
+  uint64_t kk[kSortItems];
+  int vv[kSortItems];
+  int pos[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; r++) {
+    long i = wbase + (long)r * 64 + lane;
+    bool valid = i < n;
+    uint64_t key = valid ? keys[i] : 0;
+    int d = (int)((key >> shift) & 255);
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      unsigned long long m = __ballot((d >> b) & 1);
+      peers &= ((d >> b) & 1) ? m : ~m;
+    }
+    int before = valid ? wcnt[wave][d] : 0;
+    int rk = __popcll(peers & ((1ULL << lane) - 1ULL));
+    kk[r] = key;
+    vv[r] = valid ? vals[i] : 0;
+    pos[r] = valid ? before + rk : -1;
+    // the lowest lane of each peer group advances the wave's digit counter
+    bool leader = valid && rk == 0;
+    // every lane has read wcnt for this round before any leader updates it
+    __builtin_amdgcn_wave_barrier();
+    if (leader) wcnt[wave][d] = before + __popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // exclusive over waves, per digit
+  {
+    int d = threadIdx.x;  // 256 threads == 256 digits
+    int s = 0;
+    for (int w = 0; w < kSortThreads / 64; w++) {
+      int v = wcnt[w][d];
+      wcnt[w][d] = s;
+      s += v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; r++) {
+    if (pos[r] < 0) continue;
+    int d = (int)((kk[r] >> shift) & 255);
+    long long dst = off[(long)d * tiles + blockIdx.x] + wcnt[wave][d] + pos[r];
+    keys_out[dst] = kk[r];
+    vals_out[dst] = vv[r];
+  }
+}
+
+void radix_sort_u64(Context &c, const uint64_t *keys_in, const int *vals_in, uint64_t *keys_out,
+                    int *vals_out, long n, int bits) {
+  if (n <= 0) return;
+  int passes = (bits + 7) / 8;
+  if (passes == 0) {
+    HIP_CHECK(hipMemcpyAsync(keys_out, keys_in, sizeof(uint64_t) * (size_t)n,
+                             hipMemcpyDeviceToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(vals_out, vals_in, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice,
+                             c.stream));
+    return;
+  }
+  long tiles = (n + kSortTile - 1) / kSortTile;
+  uint64_t *kb = (uint64_t *)c.ws.get("sort_kb", sizeof(uint64_t) * (size_t)n);
+  int *vb = (int *)c.ws.get("sort_vb", sizeof(int) * (size_t)n);
+  int *hist = (int *)c.ws.get("sort_hist", sizeof(int) * (size_t)(256 * tiles));
+  long long *off = (long long *)c.ws.get("sort_off", sizeof(long long) * (size_t)(256 * tiles + 1));
+  // ping-pong so that the last pass lands in *_out
+  const uint64_t *src_k = keys_in;
+  const int *src_v = vals_in;
+  for (int p = 0; p < passes; p++) {
+    bool to_out = ((passes - 1 - p) % 2) == 0;
+    uint64_t *dk = to_out ? keys_out : kb;
+    int *dv = to_out ? vals_out : vb;
+    {
+      LaunchTimer t(c, "sort_hist");
+      hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)tiles), dim3(kSortThreads), 0, c.stream,
+                         src_k, n, 8 * p, hist, tiles);
+      HIP_CHECK(hipGetLastError());
+    }
+    scan_i32_to_i64(c, hist, off, 256 * tiles);
+    {
+      LaunchTimer t(c, "sort_scatter");
+      hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)tiles), dim3(kSortThreads), 0, c.stream,
+                         src_k, src_v, n, 8 * p, off, tiles, dk, dv);
+      HIP_CHECK(hipGetLastError());
+    }
+    src_k = dk;
+    src_v = dv;
+  }
+}
+
+}  // namespace gcmf

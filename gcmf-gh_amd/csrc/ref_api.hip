@@ -1,0 +1,425 @@
+// ref_api.hip -- the reference's own call surface (Join.h, ghcuda.h,
+// ghcuda.cpp, ST_Intersect.cu) as extern "C" entry points over the gfx950
+// kernels, so a caller of the reference links against libgcmf_hip.so
+// unchanged.  Semantics follow the cited reference functions; device outputs
+// are hipMalloc'ed and owned by the caller exactly where the reference
+// cudaMalloc's them.  Failures print to stderr and return -1 / leave the
+// outputs NULL instead of calling exit().
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/gcmf_hip.h"
+#include "gcmf_internal.hpp"
+#include "gcmf_join.hpp"
+
+namespace {
+
+std::mutex g_mu;
+gcmf::Context *g_ctx = nullptr;
+
+gcmf::Context &dctx() {
+  if (!g_ctx) {
+    g_ctx = new gcmf::Context();
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    g_ctx->device = dev;
+    HIP_CHECK(hipStreamCreate(&g_ctx->stream));
+  }
+  HIP_CHECK(hipSetDevice(g_ctx->device));
+  return *g_ctx;
+}
+
+template <typename F>
+bool shim(const char *name, F &&f) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  try {
+    f(dctx());
+    return true;
+  } catch (const gcmf::Error &e) {
+    std::fprintf(stderr, "%s: %s\n", name, e.what.c_str());
+    return false;
+  }
+}
+
+template <typename T>
+T *dalloc(long count) {
+  void *p = nullptr;
+  HIP_CHECK(hipMalloc(&p, sizeof(T) * (size_t)(count > 0 ? count : 1)));
+  return (T *)p;
+}
+
+__global__ void k_swap_xy(long n, const uint64_t *in, uint64_t *out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[4 * i] = in[4 * i + 1];
+  out[4 * i + 1] = in[4 * i];
+  out[4 * i + 2] = in[4 * i + 3];
+  out[4 * i + 3] = in[4 * i + 2];
+}
+
+__global__ void k_incl_from_excl(long n, const long long *ex, long long *in) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) in[i] = ex[i + 1];
+}
+
+__global__ void k_mul(long n, const int *a, const int *b, int *c) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = a[i] * b[i];
+}
+
+__global__ void k_eq1(long n, const int *f, int *o) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = f[i] == 1;
+}
+
+// state of the last ghcuda / GH_CUDA call (results stay on the device)
+gcmf::ClipResult g_last_clip;
+
+void write_results(gcmf::Context &c, const gcmf::ClipResult &r) {
+  const char *path = std::getenv("GCMF_RESULT_FILE");
+  if (!path || r.totalVerts <= 0) return;
+  std::vector<double> xy(2 * (size_t)r.totalVerts);
+  std::vector<int> cs((size_t)r.totalComps);
+  HIP_CHECK(hipMemcpyAsync(xy.data(), r.xy, 16 * (size_t)r.totalVerts, hipMemcpyDeviceToHost,
+                           c.stream));
+  HIP_CHECK(hipMemcpyAsync(cs.data(), r.compSizes, 4 * (size_t)r.totalComps,
+                           hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  FILE *f = std::fopen(path, "w");
+  if (!f) return;
+  // savePolygon format (polygon.h:575-584) at full precision
+  long v = 0;
+  for (int comp : cs) {
+    for (int i = 0; i < comp; i++, v++)
+      std::fprintf(f, "%.17g %.17g%s\n", xy[2 * v], xy[2 * v + 1], i + 1 < comp ? "," : ";");
+  }
+  std::fclose(f);
+}
+
+int normalize_default() {
+  const char *e = std::getenv("GCMF_NORMALIZE");
+  return (e && e[0] == '0') ? 0 : 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+long SortBaseMBROverlap(long bPolNum, long oPolNum, mbr_t *dbMBR, mbr_t *doMBR, int **djxyCounter,
+                        int **djxyVector, char dimSort, char dimSelect) {
+  long K = -1;
+  *djxyCounter = nullptr;
+  *djxyVector = nullptr;
+  shim("SortBaseMBROverlap", [&](gcmf::Context &c) {
+    const uint64_t *b = (const uint64_t *)dbMBR, *o = (const uint64_t *)doMBR;
+    if (dimSort == 1 && dimSelect == 1) {  // InitializeData with x/y exchanged (Join.h:34)
+      uint64_t *sb = (uint64_t *)c.ws.get("ref_swap_b", 32 * (size_t)(bPolNum + 1));
+      uint64_t *so = (uint64_t *)c.ws.get("ref_swap_o", 32 * (size_t)(oPolNum + 1));
+      if (bPolNum > 0)
+        hipLaunchKernelGGL(k_swap_xy, dim3(gcmf::grid_for(bPolNum, 256)), dim3(256), 0, c.stream,
+                           bPolNum, (const uint64_t *)dbMBR, sb);
+      if (oPolNum > 0)
+        hipLaunchKernelGGL(k_swap_xy, dim3(gcmf::grid_for(oPolNum, 256)), dim3(256), 0, c.stream,
+                           oPolNum, (const uint64_t *)doMBR, so);
+      b = sb;
+      o = so;
+    }
+    int *counts = nullptr, *pairs = nullptr;
+    long k = gcmf::mbr_filter(c, bPolNum, oPolNum, b, o, &counts, &pairs, nullptr);
+    int *dc = dalloc<int>(bPolNum + oPolNum), *dv = dalloc<int>(2 * k);
+    HIP_CHECK(hipMemcpyAsync(dc, counts, 4 * (size_t)(bPolNum + oPolNum), hipMemcpyDeviceToDevice,
+                             c.stream));
+    if (k > 0)
+      HIP_CHECK(hipMemcpyAsync(dv, pairs, 8 * (size_t)k, hipMemcpyDeviceToDevice, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    *djxyCounter = dc;
+    *djxyVector = dv;
+    K = k;
+  });
+  return K;
+}
+
+void GetCMBR(long pairNum, int *jxyVector, coord_t *bMBR, coord_t *oMBR, coord_t **cMBR,
+             int **jpipIndexList, int **pipFlag, char **pipType, int **joinFlag, long *pipNum) {
+  shim("GetCMBR", [&](gcmf::Context &c) {
+    *cMBR = dalloc<double>(4 * pairNum);
+    *pipFlag = dalloc<int>(pairNum);
+    *pipType = dalloc<char>(pairNum);
+    *joinFlag = dalloc<int>(pairNum);
+    gcmf::launch_get_cmbr(c, pairNum, jxyVector, bMBR, oMBR, *cMBR, *pipFlag,
+                          (signed char *)*pipType, *joinFlag);
+    int *tmp = (int *)c.ws.get("ref_pip_idx", 4 * (size_t)(pairNum + 1));
+    long n = gcmf::compact_flags(c, *pipFlag, pairNum, tmp, "ref_pip");
+    *jpipIndexList = dalloc<int>(n);
+    if (n > 0)
+      HIP_CHECK(hipMemcpyAsync(*jpipIndexList, tmp, 4 * (size_t)n, hipMemcpyDeviceToDevice,
+                               c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    *pipNum = n;
+  });
+}
+
+long PointInPolygonTest(coord_t *bCoords, coord_t *oCoords, long pairNum, long pipNum,
+                        int *jxyVector, int *jPiPIndexList, char *pipType, long *bVPSNum,
+                        long *oVPSNum, int *pipFlag, int *joinFlag) {
+  long within = -1;
+  shim("PointInPolygonTest", [&](gcmf::Context &c) {
+    gcmf::launch_pip(c, pipNum, jPiPIndexList, jxyVector, (const signed char *)pipType, bCoords,
+                     bVPSNum, oCoords, oVPSNum, pipFlag, joinFlag);
+    int *tmp = (int *)c.ws.get("ref_within_idx", 4 * (size_t)(pairNum + 1));
+    within = gcmf::compact_flags(c, pipFlag, pairNum, tmp, "ref_within");
+  });
+  return within;
+}
+
+void CountCMF(coord_t *bCoords, coord_t *oCoords, long pairNum, int *jxyVector, int *joinFlag,
+              int *bVNum, int *oVNum, long *bVPSNum, long *oVPSNum, coord_t *cMBR,
+              long **bEdgePSCounter, long **oEdgePSCounter, long **workLoadPSCounter,
+              int **jxy2IndexList, poly_size_t **bEdgeList, poly_size_t **oEdgeList, long *eiNum,
+              long *workLoadNum) {
+  shim("CountCMF", [&](gcmf::Context &c) {
+    gcmf::JoinInputs in;
+    in.bCoords = bCoords;
+    in.oCoords = oCoords;
+    in.bVNum = bVNum;
+    in.oVNum = oVNum;
+    in.bVPSNum = bVPSNum;
+    in.oVPSNum = oVPSNum;
+    const long n = pairNum;
+    int *list = (int *)c.ws.get("ref_cmf_list", 4 * (size_t)(n + 1));
+    int *bc = (int *)c.ws.get("ref_cmf_bc", 4 * (size_t)(n + 1));
+    int *oc = (int *)c.ws.get("ref_cmf_oc", 4 * (size_t)(n + 1));
+    int *wl = (int *)c.ws.get("ref_cmf_wl", 4 * (size_t)(n + 1));
+    int *f01 = (int *)c.ws.get("ref_cmf_f01", 4 * (size_t)(n + 1));
+    long long *ex = (long long *)c.ws.get("ref_cmf_ex", 8 * (size_t)(n + 1));
+    hipLaunchKernelGGL(k_eq1, dim3(gcmf::grid_for(n, 256)), dim3(256), 0, c.stream, n, joinFlag,
+                       f01);
+    long n0 = gcmf::compact_flags(c, f01, n, list, "ref_cmf0");
+    gcmf::fill_i32(c, bc, 0, n);
+    gcmf::fill_i32(c, oc, 0, n);
+    gcmf::launch_cmf_count(c, n0, list, jxyVector, cMBR, in, joinFlag, bc, oc);
+    long long *bps = dalloc<long long>(n), *ops = dalloc<long long>(n), *wps = dalloc<long long>(n);
+    gcmf::scan_i32_to_i64(c, bc, ex, n);
+    hipLaunchKernelGGL(k_incl_from_excl, dim3(gcmf::grid_for(n, 256)), dim3(256), 0, c.stream, n,
+                       ex, bps);
+    long nb = (long)gcmf::read_scalar(c, ex + n);
+    gcmf::scan_i32_to_i64(c, oc, ex, n);
+    hipLaunchKernelGGL(k_incl_from_excl, dim3(gcmf::grid_for(n, 256)), dim3(256), 0, c.stream, n,
+                       ex, ops);
+    long no = (long)gcmf::read_scalar(c, ex + n);
+    hipLaunchKernelGGL(k_mul, dim3(gcmf::grid_for(n, 256)), dim3(256), 0, c.stream, n, bc, oc,
+                       wl);
+    gcmf::scan_i32_to_i64(c, wl, ex, n);
+    hipLaunchKernelGGL(k_incl_from_excl, dim3(gcmf::grid_for(n, 256)), dim3(256), 0, c.stream, n,
+                       ex, wps);
+    *workLoadNum = (long)gcmf::read_scalar(c, ex + n);
+    hipLaunchKernelGGL(k_eq1, dim3(gcmf::grid_for(n, 256)), dim3(256), 0, c.stream, n, joinFlag,
+                       f01);
+    long E = gcmf::compact_flags(c, f01, n, list, "ref_cmf1");
+    *jxy2IndexList = dalloc<int>(E);
+    if (E > 0)
+      HIP_CHECK(hipMemcpyAsync(*jxy2IndexList, list, 4 * (size_t)E, hipMemcpyDeviceToDevice,
+                               c.stream));
+    *bEdgeList = dalloc<short>(nb);
+    *oEdgeList = dalloc<short>(no);
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    *bEdgePSCounter = (long *)bps;
+    *oEdgePSCounter = (long *)ops;
+    *workLoadPSCounter = (long *)wps;
+    *eiNum = E;
+  });
+}
+
+void ApplyCMF(coord_t *bCoords, coord_t *oCoords, long pairNum, int *jxyVector, long eiNum,
+              int *jxy2IndexList, int *bVNum, int *oVNum, long *bPFVNum, long *oPFVNum,
+              coord_t *cMBR, long *bEdgePSCounter, long *oEdgePSCounter, poly_size_t *bEdgeList,
+              poly_size_t *oEdgeList) {
+  (void)pairNum;
+  shim("ApplyCMF", [&](gcmf::Context &c) {
+    gcmf::JoinInputs in;
+    in.bCoords = bCoords;
+    in.oCoords = oCoords;
+    in.bVNum = bVNum;
+    in.oVNum = oVNum;
+    in.bVPSNum = bPFVNum;
+    in.oVPSNum = oPFVNum;
+    gcmf::launch_cmf_apply_short(c, eiNum, jxy2IndexList, jxyVector, cMBR, in,
+                                 (const long long *)bEdgePSCounter,
+                                 (const long long *)oEdgePSCounter, bEdgeList, oEdgeList);
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+  });
+}
+
+long SegmentIntersectJoin(coord_t *bCoords, coord_t *oCoords, long pairNum, int *jCompactVector,
+                          int *jxy2IndexList, long *bPFVNum, long *oPFVNum,
+                          long *bEdgePSCounter, long *oEdgePSCounter, poly_size_t *bEdgeList,
+                          poly_size_t *oEdgeList, int **segIntersectJoinFlag) {
+  long hits = -1;
+  *segIntersectJoinFlag = nullptr;
+  shim("SegmentIntersectJoin", [&](gcmf::Context &c) {
+    gcmf::JoinInputs in;
+    in.bCoords = bCoords;
+    in.oCoords = oCoords;
+    in.bVPSNum = bPFVNum;
+    in.oVPSNum = oPFVNum;
+    int *flag = dalloc<int>(pairNum);
+    gcmf::launch_segjoin_short(c, pairNum, jxy2IndexList, jCompactVector, in,
+                               (const long long *)bEdgePSCounter,
+                               (const long long *)oEdgePSCounter, bEdgeList, oEdgeList, flag);
+    int *tmp = (int *)c.ws.get("ref_seg_idx", 4 * (size_t)(pairNum + 1));
+    hits = gcmf::compact_flags(c, flag, pairNum, tmp, "ref_seg");
+    *segIntersectJoinFlag = flag;
+  });
+  return hits;
+}
+
+void calculateIntersections(coord_t *baseCoord, coord_t *overlayCoords, int sizeP, int sizeQ,
+                            coord_t *cmbr, int *dbVNum, long *dbVPSNum, int *doVNum,
+                            long *doVPSNum, int bPID, int oPID, int *countNonDegenIntP,
+                            int *countNonDegenIntQ, coord_t **intersectionsP,
+                            coord_t **intersectionsQ, int **alphaValuesP, int **alphaValuesQ,
+                            int **initLabelsP, int **initLabelsQ, int **neighborP,
+                            int **neighborQ, int *invalid) {
+  (void)sizeP;
+  (void)sizeQ;
+  (void)cmbr;  // recomputed from the rings exactly as getCMBR does
+  *invalid = 1;
+  shim("calculateIntersections", [&](gcmf::Context &c) {
+    gcmf::JoinInputs in;
+    in.bCoords = baseCoord;
+    in.oCoords = overlayCoords;
+    in.bVNum = dbVNum;
+    in.oVNum = doVNum;
+    in.bVPSNum = dbVPSNum;
+    in.oVPSNum = doVPSNum;
+    int *dp = (int *)c.ws.get("ref_ci_pair", 8);
+    int hp[2] = {bPID, oPID};
+    HIP_CHECK(hipMemcpyAsync(dp, hp, 8, hipMemcpyHostToDevice, c.stream));
+    gcmf::ClipOptions opt;
+    opt.normalize = 0;
+    opt.noswap = 1;
+    gcmf::ClipResult r;
+    gcmf::Phase1View p;
+    gcmf::clip_pairs(c, in, dp, 1, opt, r, &p);
+    int st;
+    HIP_CHECK(hipMemcpy(&st, p.status, 4, hipMemcpyDeviceToHost));
+    if (st != gcmf::GH_OK && st < 0) st = gcmf::GH_OK;  // phase 1 itself succeeded
+    if (st != gcmf::GH_OK) return;
+    const long cP = p.slotsP, cQ = p.slotsQ;
+    *countNonDegenIntP = (int)cP;
+    *countNonDegenIntQ = (int)cQ;
+    *intersectionsP = (double *)std::malloc(16 * (size_t)(cP ? cP : 1));
+    *intersectionsQ = (double *)std::malloc(16 * (size_t)(cQ ? cQ : 1));
+    *alphaValuesP = (int *)std::malloc(4 * (size_t)(cP ? cP : 1));
+    *alphaValuesQ = (int *)std::malloc(4 * (size_t)(cQ ? cQ : 1));
+    *initLabelsP = (int *)std::malloc(4 * (size_t)(cP ? cP : 1));
+    *initLabelsQ = (int *)std::malloc(4 * (size_t)(cQ ? cQ : 1));
+    *neighborP = (int *)std::malloc(4 * (size_t)(cP ? cP : 1));
+    *neighborQ = (int *)std::malloc(4 * (size_t)(cQ ? cQ : 1));
+    HIP_CHECK(hipMemcpy(*intersectionsP, p.iP, 16 * (size_t)cP, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(*intersectionsQ, p.iQ, 16 * (size_t)cQ, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(*alphaValuesP, p.aP, 4 * (size_t)cP, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(*alphaValuesQ, p.aQ, 4 * (size_t)cQ, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(*initLabelsP, p.lP, 4 * (size_t)cP, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(*initLabelsQ, p.lQ, 4 * (size_t)cQ, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(*neighborP, p.nbP, 4 * (size_t)cP, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(*neighborQ, p.nbQ, 4 * (size_t)cQ, hipMemcpyDeviceToHost));
+    *invalid = 0;
+  });
+}
+
+int GH_CUDA(coord_t *bCoords, coord_t *oCoords, int *pBVNum, long *pBVPSNum, int *pOVNum,
+            long *pOVPSNum, int bPID, int oPID, int swapped) {
+  (void)swapped;  // the swap is decided from the ring sizes (ghcuda.cpp:64-91)
+  int invalid = 1;
+  shim("GH_CUDA", [&](gcmf::Context &c) {
+    gcmf::JoinInputs in;
+    in.bCoords = bCoords;
+    in.oCoords = oCoords;
+    in.bVNum = pBVNum;
+    in.oVNum = pOVNum;
+    in.bVPSNum = pBVPSNum;
+    in.oVPSNum = pOVPSNum;
+    int *dp = (int *)c.ws.get("ref_gh_pair", 8);
+    int hp[2] = {bPID, oPID};
+    HIP_CHECK(hipMemcpyAsync(dp, hp, 8, hipMemcpyHostToDevice, c.stream));
+    gcmf::ClipOptions opt;
+    opt.normalize = normalize_default();
+    gcmf::clip_pairs(c, in, dp, 1, opt, g_last_clip, nullptr);
+    int st;
+    HIP_CHECK(hipMemcpy(&st, g_last_clip.status, 4, hipMemcpyDeviceToHost));
+    invalid = st == gcmf::GH_INVALID ? 1 : 0;
+    write_results(c, g_last_clip);
+  });
+  return invalid;
+}
+
+int ghcuda(int pIDList[], int qIDList[], int totalNumPairs, coord_t *bCoords, coord_t *oCoords,
+           int *pBVNum, long *pBVPSNum, int *pOVNum, long *pOVPSNum) {
+  shim("ghcuda", [&](gcmf::Context &c) {
+    gcmf::JoinInputs in;
+    in.bCoords = bCoords;
+    in.oCoords = oCoords;
+    in.bVNum = pBVNum;
+    in.oVNum = pOVNum;
+    in.bVPSNum = pBVPSNum;
+    in.oVPSNum = pOVPSNum;
+    const long M = totalNumPairs > 0 ? totalNumPairs : 0;
+    std::vector<int> hp(2 * (size_t)(M ? M : 1));
+    for (long k = 0; k < M; k++) {
+      hp[2 * k] = pIDList[k];
+      hp[2 * k + 1] = qIDList[k];
+    }
+    int *dp = (int *)c.ws.get("ref_ghcuda_pairs", 8 * (size_t)(M ? M : 1));
+    HIP_CHECK(hipMemcpyAsync(dp, hp.data(), 8 * (size_t)M, hipMemcpyHostToDevice, c.stream));
+    gcmf::ClipOptions opt;
+    opt.normalize = normalize_default();
+    gcmf::clip_pairs(c, in, dp, M, opt, g_last_clip, nullptr);
+    write_results(c, g_last_clip);
+  });
+  return 0;
+}
+
+int ST_Intersect(long bPolNum, long oPolNum, coord_t *baseCoords, coord_t *overlayCoords,
+                 int *bVNum, int *oVNum, long *bVPSNum, long *oVPSNum, mbr_t *seqMBR,
+                 mbr_t *seqOMBR, coord_t *seqMBR2, coord_t *seqOMBR2, int **jPairs) {
+  int ret = -1;
+  *jPairs = nullptr;
+  shim("ST_Intersect", [&](gcmf::Context &c) {
+    const long nb = bPolNum > 0 ? bVPSNum[bPolNum - 1] : 0;
+    const long no = oPolNum > 0 ? oVPSNum[oPolNum - 1] : 0;
+    auto up = [&](const char *tag, const void *src, size_t bytes) {
+      void *d = c.ws.get(std::string("ref_st_") + tag, bytes);
+      if (bytes) HIP_CHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, c.stream));
+      return d;
+    };
+    gcmf::JoinInputs in;
+    in.B = bPolNum;
+    in.O = oPolNum;
+    in.bCoords = (const double *)up("bc", baseCoords, 16 * (size_t)nb);
+    in.oCoords = (const double *)up("oc", overlayCoords, 16 * (size_t)no);
+    in.bVNum = (const int *)up("bv", bVNum, 4 * (size_t)bPolNum);
+    in.oVNum = (const int *)up("ov", oVNum, 4 * (size_t)oPolNum);
+    in.bVPSNum = (const long *)up("bp", bVPSNum, 8 * (size_t)bPolNum);
+    in.oVPSNum = (const long *)up("op", oVPSNum, 8 * (size_t)oPolNum);
+    in.bMBR = (const uint64_t *)up("bm", seqMBR, 32 * (size_t)bPolNum);
+    in.oMBR = (const uint64_t *)up("om", seqOMBR, 32 * (size_t)oPolNum);
+    in.bMBR2 = (const double *)up("bm2", seqMBR2, 32 * (size_t)bPolNum);
+    in.oMBR2 = (const double *)up("om2", seqOMBR2, 32 * (size_t)oPolNum);
+    int *pairs = nullptr;
+    long K = gcmf::mbr_filter(c, bPolNum, oPolNum, in.bMBR, in.oMBR, nullptr, &pairs, nullptr);
+    gcmf::JoinRefine r;
+    gcmf::join_refine(c, in, K, pairs, r);
+    int *h = (int *)std::malloc(8 * (size_t)(r.nJoined ? r.nJoined : 1));
+    if (r.nJoined > 0)
+      HIP_CHECK(hipMemcpy(h, r.joinedPairs, 8 * (size_t)r.nJoined, hipMemcpyDeviceToHost));
+    *jPairs = h;
+    ret = (int)r.nJoined;
+  });
+  return ret;
+}
+
+}  // extern "C"

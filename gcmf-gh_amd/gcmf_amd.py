@@ -163,3 +163,223 @@ def gen_single_pair(kind=0, seed=0, nP=2000, nQ=1500, snap=0.0):
     p, q = C.c_void_p(), C.c_void_p()
     host_lib().gcmf_gen_single_pair(kind, seed, nP, nQ, snap, C.byref(p), C.byref(q))
     return Layer._from_handle(p).ring(0), Layer._from_handle(q).ring(0)
+
+
+# ---------------------------------------------------------------- GPU ----
+class JoinStats(C.Structure):
+    _fields_ = [(n, C.c_long) for n in ("candidates", "pip_candidates", "within",
+                                        "cmf_candidates", "survivors", "b_edges", "o_edges",
+                                        "intersecting", "joined")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class ClipStats(C.Structure):
+    _fields_ = [(n, C.c_long) for n in ("pairs", "ok", "invalid", "disjoint", "failed",
+                                        "total_comps", "total_verts", "edges_p", "edges_q",
+                                        "slots_p", "slots_q")] + \
+               [("n_eval", C.c_longlong), ("n_intersections", C.c_longlong)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class DLayer(C.Structure):
+    _fields_ = [("npoly", C.c_long), ("d_coords", C.c_void_p), ("d_vnum", C.c_void_p),
+                ("d_vpsnum", C.c_void_p), ("d_mbr", C.c_void_p), ("d_mbr2", C.c_void_p)]
+
+
+# field ids of gcmf_join_get / gcmf_clip_get (include/gcmf_hip.h)
+J_FIELDS = {"pairs": (0, np.int32), "counts": (1, np.int32), "perm": (2, np.int32),
+            "rank": (3, np.int32), "cmbr": (4, np.float64), "pip_type": (5, np.int8),
+            "pip_flag": (6, np.int32), "join_flag": (7, np.int32), "b_count": (8, np.int32),
+            "o_count": (9, np.int32), "b_edges": (10, np.int32), "o_edges": (11, np.int32),
+            "seg_flag": (12, np.int32), "joined": (13, np.int32)}
+C_FIELDS = {"status": (0, np.int32), "swapped": (1, np.int32), "ncomp": (2, np.int32),
+            "nverts": (3, np.int32), "comp_off": (4, np.int64), "vert_off": (5, np.int64),
+            "comp_sizes": (6, np.int32), "xy": (7, np.float64), "offP": (20, np.int64),
+            "offQ": (21, np.int64), "psP1": (22, np.int64), "psP2": (23, np.int64),
+            "psQ1": (24, np.int64), "psQ2": (25, np.int64), "boolP": (26, np.int32),
+            "boolQ": (27, np.int32), "nmapQ": (28, np.int32), "iP": (29, np.float64),
+            "iQ": (30, np.float64), "oP": (31, np.float64), "oQ": (32, np.float64),
+            "aP": (33, np.int32), "aQ": (34, np.int32), "lP": (35, np.int32),
+            "lQ": (36, np.int32), "nbP": (37, np.int32), "nbQ": (38, np.int32),
+            "norm": (39, np.float64)}
+
+CLIP_NORMALIZE = 1
+CLIP_NO_CLEANUP = 2
+
+
+def hip_lib():
+    """The HIP library.  Raises when it is missing: there is no CPU path."""
+    global _hip
+    if _hip is None:
+        path = os.path.join(HERE, "libgcmf_hip.so")
+        if not os.path.exists(path):
+            raise GcmfError("libgcmf_hip.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.gcmf_device_count.restype = C.c_int
+        L.gcmf_init.argtypes = [C.c_int, C.POINTER(vp)]
+        L.gcmf_shutdown.argtypes = [vp]
+        L.gcmf_last_error.argtypes = [vp]
+        L.gcmf_last_error.restype = C.c_char_p
+        L.gcmf_malloc.argtypes = [vp, C.POINTER(vp), C.c_size_t]
+        L.gcmf_free.argtypes = [vp, vp]
+        L.gcmf_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
+        L.gcmf_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
+        L.gcmf_synchronize.argtypes = [vp]
+        L.gcmf_set_timing.argtypes = [vp, C.c_int]
+        L.gcmf_timer_reset.argtypes = [vp]
+        L.gcmf_timer_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]
+        L.gcmf_join.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), C.POINTER(JoinStats)]
+        L.gcmf_join_get.argtypes = [vp, C.c_int, vp, C.c_long]
+        L.gcmf_clip.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), vp, C.c_long, C.c_int,
+                                C.POINTER(ClipStats)]
+        L.gcmf_join_clip.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), C.c_int,
+                                     C.POINTER(JoinStats), C.POINTER(ClipStats)]
+        L.gcmf_clip_get.argtypes = [vp, C.c_int, vp, C.c_long]
+        _hip = L
+    return _hip
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.ptr = C.c_void_p()
+        ctx._check(hip_lib().gcmf_malloc(ctx.h, C.byref(self.ptr), max(int(nbytes), 1)), "malloc")
+        self.nbytes = nbytes
+
+    def free(self):
+        if self.ptr:
+            hip_lib().gcmf_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+
+class DeviceLayer:
+    """A layer resident in HBM (the reference's device layout)."""
+
+    def __init__(self, ctx, layer):
+        self.layer = layer
+        self.bufs = [ctx.upload(a) for a in (layer.coords, layer.vnum, layer.vpsnum, layer.mbr,
+                                             layer.mbr2)]
+        self.desc = DLayer(layer.npoly, *[b.ptr for b in self.bufs])
+
+    def free(self):
+        for b in self.bufs:
+            b.free()
+
+
+class Context:
+    """A GPU context of libgcmf_hip.so: one HIP stream and its workspace."""
+
+    def __init__(self, device=0):
+        L = hip_lib()
+        if L.gcmf_device_count() <= 0:
+            raise GcmfError("no HIP device visible")
+        h = C.c_void_p()
+        rc = L.gcmf_init(device, C.byref(h))
+        if rc != 0:
+            raise GcmfError("gcmf_init failed (%d)" % rc)
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = hip_lib().gcmf_last_error(self.h)
+            raise GcmfError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def close(self):
+        if self.h:
+            hip_lib().gcmf_shutdown(self.h)
+            self.h = None
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        buf = DeviceBuffer(self, arr.nbytes)
+        if arr.nbytes:
+            self._check(hip_lib().gcmf_memcpy_h2d(self.h, buf.ptr, arr.ctypes.data, arr.nbytes),
+                        "h2d")
+        return buf
+
+    def download(self, buf, dtype, count):
+        out = np.zeros(count, dtype=dtype)
+        if out.nbytes:
+            self._check(hip_lib().gcmf_memcpy_d2h(self.h, out.ctypes.data, buf.ptr, out.nbytes),
+                        "d2h")
+        return out
+
+    def synchronize(self):
+        self._check(hip_lib().gcmf_synchronize(self.h), "synchronize")
+
+    def set_timing(self, on):
+        self._check(hip_lib().gcmf_set_timing(self.h, int(on)), "set_timing")
+
+    def timer_reset(self):
+        self._check(hip_lib().gcmf_timer_reset(self.h), "timer_reset")
+
+    def timer(self, kernel):
+        ms, n = C.c_double(), C.c_long()
+        self._check(hip_lib().gcmf_timer_get(self.h, kernel.encode(), C.byref(ms), C.byref(n)),
+                    "timer_get")
+        return ms.value, n.value
+
+    # ------------------------------------------------------------- join
+    def join(self, base, over):
+        st = JoinStats()
+        self._check(hip_lib().gcmf_join(self.h, C.byref(base.desc), C.byref(over.desc),
+                                        C.byref(st)), "gcmf_join")
+        return st.as_dict()
+
+    def join_get(self, field, count):
+        fid, dt = J_FIELDS[field]
+        out = np.zeros(int(count), dtype=dt)
+        if count:
+            self._check(hip_lib().gcmf_join_get(self.h, fid, out.ctypes.data, int(count)),
+                        "join_get")
+        return out
+
+    # ------------------------------------------------------------- clip
+    def clip(self, base, over, pairs, flags=CLIP_NORMALIZE):
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+        dp = self.upload(pairs)
+        st = ClipStats()
+        try:
+            self._check(hip_lib().gcmf_clip(self.h, C.byref(base.desc), C.byref(over.desc),
+                                            dp.ptr, len(pairs) // 2, flags, C.byref(st)),
+                        "gcmf_clip")
+        finally:
+            dp.free()
+        return st.as_dict()
+
+    def join_clip(self, base, over, flags=CLIP_NORMALIZE):
+        js, cs = JoinStats(), ClipStats()
+        self._check(hip_lib().gcmf_join_clip(self.h, C.byref(base.desc), C.byref(over.desc),
+                                             flags, C.byref(js), C.byref(cs)), "gcmf_join_clip")
+        return js.as_dict(), cs.as_dict()
+
+    def clip_get(self, field, count):
+        fid, dt = C_FIELDS[field]
+        out = np.zeros(int(count), dtype=dt)
+        if count:
+            self._check(hip_lib().gcmf_clip_get(self.h, fid, out.ctypes.data, int(count)),
+                        "clip_get")
+        return out
+
+    def clip_results(self, stats):
+        """Per-pair result components (list of lists of (n,2) arrays) + status."""
+        M = stats["pairs"]
+        status = self.clip_get("status", M)
+        comp_off = self.clip_get("comp_off", M + 1)
+        sizes = self.clip_get("comp_sizes", stats["total_comps"])
+        xy = self.clip_get("xy", 2 * stats["total_verts"]).reshape(-1, 2)
+        out = []
+        v = 0
+        for k in range(M):
+            comps = []
+            for c in range(int(comp_off[k]), int(comp_off[k + 1])):
+                n = int(sizes[c])
+                comps.append(xy[v:v + n])
+                v += n
+            out.append(comps)
+        return status, out

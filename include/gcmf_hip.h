@@ -1,0 +1,228 @@
+/*
+ * gcmf_hip.h -- C ABI of libgcmf_hip.so, the MI355X (gfx950) join-and-clip
+ * path of GCMF-GH: MBR sort-and-sweep filter, CMBR/PiP refine, CMF edge
+ * filter, segment-intersection refine and batched Greiner-Hormann clipping.
+ *
+ * Plain pointers and sizes only.  "d_" pointers are device memory (HBM),
+ * everything else is host memory.  Every entry point returns 0 on success or
+ * a negative status (GCMF_ERR_*); gcmf_last_error() describes the failure.
+ * Calls on one context are serialised on that context's HIP stream.
+ *
+ * Layer layout (the reference's, SEQ_Overlay.h:285-389 / Types.h:1-8):
+ *   coords  f64 [2*sum(VNum)]  AoS (x,y), each ring's closing vertex repeated
+ *   vnum    i32 [npoly]        ring length including the closing vertex
+ *   vpsnum  i64 [npoly]        inclusive prefix sum of vnum
+ *   mbr     u64 [4*npoly]      CoordToMBR keys (minx, miny, maxx, maxy)
+ *   mbr2    f64 [4*npoly]      the same as doubles
+ */
+#ifndef GCMF_HIP_H
+#define GCMF_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GCMF_OK 0
+#define GCMF_ERR_HIP -1
+#define GCMF_ERR_ALLOC -2
+#define GCMF_ERR_ARG -3
+#define GCMF_ERR_NODEV -4
+#define GCMF_ERR_OVERFLOW -5
+
+typedef struct gcmf_ctx gcmf_ctx;
+
+typedef struct {
+  long npoly;
+  const double *d_coords;
+  const int *d_vnum;
+  const long *d_vpsnum;
+  const uint64_t *d_mbr;
+  const double *d_mbr2;
+} gcmf_dlayer;
+
+/* ------------------------------------------------ context and memory ---- */
+/* No reference counterpart: the reference keeps global state and the legacy
+ * default stream (GPU_Manage.h).  A context owns one stream, a grow-only
+ * device workspace and optional per-kernel HIP-event timers. */
+int gcmf_device_count(void);
+int gcmf_init(int device, gcmf_ctx **out);
+void gcmf_shutdown(gcmf_ctx *ctx);
+const char *gcmf_last_error(const gcmf_ctx *ctx);
+int gcmf_malloc(gcmf_ctx *ctx, void **d_ptr, size_t bytes);
+int gcmf_free(gcmf_ctx *ctx, void *d_ptr);
+int gcmf_memcpy_h2d(gcmf_ctx *ctx, void *d_dst, const void *src, size_t bytes);
+int gcmf_memcpy_d2h(gcmf_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+int gcmf_synchronize(gcmf_ctx *ctx);
+/* Kernel timing with HIP events recorded on the context stream. */
+int gcmf_set_timing(gcmf_ctx *ctx, int on);
+int gcmf_timer_reset(gcmf_ctx *ctx);
+int gcmf_timer_get(gcmf_ctx *ctx, const char *kernel, double *total_ms, long *launches);
+
+/* ------------------------------------------------------------ join ----- */
+typedef struct {
+  long candidates;      /* K: MBR-filter pairs (SortBaseMBROverlap return) */
+  long pip_candidates;  /* pipNum (GetCMBR) */
+  long within;          /* pairs inside by PiP (PointInPolygonTest return) */
+  long cmf_candidates;  /* pairs entering CountCMF */
+  long survivors;       /* eiNum: pairs with edges on both sides after CMF */
+  long b_edges, o_edges;/* CMF edge list lengths */
+  long intersecting;    /* SegmentIntersectJoin return */
+  long joined;          /* within + intersecting (the fixed hand-off) */
+} gcmf_join_stats;
+
+/* The whole join chain of spatialJoin (spatialJoin3.cu:40-412) on
+ * device-resident layers: SortBaseMBROverlap (Join.h:5) -> GetCMBR
+ * (Join.h:397) -> PointInPolygonTest (Join.h:437) -> CountCMF / ApplyCMF
+ * (Join.h:157,257) -> SegmentIntersectJoin (Join.h:356).  Results stay on
+ * the device until the next call; read them with gcmf_join_get. */
+int gcmf_join(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+              gcmf_join_stats *stats);
+
+enum {
+  GCMF_J_PAIRS = 0,     /* i32 [2K] (base, overlay), reference order        */
+  GCMF_J_COUNTS = 1,    /* i32 [B+O] per-polygon candidate counts           */
+  GCMF_J_PERM = 2,      /* i32 [2(B+O)] endpoint sort permutation           */
+  GCMF_J_RANK = 3,      /* i32 [2(B+O)] endpoint rank                        */
+  GCMF_J_CMBR = 4,      /* f64 [4K]                                          */
+  GCMF_J_PIPTYPE = 5,   /* i8  [K]                                           */
+  GCMF_J_PIPFLAG = 6,   /* i32 [K]                                           */
+  GCMF_J_JOINFLAG = 7,  /* i32 [K] after CMF                                 */
+  GCMF_J_BCOUNT = 8,    /* i32 [K] CMF base edge counts                      */
+  GCMF_J_OCOUNT = 9,    /* i32 [K]                                           */
+  GCMF_J_BEDGE = 10,    /* i32 [b_edges] CMF base edge lists, pair order     */
+  GCMF_J_OEDGE = 11,    /* i32 [o_edges]                                     */
+  GCMF_J_SEGFLAG = 12,  /* i32 [K] segment-intersection flags                */
+  GCMF_J_JOINED = 13    /* i32 [2*joined] within U intersecting pairs        */
+};
+int gcmf_join_get(gcmf_ctx *ctx, int field, void *dst, long count);
+
+/* ------------------------------------------------------------ clip ----- */
+#define GCMF_CLIP_NORMALIZE 1 /* per-pair CMBR normalisation of predicates */
+#define GCMF_CLIP_NO_CLEANUP 2 /* skip cleanUpResult (polyclip.h:791)      */
+
+typedef struct {
+  long pairs, ok, invalid, disjoint, failed;
+  long total_comps, total_verts;
+  long edges_p, edges_q, slots_p, slots_q;
+  long long n_eval;          /* edge pairs classified by the count kernel */
+  long long n_intersections; /* intersection records over valid pairs */
+} gcmf_clip_stats;
+
+/* Batched Greiner-Hormann intersection of M pairs (d_pairs = 2M i32 of
+ * (base index, overlay index)); replaces the per-pair loop of ghcuda
+ * (ghcuda.cpp:385-425) with one pass over all pairs.  Each pair is swapped so
+ * |P| >= |Q| as loadPolygonDataFromLayer does. */
+int gcmf_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+              const int *d_pairs, long M, int flags, gcmf_clip_stats *stats);
+
+/* Join then clip the joined pairs without leaving the device. */
+int gcmf_join_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+                   int flags, gcmf_join_stats *jstats, gcmf_clip_stats *cstats);
+
+enum {
+  GCMF_C_STATUS = 0,    /* i32 [M] 0 ok, 1 invalid, 2 disjoint, <0 failure */
+  GCMF_C_SWAPPED = 1,   /* i32 [M] */
+  GCMF_C_NCOMP = 2,     /* i32 [M] */
+  GCMF_C_NVERTS = 3,    /* i32 [M] */
+  GCMF_C_COMPOFF = 4,   /* i64 [M+1] first component of each pair */
+  GCMF_C_VERTOFF = 5,   /* i64 [M+1] first vertex of each pair */
+  GCMF_C_COMPSIZES = 6, /* i32 [total_comps] */
+  GCMF_C_XY = 7,        /* f64 [2*total_verts] original coordinates */
+  /* phase-1 arrays (gppolyclip_mDIV512.cu outputs), global over the batch */
+  GCMF_C_OFFP = 20,     /* i64 [M+1] first P edge of each pair */
+  GCMF_C_OFFQ = 21,     /* i64 [M+1] */
+  GCMF_C_PSP1 = 22,     /* i64 [edges_p+1] exclusive scan of count1 (P view) */
+  GCMF_C_PSP2 = 23,     /* i64 [edges_p+1] exclusive scan of count2 */
+  GCMF_C_PSQ1 = 24,     /* i64 [edges_q+1] */
+  GCMF_C_PSQ2 = 25,     /* i64 [edges_q+1] */
+  GCMF_C_BOOLP = 26,    /* i32 [edges_p] CMBR edge flags */
+  GCMF_C_BOOLQ = 27,    /* i32 [edges_q] */
+  GCMF_C_NMAPQ = 28,    /* i32 [slots_q] neighbour map (local P edge) */
+  GCMF_C_IP = 29,       /* f64 [2*slots_p] predicate coordinates */
+  GCMF_C_IQ = 30,       /* f64 [2*slots_q] */
+  GCMF_C_OP = 31,       /* f64 [2*slots_p] original coordinates */
+  GCMF_C_OQ = 32,       /* f64 [2*slots_q] */
+  GCMF_C_AP = 33,       /* i32 [slots_p] trunc(1e7 alpha) or -100 */
+  GCMF_C_AQ = 34,       /* i32 [slots_q] */
+  GCMF_C_LP = 35,       /* i32 [slots_p] initial labels */
+  GCMF_C_LQ = 36,       /* i32 [slots_q] */
+  GCMF_C_NBP = 37,      /* i32 [slots_p] global Q slot + 1, 0 none */
+  GCMF_C_NBQ = 38,      /* i32 [slots_q] global P slot + 1, 0 none */
+  GCMF_C_NORM = 39      /* f64 [3M] cx, cy, scale */
+};
+int gcmf_clip_get(gcmf_ctx *ctx, int field, void *dst, long count);
+
+/* ----------------------------------- reference call surface (drop-in) ---- */
+typedef unsigned long long mbr_t;
+typedef double coord_t;
+typedef short poly_size_t;
+
+/* Join.h:5 -- returns K; *djxyCounter (i32 [B+O]) and *djxyVector (i32 [2K])
+ * are hipMalloc'ed and owned by the caller (hipFree).  dimSort 1 or 2 and
+ * dimSelect 0 sort on x; dimSort 1 with dimSelect 1 sorts on y. */
+long SortBaseMBROverlap(long bPolNum, long oPolNum, mbr_t *dbMBR, mbr_t *doMBR,
+                        int **djxyCounter, int **djxyVector, char dimSort, char dimSelect);
+
+/* Join.h:397 -- device outputs hipMalloc'ed, caller frees. */
+void GetCMBR(long pairNum, int *jxyVector, coord_t *bMBR, coord_t *oMBR, coord_t **cMBR,
+             int **jpipIndexList, int **pipFlag, char **pipType, int **joinFlag, long *pipNum);
+
+/* Join.h:437 -- returns the number of pairs with pipFlag==1. */
+long PointInPolygonTest(coord_t *bCoords, coord_t *oCoords, long pairNum, long pipNum,
+                        int *jxyVector, int *jPiPIndexList, char *pipType, long *bVPSNum,
+                        long *oVPSNum, int *pipFlag, int *joinFlag);
+
+/* Join.h:157 -- CMF counts, prefix sums, workload and survivor list. */
+void CountCMF(coord_t *bCoords, coord_t *oCoords, long pairNum, int *jxyVector, int *joinFlag,
+              int *bVNum, int *oVNum, long *bVPSNum, long *oVPSNum, coord_t *cMBR,
+              long **bEdgePSCounter, long **oEdgePSCounter, long **workLoadPSCounter,
+              int **jxy2IndexList, poly_size_t **bEdgeList, poly_size_t **oEdgeList,
+              long *eiNum, long *workLoadNum);
+
+/* Join.h:257 -- fills the CMF edge lists allocated by CountCMF. */
+void ApplyCMF(coord_t *bCoords, coord_t *oCoords, long pairNum, int *jxyVector, long eiNum,
+              int *jxy2IndexList, int *bVNum, int *oVNum, long *bPFVNum, long *oPFVNum,
+              coord_t *cMBR, long *bEdgePSCounter, long *oEdgePSCounter,
+              poly_size_t *bEdgeList, poly_size_t *oEdgeList);
+
+/* Join.h:356 -- returns #intersecting; *segIntersectJoinFlag (i32 [eiNum])
+ * hipMalloc'ed, caller frees. */
+long SegmentIntersectJoin(coord_t *bCoords, coord_t *oCoords, long pairNum, int *jCompactVector,
+                          int *jxy2IndexList, long *bPFVNum, long *oPFVNum,
+                          long *bEdgePSCounter, long *oEdgePSCounter, poly_size_t *bEdgeList,
+                          poly_size_t *oEdgeList, int **segIntersectJoinFlag);
+
+/* ghcuda.h:6-14 / gppolyclip_mDIV512.cu:1298 -- GH phase 1 for one pair
+ * (P = base ring bPID of baseCoord, Q = overlay ring oPID), outputs malloc'ed
+ * host arrays the caller frees; *invalid = 1 leaves them unallocated.
+ * Predicates use raw coordinates as the reference does. */
+void calculateIntersections(coord_t *baseCoord, coord_t *overlayCoords, int sizeP, int sizeQ,
+                            coord_t *cmbr, int *dbVNum, long *dbVPSNum, int *doVNum,
+                            long *doVPSNum, int bPID, int oPID, int *countNonDegenIntP,
+                            int *countNonDegenIntQ, coord_t **intersectionsP,
+                            coord_t **intersectionsQ, int **alphaValuesP, int **alphaValuesQ,
+                            int **initLabelsP, int **initLabelsQ, int **neighborP,
+                            int **neighborQ, int *invalid);
+
+/* ghcuda.cpp:341 -- one pair; returns invalid (0/1). */
+int GH_CUDA(coord_t *bCoords, coord_t *oCoords, int *pBVNum, long *pBVPSNum, int *pOVNum,
+            long *pOVPSNum, int bPID, int oPID, int swapped);
+
+/* main.h:8-11 / ghcuda.cpp:385 -- clips every listed pair of the resident
+ * layers; returns 0.  Results are kept for gcmf_ghcuda_result. */
+int ghcuda(int pIDList[], int qIDList[], int totalNumPairs, coord_t *bCoords, coord_t *oCoords,
+           int *pBVNum, long *pBVPSNum, int *pOVNum, long *pOVPSNum);
+
+/* ST_Intersect.cu:18 (fixed variant): host layers in; *jPairs (host,
+ * malloc'ed, caller frees) receives the 2*n joined pairs; returns n =
+ * #within + #intersecting. */
+int ST_Intersect(long bPolNum, long oPolNum, coord_t *baseCoords, coord_t *overlayCoords,
+                 int *bVNum, int *oVNum, long *bVPSNum, long *oVPSNum, mbr_t *seqMBR,
+                 mbr_t *seqOMBR, coord_t *seqMBR2, coord_t *seqOMBR2, int **jPairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

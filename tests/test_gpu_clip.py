@@ -1,0 +1,209 @@
+"""GPU parity of the batched Greiner-Hormann clip (gh.hip) against the oracle.
+
+Phase-1 arrays (counts, scans, CMBR flags, neighbour map, intersection
+coordinates, alpha keys, initial labels, neighbour links) must be bit-exact
+against the restatement of gppolyclip_mDIV512.cu; clipped polygons must be
+identical (same components, vertex counts and coordinates) to the CPU route
+of polyclip.h; traced areas must agree with the independent boundary-integral
+area within 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import gcmf_amd as G
+import oracle_py as O
+from conftest import GOLDEN, load_poly
+
+pytestmark = pytest.mark.gpu
+
+AREA_RTOL = 1e-12
+
+
+def fig_cases():
+    out = []
+    for fig, pn in [("Fig1", "Fig1-P"), ("Fig14", "Fig14-P"), ("Fig14cw", "Fig14-P-clockWise"),
+                    ("Fig15", "Fig15-P"), ("Fig17", "Fig17-P-Clockwise")]:
+        qn = fig.replace("cw", "") + "-Q"
+        P = load_poly("%s/fig/%s.poly" % (GOLDEN, pn))[0]
+        Q = load_poly("%s/fig/%s.poly" % (GOLDEN, qn))[0]
+        out.append((fig, P, Q))
+    return out
+
+
+def random_cases(n, seed0=0, nmax=200):
+    out = []
+    rs = np.random.RandomState(seed0)
+    for s in range(n):
+        a, b = int(rs.randint(8, nmax)), int(rs.randint(8, nmax))
+        P, Q = G.gen_single_pair(0, 1000 + seed0 + s, a, b)
+        out.append(("rand%d" % s, P, Q))
+    return out
+
+
+def run_batch(gctx, cases, flags):
+    base = G.Layer.from_rings([c[1] for c in cases])
+    over = G.Layer.from_rings([c[2] for c in cases])
+    db, do = G.DeviceLayer(gctx, base), G.DeviceLayer(gctx, over)
+    M = len(cases)
+    pairs = np.stack([np.arange(M), np.arange(M)], axis=1).astype(np.int32)
+    st = gctx.clip(db, do, pairs, flags)
+    db.free()
+    do.free()
+    return st
+
+
+def oriented(P, Q):
+    """The pair as the clip sees it: P is the ring with more vertices."""
+    return (Q, P) if len(P) < len(Q) else (P, Q)
+
+
+def check_phase1(gctx, st, cases, normalize):
+    M = len(cases)
+    f = {k: gctx.clip_get(k, n) for k, n in [
+        ("status", M), ("offP", M + 1), ("offQ", M + 1), ("psP1", st["edges_p"] + 1),
+        ("psP2", st["edges_p"] + 1), ("psQ1", st["edges_q"] + 1), ("psQ2", st["edges_q"] + 1),
+        ("boolP", st["edges_p"]), ("boolQ", st["edges_q"]), ("nmapQ", st["slots_q"]),
+        ("iP", 2 * st["slots_p"]), ("iQ", 2 * st["slots_q"]), ("oP", 2 * st["slots_p"]),
+        ("oQ", 2 * st["slots_q"]), ("aP", st["slots_p"]), ("aQ", st["slots_q"]),
+        ("lP", st["slots_p"]), ("lQ", st["slots_q"]), ("nbP", st["slots_p"]),
+        ("nbQ", st["slots_q"])]}
+    for k, (name, P0, Q0) in enumerate(cases):
+        P, Q = oriented(P0, Q0)
+        ref = O.phase1(P, Q, normalize)
+        if ref["rc"] == 2:
+            assert f["status"][k] == 2, name
+            continue
+        p0, p1 = int(f["offP"][k]), int(f["offP"][k + 1])
+        q0, q1 = int(f["offQ"][k]), int(f["offQ"][k + 1])
+        assert p1 - p0 == len(P) and q1 - q0 == len(Q), name
+        for key, lo, hi in [("psP1", p0, p1), ("psP2", p0, p1), ("psQ1", q0, q1),
+                            ("psQ2", q0, q1)]:
+            got = f[key][lo:hi + 1] - f[key][lo]
+            np.testing.assert_array_equal(got, ref[key], err_msg="%s %s" % (name, key))
+        np.testing.assert_array_equal(f["boolP"][p0:p1], ref["boolP"], err_msg=name)
+        np.testing.assert_array_equal(f["boolQ"][q0:q1], ref["boolQ"], err_msg=name)
+        if ref["invalid"]:
+            assert f["status"][k] == 1, name
+            continue
+        assert f["status"][k] in (0, -1), name
+        sbP, sbQ = int(f["psP2"][p0]), int(f["psQ2"][q0])
+        cP, cQ = ref["cP"], ref["cQ"]
+        assert int(f["psP2"][p1]) - sbP == cP and int(f["psQ2"][q1]) - sbQ == cQ, name
+        np.testing.assert_array_equal(f["iP"][2 * sbP:2 * (sbP + cP)], ref["intersectionsP"],
+                                      err_msg=name + " iP")
+        np.testing.assert_array_equal(f["iQ"][2 * sbQ:2 * (sbQ + cQ)], ref["intersectionsQ"],
+                                      err_msg=name + " iQ")
+        np.testing.assert_array_equal(f["oP"][2 * sbP:2 * (sbP + cP)], ref["origP"],
+                                      err_msg=name + " oP")
+        np.testing.assert_array_equal(f["oQ"][2 * sbQ:2 * (sbQ + cQ)], ref["origQ"],
+                                      err_msg=name + " oQ")
+        np.testing.assert_array_equal(f["aP"][sbP:sbP + cP], ref["alphaP"], err_msg=name)
+        np.testing.assert_array_equal(f["aQ"][sbQ:sbQ + cQ], ref["alphaQ"], err_msg=name)
+        np.testing.assert_array_equal(f["lP"][sbP:sbP + cP], ref["initLabelsP"], err_msg=name)
+        np.testing.assert_array_equal(f["lQ"][sbQ:sbQ + cQ], ref["initLabelsQ"], err_msg=name)
+        nbP = f["nbP"][sbP:sbP + cP].astype(np.int64)
+        nbQ = f["nbQ"][sbQ:sbQ + cQ].astype(np.int64)
+        np.testing.assert_array_equal(np.where(nbP > 0, nbP - sbQ, 0), ref["neighborP"],
+                                      err_msg=name + " nbP")
+        np.testing.assert_array_equal(np.where(nbQ > 0, nbQ - sbP, 0), ref["neighborQ"],
+                                      err_msg=name + " nbQ")
+        nm = f["nmapQ"][sbQ:sbQ + cQ]
+        np.testing.assert_array_equal(nm, ref["neighborMapQ"], err_msg=name + " nmap")
+
+
+def check_results(gctx, st, cases, mode):
+    status, results = gctx.clip_results(st)
+    for k, (name, P0, Q0) in enumerate(cases):
+        P, Q = oriented(P0, Q0)
+        s_ref, comps, _ = O.clip_cpu(P, Q, mode)
+        if s_ref == 2:
+            assert status[k] == 2, name
+            continue
+        assert status[k] == 0, (name, status[k])
+        got = results[k]
+        assert [len(c) for c in got] == [len(c) for c in comps], name
+        for a, b in zip(got, comps):
+            np.testing.assert_array_equal(a, b, err_msg=name)
+
+
+@pytest.mark.parametrize("normalize", [1, 0])
+def test_phase1_fig_fixtures(gctx, normalize):
+    cases = fig_cases()
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE if normalize else 0)
+    check_phase1(gctx, st, cases, normalize)
+
+
+def test_phase1_random_batch(gctx):
+    cases = random_cases(64)
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    assert st["ok"] == 64
+    check_phase1(gctx, st, cases, 1)
+
+
+def test_phase1_c1_pair(gctx):
+    P, Q = G.gen_single_pair(0, 0, 2000, 1500)
+    cases = [("c1", P, Q)]
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    check_phase1(gctx, st, cases, 1)
+    assert st["n_intersections"] == O.phase1(P, Q, 1)["psP1"][-1]
+
+
+def test_phase1_grid_snapped(gctx):
+    # shared vertices and collinear edges (degenerate intersection types)
+    cases = []
+    for s in range(6):
+        P, Q = G.gen_single_pair(1, 50 + s, 3000, 400, snap=1e-2)
+        cases.append(("snap%d" % s, P, Q))
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    check_phase1(gctx, st, cases, 1)
+
+
+@pytest.mark.parametrize("mode", [1, 3, 0])
+def test_clip_matches_cpu_route(gctx, mode):
+    cases = fig_cases() + random_cases(48, seed0=77)
+    flags = (G.CLIP_NORMALIZE if mode & 1 else 0) | (G.CLIP_NO_CLEANUP if mode & 2 else 0)
+    st = run_batch(gctx, cases, flags)
+    check_results(gctx, st, cases, mode)
+
+
+def test_clip_area_oracle(gctx):
+    cases = random_cases(40, seed0=300, nmax=400)
+    P, Q = G.gen_single_pair(0, 0, 2000, 1500)
+    cases.append(("c1", P, Q))
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_NO_CLEANUP)
+    status, results = gctx.clip_results(st)
+    for k, (name, P0, Q0) in enumerate(cases):
+        A = O.area_boundary(P0, Q0)
+        allxy = np.concatenate([P0, Q0])
+        ox, oy = 0.5 * (allxy[:, 0].min() + allxy[:, 0].max()), 0.5 * (
+            allxy[:, 1].min() + allxy[:, 1].max())
+        got = O.comps_area(results[k], ox, oy)
+        assert abs(got - A) <= AREA_RTOL * max(A, 1e-300), (name, got, A)
+
+
+def test_clip_batch_independence(gctx):
+    """Each pair's result is the same alone and inside a mixed batch."""
+    cases = random_cases(30, seed0=500, nmax=300)
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    _, mixed = gctx.clip_results(st)
+    for k in (0, 7, 29):
+        st1 = run_batch(gctx, [cases[k]], G.CLIP_NORMALIZE)
+        _, alone = gctx.clip_results(st1)
+        assert len(alone[0]) == len(mixed[k])
+        for a, b in zip(alone[0], mixed[k]):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_clip_disjoint_and_nested(gctx):
+    P, Q = G.gen_single_pair(0, 5, 100, 80)
+    far = Q + np.array([10.0, 10.0])
+    inner = 0.2 * P  # inside P (star about the origin scaled down)
+    cases = [("disjoint", P, far), ("nested", P, inner), ("identical", P, P.copy())]
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    status, res = gctx.clip_results(st)
+    assert status[0] == 2 and res[0] == []
+    s_ref, comps, _ = O.clip_cpu(P, inner, 1)
+    assert status[1] == 0 and len(res[1]) == len(comps) == 1
+    np.testing.assert_array_equal(res[1][0], comps[0])
+    s_ref, comps, _ = O.clip_cpu(P, P.copy(), 1)
+    assert [len(c) for c in res[2]] == [len(c) for c in comps]

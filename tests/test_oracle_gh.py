@@ -1,0 +1,216 @@
+"""Pins the CPU oracle of the Greiner-Hormann clip before it is trusted.
+
+Checks, each chosen so that a plausible slip in the restatement fails one:
+  * the reference's committed fixtures (GH-CUDA/data + GH-CUDA/results,
+    copied as data under tests/golden/fig): topology, component and vertex
+    counts, coordinates to the fixtures' printed precision;
+  * the reference's own CPU code (oracle/_ref, compiled from
+    GH-CUDA/lib/polyclip.h behind a driver) -- bit-identical results;
+  * known-answer tables for every intersection type and initial label,
+    built by hand from the definitions (integer coordinates, exact in FP64);
+  * an independent boundary-integral area and the textbook
+    Sutherland-Hodgman clip against convex polygons;
+  * closed-form special cases: P n P = P, disjoint = empty, nested = inner.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import gcmf_amd as G
+import oracle_py as O
+from conftest import GOLDEN, load_poly
+
+FIGS = [("Fig1", "Fig1-P", "Fig1-Q", "Fig1-R"), ("Fig14", "Fig14-P", "Fig14-Q", "Fig14-R"),
+        ("Fig14cw", "Fig14-P-clockWise", "Fig14-Q", "Fig14-R"),
+        ("Fig15", "Fig15-P", "Fig15-Q", "Fig15-R"),
+        ("Fig17", "Fig17-P-Clockwise", "Fig17-Q", "Fig17-R")]
+
+
+def fig(p, q, r):
+    f = lambda n: load_poly(os.path.join(GOLDEN, "fig", n + ".poly"))
+    return f(p)[0], f(q)[0], f(r)
+
+
+def canon(comp, digits=6):
+    """A component as a rotation- and direction-invariant tuple of vertices
+    rounded to the fixtures' 6 significant digits (a clockwise input ring
+    traces its result the other way round)."""
+    r = [tuple(float("%.*g" % (digits, v)) for v in p) for p in comp]
+    best = None
+    for seq in (r, r[::-1]):
+        i = min(range(len(seq)), key=lambda j: seq[j])
+        t = tuple(seq[i:] + seq[:i])
+        best = t if best is None or t < best else best
+    return best
+
+
+@pytest.mark.parametrize("name,p,q,r", FIGS)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fixture_results(name, p, q, r, mode):
+    P, Q, R = fig(p, q, r)
+    for route in (O.clip_cpu, O.clip_gpu_route):
+        st, comps, _ = route(P, Q, mode)
+        assert st == 0
+        assert sorted(len(c) for c in comps) == sorted(len(c) for c in R)
+        assert sorted(canon(c) for c in comps) == sorted(canon(c) for c in R)
+
+
+def test_fixture_fig1_exact_order():
+    P, Q, R = fig("Fig1-P", "Fig1-Q", "Fig1-R")
+    st, comps, _ = O.clip_cpu(P, Q, 1)
+    np.testing.assert_allclose(comps[0], R[0], rtol=2e-6)
+
+
+def rings_for_ref():
+    out = [fig(p, q, r)[:2] for _, p, q, r in FIGS]
+    for s in range(40):
+        rs = np.random.RandomState(s)
+        out.append(G.gen_single_pair(0, 2000 + s, int(rs.randint(8, 300)),
+                                     int(rs.randint(8, 300))))
+    out.append(G.gen_single_pair(0, 0, 2000, 1500))
+    return out
+
+
+def test_against_reference_cpu_code():
+    if O.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference tree absent when built)")
+    for P, Q in rings_for_ref():
+        ref = O.ref_clip(P, Q)
+        st, comps, _ = O.clip_cpu(P, Q, 0)
+        assert st == 0
+        assert len(ref) == len(comps)
+        for a, b in zip(ref, comps):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_routes_agree_random():
+    for s in range(60):
+        rs = np.random.RandomState(s)
+        a, b = int(rs.randint(8, 300)), int(rs.randint(8, 300))
+        P, Q = G.gen_single_pair(0, 3000 + s, max(a, b), min(a, b))
+        for mode in (1, 3):
+            s1, c1, _ = O.clip_cpu(P, Q, mode)
+            s2, c2, _ = O.clip_gpu_route(P, Q, mode)
+            assert s1 == s2 == 0
+            assert len(c1) == len(c2)
+            for x, y in zip(c1, c2):
+                np.testing.assert_array_equal(x, y)
+
+
+# --- known answers for the intersection classification (polyclip.h:48-58)
+TYPE_KAT = [
+    # P1, P2, Q1, Q2, expected type
+    ((0, 0), (4, 0), (2, -2), (2, 2), 1),    # X: proper crossing
+    ((0, 0), (4, 0), (0, -2), (0, 2), 2),    # T on Q: P1 in Q's interior
+    ((0, 0), (4, 0), (2, 0), (2, 2), 3),     # T on P: Q1 in P's interior
+    ((0, 0), (4, 0), (0, 0), (0, 2), 4),     # V: P1 == Q1
+    ((0, 0), (4, 0), (2, 0), (-2, 0), 5),    # X-overlap
+    ((0, 0), (4, 0), (6, 0), (-2, 0), 6),    # T-overlap Q: P1 in Q, Q1 outside P
+    ((0, 0), (4, 0), (2, 0), (6, 0), 7),     # T-overlap P: Q1 in P, P1 outside Q
+    ((0, 0), (4, 0), (0, 0), (-2, 0), 8),    # V-overlap
+    ((0, 0), (4, 0), (0, 1), (4, 1), 0),     # parallel, apart
+    ((0, 0), (4, 0), (5, -1), (5, 1), 0),    # lines cross outside P
+    ((0, 0), (4, 0), (4, -2), (4, 2), 0),    # crossing at P2 belongs to the next edge
+]
+
+
+@pytest.mark.parametrize("P1,P2,Q1,Q2,t", TYPE_KAT)
+def test_intersection_type_kat(P1, P2, Q1, Q2, t):
+    got, a, b = O.intersect_type(P1, P2, Q1, Q2)
+    assert got == t
+    if t == 1:
+        assert a == 0.5 and b == 0.5
+
+
+def test_initial_label_table():
+    # getInitialLabel (gppolyclip_mDIV512.cu:237-252) from the definitions:
+    # 0 LEFT, 1 RIGHT, 2 IS_P_m, 3 IS_P_p
+    want = {(0, 1): 1, (1, 0): 1, (0, 0): 2, (1, 1): 2, (1, 3): 3, (3, 1): 3, (0, 3): 4,
+            (3, 0): 4, (2, 3): 5, (3, 2): 5, (2, 1): 6, (1, 2): 6, (2, 0): 7, (0, 2): 7}
+    for m in range(4):
+        for p in range(4):
+            assert O.lib().orc_initial_label(m, p) == want.get((m, p), -102)
+
+
+# --- area checks
+def sutherland_hodgman(subject, clip):
+    """Textbook clip of any polygon by a convex CCW polygon."""
+    out = [tuple(p) for p in subject]
+    n = len(clip)
+    for i in range(n):
+        a, b = clip[i], clip[(i + 1) % n]
+        inside = lambda p: (b[0] - a[0]) * (p[1] - a[1]) - (b[1] - a[1]) * (p[0] - a[0]) >= 0
+        def inter(p, q):
+            d1 = (b[0] - a[0]) * (p[1] - a[1]) - (b[1] - a[1]) * (p[0] - a[0])
+            d2 = (b[0] - a[0]) * (q[1] - a[1]) - (b[1] - a[1]) * (q[0] - a[0])
+            t = d1 / (d1 - d2)
+            return (p[0] + t * (q[0] - p[0]), p[1] + t * (q[1] - p[1]))
+        inp, out = out, []
+        for j in range(len(inp)):
+            p, q = inp[j - 1], inp[j]
+            if inside(q):
+                if not inside(p):
+                    out.append(inter(p, q))
+                out.append(q)
+            elif inside(p):
+                out.append(inter(p, q))
+        if not out:
+            break
+    return np.array(out)
+
+
+def test_area_against_sutherland_hodgman():
+    rs = np.random.RandomState(7)
+    for s in range(20):
+        P, _ = G.gen_single_pair(0, 400 + s, int(rs.randint(20, 200)), 8)
+        k = int(rs.randint(3, 12))
+        th = np.sort(rs.uniform(0, 2 * np.pi, k))
+        Q = np.stack([0.3 + 0.8 * np.cos(th), 0.2 + 0.8 * np.sin(th)], axis=1)  # convex, CCW
+        sh = sutherland_hodgman(P, Q)
+        a_sh = O.comps_area([sh]) if len(sh) else 0.0
+        st, comps, _ = O.clip_cpu(P, Q, 3)
+        a_gh = O.comps_area(comps)
+        assert abs(a_gh - a_sh) <= 1e-12 * max(a_sh, 1e-300)
+        assert abs(O.area_boundary(P, Q) - a_sh) <= 1e-12 * max(a_sh, 1e-300)
+
+
+def test_traced_area_matches_boundary_integral():
+    for s in range(40):
+        rs = np.random.RandomState(100 + s)
+        P, Q = G.gen_single_pair(0, 5000 + s, int(rs.randint(50, 600)), int(rs.randint(50, 600)))
+        st, comps, _ = O.clip_cpu(P, Q, 3)
+        A = O.area_boundary(P, Q)
+        assert abs(O.comps_area(comps, 0.15, 0.1) - A) <= 1e-12 * A
+
+
+def test_special_cases():
+    P, Q = G.gen_single_pair(0, 9, 120, 90)
+    st, comps, _ = O.clip_cpu(P, P.copy(), 1)
+    assert st == 0 and len(comps) == 1
+    assert abs(O.comps_area(comps) - O.comps_area([P])) <= 1e-14 * O.comps_area([P])
+    st, comps, _ = O.clip_cpu(P, Q + 10.0, 1)
+    assert st == 2 and comps == []
+    inner = 0.2 * P
+    st, comps, _ = O.clip_cpu(P, inner, 1)
+    assert st == 0 and len(comps) == 1
+    np.testing.assert_array_equal(comps[0], inner)
+
+
+def test_phase1_invariants():
+    P, Q = G.gen_single_pair(0, 0, 2000, 1500)
+    t = O.phase1(P, Q, 1)
+    assert t["invalid"] == 0
+    # both views count every intersection once (count1 totals agree)
+    assert t["psP1"][-1] == t["psQ1"][-1] > 0
+    # slots = vertices + vertex-adding intersections on each side
+    assert t["cP"] == len(P) + int(np.sum(t["alphaP"] != -100)) - int(
+        np.sum(t["alphaP"][t["psP2"][:-1]] != -100))
+    # neighbour links are mutual (1-based)
+    nbP, nbQ = t["neighborP"], t["neighborQ"]
+    for i in np.nonzero(t["alphaP"] != -100)[0]:
+        assert nbQ[nbP[i] - 1] == i + 1
+    # alpha keys ascend along each edge
+    for e in range(len(P)):
+        a = t["alphaP"][t["psP2"][e] + 1:t["psP2"][e + 1]]
+        assert np.all(np.diff(a) >= 0)
