@@ -64,6 +64,7 @@ struct ClipOptions {
   int normalize = 1;  // per-pair CMBR normalisation of every predicate
   int cleanup = 1;    // run cleanUpResult
   int noswap = 0;     // keep P = base ring even when it is the smaller one
+  int label_mode = 0; // 0: wave per pair (sequential fallback), 1: thread per pair
 };
 
 struct ClipResult {

@@ -1065,6 +1065,437 @@ __global__ void __launch_bounds__(64)
   a.outNVerts[k] = nv;
 }
 
+// ---------------------------------------- wave-parallel labelling path ----
+// One wave64 per pair.  The lists are built in parallel; labelling step 1/3
+// runs per vertex; when every intersection vertex is a plain CROSSING (no
+// bouncing, chain or overlap labels, mutual links, a non-intersection vertex
+// on both rings) the remaining steps reduce to prefix parities and
+// independent cycle traces:
+//   step 4: enex of the i-th intersection vertex after the start vertex V is
+//           the start status toggled by the parity of the intersections
+//           before it (ballot prefix counts);
+//   createResult: each component is traced by one lane from its smallest
+//           P crossing vertex, which is where the sequential loop over P's
+//           crossing vertices starts it; components are emitted in that
+//           order, vertices in the sequential traversal order.
+// Every other pair (degenerate labels, inconsistent links) runs the
+// sequential code above on lane 0, so results are identical either way.
+
+__device__ __forceinline__ int wave_sum(int v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int lt_count(unsigned long long m) {
+  return __popcll(m & ((1ULL << __lane_id()) - 1ULL));
+}
+
+// parity of the winding number of R over the ring [r0, r0+n) in index order
+__device__ bool wave_pip(const Nodes &N, int r0, int n, pt R) {
+  int w = 0;
+  for (int a = __lane_id(); a < n; a += 64) {
+    int ia = r0 + a, ib = r0 + (a + 1 == n ? 0 : a + 1);
+    pt P0 = P_(N, ia), P1 = P_(N, ib);
+    if ((P0.y < R.y) != (P1.y < R.y)) {
+      if (P0.x >= R.x) {
+        if (P1.x > R.x) w += 2 * (P1.y > P0.y) - 1;
+        else if ((triA(P0, P1, R) > 0) == (P1.y > P0.y)) w += 2 * (P1.y > P0.y) - 1;
+      } else if (P1.x > R.x) {
+        if ((triA(P0, P1, R) > 0) == (P1.y > P0.y)) w += 2 * (P1.y > P0.y) - 1;
+      }
+    }
+  }
+  return (wave_sum(w) & 1) != 0;
+}
+
+// index of the first node of [r0, r0+n) without the intersection flag, or -1
+__device__ int wave_first_nonint(const Nodes &N, int r0, int n) {
+  int best = 0x7fffffff;
+  for (int a = __lane_id(); a < n; a += 64)
+    if (!isint(N, r0 + a)) best = min(best, a);
+  best = wave_min(best);
+  return best == 0x7fffffff ? -1 : r0 + best;
+}
+
+// step 4 for one ring whose intersection vertices are all CROSSING
+__device__ void wave_enex(Nodes &N, int r0, int n, int V, int status0) {
+  int carry = 0;
+  const int v0 = V - r0;
+  for (int base = 1; base < n; base += 64) {
+    int j = base + __lane_id();
+    int u = r0 + (v0 + j) % n;
+    bool bit = j < n && isint(N, u);
+    unsigned long long m = __ballot(bit);
+    if (bit) {
+      int par = (carry + lt_count(m)) & 1;
+      N.enex[u] = (signed char)(par ? (status0 == ENTRY ? EXIT : ENTRY) : status0);
+    }
+    carry += __popcll(m);
+  }
+}
+
+// read-only trace of the component through P crossing vertex S (same moves
+// as createResult); returns its length, or -1 when it does not close
+__device__ int trace_len(const Nodes &N, int S, int cP, int guard, int &minP, int &pcount) {
+  int V = S, len = 0;
+  minP = S;
+  pcount = 1;
+  do {
+    int status = N.enex[V];
+    toggle(status);
+    do {
+      V = status == EXIT ? N.next[V] : N.prev[V];
+      len++;
+      if (V != S && V < cP && isint(N, V)) {
+        minP = min(minP, V);
+        pcount++;
+      }
+      if (len > guard) return -1;
+    } while (!(N.enex[V] == status) && V != S);
+    if (V != S) {
+      V = N.nb[V];
+      if (V < 0) return -1;
+      if (V != S && V < cP && isint(N, V)) {
+        minP = min(minP, V);
+        pcount++;
+      }
+    }
+  } while (V != S);
+  return len;
+}
+
+__device__ void trace_write(const Nodes &N, int S, OutBuf &O, int off) {
+  int V = S, i = 0;
+  do {
+    int status = N.enex[V];
+    toggle(status);
+    do {
+      V = status == EXIT ? N.next[V] : N.prev[V];
+      st(O.pn, off + i, P_(N, V));
+      st(O.po, off + i, ld(N.po, V));
+      i++;
+    } while (!(N.enex[V] == status) && V != S);
+    if (V != S) V = N.nb[V];
+  } while (V != S);
+  for (int j = 0; j < i; j++) {
+    O.next[off + j] = off + (j + 1 == i ? 0 : j + 1);
+    O.prev[off + j] = off + (j == 0 ? i - 1 : j - 1);
+  }
+}
+
+__device__ void copy_ring_to(OutBuf &O, const Nodes &N, int r0, int n, int off) {
+  for (int j = __lane_id(); j < n; j += 64) {
+    st(O.pn, off + j, P_(N, r0 + j));
+    st(O.po, off + j, ld(N.po, r0 + j));
+    O.next[off + j] = off + (j + 1 == n ? 0 : j + 1);
+    O.prev[off + j] = off + (j == 0 ? n - 1 : j - 1);
+  }
+}
+
+__global__ void __launch_bounds__(64)
+    k_label_wave(long M, int *status, SlotArrays s, TraceArgs a, int *aux) {
+  const long k = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (k >= M) return;
+  if (status[k] != GH_OK) {
+    if (lane == 0) {
+      a.outNComp[k] = 0;
+      a.outNVerts[k] = 0;
+    }
+    return;
+  }
+  const long long sbP = a.psP2[a.offP[k]], cP = a.psP2[a.offP[k + 1]] - sbP;
+  const long long sbQ = a.psQ2[a.offQ[k]], cQ = a.psQ2[a.offQ[k + 1]] - sbQ;
+  const long long nb0 = a.nodeOff[k], cap = a.nodeOff[k + 1] - nb0;
+  Nodes N;
+  N.pn = a.npn + 2 * nb0;
+  N.po = a.npo + 2 * nb0;
+  N.next = a.nnext + nb0;
+  N.prev = a.nprev + nb0;
+  N.nb = a.nnb + nb0;
+  N.fl = a.nfl + nb0;
+  N.lab = a.nlab + nb0;
+  N.enex = a.nenex + nb0;
+  N.n = (int)(cP + cQ);
+  N.cap = (int)cap;
+  int *ax = aux + nb0;
+  const long long pE0 = a.offP[k], pE1 = a.offP[k + 1];
+  const long long qE0 = a.offQ[k], qE1 = a.offQ[k + 1];
+  const int nP = (int)cP, nQ = (int)cQ;
+  // -------- build (ghcuda.cpp:239-322 semantics, see k_label_trace)
+  for (int j = lane; j < nP; j += 64) {
+    const long long sl = sbP + j;
+    st(N.pn, j, ld(s.iP, sl));
+    st(N.po, j, ld(s.oP, sl));
+    N.prev[j] = j == 0 ? nP - 1 : j - 1;
+    N.next[j] = j == nP - 1 ? 0 : j + 1;
+    N.nb[j] = -1;
+    N.lab[j] = L_NONE;
+    N.enex[j] = NEITHER;
+    long long lo = pE0, hi = pE1 - 1;  // last edge whose parent slot <= sl
+    while (lo < hi) {
+      long long mid = (lo + hi + 1) >> 1;
+      if (a.psP2[mid] <= sl) lo = mid;
+      else hi = mid - 1;
+    }
+    N.fl[j] = a.psP2[lo] == sl ? (unsigned char)(F_SRC | (s.aP[sl] != kNone ? F_INT : 0))
+                               : (unsigned char)F_INT;
+  }
+  for (int j = lane; j < nQ; j += 64) {
+    const long long sl = sbQ + j;
+    const int v = nP + j;
+    st(N.pn, v, ld(s.iQ, sl));
+    st(N.po, v, ld(s.oQ, sl));
+    N.prev[v] = j == 0 ? nP + nQ - 1 : v - 1;
+    N.next[v] = j == nQ - 1 ? nP : v + 1;
+    N.lab[v] = L_NONE;
+    N.enex[v] = NEITHER;
+    long long lo = qE0, hi = qE1 - 1;
+    while (lo < hi) {
+      long long mid = (lo + hi + 1) >> 1;
+      if (a.psQ2[mid] <= sl) lo = mid;
+      else hi = mid - 1;
+    }
+    const bool parent = a.psQ2[lo] == sl;
+    unsigned char fl = parent ? F_SRC : 0;
+    const bool linked = parent ? (s.aQ[sl] != kNone || s.nbQ[sl] > 0) : true;
+    int nbv = -1;
+    if (linked) {
+      fl |= F_INT;
+      long long kp = (long long)s.nbQ[sl] - 1 - sbP;
+      if (kp >= 0 && kp < cP) nbv = (int)kp;
+    }
+    N.fl[v] = fl;
+    N.nb[v] = nbv;
+  }
+  __syncthreads();
+  // a P node linked from several Q slots keeps the last one (link() order)
+  for (int j = lane; j < nQ; j += 64)
+    if (N.nb[nP + j] >= 0) atomicMax(&N.nb[N.nb[nP + j]], nP + j);
+  __syncthreads();
+  for (int j = lane; j < nP; j += 64)
+    if (N.nb[j] >= 0) N.fl[j] |= F_INT;
+  __syncthreads();
+  // -------- step 1 (restored) + step 3, one lane per P intersection vertex
+  bool slow = false;
+  for (int I = lane; I < nP; I += 64) {
+    bool bad = false;
+    if (isint(N, I)) {
+      const int q = N.nb[I];
+      if (q < 0) {
+        bad = true;
+      } else {
+        const int Pm = I == 0 ? nP - 1 : I - 1, Pp = I == nP - 1 ? 0 : I + 1;
+        const int Qm = N.prev[q], Qp = N.next[q];
+        const int tm = node_rel(N, Qm, Pm, I, Pp), tp = node_rel(N, Qp, Pm, I, Pp);
+        int l = L_NONE;
+        if ((tm == 0 && tp == 1) || (tm == 1 && tp == 0)) l = L_CROSSING;
+        if ((tm == 0 && tp == 0) || (tm == 1 && tp == 1)) l = L_BOUNCING;
+        if ((tp == 3 && tm == 1) || (tm == 3 && tp == 1)) l = L_LEFT_ON;
+        if ((tp == 3 && tm == 0) || (tm == 3 && tp == 0)) l = L_RIGHT_ON;
+        if ((tp == 3 && tm == 2) || (tm == 3 && tp == 2)) l = L_ON_ON;
+        if ((tm == 2 && tp == 1) || (tp == 2 && tm == 1)) l = L_ON_LEFT;
+        if ((tm == 2 && tp == 0) || (tp == 2 && tm == 0)) l = L_ON_RIGHT;
+        N.lab[I] = (signed char)l;
+        N.lab[q] = (signed char)l;
+        bad = l != L_CROSSING;
+      }
+    }
+    slow |= bad;
+  }
+  __syncthreads();
+  for (int j = lane; j < nQ; j += 64) {
+    const int v = nP + j;
+    if (isint(N, v)) {
+      const int q = N.nb[v];
+      if (q < 0 || N.nb[q] != v) slow = true;
+    }
+  }
+  slow = __ballot(slow) != 0ULL;
+  // -------- output regions
+  OutBuf O;
+  O.pn = a.opn + 2 * nb0;
+  O.po = a.opo + 2 * nb0;
+  O.next = a.onext + nb0;
+  O.prev = a.oprev + nb0;
+  O.n = 0;
+  O.cap = (int)cap;
+  O.compRoot = a.ocompRoot + a.compOff[k];
+  O.ncomp = 0;
+  O.capComp = (int)(a.compOff[k + 1] - a.compOff[k]);
+  int nInterP = 0;
+  for (int j = lane; j < nP; j += 64) nInterP += isint(N, j) ? 1 : 0;
+  nInterP = wave_sum(nInterP);
+  int VP = -1, VQ = -1;
+  if (!slow && nInterP > 0) {
+    VP = wave_first_nonint(N, 0, nP);
+    VQ = wave_first_nonint(N, nP, nQ);
+    if (VP < 0 || VQ < 0) slow = true;
+  }
+  if (!slow) {
+    if (nInterP == 0) {
+      // step 3.5: no crossings on either ring -> containment by PiP
+      if (wave_pip(N, nP, nQ, P_(N, 0))) {
+        if (nP > O.cap || O.capComp < 1) {
+          slow = true;
+        } else {
+          copy_ring_to(O, N, 0, nP, 0);
+          O.compRoot[O.ncomp++] = 0;
+          O.n = nP;
+        }
+      }
+      if (!slow && wave_pip(N, 0, nP, P_(N, nP))) {
+        if (O.n + nQ > O.cap || O.ncomp >= O.capComp) {
+          slow = true;
+        } else {
+          copy_ring_to(O, N, nP, nQ, O.n);
+          O.compRoot[O.ncomp++] = O.n;
+          O.n += nQ;
+        }
+      }
+    } else {
+      // step 4
+      wave_enex(N, 0, nP, VP, wave_pip(N, nP, nQ, P_(N, VP)) ? EXIT : ENTRY);
+      wave_enex(N, nP, nQ, VQ, wave_pip(N, 0, nP, P_(N, VQ)) ? EXIT : ENTRY);
+      __syncthreads();
+      // createResult, pass 1: which crossing vertices start a component
+      const int guard = 4 * N.n + 64;
+      int pcnt = 0, fail = 0;
+      for (int S = lane; S < nP; S += 64) {
+        if (!isint(N, S)) continue;
+        int minP, pc;
+        int len = trace_len(N, S, nP, guard, minP, pc);
+        if (len < 0) {
+          fail = 1;
+          ax[S] = -1;
+        } else if (minP == S) {
+          ax[S] = len;
+          pcnt += pc;
+        } else {
+          ax[S] = -1;
+        }
+      }
+      fail = __ballot(fail != 0) != 0ULL;
+      pcnt = wave_sum(pcnt);
+      if (fail || pcnt != nInterP) slow = true;
+      __syncthreads();
+      if (!slow) {
+        // pass 2: offsets by a prefix over starts in P order, then write
+        int vcarry = 0, ccarry = 0;
+        for (int base = 0; base < nP; base += 64) {
+          const int S = base + lane;
+          const int len = (S < nP && isint(N, S)) ? ax[S] : -1;
+          const bool start = len >= 0;
+          unsigned long long m = __ballot(start);
+          int v = start ? len : 0, incl = v;
+          for (int off = 1; off < 64; off <<= 1) {
+            int u = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += u;
+          }
+          const int voff = vcarry + incl - v, cidx = ccarry + lt_count(m);
+          if (start && voff + len <= O.cap && cidx < O.capComp) {
+            trace_write(N, S, O, voff);
+            O.compRoot[cidx] = voff;
+          }
+          vcarry += __shfl(incl, 63, 64);
+          ccarry += __popcll(m);
+        }
+        if (vcarry > O.cap || ccarry > O.capComp) {
+          if (lane == 0) {
+            status[k] = GH_OVERFLOW;
+            a.outNComp[k] = 0;
+            a.outNVerts[k] = 0;
+          }
+          return;
+        }
+        O.n = vcarry;
+        O.ncomp = ccarry;
+      }
+    }
+  }
+  if (slow) {
+    // sequential path for degenerate pairs (labels are recomputed there)
+    if (lane != 0) return;
+    O.n = 0;
+    O.ncomp = 0;
+    int rc = label_and_trace(N, 0, nP, O);
+    if (rc != GH_OK) {
+      status[k] = rc;
+      a.outNComp[k] = 0;
+      a.outNVerts[k] = 0;
+      return;
+    }
+    int nc = 0, nv = 0;
+    double *xy = a.xyTmp + 2 * nb0;
+    int *clen = a.compLen + a.compOff[k];
+    for (int c = 0; c < O.ncomp; c++) {
+      int r = O.compRoot[c];
+      if (a.cleanup) r = out_cleanup(O, r);
+      if (r < 0) continue;
+      int len = 0, v = r;
+      do {
+        st(xy, nv + len, ld(O.po, v));
+        len++;
+        v = O.next[v];
+      } while (v != r);
+      clen[nc++] = len;
+      nv += len;
+    }
+    a.outNComp[k] = nc;
+    a.outNVerts[k] = nv;
+    return;
+  }
+  __syncthreads();
+  // cleanUpResult per component (one lane each), then compaction
+  for (int c = lane; c < O.ncomp; c += 64) {
+    int r = O.compRoot[c];
+    if (a.cleanup) r = out_cleanup(O, r);
+    O.compRoot[c] = r;
+    int len = 0;
+    if (r >= 0) {
+      int v = r;
+      do {
+        len++;
+        v = O.next[v];
+      } while (v != r);
+    }
+    ax[c] = len;
+  }
+  __syncthreads();
+  double *xy = a.xyTmp + 2 * nb0;
+  int *clen = a.compLen + a.compOff[k];
+  int vcarry = 0, ccarry = 0;
+  for (int base = 0; base < O.ncomp; base += 64) {
+    const int c = base + lane;
+    const int len = c < O.ncomp ? ax[c] : 0;
+    const bool keep = len > 0;
+    unsigned long long m = __ballot(keep);
+    int incl = len;
+    for (int off = 1; off < 64; off <<= 1) {
+      int u = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += u;
+    }
+    if (keep) {
+      const int voff = vcarry + incl - len;
+      clen[ccarry + lt_count(m)] = len;
+      int v = O.compRoot[c];
+      for (int i = 0; i < len; i++) {
+        st(xy, voff + i, ld(O.po, v));
+        v = O.next[v];
+      }
+    }
+    vcarry += __shfl(incl, 63, 64);
+    ccarry += __popcll(m);
+  }
+  if (lane == 0) {
+    a.outNComp[k] = ccarry;
+    a.outNVerts[k] = vcarry;
+  }
+}
+
 // node / component capacity per pair
 __global__ void k_capacity(long M, const long long *offP, const long long *offQ,
                            const long long *psP2, const long long *psQ2, const int *status,
@@ -1298,10 +1729,16 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   ta.compLen = wsT<int>(c, "gh_compLen", comps);
   ta.xyTmp = wsT<double>(c, "gh_xyTmp", 2 * nodes);
   ta.cleanup = opt.cleanup;
-  {
+  if (opt.label_mode == 1) {
     LaunchTimer t(c, "gh_label_trace");
     hipLaunchKernelGGL(k_label_trace, dim3(grid_for(M, 64)), dim3(64), 0, c.stream, M,
                        pm.status, s, ta);
+    HIP_CHECK(hipGetLastError());
+  } else {
+    int *aux = wsT<int>(c, "gh_aux", nodes);
+    LaunchTimer t(c, "gh_label_trace");
+    hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), 0, c.stream, M, pm.status, s,
+                       ta, aux);
     HIP_CHECK(hipGetLastError());
   }
   res.vertOff = wsT<long long>(c, "gh_vOff", M + 1);

@@ -209,6 +209,7 @@ C_FIELDS = {"status": (0, np.int32), "swapped": (1, np.int32), "ncomp": (2, np.i
 
 CLIP_NORMALIZE = 1
 CLIP_NO_CLEANUP = 2
+CLIP_SEQ_LABEL = 4
 
 
 def hip_lib():

@@ -101,6 +101,7 @@ int gcmf_join_get(gcmf_ctx *ctx, int field, void *dst, long count);
 /* ------------------------------------------------------------ clip ----- */
 #define GCMF_CLIP_NORMALIZE 1 /* per-pair CMBR normalisation of predicates */
 #define GCMF_CLIP_NO_CLEANUP 2 /* skip cleanUpResult (polyclip.h:791)      */
+#define GCMF_CLIP_SEQ_LABEL 4  /* label/trace one thread per pair (checking) */
 
 typedef struct {
   long pairs, ok, invalid, disjoint, failed;
