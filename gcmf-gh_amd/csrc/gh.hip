@@ -1069,17 +1069,22 @@ __global__ void __launch_bounds__(64)
 // One wave64 per pair.  The lists are built in parallel; labelling step 1/3
 // runs per vertex; when every intersection vertex is a plain CROSSING (no
 // bouncing, chain or overlap labels, mutual links, a non-intersection vertex
-// on both rings) the remaining steps reduce to prefix parities and
-// independent cycle traces:
+// on both rings) the remaining steps reduce to prefix parities and list
+// ranking over crossing segments:
 //   step 4: enex of the i-th intersection vertex after the start vertex V is
 //           the start status toggled by the parity of the intersections
 //           before it (ballot prefix counts);
-//   createResult: each component is traced by one lane from its smallest
-//           P crossing vertex, which is where the sequential loop over P's
-//           crossing vertices starts it; components are emitted in that
-//           order, vertices in the sequential traversal order.
-// Every other pair (degenerate labels, inconsistent links) runs the
-// sequential code above on lane 0, so results are identical either way.
+//   createResult: every crossing vertex c departs along its ring to the
+//           adjacent crossing A(c) in its direction of travel (one segment)
+//           and continues from the neighbour of A(c).  Those successor links
+//           form cycles, each result component and its reverse; pointer
+//           jumping finds each cycle's smallest P crossing (where the
+//           sequential loop over P's crossing vertices starts it) and the
+//           output offset of every segment, and the ring vertices are then
+//           written in one flat pass.
+// Every other pair (degenerate labels, inconsistent links, a point met twice
+// by one cycle) runs the sequential code above on lane 0, so results are
+// identical either way.
 
 __device__ __forceinline__ int wave_sum(int v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -1137,53 +1142,254 @@ __device__ void wave_enex(Nodes &N, int r0, int n, int V, int status0) {
   }
 }
 
-// read-only trace of the component through P crossing vertex S (same moves
-// as createResult); returns its length, or -1 when it does not close
-__device__ int trace_len(const Nodes &N, int S, int cP, int guard, int &minP, int &pcount) {
-  int V = S, len = 0;
-  minP = S;
-  pcount = 1;
-  do {
-    int status = N.enex[V];
-    toggle(status);
-    do {
-      V = status == EXIT ? N.next[V] : N.prev[V];
-      len++;
-      if (V != S && V < cP && isint(N, V)) {
-        minP = min(minP, V);
-        pcount++;
-      }
-      if (len > guard) return -1;
-    } while (!(N.enex[V] == status) && V != S);
-    if (V != S) {
-      V = N.nb[V];
-      if (V < 0) return -1;
-      if (V != S && V < cP && isint(N, V)) {
-        minP = min(minP, V);
-        pcount++;
-      }
-    }
-  } while (V != S);
-  return len;
+// One departure of createResult (polyclip.h:743-783): crossing vertex `node`
+// moves to next (fwd, status EXIT after the toggle) or prev and stops at the
+// adjacent crossing `arr` of its ring, `dist` vertices on (arrival included).
+// Departures 0..XP-1 are P's crossings in ring order, XP..XP+XQ-1 Q's; dn maps
+// a departure to its node.
+struct Departure {
+  int node, arr, dist;
+  bool fwd;
+};
+
+__device__ __forceinline__ Departure departure(const Nodes &N, const int *dn, int d, int nP,
+                                               int nQ, int XP, int XQ) {
+  const bool onP = d < XP;
+  const int base = onP ? 0 : XP, XR = onP ? XP : XQ, nR = onP ? nP : nQ, i = d - base;
+  Departure r;
+  r.node = dn[d];
+  r.fwd = N.enex[r.node] == ENTRY;
+  r.arr = dn[base + (r.fwd ? (i + 1 == XR ? 0 : i + 1) : (i == 0 ? XR - 1 : i - 1))];
+  int dist = r.fwd ? r.arr - r.node : r.node - r.arr;
+  if (dist <= 0) dist += nR;
+  r.dist = dist;
+  return r;
 }
 
-__device__ void trace_write(const Nodes &N, int S, OutBuf &O, int off) {
-  int V = S, i = 0;
-  do {
-    int status = N.enex[V];
-    toggle(status);
-    do {
-      V = status == EXIT ? N.next[V] : N.prev[V];
-      st(O.pn, off + i, P_(N, V));
-      st(O.po, off + i, ld(N.po, V));
-      i++;
-    } while (!(N.enex[V] == status) && V != S);
-    if (V != S) V = N.nb[V];
-  } while (V != S);
-  for (int j = 0; j < i; j++) {
-    O.next[off + j] = off + (j + 1 == i ? 0 : j + 1);
-    O.prev[off + j] = off + (j == 0 ? i - 1 : j - 1);
+enum { SEG_DONE = 0, SEG_CLEANUP = 1, SEG_SLOW = 2, SEG_OVERFLOW = 3 };
+
+// createResult for a pair whose intersection vertices are all CROSSING with
+// mutual links and whose enex alternates along both rings.  `rg` holds
+// 10 * (nP + nQ) ints of scratch.  Writes the components (roots in
+// O.compRoot, lengths in clen, result vertices in xy, normalised copies in
+// O.pn) in the sequential order.  Returns SEG_DONE when nothing is left to
+// clean up, SEG_CLEANUP when some output vertex meets cleanUpResult's test
+// (O.po / next / prev are then filled for the sequential clean-up), SEG_SLOW
+// when the pair needs the sequential trace.
+__device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, int *rg, int nP,
+                             int nQ, int cleanup) {
+  const int lane = __lane_id();
+  const int n = nP + nQ;
+  int *rk = rg, *dn = rg + n;
+  int2 *vc[2] = {(int2 *)(rg + 2 * n), (int2 *)(rg + 4 * n)};
+  int *nx[2] = {rg + 6 * n, rg + 7 * n};
+  int *cbase = rg + 8 * n, *off = rg + 9 * n;
+  // departure index of every crossing vertex, by ring order
+  int carry = 0;
+  for (int b = 0; b < nP; b += 64) {
+    const int j = b + lane;
+    const bool c = j < nP && isint(N, j);
+    const unsigned long long m = __ballot(c);
+    if (c) {
+      const int r = carry + lt_count(m);
+      rk[j] = r;
+      dn[r] = j;
+    }
+    carry += __popcll(m);
   }
+  const int XP = carry;
+  for (int b = 0; b < nQ; b += 64) {
+    const int v = nP + b + lane;
+    const bool c = b + lane < nQ && isint(N, v);
+    const unsigned long long m = __ballot(c);
+    if (c) {
+      const int r = carry + lt_count(m);
+      rk[v] = r;
+      dn[r] = v;
+    }
+    carry += __popcll(m);
+  }
+  const int XQ = carry - XP, D = carry;
+  if (XP != XQ || XP == 0) return SEG_SLOW;
+  __syncthreads();
+  // successor of each departure: the departure from the neighbour of its
+  // arrival; the arrival must carry the status the inner loop stops on
+  // (partners are crossings of the other ring by the mutual-link check the
+  // caller made; they are checked again here since rk is only defined there)
+  bool bad = false;
+  for (int d = lane; d < D; d += 64) {
+    const Departure p = departure(N, dn, d, nP, nQ, XP, XQ);
+    const bool onP = d < XP;
+    const int q = N.nb[p.arr], pn = N.nb[p.node];
+    if (N.enex[p.arr] != (p.fwd ? EXIT : ENTRY) || q < 0 || pn < 0 || (q < nP) == onP ||
+        (pn < nP) == onP || !isint(N, q) || !isint(N, pn)) {
+      bad = true;
+      continue;
+    }
+    nx[0][d] = rk[q];
+    vc[0][d] = make_int2(onP ? p.node : pn, d);
+    off[d] = p.dist;
+  }
+  if (__ballot(bad)) return SEG_SLOW;
+  // pointer jumping: smallest P vertex (.x) and smallest departure (.y) of
+  // every cycle
+  int rounds = 0;
+  while ((1 << rounds) < D) rounds++;
+  int cur = 0;
+  for (int t = 0; t < rounds; t++) {
+    __syncthreads();
+    const int2 *vs = vc[cur];
+    const int *ns = nx[cur];
+    int2 *vd = vc[cur ^ 1];
+    int *nd = nx[cur ^ 1];
+    for (int d = lane; d < D; d += 64) {
+      const int x = ns[d];
+      const int2 a = vs[d], b = vs[x];
+      vd[d] = make_int2(min(a.x, b.x), min(a.y, b.y));
+      nd[d] = ns[x];
+    }
+    cur ^= 1;
+  }
+  __syncthreads();
+  const int2 *mc = vc[cur];
+  // a cycle is traced when it holds the departure of its smallest P vertex;
+  // its list starts there.  S: segment lengths, then suffix sums
+  int *S[2] = {(int *)vc[cur ^ 1], (int *)vc[cur ^ 1] + n};
+  for (int d = lane; d < D; d += 64) {
+    const int2 c = mc[d];
+    const int sd = rk[c.x];
+    const bool sel = mc[sd].y == c.y;
+    int nxt = -1;
+    if (sel) {
+      const Departure p = departure(N, dn, d, nP, nQ, XP, XQ);
+      const int s = rk[N.nb[p.arr]];
+      nxt = s == sd ? -1 : s;
+    }
+    S[1][d] = sel ? 1 : 0;
+    S[0][d] = off[d];
+    nx[0][d] = nxt;
+  }
+  __syncthreads();
+  // each intersection point departs exactly once over the traced cycles
+  for (int d = lane; d < XP; d += 64)
+    if (S[1][d] == S[1][rk[N.nb[dn[d]]]]) bad = true;
+  if (__ballot(bad)) return SEG_SLOW;
+  cur = 0;
+  for (int t = 0; t < rounds; t++) {
+    __syncthreads();
+    const int *ss = S[cur], *ns = nx[cur];
+    int *sd = S[cur ^ 1], *nd = nx[cur ^ 1];
+    for (int d = lane; d < D; d += 64) {
+      const int x = ns[d];
+      if (x >= 0) {
+        sd[d] = ss[d] + ss[x];
+        nd[d] = ns[x];
+      } else {
+        sd[d] = ss[d];
+        nd[d] = -1;
+      }
+    }
+    cur ^= 1;
+  }
+  __syncthreads();
+  const int *Sf = S[cur];
+  // components in order of their start vertex
+  int *cix = nx[0], *ciD = nx[1];
+  int vcarry = 0, ccarry = 0;
+  for (int b = 0; b < XP; b += 64) {
+    const int d = b + lane;
+    const bool start = d < XP && mc[d].x == dn[d];
+    const unsigned long long m = __ballot(start);
+    const int len = start ? Sf[d] : 0;
+    int incl = len;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    if (start) {
+      const int ci = ccarry + lt_count(m), base = vcarry + incl - len;
+      cbase[d] = base;
+      cix[d] = ci;
+      if (ci < O.capComp) {
+        O.compRoot[ci] = base;
+        clen[ci] = len;
+      }
+    }
+    vcarry += __shfl(incl, 63, 64);
+    ccarry += __popcll(m);
+  }
+  if (vcarry > O.cap || ccarry > O.capComp) return SEG_OVERFLOW;
+  // each ring vertex lies on at most one traced segment
+  if (vcarry > n) return SEG_SLOW;
+  O.n = vcarry;
+  O.ncomp = ccarry;
+  __syncthreads();
+  for (int d = lane; d < D; d += 64) {
+    const int2 c = mc[d];
+    const int sd = rk[c.x];
+    if (mc[sd].y == c.y) {
+      off[d] = cbase[sd] + Sf[sd] - Sf[d];
+      ciD[d] = cix[sd];
+    } else {
+      off[d] = -1;
+    }
+  }
+  __syncthreads();
+  // every ring vertex lies on the forward segment of the crossing before it
+  // or the backward segment of the crossing after it; at most one of the two
+  // is traced
+  int *cw = rk;  // output position -> component
+  for (int ring = 0; ring < 2; ring++) {
+    const int rb = ring ? nP : 0, nR = ring ? nQ : nP, base = ring ? XP : 0;
+    const int XR = ring ? XQ : XP;
+    carry = 0;
+    for (int b = 0; b < nR; b += 64) {
+      const int j = b + lane, u = rb + j;
+      const bool c = j < nR && isint(N, u);
+      const unsigned long long m = __ballot(c);
+      if (j < nR) {
+        const int r = carry + lt_count(m), ri = r + (c ? 1 : 0);
+        const int df = base + (r == 0 ? XR - 1 : r - 1), db = base + (ri == XR ? 0 : ri);
+        int w = -1, ci = 0;
+        if (off[df] >= 0 && N.enex[dn[df]] == ENTRY) {
+          int dist = u - dn[df];
+          if (dist <= 0) dist += nR;
+          w = off[df] + dist - 1;
+          ci = ciD[df];
+        } else if (off[db] >= 0 && N.enex[dn[db]] == EXIT) {
+          int dist = dn[db] - u;
+          if (dist <= 0) dist += nR;
+          w = off[db] + dist - 1;
+          ci = ciD[db];
+        }
+        if (w >= 0 && w < O.n) {
+          st(O.pn, w, P_(N, u));
+          st(xy, w, ld(N.po, u));
+          cw[w] = ci;
+        }
+      }
+      carry += __popcll(m);
+    }
+  }
+  __syncthreads();
+  const int T = O.n;
+  bool cand = false;
+  if (cleanup) {
+    for (int w = lane; w < T; w += 64) {
+      const int c = min(max(cw[w], 0), O.ncomp - 1), b = O.compRoot[c], L = clen[c];
+      const int pw = w == b ? b + L - 1 : w - 1, nw = w == b + L - 1 ? b : w + 1;
+      if (fabs(triA(ld(O.pn, pw), ld(O.pn, w), ld(O.pn, nw))) < kGhEps) cand = true;
+    }
+  }
+  if (!__ballot(cand)) return SEG_DONE;
+  for (int w = lane; w < T; w += 64) {
+    const int c = min(max(cw[w], 0), O.ncomp - 1), b = O.compRoot[c], L = clen[c];
+    st(O.po, w, ld(xy, w));
+    O.prev[w] = w == b ? b + L - 1 : w - 1;
+    O.next[w] = w == b + L - 1 ? b : w + 1;
+  }
+  __syncthreads();
+  return SEG_CLEANUP;
 }
 
 __device__ void copy_ring_to(OutBuf &O, const Nodes &N, int r0, int n, int off) {
@@ -1221,11 +1427,14 @@ __global__ void __launch_bounds__(64)
   N.enex = a.nenex + nb0;
   N.n = (int)(cP + cQ);
   N.cap = (int)cap;
-  int *ax = aux + nb0;
   const long long pE0 = a.offP[k], pE1 = a.offP[k + 1];
   const long long qE0 = a.offQ[k], qE1 = a.offQ[k + 1];
   const int nP = (int)cP, nQ = (int)cQ;
-  // -------- build (ghcuda.cpp:239-322 semantics, see k_label_trace)
+  int *rg = aux + 5 * nb0;  // 10 * (nP + nQ) ints of segment scratch
+  int *ax = rg;
+  // -------- build (ghcuda.cpp:239-322 semantics, see k_label_trace): every
+  // slot is first taken as an intersection vertex, then the first slot of
+  // each edge is re-flagged as its parent vertex
   for (int j = lane; j < nP; j += 64) {
     const long long sl = sbP + j;
     st(N.pn, j, ld(s.iP, sl));
@@ -1235,14 +1444,7 @@ __global__ void __launch_bounds__(64)
     N.nb[j] = -1;
     N.lab[j] = L_NONE;
     N.enex[j] = NEITHER;
-    long long lo = pE0, hi = pE1 - 1;  // last edge whose parent slot <= sl
-    while (lo < hi) {
-      long long mid = (lo + hi + 1) >> 1;
-      if (a.psP2[mid] <= sl) lo = mid;
-      else hi = mid - 1;
-    }
-    N.fl[j] = a.psP2[lo] == sl ? (unsigned char)(F_SRC | (s.aP[sl] != kNone ? F_INT : 0))
-                               : (unsigned char)F_INT;
+    N.fl[j] = F_INT;
   }
   for (int j = lane; j < nQ; j += 64) {
     const long long sl = sbQ + j;
@@ -1253,23 +1455,20 @@ __global__ void __launch_bounds__(64)
     N.next[v] = j == nQ - 1 ? nP : v + 1;
     N.lab[v] = L_NONE;
     N.enex[v] = NEITHER;
-    long long lo = qE0, hi = qE1 - 1;
-    while (lo < hi) {
-      long long mid = (lo + hi + 1) >> 1;
-      if (a.psQ2[mid] <= sl) lo = mid;
-      else hi = mid - 1;
-    }
-    const bool parent = a.psQ2[lo] == sl;
-    unsigned char fl = parent ? F_SRC : 0;
-    const bool linked = parent ? (s.aQ[sl] != kNone || s.nbQ[sl] > 0) : true;
-    int nbv = -1;
-    if (linked) {
-      fl |= F_INT;
-      long long kp = (long long)s.nbQ[sl] - 1 - sbP;
-      if (kp >= 0 && kp < cP) nbv = (int)kp;
-    }
-    N.fl[v] = fl;
-    N.nb[v] = nbv;
+    // an unlinked parent has nbQ <= 0, which lands outside [0, cP) as well
+    const long long kp = (long long)s.nbQ[sl] - 1 - sbP;
+    N.nb[v] = (kp >= 0 && kp < cP) ? (int)kp : -1;
+    N.fl[v] = F_INT;
+  }
+  __syncthreads();
+  for (long long e = pE0 + lane; e < pE1; e += 64) {
+    const long long sl = a.psP2[e];
+    N.fl[sl - sbP] = (unsigned char)(F_SRC | (s.aP[sl] != kNone ? F_INT : 0));
+  }
+  for (long long e = qE0 + lane; e < qE1; e += 64) {
+    const long long sl = a.psQ2[e];
+    const bool linked = s.aQ[sl] != kNone || s.nbQ[sl] > 0;
+    N.fl[nP + (sl - sbQ)] = (unsigned char)(F_SRC | (linked ? F_INT : 0));
   }
   __syncthreads();
   // a P node linked from several Q slots keeps the last one (link() order)
@@ -1361,59 +1560,24 @@ __global__ void __launch_bounds__(64)
       wave_enex(N, 0, nP, VP, wave_pip(N, nP, nQ, P_(N, VP)) ? EXIT : ENTRY);
       wave_enex(N, nP, nQ, VQ, wave_pip(N, 0, nP, P_(N, VQ)) ? EXIT : ENTRY);
       __syncthreads();
-      // createResult, pass 1: which crossing vertices start a component
-      const int guard = 4 * N.n + 64;
-      int pcnt = 0, fail = 0;
-      for (int S = lane; S < nP; S += 64) {
-        if (!isint(N, S)) continue;
-        int minP, pc;
-        int len = trace_len(N, S, nP, guard, minP, pc);
-        if (len < 0) {
-          fail = 1;
-          ax[S] = -1;
-        } else if (minP == S) {
-          ax[S] = len;
-          pcnt += pc;
-        } else {
-          ax[S] = -1;
+      const int rc = segment_trace(N, O, a.xyTmp + 2 * nb0, a.compLen + a.compOff[k], rg, nP,
+                                   nQ, a.cleanup);
+      if (rc == SEG_OVERFLOW) {
+        if (lane == 0) {
+          status[k] = GH_OVERFLOW;
+          a.outNComp[k] = 0;
+          a.outNVerts[k] = 0;
         }
+        return;
       }
-      fail = __ballot(fail != 0) != 0ULL;
-      pcnt = wave_sum(pcnt);
-      if (fail || pcnt != nInterP) slow = true;
-      __syncthreads();
-      if (!slow) {
-        // pass 2: offsets by a prefix over starts in P order, then write
-        int vcarry = 0, ccarry = 0;
-        for (int base = 0; base < nP; base += 64) {
-          const int S = base + lane;
-          const int len = (S < nP && isint(N, S)) ? ax[S] : -1;
-          const bool start = len >= 0;
-          unsigned long long m = __ballot(start);
-          int v = start ? len : 0, incl = v;
-          for (int off = 1; off < 64; off <<= 1) {
-            int u = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += u;
-          }
-          const int voff = vcarry + incl - v, cidx = ccarry + lt_count(m);
-          if (start && voff + len <= O.cap && cidx < O.capComp) {
-            trace_write(N, S, O, voff);
-            O.compRoot[cidx] = voff;
-          }
-          vcarry += __shfl(incl, 63, 64);
-          ccarry += __popcll(m);
+      if (rc == SEG_DONE) {
+        if (lane == 0) {
+          a.outNComp[k] = O.ncomp;
+          a.outNVerts[k] = O.n;
         }
-        if (vcarry > O.cap || ccarry > O.capComp) {
-          if (lane == 0) {
-            status[k] = GH_OVERFLOW;
-            a.outNComp[k] = 0;
-            a.outNVerts[k] = 0;
-          }
-          return;
-        }
-        O.n = vcarry;
-        O.ncomp = ccarry;
+        return;
       }
+      if (rc == SEG_SLOW) slow = true;
     }
   }
   if (slow) {
@@ -1735,7 +1899,9 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                        pm.status, s, ta);
     HIP_CHECK(hipGetLastError());
   } else {
-    int *aux = wsT<int>(c, "gh_aux", nodes);
+    // node capacity per pair is >= 2 (nP + nQ), so 5 ints per node give each
+    // pair the 10 (nP + nQ) ints of segment scratch
+    int *aux = wsT<int>(c, "gh_aux", 5 * nodes);
     LaunchTimer t(c, "gh_label_trace");
     hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), 0, c.stream, M, pm.status, s,
                        ta, aux);
