@@ -105,8 +105,8 @@ def main():
     kernels = ["mbr_endpoints", "sort_hist", "sort_scatter", "scan_tile", "scan_reduce",
                "scan_downsweep", "mbr_sweep_prep", "mbr_sweep_count", "mbr_sweep_write",
                "get_cmbr", "pip", "cmf_count", "cmf_apply", "segment_join", "gh_pair_setup",
-               "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own", "gh_calc", "gh_sort_q",
-               "gh_init_label", "gh_label_trace", "gh_gather_out", "fill_i32"]
+               "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own", "gh_calc", "gh_sort_p",
+               "gh_sort_q", "gh_init_label", "gh_label_trace", "gh_gather_out", "fill_i32"]
     times = {k: ctx.timer(k) for k in kernels}
     ms_step = 1e3 * elapsed / args.steps
     pairs = cs["pairs"]
@@ -187,7 +187,8 @@ def batched(args):
     avg_ms = cnt_ms / max(cnt_n, 1)
     achieved = FLOPS_PER_EVAL * cs["n_eval"] / (avg_ms * 1e-3) / 1e12 if avg_ms else 0.0
     kernels = ["gh_pair_setup", "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own",
-               "gh_calc", "gh_sort_q", "gh_init_label", "gh_label_trace", "gh_gather_out"]
+               "gh_calc", "gh_sort_p", "gh_sort_q", "gh_init_label", "gh_label_trace",
+               "gh_gather_out", "sort_hist", "sort_scatter"]
     line = {"metric": METRIC, "value": round(args.pairs / (elapsed / args.steps), 2),
             "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,

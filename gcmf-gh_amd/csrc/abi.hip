@@ -289,6 +289,8 @@ int gcmf_clip_get(gcmf_ctx *ctx, int field, void *dst, long count) {
       case GCMF_C_VERTOFF: src = r.vertOff; maxc = M + 1; elem = 8; break;
       case GCMF_C_COMPSIZES: src = r.compSizes; maxc = r.totalComps; break;
       case GCMF_C_XY: src = r.xy; maxc = 2 * r.totalVerts; elem = 8; break;
+      case GCMF_C_LABEL_PATH: src = r.labelPath; maxc = r.labelPath ? M : 0; break;
+      case GCMF_C_LABEL_TICKS: src = r.labelTicks; maxc = r.labelTicks ? M : 0; elem = 8; break;
       case GCMF_C_OFFP: src = p.offP; maxc = M + 1; elem = 8; break;
       case GCMF_C_OFFQ: src = p.offQ; maxc = M + 1; elem = 8; break;
       case GCMF_C_PSP1: src = p.psP1; maxc = p.edgesP + 1; elem = 8; break;

@@ -75,6 +75,10 @@ struct ClipResult {
   long totalComps = 0, totalVerts = 0;
   int *compSizes = nullptr;  // [totalComps]
   double *xy = nullptr;      // [2*totalVerts] original coordinates
+  // labelling path of each pair (LP_* in gh.hip) and its duration in ticks
+  // of the device's constant-rate wall clock; null for the one-thread path
+  int *labelPath = nullptr;
+  long long *labelTicks = nullptr;
   // phase-1 statistics
   long long nEvalP = 0, totalIntersections = 0;
 };

@@ -16,6 +16,8 @@
 // Every predicate runs on per-pair normalised coordinates
 // ((x-cx)*s, (y-cy)*s), (cx,cy) = CMBR corner, s = 1/max CMBR extent, and
 // every output vertex is produced in original coordinates from the same alpha.
+#include <functional>
+
 #include "gcmf_device.hpp"
 #include "gcmf_internal.hpp"
 #include "gcmf_join.hpp"
@@ -455,45 +457,122 @@ __global__ void __launch_bounds__(256)
       setA(start, kb);
     }
   }
-  // per-edge alpha sort (:1103-1120)
-  const long long lo = base + 1, hi = psP2[g + 1];
-  if (hi - lo >= 2) {
-    for (long long j = lo; j < hi; j++) {
-      const int nb = OWN ? 0 : s.nbP2[j];
-      if (OWN) continue;
-      const long long d = lo + stable_dest(s.aP2, lo, hi, j);
-      s.aP[d] = s.aP2[j];
-      st(s.iP, d, ld(s.iP2, j));
-      st(s.oP, d, ld(s.oP2, j));
-      s.nbP[d] = nb;
-      if (nb > 0) setN(nb - 1, (int)(d + 1));
-    }
+}
+
+// ------------------------------------------------ per-edge alpha sorts ----
+// The intersection slots after each edge's parent slot are put in order of
+// their truncated alpha, stably.  Runs of up to kSmallSort slots are ordered
+// by their edge's thread (stable_dest); longer runs go through one segmented
+// radix sort keyed (run, alpha) in slot order, which is the same stable order.
+constexpr int kSmallSort = 32;
+
+// move of P slot j to sorted position d (calculateIntersections :1103-1120);
+// the Q slot's back link follows when edge pid owns it
+__device__ __forceinline__ void move_p(SlotArrays &s, const Owners &own, long long j,
+                                       long long d, int pid) {
+  const int nb = s.nbP2[j];
+  s.aP[d] = s.aP2[j];
+  st(s.iP, d, ld(s.iP2, j));
+  st(s.oP, d, ld(s.oP2, j));
+  s.nbP[d] = nb;
+  if (nb > 0 && own.n[nb - 1] == pid) {
+    s.nbQ[nb - 1] = (int)(d + 1);
+    s.nbQ2[nb - 1] = (int)(d + 1);
   }
 }
 
-// gpuSortPolyQ (:1132-1160).  OWN pass: the highest Q edge writing a P
-// slot's back link owns it (sequential loop order over Q edges).
+// move of Q slot j to sorted position d (gpuSortPolyQ :1132-1160); the P
+// slot's back link follows when Q edge g owns it (the highest Q edge writing
+// it, the order of a sequential loop over Q edges)
+__device__ __forceinline__ void move_q(SlotArrays &s, const int *ownP, long long j, long long d,
+                                       long g) {
+  const int nb = s.nbQ2[j];
+  s.aQ[d] = s.aQ2[j];
+  st(s.iQ, d, ld(s.iQ2, j));
+  st(s.oQ, d, ld(s.oQ2, j));
+  s.nbQ[d] = nb;
+  if (nb > 0 && ownP[nb - 1] == (int)g) s.nbP[nb - 1] = (int)(d + 1);
+}
+
+// edges whose run is longer than kSmallSort, with their run lengths
+__global__ void k_list_long_runs(long edges, const int *map, const long long *ps2,
+                                 const int *status, int *list, int *runLen, int *count) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edges) return;
+  if (status[map[g]] != GH_OK) return;
+  const long long m = ps2[g + 1] - ps2[g] - 1;
+  if (m > kSmallSort) {
+    const int i = atomicAdd(count, 1);
+    list[i] = (int)g;
+    runLen[i] = (int)m;
+  }
+}
+
+__global__ void k_sort_p_short(long edgesP, const int *mapP, const long long *offP,
+                               const long long *psP2, const int *status, SlotArrays s,
+                               Owners own) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edgesP) return;
+  const int k = mapP[g];
+  if (status[k] != GH_OK) return;
+  const long long lo = psP2[g] + 1, hi = psP2[g + 1];
+  if (hi - lo < 2 || hi - lo > kSmallSort) return;
+  const int pid = (int)(g - offP[k]);
+  for (long long j = lo; j < hi; j++) move_p(s, own, j, lo + stable_dest(s.aP2, lo, hi, j), pid);
+}
+
+// OWN pass: record the highest Q edge writing each P slot's back link
 template <bool OWN>
-__global__ void k_sort_q(long edgesQ, const int *mapQ, const long long *psQ2, const int *status,
-                         SlotArrays s, int *ownP) {
+__global__ void k_sort_q_short(long edgesQ, const int *mapQ, const long long *psQ2,
+                               const int *status, SlotArrays s, int *ownP) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= edgesQ) return;
   if (status[mapQ[g]] != GH_OK) return;
   const long long lo = psQ2[g] + 1, hi = psQ2[g + 1];
-  if (hi - lo < 2) return;
+  if (hi - lo < 2 || hi - lo > kSmallSort) return;
   for (long long j = lo; j < hi; j++) {
-    const int nb = s.nbQ2[j];
     if (OWN) {
+      const int nb = s.nbQ2[j];
       if (nb > 0) atomicMax(&ownP[nb - 1], (int)g);
-      continue;
+    } else {
+      move_q(s, ownP, j, lo + stable_dest(s.aQ2, lo, hi, j), g);
     }
-    const long long d = lo + stable_dest(s.aQ2, lo, hi, j);
-    s.aQ[d] = s.aQ2[j];
-    st(s.iQ, d, ld(s.iQ2, j));
-    st(s.oQ, d, ld(s.oQ2, j));
-    s.nbQ[d] = nb;
-    if (nb > 0 && ownP[nb - 1] == (int)g) s.nbP[nb - 1] = (int)(d + 1);
   }
+}
+
+// keys of the long runs: (run index, alpha as an order-preserving u32), the
+// slot as value; for Q runs also the OWN pass of their slots
+template <bool QOWN>
+__global__ void k_long_run_keys(int nRuns, const int *list, const long long *runOff,
+                                const long long *ps2, const int *a2, const int *nb2, int *ownP,
+                                uint64_t *keys, int *vals) {
+  for (int r = blockIdx.x; r < nRuns; r += gridDim.x) {
+    const long g = list[r];
+    const long long lo = ps2[g] + 1, m = ps2[g + 1] - lo, o = runOff[r];
+    for (long long t = threadIdx.x; t < m; t += blockDim.x) {
+      const long long j = lo + t;
+      keys[o + t] = ((uint64_t)r << 32) | (uint64_t)((unsigned)a2[j] ^ 0x80000000u);
+      vals[o + t] = (int)j;
+      if (QOWN) {
+        const int nb = nb2[j];
+        if (nb > 0) atomicMax(&ownP[nb - 1], (int)g);
+      }
+    }
+  }
+}
+
+template <bool QSIDE>
+__global__ void k_long_run_apply(long n, const uint64_t *keys, const int *vals, const int *list,
+                                 const long long *runOff, const long long *ps2, const int *map,
+                                 const long long *off, SlotArrays s, Owners own,
+                                 const int *ownP) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = (int)(keys[i] >> 32);
+  const long g = list[r];
+  const long long d = ps2[g] + 1 + (i - runOff[r]);
+  if (QSIDE) move_q(s, ownP, vals[i], d, g);
+  else move_p(s, own, vals[i], d, (int)(g - off[map[g]]));
 }
 
 // gpuCalculateInitLabel (:1171-1223); circular ids within the pair's slots.
@@ -895,6 +974,9 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
       } while (!IS_CI(I) && I != rootP);
       have = I != rootP;
     }
+    // a trace that outgrows the output region keeps walking without writing
+    // so that a runaway trace still ends in the guard's failure status
+    bool full = false;
     while (have) {
       if (O.ncomp >= O.capComp) return GH_OVERFLOW;
       int R = -1, V = I;
@@ -906,10 +988,12 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
         do {
           V = status == EXIT ? N.next[V] : N.prev[V];
           N.fl[V] &= ~F_INT;
-          int w = out_new(O, P_(N, V), ld(N.po, V));
-          if (w < 0) return GH_OVERFLOW;
-          if (R < 0) R = w;
-          else out_append(O, R, w);
+          if (!full) {
+            int w = out_new(O, P_(N, V), ld(N.po, V));
+            if (w < 0) full = true;
+            else if (R < 0) R = w;
+            else out_append(O, R, w);
+          }
           if (++steps > guard) return GH_TRACE_FAIL;
         } while (!(N.enex[V] == status) && V != I);
         if (V != I) {
@@ -918,6 +1002,7 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
           N.fl[V] &= ~F_INT;
         }
       } while (V != I);
+      if (full) return GH_OVERFLOW;
       O.compRoot[O.ncomp++] = R;
       do {
         I = N.next[I];
@@ -1125,30 +1210,79 @@ __device__ int wave_first_nonint(const Nodes &N, int r0, int n) {
   return best == 0x7fffffff ? -1 : r0 + best;
 }
 
-// step 4 for one ring whose intersection vertices are all CROSSING
+__device__ __forceinline__ bool iscross(const Nodes &N, int v) {
+  return isint(N, v) && N.lab[v] == L_CROSSING;
+}
+
+// step 4 for one ring whose intersection vertices are CROSSING or BOUNCING:
+// the status at a vertex is the start status toggled once per crossing
+// before it; crossings take it as enex, bouncing vertices met with status
+// EXIT are flagged for splitting
 __device__ void wave_enex(Nodes &N, int r0, int n, int V, int status0) {
   int carry = 0;
   const int v0 = V - r0;
   for (int base = 1; base < n; base += 64) {
-    int j = base + __lane_id();
-    int u = r0 + (v0 + j) % n;
-    bool bit = j < n && isint(N, u);
-    unsigned long long m = __ballot(bit);
-    if (bit) {
-      int par = (carry + lt_count(m)) & 1;
-      N.enex[u] = (signed char)(par ? (status0 == ENTRY ? EXIT : ENTRY) : status0);
+    const int j = base + __lane_id();
+    int u = v0 + j;
+    u = r0 + (u >= n ? u - n : u);
+    const bool in = j < n && isint(N, u);
+    const bool cr = in && N.lab[u] == L_CROSSING;
+    const unsigned long long m = __ballot(cr);
+    if (in) {
+      const int par = (carry + lt_count(m)) & 1;
+      const int status = par ? (status0 == ENTRY ? EXIT : ENTRY) : status0;
+      if (cr) N.enex[u] = (signed char)status;
+      else if (status == EXIT) N.fl[u] |= F_SPLIT;
     }
     carry += __popcll(m);
   }
 }
 
-// One departure of createResult (polyclip.h:743-783): crossing vertex `node`
-// moves to next (fwd, status EXIT after the toggle) or prev and stops at the
-// adjacent crossing `arr` of its ring, `dist` vertices on (arrival included).
-// Departures 0..XP-1 are P's crossings in ring order, XP..XP+XQ-1 Q's; dn maps
-// a departure to its node.
+// Departures of createResult (polyclip.h:743-783).  Every CROSSING vertex
+// is one.  A BOUNCING pair flagged for splitting on both rings gives two at
+// its vertex u: u itself (enex EXIT) and the copy that step 5
+// (polyclip.h:661-714) inserts after u (enex ENTRY).  Departures 0..XP-1 are
+// P's in ring order, XP..XP+XQ-1 Q's; dn[d] = 2 u + (second at u), rk[u] =
+// the first departure at u.  A departure moves to next (enex ENTRY: status
+// EXIT after the toggle) or prev and stops at the adjacent departure of its
+// ring, `dist` vertices on (arrival included).
+__device__ __forceinline__ bool issplit(const Nodes &N, int u) {
+  return isint(N, u) && N.lab[u] == L_BOUNCING && (N.fl[u] & F_SPLIT) && N.nb[u] >= 0 &&
+         (N.fl[N.nb[u]] & F_SPLIT);
+}
+__device__ __forceinline__ int dmult(const Nodes &N, int u) {
+  return iscross(N, u) ? 1 : (issplit(N, u) ? 2 : 0);
+}
+
+// step 5's link choice for the split pair at P vertex I: sP * sQ > 0 links
+// each vertex to the other ring's copy.  Copies inserted by earlier splits
+// repeat their vertex's coordinates, so the original neighbours give the
+// same areas.
+__device__ __forceinline__ bool split_cross(const Nodes &N, int I, int nP, int nQ) {
+  const int IQ = N.nb[I];
+  const int Pm = I == 0 ? nP - 1 : I - 1, Pp = I == nP - 1 ? 0 : I + 1;
+  const int Qm = IQ == nP ? nP + nQ - 1 : IQ - 1, Qp = IQ == nP + nQ - 1 ? nP : IQ + 1;
+  const double sP = triA(P_(N, Pm), P_(N, I), P_(N, Pp));
+  const double sQ = triA(P_(N, Qm), P_(N, IQ), P_(N, Qp));
+  return sP * sQ > 0;
+}
+
+__device__ __forceinline__ int dep_enex(const Nodes &N, int code) {
+  const int u = code >> 1;
+  return N.lab[u] == L_CROSSING ? N.enex[u] : ((code & 1) ? ENTRY : EXIT);
+}
+
+// departure of the neighbour of departure `code`
+__device__ __forceinline__ int dep_partner(const Nodes &N, const int *rk, int code, int nP,
+                                           int nQ) {
+  const int u = code >> 1, q = N.nb[u];
+  if (N.lab[u] == L_CROSSING) return rk[q];
+  const bool cross = u < nP ? split_cross(N, u, nP, nQ) : split_cross(N, q, nP, nQ);
+  return rk[q] + (((code & 1) == 0) == cross ? 1 : 0);
+}
+
 struct Departure {
-  int node, arr, dist;
+  int code, arr, dist;
   bool fwd;
 };
 
@@ -1157,112 +1291,135 @@ __device__ __forceinline__ Departure departure(const Nodes &N, const int *dn, in
   const bool onP = d < XP;
   const int base = onP ? 0 : XP, XR = onP ? XP : XQ, nR = onP ? nP : nQ, i = d - base;
   Departure r;
-  r.node = dn[d];
-  r.fwd = N.enex[r.node] == ENTRY;
-  r.arr = dn[base + (r.fwd ? (i + 1 == XR ? 0 : i + 1) : (i == 0 ? XR - 1 : i - 1))];
-  int dist = r.fwd ? r.arr - r.node : r.node - r.arr;
+  r.code = dn[d];
+  r.fwd = dep_enex(N, r.code) == ENTRY;
+  r.arr = base + (r.fwd ? (i + 1 == XR ? 0 : i + 1) : (i == 0 ? XR - 1 : i - 1));
+  const int u = r.code >> 1, ua = dn[r.arr] >> 1;
+  int dist = r.fwd ? ua - u : u - ua;
   if (dist <= 0) dist += nR;
   r.dist = dist;
   return r;
 }
 
-enum { SEG_DONE = 0, SEG_CLEANUP = 1, SEG_SLOW = 2, SEG_OVERFLOW = 3 };
+enum { SEG_DONE = 0, SEG_SLOW = 2, SEG_OVERFLOW = 3 };
 
-// createResult for a pair whose intersection vertices are all CROSSING with
-// mutual links and whose enex alternates along both rings.  `rg` holds
+// createResult for a pair whose intersection vertices are CROSSING or
+// BOUNCING (passed over like ordinary vertices unless split), with mutual
+// links and enex alternating over the departures of both rings.  `rg` holds
 // 10 * (nP + nQ) ints of scratch.  Writes the components (roots in
 // O.compRoot, lengths in clen, result vertices in xy, normalised copies in
-// O.pn) in the sequential order.  Returns SEG_DONE when nothing is left to
-// clean up, SEG_CLEANUP when some output vertex meets cleanUpResult's test
-// (O.po / next / prev are then filled for the sequential clean-up), SEG_SLOW
-// when the pair needs the sequential trace.
+// O.pn, rg[0, n) mapping output positions to components) in the sequential
+// order.  Returns SEG_DONE, SEG_SLOW when the pair needs the sequential
+// trace, or SEG_OVERFLOW.
 __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, int *rg, int nP,
-                             int nQ, int cleanup) {
+                             int nQ) {
   const int lane = __lane_id();
   const int n = nP + nQ;
   int *rk = rg, *dn = rg + n;
   int2 *vc[2] = {(int2 *)(rg + 2 * n), (int2 *)(rg + 4 * n)};
   int *nx[2] = {rg + 6 * n, rg + 7 * n};
   int *cbase = rg + 8 * n, *off = rg + 9 * n;
-  // departure index of every crossing vertex, by ring order
-  int carry = 0;
-  for (int b = 0; b < nP; b += 64) {
-    const int j = b + lane;
-    const bool c = j < nP && isint(N, j);
-    const unsigned long long m = __ballot(c);
-    if (c) {
-      const int r = carry + lt_count(m);
-      rk[j] = r;
-      dn[r] = j;
+  // departures by ring order
+  int carry = 0, XP = 0;
+  for (int ring = 0; ring < 2; ring++) {
+    const int rb = ring ? nP : 0, nR = ring ? nQ : nP;
+    for (int b = 0; b < nR; b += 64) {
+      const int j = b + lane, u = rb + j;
+      const int mu = j < nR ? dmult(N, u) : 0;
+      int incl = mu;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+      }
+      const int r = carry + incl - mu;
+      if (mu > 0 && r + mu <= n) {
+        rk[u] = r;
+        dn[r] = 2 * u;
+        if (mu == 2) dn[r + 1] = 2 * u + 1;
+      }
+      carry += __shfl(incl, 63, 64);
     }
-    carry += __popcll(m);
-  }
-  const int XP = carry;
-  for (int b = 0; b < nQ; b += 64) {
-    const int v = nP + b + lane;
-    const bool c = b + lane < nQ && isint(N, v);
-    const unsigned long long m = __ballot(c);
-    if (c) {
-      const int r = carry + lt_count(m);
-      rk[v] = r;
-      dn[r] = v;
-    }
-    carry += __popcll(m);
+    if (ring == 0) XP = carry;
   }
   const int XQ = carry - XP, D = carry;
-  if (XP != XQ || XP == 0) return SEG_SLOW;
+  if (XP != XQ || XP == 0 || D > n) return SEG_SLOW;
   __syncthreads();
   // successor of each departure: the departure from the neighbour of its
-  // arrival; the arrival must carry the status the inner loop stops on
-  // (partners are crossings of the other ring by the mutual-link check the
-  // caller made; they are checked again here since rk is only defined there)
+  // arrival; the arrival must carry the status the inner loop stops on.
+  // Key .x: the P departure of the departure's intersection point.
   bool bad = false;
   for (int d = lane; d < D; d += 64) {
     const Departure p = departure(N, dn, d, nP, nQ, XP, XQ);
     const bool onP = d < XP;
-    const int q = N.nb[p.arr], pn = N.nb[p.node];
-    if (N.enex[p.arr] != (p.fwd ? EXIT : ENTRY) || q < 0 || pn < 0 || (q < nP) == onP ||
-        (pn < nP) == onP || !isint(N, q) || !isint(N, pn)) {
+    const int ua = dn[p.arr] >> 1, u = p.code >> 1;
+    const int q = N.nb[ua], pn = N.nb[u];
+    if (dep_enex(N, dn[p.arr]) != (p.fwd ? EXIT : ENTRY) || q < 0 || pn < 0 ||
+        (q < nP) == onP || (pn < nP) == onP || dmult(N, q) == 0 || dmult(N, pn) == 0) {
       bad = true;
       continue;
     }
-    nx[0][d] = rk[q];
-    vc[0][d] = make_int2(onP ? p.node : pn, d);
+    const int s = dep_partner(N, rk, dn[p.arr], nP, nQ);
+    const int key = onP ? d : dep_partner(N, rk, p.code, nP, nQ);
+    if (s < 0 || s >= D || key < 0 || key >= XP) {
+      bad = true;
+      continue;
+    }
+    nx[0][d] = s;
+    vc[0][d] = make_int2(key, d);
     off[d] = p.dist;
   }
   if (__ballot(bad)) return SEG_SLOW;
-  // pointer jumping: smallest P vertex (.x) and smallest departure (.y) of
-  // every cycle
+  // pointer jumping: smallest P key (.x) and smallest departure (.y) of
+  // every cycle; four independent chains per lane in flight
   int rounds = 0;
   while ((1 << rounds) < D) rounds++;
   int cur = 0;
   for (int t = 0; t < rounds; t++) {
     __syncthreads();
-    const int2 *vs = vc[cur];
-    const int *ns = nx[cur];
-    int2 *vd = vc[cur ^ 1];
-    int *nd = nx[cur ^ 1];
-    for (int d = lane; d < D; d += 64) {
-      const int x = ns[d];
-      const int2 a = vs[d], b = vs[x];
-      vd[d] = make_int2(min(a.x, b.x), min(a.y, b.y));
-      nd[d] = ns[x];
+    const int2 *__restrict__ vs = vc[cur];
+    const int *__restrict__ ns = nx[cur];
+    int2 *__restrict__ vd = vc[cur ^ 1];
+    int *__restrict__ nd = nx[cur ^ 1];
+    for (int d0 = lane; d0 < D; d0 += 256) {
+      int x[4];
+      int2 a[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int d = d0 + 64 * k;
+        x[k] = d < D ? ns[d] : 0;
+        a[k] = d < D ? vs[d] : make_int2(0, 0);
+      }
+      int2 b[4];
+      int y[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        b[k] = vs[x[k]];
+        y[k] = ns[x[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int d = d0 + 64 * k;
+        if (d < D) {
+          vd[d] = make_int2(min(a[k].x, b[k].x), min(a[k].y, b[k].y));
+          nd[d] = y[k];
+        }
+      }
     }
     cur ^= 1;
   }
   __syncthreads();
   const int2 *mc = vc[cur];
-  // a cycle is traced when it holds the departure of its smallest P vertex;
-  // its list starts there.  S: segment lengths, then suffix sums
+  // a cycle is traced when it holds the departure of its smallest P key; its
+  // list starts there.  S: segment lengths, then suffix sums
   int *S[2] = {(int *)vc[cur ^ 1], (int *)vc[cur ^ 1] + n};
   for (int d = lane; d < D; d += 64) {
     const int2 c = mc[d];
-    const int sd = rk[c.x];
+    const int sd = c.x;
     const bool sel = mc[sd].y == c.y;
     int nxt = -1;
     if (sel) {
       const Departure p = departure(N, dn, d, nP, nQ, XP, XQ);
-      const int s = rk[N.nb[p.arr]];
+      const int s = dep_partner(N, rk, dn[p.arr], nP, nQ);
       nxt = s == sd ? -1 : s;
     }
     S[1][d] = sel ? 1 : 0;
@@ -1272,33 +1429,48 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   __syncthreads();
   // each intersection point departs exactly once over the traced cycles
   for (int d = lane; d < XP; d += 64)
-    if (S[1][d] == S[1][rk[N.nb[dn[d]]]]) bad = true;
+    if (S[1][d] == S[1][dep_partner(N, rk, dn[d], nP, nQ)]) bad = true;
   if (__ballot(bad)) return SEG_SLOW;
   cur = 0;
   for (int t = 0; t < rounds; t++) {
     __syncthreads();
-    const int *ss = S[cur], *ns = nx[cur];
-    int *sd = S[cur ^ 1], *nd = nx[cur ^ 1];
-    for (int d = lane; d < D; d += 64) {
-      const int x = ns[d];
-      if (x >= 0) {
-        sd[d] = ss[d] + ss[x];
-        nd[d] = ns[x];
-      } else {
-        sd[d] = ss[d];
-        nd[d] = -1;
+    const int *__restrict__ ss = S[cur];
+    const int *__restrict__ ns = nx[cur];
+    int *__restrict__ sd = S[cur ^ 1];
+    int *__restrict__ nd = nx[cur ^ 1];
+    for (int d0 = lane; d0 < D; d0 += 256) {
+      int x[4], a[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int d = d0 + 64 * k;
+        x[k] = d < D ? ns[d] : -1;
+        a[k] = d < D ? ss[d] : 0;
+      }
+      int b[4], y[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        b[k] = x[k] >= 0 ? ss[x[k]] : 0;
+        y[k] = x[k] >= 0 ? ns[x[k]] : -1;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int d = d0 + 64 * k;
+        if (d < D) {
+          sd[d] = a[k] + b[k];
+          nd[d] = y[k];
+        }
       }
     }
     cur ^= 1;
   }
   __syncthreads();
   const int *Sf = S[cur];
-  // components in order of their start vertex
+  // components in order of their start departure
   int *cix = nx[0], *ciD = nx[1];
   int vcarry = 0, ccarry = 0;
   for (int b = 0; b < XP; b += 64) {
     const int d = b + lane;
-    const bool start = d < XP && mc[d].x == dn[d];
+    const bool start = d < XP && mc[d].x == d;
     const unsigned long long m = __ballot(start);
     const int len = start ? Sf[d] : 0;
     int incl = len;
@@ -1319,14 +1491,14 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     ccarry += __popcll(m);
   }
   if (vcarry > O.cap || ccarry > O.capComp) return SEG_OVERFLOW;
-  // each ring vertex lies on at most one traced segment
+  // the output position map holds n entries
   if (vcarry > n) return SEG_SLOW;
   O.n = vcarry;
   O.ncomp = ccarry;
   __syncthreads();
   for (int d = lane; d < D; d += 64) {
     const int2 c = mc[d];
-    const int sd = rk[c.x];
+    const int sd = c.x;
     if (mc[sd].y == c.y) {
       off[d] = cbase[sd] + Sf[sd] - Sf[d];
       ciD[d] = cix[sd];
@@ -1335,9 +1507,10 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     }
   }
   __syncthreads();
-  // every ring vertex lies on the forward segment of the crossing before it
-  // or the backward segment of the crossing after it; at most one of the two
-  // is traced
+  // a ring vertex lies on the forward segment of the departure before it
+  // and on the backward segment of the departure after it; of these two,
+  // which are reverses of each other, at most one is traced, except at a
+  // split vertex, where the two arrive at its two copies
   int *cw = rk;  // output position -> component
   for (int ring = 0; ring < 2; ring++) {
     const int rb = ring ? nP : 0, nR = ring ? nQ : nP, base = ring ? XP : 0;
@@ -1345,73 +1518,230 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     carry = 0;
     for (int b = 0; b < nR; b += 64) {
       const int j = b + lane, u = rb + j;
-      const bool c = j < nR && isint(N, u);
-      const unsigned long long m = __ballot(c);
+      const int mu = j < nR ? dmult(N, u) : 0;
+      int incl = mu;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+      }
       if (j < nR) {
-        const int r = carry + lt_count(m), ri = r + (c ? 1 : 0);
+        const int r = carry + incl - mu, ri = r + mu;
         const int df = base + (r == 0 ? XR - 1 : r - 1), db = base + (ri == XR ? 0 : ri);
-        int w = -1, ci = 0;
-        if (off[df] >= 0 && N.enex[dn[df]] == ENTRY) {
-          int dist = u - dn[df];
+        for (int side = 0; side < 2; side++) {
+          const int dd = side ? db : df;
+          if (off[dd] < 0 || dep_enex(N, dn[dd]) != (side ? EXIT : ENTRY)) continue;
+          const int ud = dn[dd] >> 1;
+          int dist = side ? ud - u : u - ud;
           if (dist <= 0) dist += nR;
-          w = off[df] + dist - 1;
-          ci = ciD[df];
-        } else if (off[db] >= 0 && N.enex[dn[db]] == EXIT) {
-          int dist = dn[db] - u;
-          if (dist <= 0) dist += nR;
-          w = off[db] + dist - 1;
-          ci = ciD[db];
-        }
-        if (w >= 0 && w < O.n) {
-          st(O.pn, w, P_(N, u));
-          st(xy, w, ld(N.po, u));
-          cw[w] = ci;
+          const int w = off[dd] + dist - 1;
+          if (w >= 0 && w < O.n) {
+            st(O.pn, w, P_(N, u));
+            st(xy, w, ld(N.po, u));
+            cw[w] = ciD[dd];
+          }
         }
       }
-      carry += __popcll(m);
+      carry += __shfl(incl, 63, 64);
     }
   }
   __syncthreads();
-  const int T = O.n;
-  bool cand = false;
-  if (cleanup) {
-    for (int w = lane; w < T; w += 64) {
-      const int c = min(max(cw[w], 0), O.ncomp - 1), b = O.compRoot[c], L = clen[c];
-      const int pw = w == b ? b + L - 1 : w - 1, nw = w == b + L - 1 ? b : w + 1;
-      if (fabs(triA(ld(O.pn, pw), ld(O.pn, w), ld(O.pn, nw))) < kGhEps) cand = true;
-    }
-  }
-  if (!__ballot(cand)) return SEG_DONE;
-  for (int w = lane; w < T; w += 64) {
-    const int c = min(max(cw[w], 0), O.ncomp - 1), b = O.compRoot[c], L = clen[c];
-    st(O.po, w, ld(xy, w));
-    O.prev[w] = w == b ? b + L - 1 : w - 1;
-    O.next[w] = w == b + L - 1 ? b : w + 1;
-  }
-  __syncthreads();
-  return SEG_CLEANUP;
+  return SEG_DONE;
 }
 
-__device__ void copy_ring_to(OutBuf &O, const Nodes &N, int r0, int n, int off) {
+// a whole node ring as output component c at [off, off + n)
+__device__ void copy_ring_out(OutBuf &O, double *xy, int *cw, const Nodes &N, int r0, int n,
+                              int off, int c) {
   for (int j = __lane_id(); j < n; j += 64) {
     st(O.pn, off + j, P_(N, r0 + j));
-    st(O.po, off + j, ld(N.po, r0 + j));
-    O.next[off + j] = off + (j + 1 == n ? 0 : j + 1);
-    O.prev[off + j] = off + (j == 0 ? n - 1 : j - 1);
+    st(xy, off + j, ld(N.po, r0 + j));
+    cw[off + j] = c;
   }
 }
 
-__global__ void __launch_bounds__(64)
-    k_label_wave(long M, int *status, SlotArrays s, TraceArgs a, int *aux) {
-  const long k = blockIdx.x;
+// ------------------------------------------------- parallel clean-up ----
+// cleanUpResult (polyclip.h:791-807) on one component stored in list order
+// at [b, b + L).  Sequentially it (A) drops the root while
+// |A(prev, root, next)| < eps, the prev of every later root being the last
+// vertex; then (B) makes one pass from the root in which a vertex is dropped
+// when |A(last kept, it, next)| < eps.  (A) tests against a fixed vertex, so
+// it is one parallel search for the first vertex that stays.  In (B) a
+// vertex's outcome depends on the distance d back to the last kept vertex;
+// each lane evaluates its vertex for d = 1..kCleanW, which makes it a map on
+// the states {1..kCleanW, overflow}, and a wave prefix of map compositions
+// gives every lane its state.  A run of more than kCleanW dropped vertices
+// ends in the overflow state, and the component then repeats (B) on one lane.
+constexpr int kCleanW = 7;
+
+__device__ __forceinline__ unsigned nib(unsigned f, unsigned s) { return (f >> (4 * s)) & 15u; }
+// (g o f)(s) = g(f(s)) over the 8 states held as nibbles
+__device__ __forceinline__ unsigned compose(unsigned g, unsigned f) {
+  unsigned h = 0;
+  for (unsigned s = 0; s < 8; s++) h |= nib(g, nib(f, s)) << (4 * s);
+  return h;
+}
+
+// writes keep[b..b+L) (0/1) and returns the number of vertices kept
+__device__ int clean_component(const double *pn, int *keep, int b, int L) {
+  const int lane = __lane_id();
+  int r = -1;
+  if (L >= 3) {
+    const pt Pl = ld(pn, b + L - 1);
+    for (int base = 0; base <= L - 3 && r < 0; base += 64) {
+      const int i = base + lane;
+      bool stays = false;
+      if (i <= L - 3) stays = !(fabs(triA(Pl, ld(pn, b + i), ld(pn, b + i + 1))) < kGhEps);
+      const unsigned long long m = __ballot(stays);
+      if (m) r = base + __ffsll((long long)m) - 1;
+    }
+  }
+  for (int j = lane; j < (r < 0 ? L : r); j += 64) keep[b + j] = 0;
+  if (r < 0) return 0;
+  if (lane == 0) keep[b + r] = 1;
+  const int x0 = b + r, m = L - 1 - r;
+  int s = 1, kept = 1;
+  bool ovf = false;
+  for (int base = 1; base <= m; base += 64) {
+    const int j = base + lane;
+    unsigned K = 0;
+    if (j <= m) {
+      const pt cur = ld(pn, x0 + j), nxt = ld(pn, j == m ? x0 : x0 + j + 1);
+      for (int d = 1; d <= kCleanW && d <= j; d++)
+        if (!(fabs(triA(ld(pn, x0 + j - d), cur, nxt)) < kGhEps)) K |= 1u << (d - 1);
+    }
+    unsigned f = 0x76543210u;  // identity past the end
+    if (j <= m) {
+      f = 0;  // state 0 (overflow) stays 0
+      for (unsigned t = 1; t <= kCleanW; t++) {
+        const unsigned to = ((K >> (t - 1)) & 1u) ? 1u : (t < kCleanW ? t + 1 : 0u);
+        f |= to << (4 * t);
+      }
+    }
+    unsigned F = f;
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned G = __shfl_up(F, o, 64);
+      if (lane >= o) F = compose(F, G);
+    }
+    const unsigned Fex = __shfl_up(F, 1, 64);
+    const unsigned sj = lane == 0 ? (unsigned)s : nib(Fex, (unsigned)s);
+    bool kp = false;
+    if (j <= m) {
+      if (sj == 0) ovf = true;
+      else kp = ((K >> (sj - 1)) & 1u) != 0;
+      keep[x0 + j] = kp ? 1 : 0;
+    }
+    kept += __popcll(__ballot(kp));
+    s = (int)nib(__shfl(F, 63, 64), (unsigned)s);
+  }
+  if (__ballot(ovf)) {
+    if (lane == 0) {
+      pt last = ld(pn, x0);
+      kept = 1;
+      for (int j = 1; j <= m; j++) {
+        const pt cur = ld(pn, x0 + j), nxt = ld(pn, j == m ? x0 : x0 + j + 1);
+        const bool kp = !(fabs(triA(last, cur, nxt)) < kGhEps);
+        keep[x0 + j] = kp ? 1 : 0;
+        if (kp) {
+          last = cur;
+          kept++;
+        }
+      }
+    }
+    kept = __shfl(kept, 0, 64);
+  }
+  return kept;
+}
+
+// clean-up of every component of O (component c at [O.compRoot[c], +clen[c])
+// in list order, cw: position -> component), then compaction of xy and clen
+// over the kept vertices and non-empty components.  Scratch: rg[n, 5n).
+// Returns true when some vertex was dropped; O.n / O.ncomp are updated.
+__device__ bool wave_cleanup_compact(OutBuf &O, double *xy, int *clen, const int *cw, int *rg,
+                                     int n) {
+  const int lane = __lane_id();
+  const int T = O.n, NC = O.ncomp;
+  int *dirty = rg + n, *dl = rg + 2 * n, *keep = rg + 3 * n, *nlen = rg + 4 * n;
+  for (int c = lane; c < NC; c += 64) dirty[c] = 0;
+  __syncthreads();
+  for (int w = lane; w < T; w += 64) {
+    const int c = min(max(cw[w], 0), NC - 1), b = O.compRoot[c], L = clen[c];
+    const int pw = w == b ? b + L - 1 : w - 1, nw = w == b + L - 1 ? b : w + 1;
+    if (fabs(triA(ld(O.pn, pw), ld(O.pn, w), ld(O.pn, nw))) < kGhEps) dirty[c] = 1;
+  }
+  __syncthreads();
+  int nd = 0;
+  for (int base = 0; base < NC; base += 64) {
+    const int c = base + lane;
+    const bool f = c < NC && dirty[c] != 0;
+    const unsigned long long m = __ballot(f);
+    if (f) dl[nd + lt_count(m)] = c;
+    nd += __popcll(m);
+  }
+  if (nd == 0) return false;
+  __syncthreads();
+  for (int i = 0; i < nd; i++) {
+    const int c = dl[i];
+    const int kept = clean_component(O.pn, keep, O.compRoot[c], clen[c]);
+    if (lane == 0) nlen[c] = kept;
+  }
+  __syncthreads();
+  int vout = 0;
+  for (int base = 0; base < T; base += 64) {
+    const int w = base + lane;
+    bool kp = false;
+    pt v;
+    if (w < T) {
+      const int c = min(max(cw[w], 0), NC - 1);
+      kp = dirty[c] ? keep[w] != 0 : true;
+      if (kp) v = ld(xy, w);
+    }
+    const unsigned long long m = __ballot(kp);
+    if (kp) st(xy, vout + lt_count(m), v);
+    vout += __popcll(m);
+  }
+  int cout = 0;
+  for (int base = 0; base < NC; base += 64) {
+    const int c = base + lane;
+    int len = 0;
+    if (c < NC) len = dirty[c] ? nlen[c] : clen[c];
+    const unsigned long long m = __ballot(len > 0);
+    if (len > 0) clen[cout + lt_count(m)] = len;
+    cout += __popcll(m);
+  }
+  O.n = vout;
+  O.ncomp = cout;
+  return true;
+}
+
+// which way a pair went through the wave labelling kernel
+enum {
+  LP_SKIPPED = 0,      // status was not OK on entry
+  LP_SEGMENTS = 1,     // segment-parallel trace, nothing to clean up
+  LP_SEG_CLEANUP = 2,  // segment-parallel trace, then the per-component clean-up
+  LP_NO_CROSSING = 3,  // no intersection vertices: containment tests
+  LP_SEQUENTIAL = 4,   // the sequential steps on lane 0
+  LP_FAILED = 5        // overflow or failed trace
+};
+
+// true when some vertex of the ring [r0, r0+n) of O (cyclic) meets
+// cleanUpResult's removal test
+__device__ bool wave_any_cleanup(const OutBuf &O, int r0, int n) {
+  bool cand = false;
+  for (int j = __lane_id(); j < n; j += 64) {
+    const int w = r0 + j, pw = r0 + (j == 0 ? n - 1 : j - 1), nw = r0 + (j + 1 == n ? 0 : j + 1);
+    if (fabs(triA(ld(O.pn, pw), ld(O.pn, w), ld(O.pn, nw))) < kGhEps) cand = true;
+  }
+  return __ballot(cand) != 0ULL;
+}
+
+__device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const TraceArgs &a,
+                               int *aux) {
   const int lane = threadIdx.x;
-  if (k >= M) return;
   if (status[k] != GH_OK) {
     if (lane == 0) {
       a.outNComp[k] = 0;
       a.outNVerts[k] = 0;
     }
-    return;
+    return LP_SKIPPED;
   }
   const long long sbP = a.psP2[a.offP[k]], cP = a.psP2[a.offP[k + 1]] - sbP;
   const long long sbQ = a.psQ2[a.offQ[k]], cQ = a.psQ2[a.offQ[k + 1]] - sbQ;
@@ -1431,7 +1761,6 @@ __global__ void __launch_bounds__(64)
   const long long qE0 = a.offQ[k], qE1 = a.offQ[k + 1];
   const int nP = (int)cP, nQ = (int)cQ;
   int *rg = aux + 5 * nb0;  // 10 * (nP + nQ) ints of segment scratch
-  int *ax = rg;
   // -------- build (ghcuda.cpp:239-322 semantics, see k_label_trace): every
   // slot is first taken as an intersection vertex, then the first slot of
   // each edge is re-flagged as its parent vertex
@@ -1500,7 +1829,7 @@ __global__ void __launch_bounds__(64)
         if ((tm == 2 && tp == 0) || (tp == 2 && tm == 0)) l = L_ON_RIGHT;
         N.lab[I] = (signed char)l;
         N.lab[q] = (signed char)l;
-        bad = l != L_CROSSING;
+        bad = l != L_CROSSING && l != L_BOUNCING;
       }
     }
     slow |= bad;
@@ -1525,33 +1854,44 @@ __global__ void __launch_bounds__(64)
   O.compRoot = a.ocompRoot + a.compOff[k];
   O.ncomp = 0;
   O.capComp = (int)(a.compOff[k + 1] - a.compOff[k]);
-  int nInterP = 0;
-  for (int j = lane; j < nP; j += 64) nInterP += isint(N, j) ? 1 : 0;
-  nInterP = wave_sum(nInterP);
+  // crossings on P (the links are mutual here, so Q has as many)
+  int nCrossP = 0;
+  for (int j = lane; j < nP; j += 64) nCrossP += iscross(N, j) ? 1 : 0;
+  nCrossP = wave_sum(nCrossP);
   int VP = -1, VQ = -1;
-  if (!slow && nInterP > 0) {
+  if (!slow) {
     VP = wave_first_nonint(N, 0, nP);
     VQ = wave_first_nonint(N, nP, nQ);
     if (VP < 0 || VQ < 0) slow = true;
   }
+  double *xy = a.xyTmp + 2 * nb0;
+  int *clen = a.compLen + a.compOff[k];
+  int path = LP_SEGMENTS;
   if (!slow) {
-    if (nInterP == 0) {
-      // step 3.5: no crossings on either ring -> containment by PiP
-      if (wave_pip(N, nP, nQ, P_(N, 0))) {
-        if (nP > O.cap || O.capComp < 1) {
-          slow = true;
-        } else {
-          copy_ring_to(O, N, 0, nP, 0);
-          O.compRoot[O.ncomp++] = 0;
+    if (nCrossP == 0) {
+      // step 3.5: no crossings on either ring -> containment by PiP of
+      // each ring's first non-intersection vertex
+      path = LP_NO_CROSSING;
+      const bool pin = wave_pip(N, nP, nQ, P_(N, VP)), qin = wave_pip(N, 0, nP, P_(N, VQ));
+      if ((pin ? nP : 0) + (qin ? nQ : 0) > O.cap || (pin ? 1 : 0) + (qin ? 1 : 0) > O.capComp) {
+        slow = true;
+      } else {
+        if (pin) {
+          copy_ring_out(O, xy, rg, N, 0, nP, 0, 0);
+          if (lane == 0) {
+            O.compRoot[0] = 0;
+            clen[0] = nP;
+          }
+          O.ncomp = 1;
           O.n = nP;
         }
-      }
-      if (!slow && wave_pip(N, 0, nP, P_(N, nP))) {
-        if (O.n + nQ > O.cap || O.ncomp >= O.capComp) {
-          slow = true;
-        } else {
-          copy_ring_to(O, N, nP, nQ, O.n);
-          O.compRoot[O.ncomp++] = O.n;
+        if (qin) {
+          copy_ring_out(O, xy, rg, N, nP, nQ, O.n, O.ncomp);
+          if (lane == 0) {
+            O.compRoot[O.ncomp] = O.n;
+            clen[O.ncomp] = nQ;
+          }
+          O.ncomp++;
           O.n += nQ;
         }
       }
@@ -1560,29 +1900,21 @@ __global__ void __launch_bounds__(64)
       wave_enex(N, 0, nP, VP, wave_pip(N, nP, nQ, P_(N, VP)) ? EXIT : ENTRY);
       wave_enex(N, nP, nQ, VQ, wave_pip(N, 0, nP, P_(N, VQ)) ? EXIT : ENTRY);
       __syncthreads();
-      const int rc = segment_trace(N, O, a.xyTmp + 2 * nb0, a.compLen + a.compOff[k], rg, nP,
-                                   nQ, a.cleanup);
+      const int rc = segment_trace(N, O, xy, clen, rg, nP, nQ);
       if (rc == SEG_OVERFLOW) {
         if (lane == 0) {
           status[k] = GH_OVERFLOW;
           a.outNComp[k] = 0;
           a.outNVerts[k] = 0;
         }
-        return;
-      }
-      if (rc == SEG_DONE) {
-        if (lane == 0) {
-          a.outNComp[k] = O.ncomp;
-          a.outNVerts[k] = O.n;
-        }
-        return;
+        return LP_FAILED;
       }
       if (rc == SEG_SLOW) slow = true;
     }
   }
   if (slow) {
     // sequential path for degenerate pairs (labels are recomputed there)
-    if (lane != 0) return;
+    if (lane != 0) return LP_SEQUENTIAL;
     O.n = 0;
     O.ncomp = 0;
     int rc = label_and_trace(N, 0, nP, O);
@@ -1590,11 +1922,9 @@ __global__ void __launch_bounds__(64)
       status[k] = rc;
       a.outNComp[k] = 0;
       a.outNVerts[k] = 0;
-      return;
+      return LP_FAILED;
     }
     int nc = 0, nv = 0;
-    double *xy = a.xyTmp + 2 * nb0;
-    int *clen = a.compLen + a.compOff[k];
     for (int c = 0; c < O.ncomp; c++) {
       int r = O.compRoot[c];
       if (a.cleanup) r = out_cleanup(O, r);
@@ -1610,53 +1940,30 @@ __global__ void __launch_bounds__(64)
     }
     a.outNComp[k] = nc;
     a.outNVerts[k] = nv;
-    return;
+    return LP_SEQUENTIAL;
   }
   __syncthreads();
-  // cleanUpResult per component (one lane each), then compaction
-  for (int c = lane; c < O.ncomp; c += 64) {
-    int r = O.compRoot[c];
-    if (a.cleanup) r = out_cleanup(O, r);
-    O.compRoot[c] = r;
-    int len = 0;
-    if (r >= 0) {
-      int v = r;
-      do {
-        len++;
-        v = O.next[v];
-      } while (v != r);
-    }
-    ax[c] = len;
-  }
-  __syncthreads();
-  double *xy = a.xyTmp + 2 * nb0;
-  int *clen = a.compLen + a.compOff[k];
-  int vcarry = 0, ccarry = 0;
-  for (int base = 0; base < O.ncomp; base += 64) {
-    const int c = base + lane;
-    const int len = c < O.ncomp ? ax[c] : 0;
-    const bool keep = len > 0;
-    unsigned long long m = __ballot(keep);
-    int incl = len;
-    for (int off = 1; off < 64; off <<= 1) {
-      int u = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += u;
-    }
-    if (keep) {
-      const int voff = vcarry + incl - len;
-      clen[ccarry + lt_count(m)] = len;
-      int v = O.compRoot[c];
-      for (int i = 0; i < len; i++) {
-        st(xy, voff + i, ld(O.po, v));
-        v = O.next[v];
-      }
-    }
-    vcarry += __shfl(incl, 63, 64);
-    ccarry += __popcll(m);
-  }
+  if (a.cleanup && wave_cleanup_compact(O, xy, clen, rg, rg, nP + nQ) && path == LP_SEGMENTS)
+    path = LP_SEG_CLEANUP;
   if (lane == 0) {
-    a.outNComp[k] = ccarry;
-    a.outNVerts[k] = vcarry;
+    a.outNComp[k] = O.ncomp;
+    a.outNVerts[k] = O.n;
+  }
+  return path;
+}
+
+// One wave per pair; records the path each pair took and its duration in
+// ticks of the constant-rate wall clock (both optional).
+__global__ void __launch_bounds__(64)
+    k_label_wave(long M, int *status, SlotArrays s, TraceArgs a, int *aux, int *lpath,
+                 long long *lticks) {
+  const long k = blockIdx.x;
+  if (k >= M) return;
+  const long long t0 = wall_clock64();
+  const int path = label_wave_pair(k, status, s, a, aux);
+  if (threadIdx.x == 0 && lpath) {
+    lpath[k] = path;
+    lticks[k] = wall_clock64() - t0;
   }
 }
 
@@ -1836,14 +2143,85 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                        in.bCoords, in.oCoords, nmap, s, own);
     HIP_CHECK(hipGetLastError());
   }
-  if (edgesQ > 0) {
-    hipLaunchKernelGGL(k_sort_q<true>, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
-                       edgesQ, mapQ, (const long long *)psQ2, pm.status, s, ownP);
+  // per-edge alpha sorts, all P edges before the Q edges (the Q sort reads
+  // the back links the P sort moves)
+  {
+    int *runCnt = wsT<int>(c, "gh_runCnt", 2);
+    int *listP = wsT<int>(c, "gh_runListP", edgesP), *lenP = wsT<int>(c, "gh_runLenP", edgesP);
+    int *listQ = wsT<int>(c, "gh_runListQ", edgesQ), *lenQ = wsT<int>(c, "gh_runLenQ", edgesQ);
+    fill_i32(c, runCnt, 0, 2);
+    if (edgesP > 0)
+      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
+                         edgesP, mapP, (const long long *)psP2, pm.status, listP, lenP, runCnt);
+    if (edgesQ > 0)
+      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
+                         edgesQ, mapQ, (const long long *)psQ2, pm.status, listQ, lenQ,
+                         runCnt + 1);
     HIP_CHECK(hipGetLastError());
-    LaunchTimer t(c, "gh_sort_q");
-    hipLaunchKernelGGL(k_sort_q<false>, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
-                       edgesQ, mapQ, (const long long *)psQ2, pm.status, s, ownP);
-    HIP_CHECK(hipGetLastError());
+    int nRuns[2];
+    HIP_CHECK(hipMemcpyAsync(nRuns, runCnt, sizeof nRuns, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    // keys, segmented sort and moves of one side's long runs
+    auto long_runs = [&](bool qside, int n, const int *list, const int *len,
+                         const long long *ps2, const int *map, const long long *off) {
+      long long *runOff = wsT<long long>(c, "gh_runOff", n + 1);
+      scan_i32_to_i64(c, len, runOff, n);
+      const long total = (long)read_scalar(c, runOff + n);
+      uint64_t *k0 = wsT<uint64_t>(c, "gh_runK0", total), *k1 = wsT<uint64_t>(c, "gh_runK1", total);
+      int *v0 = wsT<int>(c, "gh_runV0", total), *v1 = wsT<int>(c, "gh_runV1", total);
+      const unsigned blocks = (unsigned)(n < 4096 ? n : 4096);
+      if (qside)
+        hipLaunchKernelGGL(k_long_run_keys<true>, dim3(blocks), dim3(256), 0, c.stream, n, list,
+                           (const long long *)runOff, ps2, (const int *)s.aQ2,
+                           (const int *)s.nbQ2, ownP, k0, v0);
+      else
+        hipLaunchKernelGGL(k_long_run_keys<false>, dim3(blocks), dim3(256), 0, c.stream, n, list,
+                           (const long long *)runOff, ps2, (const int *)s.aP2,
+                           (const int *)s.nbP2, ownP, k0, v0);
+      HIP_CHECK(hipGetLastError());
+      return [=, &c, &s, &own](void) {
+        int bits = 32;
+        for (int x = n - 1; x > 0; x >>= 1) bits++;
+        radix_sort_u64(c, k0, v0, k1, v1, total, bits);
+        if (qside)
+          hipLaunchKernelGGL(k_long_run_apply<true>, dim3(grid_for(total, 256)), dim3(256), 0,
+                             c.stream, total, (const uint64_t *)k1, (const int *)v1, list,
+                             (const long long *)runOff, ps2, map, off, s, own,
+                             (const int *)ownP);
+        else
+          hipLaunchKernelGGL(k_long_run_apply<false>, dim3(grid_for(total, 256)), dim3(256), 0,
+                             c.stream, total, (const uint64_t *)k1, (const int *)v1, list,
+                             (const long long *)runOff, ps2, map, off, s, own,
+                             (const int *)ownP);
+        HIP_CHECK(hipGetLastError());
+      };
+    };
+    if (edgesP > 0) {
+      LaunchTimer t(c, "gh_sort_p");
+      hipLaunchKernelGGL(k_sort_p_short, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
+                         edgesP, mapP, (const long long *)offP, (const long long *)psP2,
+                         pm.status, s, own);
+      HIP_CHECK(hipGetLastError());
+    }
+    if (nRuns[0] > 0)
+      long_runs(false, nRuns[0], listP, lenP, (const long long *)psP2, mapP,
+                (const long long *)offP)();
+    if (edgesQ > 0) {
+      hipLaunchKernelGGL(k_sort_q_short<true>, dim3(grid_for(edgesQ, 256)), dim3(256), 0,
+                         c.stream, edgesQ, mapQ, (const long long *)psQ2, pm.status, s, ownP);
+      HIP_CHECK(hipGetLastError());
+      auto finish = nRuns[1] > 0 ? long_runs(true, nRuns[1], listQ, lenQ,
+                                             (const long long *)psQ2, mapQ,
+                                             (const long long *)offQ)
+                                 : std::function<void()>();
+      {
+        LaunchTimer t(c, "gh_sort_q");
+        hipLaunchKernelGGL(k_sort_q_short<false>, dim3(grid_for(edgesQ, 256)), dim3(256), 0,
+                           c.stream, edgesQ, mapQ, (const long long *)psQ2, pm.status, s, ownP);
+        HIP_CHECK(hipGetLastError());
+      }
+      if (finish) finish();
+    }
   }
   if (edgesP > 0) {
     hipLaunchKernelGGL(k_init_label<true>, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
@@ -1902,9 +2280,11 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     // node capacity per pair is >= 2 (nP + nQ), so 5 ints per node give each
     // pair the 10 (nP + nQ) ints of segment scratch
     int *aux = wsT<int>(c, "gh_aux", 5 * nodes);
+    res.labelPath = wsT<int>(c, "gh_lpath", M);
+    res.labelTicks = wsT<long long>(c, "gh_lticks", M);
     LaunchTimer t(c, "gh_label_trace");
     hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), 0, c.stream, M, pm.status, s,
-                       ta, aux);
+                       ta, aux, res.labelPath, res.labelTicks);
     HIP_CHECK(hipGetLastError());
   }
   res.vertOff = wsT<long long>(c, "gh_vOff", M + 1);

@@ -198,7 +198,8 @@ J_FIELDS = {"pairs": (0, np.int32), "counts": (1, np.int32), "perm": (2, np.int3
             "seg_flag": (12, np.int32), "joined": (13, np.int32)}
 C_FIELDS = {"status": (0, np.int32), "swapped": (1, np.int32), "ncomp": (2, np.int32),
             "nverts": (3, np.int32), "comp_off": (4, np.int64), "vert_off": (5, np.int64),
-            "comp_sizes": (6, np.int32), "xy": (7, np.float64), "offP": (20, np.int64),
+            "comp_sizes": (6, np.int32), "xy": (7, np.float64), "label_path": (8, np.int32),
+            "label_ticks": (9, np.int64), "offP": (20, np.int64),
             "offQ": (21, np.int64), "psP1": (22, np.int64), "psP2": (23, np.int64),
             "psQ1": (24, np.int64), "psQ2": (25, np.int64), "boolP": (26, np.int32),
             "boolQ": (27, np.int32), "nmapQ": (28, np.int32), "iP": (29, np.float64),
@@ -206,6 +207,9 @@ C_FIELDS = {"status": (0, np.int32), "swapped": (1, np.int32), "ncomp": (2, np.i
             "aP": (33, np.int32), "aQ": (34, np.int32), "lP": (35, np.int32),
             "lQ": (36, np.int32), "nbP": (37, np.int32), "nbQ": (38, np.int32),
             "norm": (39, np.float64)}
+
+# labelling path of a pair (clip field "label_path")
+LP_SKIPPED, LP_SEGMENTS, LP_SEG_CLEANUP, LP_NO_CROSSING, LP_SEQUENTIAL, LP_FAILED = range(6)
 
 CLIP_NORMALIZE = 1
 CLIP_NO_CLEANUP = 2

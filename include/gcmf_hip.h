@@ -131,6 +131,9 @@ enum {
   GCMF_C_VERTOFF = 5,   /* i64 [M+1] first vertex of each pair */
   GCMF_C_COMPSIZES = 6, /* i32 [total_comps] */
   GCMF_C_XY = 7,        /* f64 [2*total_verts] original coordinates */
+  GCMF_C_LABEL_PATH = 8,  /* i32 [M] labelling path: 0 skipped, 1 segments, 2 segments +
+                             clean-up, 3 no crossing, 4 sequential, 5 failed */
+  GCMF_C_LABEL_TICKS = 9, /* i64 [M] labelling time per pair, device wall-clock ticks */
   /* phase-1 arrays (gppolyclip_mDIV512.cu outputs), global over the batch */
   GCMF_C_OFFP = 20,     /* i64 [M+1] first P edge of each pair */
   GCMF_C_OFFQ = 21,     /* i64 [M+1] */

@@ -115,7 +115,9 @@ def check_results(gctx, st, cases, mode):
     """Every pair equals the GPU-route oracle (the reference's GPU phase 1
     spliced and labelled as ghcuda.cpp does); pairs in general position also
     equal the CPU route of polyclip.h.  On degenerate (grid-snapped) inputs
-    the reference's two routes can differ, see DESIGN.md."""
+    the reference's two routes can differ: the GPU route's phase-1 lists
+    order coincident intersections by truncated alpha, the CPU route by
+    exact insertion."""
     status, results = gctx.clip_results(st)
     for k, (name, P0, Q0) in enumerate(cases):
         P, Q = oriented(P0, Q0)
@@ -179,6 +181,14 @@ def test_clip_matches_cpu_route(gctx, mode, seq):
     flags |= G.CLIP_SEQ_LABEL if seq else 0
     st = run_batch(gctx, cases, flags)
     check_results(gctx, st, cases, mode)
+    if not seq:
+        # pairs in general position take the parallel paths, not the
+        # sequential fallback
+        path = gctx.clip_get("label_path", len(cases))
+        for k, (name, _, _) in enumerate(cases):
+            if name.startswith("rand"):
+                assert path[k] in (G.LP_SEGMENTS, G.LP_SEG_CLEANUP, G.LP_NO_CROSSING), (name,
+                                                                                      path[k])
 
 
 def test_clip_area_oracle(gctx):
@@ -243,3 +253,56 @@ def test_clip_disjoint_and_nested(gctx):
     np.testing.assert_array_equal(res[1][0], comps[0])
     s_ref, comps, _ = O.clip_cpu(P, P.copy(), 1)
     assert [len(c) for c in res[2]] == [len(c) for c in comps]
+
+
+def long_run_cases():
+    """Pairs whose edges carry more than 32 intersections, so the per-edge
+    alpha sorts take the segmented radix-sort path: a jagged 4000-vertex star
+    against an 8-vertex star (long Q runs), and a 12-gon whose corner 0 is
+    replaced by a 700-point dent well inside the other ring, against a
+    jagged 690-vertex star (long P runs on the 12-gon's chords)."""
+    P1, Q1 = G.gen_single_pair(0, 4242, 4000, 8)
+    ang = 2 * np.pi * np.arange(12) / 12
+    gon = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    t = np.linspace(-0.5 * np.pi, 0.5 * np.pi, 700)
+    dent = np.array([0.5, 0.0]) + 0.05 * np.stack([np.cos(t), np.sin(t)], axis=1)
+    P2 = np.concatenate([gon[1:], dent])
+    Q2, _ = G.gen_single_pair(0, 4343, 690, 8)
+    return [("longQ", P1, Q1), ("longP", P2, Q2)]
+
+
+def test_clip_long_runs(gctx):
+    cases = long_run_cases()
+    for name, P, Q in cases:
+        Po, Qo = oriented(P, Q)
+        ref = O.phase1(Po, Qo, 1)
+        side = "psP2" if name == "longP" else "psQ2"
+        assert int(np.diff(ref[side]).max()) - 1 > 32, name
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    check_phase1(gctx, st, cases, 1)
+    check_results(gctx, st, cases, 1)
+    for p in gctx.clip_get("label_path", 2):
+        assert p in (G.LP_SEGMENTS, G.LP_SEG_CLEANUP)
+
+
+def test_clip_batched_rare_pairs(gctx):
+    """Pairs from the batched workload that reach rare paths (a trace that
+    does not close, BOUNCING intersections among thousands of crossings)
+    equal the GPU-route oracle, status included, and the BOUNCING pairs stay
+    on the parallel labelling path."""
+    z = np.load(GOLDEN + "/batched_pairs.npz")
+    names = ["p4596", "p156581", "p73779", "p176719"]
+    cases = [(n, z[n + "_P"], z[n + "_Q"]) for n in names]
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    status, results = gctx.clip_results(st)
+    path = gctx.clip_get("label_path", len(cases))
+    for k, (name, P0, Q0) in enumerate(cases):
+        P, Q = oriented(P0, Q0)
+        s_ref, comps, _ = O.clip_gpu_route(P, Q, 1)
+        assert status[k] == s_ref, (name, status[k], s_ref)
+        assert [len(c) for c in results[k]] == [len(c) for c in comps], name
+        for a, b in zip(results[k], comps):
+            np.testing.assert_array_equal(a, b, err_msg=name)
+    assert status[0] == -1 and path[0] == G.LP_FAILED
+    for k in (1, 2, 3):  # split and unsplit BOUNCING pairs
+        assert path[k] in (G.LP_SEGMENTS, G.LP_SEG_CLEANUP), (names[k], path[k])

@@ -1,6 +1,6 @@
 """Pins the CPU oracle of the spatial join: CoordToMBR keys (known answers
 and a Decimal closed form), the endpoint sort (stable argsort), the MBR
-candidate predicate and order (brute force over all pairs, SURVEY App. C),
+candidate predicate and order (brute force over all pairs),
 and the refine stages (numpy restatements of the cited formulas)."""
 from decimal import Decimal
 
