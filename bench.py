@@ -59,6 +59,17 @@ def cpu_baseline(base, over, joined, sample_pairs):
                                                                    t_clip)}
 
 
+def copy_calibration():
+    import torch
+    a = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel() * a.element_size()}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,7 +82,12 @@ def main():
                     help="join-clip: the two-layer join + clip (default); batched: "
                          "independent uneven pairs clipped in one pass")
     ap.add_argument("--pairs", type=int, default=200000, help="pairs for --workload batched")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one plain 256 MiB device-to-device copy and exit (the reference "
+                         "reading for the HBM byte counters FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(use --gpus 1)", file=sys.stderr)

@@ -145,6 +145,61 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ------------------------------------------------- edge-pair prefilter ----
+// The edge-pair loops walk the other ring with the next vertex loaded one
+// iteration ahead, and reject an edge first by plain comparisons of its two
+// endpoints against this edge's box: both endpoints on one side of the box
+// means lsmf (which takes the boxes' min / max) fails as well, for finite
+// coordinates and NaN alike, so only pairs lsmf could accept go on to it.
+struct EdgeBox {
+  double x0, y0, x1, y1;
+};
+__device__ __forceinline__ EdgeBox edge_box(pt a, pt b) {
+  return EdgeBox{fmin(a.x, b.x), fmin(a.y, b.y), fmax(a.x, b.x), fmax(a.y, b.y)};
+}
+__device__ __forceinline__ bool box_misses(const EdgeBox &e, pt b1, pt b2) {
+  // bitwise, so that the eight compares combine without branches
+  return ((b1.x < e.x0) & (b2.x < e.x0)) | ((b1.x > e.x1) & (b2.x > e.x1)) |
+         ((b1.y < e.y0) & (b2.y < e.y0)) | ((b1.y > e.y1) & (b2.y > e.y1));
+}
+// f(q, B_q, B_q+1) for the n edges of ring V in order, vertex q + 2 loaded
+// while edge q is tested
+template <class F>
+__device__ __forceinline__ void ring_edges(const double *V, int n, F &&f) {
+  if (n <= 0) return;
+  pt B1 = ld(V, 0), B2 = ld(V, n == 1 ? 0 : 1);
+  int q = 0;
+  for (; q + 2 < n; q++) {
+    const pt B3 = ld(V, q + 2);
+    f(q, B1, B2);
+    B1 = B2;
+    B2 = B3;
+  }
+  for (; q < n; q++) {
+    const pt B3 = ld(V, (q + 2) % n);
+    f(q, B1, B2);
+    B1 = B2;
+    B2 = B3;
+  }
+}
+
+// The count pass records, for every edge, the first kPartners edges of the
+// other ring with a nonzero intersection type, in index order; the neighbour
+// map and intersection passes act only on those edges, so they walk that
+// list instead of the whole ring when the edge has no more than kPartners
+// of them (its count1).
+constexpr int kPartners = 4;
+
+// f(q, B_q, B_q+1) for the listed edges lst[0, c) of ring V (n vertices)
+template <class F>
+__device__ __forceinline__ void listed_edges(const double *V, int n, const int *lst, int c,
+                                             F &&f) {
+  for (int i = 0; i < c; i++) {
+    const int q = lst[i];
+    f(q, ld(V, q), ld(V, q + 1 == n ? 0 : q + 1));
+  }
+}
+
 // ------------------------------------------------------------- count ----
 // gpuCMBRFilter (:382-415) + gpuCountIntersections (:463-548), both views in
 // one launch: threads [0, edgesP) count P edges against Q, the rest count Q
@@ -153,7 +208,7 @@ __global__ void __launch_bounds__(256)
     k_count(long edgesP, long edgesQ, const int *mapP, const int *mapQ, const long long *offP,
             const long long *offQ, const double *ringP, const double *ringQ,
             const double *cmbrN, int *cnt1P, int *cnt2P, int *cnt1Q, int *cnt2Q, int *boolP,
-            int *boolQ, unsigned long long *nEval) {
+            int *boolQ, unsigned long long *nEval, int *lstP, int *lstQ) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= edgesP + edgesQ) return;
   const bool sideP = g < edgesP;
@@ -175,18 +230,19 @@ __global__ void __launch_bounds__(256)
   int c1 = 0, c2 = 0;
   unsigned long long ev = 0;
   if (flag) {
-    pt B1 = ld(Bv, 0);
-    for (long long q = 0; q < bN; q++) {
-      pt B2 = ld(Bv, q == bN - 1 ? 0 : q + 1);
-      if (lsmf(A1, A2, B1, B2)) {
-        double al, be;
-        ev++;
-        int t = itype(A1, A2, B1, B2, al, be);
-        c1 += t != 0;
-        c2 += adds_vertex(t);
+    const EdgeBox ab = edge_box(A1, A2);
+    int *lst = (sideP ? lstP : lstQ) + (long long)ga * kPartners;
+    ring_edges(Bv, (int)bN, [&](int q, pt B1, pt B2) {
+      if (box_misses(ab, B1, B2) || !lsmf(A1, A2, B1, B2)) return;
+      double al, be;
+      ev++;
+      const int t = itype(A1, A2, B1, B2, al, be);
+      if (t != 0) {
+        if (c1 < kPartners) lst[c1] = q;
+        c1++;
       }
-      B1 = B2;
-    }
+      c2 += adds_vertex(t);
+    });
   }
   if (sideP) {
     cnt1P[ga] = c1;
@@ -222,7 +278,8 @@ __global__ void k_pair_check(long M, const long long *offP, const long long *off
 __global__ void __launch_bounds__(256)
     k_nmap(long edgesQ, const int *mapQ, const long long *offP, const long long *offQ,
            const double *ringP, const double *ringQ, const long long *psP1,
-           const long long *psQ1, const long long *psQ2, const int *status, int *nmap) {
+           const long long *psQ1, const long long *psQ2, const int *status, int *nmap,
+           const int *lstQ) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= edgesQ) return;
   const int k = mapQ[g];
@@ -234,19 +291,22 @@ __global__ void __launch_bounds__(256)
   const pt A1 = ld(ringQ, g), A2 = ld(ringQ, i == qN - 1 ? qBase : g + 1);
   const long long slot = psQ2[g];
   int c2 = 0, nd = 0;
-  for (long long p = 0; p < pN; p++) {
+  const EdgeBox ab = edge_box(A1, A2);
+  auto visit = [&](int p, pt b1, pt b2) {
     const long long pg = pBase + p;
-    if (psP1[pg + 1] == psP1[pg]) continue;
-    pt B1 = ld(ringP, pg), B2 = ld(ringP, p == pN - 1 ? pBase : pg + 1);
-    if (!lsmf(A1, A2, B1, B2)) continue;
+    if (box_misses(ab, b1, b2) || psP1[pg + 1] == psP1[pg]) return;
+    if (!lsmf(A1, A2, b1, b2)) return;
     double al, be;
-    int t = itype(A1, A2, B1, B2, al, be);
+    const int t = itype(A1, A2, b1, b2, al, be);
     if (t) {
       if (adds_vertex(t)) c2 = ++nd;
       else c2 = 0;
-      nmap[slot + c2] = (int)p;
+      nmap[slot + c2] = p;
     }
-  }
+  };
+  const int c1 = (int)(psQ1[g + 1] - psQ1[g]);
+  if (c1 <= kPartners) listed_edges(ringP + 2 * pBase, (int)pN, lstQ + g * kPartners, c1, visit);
+  else ring_edges(ringP + 2 * pBase, (int)pN, visit);
 }
 
 // ----------------------------------------------------- intersections ----
@@ -330,7 +390,7 @@ __global__ void __launch_bounds__(256)
            const double *ringP, const double *ringQ, const long long *psP1,
            const long long *psP2, const long long *psQ1, const long long *psQ2,
            const int *status, PairMeta pm, const double *bCoords, const double *oCoords,
-           const int *nmap, SlotArrays s, Owners own) {
+           const int *nmap, SlotArrays s, Owners own, const int *lstP) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= edgesP) return;
   const int k = mapP[g];
@@ -383,14 +443,14 @@ __global__ void __launch_bounds__(256)
   };
   int c2 = 0, nd = 0;
   long long nQId = -1;
-  for (long long q = 0; q < qN; q++) {
+  const EdgeBox ab = edge_box(P1, P2);
+  auto visit = [&](int q, pt Q1, pt Q2) {
     const long long qg = qBase + q;
-    if (psQ1[qg + 1] == psQ1[qg]) continue;
-    const pt Q1 = ld(ringQ, qg), Q2 = ld(ringQ, q == qN - 1 ? qBase : qg + 1);
-    if (!lsmf(P1, P2, Q1, Q2)) continue;
+    if (box_misses(ab, Q1, Q2) || psQ1[qg + 1] == psQ1[qg]) return;
+    if (!lsmf(P1, P2, Q1, Q2)) return;
     double alpha, beta;
     const int t = itype(P1, P2, Q1, Q2, alpha, beta);
-    if (!t) continue;
+    if (!t) return;
     if (adds_vertex(t)) c2 = ++nd;
     else c2 = 0;
     const long long start = psQ2[qg], end = psQ2[qg + 1];
@@ -456,7 +516,10 @@ __global__ void __launch_bounds__(256)
     } else {
       setA(start, kb);
     }
-  }
+  };
+  const int c1 = (int)(psP1[g + 1] - psP1[g]);
+  if (c1 <= kPartners) listed_edges(ringQ + 2 * qBase, (int)qN, lstP + g * kPartners, c1, visit);
+  else ring_edges(ringQ + 2 * qBase, (int)qN, visit);
 }
 
 // ------------------------------------------------ per-edge alpha sorts ----
@@ -2047,6 +2110,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   int *cnt1P = wsT<int>(c, "gh_cnt1P", edgesP), *cnt2P = wsT<int>(c, "gh_cnt2P", edgesP);
   int *cnt1Q = wsT<int>(c, "gh_cnt1Q", edgesQ), *cnt2Q = wsT<int>(c, "gh_cnt2Q", edgesQ);
   int *boolP = wsT<int>(c, "gh_boolP", edgesP), *boolQ = wsT<int>(c, "gh_boolQ", edgesQ);
+  int *lstP = wsT<int>(c, "gh_lstP", (long)kPartners * edgesP);
+  int *lstQ = wsT<int>(c, "gh_lstQ", (long)kPartners * edgesQ);
   unsigned long long *stats = wsT<unsigned long long>(c, "gh_stats", 2);
   hipLaunchKernelGGL(k_fill_u64_pair, dim3(1), dim3(64), 0, c.stream, stats);
   if (edgesP + edgesQ > 0) {
@@ -2054,7 +2119,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     hipLaunchKernelGGL(k_count, dim3(grid_for(edgesP + edgesQ, 256)), dim3(256), 0, c.stream,
                        edgesP, edgesQ, mapP, mapQ, (const long long *)offP,
                        (const long long *)offQ, ringP, ringQ, pm.cmbrN, cnt1P, cnt2P, cnt1Q,
-                       cnt2Q, boolP, boolQ, stats);
+                       cnt2Q, boolP, boolQ, stats, lstP, lstQ);
     HIP_CHECK(hipGetLastError());
   }
   long long *psP1 = wsT<long long>(c, "gh_psP1", edgesP + 1);
@@ -2106,7 +2171,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       hipLaunchKernelGGL(k_nmap, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream, edgesQ,
                          mapQ, (const long long *)offP, (const long long *)offQ, ringP, ringQ,
                          (const long long *)psP1, (const long long *)psQ1,
-                         (const long long *)psQ2, pm.status, nmap);
+                         (const long long *)psQ2, pm.status, nmap, lstQ);
       HIP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_q_parents, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream, edgesQ,
@@ -2132,7 +2197,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          edgesP, mapP, (const long long *)offP, (const long long *)offQ, ringP,
                          ringQ, (const long long *)psP1, (const long long *)psP2,
                          (const long long *)psQ1, (const long long *)psQ2, pm.status, pm,
-                         in.bCoords, in.oCoords, nmap, s, own);
+                         in.bCoords, in.oCoords, nmap, s, own, lstP);
       HIP_CHECK(hipGetLastError());
     }
     LaunchTimer t(c, "gh_calc");
@@ -2140,7 +2205,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                        edgesP, mapP, (const long long *)offP, (const long long *)offQ, ringP,
                        ringQ, (const long long *)psP1, (const long long *)psP2,
                        (const long long *)psQ1, (const long long *)psQ2, pm.status, pm,
-                       in.bCoords, in.oCoords, nmap, s, own);
+                       in.bCoords, in.oCoords, nmap, s, own, lstP);
     HIP_CHECK(hipGetLastError());
   }
   // per-edge alpha sorts, all P edges before the Q edges (the Q sort reads
