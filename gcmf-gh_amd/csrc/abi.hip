@@ -92,18 +92,10 @@ void fill_clip_stats(gcmf_ctx *ctx, gcmf_clip_stats *st) {
   st->slots_q = ctx->p1.slotsQ;
   st->n_eval = ctx->cr.nEvalP;
   st->n_intersections = ctx->cr.totalIntersections;
-  if (M > 0) {
-    std::string buf;
-    buf.resize(sizeof(int) * (size_t)M);
-    HIP_CHECK(hipMemcpy(&buf[0], ctx->cr.status, sizeof(int) * (size_t)M, hipMemcpyDeviceToHost));
-    const int *s = (const int *)buf.data();
-    for (long k = 0; k < M; k++) {
-      if (s[k] == gcmf::GH_OK) st->ok++;
-      else if (s[k] == gcmf::GH_INVALID) st->invalid++;
-      else if (s[k] == gcmf::GH_DISJOINT) st->disjoint++;
-      else st->failed++;
-    }
-  }
+  st->ok = ctx->cr.nOk;
+  st->invalid = ctx->cr.nInvalid;
+  st->disjoint = ctx->cr.nDisjoint;
+  st->failed = ctx->cr.nFailed;
 }
 
 void run_clip(gcmf_ctx *ctx, const gcmf::JoinInputs &in, const int *d_pairs, long M, int flags,
@@ -165,6 +157,7 @@ void gcmf_shutdown(gcmf_ctx *ctx) {
     (void)hipEventDestroy(p.second.second);
   }
   for (auto e : ctx->c.event_pool) (void)hipEventDestroy(e);
+  if (ctx->c.hbuf) (void)hipHostFree(ctx->c.hbuf);
   (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
 }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <initializer_list>
 #include <map>
 #include <string>
 #include <vector>
@@ -61,6 +62,7 @@ struct Context {
   std::vector<hipEvent_t> event_pool;
   std::string last_error;
   int cu_count = 256;
+  long long *hbuf = nullptr;  // pinned host staging for small read-backs (kHostSlots)
 
   hipEvent_t take_event();
   void flush_timers();  // synchronises on pending events and accumulates
@@ -81,6 +83,10 @@ void scan_i32_to_i64(Context &c, const int *in, long long *out, long n);
 void scan_i32_to_i32(Context &c, const int *in, int *out, long n);
 // out[k] = i for the k-th i with flag[i] != 0; returns the count (synchronises).
 long compact_flags(Context &c, const int *flag, long n, int *out_idx, const char *tag);
+// the same without the read-back: returns the device address of the count,
+// valid until the next call with the same tag
+const long long *compact_flags_dev(Context &c, const int *flag, long n, int *out_idx,
+                                   const char *tag);
 // Stable LSD radix sort of keys (u64) carrying int32 values; bits = number of
 // significant low bits of every key.  Results land in keys_out / vals_out.
 void radix_sort_u64(Context &c, const uint64_t *keys_in, const int *vals_in, uint64_t *keys_out,
@@ -95,6 +101,11 @@ T read_scalar(Context &c, const T *dptr) {
   HIP_CHECK(hipStreamSynchronize(c.stream));
   return v;
 }
+
+// copies up to kHostSlots int64 values back to the host in one transfer
+// (synchronises); null sources read as 0
+constexpr int kHostSlots = 16;
+void read_scalars(Context &c, std::initializer_list<const long long *> src, long long *out);
 
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }
 

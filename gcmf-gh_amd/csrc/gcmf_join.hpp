@@ -81,6 +81,7 @@ struct ClipResult {
   long long *labelTicks = nullptr;
   // phase-1 statistics
   long long nEvalP = 0, totalIntersections = 0;
+  long long nOk = 0, nInvalid = 0, nDisjoint = 0, nFailed = 0;  // pairs by status
 };
 
 // Per-pair phase-1 arrays for one clip launch (device, workspace-owned); slot

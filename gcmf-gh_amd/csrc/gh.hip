@@ -557,17 +557,68 @@ __device__ __forceinline__ void move_q(SlotArrays &s, const int *ownP, long long
   if (nb > 0 && ownP[nb - 1] == (int)g) s.nbP[nb - 1] = (int)(d + 1);
 }
 
-// edges whose run is longer than kSmallSort, with their run lengths
+// Runs of kSmallSort < m <= kBlockSort slots are ranked by one block each
+// from a copy of their keys in LDS; longer ones go through the radix sort.
+constexpr int kBlockSort = 2048;
+
+// edges whose run is longer than kSmallSort: the medium ones (up to
+// kBlockSort) in list / runLen with cnt[0] their number, the longer ones in
+// listL / lenL with cnt[2] their number and cnt[3] their total length
 __global__ void k_list_long_runs(long edges, const int *map, const long long *ps2,
-                                 const int *status, int *list, int *runLen, int *count) {
+                                 const int *status, int *list, int *listL, int *lenL,
+                                 unsigned long long *cnt) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= edges) return;
   if (status[map[g]] != GH_OK) return;
   const long long m = ps2[g + 1] - ps2[g] - 1;
-  if (m > kSmallSort) {
-    const int i = atomicAdd(count, 1);
+  if (m > kBlockSort) {
+    const int i = (int)atomicAdd(cnt + 2, 1ULL);
+    listL[i] = (int)g;
+    lenL[i] = (int)m;
+    atomicAdd(cnt + 3, (unsigned long long)m);
+  } else if (m > kSmallSort) {
+    const int i = (int)atomicAdd(cnt, 1ULL);
     list[i] = (int)g;
-    runLen[i] = (int)m;
+  }
+}
+
+// OWN pass of the Q sort over listed runs, one block per run
+__global__ void k_run_own(int nRuns, const int *list, const long long *psQ2, const int *nbQ2,
+                          int *ownP) {
+  for (int r = blockIdx.x; r < nRuns; r += gridDim.x) {
+    const long g = list[r];
+    const long long lo = psQ2[g] + 1, hi = psQ2[g + 1];
+    for (long long j = lo + threadIdx.x; j < hi; j += blockDim.x) {
+      const int nb = nbQ2[j];
+      if (nb > 0) atomicMax(&ownP[nb - 1], (int)g);
+    }
+  }
+}
+
+// stable ranks of a medium run from its keys in LDS, then the moves
+template <bool QSIDE>
+__global__ void __launch_bounds__(256)
+    k_run_sort_block(int nRuns, const int *list, const long long *ps2, const int *map,
+                     const long long *off, SlotArrays s, Owners own, const int *ownP) {
+  __shared__ int key[kBlockSort];
+  const int *a2 = QSIDE ? s.aQ2 : s.aP2;
+  for (int r = blockIdx.x; r < nRuns; r += gridDim.x) {
+    const long g = list[r];
+    const long long lo = ps2[g] + 1;
+    const int m = (int)(ps2[g + 1] - lo);
+    __syncthreads();
+    for (int t = threadIdx.x; t < m; t += blockDim.x) key[t] = a2[lo + t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+      const int kt = key[t];
+      int d = 0;
+      for (int i = 0; i < m; i++) {
+        const int ki = key[i];
+        d += (ki < kt) || (ki == kt && i < t);
+      }
+      if (QSIDE) move_q(s, ownP, lo + t, lo + d, g);
+      else move_p(s, own, lo + t, lo + d, (int)(g - off[map[g]]));
+    }
   }
 }
 
@@ -2030,6 +2081,20 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// pairs by final status: OK, invalid, disjoint, failed
+__global__ void k_status_counts(long M, const int *status, unsigned long long *cnt) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int which = -1;
+  if (k < M) {
+    const int s = status[k];
+    which = s == GH_OK ? 0 : s == GH_INVALID ? 1 : s == GH_DISJOINT ? 2 : 3;
+  }
+  for (int w = 0; w < 4; w++) {
+    const unsigned long long m = __ballot(which == w);
+    if (__lane_id() == 0 && m) atomicAdd(cnt + w, (unsigned long long)__popcll(m));
+  }
+}
+
 // node / component capacity per pair
 __global__ void k_capacity(long M, const long long *offP, const long long *offQ,
                            const long long *psP2, const long long *psQ2, const int *status,
@@ -2095,8 +2160,9 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   }
   scan_i32_to_i64(c, eP, offP, M);
   scan_i32_to_i64(c, eQ, offQ, M);
-  const long edgesP = (long)read_scalar(c, offP + M);
-  const long edgesQ = (long)read_scalar(c, offQ + M);
+  long long rb[kHostSlots];
+  read_scalars(c, {offP + M, offQ + M}, rb);
+  const long edgesP = (long)rb[0], edgesQ = (long)rb[1];
   int *mapP = wsT<int>(c, "gh_mapP", edgesP), *mapQ = wsT<int>(c, "gh_mapQ", edgesQ);
   double *ringP = wsT<double>(c, "gh_ringP", 2 * edgesP);
   double *ringQ = wsT<double>(c, "gh_ringQ", 2 * edgesQ);
@@ -2135,8 +2201,30 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                      (const long long *)psQ1, (const long long *)psP2, (const long long *)psQ2,
                      pm.status, stats + 1);
   HIP_CHECK(hipGetLastError());
-  const long slotsP = (long)read_scalar(c, psP2 + edgesP);
-  const long slotsQ = (long)read_scalar(c, psQ2 + edgesQ);
+  // edges whose alpha runs go through the segmented sort (statuses are final
+  // here), read back with the slot totals
+  unsigned long long *runCnt = wsT<unsigned long long>(c, "gh_runCnt", 8);
+  int *listMP = wsT<int>(c, "gh_runListMP", edgesP), *listMQ = wsT<int>(c, "gh_runListMQ", edgesQ);
+  int *listP = wsT<int>(c, "gh_runListP", edgesP), *lenP = wsT<int>(c, "gh_runLenP", edgesP);
+  int *listQ = wsT<int>(c, "gh_runListQ", edgesQ), *lenQ = wsT<int>(c, "gh_runLenQ", edgesQ);
+  fill_i32(c, (int *)runCnt, 0, 16);
+  if (edgesP > 0)
+    hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
+                       edgesP, mapP, (const long long *)psP2, pm.status, listMP, listP, lenP,
+                       runCnt);
+  if (edgesQ > 0)
+    hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
+                       edgesQ, mapQ, (const long long *)psQ2, pm.status, listMQ, listQ, lenQ,
+                       runCnt + 4);
+  HIP_CHECK(hipGetLastError());
+  {
+    const long long *rc = (const long long *)runCnt;
+    read_scalars(c, {psP2 + edgesP, psQ2 + edgesQ, rc, rc + 2, rc + 3, rc + 4, rc + 6, rc + 7},
+                 rb);
+  }
+  const long slotsP = (long)rb[0], slotsQ = (long)rb[1];
+  const int nMedP = (int)rb[2], nRunsP = (int)rb[3], nMedQ = (int)rb[5], nRunsQ = (int)rb[6];
+  const long runTotP = (long)rb[4], runTotQ = (long)rb[7];
   if (slotsP >= 0x7fffffffL || slotsQ >= 0x7fffffffL)
     throw Error{GCMF_ERR_OVERFLOW, "clip batch exceeds 2^31 intersection slots"};
   SlotArrays s;
@@ -2211,38 +2299,18 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   // per-edge alpha sorts, all P edges before the Q edges (the Q sort reads
   // the back links the P sort moves)
   {
-    int *runCnt = wsT<int>(c, "gh_runCnt", 2);
-    int *listP = wsT<int>(c, "gh_runListP", edgesP), *lenP = wsT<int>(c, "gh_runLenP", edgesP);
-    int *listQ = wsT<int>(c, "gh_runListQ", edgesQ), *lenQ = wsT<int>(c, "gh_runLenQ", edgesQ);
-    fill_i32(c, runCnt, 0, 2);
-    if (edgesP > 0)
-      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
-                         edgesP, mapP, (const long long *)psP2, pm.status, listP, lenP, runCnt);
-    if (edgesQ > 0)
-      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
-                         edgesQ, mapQ, (const long long *)psQ2, pm.status, listQ, lenQ,
-                         runCnt + 1);
-    HIP_CHECK(hipGetLastError());
-    int nRuns[2];
-    HIP_CHECK(hipMemcpyAsync(nRuns, runCnt, sizeof nRuns, hipMemcpyDeviceToHost, c.stream));
-    HIP_CHECK(hipStreamSynchronize(c.stream));
     // keys, segmented sort and moves of one side's long runs
-    auto long_runs = [&](bool qside, int n, const int *list, const int *len,
+    auto long_runs = [&](bool qside, int n, long total, const int *list, const int *len,
                          const long long *ps2, const int *map, const long long *off) {
-      long long *runOff = wsT<long long>(c, "gh_runOff", n + 1);
+      long long *runOff = wsT<long long>(c, qside ? "gh_runOffQ" : "gh_runOffP", n + 1);
       scan_i32_to_i64(c, len, runOff, n);
-      const long total = (long)read_scalar(c, runOff + n);
       uint64_t *k0 = wsT<uint64_t>(c, "gh_runK0", total), *k1 = wsT<uint64_t>(c, "gh_runK1", total);
       int *v0 = wsT<int>(c, "gh_runV0", total), *v1 = wsT<int>(c, "gh_runV1", total);
       const unsigned blocks = (unsigned)(n < 4096 ? n : 4096);
-      if (qside)
-        hipLaunchKernelGGL(k_long_run_keys<true>, dim3(blocks), dim3(256), 0, c.stream, n, list,
-                           (const long long *)runOff, ps2, (const int *)s.aQ2,
-                           (const int *)s.nbQ2, ownP, k0, v0);
-      else
-        hipLaunchKernelGGL(k_long_run_keys<false>, dim3(blocks), dim3(256), 0, c.stream, n, list,
-                           (const long long *)runOff, ps2, (const int *)s.aP2,
-                           (const int *)s.nbP2, ownP, k0, v0);
+      hipLaunchKernelGGL(k_long_run_keys<false>, dim3(blocks), dim3(256), 0, c.stream, n, list,
+                         (const long long *)runOff, ps2,
+                         (const int *)(qside ? s.aQ2 : s.aP2),
+                         (const int *)(qside ? s.nbQ2 : s.nbP2), ownP, k0, v0);
       HIP_CHECK(hipGetLastError());
       return [=, &c, &s, &own](void) {
         int bits = 32;
@@ -2268,21 +2336,43 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          pm.status, s, own);
       HIP_CHECK(hipGetLastError());
     }
-    if (nRuns[0] > 0)
-      long_runs(false, nRuns[0], listP, lenP, (const long long *)psP2, mapP,
+    if (nMedP > 0) {
+      LaunchTimer t(c, "gh_sort_runs");
+      hipLaunchKernelGGL(k_run_sort_block<false>, dim3(nMedP < 8192 ? nMedP : 8192), dim3(256),
+                         0, c.stream, nMedP, (const int *)listMP, (const long long *)psP2, mapP,
+                         (const long long *)offP, s, own, (const int *)ownP);
+      HIP_CHECK(hipGetLastError());
+    }
+    if (nRunsP > 0)
+      long_runs(false, nRunsP, runTotP, listP, lenP, (const long long *)psP2, mapP,
                 (const long long *)offP)();
     if (edgesQ > 0) {
       hipLaunchKernelGGL(k_sort_q_short<true>, dim3(grid_for(edgesQ, 256)), dim3(256), 0,
                          c.stream, edgesQ, mapQ, (const long long *)psQ2, pm.status, s, ownP);
+      if (nMedQ > 0)
+        hipLaunchKernelGGL(k_run_own, dim3(nMedQ < 8192 ? nMedQ : 8192), dim3(256), 0, c.stream,
+                           nMedQ, (const int *)listMQ, (const long long *)psQ2,
+                           (const int *)s.nbQ2, ownP);
+      if (nRunsQ > 0)
+        hipLaunchKernelGGL(k_run_own, dim3(nRunsQ < 8192 ? nRunsQ : 8192), dim3(256), 0,
+                           c.stream, nRunsQ, (const int *)listQ, (const long long *)psQ2,
+                           (const int *)s.nbQ2, ownP);
       HIP_CHECK(hipGetLastError());
-      auto finish = nRuns[1] > 0 ? long_runs(true, nRuns[1], listQ, lenQ,
-                                             (const long long *)psQ2, mapQ,
-                                             (const long long *)offQ)
-                                 : std::function<void()>();
+      auto finish = nRunsQ > 0 ? long_runs(true, nRunsQ, runTotQ, listQ, lenQ,
+                                           (const long long *)psQ2, mapQ,
+                                           (const long long *)offQ)
+                               : std::function<void()>();
       {
         LaunchTimer t(c, "gh_sort_q");
         hipLaunchKernelGGL(k_sort_q_short<false>, dim3(grid_for(edgesQ, 256)), dim3(256), 0,
                            c.stream, edgesQ, mapQ, (const long long *)psQ2, pm.status, s, ownP);
+        HIP_CHECK(hipGetLastError());
+      }
+      if (nMedQ > 0) {
+        LaunchTimer t(c, "gh_sort_runs");
+        hipLaunchKernelGGL(k_run_sort_block<true>, dim3(nMedQ < 8192 ? nMedQ : 8192), dim3(256),
+                           0, c.stream, nMedQ, (const int *)listMQ, (const long long *)psQ2,
+                           mapQ, (const long long *)offQ, s, own, (const int *)ownP);
         HIP_CHECK(hipGetLastError());
       }
       if (finish) finish();
@@ -2309,8 +2399,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   HIP_CHECK(hipGetLastError());
   scan_i32_to_i64(c, nodeCap, nodeOff, M);
   scan_i32_to_i64(c, compCap, compCapOff, M);
-  const long nodes = (long)read_scalar(c, nodeOff + M);
-  const long comps = (long)read_scalar(c, compCapOff + M);
+  // bounds of the capacity sums (k_capacity gives an OK pair 2 (cP + cQ) + 4
+  // nodes and cP + cQ + 6 components), so no read-back is needed
+  const long nodes = 2 * (slotsP + slotsQ) + 4 * M;
+  const long comps = slotsP + slotsQ + 6 * M;
   TraceArgs ta;
   ta.offP = offP;
   ta.offQ = offQ;
@@ -2356,10 +2448,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   res.compOff = wsT<long long>(c, "gh_cOff", M + 1);
   scan_i32_to_i64(c, ta.outNVerts, res.vertOff, M);
   scan_i32_to_i64(c, ta.outNComp, res.compOff, M);
-  res.totalVerts = (long)read_scalar(c, res.vertOff + M);
-  res.totalComps = (long)read_scalar(c, res.compOff + M);
-  res.xy = wsT<double>(c, "gh_out_xy", 2 * res.totalVerts);
-  res.compSizes = wsT<int>(c, "gh_out_comp", res.totalComps);
+  // a pair's result fits its node / component region, so the region totals
+  // bound the outputs
+  res.xy = wsT<double>(c, "gh_out_xy", 2 * nodes);
+  res.compSizes = wsT<int>(c, "gh_out_comp", comps);
   {
     LaunchTimer t(c, "gh_gather_out");
     hipLaunchKernelGGL(k_gather_out, dim3(grid_for(M, 4)), dim3(256), 0, c.stream, M,
@@ -2372,11 +2464,24 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   res.swapped = pm.swapped;
   res.ncomp = ta.outNComp;
   res.nverts = ta.outNVerts;
-  unsigned long long hs[2];
-  HIP_CHECK(hipMemcpyAsync(hs, stats, 16, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
-  res.nEvalP = (long long)hs[0];
-  res.totalIntersections = (long long)hs[1];
+  unsigned long long *sc = wsT<unsigned long long>(c, "gh_status_counts", 4);
+  fill_i32(c, (int *)sc, 0, 8);
+  hipLaunchKernelGGL(k_status_counts, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
+                     (const int *)pm.status, sc);
+  HIP_CHECK(hipGetLastError());
+  {
+    const long long *st = (const long long *)stats, *sl = (const long long *)sc;
+    read_scalars(c, {st, st + 1, res.vertOff + M, res.compOff + M, sl, sl + 1, sl + 2, sl + 3},
+                 rb);
+  }
+  res.nEvalP = rb[0];
+  res.totalIntersections = rb[1];
+  res.totalVerts = (long)rb[2];
+  res.totalComps = (long)rb[3];
+  res.nOk = rb[4];
+  res.nInvalid = rb[5];
+  res.nDisjoint = rb[6];
+  res.nFailed = rb[7];
   if (p1) {
     p1->M = M;
     p1->edgesP = edgesP;

@@ -212,9 +212,9 @@ __global__ void k_get_cmbr(long K, const int *pairs, const double *bMBR2, const 
 __global__ void __launch_bounds__(256)
     k_pip(long nCand, const int *cand, const int *pairs, const signed char *pipType,
           const double *bCoords, const long *bVPSNum, const double *oCoords, const long *oVPSNum,
-          int *pipFlag, int *joinFlag) {
+          int *pipFlag, int *joinFlag, const long long *nDev) {
   const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (w >= nCand) return;
+  if (w >= (nDev ? (long)*nDev : nCand)) return;
   const int lane = __lane_id();
   const int k = cand[w];
   const int b = pairs[2 * k], o = pairs[2 * k + 1];
@@ -279,9 +279,9 @@ __global__ void __launch_bounds__(256)
           const double *bCoords, const int *bVNum, const long *bVPSNum, const double *oCoords,
           const int *oVNum, const long *oVPSNum, int *joinFlag, int *bCount, int *oCount,
           const long long *bOff, const long long *oOff, EdgeT *bEdge, EdgeT *oEdge,
-          bool inclusive) {
+          bool inclusive, const long long *nDev) {
   const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (w >= nList) return;
+  if (w >= (nDev ? (long)*nDev : nList)) return;
   const int lane = __lane_id();
   const int k = list[w];
   const int b = pairs[2 * k], o = pairs[2 * k + 1];
@@ -353,9 +353,9 @@ __global__ void __launch_bounds__(256)
               const long *bVPSNum, const double *oCoords, const long *oVPSNum, const int *bCount,
               const int *oCount, const long long *bOff, const long long *oOff,
               const EdgeT *bEdge, const EdgeT *oEdge, int *segFlag, bool inclusive,
-              bool flagByList) {
+              bool flagByList, const long long *nDev) {
   const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (w >= nList) return;
+  if (w >= (nDev ? (long)*nDev : nList)) return;
   const int lane = __lane_id();
   const int k = list[w];
   const int b = pairs[2 * k], o = pairs[2 * k + 1];
@@ -437,62 +437,69 @@ void join_refine(Context &c, const JoinInputs &in, long K, const int *pairs, Joi
                        in.bMBR2, in.oMBR2, r.cmbr, r.pipFlag, r.pipType, r.joinFlag);
     HIP_CHECK(hipGetLastError());
   }
-  r.pipNum = compact_flags(c, r.pipFlag, K, r.pipCand, "pip");
-  if (r.pipNum > 0) {
+  // list sizes stay on the device (grids are sized by K); they come back in
+  // one read at the end
+  const long long *pipN = compact_flags_dev(c, r.pipFlag, K, r.pipCand, "pip");
+  {
     LaunchTimer t(c, "pip");
-    hipLaunchKernelGGL(k_pip, dim3(grid_for(r.pipNum, 4)), dim3(256), 0, c.stream, r.pipNum,
-                       r.pipCand, pairs, r.pipType, in.bCoords, in.bVPSNum, in.oCoords,
-                       in.oVPSNum, r.pipFlag, r.joinFlag);
+    hipLaunchKernelGGL(k_pip, dim3(grid_for(K, 4)), dim3(256), 0, c.stream, K, r.pipCand, pairs,
+                       r.pipType, in.bCoords, in.bVPSNum, in.oCoords, in.oVPSNum, r.pipFlag,
+                       r.joinFlag, pipN);
     HIP_CHECK(hipGetLastError());
   }
   // within count = number of pipFlag==1 after the test
   fill_i32(c, r.bCount, 0, K);
   fill_i32(c, r.oCount, 0, K);
   fill_i32(c, r.segFlag, 0, K);
-  r.within = compact_flags(c, r.pipFlag, K, r.list, "within");
-  r.cmfCand = compact_flags(c, r.joinFlag, K, r.list, "cmf");
-  if (r.cmfCand > 0) {
+  const long long *withinN = compact_flags_dev(c, r.pipFlag, K, r.list, "within");
+  const long long *cmfN = compact_flags_dev(c, r.joinFlag, K, r.list, "cmf");
+  {
     LaunchTimer t(c, "cmf_count");
-    hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(r.cmfCand, 4)), dim3(256), 0, c.stream,
-                       r.cmfCand, r.list, pairs, r.cmbr, in.bCoords, in.bVNum, in.bVPSNum,
-                       in.oCoords, in.oVNum, in.oVPSNum, r.joinFlag, r.bCount, r.oCount,
-                       (const long long *)nullptr, (const long long *)nullptr, (int *)nullptr,
-                       (int *)nullptr, false);
+    hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(K, 4)), dim3(256), 0, c.stream, K, r.list,
+                       pairs, r.cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum,
+                       in.oVPSNum, r.joinFlag, r.bCount, r.oCount, (const long long *)nullptr,
+                       (const long long *)nullptr, (int *)nullptr, (int *)nullptr, false, cmfN);
     HIP_CHECK(hipGetLastError());
   }
   scan_i32_to_i64(c, r.bCount, r.bOff, K);
   scan_i32_to_i64(c, r.oCount, r.oOff, K);
-  r.nbEdges = (long)read_scalar(c, r.bOff + K);
-  r.noEdges = (long)read_scalar(c, r.oOff + K);
+  long long rb[kHostSlots];
+  read_scalars(c, {r.bOff + K, r.oOff + K}, rb);
+  r.nbEdges = (long)rb[0];
+  r.noEdges = (long)rb[1];
   r.bEdge = (int *)c.ws.get("jr_bEdge", 4 * (size_t)(r.nbEdges + 1));
   r.oEdge = (int *)c.ws.get("jr_oEdge", 4 * (size_t)(r.noEdges + 1));
-  r.E = compact_flags(c, r.joinFlag, K, r.list, "cmf2");
-  if (r.E > 0) {
-    {
-      LaunchTimer t(c, "cmf_apply");
-      hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(r.E, 4)), dim3(256), 0, c.stream, r.E,
-                         r.list, pairs, r.cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords,
-                         in.oVNum, in.oVPSNum, r.joinFlag, r.bCount, r.oCount,
-                         (const long long *)r.bOff, (const long long *)r.oOff, r.bEdge, r.oEdge,
-                         false);
-      HIP_CHECK(hipGetLastError());
-    }
-    {
-      LaunchTimer t(c, "segment_join");
-      hipLaunchKernelGGL(k_segjoin<int>, dim3(grid_for(r.E, 4)), dim3(256), 0, c.stream, r.E,
-                         r.list, pairs, in.bCoords, in.bVPSNum, in.oCoords, in.oVPSNum, r.bCount,
-                         r.oCount, (const long long *)r.bOff, (const long long *)r.oOff, r.bEdge,
-                         r.oEdge, r.segFlag, false, false);
-      HIP_CHECK(hipGetLastError());
-    }
+  const long long *EN = compact_flags_dev(c, r.joinFlag, K, r.list, "cmf2");
+  {
+    LaunchTimer t(c, "cmf_apply");
+    hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(K, 4)), dim3(256), 0, c.stream, K, r.list,
+                       pairs, r.cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum,
+                       in.oVPSNum, r.joinFlag, r.bCount, r.oCount, (const long long *)r.bOff,
+                       (const long long *)r.oOff, r.bEdge, r.oEdge, false, EN);
+    HIP_CHECK(hipGetLastError());
+  }
+  {
+    LaunchTimer t(c, "segment_join");
+    hipLaunchKernelGGL(k_segjoin<int>, dim3(grid_for(K, 4)), dim3(256), 0, c.stream, K, r.list,
+                       pairs, in.bCoords, in.bVPSNum, in.oCoords, in.oVPSNum, r.bCount, r.oCount,
+                       (const long long *)r.bOff, (const long long *)r.oOff, r.bEdge, r.oEdge,
+                       r.segFlag, false, false, EN);
+    HIP_CHECK(hipGetLastError());
   }
   int *tmp = (int *)c.ws.get("jr_hits_idx", 4 * (size_t)Kc);
-  r.hits = compact_flags(c, r.segFlag, K, tmp, "hits");
+  const long long *hitsN = compact_flags_dev(c, r.segFlag, K, tmp, "hits");
   hipLaunchKernelGGL(k_join_final, dim3(grid_for(K, 256)), dim3(256), 0, c.stream, K, r.pipFlag,
                      r.segFlag, r.joined);
   HIP_CHECK(hipGetLastError());
   int *jidx = (int *)c.ws.get("jr_joined_idx", 4 * (size_t)Kc);
-  r.nJoined = compact_flags(c, r.joined, K, jidx, "joined");
+  const long long *joinedN = compact_flags_dev(c, r.joined, K, jidx, "joined");
+  read_scalars(c, {pipN, withinN, cmfN, EN, hitsN, joinedN}, rb);
+  r.pipNum = (long)rb[0];
+  r.within = (long)rb[1];
+  r.cmfCand = (long)rb[2];
+  r.E = (long)rb[3];
+  r.hits = (long)rb[4];
+  r.nJoined = (long)rb[5];
   r.joinedIdx = jidx;
   r.joinedPairs = (int *)c.ws.get("jr_joinedPairs", 8 * (size_t)(r.nJoined > 0 ? r.nJoined : 1));
   if (r.nJoined > 0) {
@@ -519,7 +526,8 @@ void launch_pip(Context &c, long nCand, const int *cand, const int *pairs,
   if (nCand <= 0) return;
   LaunchTimer t(c, "pip");
   hipLaunchKernelGGL(k_pip, dim3(grid_for(nCand, 4)), dim3(256), 0, c.stream, nCand, cand, pairs,
-                     pipType, bCoords, bVPSNum, oCoords, oVPSNum, pipFlag, joinFlag);
+                     pipType, bCoords, bVPSNum, oCoords, oVPSNum, pipFlag, joinFlag,
+                     (const long long *)nullptr);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -531,7 +539,8 @@ void launch_cmf_count(Context &c, long nList, const int *list, const int *pairs,
   hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(nList, 4)), dim3(256), 0, c.stream, nList, list,
                      pairs, cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum,
                      in.oVPSNum, joinFlag, bCount, oCount, (const long long *)nullptr,
-                     (const long long *)nullptr, (int *)nullptr, (int *)nullptr, false);
+                     (const long long *)nullptr, (int *)nullptr, (int *)nullptr, false,
+                     (const long long *)nullptr);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -543,7 +552,7 @@ void launch_cmf_apply_short(Context &c, long nList, const int *list, const int *
   hipLaunchKernelGGL(k_cmf<short>, dim3(grid_for(nList, 4)), dim3(256), 0, c.stream, nList, list,
                      pairs, cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum,
                      in.oVPSNum, (int *)nullptr, (int *)nullptr, (int *)nullptr, bIncl, oIncl,
-                     bEdge, oEdge, true);
+                     bEdge, oEdge, true, (const long long *)nullptr);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -555,7 +564,7 @@ void launch_segjoin_short(Context &c, long nList, const int *list, const int *pa
   hipLaunchKernelGGL(k_segjoin<short>, dim3(grid_for(nList, 4)), dim3(256), 0, c.stream, nList,
                      list, pairs, in.bCoords, in.bVPSNum, in.oCoords, in.oVPSNum,
                      (const int *)nullptr, (const int *)nullptr, bIncl, oIncl, bEdge, oEdge,
-                     flagByList, true, true);
+                     flagByList, true, true, (const long long *)nullptr);
   HIP_CHECK(hipGetLastError());
 }
 

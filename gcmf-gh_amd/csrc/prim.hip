@@ -74,6 +74,32 @@ LaunchTimer::~LaunchTimer() {
   }
 }
 
+// --------------------------------------------------------- read-backs ----
+struct ScalarSrc {
+  const long long *p[kHostSlots];
+};
+
+__global__ void k_gather_scalars(ScalarSrc s, int n, long long *dst) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = s.p[i] ? *s.p[i] : 0;
+}
+
+void read_scalars(Context &c, std::initializer_list<const long long *> src, long long *out) {
+  const int n = (int)src.size();
+  if (n > kHostSlots) throw Error{GCMF_ERR_ARG, "too many scalars in one read-back"};
+  ScalarSrc s{};
+  int i = 0;
+  for (const long long *p : src) s.p[i++] = p;
+  long long *d = (long long *)c.ws.get("read_scalars", sizeof(long long) * kHostSlots);
+  hipLaunchKernelGGL(k_gather_scalars, dim3(1), dim3(64), 0, c.stream, s, n, d);
+  HIP_CHECK(hipGetLastError());
+  if (!c.hbuf) HIP_CHECK(hipHostMalloc((void **)&c.hbuf, sizeof(long long) * kHostSlots));
+  HIP_CHECK(hipMemcpyAsync(c.hbuf, d, sizeof(long long) * (size_t)n, hipMemcpyDeviceToHost,
+                           c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  for (int j = 0; j < n; j++) out[j] = c.hbuf[j];
+}
+
 // --------------------------------------------------------------- fill ----
 __global__ void k_fill_i32(int *p, int v, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,6 +117,8 @@ void fill_i32(Context &c, int *p, int v, long n) {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
+// up to this many tiles every tile sums its predecessors' partials itself
+constexpr long kScanPrefixTiles = 2048;
 
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
@@ -178,6 +206,50 @@ __global__ void __launch_bounds__(kScanThreads) k_tile_scan(const TI *in, long n
   if (total_out && threadIdx.x == 0) *total_out = (TO)(off + tot);
 }
 
+// Scans tile b after summing the partials of tiles [0, b) itself; the last
+// tile writes the grand total to out[n].  Used while the partials are few
+// enough for every block to read them.
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(kScanThreads)
+    k_tile_scan_prefix(const TI *in, long n, TO *out, const long long *partials, long tiles) {
+  __shared__ long long lds[kScanThreads / 64];
+  long long pre = 0;
+  for (long t = threadIdx.x; t < blockIdx.x; t += kScanThreads) pre += partials[t];
+  long long ptot;
+  block_excl_scan<long long>(pre, lds, ptot);
+  __shared__ long long tile[kScanTile];
+  const long base = (long)blockIdx.x * kScanTile;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    int j = k * kScanThreads + threadIdx.x;
+    long i = base + j;
+    tile[j] = i < n ? (long long)in[i] : 0LL;
+  }
+  __syncthreads();
+  long long local[kScanItems];
+  long long s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    local[k] = tile[threadIdx.x * kScanItems + k];
+    s += local[k];
+  }
+  long long tot;
+  long long run = block_excl_scan<long long>(s, lds, tot) + ptot;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    tile[threadIdx.x * kScanItems + k] = run;
+    run += local[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    int j = k * kScanThreads + threadIdx.x;
+    long i = base + j;
+    if (i < n) out[i] = (TO)tile[j];
+  }
+  if (blockIdx.x == tiles - 1 && threadIdx.x == 0) out[n] = (TO)(ptot + tot);
+}
+
 template <typename TO>
 __global__ void k_copy_total(TO *dst, const long long *src) {
   if (threadIdx.x == 0) *dst = (TO)*src;
@@ -201,6 +273,13 @@ static void scan_impl(Context &c, const TI *in, TO *out, long n, int depth) {
     hipLaunchKernelGGL((k_tile_reduce<TI, long long>), dim3((unsigned)tiles), dim3(kScanThreads),
                        0, c.stream, in, n, partials);
     HIP_CHECK(hipGetLastError());
+  }
+  if (tiles <= kScanPrefixTiles) {
+    LaunchTimer t(c, "scan_downsweep");
+    hipLaunchKernelGGL((k_tile_scan_prefix<TI, TO>), dim3((unsigned)tiles), dim3(kScanThreads), 0,
+                       c.stream, in, n, out, (const long long *)partials, tiles);
+    HIP_CHECK(hipGetLastError());
+    return;
   }
   scan_impl<long long, long long>(c, partials, partials, tiles, depth + 1);
   {
@@ -267,6 +346,25 @@ long compact_flags(Context &c, const int *flag, long n, int *out_idx, const char
                      out_idx);
   HIP_CHECK(hipGetLastError());
   return (long)read_scalar(c, pos + n);
+}
+
+const long long *compact_flags_dev(Context &c, const int *flag, long n, int *out_idx,
+                                   const char *tag) {
+  std::string t = std::string("compact_") + tag;
+  long long *pos = (long long *)c.ws.get(t + "_p", sizeof(long long) * (size_t)(n > 0 ? n + 1 : 1));
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_zero_total, dim3(1), dim3(64), 0, c.stream, (void *)pos, 8);
+    HIP_CHECK(hipGetLastError());
+    return pos;
+  }
+  int *f01 = (int *)c.ws.get(t + "_f", sizeof(int) * (size_t)n);
+  hipLaunchKernelGGL(k_flags01, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, flag, f01, n);
+  HIP_CHECK(hipGetLastError());
+  scan_i32_to_i64(c, f01, pos, n);
+  hipLaunchKernelGGL(k_compact, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, flag, pos, n,
+                     out_idx);
+  HIP_CHECK(hipGetLastError());
+  return pos + n;
 }
 
 // --------------------------------------------------------- radix sort ----

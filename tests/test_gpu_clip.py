@@ -257,10 +257,12 @@ def test_clip_disjoint_and_nested(gctx):
 
 def long_run_cases():
     """Pairs whose edges carry more than 32 intersections, so the per-edge
-    alpha sorts take the segmented radix-sort path: a jagged 4000-vertex star
-    against an 8-vertex star (long Q runs), and a 12-gon whose corner 0 is
-    replaced by a 700-point dent well inside the other ring, against a
-    jagged 690-vertex star (long P runs on the 12-gon's chords)."""
+    alpha sorts leave the per-edge thread: a jagged 4000-vertex star against
+    an 8-vertex star (Q runs up to 274, ranked by one block each), a 12-gon
+    whose corner 0 is replaced by a 700-point dent well inside the other ring
+    against a jagged 690-vertex star (P runs of 33-36 on the 12-gon's
+    chords), and a jagged 50000-vertex star against a 6-vertex star (a Q run
+    of 3290, beyond one block, through the segmented radix sort)."""
     P1, Q1 = G.gen_single_pair(0, 4242, 4000, 8)
     ang = 2 * np.pi * np.arange(12) / 12
     gon = np.stack([np.cos(ang), np.sin(ang)], axis=1)
@@ -268,7 +270,8 @@ def long_run_cases():
     dent = np.array([0.5, 0.0]) + 0.05 * np.stack([np.cos(t), np.sin(t)], axis=1)
     P2 = np.concatenate([gon[1:], dent])
     Q2, _ = G.gen_single_pair(0, 4343, 690, 8)
-    return [("longQ", P1, Q1), ("longP", P2, Q2)]
+    P3, Q3 = G.gen_single_pair(0, 3, 50000, 6)
+    return [("longQ", P1, Q1), ("longP", P2, Q2), ("longQ3290", P3, Q3)]
 
 
 def test_clip_long_runs(gctx):
@@ -277,11 +280,11 @@ def test_clip_long_runs(gctx):
         Po, Qo = oriented(P, Q)
         ref = O.phase1(Po, Qo, 1)
         side = "psP2" if name == "longP" else "psQ2"
-        assert int(np.diff(ref[side]).max()) - 1 > 32, name
+        assert int(np.diff(ref[side]).max()) - 1 > (2048 if name == "longQ3290" else 32), name
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
     check_phase1(gctx, st, cases, 1)
     check_results(gctx, st, cases, 1)
-    for p in gctx.clip_get("label_path", 2):
+    for p in gctx.clip_get("label_path", len(cases)):
         assert p in (G.LP_SEGMENTS, G.LP_SEG_CLEANUP)
 
 
