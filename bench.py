@@ -30,6 +30,37 @@ FP64_PEAK_TFS = 78.6
 # algorithmic FP64 operations per classified edge pair: four orientation
 # determinants (7 flops each), two subtractions and two divisions
 FLOPS_PER_EVAL = 32
+# the dominant kernel, timed live with HIP events inside the timed region
+ROOF_KERNEL = "gh_count"
+BREAKDOWN_STEPS = 3
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def traffic_bytes(workload, args):
+    """HBM bytes per launch of the roofline kernel measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE (scaled by a plain copy's reading) for this
+    workload at its default size, from profiles/; None otherwise."""
+    fn = os.path.join(PROFILES, "k_count_traffic.json")
+    if not os.path.exists(fn):
+        return None
+    with open(fn) as f:
+        t = json.load(f)
+    rec = t.get(workload)
+    if not rec or rec.get("size") != (args.n if workload == "join-clip" else args.pairs):
+        return None
+    return rec["bytes_per_launch"]
+
+
+def roofline_line(total_ms, launches, n_eval, traffic):
+    """FP64 work of the count kernel: 32 flops per classified edge pair
+    (four 7-flop orientation determinants, two subtractions, two divisions)."""
+    avg_ms = total_ms / max(launches, 1)
+    flops = FLOPS_PER_EVAL * n_eval
+    achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    return {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFS, 6), "traffic": traffic,
+            "kernel": "k_count", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+            "work": "%d classified edge pairs x %d FP64 flops" % (n_eval, FLOPS_PER_EVAL)}
 
 
 def cpu_baseline(base, over, joined, sample_pairs):
@@ -104,8 +135,9 @@ def main():
     for _ in range(args.warmup):
         js, cs = ctx.join_clip(db, do, flags)
     ctx.synchronize()
+    # timed region: only the roofline kernel carries events
     ctx.timer_reset()
-    ctx.set_timing(True)
+    ctx.set_timing(True, only=ROOF_KERNEL)
     lat = []
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -117,27 +149,26 @@ def main():
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     ctx.set_timing(False)
+    roof_ms, roof_n = ctx.timer(ROOF_KERNEL)
+    # per-kernel breakdown from separate, fully timed steps
+    ctx.timer_reset()
+    ctx.set_timing(True)
+    for _ in range(BREAKDOWN_STEPS):
+        ctx.join_clip(db, do, flags)
+    ctx.synchronize()
+    ctx.set_timing(False)
 
     kernels = ["mbr_endpoints", "sort_hist", "sort_scatter", "scan_tile", "scan_reduce",
                "scan_downsweep", "mbr_sweep_prep", "mbr_sweep_count", "mbr_sweep_write",
                "get_cmbr", "pip", "cmf_count", "cmf_apply", "segment_join", "gh_pair_setup",
                "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own", "gh_calc", "gh_sort_p",
-               "gh_sort_q", "gh_init_label", "gh_label_trace", "gh_gather_out", "fill_i32"]
+               "gh_sort_runs", "gh_sort_q", "gh_init_label", "gh_label_trace", "gh_gather_out",
+               "fill_i32"]
     times = {k: ctx.timer(k) for k in kernels}
     ms_step = 1e3 * elapsed / args.steps
     pairs = cs["pairs"]
     value = pairs / (elapsed / args.steps)
-
-    # dominant kernel roofline: the GH count kernel is FP64-VALU bound
-    cnt_ms, cnt_n = times["gh_count"]
-    avg_ms = cnt_ms / max(cnt_n, 1)
-    flops = FLOPS_PER_EVAL * cs["n_eval"]
-    achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
-    roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFS, 6), "traffic": None,
-                "kernel": "k_count", "avg_launch_ms": round(avg_ms, 5),
-                "work": "%d classified edge pairs x %d FP64 flops" % (cs["n_eval"],
-                                                                     FLOPS_PER_EVAL)}
+    roofline = roofline_line(roof_ms, roof_n, cs["n_eval"], traffic_bytes("join-clip", args))
     line = {"metric": METRIC, "value": round(value, 2), "unit": "pairs/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -153,8 +184,8 @@ def main():
             "join_candidates_per_s": round(js["candidates"] / (elapsed / args.steps), 2),
             "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 4),
             "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 4),
-            "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in times.items()
-                                   if v[1]},
+            "kernel_ms_per_step": {k: round(v[0] / BREAKDOWN_STEPS, 4)
+                                   for k, v in times.items() if v[1]},
             "roofline": roofline}
     if not args.no_cpu:
         joined = ctx.join_get("joined", 2 * js["joined"]).reshape(-1, 2)
@@ -187,7 +218,7 @@ def batched(args):
     for _ in range(args.warmup):
         cs = step()
     ctx.timer_reset()
-    ctx.set_timing(True)
+    ctx.set_timing(True, only=ROOF_KERNEL)
     lat = []
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -199,12 +230,15 @@ def batched(args):
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     ctx.set_timing(False)
-    cnt_ms, cnt_n = ctx.timer("gh_count")
-    avg_ms = cnt_ms / max(cnt_n, 1)
-    achieved = FLOPS_PER_EVAL * cs["n_eval"] / (avg_ms * 1e-3) / 1e12 if avg_ms else 0.0
+    roof_ms, roof_n = ctx.timer(ROOF_KERNEL)
+    ctx.timer_reset()
+    ctx.set_timing(True)
+    step()
+    ctx.synchronize()
+    ctx.set_timing(False)
     kernels = ["gh_pair_setup", "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own",
-               "gh_calc", "gh_sort_p", "gh_sort_q", "gh_init_label", "gh_label_trace",
-               "gh_gather_out", "sort_hist", "sort_scatter"]
+               "gh_calc", "gh_sort_p", "gh_sort_runs", "gh_sort_q", "gh_init_label",
+               "gh_label_trace", "gh_gather_out", "sort_hist", "sort_scatter"]
     line = {"metric": METRIC, "value": round(args.pairs / (elapsed / args.steps), 2),
             "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,
@@ -215,11 +249,11 @@ def batched(args):
                                    "overlapping, FP64, seed 0" % args.pairs,
                        "ok": cs["ok"], "disjoint": cs["disjoint"], "failed": cs["failed"],
                        "result_vertices": cs["total_verts"]},
-            "kernel_ms_per_step": {k: round(ctx.timer(k)[0] / args.steps, 4) for k in kernels},
-            "roofline": {"bound": "mfma", "achieved": round(achieved, 4),
-                         "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
-                         "frac": round(achieved / FP64_PEAK_TFS, 6), "traffic": None,
-                         "kernel": "k_count", "avg_launch_ms": round(avg_ms, 5)}}
+            "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 4),
+            "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 4),
+            "kernel_ms_per_step": {k: round(ctx.timer(k)[0], 4) for k in kernels},
+            "roofline": roofline_line(roof_ms, roof_n, cs["n_eval"],
+                                      traffic_bytes("batched", args))}
     d_pairs.free()
     dP.free()
     dQ.free()

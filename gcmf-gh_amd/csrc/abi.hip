@@ -194,6 +194,13 @@ int gcmf_set_timing(gcmf_ctx *ctx, int on) {
   });
 }
 
+int gcmf_set_timing_only(gcmf_ctx *ctx, const char *kernel) {
+  return guarded(ctx, [&] {
+    ctx->c.flush_timers();
+    ctx->c.timing_only = kernel ? kernel : "";
+  });
+}
+
 int gcmf_timer_reset(gcmf_ctx *ctx) {
   return guarded(ctx, [&] {
     ctx->c.flush_timers();

@@ -57,6 +57,7 @@ struct Context {
   hipStream_t stream = nullptr;
   Workspace ws;
   bool timing = false;
+  std::string timing_only;  // when set, only launches of this name are timed
   std::map<std::string, KernelTimer> timers;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> event_pool;

@@ -258,6 +258,37 @@ __global__ void __launch_bounds__(256)
   if (__lane_id() == 0 && ev) atomicAdd(nEval, ev);
 }
 
+// Complete partner lists at the count1 offsets: the count pass's first
+// kPartners entries, or, for an edge with more, a rescan of the other ring
+// with the count pass's tests (same order, same set).
+__global__ void k_partner_lists(long nAct, const int *act, bool sideP, const int *mapA,
+                                const long long *offA, const long long *offB, const double *ringA,
+                                const double *ringB, const long long *psA1, const int *lstFixed,
+                                int *lst) {
+  const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= nAct) return;
+  const long g = act[ia];
+  const int k = mapA[g];
+  const long long c1 = psA1[g + 1] - psA1[g];
+  int *dst = lst + psA1[g];
+  if (c1 <= kPartners) {
+    for (int i = 0; i < (int)c1; i++) dst[i] = lstFixed[(long long)g * kPartners + i];
+    return;
+  }
+  const long long aBase = offA[k], aN = offA[k + 1] - aBase;
+  const long long bBase = offB[k], bN = offB[k + 1] - bBase;
+  const long long i = g - aBase;
+  const pt A1 = ld(ringA, g), A2 = ld(ringA, i == aN - 1 ? aBase : g + 1);
+  const EdgeBox ab = edge_box(A1, A2);
+  int n = 0;
+  ring_edges(ringB + 2 * bBase, (int)bN, [&](int q, pt B1, pt B2) {
+    if (box_misses(ab, B1, B2) || !lsmf(A1, A2, B1, B2)) return;
+    double al, be;
+    if (itype(A1, A2, B1, B2, al, be) != 0 && n < c1) dst[n++] = q;
+  });
+  (void)sideP;
+}
+
 // per pair: invalid check (:1460-1476) and slot bases
 __global__ void k_pair_check(long M, const long long *offP, const long long *offQ,
                              const long long *psP1, const long long *psQ1,
@@ -276,12 +307,15 @@ __global__ void k_pair_check(long M, const long long *offP, const long long *off
 // gpuNeighborMap (:549-624): for each Q edge, the P edge of its k-th
 // vertex-adding intersection (degenerate types use the parent slot).
 __global__ void __launch_bounds__(256)
-    k_nmap(long edgesQ, const int *mapQ, const long long *offP, const long long *offQ,
+    k_nmap(long edgesQ, const int *act, const int *mapQ, const long long *offP,
+           const long long *offQ,
            const double *ringP, const double *ringQ, const long long *psP1,
            const long long *psQ1, const long long *psQ2, const int *status, int *nmap,
            const int *lstQ) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= edgesQ) return;
+  // edgesQ: number of active Q edges, listed in act
+  const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= edgesQ) return;
+  const long g = act[ia];
   const int k = mapQ[g];
   if (status[k] != GH_OK) return;
   if (psQ1[g + 1] == psQ1[g]) return;
@@ -304,9 +338,7 @@ __global__ void __launch_bounds__(256)
       nmap[slot + c2] = p;
     }
   };
-  const int c1 = (int)(psQ1[g + 1] - psQ1[g]);
-  if (c1 <= kPartners) listed_edges(ringP + 2 * pBase, (int)pN, lstQ + g * kPartners, c1, visit);
-  else ring_edges(ringP + 2 * pBase, (int)pN, visit);
+  listed_edges(ringP + 2 * pBase, (int)pN, lstQ + psQ1[g], (int)(psQ1[g + 1] - psQ1[g]), visit);
 }
 
 // ----------------------------------------------------- intersections ----
@@ -371,6 +403,24 @@ __device__ __forceinline__ long long stable_dest(const int *keys, long long lo, 
   return d;
 }
 
+// First slot of [start, end) (a Q edge's slots) whose neighbour-map entry is
+// P edge pid, or -1.  After the parent slot the map holds the P edges of the
+// vertex-adding intersections in increasing order, one slot each, written
+// from start + 1 on, so the search is a lower bound with unwritten (kNone)
+// slots, which can only trail, taken as +infinity.
+__device__ __forceinline__ long long find_slot(const int *nmap, long long start, long long end,
+                                               int pid) {
+  if (nmap[start] == pid) return start;
+  long long lo = start + 1, hi = end;
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    const int v = nmap[mid];
+    if ((v == kNone ? 0x7fffffff : v) < pid) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < end && nmap[lo] == pid) ? lo : -1;
+}
+
 // Several P edges may write the same Q slot (degenerate configurations, and
 // the reference's neighbour index that persists across Q edges when a search
 // fails).  The reference resolves these races arbitrarily; here the writes of
@@ -383,16 +433,39 @@ struct Owners {
   int *n;  // nbQ / nbQ2
 };
 
-// gpuCalculateIntersections (:897-1123) including the per-edge sort
+// P parents: slot psP2[g] holds P vertex g (:897-1123, the parent store of
+// gpuCalculateIntersections)
+__global__ void k_p_parents(long edgesP, const int *mapP, const long long *offP,
+                            const double *ringP, const long long *psP2, const int *status,
+                            PairMeta pm, const double *bCoords, const double *oCoords,
+                            SlotArrays s) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edgesP) return;
+  const int k = mapP[g];
+  if (status[k] != GH_OK) return;
+  const double *cP = pm.swapped[k] ? oCoords : bCoords;
+  const long long sl = psP2[g];
+  const pt n = ld(ringP, g), o = ld(cP, pm.startP[k] + (g - offP[k]));
+  st(s.iP, sl, n);
+  st(s.iP2, sl, n);
+  st(s.oP, sl, o);
+  st(s.oP2, sl, o);
+}
+
+// gpuCalculateIntersections (:897-1123) without the per-edge sort, over the
+// P edges with intersections
 template <bool OWN>
 __global__ void __launch_bounds__(256)
-    k_calc(long edgesP, const int *mapP, const long long *offP, const long long *offQ,
+    k_calc(long edgesP, const int *act, const int *mapP, const long long *offP,
+           const long long *offQ,
            const double *ringP, const double *ringQ, const long long *psP1,
            const long long *psP2, const long long *psQ1, const long long *psQ2,
            const int *status, PairMeta pm, const double *bCoords, const double *oCoords,
            const int *nmap, SlotArrays s, Owners own, const int *lstP) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= edgesP) return;
+  // edgesP: number of active P edges, listed in act
+  const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= edgesP) return;
+  const long g = act[ia];
   const int k = mapP[g];
   if (status[k] != GH_OK) return;
   const long long pBase = offP[k], pN = offP[k + 1] - pBase;
@@ -404,12 +477,6 @@ __global__ void __launch_bounds__(256)
   const long long base = psP2[g];
   const pt P1 = ld(ringP, g), P2 = ld(ringP, pid == pN - 1 ? pBase : g + 1);
   const pt P1o = ld(cP, sP + pid), P2o = ld(cP, sP + (pid == pN - 1 ? 0 : pid + 1));
-  if (!OWN) {
-    st(s.iP, base, P1);
-    st(s.iP2, base, P1);
-    st(s.oP, base, P1o);
-    st(s.oP2, base, P1o);
-  }
   if (psP1[g + 1] == psP1[g]) return;
   // Q-slot stores (conditional on ownership in the WRITE pass)
   auto setA = [&](long long sl, int v) {
@@ -455,26 +522,21 @@ __global__ void __launch_bounds__(256)
     else c2 = 0;
     const long long start = psQ2[qg], end = psQ2[qg + 1];
     const long long ps = base + c2;
+    const long long l = find_slot(nmap, start, end, pid);
     if (t != 5) {
-      for (long long l = start; l < end; l++) {
-        if (nmap[l] == pid) {
-          nQId = l;
-          setP(ps, (int)(l + 1));
-          setN(l, (int)(ps + 1));
-          break;
-        }
+      if (l >= 0) {
+        nQId = l;
+        setP(ps, (int)(l + 1));
+        setN(l, (int)(ps + 1));
       }
     } else {
       nQId = start;
       setP(ps, (int)(start + 1));
       setN(start, (int)(ps + 1));
-      for (long long l = start; l < end; l++) {
-        if (nmap[l] == pid) {
-          nQId = l;
-          setP(base, (int)(l + 1));
-          setN(l, (int)(base + 1));
-          break;
-        }
+      if (l >= 0) {
+        nQId = l;
+        setP(base, (int)(l + 1));
+        setN(l, (int)(base + 1));
       }
     }
     const int ka = trunc_sat((double)kAlphaScale * alpha);
@@ -517,9 +579,7 @@ __global__ void __launch_bounds__(256)
       setA(start, kb);
     }
   };
-  const int c1 = (int)(psP1[g + 1] - psP1[g]);
-  if (c1 <= kPartners) listed_edges(ringQ + 2 * qBase, (int)qN, lstP + g * kPartners, c1, visit);
-  else ring_edges(ringQ + 2 * qBase, (int)qN, visit);
+  listed_edges(ringQ + 2 * qBase, (int)qN, lstP + psP1[g], (int)(psP1[g + 1] - psP1[g]), visit);
 }
 
 // ------------------------------------------------ per-edge alpha sorts ----
@@ -622,11 +682,12 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-__global__ void k_sort_p_short(long edgesP, const int *mapP, const long long *offP,
-                               const long long *psP2, const int *status, SlotArrays s,
-                               Owners own) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= edgesP) return;
+__global__ void k_sort_p_short(long edgesP, const int *act, const int *mapP,
+                               const long long *offP, const long long *psP2, const int *status,
+                               SlotArrays s, Owners own) {
+  const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= edgesP) return;
+  const long g = act[ia];
   const int k = mapP[g];
   if (status[k] != GH_OK) return;
   const long long lo = psP2[g] + 1, hi = psP2[g + 1];
@@ -637,10 +698,12 @@ __global__ void k_sort_p_short(long edgesP, const int *mapP, const long long *of
 
 // OWN pass: record the highest Q edge writing each P slot's back link
 template <bool OWN>
-__global__ void k_sort_q_short(long edgesQ, const int *mapQ, const long long *psQ2,
-                               const int *status, SlotArrays s, int *ownP) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= edgesQ) return;
+__global__ void k_sort_q_short(long edgesQ, const int *act, const int *mapQ,
+                               const long long *psQ2, const int *status, SlotArrays s,
+                               int *ownP) {
+  const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= edgesQ) return;
+  const long g = act[ia];
   if (status[mapQ[g]] != GH_OK) return;
   const long long lo = psQ2[g] + 1, hi = psQ2[g + 1];
   if (hi - lo < 2 || hi - lo > kSmallSort) return;
@@ -692,12 +755,14 @@ __global__ void k_long_run_apply(long n, const uint64_t *keys, const int *vals, 
 // gpuCalculateInitLabel (:1171-1223); circular ids within the pair's slots.
 // OWN pass: the highest P slot writing a Q label owns it.
 template <bool OWN>
-__global__ void k_init_label(long edgesP, const int *mapP, const long long *offP,
-                             const long long *offQ, const long long *psP2,
-                             const long long *psQ2, const int *status, SlotArrays s,
-                             int *ownL) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= edgesP) return;
+__global__ void k_init_label(long edgesP, const int *act, const int *mapP,
+                             const long long *offP, const long long *offQ,
+                             const long long *psP2, const long long *psQ2, const int *status,
+                             SlotArrays s, int *ownL) {
+  // active P edges only: the slots of the others keep their initial kNone
+  const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= edgesP) return;
+  const long g = act[ia];
   const int k = mapP[g];
   if (status[k] != GH_OK) return;
   const long long sbP = psP2[offP[k]], cP = psP2[offP[k + 1]] - sbP;
@@ -2217,11 +2282,30 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                        edgesQ, mapQ, (const long long *)psQ2, pm.status, listMQ, listQ, lenQ,
                        runCnt + 4);
   HIP_CHECK(hipGetLastError());
+  // edges with intersections (count1 > 0): the later passes run over these
+  int *actP = wsT<int>(c, "gh_actP", edgesP), *actQ = wsT<int>(c, "gh_actQ", edgesQ);
+  const long long *nActPd = compact_flags_dev(c, cnt1P, edgesP, actP, "actP");
+  const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
   {
     const long long *rc = (const long long *)runCnt;
-    read_scalars(c, {psP2 + edgesP, psQ2 + edgesQ, rc, rc + 2, rc + 3, rc + 4, rc + 6, rc + 7},
+    read_scalars(c, {psP2 + edgesP, psQ2 + edgesQ, rc, rc + 2, rc + 3, rc + 4, rc + 6, rc + 7,
+                     nActPd, nActQd, psP1 + edgesP, psQ1 + edgesQ},
                  rb);
   }
+  const long nActP = (long)rb[8], nActQ = (long)rb[9];
+  // every active edge's partners at its count1 offset
+  int *lstAllP = wsT<int>(c, "gh_lstAllP", rb[10]), *lstAllQ = wsT<int>(c, "gh_lstAllQ", rb[11]);
+  if (nActP > 0)
+    hipLaunchKernelGGL(k_partner_lists, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream, nActP,
+                       (const int *)actP, true, mapP, (const long long *)offP,
+                       (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
+                       (const long long *)psP1, (const int *)lstP, lstAllP);
+  if (nActQ > 0)
+    hipLaunchKernelGGL(k_partner_lists, dim3(grid_for(nActQ, 256)), dim3(256), 0, c.stream, nActQ,
+                       (const int *)actQ, false, mapQ, (const long long *)offQ,
+                       (const long long *)offP, (const double *)ringQ, (const double *)ringP,
+                       (const long long *)psQ1, (const int *)lstQ, lstAllQ);
+  HIP_CHECK(hipGetLastError());
   const long slotsP = (long)rb[0], slotsQ = (long)rb[1];
   const int nMedP = (int)rb[2], nRunsP = (int)rb[3], nMedQ = (int)rb[5], nRunsQ = (int)rb[6];
   const long runTotP = (long)rb[4], runTotQ = (long)rb[7];
@@ -2256,10 +2340,11 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   if (edgesQ > 0) {
     {
       LaunchTimer t(c, "gh_nmap");
-      hipLaunchKernelGGL(k_nmap, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream, edgesQ,
-                         mapQ, (const long long *)offP, (const long long *)offQ, ringP, ringQ,
-                         (const long long *)psP1, (const long long *)psQ1,
-                         (const long long *)psQ2, pm.status, nmap, lstQ);
+      if (nActQ > 0)
+      hipLaunchKernelGGL(k_nmap, dim3(grid_for(nActQ, 256)), dim3(256), 0, c.stream, nActQ,
+                         (const int *)actQ, mapQ, (const long long *)offP, (const long long *)offQ,
+                         ringP, ringQ, (const long long *)psP1, (const long long *)psQ1,
+                         (const long long *)psQ2, pm.status, nmap, (const int *)lstAllQ);
       HIP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_q_parents, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream, edgesQ,
@@ -2279,21 +2364,28 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   fill_i32(c, ownP, -1, slotsP);
   fill_i32(c, ownL, -1, slotsQ);
   if (edgesP > 0) {
+    hipLaunchKernelGGL(k_p_parents, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream, edgesP,
+                       mapP, (const long long *)offP, ringP, (const long long *)psP2, pm.status,
+                       pm, in.bCoords, in.oCoords, s);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (nActP > 0) {
     {
       LaunchTimer t(c, "gh_calc_own");
-      hipLaunchKernelGGL(k_calc<true>, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
-                         edgesP, mapP, (const long long *)offP, (const long long *)offQ, ringP,
-                         ringQ, (const long long *)psP1, (const long long *)psP2,
-                         (const long long *)psQ1, (const long long *)psQ2, pm.status, pm,
-                         in.bCoords, in.oCoords, nmap, s, own, lstP);
+      hipLaunchKernelGGL(k_calc<true>, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream,
+                         nActP, (const int *)actP, mapP, (const long long *)offP,
+                         (const long long *)offQ, ringP, ringQ, (const long long *)psP1,
+                         (const long long *)psP2, (const long long *)psQ1,
+                         (const long long *)psQ2, pm.status, pm, in.bCoords, in.oCoords, nmap, s,
+                         own, (const int *)lstAllP);
       HIP_CHECK(hipGetLastError());
     }
     LaunchTimer t(c, "gh_calc");
-    hipLaunchKernelGGL(k_calc<false>, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
-                       edgesP, mapP, (const long long *)offP, (const long long *)offQ, ringP,
-                       ringQ, (const long long *)psP1, (const long long *)psP2,
+    hipLaunchKernelGGL(k_calc<false>, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream, nActP,
+                       (const int *)actP, mapP, (const long long *)offP, (const long long *)offQ,
+                       ringP, ringQ, (const long long *)psP1, (const long long *)psP2,
                        (const long long *)psQ1, (const long long *)psQ2, pm.status, pm,
-                       in.bCoords, in.oCoords, nmap, s, own, lstP);
+                       in.bCoords, in.oCoords, nmap, s, own, (const int *)lstAllP);
     HIP_CHECK(hipGetLastError());
   }
   // per-edge alpha sorts, all P edges before the Q edges (the Q sort reads
@@ -2329,11 +2421,11 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
         HIP_CHECK(hipGetLastError());
       };
     };
-    if (edgesP > 0) {
+    if (nActP > 0) {
       LaunchTimer t(c, "gh_sort_p");
-      hipLaunchKernelGGL(k_sort_p_short, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
-                         edgesP, mapP, (const long long *)offP, (const long long *)psP2,
-                         pm.status, s, own);
+      hipLaunchKernelGGL(k_sort_p_short, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream,
+                         nActP, (const int *)actP, mapP, (const long long *)offP,
+                         (const long long *)psP2, pm.status, s, own);
       HIP_CHECK(hipGetLastError());
     }
     if (nMedP > 0) {
@@ -2347,8 +2439,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       long_runs(false, nRunsP, runTotP, listP, lenP, (const long long *)psP2, mapP,
                 (const long long *)offP)();
     if (edgesQ > 0) {
-      hipLaunchKernelGGL(k_sort_q_short<true>, dim3(grid_for(edgesQ, 256)), dim3(256), 0,
-                         c.stream, edgesQ, mapQ, (const long long *)psQ2, pm.status, s, ownP);
+      if (nActQ > 0)
+        hipLaunchKernelGGL(k_sort_q_short<true>, dim3(grid_for(nActQ, 256)), dim3(256), 0,
+                           c.stream, nActQ, (const int *)actQ, mapQ, (const long long *)psQ2,
+                           pm.status, s, ownP);
       if (nMedQ > 0)
         hipLaunchKernelGGL(k_run_own, dim3(nMedQ < 8192 ? nMedQ : 8192), dim3(256), 0, c.stream,
                            nMedQ, (const int *)listMQ, (const long long *)psQ2,
@@ -2364,8 +2458,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                                : std::function<void()>();
       {
         LaunchTimer t(c, "gh_sort_q");
-        hipLaunchKernelGGL(k_sort_q_short<false>, dim3(grid_for(edgesQ, 256)), dim3(256), 0,
-                           c.stream, edgesQ, mapQ, (const long long *)psQ2, pm.status, s, ownP);
+        if (nActQ > 0)
+          hipLaunchKernelGGL(k_sort_q_short<false>, dim3(grid_for(nActQ, 256)), dim3(256), 0,
+                             c.stream, nActQ, (const int *)actQ, mapQ, (const long long *)psQ2,
+                             pm.status, s, ownP);
         HIP_CHECK(hipGetLastError());
       }
       if (nMedQ > 0) {
@@ -2378,17 +2474,20 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       if (finish) finish();
     }
   }
-  if (edgesP > 0) {
-    hipLaunchKernelGGL(k_init_label<true>, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
-                       edgesP, mapP, (const long long *)offP, (const long long *)offQ,
-                       (const long long *)psP2, (const long long *)psQ2, pm.status, s, ownL);
+  if (nActP > 0) {
+    hipLaunchKernelGGL(k_init_label<true>, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream,
+                       nActP, (const int *)actP, mapP, (const long long *)offP,
+                       (const long long *)offQ, (const long long *)psP2, (const long long *)psQ2,
+                       pm.status, s, ownL);
     HIP_CHECK(hipGetLastError());
     LaunchTimer t(c, "gh_init_label");
-    hipLaunchKernelGGL(k_init_label<false>, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
-                       edgesP, mapP, (const long long *)offP, (const long long *)offQ,
-                       (const long long *)psP2, (const long long *)psQ2, pm.status, s, ownL);
+    hipLaunchKernelGGL(k_init_label<false>, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream,
+                       nActP, (const int *)actP, mapP, (const long long *)offP,
+                       (const long long *)offQ, (const long long *)psP2, (const long long *)psQ2,
+                       pm.status, s, ownL);
     HIP_CHECK(hipGetLastError());
   }
+
   // capacities of the per-pair node, output and component regions
   int *nodeCap = wsT<int>(c, "gh_nodeCap", M), *compCap = wsT<int>(c, "gh_compCap", M);
   long long *nodeOff = wsT<long long>(c, "gh_nodeOff", M + 1);

@@ -61,7 +61,7 @@ void Context::flush_timers() {
 }
 
 LaunchTimer::LaunchTimer(Context &ctx, const char *n) : c(ctx), name(n) {
-  if (c.timing) {
+  if (c.timing && (c.timing_only.empty() || c.timing_only == n)) {
     a = c.take_event();
     b = c.take_event();
     HIP_CHECK(hipEventRecord(a, c.stream));

@@ -236,6 +236,7 @@ def hip_lib():
         L.gcmf_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
         L.gcmf_synchronize.argtypes = [vp]
         L.gcmf_set_timing.argtypes = [vp, C.c_int]
+        L.gcmf_set_timing_only.argtypes = [vp, C.c_char_p]
         L.gcmf_timer_reset.argtypes = [vp]
         L.gcmf_timer_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]
         L.gcmf_join.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), C.POINTER(JoinStats)]
@@ -317,8 +318,12 @@ class Context:
     def synchronize(self):
         self._check(hip_lib().gcmf_synchronize(self.h), "synchronize")
 
-    def set_timing(self, on):
-        self._check(hip_lib().gcmf_set_timing(self.h, int(on)), "set_timing")
+    def set_timing(self, on, only=None):
+        """Kernel timing on/off; `only` restricts it to one kernel name."""
+        L = hip_lib()
+        self._check(L.gcmf_set_timing_only(self.h, only.encode() if only else None),
+                    "set_timing_only")
+        self._check(L.gcmf_set_timing(self.h, int(on)), "set_timing")
 
     def timer_reset(self):
         self._check(hip_lib().gcmf_timer_reset(self.h), "timer_reset")

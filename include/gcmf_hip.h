@@ -57,6 +57,9 @@ int gcmf_memcpy_d2h(gcmf_ctx *ctx, void *dst, const void *d_src, size_t bytes);
 int gcmf_synchronize(gcmf_ctx *ctx);
 /* Kernel timing with HIP events recorded on the context stream. */
 int gcmf_set_timing(gcmf_ctx *ctx, int on);
+/* Restricts timing to the launches named `kernel` (NULL or "" times all),
+ * so that one kernel can be timed inside an otherwise untimed region. */
+int gcmf_set_timing_only(gcmf_ctx *ctx, const char *kernel);
 int gcmf_timer_reset(gcmf_ctx *ctx);
 int gcmf_timer_get(gcmf_ctx *ctx, const char *kernel, double *total_ms, long *launches);
 
