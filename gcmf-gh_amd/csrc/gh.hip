@@ -30,6 +30,43 @@ __device__ __forceinline__ void st(double *c, long i, pt p) {
   c[2 * i + 1] = p.y;
 }
 
+// Appends the lanes with pred to a list counted by *cnt with one atomic per
+// wave; returns each such lane's index (lane order within the wave).
+__device__ __forceinline__ long wave_append(unsigned long long *cnt, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (!m) return -1;
+  const int leader = __ffsll((long long)m) - 1;
+  unsigned long long base = 0;
+  if (__lane_id() == leader) base = atomicAdd(cnt, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader, 64);
+  return pred ? (long)base + __popcll(m & ((1ULL << __lane_id()) - 1ULL)) : -1;
+}
+
+// The same with one atomic per block (blockDim.x <= 1024); every thread of
+// the block must call it.  lds: 17 values of scratch.
+__device__ __forceinline__ long block_append(unsigned long long *cnt, bool pred,
+                                             unsigned long long *lds) {
+  const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  const unsigned long long m = __ballot(pred);
+  if (__lane_id() == 0) lds[wave] = (unsigned long long)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < nw; w++) {
+      const unsigned long long c = lds[w];
+      lds[w] = t;
+      t += c;
+    }
+    lds[16] = t ? atomicAdd(cnt, t) : 0ULL;
+  }
+  __syncthreads();
+  const long r = pred ? (long)(lds[16] + lds[wave]) +
+                            __popcll(m & ((1ULL << __lane_id()) - 1ULL))
+                      : -1;
+  __syncthreads();
+  return r;
+}
+
 __global__ void k_fill_u64_pair(unsigned long long *p) {
   if (threadIdx.x == 0) {
     p[0] = 0ULL;
@@ -188,7 +225,7 @@ __device__ __forceinline__ void ring_edges(const double *V, int n, F &&f) {
 // map and intersection passes act only on those edges, so they walk that
 // list instead of the whole ring when the edge has no more than kPartners
 // of them (its count1).
-constexpr int kPartners = 4;
+constexpr int kPartners = 16;
 
 // f(q, B_q, B_q+1) for the listed edges lst[0, c) of ring V (n vertices)
 template <class F>
@@ -271,22 +308,47 @@ __global__ void k_partner_lists(long nAct, const int *act, bool sideP, const int
   const int k = mapA[g];
   const long long c1 = psA1[g + 1] - psA1[g];
   int *dst = lst + psA1[g];
-  if (c1 <= kPartners) {
-    for (int i = 0; i < (int)c1; i++) dst[i] = lstFixed[(long long)g * kPartners + i];
-    return;
-  }
-  const long long aBase = offA[k], aN = offA[k + 1] - aBase;
-  const long long bBase = offB[k], bN = offB[k + 1] - bBase;
-  const long long i = g - aBase;
-  const pt A1 = ld(ringA, g), A2 = ld(ringA, i == aN - 1 ? aBase : g + 1);
-  const EdgeBox ab = edge_box(A1, A2);
-  int n = 0;
-  ring_edges(ringB + 2 * bBase, (int)bN, [&](int q, pt B1, pt B2) {
-    if (box_misses(ab, B1, B2) || !lsmf(A1, A2, B1, B2)) return;
-    double al, be;
-    if (itype(A1, A2, B1, B2, al, be) != 0 && n < c1) dst[n++] = q;
-  });
+  (void)k;
   (void)sideP;
+  if (c1 <= kPartners)
+    for (int i = 0; i < (int)c1; i++) dst[i] = lstFixed[(long long)g * kPartners + i];
+}
+
+// Partner lists of the edges with more than kPartners partners: one wave
+// per edge rescans the other ring with the count pass's tests, 64 edges at a
+// time, and appends the hits in ring order.
+__global__ void __launch_bounds__(64)
+    k_partner_rescan(long nOver, const int *listO, const int *mapA, const long long *offA,
+                     const long long *offB, const double *ringA, const double *ringB,
+                     const long long *psA1, int *lst) {
+  for (long w = blockIdx.x; w < nOver; w += gridDim.x) {
+    const long g = listO[w];
+    const int k = mapA[g];
+    const long long c1 = psA1[g + 1] - psA1[g];
+    int *dst = lst + psA1[g];
+    const long long aBase = offA[k], aN = offA[k + 1] - aBase;
+    const long long bBase = offB[k], bN = offB[k + 1] - bBase;
+    const long long i = g - aBase;
+    const pt A1 = ld(ringA, g), A2 = ld(ringA, i == aN - 1 ? aBase : g + 1);
+    const EdgeBox ab = edge_box(A1, A2);
+    const double *V = ringB + 2 * bBase;
+    long long n = 0;
+    for (long long q0 = 0; q0 < bN; q0 += 64) {
+      const long long q = q0 + __lane_id();
+      bool hit = false;
+      if (q < bN) {
+        const pt B1 = ld(V, q), B2 = ld(V, q + 1 == bN ? 0 : q + 1);
+        if (!box_misses(ab, B1, B2) && lsmf(A1, A2, B1, B2)) {
+          double al, be;
+          hit = itype(A1, A2, B1, B2, al, be) != 0;
+        }
+      }
+      const unsigned long long m = __ballot(hit);
+      const long long r = n + __popcll(m & ((1ULL << __lane_id()) - 1ULL));
+      if (hit && r < c1) dst[r] = (int)q;
+      n += __popcll(m);
+    }
+  }
 }
 
 // per pair: invalid check (:1460-1476) and slot bases
@@ -380,13 +442,10 @@ __global__ void k_q_parents(long edgesQ, const int *mapQ, const long long *offQ,
   if (status[k] != GH_OK) return;
   const long long i = g - offQ[k];
   const double *cQ = pm.swapped[k] ? bCoords : oCoords;
-  const pt o = ld(cQ, pm.startQ[k] + i);
-  const pt n = ld(ringQ, g);
   const long long sl = psQ2[g];
-  st(s.iQ, sl, n);
-  st(s.iQ2, sl, n);
-  st(s.oQ, sl, o);
-  st(s.oQ2, sl, o);
+  // (the sort sources iQ2 / oQ2 are read for intersection slots only)
+  st(s.iQ, sl, ld(ringQ, g));
+  st(s.oQ, sl, ld(cQ, pm.startQ[k] + i));
 }
 
 // Stable order of keys[lo..hi) by value: dest[j] = #{i : k_i < k_j} +
@@ -445,11 +504,9 @@ __global__ void k_p_parents(long edgesP, const int *mapP, const long long *offP,
   if (status[k] != GH_OK) return;
   const double *cP = pm.swapped[k] ? oCoords : bCoords;
   const long long sl = psP2[g];
-  const pt n = ld(ringP, g), o = ld(cP, pm.startP[k] + (g - offP[k]));
-  st(s.iP, sl, n);
-  st(s.iP2, sl, n);
-  st(s.oP, sl, o);
-  st(s.oP2, sl, o);
+  // (the sort sources iP2 / oP2 are read for intersection slots only)
+  st(s.iP, sl, ld(ringP, g));
+  st(s.oP, sl, ld(cP, pm.startP[k] + (g - offP[k])));
 }
 
 // gpuCalculateIntersections (:897-1123) without the per-edge sort, over the
@@ -584,9 +641,10 @@ __global__ void __launch_bounds__(256)
 
 // ------------------------------------------------ per-edge alpha sorts ----
 // The intersection slots after each edge's parent slot are put in order of
-// their truncated alpha, stably.  Runs of up to kSmallSort slots are ordered
-// by their edge's thread (stable_dest); longer runs go through one segmented
-// radix sort keyed (run, alpha) in slot order, which is the same stable order.
+// their truncated alpha, stably (the order stable_dest defines).  Runs of up
+// to kSmallSort slots are ranked by a 32-lane group, runs up to kBlockSort by
+// one block, longer runs by one segmented radix sort keyed (run, alpha) in
+// slot order, which gives the same stable order.
 constexpr int kSmallSort = 32;
 
 // move of P slot j to sorted position d (calculateIntersections :1103-1120);
@@ -621,25 +679,32 @@ __device__ __forceinline__ void move_q(SlotArrays &s, const int *ownP, long long
 // from a copy of their keys in LDS; longer ones go through the radix sort.
 constexpr int kBlockSort = 2048;
 
-// edges whose run is longer than kSmallSort: the medium ones (up to
-// kBlockSort) in list / runLen with cnt[0] their number, the longer ones in
-// listL / lenL with cnt[2] their number and cnt[3] their total length
-__global__ void k_list_long_runs(long edges, const int *map, const long long *ps2,
-                                 const int *status, int *list, int *listL, int *lenL,
-                                 unsigned long long *cnt) {
+// edges by run length m: the short ones (2 <= m <= kSmallSort) in listS
+// with cnt[1] their number, the medium ones (up to kBlockSort) in list with
+// cnt[0] their number, the longer ones in listL / lenL with cnt[2] their
+// number and cnt[3] their total length; cnt[4] counts the edges in listO,
+// those with more than kPartners intersecting partners
+__global__ void __launch_bounds__(1024)
+    k_list_long_runs(long edges, const int *map, const long long *ps2, const long long *ps1,
+                     const int *status, int *listS, int *list, int *listL, int *lenL, int *listO,
+                     unsigned long long *cnt) {
+  __shared__ unsigned long long lds[17];
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= edges) return;
-  if (status[map[g]] != GH_OK) return;
-  const long long m = ps2[g + 1] - ps2[g] - 1;
-  if (m > kBlockSort) {
-    const int i = (int)atomicAdd(cnt + 2, 1ULL);
+  const bool ok = g < edges && status[map[g]] == GH_OK;
+  const long long m = ok ? ps2[g + 1] - ps2[g] - 1 : 0;
+  long i = block_append(cnt + 1, m >= 2 && m <= kSmallSort, lds);
+  if (i >= 0) listS[i] = (int)g;
+  i = block_append(cnt, m > kSmallSort && m <= kBlockSort, lds);
+  if (i >= 0) list[i] = (int)g;
+  i = block_append(cnt + 2, m > kBlockSort, lds);
+  if (i >= 0) {
     listL[i] = (int)g;
     lenL[i] = (int)m;
     atomicAdd(cnt + 3, (unsigned long long)m);
-  } else if (m > kSmallSort) {
-    const int i = (int)atomicAdd(cnt, 1ULL);
-    list[i] = (int)g;
   }
+  // edges with more intersecting partners than the count pass recorded
+  i = block_append(cnt + 4, ok && ps1[g + 1] - ps1[g] > kPartners, lds);
+  if (i >= 0) listO[i] = (int)g;
 }
 
 // OWN pass of the Q sort over listed runs, one block per run
@@ -682,39 +747,56 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-__global__ void k_sort_p_short(long edgesP, const int *act, const int *mapP,
-                               const long long *offP, const long long *psP2, const int *status,
-                               SlotArrays s, Owners own) {
-  const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ia >= edgesP) return;
+// Runs of up to kSmallSort slots: 32 lanes per active edge, lane l ranking
+// slot lo + l against the run's keys broadcast by shuffles (the rank of
+// stable_dest).
+__device__ __forceinline__ int short_rank(int key, int l, int m) {
+  int d = 0;
+  for (int i = 0; i < m; i++) {
+    const int ki = __shfl(key, i, 32);
+    d += (ki < key) || (ki == key && i < l);
+  }
+  return d;
+}
+
+__global__ void __launch_bounds__(256)
+    k_sort_p_short(long nAct, const int *act, const int *mapP, const long long *offP,
+                   const long long *psP2, const int *status, SlotArrays s, Owners own) {
+  const long ia = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+  const int l = threadIdx.x & 31;
+  if (ia >= nAct) return;
   const long g = act[ia];
   const int k = mapP[g];
   if (status[k] != GH_OK) return;
-  const long long lo = psP2[g] + 1, hi = psP2[g + 1];
-  if (hi - lo < 2 || hi - lo > kSmallSort) return;
-  const int pid = (int)(g - offP[k]);
-  for (long long j = lo; j < hi; j++) move_p(s, own, j, lo + stable_dest(s.aP2, lo, hi, j), pid);
+  const long long lo = psP2[g] + 1;
+  const int m = (int)(psP2[g + 1] - lo);
+  if (m < 2 || m > kSmallSort) return;
+  const int d = short_rank(l < m ? s.aP2[lo + l] : 0, l, m);
+  if (l < m) move_p(s, own, lo + l, lo + d, (int)(g - offP[k]));
 }
 
 // OWN pass: record the highest Q edge writing each P slot's back link
 template <bool OWN>
-__global__ void k_sort_q_short(long edgesQ, const int *act, const int *mapQ,
-                               const long long *psQ2, const int *status, SlotArrays s,
-                               int *ownP) {
-  const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ia >= edgesQ) return;
+__global__ void __launch_bounds__(256)
+    k_sort_q_short(long nAct, const int *act, const int *mapQ, const long long *psQ2,
+                   const int *status, SlotArrays s, int *ownP) {
+  const long ia = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+  const int l = threadIdx.x & 31;
+  if (ia >= nAct) return;
   const long g = act[ia];
   if (status[mapQ[g]] != GH_OK) return;
-  const long long lo = psQ2[g] + 1, hi = psQ2[g + 1];
-  if (hi - lo < 2 || hi - lo > kSmallSort) return;
-  for (long long j = lo; j < hi; j++) {
-    if (OWN) {
-      const int nb = s.nbQ2[j];
+  const long long lo = psQ2[g] + 1;
+  const int m = (int)(psQ2[g + 1] - lo);
+  if (m < 2 || m > kSmallSort) return;
+  if (OWN) {
+    if (l < m) {
+      const int nb = s.nbQ2[lo + l];
       if (nb > 0) atomicMax(&ownP[nb - 1], (int)g);
-    } else {
-      move_q(s, ownP, j, lo + stable_dest(s.aQ2, lo, hi, j), g);
     }
+    return;
   }
+  const int d = short_rank(l < m ? s.aQ2[lo + l] : 0, l, m);
+  if (l < m) move_q(s, ownP, lo + l, lo + d, g);
 }
 
 // keys of the long runs: (run index, alpha as an order-preserving u32), the
@@ -755,19 +837,20 @@ __global__ void k_long_run_apply(long n, const uint64_t *keys, const int *vals, 
 // gpuCalculateInitLabel (:1171-1223); circular ids within the pair's slots.
 // OWN pass: the highest P slot writing a Q label owns it.
 template <bool OWN>
-__global__ void k_init_label(long edgesP, const int *act, const int *mapP,
-                             const long long *offP, const long long *offQ,
-                             const long long *psP2, const long long *psQ2, const int *status,
-                             SlotArrays s, int *ownL) {
-  // active P edges only: the slots of the others keep their initial kNone
-  const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ia >= edgesP) return;
+__global__ void __launch_bounds__(256)
+    k_init_label(long nAct, const int *act, const int *mapP, const long long *offP,
+                 const long long *offQ, const long long *psP2, const long long *psQ2,
+                 const int *status, SlotArrays s, int *ownL) {
+  // 4 lanes per active P edge, one slot each in turn; the slots of inactive
+  // edges keep their initial kNone
+  const long ia = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+  if (ia >= nAct) return;
   const long g = act[ia];
   const int k = mapP[g];
   if (status[k] != GH_OK) return;
   const long long sbP = psP2[offP[k]], cP = psP2[offP[k + 1]] - sbP;
   const long long sbQ = psQ2[offQ[k]], cQ = psQ2[offQ[k + 1]] - sbQ;
-  for (long long i = psP2[g]; i < psP2[g + 1]; i++) {
+  for (long long i = psP2[g] + (threadIdx.x & 3); i < psP2[g + 1]; i += 4) {
     if (!OWN) s.lP[i] = kNone;
     if (s.aP[i] == kNone) continue;
     const long long nId = (long long)s.nbP[i] - 1;
@@ -2268,19 +2351,21 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   HIP_CHECK(hipGetLastError());
   // edges whose alpha runs go through the segmented sort (statuses are final
   // here), read back with the slot totals
-  unsigned long long *runCnt = wsT<unsigned long long>(c, "gh_runCnt", 8);
+  unsigned long long *runCnt = wsT<unsigned long long>(c, "gh_runCnt", 16);
   int *listMP = wsT<int>(c, "gh_runListMP", edgesP), *listMQ = wsT<int>(c, "gh_runListMQ", edgesQ);
+  int *listSP = wsT<int>(c, "gh_runListSP", edgesP), *listSQ = wsT<int>(c, "gh_runListSQ", edgesQ);
+  int *listOP = wsT<int>(c, "gh_runListOP", edgesP), *listOQ = wsT<int>(c, "gh_runListOQ", edgesQ);
   int *listP = wsT<int>(c, "gh_runListP", edgesP), *lenP = wsT<int>(c, "gh_runLenP", edgesP);
   int *listQ = wsT<int>(c, "gh_runListQ", edgesQ), *lenQ = wsT<int>(c, "gh_runLenQ", edgesQ);
-  fill_i32(c, (int *)runCnt, 0, 16);
+  fill_i32(c, (int *)runCnt, 0, 32);  // 8 counters per side
   if (edgesP > 0)
-    hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
-                       edgesP, mapP, (const long long *)psP2, pm.status, listMP, listP, lenP,
-                       runCnt);
+    hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesP, 1024)), dim3(1024), 0, c.stream,
+                       edgesP, mapP, (const long long *)psP2, (const long long *)psP1, pm.status,
+                       listSP, listMP, listP, lenP, listOP, runCnt);
   if (edgesQ > 0)
-    hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
-                       edgesQ, mapQ, (const long long *)psQ2, pm.status, listMQ, listQ, lenQ,
-                       runCnt + 4);
+    hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesQ, 1024)), dim3(1024), 0, c.stream,
+                       edgesQ, mapQ, (const long long *)psQ2, (const long long *)psQ1, pm.status,
+                       listSQ, listMQ, listQ, lenQ, listOQ, runCnt + 8);
   HIP_CHECK(hipGetLastError());
   // edges with intersections (count1 > 0): the later passes run over these
   int *actP = wsT<int>(c, "gh_actP", edgesP), *actQ = wsT<int>(c, "gh_actQ", edgesQ);
@@ -2288,11 +2373,14 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
   {
     const long long *rc = (const long long *)runCnt;
-    read_scalars(c, {psP2 + edgesP, psQ2 + edgesQ, rc, rc + 2, rc + 3, rc + 4, rc + 6, rc + 7,
-                     nActPd, nActQd, psP1 + edgesP, psQ1 + edgesQ},
+    read_scalars(c, {psP2 + edgesP, psQ2 + edgesQ, rc, rc + 2, rc + 3, rc + 8, rc + 10, rc + 11,
+                     nActPd, nActQd, psP1 + edgesP, psQ1 + edgesQ, rc + 1, rc + 9, rc + 4,
+                     rc + 12},
                  rb);
   }
   const long nActP = (long)rb[8], nActQ = (long)rb[9];
+  const long nShortP = (long)rb[12], nShortQ = (long)rb[13];
+  const long nOverP = (long)rb[14], nOverQ = (long)rb[15];
   // every active edge's partners at its count1 offset
   int *lstAllP = wsT<int>(c, "gh_lstAllP", rb[10]), *lstAllQ = wsT<int>(c, "gh_lstAllQ", rb[11]);
   if (nActP > 0)
@@ -2305,6 +2393,16 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                        (const int *)actQ, false, mapQ, (const long long *)offQ,
                        (const long long *)offP, (const double *)ringQ, (const double *)ringP,
                        (const long long *)psQ1, (const int *)lstQ, lstAllQ);
+  if (nOverP > 0)
+    hipLaunchKernelGGL(k_partner_rescan, dim3(nOverP < 65536 ? nOverP : 65536), dim3(64), 0,
+                       c.stream, nOverP, (const int *)listOP, mapP, (const long long *)offP,
+                       (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
+                       (const long long *)psP1, lstAllP);
+  if (nOverQ > 0)
+    hipLaunchKernelGGL(k_partner_rescan, dim3(nOverQ < 65536 ? nOverQ : 65536), dim3(64), 0,
+                       c.stream, nOverQ, (const int *)listOQ, mapQ, (const long long *)offQ,
+                       (const long long *)offP, (const double *)ringQ, (const double *)ringP,
+                       (const long long *)psQ1, lstAllQ);
   HIP_CHECK(hipGetLastError());
   const long slotsP = (long)rb[0], slotsQ = (long)rb[1];
   const int nMedP = (int)rb[2], nRunsP = (int)rb[3], nMedQ = (int)rb[5], nRunsQ = (int)rb[6];
@@ -2421,10 +2519,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
         HIP_CHECK(hipGetLastError());
       };
     };
-    if (nActP > 0) {
+    if (nShortP > 0) {
       LaunchTimer t(c, "gh_sort_p");
-      hipLaunchKernelGGL(k_sort_p_short, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream,
-                         nActP, (const int *)actP, mapP, (const long long *)offP,
+      hipLaunchKernelGGL(k_sort_p_short, dim3(grid_for(32 * nShortP, 256)), dim3(256), 0,
+                         c.stream, nShortP, (const int *)listSP, mapP, (const long long *)offP,
                          (const long long *)psP2, pm.status, s, own);
       HIP_CHECK(hipGetLastError());
     }
@@ -2439,10 +2537,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       long_runs(false, nRunsP, runTotP, listP, lenP, (const long long *)psP2, mapP,
                 (const long long *)offP)();
     if (edgesQ > 0) {
-      if (nActQ > 0)
-        hipLaunchKernelGGL(k_sort_q_short<true>, dim3(grid_for(nActQ, 256)), dim3(256), 0,
-                           c.stream, nActQ, (const int *)actQ, mapQ, (const long long *)psQ2,
-                           pm.status, s, ownP);
+      if (nShortQ > 0)
+        hipLaunchKernelGGL(k_sort_q_short<true>, dim3(grid_for(32 * nShortQ, 256)), dim3(256), 0,
+                           c.stream, nShortQ, (const int *)listSQ, mapQ,
+                           (const long long *)psQ2, pm.status, s, ownP);
       if (nMedQ > 0)
         hipLaunchKernelGGL(k_run_own, dim3(nMedQ < 8192 ? nMedQ : 8192), dim3(256), 0, c.stream,
                            nMedQ, (const int *)listMQ, (const long long *)psQ2,
@@ -2458,10 +2556,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                                : std::function<void()>();
       {
         LaunchTimer t(c, "gh_sort_q");
-        if (nActQ > 0)
-          hipLaunchKernelGGL(k_sort_q_short<false>, dim3(grid_for(nActQ, 256)), dim3(256), 0,
-                             c.stream, nActQ, (const int *)actQ, mapQ, (const long long *)psQ2,
-                             pm.status, s, ownP);
+        if (nShortQ > 0)
+          hipLaunchKernelGGL(k_sort_q_short<false>, dim3(grid_for(32 * nShortQ, 256)),
+                             dim3(256), 0, c.stream, nShortQ, (const int *)listSQ, mapQ,
+                             (const long long *)psQ2, pm.status, s, ownP);
         HIP_CHECK(hipGetLastError());
       }
       if (nMedQ > 0) {
@@ -2475,13 +2573,13 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     }
   }
   if (nActP > 0) {
-    hipLaunchKernelGGL(k_init_label<true>, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream,
+    hipLaunchKernelGGL(k_init_label<true>, dim3(grid_for(4 * nActP, 256)), dim3(256), 0, c.stream,
                        nActP, (const int *)actP, mapP, (const long long *)offP,
                        (const long long *)offQ, (const long long *)psP2, (const long long *)psQ2,
                        pm.status, s, ownL);
     HIP_CHECK(hipGetLastError());
     LaunchTimer t(c, "gh_init_label");
-    hipLaunchKernelGGL(k_init_label<false>, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream,
+    hipLaunchKernelGGL(k_init_label<false>, dim3(grid_for(4 * nActP, 256)), dim3(256), 0, c.stream,
                        nActP, (const int *)actP, mapP, (const long long *)offP,
                        (const long long *)offQ, (const long long *)psP2, (const long long *)psQ2,
                        pm.status, s, ownL);
