@@ -87,8 +87,30 @@ struct PairMeta {
 __global__ void __launch_bounds__(256)
     k_pair_setup(long M, const int *pairs, const double *bCoords, const int *bVNum,
                  const long *bVPSNum, const double *oCoords, const int *oVNum, const long *oVPSNum,
-                 int normalize, int noswap, PairMeta pm, int *eP, int *eQ) {
+                 int normalize, int noswap, PairMeta pm, int *eP, int *eQ,
+                 unsigned long long *maxRing) {
+  __shared__ int lmax[2][4];
   const long k = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (threadIdx.x < 8) lmax[threadIdx.x >> 2][threadIdx.x & 3] = 0;
+  __syncthreads();
+  if (k < M) {
+    const int b = pairs[2 * k], o = pairs[2 * k + 1];
+    const int nb = bVNum[b] - 1, no = oVNum[o] - 1;
+    if (__lane_id() == 0) {
+      lmax[0][threadIdx.x >> 6] = nb > no ? nb : no;
+      lmax[1][threadIdx.x >> 6] = nb > no ? no : nb;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0, q = 0;
+    for (int w = 0; w < 4; w++) {
+      a = max(a, lmax[0][w]);
+      q = max(q, lmax[1][w]);
+    }
+    if (a) atomicMax(maxRing, (unsigned long long)a);
+    if (q) atomicMax(maxRing + 1, (unsigned long long)q);
+  }
   if (k >= M) return;
   const int lane = __lane_id();
   const int b = pairs[2 * k], o = pairs[2 * k + 1];
@@ -220,6 +242,86 @@ __device__ __forceinline__ void ring_edges(const double *V, int n, F &&f) {
   }
 }
 
+// ----------------------------------------------------------- run boxes ----
+// Boxes of runs of kRun1, kRun2 = 16 kRun1 and kRun3 = 16 kRun2 consecutive
+// edges in global edge order (a run spanning two pairs covers both, which
+// only loosens it).  Rings longer than kCullMin are walked through them: a
+// run whose box misses the edge's box holds no edge that lsmf could accept,
+// and the edges of the other runs are still visited in index order.
+constexpr int kRun1 = 16, kRun2 = 256, kRun3 = 4096;
+constexpr long kCullMin = 4096;
+
+struct RunBoxes {
+  const double *b1, *b2, *b3;  // [4 * runs] x0, y0, x1, y1
+};
+
+__global__ void k_run_boxes(long edges, const int *map, const long long *off, const double *ring,
+                            double *box1) {
+  const long G = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long e0 = G * kRun1;
+  if (e0 >= edges) return;
+  double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+  for (long e = e0; e < e0 + kRun1 && e < edges; e++) {
+    const int k = map[e];
+    const long long b = off[k], n = off[k + 1] - b;
+    const pt p = ld(ring, e), q = ld(ring, e - b == n - 1 ? b : e + 1);
+    x0 = fmin(x0, fmin(p.x, q.x));
+    y0 = fmin(y0, fmin(p.y, q.y));
+    x1 = fmax(x1, fmax(p.x, q.x));
+    y1 = fmax(y1, fmax(p.y, q.y));
+  }
+  box1[4 * G] = x0;
+  box1[4 * G + 1] = y0;
+  box1[4 * G + 2] = x1;
+  box1[4 * G + 3] = y1;
+}
+
+// boxes of the next level: 16 boxes merged
+__global__ void k_run_boxes_merge(long runsIn, const double *in, double *out) {
+  const long S = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long g0 = S * 16;
+  if (g0 >= runsIn) return;
+  double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+  for (long g = g0; g < g0 + 16 && g < runsIn; g++) {
+    x0 = fmin(x0, in[4 * g]);
+    y0 = fmin(y0, in[4 * g + 1]);
+    x1 = fmax(x1, in[4 * g + 2]);
+    y1 = fmax(y1, in[4 * g + 3]);
+  }
+  out[4 * S] = x0;
+  out[4 * S + 1] = y0;
+  out[4 * S + 2] = x1;
+  out[4 * S + 3] = y1;
+}
+
+__device__ __forceinline__ bool box_hit(const double *b, const EdgeBox &e) {
+  return !(e.x0 > b[2] || e.x1 < b[0] || e.y0 > b[3] || e.y1 < b[1]);
+}
+
+// f(q, B_q, B_q+1) over the edges of the ring at global edges [e0, e1)
+// (vertices V) whose runs' boxes meet ab, in index order
+template <class F>
+__device__ __forceinline__ void culled_edges(const double *V, long long e0, long long e1,
+                                             const RunBoxes &rb, const EdgeBox &ab, F &&f) {
+  const long long n = e1 - e0;
+  for (long long S3 = e0 / kRun3; S3 * kRun3 < e1; S3++) {
+    if (!box_hit(rb.b3 + 4 * S3, ab)) continue;
+    const long long a2 = max(S3 * 16, e0 / kRun2), z2 = min(S3 * 16 + 16, (e1 - 1) / kRun2 + 1);
+    for (long long S2 = a2; S2 < z2; S2++) {
+      if (!box_hit(rb.b2 + 4 * S2, ab)) continue;
+      const long long a1 = max(S2 * 16, e0 / kRun1), z1 = min(S2 * 16 + 16, (e1 - 1) / kRun1 + 1);
+      for (long long S1 = a1; S1 < z1; S1++) {
+        if (!box_hit(rb.b1 + 4 * S1, ab)) continue;
+        const long long a = max(S1 * kRun1, e0), z = min(S1 * kRun1 + kRun1, e1);
+        for (long long e = a; e < z; e++) {
+          const long long q = e - e0;
+          f((int)q, ld(V, q), ld(V, q + 1 == n ? 0 : q + 1));
+        }
+      }
+    }
+  }
+}
+
 // The count pass records, for every edge, the first kPartners edges of the
 // other ring with a nonzero intersection type, in index order; the neighbour
 // map and intersection passes act only on those edges, so they walk that
@@ -240,12 +342,15 @@ __device__ __forceinline__ void listed_edges(const double *V, int n, const int *
 // ------------------------------------------------------------- count ----
 // gpuCMBRFilter (:382-415) + gpuCountIntersections (:463-548), both views in
 // one launch: threads [0, edgesP) count P edges against Q, the rest count Q
-// edges against P.
+// edges against P.  CULL selects the run-box traversal for rings longer than
+// kCullMin; the plain variant keeps the register budget of the short-ring walk.
+template <bool CULL>
 __global__ void __launch_bounds__(256)
     k_count(long edgesP, long edgesQ, const int *mapP, const int *mapQ, const long long *offP,
             const long long *offQ, const double *ringP, const double *ringQ,
             const double *cmbrN, int *cnt1P, int *cnt2P, int *cnt1Q, int *cnt2Q, int *boolP,
-            int *boolQ, unsigned long long *nEval, int *lstP, int *lstQ) {
+            int *boolQ, unsigned long long *nEval, int *lstP, int *lstQ, RunBoxes boxP,
+            RunBoxes boxQ) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= edgesP + edgesQ) return;
   const bool sideP = g < edgesP;
@@ -269,7 +374,7 @@ __global__ void __launch_bounds__(256)
   if (flag) {
     const EdgeBox ab = edge_box(A1, A2);
     int *lst = (sideP ? lstP : lstQ) + (long long)ga * kPartners;
-    ring_edges(Bv, (int)bN, [&](int q, pt B1, pt B2) {
+    auto visit = [&](int q, pt B1, pt B2) {
       if (box_misses(ab, B1, B2) || !lsmf(A1, A2, B1, B2)) return;
       double al, be;
       ev++;
@@ -279,7 +384,11 @@ __global__ void __launch_bounds__(256)
         c1++;
       }
       c2 += adds_vertex(t);
-    });
+    };
+    if (CULL && bN > kCullMin)
+      culled_edges(Bv, bBase, bBase + bN, sideP ? boxQ : boxP, ab, visit);
+    else
+      ring_edges(Bv, (int)bN, visit);
   }
   if (sideP) {
     cnt1P[ga] = c1;
@@ -320,7 +429,7 @@ __global__ void k_partner_lists(long nAct, const int *act, bool sideP, const int
 __global__ void __launch_bounds__(64)
     k_partner_rescan(long nOver, const int *listO, const int *mapA, const long long *offA,
                      const long long *offB, const double *ringA, const double *ringB,
-                     const long long *psA1, int *lst) {
+                     const long long *psA1, int *lst, RunBoxes rb) {
   for (long w = blockIdx.x; w < nOver; w += gridDim.x) {
     const long g = listO[w];
     const int k = mapA[g];
@@ -333,20 +442,35 @@ __global__ void __launch_bounds__(64)
     const EdgeBox ab = edge_box(A1, A2);
     const double *V = ringB + 2 * bBase;
     long long n = 0;
-    for (long long q0 = 0; q0 < bN; q0 += 64) {
-      const long long q = q0 + __lane_id();
-      bool hit = false;
-      if (q < bN) {
-        const pt B1 = ld(V, q), B2 = ld(V, q + 1 == bN ? 0 : q + 1);
-        if (!box_misses(ab, B1, B2) && lsmf(A1, A2, B1, B2)) {
-          double al, be;
-          hit = itype(A1, A2, B1, B2, al, be) != 0;
+    // edges [a, z) of the ring, 64 at a time, hits appended in order
+    auto scan = [&](long long a, long long z) {
+      for (long long q0 = a; q0 < z; q0 += 64) {
+        const long long q = q0 + __lane_id();
+        bool hit = false;
+        if (q < z) {
+          const pt B1 = ld(V, q), B2 = ld(V, q + 1 == bN ? 0 : q + 1);
+          if (!box_misses(ab, B1, B2) && lsmf(A1, A2, B1, B2)) {
+            double al, be;
+            hit = itype(A1, A2, B1, B2, al, be) != 0;
+          }
         }
+        const unsigned long long m = __ballot(hit);
+        const long long r = n + __popcll(m & ((1ULL << __lane_id()) - 1ULL));
+        if (hit && r < c1) dst[r] = (int)q;
+        n += __popcll(m);
       }
-      const unsigned long long m = __ballot(hit);
-      const long long r = n + __popcll(m & ((1ULL << __lane_id()) - 1ULL));
-      if (hit && r < c1) dst[r] = (int)q;
-      n += __popcll(m);
+    };
+    if (bN <= kCullMin) {
+      scan(0, bN);
+      continue;
+    }
+    const long long e0 = bBase, e1 = bBase + bN;
+    for (long long S3 = e0 / kRun3; S3 * kRun3 < e1; S3++) {
+      if (!box_hit(rb.b3 + 4 * S3, ab)) continue;
+      const long long a2 = max(S3 * 16, e0 / kRun2), z2 = min(S3 * 16 + 16, (e1 - 1) / kRun2 + 1);
+      for (long long S2 = a2; S2 < z2; S2++)
+        if (box_hit(rb.b2 + 4 * S2, ab))
+          scan(max(S2 * kRun2, e0) - e0, min(S2 * kRun2 + kRun2, e1) - e0);
     }
   }
 }
@@ -1244,6 +1368,11 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
       int R = -1, V = I;
       N.fl[V] &= ~F_INT;
       long long steps = 0;
+      // (V, status) after a move fixes the rest of the walk, so a repeated
+      // pair means the walk never returns to I and the guard will fail it:
+      // Brent's cycle detection ends such a walk early with the same status
+      int savedV = -1, savedS = -1;
+      long long power = 1, lam = 0;
       do {
         int status = N.enex[V];
         toggle(status);
@@ -1257,6 +1386,14 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
             else out_append(O, R, w);
           }
           if (++steps > guard) return GH_TRACE_FAIL;
+          if (V == savedV && status == savedS) return GH_TRACE_FAIL;
+          if (lam == power) {
+            savedV = V;
+            savedS = status;
+            power <<= 1;
+            lam = 0;
+          }
+          lam++;
         } while (!(N.enex[V] == status) && V != I);
         if (V != I) {
           V = N.nb[V];
@@ -2299,18 +2436,22 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   int *eP = wsT<int>(c, "gh_eP", M), *eQ = wsT<int>(c, "gh_eQ", M);
   long long *offP = wsT<long long>(c, "gh_offP", M + 1);
   long long *offQ = wsT<long long>(c, "gh_offQ", M + 1);
+  unsigned long long *maxRing = wsT<unsigned long long>(c, "gh_maxRing", 2);
+  fill_i32(c, (int *)maxRing, 0, 4);
   {
     LaunchTimer t(c, "gh_pair_setup");
     hipLaunchKernelGGL(k_pair_setup, dim3(grid_for(M, 4)), dim3(256), 0, c.stream, M, pairs,
                        in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum, in.oVPSNum,
-                       opt.normalize, opt.noswap, pm, eP, eQ);
+                       opt.normalize, opt.noswap, pm, eP, eQ, maxRing);
     HIP_CHECK(hipGetLastError());
   }
   scan_i32_to_i64(c, eP, offP, M);
   scan_i32_to_i64(c, eQ, offQ, M);
   long long rb[kHostSlots];
-  read_scalars(c, {offP + M, offQ + M}, rb);
+  read_scalars(c, {offP + M, offQ + M, (const long long *)maxRing,
+                   (const long long *)maxRing + 1}, rb);
   const long edgesP = (long)rb[0], edgesQ = (long)rb[1];
+  const long maxRingP = (long)rb[2], maxRingQ = (long)rb[3];
   int *mapP = wsT<int>(c, "gh_mapP", edgesP), *mapQ = wsT<int>(c, "gh_mapQ", edgesQ);
   double *ringP = wsT<double>(c, "gh_ringP", 2 * edgesP);
   double *ringQ = wsT<double>(c, "gh_ringQ", 2 * edgesQ);
@@ -2328,12 +2469,39 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   int *lstQ = wsT<int>(c, "gh_lstQ", (long)kPartners * edgesQ);
   unsigned long long *stats = wsT<unsigned long long>(c, "gh_stats", 2);
   hipLaunchKernelGGL(k_fill_u64_pair, dim3(1), dim3(64), 0, c.stream, stats);
+  // run boxes of a side, when some ring of it is long enough to be culled
+  auto run_boxes = [&](bool qside, long edges, const int *map, const long long *off,
+                       const double *ring) {
+    RunBoxes r{nullptr, nullptr, nullptr};
+    const long r1 = (edges + kRun1 - 1) / kRun1, r2 = (r1 + 15) / 16, r3 = (r2 + 15) / 16;
+    double *b1 = wsT<double>(c, qside ? "gh_rb1Q" : "gh_rb1P", 4 * r1);
+    double *b2 = wsT<double>(c, qside ? "gh_rb2Q" : "gh_rb2P", 4 * r2);
+    double *b3 = wsT<double>(c, qside ? "gh_rb3Q" : "gh_rb3P", 4 * r3);
+    hipLaunchKernelGGL(k_run_boxes, dim3(grid_for(r1, 256)), dim3(256), 0, c.stream, edges, map,
+                       off, ring, b1);
+    hipLaunchKernelGGL(k_run_boxes_merge, dim3(grid_for(r2, 256)), dim3(256), 0, c.stream, r1,
+                       (const double *)b1, b2);
+    hipLaunchKernelGGL(k_run_boxes_merge, dim3(grid_for(r3, 256)), dim3(256), 0, c.stream, r2,
+                       (const double *)b2, b3);
+    HIP_CHECK(hipGetLastError());
+    r.b1 = b1;
+    r.b2 = b2;
+    r.b3 = b3;
+    return r;
+  };
+  RunBoxes boxP{nullptr, nullptr, nullptr}, boxQ{nullptr, nullptr, nullptr};
+  if (maxRingP > kCullMin && edgesP > 0)
+    boxP = run_boxes(false, edgesP, mapP, (const long long *)offP, (const double *)ringP);
+  if (maxRingQ > kCullMin && edgesQ > 0)
+    boxQ = run_boxes(true, edgesQ, mapQ, (const long long *)offQ, (const double *)ringQ);
   if (edgesP + edgesQ > 0) {
     LaunchTimer t(c, "gh_count");
-    hipLaunchKernelGGL(k_count, dim3(grid_for(edgesP + edgesQ, 256)), dim3(256), 0, c.stream,
-                       edgesP, edgesQ, mapP, mapQ, (const long long *)offP,
-                       (const long long *)offQ, ringP, ringQ, pm.cmbrN, cnt1P, cnt2P, cnt1Q,
-                       cnt2Q, boolP, boolQ, stats, lstP, lstQ);
+    const bool cull = boxP.b1 != nullptr || boxQ.b1 != nullptr;
+    hipLaunchKernelGGL(cull ? k_count<true> : k_count<false>,
+                       dim3(grid_for(edgesP + edgesQ, 256)), dim3(256), 0, c.stream, edgesP,
+                       edgesQ, mapP, mapQ, (const long long *)offP, (const long long *)offQ,
+                       ringP, ringQ, pm.cmbrN, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, stats,
+                       lstP, lstQ, boxP, boxQ);
     HIP_CHECK(hipGetLastError());
   }
   long long *psP1 = wsT<long long>(c, "gh_psP1", edgesP + 1);
@@ -2397,12 +2565,12 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     hipLaunchKernelGGL(k_partner_rescan, dim3(nOverP < 65536 ? nOverP : 65536), dim3(64), 0,
                        c.stream, nOverP, (const int *)listOP, mapP, (const long long *)offP,
                        (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
-                       (const long long *)psP1, lstAllP);
+                       (const long long *)psP1, lstAllP, boxQ);
   if (nOverQ > 0)
     hipLaunchKernelGGL(k_partner_rescan, dim3(nOverQ < 65536 ? nOverQ : 65536), dim3(64), 0,
                        c.stream, nOverQ, (const int *)listOQ, mapQ, (const long long *)offQ,
                        (const long long *)offP, (const double *)ringQ, (const double *)ringP,
-                       (const long long *)psQ1, lstAllQ);
+                       (const long long *)psQ1, lstAllQ, boxP);
   HIP_CHECK(hipGetLastError());
   const long slotsP = (long)rb[0], slotsQ = (long)rb[1];
   const int nMedP = (int)rb[2], nRunsP = (int)rb[3], nMedQ = (int)rb[5], nRunsQ = (int)rb[6];
