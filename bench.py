@@ -162,7 +162,8 @@ def main():
                "scan_downsweep", "mbr_sweep_prep", "mbr_sweep_count", "mbr_sweep_write",
                "get_cmbr", "pip", "cmf_count", "cmf_apply", "segment_join", "gh_pair_setup",
                "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own", "gh_calc", "gh_sort_p",
-               "gh_sort_runs", "gh_sort_q", "gh_init_label", "gh_label_trace", "gh_gather_out",
+               "gh_sort_runs", "gh_sort_q", "gh_init_label", "gh_cpu_fill", "gh_cpu_sort",
+               "gh_cpu_link", "gh_label_trace", "gh_gather_out",
                "fill_i32"]
     times = {k: ctx.timer(k) for k in kernels}
     ms_step = 1e3 * elapsed / args.steps
@@ -238,6 +239,7 @@ def batched(args):
     ctx.set_timing(False)
     kernels = ["gh_pair_setup", "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own",
                "gh_calc", "gh_sort_p", "gh_sort_runs", "gh_sort_q", "gh_init_label",
+               "gh_cpu_fill", "gh_cpu_sort", "gh_cpu_link",
                "gh_label_trace", "gh_gather_out", "sort_hist", "sort_scatter"]
     line = {"metric": METRIC, "value": round(args.pairs / (elapsed / args.steps), 2),
             "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

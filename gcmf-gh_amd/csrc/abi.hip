@@ -105,6 +105,7 @@ void run_clip(gcmf_ctx *ctx, const gcmf::JoinInputs &in, const int *d_pairs, lon
   opt.normalize = (flags & GCMF_CLIP_NORMALIZE) ? 1 : 0;
   opt.cleanup = (flags & GCMF_CLIP_NO_CLEANUP) ? 0 : 1;
   opt.label_mode = (flags & GCMF_CLIP_SEQ_LABEL) ? 1 : 0;
+  opt.route = (flags & GCMF_CLIP_GPU_ROUTE) ? 0 : 1;
   gcmf::clip_pairs(ctx->c, in, d_pairs, M, opt, ctx->cr, &ctx->p1);
   ctx->c.flush_timers();
   ctx->clip_valid = true;

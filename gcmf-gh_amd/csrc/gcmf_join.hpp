@@ -65,6 +65,7 @@ struct ClipOptions {
   int cleanup = 1;    // run cleanUpResult
   int noswap = 0;     // keep P = base ring even when it is the smaller one
   int label_mode = 0; // 0: wave per pair (sequential fallback), 1: thread per pair
+  int route = 1;      // phase 1: 1 the CPU route (computeIntersections), 0 the GPU route
 };
 
 struct ClipResult {

@@ -462,16 +462,20 @@ __global__ void __launch_bounds__(64)
     };
     if (bN <= kCullMin) {
       scan(0, bN);
-      continue;
+    } else {
+      const long long e0 = bBase, e1 = bBase + bN;
+      for (long long S3 = e0 / kRun3; S3 * kRun3 < e1; S3++) {
+        if (!box_hit(rb.b3 + 4 * S3, ab)) continue;
+        const long long a2 = max(S3 * 16, e0 / kRun2),
+                        z2 = min(S3 * 16 + 16, (e1 - 1) / kRun2 + 1);
+        for (long long S2 = a2; S2 < z2; S2++)
+          if (box_hit(rb.b2 + 4 * S2, ab))
+            scan(max(S2 * kRun2, e0) - e0, min(S2 * kRun2 + kRun2, e1) - e0);
+      }
     }
-    const long long e0 = bBase, e1 = bBase + bN;
-    for (long long S3 = e0 / kRun3; S3 * kRun3 < e1; S3++) {
-      if (!box_hit(rb.b3 + 4 * S3, ab)) continue;
-      const long long a2 = max(S3 * 16, e0 / kRun2), z2 = min(S3 * 16 + 16, (e1 - 1) / kRun2 + 1);
-      for (long long S2 = a2; S2 < z2; S2++)
-        if (box_hit(rb.b2 + 4 * S2, ab))
-          scan(max(S2 * kRun2, e0) - e0, min(S2 * kRun2 + kRun2, e1) - e0);
-    }
+    // entries the rescan did not fill (none when it agrees with the count
+    // pass) name edge 0, so that no later pass reads an unset index
+    for (long long r = n + __lane_id(); r < c1; r += 64) dst[r] = 0;
   }
 }
 
@@ -1002,6 +1006,527 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ================================================= CPU-route phase 1 ====
+// computeIntersections (polyclip.h:188-281) classifies every pair (P edge p,
+// Q edge q) once, as intersect(P_p, Q_q) (polyclip.h:60-179), and inserts
+// the vertices the type adds with insertVertex (polygon.h:101-113): on a P
+// edge in increasing alpha, on a Q edge in increasing beta, a later insertion
+// before an earlier one of equal value.  The loop runs over P edges outside
+// and Q edges inside, so ties come in decreasing partner order on both
+// sides.  link() (polygon.h:89-94) sets both ends, so a source vertex keeps
+// the partner of its last link in loop order: the highest Q edge for a P
+// source, the highest P edge for a Q source.
+//
+// Edge-pair prefilter.  A pair is skipped only when intersect() returns 0 for
+// it.  With L the edge lengths, u the unit roundoff, D the diameter of the
+// pair's joint box (normalised coordinates) and eps = 1e-7:
+//  - types 1-4 put the lines' intersection within (eps + eA) L of each
+//    segment, where eA = 3e-8 D^2 + 1e-15 bounds the rounding of alpha, beta;
+//  - types 5-8 need |A(P1,Q1,Q2)| < eps and |A(P1,Q1,Q2) - A(P2,Q1,Q2)| <=
+//    eps, so P1 lies within (eps + dl) / L_Q and P2 within (2 eps + 3 dl) /
+//    L_Q of Q's line (dl = 8 u D^2 bounds the rounding of A).  Types 5/6 put
+//    P1's foot inside segment Q; types 7/8 put Q1's foot on P's line inside
+//    segment P, and when L_P L_Q >= T = 4.5 (eps + 2 dl) the two lines meet at
+//    a sine <= 1/4, which puts Q1 within 2.2 (eps + 2 dl) / L_Q of segment P.
+// Hence the Chebyshev gap of the closed edge boxes of a nonzero pair is at
+// most mP + mQ, mP = kA L_P + c0, mQ = kA L_Q + kB / L_Q + c0, with kA = 1.5
+// (eps + eA), kB = 1.5 * 2.2 (eps + 2 dl), c0 = 16 u D, unless L_P L_Q < T,
+// in which case the pair is always classified.  Both views take the same
+// decision for a pair (the expressions are symmetric), so their partner sets
+// agree.
+
+// per-pair filter constants (cf[16 k + i])
+enum { CF_BOXP = 0, CF_BOXQ = 4, CF_MAXMP = 8, CF_MAXMQ = 9, CF_MINLP = 10, CF_MINLQ = 11,
+       CF_TINY = 12, CF_N = 16 };
+
+// one wave per pair: ring boxes, edge lengths and margins
+__global__ void __launch_bounds__(256)
+    k_cpu_filter_setup(long M, const int *status, const long long *offP, const long long *offQ,
+                       const double *ringP, const double *ringQ, double *lenP, double *mrgP,
+                       double *lenQ, double *mrgQ, double *cf) {
+  const long k = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= M || status[k] != GH_OK) return;
+  const int lane = __lane_id();
+  double bx[2][4];
+  for (int side = 0; side < 2; side++) {
+    const long long b = side ? offQ[k] : offP[k], n = (side ? offQ[k + 1] : offP[k + 1]) - b;
+    const double *V = side ? ringQ : ringP;
+    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    for (long long i = lane; i < n; i += 64) {
+      const pt p = ld(V, b + i);
+      x0 = fmin(x0, p.x);
+      y0 = fmin(y0, p.y);
+      x1 = fmax(x1, p.x);
+      y1 = fmax(y1, p.y);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      x0 = fmin(x0, __shfl_xor(x0, off, 64));
+      y0 = fmin(y0, __shfl_xor(y0, off, 64));
+      x1 = fmax(x1, __shfl_xor(x1, off, 64));
+      y1 = fmax(y1, __shfl_xor(y1, off, 64));
+    }
+    bx[side][0] = x0;
+    bx[side][1] = y0;
+    bx[side][2] = x1;
+    bx[side][3] = y1;
+  }
+  const double w = fmax(bx[0][2], bx[1][2]) - fmin(bx[0][0], bx[1][0]);
+  const double h = fmax(bx[0][3], bx[1][3]) - fmin(bx[0][1], bx[1][1]);
+  const double D2 = w * w + h * h, D = sqrt(D2);
+  const double u = 1.1102230246251565e-16;
+  const double dl = 8.0 * u * D2, eA = 3e-8 * D2 + 1e-15;
+  const double kA = 1.5 * (kGhEps + eA), kB = 1.5 * 2.2 * (kGhEps + 2.0 * dl), c0 = 16.0 * u * D;
+  double minL[2], maxM[2];
+  for (int side = 0; side < 2; side++) {
+    const long long b = side ? offQ[k] : offP[k], n = (side ? offQ[k + 1] : offP[k + 1]) - b;
+    const double *V = side ? ringQ : ringP;
+    double *len = side ? lenQ : lenP, *mrg = side ? mrgQ : mrgP;
+    double mn = INFINITY, mx = 0.0;
+    for (long long i = lane; i < n; i += 64) {
+      const pt p = ld(V, b + i), q = ld(V, b + (i + 1 == n ? 0 : i + 1));
+      const double dx = q.x - p.x, dy = q.y - p.y;
+      const double L = sqrt(dx * dx + dy * dy);
+      const double m = side ? kA * L + kB / L + c0 : kA * L + c0;
+      len[b + i] = L;
+      mrg[b + i] = m;
+      mn = fmin(mn, L);
+      mx = fmax(mx, m);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      mn = fmin(mn, __shfl_xor(mn, off, 64));
+      mx = fmax(mx, __shfl_xor(mx, off, 64));
+    }
+    minL[side] = mn;
+    maxM[side] = mx;
+  }
+  if (lane == 0) {
+    double *c = cf + (long long)CF_N * k;
+    for (int i = 0; i < 4; i++) {
+      c[CF_BOXP + i] = bx[0][i];
+      c[CF_BOXQ + i] = bx[1][i];
+    }
+    c[CF_MAXMP] = maxM[0];
+    c[CF_MAXMQ] = maxM[1];
+    c[CF_MINLP] = minL[0];
+    c[CF_MINLQ] = minL[1];
+    c[CF_TINY] = 4.5 * (kGhEps + 2.0 * dl);
+  }
+}
+
+// Chebyshev gap of two closed edge boxes (negative when they overlap);
+// symmetric in the two edges
+__device__ __forceinline__ double edge_gap(pt a1, pt a2, pt b1, pt b2) {
+  const double gx = fmax(fmin(b1.x, b2.x) - fmax(a1.x, a2.x), fmin(a1.x, a2.x) - fmax(b1.x, b2.x));
+  const double gy = fmax(fmin(b1.y, b2.y) - fmax(a1.y, a2.y), fmin(a1.y, a2.y) - fmax(b1.y, b2.y));
+  return fmax(gx, gy);
+}
+
+// Per-edge view of the filter: an edge A of one ring against the edges of
+// the other ring (B).  skip() is the pair decision; the ring-level bound G
+// (A's margin plus the largest margin of B) rejects without loading B's
+// length and margin when no pair of A can be tiny.
+struct CpuFilter {
+  double la, ma, G, T;
+  bool tiny;  // some pair of A may have L_A L_B < T
+  const double *lenB, *mrgB;
+  __device__ __forceinline__ bool skip(int q, pt A1, pt A2, pt B1, pt B2) const {
+    const double gap = edge_gap(A1, A2, B1, B2);
+    if (!(gap > 0.0)) return false;
+    if (!tiny && gap > G) return true;
+    return gap > ma + mrgB[q] && !(la * lenB[q] < T);
+  }
+};
+
+__device__ __forceinline__ CpuFilter cpu_filter(const double *cf, int k, bool sideP, long long ga,
+                                                long long bBase, const double *lenP,
+                                                const double *mrgP, const double *lenQ,
+                                                const double *mrgQ) {
+  const double *c = cf + (long long)CF_N * k;
+  CpuFilter f;
+  f.la = sideP ? lenP[ga] : lenQ[ga];
+  f.ma = sideP ? mrgP[ga] : mrgQ[ga];
+  f.T = c[CF_TINY];
+  f.G = f.ma + (sideP ? c[CF_MAXMQ] : c[CF_MAXMP]);
+  f.tiny = f.la * (sideP ? c[CF_MINLQ] : c[CF_MINLP]) < f.T;
+  f.lenB = (sideP ? lenQ : lenP) + bBase;
+  f.mrgB = (sideP ? mrgQ : mrgP) + bBase;
+  return f;
+}
+
+__device__ __forceinline__ bool ins_on_p(int t) { return t == 1 || t == 3 || t == 5 || t == 7; }
+__device__ __forceinline__ bool ins_on_q(int t) { return t == 1 || t == 2 || t == 5 || t == 6; }
+// types whose link() touches the P source (one) / the Q source (q1)
+__device__ __forceinline__ bool src_on_p(int t) {
+  return t == 2 || t == 4 || t == 5 || t == 6 || t == 8;
+}
+__device__ __forceinline__ bool src_on_q(int t) {
+  return t == 3 || t == 4 || t == 5 || t == 7 || t == 8;
+}
+
+// Count pass of the CPU route, both views in one launch: count1 = partners
+// with a nonzero type, count2 = 1 + the vertices the pairs add on this
+// edge's ring.  The Q view evaluates intersect(P edge, Q edge) with the same
+// argument order as the P view.
+template <bool CULL>
+__global__ void __launch_bounds__(256)
+    k_count_cpu(long edgesP, long edgesQ, const int *mapP, const int *mapQ,
+                const long long *offP, const long long *offQ, const double *ringP,
+                const double *ringQ, const double *lenP, const double *mrgP, const double *lenQ,
+                const double *mrgQ, const double *cf, int *cnt1P, int *cnt2P, int *cnt1Q,
+                int *cnt2Q, int *boolP, int *boolQ, unsigned long long *nEval, int *lstP,
+                int *lstQ, RunBoxes boxP, RunBoxes boxQ) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edgesP + edgesQ) return;
+  const bool sideP = g < edgesP;
+  const long ga = sideP ? g : g - edgesP;
+  const int k = sideP ? mapP[ga] : mapQ[ga];
+  const long long aBase = sideP ? offP[k] : offQ[k];
+  const long long aN = (sideP ? offP[k + 1] : offQ[k + 1]) - aBase;
+  const long long bBase = sideP ? offQ[k] : offP[k];
+  const long long bN = (sideP ? offQ[k + 1] : offP[k + 1]) - bBase;
+  const double *A = sideP ? ringP : ringQ;
+  const double *Bv = (sideP ? ringQ : ringP) + 2 * bBase;
+  const long long i = ga - aBase;
+  const pt A1 = ld(A, ga), A2 = ld(A, i == aN - 1 ? aBase : ga + 1);
+  const CpuFilter f = cpu_filter(cf, k, sideP, ga, bBase, lenP, mrgP, lenQ, mrgQ);
+  const double *ob = cf + (long long)CF_N * k + (sideP ? CF_BOXQ : CF_BOXP);
+  const EdgeBox ab = edge_box(A1, A2);
+  const double ogap = fmax(fmax(ob[0] - ab.x1, ab.x0 - ob[2]), fmax(ob[1] - ab.y1, ab.y0 - ob[3]));
+  const bool flag = f.tiny || !(ogap > f.G);
+  int c1 = 0, c2 = 0;
+  unsigned long long ev = 0;
+  if (flag) {
+    int *lst = (sideP ? lstP : lstQ) + (long long)ga * kPartners;
+    auto visit = [&](int q, pt B1, pt B2) {
+      if (f.skip(q, A1, A2, B1, B2)) return;
+      double al, be;
+      ev++;
+      const int t = sideP ? itype(A1, A2, B1, B2, al, be) : itype(B1, B2, A1, A2, al, be);
+      if (t != 0) {
+        if (c1 < kPartners) lst[c1] = q;
+        c1++;
+      }
+      c2 += (sideP ? ins_on_p(t) : ins_on_q(t)) ? 1 : 0;
+    };
+    if (CULL && !f.tiny && bN > kCullMin) {
+      const double G = f.G * (1.0 + 1e-12) + 1e-300;
+      const EdgeBox wide{ab.x0 - G, ab.y0 - G, ab.x1 + G, ab.y1 + G};
+      culled_edges(Bv, bBase, bBase + bN, sideP ? boxQ : boxP, wide, visit);
+    } else {
+      ring_edges(Bv, (int)bN, visit);
+    }
+  }
+  if (sideP) {
+    cnt1P[ga] = c1;
+    cnt2P[ga] = c2 + 1;
+    boolP[ga] = flag;
+  } else {
+    cnt1Q[ga] = c1;
+    cnt2Q[ga] = c2 + 1;
+    boolQ[ga] = flag;
+  }
+  for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off, 64);
+  if (__lane_id() == 0 && ev) atomicAdd(nEval, ev);
+}
+
+// Partner lists of the edges with more than kPartners partners, CPU route:
+// one wave per edge, 64 edges of the other ring at a time, hits in order.
+__global__ void __launch_bounds__(64)
+    k_partner_rescan_cpu(long nOver, const int *listO, bool sideP, const int *mapA,
+                         const long long *offA, const long long *offB, const double *ringA,
+                         const double *ringB, const long long *psA1, int *lst,
+                         const double *lenP, const double *mrgP, const double *lenQ,
+                         const double *mrgQ, const double *cf, RunBoxes rb) {
+  for (long w = blockIdx.x; w < nOver; w += gridDim.x) {
+    const long g = listO[w];
+    const int k = mapA[g];
+    const long long c1 = psA1[g + 1] - psA1[g];
+    int *dst = lst + psA1[g];
+    const long long aBase = offA[k], aN = offA[k + 1] - aBase;
+    const long long bBase = offB[k], bN = offB[k + 1] - bBase;
+    const long long i = g - aBase;
+    const pt A1 = ld(ringA, g), A2 = ld(ringA, i == aN - 1 ? aBase : g + 1);
+    const CpuFilter f = cpu_filter(cf, k, sideP, g, bBase, lenP, mrgP, lenQ, mrgQ);
+    const double *V = ringB + 2 * bBase;
+    long long n = 0;
+    auto scan = [&](long long a, long long z) {
+      for (long long q0 = a; q0 < z; q0 += 64) {
+        const long long q = q0 + __lane_id();
+        bool hit = false;
+        if (q < z) {
+          const pt B1 = ld(V, q), B2 = ld(V, q + 1 == bN ? 0 : q + 1);
+          if (!f.skip((int)q, A1, A2, B1, B2)) {
+            double al, be;
+            hit = (sideP ? itype(A1, A2, B1, B2, al, be) : itype(B1, B2, A1, A2, al, be)) != 0;
+          }
+        }
+        const unsigned long long m = __ballot(hit);
+        const long long r = n + __popcll(m & ((1ULL << __lane_id()) - 1ULL));
+        if (hit && r < c1) dst[r] = (int)q;
+        n += __popcll(m);
+      }
+    };
+    if (rb.b1 == nullptr || f.tiny || bN <= kCullMin) {
+      scan(0, bN);
+    } else {
+      const double G = f.G * (1.0 + 1e-12) + 1e-300;
+      const EdgeBox ab0 = edge_box(A1, A2);
+      const EdgeBox ab{ab0.x0 - G, ab0.y0 - G, ab0.x1 + G, ab0.y1 + G};
+      const long long e0 = bBase, e1 = bBase + bN;
+      for (long long S3 = e0 / kRun3; S3 * kRun3 < e1; S3++) {
+        if (!box_hit(rb.b3 + 4 * S3, ab)) continue;
+        const long long a2 = max(S3 * 16, e0 / kRun2),
+                        z2 = min(S3 * 16 + 16, (e1 - 1) / kRun2 + 1);
+        for (long long S2 = a2; S2 < z2; S2++)
+          if (box_hit(rb.b2 + 4 * S2, ab))
+            scan(max(S2 * kRun2, e0) - e0, min(S2 * kRun2 + kRun2, e1) - e0);
+      }
+    }
+    for (long long r = n + __lane_id(); r < c1; r += 64) dst[r] = 0;
+  }
+}
+
+// CPU-route slot records
+struct CpuSlots {
+  uint64_t *kP2, *kQ2;       // exact alpha / beta (IEEE bits) of the unsorted slots
+  int *rP2, *rQ2, *rP, *rQ;  // (partner index in its ring << 4) | type; unsorted, sorted
+  int *dP, *dQ;              // sorted position of each unsorted slot
+  int *srcP, *srcQ;          // per edge: its source vertex's last link, (partner << 4) | type
+};
+
+__device__ __forceinline__ uint64_t dbits(double v) {
+  union {
+    double d;
+    uint64_t u;
+  } c;
+  c.d = v;
+  return c.u;
+}
+
+// The inserted vertices of every active edge, in decreasing partner order
+// after the parent slot (so that a stable sort by alpha gives insertVertex's
+// order), and the source vertex's last link.  Coordinates as
+// computeIntersections makes them: I = (1 - alpha) P1 + alpha P2 for type 1
+// on both rings, Q1 on P for types 3/5/7, P1 on Q for types 2/5/6.
+__global__ void __launch_bounds__(256)
+    k_cpu_fill(long nActP, long nActQ, const int *actP, const int *actQ, const int *mapP,
+               const int *mapQ, const long long *offP, const long long *offQ,
+               const double *ringP, const double *ringQ, const long long *psP1,
+               const long long *psQ1, const long long *psP2, const long long *psQ2,
+               const int *status, PairMeta pm, const double *bCoords, const double *oCoords,
+               const int *lstP, const int *lstQ, SlotArrays s, CpuSlots cs) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nActP + nActQ) return;
+  const bool sideP = i < nActP;
+  const long g = sideP ? actP[i] : actQ[i - nActP];
+  const int k = sideP ? mapP[g] : mapQ[g];
+  if (status[k] != GH_OK) return;
+  const int sw = pm.swapped[k];
+  const double *cP = sw ? oCoords : bCoords, *cQ = sw ? bCoords : oCoords;
+  const long long aBase = sideP ? offP[k] : offQ[k];
+  const long long aN = (sideP ? offP[k + 1] : offQ[k + 1]) - aBase;
+  const long long bBase = sideP ? offQ[k] : offP[k];
+  const long long bN = (sideP ? offQ[k + 1] : offP[k + 1]) - bBase;
+  const double *cA = sideP ? cP : cQ, *cB = sideP ? cQ : cP;
+  const long sA = sideP ? pm.startP[k] : pm.startQ[k], sB = sideP ? pm.startQ[k] : pm.startP[k];
+  const int ia = (int)(g - aBase);
+  const int ia2 = ia == aN - 1 ? 0 : ia + 1;
+  const double *ringA = sideP ? ringP : ringQ, *ringB = sideP ? ringQ : ringP;
+  const pt A1 = ld(ringA, g), A2 = ld(ringA, aBase + ia2);
+  const long long *ps2 = sideP ? psP2 : psQ2, *ps1 = sideP ? psP1 : psQ1;
+  const long long base = ps2[g], nIns = ps2[g + 1] - base - 1;
+  double *i2 = sideP ? s.iP2 : s.iQ2, *o2 = sideP ? s.oP2 : s.oQ2;
+  uint64_t *k2 = sideP ? cs.kP2 : cs.kQ2;
+  int *r2 = sideP ? cs.rP2 : cs.rQ2;
+  long long r = 0;
+  int src = -1;
+  auto visit = [&](int q, pt B1, pt B2) {
+    double al, be;
+    const int t = sideP ? itype(A1, A2, B1, B2, al, be) : itype(B1, B2, A1, A2, al, be);
+    if (t == 0) return;
+    if (sideP ? src_on_p(t) : src_on_q(t)) src = (q << 4) | t;
+    if (!(sideP ? ins_on_p(t) : ins_on_q(t))) return;
+    const long long u = base + nIns - r;
+    r++;
+    pt v, vo;
+    if (t == 1) {
+      const int q2 = q + 1 == bN ? 0 : q + 1;
+      const pt P1 = sideP ? A1 : B1, P2 = sideP ? A2 : B2;
+      const pt P1o = sideP ? ld(cA, sA + ia) : ld(cB, sB + q);
+      const pt P2o = sideP ? ld(cA, sA + ia2) : ld(cB, sB + q2);
+      v = pt{(1.0 - al) * P1.x + al * P2.x, (1.0 - al) * P1.y + al * P2.y};
+      vo = pt{(1.0 - al) * P1o.x + al * P2o.x, (1.0 - al) * P1o.y + al * P2o.y};
+    } else {
+      // the other ring's first vertex: Q1 on P, P1 on Q
+      v = B1;
+      vo = ld(cB, sB + q);
+    }
+    if (u > base && u <= base + nIns) {
+      st(i2, u, v);
+      st(o2, u, vo);
+      k2[u] = dbits(sideP ? al : be);
+      r2[u] = (q << 4) | t;
+    }
+  };
+  listed_edges(ringB + 2 * bBase, (int)bN, (sideP ? lstP : lstQ) + ps1[g],
+               (int)(ps1[g + 1] - ps1[g]), visit);
+  (sideP ? cs.srcP : cs.srcQ)[g] = src;
+  if (nIns == 1) {
+    // a run of one is in place
+    const long long u = base + 1;
+    st(sideP ? s.iP : s.iQ, u, ld(i2, u));
+    st(sideP ? s.oP : s.oQ, u, ld(o2, u));
+    (sideP ? cs.rP : cs.rQ)[u] = r2[u];
+    (sideP ? cs.dP : cs.dQ)[u] = (int)u;
+  }
+}
+
+template <bool QSIDE>
+__device__ __forceinline__ void cpu_move(SlotArrays &s, CpuSlots &cs, long long j, long long d) {
+  st(QSIDE ? s.iQ : s.iP, d, ld(QSIDE ? s.iQ2 : s.iP2, j));
+  st(QSIDE ? s.oQ : s.oP, d, ld(QSIDE ? s.oQ2 : s.oP2, j));
+  (QSIDE ? cs.rQ : cs.rP)[d] = (QSIDE ? cs.rQ2 : cs.rP2)[j];
+  (QSIDE ? cs.dQ : cs.dP)[j] = (int)d;
+}
+
+// runs of 2..kSmallSort slots: 32 lanes per edge rank by (key, slot)
+template <bool QSIDE>
+__global__ void __launch_bounds__(256)
+    k_cpu_sort_short(long n, const int *list, const long long *ps2, SlotArrays s, CpuSlots cs) {
+  const long ia = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 5;
+  const int l = threadIdx.x & 31;
+  if (ia >= n) return;
+  const long g = list[ia];
+  const long long lo = ps2[g] + 1;
+  const int m = (int)(ps2[g + 1] - lo);
+  const uint64_t *k2 = QSIDE ? cs.kQ2 : cs.kP2;
+  const uint64_t key = l < m ? k2[lo + l] : 0ULL;
+  int d = 0;
+  for (int i = 0; i < m; i++) {
+    const uint64_t ki = __shfl(key, i, 32);
+    d += (ki < key) || (ki == key && i < l);
+  }
+  if (l < m) cpu_move<QSIDE>(s, cs, lo + l, lo + d);
+}
+
+// runs of kSmallSort < m <= kBlockSort slots: one block per run from LDS
+template <bool QSIDE>
+__global__ void __launch_bounds__(256)
+    k_cpu_sort_block(int nRuns, const int *list, const long long *ps2, SlotArrays s,
+                     CpuSlots cs) {
+  __shared__ uint64_t key[kBlockSort];
+  const uint64_t *k2 = QSIDE ? cs.kQ2 : cs.kP2;
+  for (int r = blockIdx.x; r < nRuns; r += gridDim.x) {
+    const long g = list[r];
+    const long long lo = ps2[g] + 1;
+    const int m = (int)(ps2[g + 1] - lo);
+    __syncthreads();
+    for (int t = threadIdx.x; t < m; t += blockDim.x) key[t] = k2[lo + t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+      const uint64_t kt = key[t];
+      int d = 0;
+      for (int i = 0; i < m; i++) {
+        const uint64_t ki = key[i];
+        d += (ki < kt) || (ki == kt && i < t);
+      }
+      cpu_move<QSIDE>(s, cs, lo + t, lo + d);
+    }
+  }
+}
+
+// longer runs: the keys of all long runs concatenated (run order, slot order
+// inside a run), sorted by key and then stably by run
+__global__ void k_cpu_long_keys(int nRuns, const int *list, const long long *runOff,
+                                const long long *ps2, const uint64_t *k2, uint64_t *keys,
+                                int *vals, int *runOf) {
+  for (int r = blockIdx.x; r < nRuns; r += gridDim.x) {
+    const long g = list[r];
+    const long long lo = ps2[g] + 1, m = ps2[g + 1] - lo, o = runOff[r];
+    for (long long t = threadIdx.x; t < m; t += blockDim.x) {
+      keys[o + t] = k2[lo + t];
+      vals[o + t] = (int)(o + t);
+      runOf[o + t] = r;
+    }
+  }
+}
+
+__global__ void k_cpu_long_runkey(long n, const int *vals, const int *runOf, uint64_t *keys) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = (uint64_t)runOf[vals[i]];
+}
+
+template <bool QSIDE>
+__global__ void k_cpu_long_apply(long n, const int *vals, const int *runOf, const int *list,
+                                 const long long *runOff, const long long *ps2, SlotArrays s,
+                                 CpuSlots cs) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = vals[i];
+  const int r = runOf[c];
+  const long long lo = ps2[list[r]] + 1;
+  cpu_move<QSIDE>(s, cs, lo + (c - runOff[r]), lo + (i - runOff[r]));
+}
+
+// the unsorted slot of [lo, hi) (partners strictly decreasing) whose partner
+// is p, or -1
+__device__ __forceinline__ long long find_partner(const int *r2, long long lo, long long hi,
+                                                  int p) {
+  const long long end = hi;
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if ((r2[mid] >> 4) > p) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < end && (r2[lo] >> 4) == p) ? lo : -1;
+}
+
+// Links of the CPU route (global slot + 1 in nbP / nbQ), one thread per
+// active edge writing its own slots: an inserted vertex links to its
+// counterpart (type 1) or to the other ring's source vertex of the pair; a
+// source vertex to the partner of its last link.  A counterpart that is
+// missing marks the pair invalid.
+__global__ void __launch_bounds__(256)
+    k_cpu_link(long nActP, long nActQ, const int *actP, const int *actQ, const int *mapP,
+               const int *mapQ, const long long *offP, const long long *offQ,
+               const long long *psP2, const long long *psQ2, int *status, SlotArrays s,
+               CpuSlots cs) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nActP + nActQ) return;
+  const bool sideP = i < nActP;
+  const long g = sideP ? actP[i] : actQ[i - nActP];
+  const int k = sideP ? mapP[g] : mapQ[g];
+  if (status[k] != GH_OK) return;
+  const long long aBase = sideP ? offP[k] : offQ[k], bBase = sideP ? offQ[k] : offP[k];
+  const int ia = (int)(g - aBase);
+  const long long *ps2A = sideP ? psP2 : psQ2, *ps2B = sideP ? psQ2 : psP2;
+  const int *rA = sideP ? cs.rP : cs.rQ, *rB2 = sideP ? cs.rQ2 : cs.rP2;
+  const int *dB = sideP ? cs.dQ : cs.dP;
+  int *nbA = sideP ? s.nbP : s.nbQ;
+  bool bad = false;
+  // the counterpart on the other ring: the inserted vertex of this pair
+  auto counterpart = [&](int q) -> long long {
+    const long long qg = bBase + q;
+    const long long u = find_partner(rB2, ps2B[qg] + 1, ps2B[qg + 1], ia);
+    return u >= 0 ? (long long)dB[u] : -1;
+  };
+  const long long base = ps2A[g], hi = ps2A[g + 1];
+  for (long long d = base + 1; d < hi; d++) {
+    const int rec = rA[d], q = rec >> 4, t = rec & 15;
+    const long long nb = t == 1 ? counterpart(q) : ps2B[bBase + q];
+    if (nb < 0) bad = true;
+    else nbA[d] = (int)(nb + 1);
+  }
+  const int src = (sideP ? cs.srcP : cs.srcQ)[g];
+  if (src >= 0) {
+    const int q = src >> 4, t = src & 15;
+    const long long nb = (t == 4 || t == 8) ? ps2B[bBase + q] : counterpart(q);
+    if (nb < 0) bad = true;
+    else nbA[base] = (int)(nb + 1);
+  }
+  if (bad) status[k] = GH_INVALID;
+}
+
 // ------------------------------------------------ labelling and tracing ----
 enum { L_NONE = 0, L_CROSSING, L_BOUNCING, L_LEFT_ON, L_RIGHT_ON, L_ON_ON, L_ON_LEFT,
        L_ON_RIGHT, L_DELAYED_CROSSING, L_DELAYED_BOUNCING };
@@ -1451,6 +1976,9 @@ struct TraceArgs {
   int *compLen;  // [comp capacity] final component sizes
   double *xyTmp; // [node capacity] compacted result vertices per pair
   int cleanup;
+  // links given on both rings (nbP and nbQ, the CPU route); otherwise only
+  // the Q slots' links are read and applied with link() (the GPU route)
+  int explicitLinks;
 };
 
 __global__ void __launch_bounds__(64)
@@ -1483,10 +2011,15 @@ __global__ void __launch_bounds__(64)
     N.prev[v] = v == 0 ? (int)(cP - 1) : v - 1;
     N.next[v] = v == cP - 1 ? 0 : v + 1;
     if (pe < pE1 && a.psP2[pe] == sl) {
-      N.fl[v] = F_SRC | (s.aP[sl] != kNone ? F_INT : 0);
+      const bool linked = a.explicitLinks ? s.nbP[sl] > 0 : s.aP[sl] != kNone;
+      N.fl[v] = F_SRC | (linked ? F_INT : 0);
       pe++;
     } else {
       N.fl[v] = F_INT;
+    }
+    if (a.explicitLinks) {
+      const long long kq = (long long)s.nbP[sl] - 1 - sbQ;
+      N.nb[v] = (kq >= 0 && kq < cQ) ? (int)(cP + kq) : -1;
     }
   }
   // Q list; parents are linked when phase 1 recorded a neighbour
@@ -1507,7 +2040,11 @@ __global__ void __launch_bounds__(64)
       N.fl[v] = 0;
       linked = true;
     }
-    if (linked) {
+    if (a.explicitLinks) {
+      const long long kp = (long long)s.nbQ[sl] - 1 - sbP;
+      N.nb[v] = (kp >= 0 && kp < cP) ? (int)kp : -1;
+      if (linked) N.fl[v] |= F_INT;
+    } else if (linked) {
       long long kp = (long long)s.nbQ[sl] - 1 - sbP;
       if (kp >= 0 && kp < cP) node_link(N, v, (int)kp);
       else N.fl[v] |= F_INT;
@@ -2163,13 +2700,19 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
   // -------- build (ghcuda.cpp:239-322 semantics, see k_label_trace): every
   // slot is first taken as an intersection vertex, then the first slot of
   // each edge is re-flagged as its parent vertex
+  const bool xl = a.explicitLinks != 0;
   for (int j = lane; j < nP; j += 64) {
     const long long sl = sbP + j;
     st(N.pn, j, ld(s.iP, sl));
     st(N.po, j, ld(s.oP, sl));
     N.prev[j] = j == 0 ? nP - 1 : j - 1;
     N.next[j] = j == nP - 1 ? 0 : j + 1;
-    N.nb[j] = -1;
+    if (xl) {
+      const long long kq = (long long)s.nbP[sl] - 1 - sbQ;
+      N.nb[j] = (kq >= 0 && kq < cQ) ? nP + (int)kq : -1;
+    } else {
+      N.nb[j] = -1;
+    }
     N.lab[j] = L_NONE;
     N.enex[j] = NEITHER;
     N.fl[j] = F_INT;
@@ -2191,7 +2734,8 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
   __syncthreads();
   for (long long e = pE0 + lane; e < pE1; e += 64) {
     const long long sl = a.psP2[e];
-    N.fl[sl - sbP] = (unsigned char)(F_SRC | (s.aP[sl] != kNone ? F_INT : 0));
+    const bool linked = xl ? s.nbP[sl] > 0 : s.aP[sl] != kNone;
+    N.fl[sl - sbP] = (unsigned char)(F_SRC | (linked ? F_INT : 0));
   }
   for (long long e = qE0 + lane; e < qE1; e += 64) {
     const long long sl = a.psQ2[e];
@@ -2199,13 +2743,15 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
     N.fl[nP + (sl - sbQ)] = (unsigned char)(F_SRC | (linked ? F_INT : 0));
   }
   __syncthreads();
-  // a P node linked from several Q slots keeps the last one (link() order)
-  for (int j = lane; j < nQ; j += 64)
-    if (N.nb[nP + j] >= 0) atomicMax(&N.nb[N.nb[nP + j]], nP + j);
-  __syncthreads();
-  for (int j = lane; j < nP; j += 64)
-    if (N.nb[j] >= 0) N.fl[j] |= F_INT;
-  __syncthreads();
+  if (!xl) {
+    // a P node linked from several Q slots keeps the last one (link() order)
+    for (int j = lane; j < nQ; j += 64)
+      if (N.nb[nP + j] >= 0) atomicMax(&N.nb[N.nb[nP + j]], nP + j);
+    __syncthreads();
+    for (int j = lane; j < nP; j += 64)
+      if (N.nb[j] >= 0) N.fl[j] |= F_INT;
+    __syncthreads();
+  }
   // -------- step 1 (restored) + step 3, one lane per P intersection vertex
   bool slow = false;
   for (int I = lane; I < nP; I += 64) {
@@ -2234,6 +2780,8 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
     slow |= bad;
   }
   __syncthreads();
+  // links must be mutual on both rings (several P sources may point at one
+  // Q vertex when links are explicit)
   for (int j = lane; j < nQ; j += 64) {
     const int v = nP + j;
     if (isint(N, v)) {
@@ -2241,6 +2789,8 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
       if (q < 0 || N.nb[q] != v) slow = true;
     }
   }
+  for (int j = lane; j < nP; j += 64)
+    if (isint(N, j) && (N.nb[j] < 0 || N.nb[N.nb[j]] != j)) slow = true;
   slow = __ballot(slow) != 0ULL;
   // -------- output regions
   OutBuf O;
@@ -2462,6 +3012,24 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                        (const long long *)offQ, mapP, mapQ, ringP, ringQ);
     HIP_CHECK(hipGetLastError());
   }
+  // phase 1 of the CPU route (computeIntersections) or of the GPU route
+  // (calculateIntersections, whose phase-1 arrays the view then holds)
+  const bool cpuRoute = opt.route != 0;
+  double *elenP = nullptr, *emrgP = nullptr, *elenQ = nullptr, *emrgQ = nullptr, *cf = nullptr;
+  if (cpuRoute) {
+    if (maxRingP >= (1L << 27) || maxRingQ >= (1L << 27))
+      throw Error{GCMF_ERR_OVERFLOW, "CPU-route clip takes rings of fewer than 2^27 vertices"};
+    elenP = wsT<double>(c, "gh_elenP", edgesP);
+    emrgP = wsT<double>(c, "gh_emrgP", edgesP);
+    elenQ = wsT<double>(c, "gh_elenQ", edgesQ);
+    emrgQ = wsT<double>(c, "gh_emrgQ", edgesQ);
+    cf = wsT<double>(c, "gh_cf", (long long)CF_N * M);
+    hipLaunchKernelGGL(k_cpu_filter_setup, dim3(grid_for(M, 4)), dim3(256), 0, c.stream, M,
+                       (const int *)pm.status, (const long long *)offP, (const long long *)offQ,
+                       (const double *)ringP, (const double *)ringQ, elenP, emrgP, elenQ, emrgQ,
+                       cf);
+    HIP_CHECK(hipGetLastError());
+  }
   int *cnt1P = wsT<int>(c, "gh_cnt1P", edgesP), *cnt2P = wsT<int>(c, "gh_cnt2P", edgesP);
   int *cnt1Q = wsT<int>(c, "gh_cnt1Q", edgesQ), *cnt2Q = wsT<int>(c, "gh_cnt2Q", edgesQ);
   int *boolP = wsT<int>(c, "gh_boolP", edgesP), *boolQ = wsT<int>(c, "gh_boolQ", edgesQ);
@@ -2497,11 +3065,20 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   if (edgesP + edgesQ > 0) {
     LaunchTimer t(c, "gh_count");
     const bool cull = boxP.b1 != nullptr || boxQ.b1 != nullptr;
-    hipLaunchKernelGGL(cull ? k_count<true> : k_count<false>,
-                       dim3(grid_for(edgesP + edgesQ, 256)), dim3(256), 0, c.stream, edgesP,
-                       edgesQ, mapP, mapQ, (const long long *)offP, (const long long *)offQ,
-                       ringP, ringQ, pm.cmbrN, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, stats,
-                       lstP, lstQ, boxP, boxQ);
+    if (cpuRoute)
+      hipLaunchKernelGGL(cull ? k_count_cpu<true> : k_count_cpu<false>,
+                         dim3(grid_for(edgesP + edgesQ, 256)), dim3(256), 0, c.stream, edgesP,
+                         edgesQ, mapP, mapQ, (const long long *)offP, (const long long *)offQ,
+                         ringP, ringQ, (const double *)elenP, (const double *)emrgP,
+                         (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
+                         cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, stats, lstP, lstQ, boxP,
+                         boxQ);
+    else
+      hipLaunchKernelGGL(cull ? k_count<true> : k_count<false>,
+                         dim3(grid_for(edgesP + edgesQ, 256)), dim3(256), 0, c.stream, edgesP,
+                         edgesQ, mapP, mapQ, (const long long *)offP, (const long long *)offQ,
+                         ringP, ringQ, pm.cmbrN, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ,
+                         stats, lstP, lstQ, boxP, boxQ);
     HIP_CHECK(hipGetLastError());
   }
   long long *psP1 = wsT<long long>(c, "gh_psP1", edgesP + 1);
@@ -2561,12 +3138,26 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                        (const int *)actQ, false, mapQ, (const long long *)offQ,
                        (const long long *)offP, (const double *)ringQ, (const double *)ringP,
                        (const long long *)psQ1, (const int *)lstQ, lstAllQ);
-  if (nOverP > 0)
+  if (nOverP > 0 && cpuRoute)
+    hipLaunchKernelGGL(k_partner_rescan_cpu, dim3(nOverP < 65536 ? nOverP : 65536), dim3(64), 0,
+                       c.stream, nOverP, (const int *)listOP, true, mapP,
+                       (const long long *)offP, (const long long *)offQ, (const double *)ringP,
+                       (const double *)ringQ, (const long long *)psP1, lstAllP,
+                       (const double *)elenP, (const double *)emrgP, (const double *)elenQ,
+                       (const double *)emrgQ, (const double *)cf, boxQ);
+  else if (nOverP > 0)
     hipLaunchKernelGGL(k_partner_rescan, dim3(nOverP < 65536 ? nOverP : 65536), dim3(64), 0,
                        c.stream, nOverP, (const int *)listOP, mapP, (const long long *)offP,
                        (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
                        (const long long *)psP1, lstAllP, boxQ);
-  if (nOverQ > 0)
+  if (nOverQ > 0 && cpuRoute)
+    hipLaunchKernelGGL(k_partner_rescan_cpu, dim3(nOverQ < 65536 ? nOverQ : 65536), dim3(64), 0,
+                       c.stream, nOverQ, (const int *)listOQ, false, mapQ,
+                       (const long long *)offQ, (const long long *)offP, (const double *)ringQ,
+                       (const double *)ringP, (const long long *)psQ1, lstAllQ,
+                       (const double *)elenP, (const double *)emrgP, (const double *)elenQ,
+                       (const double *)emrgQ, (const double *)cf, boxP);
+  else if (nOverQ > 0)
     hipLaunchKernelGGL(k_partner_rescan, dim3(nOverQ < 65536 ? nOverQ : 65536), dim3(64), 0,
                        c.stream, nOverQ, (const int *)listOQ, mapQ, (const long long *)offQ,
                        (const long long *)offP, (const double *)ringQ, (const double *)ringP,
@@ -2603,155 +3194,244 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                        slotsP, slotsQ, s, nmap);
     HIP_CHECK(hipGetLastError());
   }
-  if (edgesQ > 0) {
-    {
-      LaunchTimer t(c, "gh_nmap");
-      if (nActQ > 0)
-      hipLaunchKernelGGL(k_nmap, dim3(grid_for(nActQ, 256)), dim3(256), 0, c.stream, nActQ,
-                         (const int *)actQ, mapQ, (const long long *)offP, (const long long *)offQ,
-                         ringP, ringQ, (const long long *)psP1, (const long long *)psQ1,
-                         (const long long *)psQ2, pm.status, nmap, (const int *)lstAllQ);
-      HIP_CHECK(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_q_parents, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream, edgesQ,
-                       mapQ, (const long long *)offQ, ringQ, (const long long *)psQ2, pm.status,
-                       pm, in.bCoords, in.oCoords, s);
+  if (cpuRoute) {
+    CpuSlots cs;
+    cs.kP2 = wsT<uint64_t>(c, "gh_kP2", slotsP);
+    cs.kQ2 = wsT<uint64_t>(c, "gh_kQ2", slotsQ);
+    cs.rP2 = wsT<int>(c, "gh_rP2", slotsP);
+    cs.rQ2 = wsT<int>(c, "gh_rQ2", slotsQ);
+    cs.rP = wsT<int>(c, "gh_rP", slotsP);
+    cs.rQ = wsT<int>(c, "gh_rQ", slotsQ);
+    cs.dP = wsT<int>(c, "gh_dP", slotsP);
+    cs.dQ = wsT<int>(c, "gh_dQ", slotsQ);
+    cs.srcP = wsT<int>(c, "gh_srcP", edgesP);
+    cs.srcQ = wsT<int>(c, "gh_srcQ", edgesQ);
+    if (edgesQ > 0)
+      hipLaunchKernelGGL(k_q_parents, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
+                         edgesQ, mapQ, (const long long *)offQ, ringQ, (const long long *)psQ2,
+                         pm.status, pm, in.bCoords, in.oCoords, s);
+    if (edgesP > 0)
+      hipLaunchKernelGGL(k_p_parents, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
+                         edgesP, mapP, (const long long *)offP, ringP, (const long long *)psP2,
+                         pm.status, pm, in.bCoords, in.oCoords, s);
     HIP_CHECK(hipGetLastError());
-  }
-  Owners own;
-  own.a = wsT<int>(c, "gh_ownA", slotsQ);
-  own.c = wsT<int>(c, "gh_ownC", slotsQ);
-  own.n = wsT<int>(c, "gh_ownN", slotsQ);
-  int *ownP = wsT<int>(c, "gh_ownP", slotsP);
-  int *ownL = wsT<int>(c, "gh_ownL", slotsQ);
-  fill_i32(c, own.a, -1, slotsQ);
-  fill_i32(c, own.c, -1, slotsQ);
-  fill_i32(c, own.n, -1, slotsQ);
-  fill_i32(c, ownP, -1, slotsP);
-  fill_i32(c, ownL, -1, slotsQ);
-  if (edgesP > 0) {
-    hipLaunchKernelGGL(k_p_parents, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream, edgesP,
-                       mapP, (const long long *)offP, ringP, (const long long *)psP2, pm.status,
-                       pm, in.bCoords, in.oCoords, s);
-    HIP_CHECK(hipGetLastError());
-  }
-  if (nActP > 0) {
-    {
-      LaunchTimer t(c, "gh_calc_own");
-      hipLaunchKernelGGL(k_calc<true>, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream,
-                         nActP, (const int *)actP, mapP, (const long long *)offP,
-                         (const long long *)offQ, ringP, ringQ, (const long long *)psP1,
-                         (const long long *)psP2, (const long long *)psQ1,
-                         (const long long *)psQ2, pm.status, pm, in.bCoords, in.oCoords, nmap, s,
-                         own, (const int *)lstAllP);
+    const long nAct = nActP + nActQ;
+    if (nAct > 0) {
+      LaunchTimer t(c, "gh_cpu_fill");
+      hipLaunchKernelGGL(k_cpu_fill, dim3(grid_for(nAct, 256)), dim3(256), 0, c.stream, nActP,
+                         nActQ, (const int *)actP, (const int *)actQ, mapP, mapQ,
+                         (const long long *)offP, (const long long *)offQ, ringP, ringQ,
+                         (const long long *)psP1, (const long long *)psQ1,
+                         (const long long *)psP2, (const long long *)psQ2, pm.status, pm,
+                         in.bCoords, in.oCoords, (const int *)lstAllP, (const int *)lstAllQ, s,
+                         cs);
       HIP_CHECK(hipGetLastError());
     }
-    LaunchTimer t(c, "gh_calc");
-    hipLaunchKernelGGL(k_calc<false>, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream, nActP,
-                       (const int *)actP, mapP, (const long long *)offP, (const long long *)offQ,
-                       ringP, ringQ, (const long long *)psP1, (const long long *)psP2,
-                       (const long long *)psQ1, (const long long *)psQ2, pm.status, pm,
-                       in.bCoords, in.oCoords, nmap, s, own, (const int *)lstAllP);
-    HIP_CHECK(hipGetLastError());
-  }
-  // per-edge alpha sorts, all P edges before the Q edges (the Q sort reads
-  // the back links the P sort moves)
-  {
-    // keys, segmented sort and moves of one side's long runs
-    auto long_runs = [&](bool qside, int n, long total, const int *list, const int *len,
-                         const long long *ps2, const int *map, const long long *off) {
-      long long *runOff = wsT<long long>(c, qside ? "gh_runOffQ" : "gh_runOffP", n + 1);
-      scan_i32_to_i64(c, len, runOff, n);
-      uint64_t *k0 = wsT<uint64_t>(c, "gh_runK0", total), *k1 = wsT<uint64_t>(c, "gh_runK1", total);
-      int *v0 = wsT<int>(c, "gh_runV0", total), *v1 = wsT<int>(c, "gh_runV1", total);
-      const unsigned blocks = (unsigned)(n < 4096 ? n : 4096);
-      hipLaunchKernelGGL(k_long_run_keys<false>, dim3(blocks), dim3(256), 0, c.stream, n, list,
-                         (const long long *)runOff, ps2,
-                         (const int *)(qside ? s.aQ2 : s.aP2),
-                         (const int *)(qside ? s.nbQ2 : s.nbP2), ownP, k0, v0);
-      HIP_CHECK(hipGetLastError());
-      return [=, &c, &s, &own](void) {
-        int bits = 32;
-        for (int x = n - 1; x > 0; x >>= 1) bits++;
-        radix_sort_u64(c, k0, v0, k1, v1, total, bits);
-        if (qside)
-          hipLaunchKernelGGL(k_long_run_apply<true>, dim3(grid_for(total, 256)), dim3(256), 0,
-                             c.stream, total, (const uint64_t *)k1, (const int *)v1, list,
-                             (const long long *)runOff, ps2, map, off, s, own,
-                             (const int *)ownP);
-        else
-          hipLaunchKernelGGL(k_long_run_apply<false>, dim3(grid_for(total, 256)), dim3(256), 0,
-                             c.stream, total, (const uint64_t *)k1, (const int *)v1, list,
-                             (const long long *)runOff, ps2, map, off, s, own,
-                             (const int *)ownP);
-        HIP_CHECK(hipGetLastError());
-      };
-    };
-    if (nShortP > 0) {
-      LaunchTimer t(c, "gh_sort_p");
-      hipLaunchKernelGGL(k_sort_p_short, dim3(grid_for(32 * nShortP, 256)), dim3(256), 0,
-                         c.stream, nShortP, (const int *)listSP, mapP, (const long long *)offP,
-                         (const long long *)psP2, pm.status, s, own);
-      HIP_CHECK(hipGetLastError());
-    }
-    if (nMedP > 0) {
-      LaunchTimer t(c, "gh_sort_runs");
-      hipLaunchKernelGGL(k_run_sort_block<false>, dim3(nMedP < 8192 ? nMedP : 8192), dim3(256),
-                         0, c.stream, nMedP, (const int *)listMP, (const long long *)psP2, mapP,
-                         (const long long *)offP, s, own, (const int *)ownP);
-      HIP_CHECK(hipGetLastError());
-    }
-    if (nRunsP > 0)
-      long_runs(false, nRunsP, runTotP, listP, lenP, (const long long *)psP2, mapP,
-                (const long long *)offP)();
-    if (edgesQ > 0) {
-      if (nShortQ > 0)
-        hipLaunchKernelGGL(k_sort_q_short<true>, dim3(grid_for(32 * nShortQ, 256)), dim3(256), 0,
-                           c.stream, nShortQ, (const int *)listSQ, mapQ,
-                           (const long long *)psQ2, pm.status, s, ownP);
-      if (nMedQ > 0)
-        hipLaunchKernelGGL(k_run_own, dim3(nMedQ < 8192 ? nMedQ : 8192), dim3(256), 0, c.stream,
-                           nMedQ, (const int *)listMQ, (const long long *)psQ2,
-                           (const int *)s.nbQ2, ownP);
-      if (nRunsQ > 0)
-        hipLaunchKernelGGL(k_run_own, dim3(nRunsQ < 8192 ? nRunsQ : 8192), dim3(256), 0,
-                           c.stream, nRunsQ, (const int *)listQ, (const long long *)psQ2,
-                           (const int *)s.nbQ2, ownP);
-      HIP_CHECK(hipGetLastError());
-      auto finish = nRunsQ > 0 ? long_runs(true, nRunsQ, runTotQ, listQ, lenQ,
-                                           (const long long *)psQ2, mapQ,
-                                           (const long long *)offQ)
-                               : std::function<void()>();
-      {
-        LaunchTimer t(c, "gh_sort_q");
-        if (nShortQ > 0)
-          hipLaunchKernelGGL(k_sort_q_short<false>, dim3(grid_for(32 * nShortQ, 256)),
-                             dim3(256), 0, c.stream, nShortQ, (const int *)listSQ, mapQ,
-                             (const long long *)psQ2, pm.status, s, ownP);
+    // per-edge sorts by exact alpha (beta), stable in slot order
+    auto sort_side = [&](bool qside, long nShort, const int *lS, long nMed, const int *lM,
+                         long nRuns, long runTot, const int *lL, const int *lenL,
+                         const long long *ps2) {
+      if (nShort > 0) {
+        LaunchTimer t(c, "gh_cpu_sort");
+        hipLaunchKernelGGL(qside ? k_cpu_sort_short<true> : k_cpu_sort_short<false>,
+                           dim3(grid_for(32 * nShort, 256)), dim3(256), 0, c.stream, nShort, lS,
+                           ps2, s, cs);
         HIP_CHECK(hipGetLastError());
       }
-      if (nMedQ > 0) {
+      if (nMed > 0) {
         LaunchTimer t(c, "gh_sort_runs");
-        hipLaunchKernelGGL(k_run_sort_block<true>, dim3(nMedQ < 8192 ? nMedQ : 8192), dim3(256),
-                           0, c.stream, nMedQ, (const int *)listMQ, (const long long *)psQ2,
-                           mapQ, (const long long *)offQ, s, own, (const int *)ownP);
+        hipLaunchKernelGGL(qside ? k_cpu_sort_block<true> : k_cpu_sort_block<false>,
+                           dim3(nMed < 8192 ? nMed : 8192), dim3(256), 0, c.stream, (int)nMed,
+                           lM, ps2, s, cs);
         HIP_CHECK(hipGetLastError());
       }
-      if (finish) finish();
+      if (nRuns > 0) {
+        long long *runOff = wsT<long long>(c, qside ? "gh_runOffQ" : "gh_runOffP", nRuns + 1);
+        scan_i32_to_i64(c, lenL, runOff, nRuns);
+        uint64_t *k0 = wsT<uint64_t>(c, "gh_runK0", runTot);
+        uint64_t *k1 = wsT<uint64_t>(c, "gh_runK1", runTot);
+        int *v0 = wsT<int>(c, "gh_runV0", runTot), *v1 = wsT<int>(c, "gh_runV1", runTot);
+        int *runOf = wsT<int>(c, "gh_runOf", runTot);
+        const unsigned blocks = (unsigned)(nRuns < 4096 ? nRuns : 4096);
+        hipLaunchKernelGGL(k_cpu_long_keys, dim3(blocks), dim3(256), 0, c.stream, (int)nRuns, lL,
+                           (const long long *)runOff, ps2,
+                           (const uint64_t *)(qside ? cs.kQ2 : cs.kP2), k0, v0, runOf);
+        radix_sort_u64(c, k0, v0, k1, v1, runTot, 64);
+        hipLaunchKernelGGL(k_cpu_long_runkey, dim3(grid_for(runTot, 256)), dim3(256), 0,
+                           c.stream, runTot, (const int *)v1, (const int *)runOf, k0);
+        int bits = 0;
+        for (long x = nRuns; x > 0; x >>= 1) bits++;
+        radix_sort_u64(c, k0, v1, k1, v0, runTot, bits);
+        hipLaunchKernelGGL(qside ? k_cpu_long_apply<true> : k_cpu_long_apply<false>,
+                           dim3(grid_for(runTot, 256)), dim3(256), 0, c.stream, runTot,
+                           (const int *)v0, (const int *)runOf, lL, (const long long *)runOff,
+                           ps2, s, cs);
+        HIP_CHECK(hipGetLastError());
+      }
+    };
+    sort_side(false, nShortP, listSP, nMedP, listMP, nRunsP, runTotP, listP, lenP,
+              (const long long *)psP2);
+    sort_side(true, nShortQ, listSQ, nMedQ, listMQ, nRunsQ, runTotQ, listQ, lenQ,
+              (const long long *)psQ2);
+    if (nAct > 0) {
+      LaunchTimer t(c, "gh_cpu_link");
+      hipLaunchKernelGGL(k_cpu_link, dim3(grid_for(nAct, 256)), dim3(256), 0, c.stream, nActP,
+                         nActQ, (const int *)actP, (const int *)actQ, mapP, mapQ,
+                         (const long long *)offP, (const long long *)offQ,
+                         (const long long *)psP2, (const long long *)psQ2, pm.status, s, cs);
+      HIP_CHECK(hipGetLastError());
     }
-  }
-  if (nActP > 0) {
-    hipLaunchKernelGGL(k_init_label<true>, dim3(grid_for(4 * nActP, 256)), dim3(256), 0, c.stream,
-                       nActP, (const int *)actP, mapP, (const long long *)offP,
-                       (const long long *)offQ, (const long long *)psP2, (const long long *)psQ2,
-                       pm.status, s, ownL);
-    HIP_CHECK(hipGetLastError());
-    LaunchTimer t(c, "gh_init_label");
-    hipLaunchKernelGGL(k_init_label<false>, dim3(grid_for(4 * nActP, 256)), dim3(256), 0, c.stream,
-                       nActP, (const int *)actP, mapP, (const long long *)offP,
-                       (const long long *)offQ, (const long long *)psP2, (const long long *)psQ2,
-                       pm.status, s, ownL);
-    HIP_CHECK(hipGetLastError());
+  } else {
+    if (edgesQ > 0) {
+      {
+        LaunchTimer t(c, "gh_nmap");
+        if (nActQ > 0)
+        hipLaunchKernelGGL(k_nmap, dim3(grid_for(nActQ, 256)), dim3(256), 0, c.stream, nActQ,
+                           (const int *)actQ, mapQ, (const long long *)offP, (const long long *)offQ,
+                           ringP, ringQ, (const long long *)psP1, (const long long *)psQ1,
+                           (const long long *)psQ2, pm.status, nmap, (const int *)lstAllQ);
+        HIP_CHECK(hipGetLastError());
+      }
+      hipLaunchKernelGGL(k_q_parents, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream, edgesQ,
+                         mapQ, (const long long *)offQ, ringQ, (const long long *)psQ2, pm.status,
+                         pm, in.bCoords, in.oCoords, s);
+      HIP_CHECK(hipGetLastError());
+    }
+    Owners own;
+    own.a = wsT<int>(c, "gh_ownA", slotsQ);
+    own.c = wsT<int>(c, "gh_ownC", slotsQ);
+    own.n = wsT<int>(c, "gh_ownN", slotsQ);
+    int *ownP = wsT<int>(c, "gh_ownP", slotsP);
+    int *ownL = wsT<int>(c, "gh_ownL", slotsQ);
+    fill_i32(c, own.a, -1, slotsQ);
+    fill_i32(c, own.c, -1, slotsQ);
+    fill_i32(c, own.n, -1, slotsQ);
+    fill_i32(c, ownP, -1, slotsP);
+    fill_i32(c, ownL, -1, slotsQ);
+    if (edgesP > 0) {
+      hipLaunchKernelGGL(k_p_parents, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream, edgesP,
+                         mapP, (const long long *)offP, ringP, (const long long *)psP2, pm.status,
+                         pm, in.bCoords, in.oCoords, s);
+      HIP_CHECK(hipGetLastError());
+    }
+    if (nActP > 0) {
+      {
+        LaunchTimer t(c, "gh_calc_own");
+        hipLaunchKernelGGL(k_calc<true>, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream,
+                           nActP, (const int *)actP, mapP, (const long long *)offP,
+                           (const long long *)offQ, ringP, ringQ, (const long long *)psP1,
+                           (const long long *)psP2, (const long long *)psQ1,
+                           (const long long *)psQ2, pm.status, pm, in.bCoords, in.oCoords, nmap, s,
+                           own, (const int *)lstAllP);
+        HIP_CHECK(hipGetLastError());
+      }
+      LaunchTimer t(c, "gh_calc");
+      hipLaunchKernelGGL(k_calc<false>, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream, nActP,
+                         (const int *)actP, mapP, (const long long *)offP, (const long long *)offQ,
+                         ringP, ringQ, (const long long *)psP1, (const long long *)psP2,
+                         (const long long *)psQ1, (const long long *)psQ2, pm.status, pm,
+                         in.bCoords, in.oCoords, nmap, s, own, (const int *)lstAllP);
+      HIP_CHECK(hipGetLastError());
+    }
+    // per-edge alpha sorts, all P edges before the Q edges (the Q sort reads
+    // the back links the P sort moves)
+    {
+      // keys, segmented sort and moves of one side's long runs
+      auto long_runs = [&](bool qside, int n, long total, const int *list, const int *len,
+                           const long long *ps2, const int *map, const long long *off) {
+        long long *runOff = wsT<long long>(c, qside ? "gh_runOffQ" : "gh_runOffP", n + 1);
+        scan_i32_to_i64(c, len, runOff, n);
+        uint64_t *k0 = wsT<uint64_t>(c, "gh_runK0", total), *k1 = wsT<uint64_t>(c, "gh_runK1", total);
+        int *v0 = wsT<int>(c, "gh_runV0", total), *v1 = wsT<int>(c, "gh_runV1", total);
+        const unsigned blocks = (unsigned)(n < 4096 ? n : 4096);
+        hipLaunchKernelGGL(k_long_run_keys<false>, dim3(blocks), dim3(256), 0, c.stream, n, list,
+                           (const long long *)runOff, ps2,
+                           (const int *)(qside ? s.aQ2 : s.aP2),
+                           (const int *)(qside ? s.nbQ2 : s.nbP2), ownP, k0, v0);
+        HIP_CHECK(hipGetLastError());
+        return [=, &c, &s, &own](void) {
+          int bits = 32;
+          for (int x = n - 1; x > 0; x >>= 1) bits++;
+          radix_sort_u64(c, k0, v0, k1, v1, total, bits);
+          if (qside)
+            hipLaunchKernelGGL(k_long_run_apply<true>, dim3(grid_for(total, 256)), dim3(256), 0,
+                               c.stream, total, (const uint64_t *)k1, (const int *)v1, list,
+                               (const long long *)runOff, ps2, map, off, s, own,
+                               (const int *)ownP);
+          else
+            hipLaunchKernelGGL(k_long_run_apply<false>, dim3(grid_for(total, 256)), dim3(256), 0,
+                               c.stream, total, (const uint64_t *)k1, (const int *)v1, list,
+                               (const long long *)runOff, ps2, map, off, s, own,
+                               (const int *)ownP);
+          HIP_CHECK(hipGetLastError());
+        };
+      };
+      if (nShortP > 0) {
+        LaunchTimer t(c, "gh_sort_p");
+        hipLaunchKernelGGL(k_sort_p_short, dim3(grid_for(32 * nShortP, 256)), dim3(256), 0,
+                           c.stream, nShortP, (const int *)listSP, mapP, (const long long *)offP,
+                           (const long long *)psP2, pm.status, s, own);
+        HIP_CHECK(hipGetLastError());
+      }
+      if (nMedP > 0) {
+        LaunchTimer t(c, "gh_sort_runs");
+        hipLaunchKernelGGL(k_run_sort_block<false>, dim3(nMedP < 8192 ? nMedP : 8192), dim3(256),
+                           0, c.stream, nMedP, (const int *)listMP, (const long long *)psP2, mapP,
+                           (const long long *)offP, s, own, (const int *)ownP);
+        HIP_CHECK(hipGetLastError());
+      }
+      if (nRunsP > 0)
+        long_runs(false, nRunsP, runTotP, listP, lenP, (const long long *)psP2, mapP,
+                  (const long long *)offP)();
+      if (edgesQ > 0) {
+        if (nShortQ > 0)
+          hipLaunchKernelGGL(k_sort_q_short<true>, dim3(grid_for(32 * nShortQ, 256)), dim3(256), 0,
+                             c.stream, nShortQ, (const int *)listSQ, mapQ,
+                             (const long long *)psQ2, pm.status, s, ownP);
+        if (nMedQ > 0)
+          hipLaunchKernelGGL(k_run_own, dim3(nMedQ < 8192 ? nMedQ : 8192), dim3(256), 0, c.stream,
+                             nMedQ, (const int *)listMQ, (const long long *)psQ2,
+                             (const int *)s.nbQ2, ownP);
+        if (nRunsQ > 0)
+          hipLaunchKernelGGL(k_run_own, dim3(nRunsQ < 8192 ? nRunsQ : 8192), dim3(256), 0,
+                             c.stream, nRunsQ, (const int *)listQ, (const long long *)psQ2,
+                             (const int *)s.nbQ2, ownP);
+        HIP_CHECK(hipGetLastError());
+        auto finish = nRunsQ > 0 ? long_runs(true, nRunsQ, runTotQ, listQ, lenQ,
+                                             (const long long *)psQ2, mapQ,
+                                             (const long long *)offQ)
+                                 : std::function<void()>();
+        {
+          LaunchTimer t(c, "gh_sort_q");
+          if (nShortQ > 0)
+            hipLaunchKernelGGL(k_sort_q_short<false>, dim3(grid_for(32 * nShortQ, 256)),
+                               dim3(256), 0, c.stream, nShortQ, (const int *)listSQ, mapQ,
+                               (const long long *)psQ2, pm.status, s, ownP);
+          HIP_CHECK(hipGetLastError());
+        }
+        if (nMedQ > 0) {
+          LaunchTimer t(c, "gh_sort_runs");
+          hipLaunchKernelGGL(k_run_sort_block<true>, dim3(nMedQ < 8192 ? nMedQ : 8192), dim3(256),
+                             0, c.stream, nMedQ, (const int *)listMQ, (const long long *)psQ2,
+                             mapQ, (const long long *)offQ, s, own, (const int *)ownP);
+          HIP_CHECK(hipGetLastError());
+        }
+        if (finish) finish();
+      }
+    }
+    if (nActP > 0) {
+      hipLaunchKernelGGL(k_init_label<true>, dim3(grid_for(4 * nActP, 256)), dim3(256), 0, c.stream,
+                         nActP, (const int *)actP, mapP, (const long long *)offP,
+                         (const long long *)offQ, (const long long *)psP2, (const long long *)psQ2,
+                         pm.status, s, ownL);
+      HIP_CHECK(hipGetLastError());
+      LaunchTimer t(c, "gh_init_label");
+      hipLaunchKernelGGL(k_init_label<false>, dim3(grid_for(4 * nActP, 256)), dim3(256), 0, c.stream,
+                         nActP, (const int *)actP, mapP, (const long long *)offP,
+                         (const long long *)offQ, (const long long *)psP2, (const long long *)psQ2,
+                         pm.status, s, ownL);
+      HIP_CHECK(hipGetLastError());
+    }
   }
 
   // capacities of the per-pair node, output and component regions
@@ -2793,6 +3473,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   ta.compLen = wsT<int>(c, "gh_compLen", comps);
   ta.xyTmp = wsT<double>(c, "gh_xyTmp", 2 * nodes);
   ta.cleanup = opt.cleanup;
+  ta.explicitLinks = cpuRoute ? 1 : 0;
   if (opt.label_mode == 1) {
     LaunchTimer t(c, "gh_label_trace");
     hipLaunchKernelGGL(k_label_trace, dim3(grid_for(M, 64)), dim3(64), 0, c.stream, M,

@@ -301,6 +301,7 @@ void calculateIntersections(coord_t *baseCoord, coord_t *overlayCoords, int size
     gcmf::ClipOptions opt;
     opt.normalize = 0;
     opt.noswap = 1;
+    opt.route = 0;  // the phase-1 arrays of calculateIntersections
     gcmf::ClipResult r;
     gcmf::Phase1View p;
     gcmf::clip_pairs(c, in, dp, 1, opt, r, &p);

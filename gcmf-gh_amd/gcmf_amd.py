@@ -214,6 +214,7 @@ LP_SKIPPED, LP_SEGMENTS, LP_SEG_CLEANUP, LP_NO_CROSSING, LP_SEQUENTIAL, LP_FAILE
 CLIP_NORMALIZE = 1
 CLIP_NO_CLEANUP = 2
 CLIP_SEQ_LABEL = 4
+CLIP_GPU_ROUTE = 8
 
 
 def hip_lib():

@@ -105,6 +105,11 @@ int gcmf_join_get(gcmf_ctx *ctx, int field, void *dst, long count);
 #define GCMF_CLIP_NORMALIZE 1 /* per-pair CMBR normalisation of predicates */
 #define GCMF_CLIP_NO_CLEANUP 2 /* skip cleanUpResult (polyclip.h:791)      */
 #define GCMF_CLIP_SEQ_LABEL 4  /* label/trace one thread per pair (checking) */
+/* Phase 1 as the reference's GPU route (calculateIntersections,
+ * gppolyclip_mDIV512.cu:1298) instead of its CPU route (computeIntersections,
+ * polyclip.h:188-281).  The two agree in general position; on degenerate
+ * inputs only the CPU route is well defined. */
+#define GCMF_CLIP_GPU_ROUTE 8
 
 typedef struct {
   long pairs, ok, invalid, disjoint, failed;

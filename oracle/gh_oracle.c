@@ -1377,3 +1377,99 @@ int orc_debug_lists(const double *P, int nP, const double *Q, int nQ, int normal
   gh_free(&g);
   return 0;
 }
+
+/* ----------------------------------------------------------------------
+ * Soundness check of the batched clip's CPU-route edge-pair prefilter (the
+ * skip rule of k_count_cpu and k_partner_rescan_cpu, and the edge relevance
+ * test of k_count_cpu in gcmf-gh_amd/csrc/gh.hip).  The rule is restated
+ * with the same constants and evaluated on every pair of edges next to
+ * intersect(); out[0] pairs, out[1] pairs skipped, out[2] pairs skipped
+ * although intersect() is nonzero (must be 0), out[3] edges dropped by the
+ * relevance test, out[4] dropped edges that have a nonzero pair (must be 0).
+ * -------------------------------------------------------------------- */
+static double edge_gap_c(pt a1, pt a2, pt b1, pt b2) {
+  double gx = fmax(fmin(b1.x, b2.x) - fmax(a1.x, a2.x), fmin(a1.x, a2.x) - fmax(b1.x, b2.x));
+  double gy = fmax(fmin(b1.y, b2.y) - fmax(a1.y, a2.y), fmin(a1.y, a2.y) - fmax(b1.y, b2.y));
+  return fmax(gx, gy);
+}
+
+int orc_prefilter_check(const double *P, int nP, const double *Q, int nQ, int normalize,
+                        long long *out) {
+  double cmbr[4], norm[3];
+  for (int i = 0; i < 5; i++) out[i] = 0;
+  if (!pair_frame(P, nP, Q, nQ, normalize, cmbr, norm)) return 2;
+  int n[2] = {nP, nQ};
+  pt *r[2];
+  double *L[2], *m[2], bx[2][4], minL[2], maxM[2];
+  for (int s = 0; s < 2; s++) {
+    const double *c = s ? Q : P;
+    r[s] = (pt *)malloc(sizeof(pt) * (size_t)n[s]);
+    L[s] = (double *)malloc(sizeof(double) * (size_t)n[s]);
+    m[s] = (double *)malloc(sizeof(double) * (size_t)n[s]);
+    bx[s][0] = bx[s][1] = INFINITY;
+    bx[s][2] = bx[s][3] = -INFINITY;
+    for (int i = 0; i < n[s]; i++) {
+      r[s][i] = tf(c[2 * i], c[2 * i + 1], norm);
+      bx[s][0] = fmin(bx[s][0], r[s][i].x);
+      bx[s][1] = fmin(bx[s][1], r[s][i].y);
+      bx[s][2] = fmax(bx[s][2], r[s][i].x);
+      bx[s][3] = fmax(bx[s][3], r[s][i].y);
+    }
+  }
+  double w = fmax(bx[0][2], bx[1][2]) - fmin(bx[0][0], bx[1][0]);
+  double h = fmax(bx[0][3], bx[1][3]) - fmin(bx[0][1], bx[1][1]);
+  double D2 = w * w + h * h, D = sqrt(D2), u = 1.1102230246251565e-16;
+  double dl = 8.0 * u * D2, eA = 3e-8 * D2 + 1e-15;
+  double kA = 1.5 * (GH_EPS + eA), kB = 1.5 * 2.2 * (GH_EPS + 2.0 * dl), c0 = 16.0 * u * D;
+  double T = 4.5 * (GH_EPS + 2.0 * dl);
+  for (int s = 0; s < 2; s++) {
+    minL[s] = INFINITY;
+    maxM[s] = 0.0;
+    for (int i = 0; i < n[s]; i++) {
+      pt a = r[s][i], b = r[s][i + 1 == n[s] ? 0 : i + 1];
+      double dx = b.x - a.x, dy = b.y - a.y;
+      L[s][i] = sqrt(dx * dx + dy * dy);
+      m[s][i] = s ? kA * L[s][i] + kB / L[s][i] + c0 : kA * L[s][i] + c0;
+      minL[s] = fmin(minL[s], L[s][i]);
+      maxM[s] = fmax(maxM[s], m[s][i]);
+    }
+  }
+  char *hasP = (char *)calloc((size_t)nP, 1), *hasQ = (char *)calloc((size_t)nQ, 1);
+  for (int p = 0; p < nP; p++) {
+    pt P1 = r[0][p], P2 = r[0][p + 1 == nP ? 0 : p + 1];
+    for (int q = 0; q < nQ; q++) {
+      pt Q1 = r[1][q], Q2 = r[1][q + 1 == nQ ? 0 : q + 1];
+      double gap = edge_gap_c(P1, P2, Q1, Q2);
+      int skip = gap > 0.0 && gap > m[0][p] + m[1][q] && !(L[0][p] * L[1][q] < T);
+      double al, be;
+      int t = itype(P1, P2, Q1, Q2, &al, &be);
+      out[0]++;
+      if (skip) out[1]++;
+      if (skip && t) out[2]++;
+      if (t) hasP[p] = hasQ[q] = 1;
+    }
+  }
+  for (int s = 0; s < 2; s++) {
+    const double *ob = bx[1 - s];
+    for (int i = 0; i < n[s]; i++) {
+      pt a = r[s][i], b = r[s][i + 1 == n[s] ? 0 : i + 1];
+      double x0 = fmin(a.x, b.x), y0 = fmin(a.y, b.y), x1 = fmax(a.x, b.x), y1 = fmax(a.y, b.y);
+      double ogap = fmax(fmax(ob[0] - x1, x0 - ob[2]), fmax(ob[1] - y1, y0 - ob[3]));
+      int tiny = L[s][i] * minL[1 - s] < T;
+      double G = m[s][i] + maxM[1 - s];
+      int flag = tiny || !(ogap > G);
+      if (!flag) {
+        out[3]++;
+        if ((s ? hasQ : hasP)[i]) out[4]++;
+      }
+    }
+  }
+  for (int s = 0; s < 2; s++) {
+    free(r[s]);
+    free(L[s]);
+    free(m[s]);
+  }
+  free(hasP);
+  free(hasQ);
+  return 0;
+}

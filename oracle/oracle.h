@@ -74,6 +74,11 @@ int orc_clip_gpu_route(const double *P, int nP, const double *Q, int nQ,
 /* Independent area of P intersect Q: boundary integral over the pieces of
  * each boundary inside the other polygon (general position only). */
 double orc_area_boundary(const double *P, int nP, const double *Q, int nQ);
+/* soundness of the batched clip's CPU-route edge-pair prefilter: out[5] =
+ * pairs, skipped, skipped with a nonzero type, edges dropped, dropped edges
+ * with a nonzero pair; returns 2 for disjoint MBRs */
+int orc_prefilter_check(const double *P, int nP, const double *Q, int nQ, int normalize,
+                        long long *out);
 
 /* Shoelace area of the components in r about the origin (ox, oy),
  * compensated summation.  Signed per component, summed as absolute values. */

@@ -57,6 +57,7 @@ def lib():
         L.orc_phase1_free.argtypes = [C.POINTER(Phase1)]
         L.orc_area_boundary.argtypes = [f64p, C.c_int, f64p, C.c_int]
         L.orc_area_boundary.restype = C.c_double
+        L.orc_prefilter_check.argtypes = [f64p, C.c_int, f64p, C.c_int, C.c_int, i64p]
         L.orc_result_area.argtypes = [C.POINTER(Result), C.c_double, C.c_double]
         L.orc_result_area.restype = C.c_double
         L.orc_intersect_type.argtypes = [f64p, f64p, f64p, f64p, C.POINTER(C.c_double),
@@ -159,6 +160,15 @@ def phase1(P, Q, normalize=True):
             neighborQ=g(t.neighborQ, (cQ,))[:t.cQ].copy())
     L.orc_phase1_free(C.byref(t))
     return out
+
+
+def prefilter_check(P, Q, normalize=True):
+    """Counts of the CPU-route prefilter check: pairs, skipped, unsound
+    skips, dropped edges, unsound edge drops (see orc_prefilter_check)."""
+    P, Q = _ring(P), _ring(Q)
+    out = np.zeros(5, np.int64)
+    rc = lib().orc_prefilter_check(P, len(P) // 2, Q, len(Q) // 2, int(normalize), out)
+    return rc, dict(zip(["pairs", "skipped", "bad_skips", "dropped", "bad_drops"], out.tolist()))
 
 
 def area_boundary(P, Q):

@@ -1,11 +1,14 @@
 """GPU parity of the batched Greiner-Hormann clip (gh.hip) against the oracle.
 
-Phase-1 arrays (counts, scans, CMBR flags, neighbour map, intersection
-coordinates, alpha keys, initial labels, neighbour links) must be bit-exact
-against the restatement of gppolyclip_mDIV512.cu; clipped polygons must be
-identical (same components, vertex counts and coordinates) to the CPU route
-of polyclip.h; traced areas must agree with the independent boundary-integral
-area within 1e-12 relative.
+The clip's default phase 1 is the reference's CPU route (computeIntersections,
+polyclip.h:188-281): its results must be identical (same status, components,
+vertex counts and coordinates) to the oracle's CPU route on every input,
+degenerate ones included.  With GCMF_CLIP_GPU_ROUTE the phase-1 arrays
+(counts, scans, CMBR flags, neighbour map, intersection coordinates, alpha
+keys, initial labels, neighbour links) must be bit-exact against the
+restatement of gppolyclip_mDIV512.cu and the results equal to the GPU-route
+oracle.  Traced areas must agree with the independent boundary-integral area
+within 1e-12 relative.
 """
 import numpy as np
 import pytest
@@ -111,19 +114,23 @@ def check_phase1(gctx, st, cases, normalize):
         np.testing.assert_array_equal(nm, ref["neighborMapQ"], err_msg=name + " nmap")
 
 
-def check_results(gctx, st, cases, mode):
-    """Every pair equals the GPU-route oracle (the reference's GPU phase 1
-    spliced and labelled as ghcuda.cpp does); pairs in general position also
-    equal the CPU route of polyclip.h.  On degenerate (grid-snapped) inputs
-    the reference's two routes can differ: the GPU route's phase-1 lists
-    order coincident intersections by truncated alpha, the CPU route by
-    exact insertion."""
+def check_results(gctx, st, cases, mode, gpu_route=False):
+    """CPU route (default): every pair equals the oracle's CPU route of
+    polyclip.h.  GPU route: every pair equals the GPU-route oracle (the
+    reference's GPU phase 1 spliced and labelled as ghcuda.cpp does), and
+    pairs in general position also equal the CPU route.  On degenerate
+    (grid-snapped) inputs the reference's two routes can differ: the GPU
+    route's phase-1 lists order coincident intersections by truncated alpha,
+    the CPU route by exact insertion."""
     status, results = gctx.clip_results(st)
     for k, (name, P0, Q0) in enumerate(cases):
         P, Q = oriented(P0, Q0)
-        refs = [O.clip_gpu_route(P, Q, mode)]
-        if not name.startswith("snap"):
-            refs.append(O.clip_cpu(P, Q, mode))
+        if gpu_route:
+            refs = [O.clip_gpu_route(P, Q, mode)]
+            if not name.startswith("snap"):
+                refs.append(O.clip_cpu(P, Q, mode))
+        else:
+            refs = [O.clip_cpu(P, Q, mode)]
         for s_ref, comps, _ in refs:
             if s_ref == 2:
                 assert status[k] == 2, name
@@ -138,13 +145,13 @@ def check_results(gctx, st, cases, mode):
 @pytest.mark.parametrize("normalize", [1, 0])
 def test_phase1_fig_fixtures(gctx, normalize):
     cases = fig_cases()
-    st = run_batch(gctx, cases, G.CLIP_NORMALIZE if normalize else 0)
+    st = run_batch(gctx, cases, (G.CLIP_NORMALIZE if normalize else 0) | G.CLIP_GPU_ROUTE)
     check_phase1(gctx, st, cases, normalize)
 
 
 def test_phase1_random_batch(gctx):
     cases = random_cases(64)
-    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
     assert st["ok"] == 64
     check_phase1(gctx, st, cases, 1)
 
@@ -152,7 +159,7 @@ def test_phase1_random_batch(gctx):
 def test_phase1_c1_pair(gctx):
     P, Q = G.gen_single_pair(0, 0, 2000, 1500)
     cases = [("c1", P, Q)]
-    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
     check_phase1(gctx, st, cases, 1)
     assert st["n_intersections"] == O.phase1(P, Q, 1)["psP1"][-1]
 
@@ -163,20 +170,25 @@ def test_phase1_grid_snapped(gctx):
     for s in range(6):
         P, Q = G.gen_single_pair(1, 50 + s, 3000, 400, snap=1e-2)
         cases.append(("snap%d" % s, P, Q))
-    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
     check_phase1(gctx, st, cases, 1)
+    check_results(gctx, st, cases, 1, gpu_route=True)
+
+
+def snapped_cases(seed0, n=4, grid=1e-2):
+    out = []
+    for s in range(n):
+        P, Q = G.gen_single_pair(1, seed0 + s, 1500, 300, snap=grid)
+        out.append(("snap%d" % s, P, Q))
+    return out
 
 
 @pytest.mark.parametrize("seq", [0, 1])
 @pytest.mark.parametrize("mode", [1, 3, 0])
 def test_clip_matches_cpu_route(gctx, mode, seq):
-    # seq=1 runs labelling/tracing one thread per pair, seq=0 the wave path
-    cases = fig_cases() + random_cases(48, seed0=77)
-    snap = []
-    for s in range(4):
-        P, Q = G.gen_single_pair(1, 90 + s, 1500, 300, snap=1e-2)
-        snap.append(("snap%d" % s, P, Q))
-    cases += snap
+    # seq=1 runs labelling/tracing one thread per pair, seq=0 the wave path;
+    # grid-snapped pairs (shared vertices, collinear edges) included
+    cases = fig_cases() + random_cases(48, seed0=77) + snapped_cases(90)
     flags = (G.CLIP_NORMALIZE if mode & 1 else 0) | (G.CLIP_NO_CLEANUP if mode & 2 else 0)
     flags |= G.CLIP_SEQ_LABEL if seq else 0
     st = run_batch(gctx, cases, flags)
@@ -189,6 +201,34 @@ def test_clip_matches_cpu_route(gctx, mode, seq):
             if name.startswith("rand"):
                 assert path[k] in (G.LP_SEGMENTS, G.LP_SEG_CLEANUP, G.LP_NO_CROSSING), (name,
                                                                                       path[k])
+
+
+def test_clip_gpu_route_results(gctx):
+    """The GPU route gives the GPU-route oracle's results (and the CPU
+    route's in general position), through the wave path."""
+    cases = fig_cases() + random_cases(48, seed0=77) + snapped_cases(90)
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
+    check_results(gctx, st, cases, 1, gpu_route=True)
+
+
+def test_clip_degenerate_wavy(gctx):
+    """A 20000 x 5000 pair of the large-pair shape (radii from six
+    random-phase sinusoids plus noise) and its copy snapped to a 1e-3 grid
+    (shared vertices, collinear edges): identical to the CPU route; the
+    unsnapped pair's traced area equals the boundary integral."""
+    P, Q = G.gen_single_pair(1, 0, 20000, 5000)
+    Ps, Qs = G.gen_single_pair(1, 0, 20000, 5000, snap=1e-3)
+    cases = [("wavy", P, Q), ("wavy_snap", Ps, Qs)]
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    assert st["ok"] == 2
+    check_results(gctx, st, cases, 1)
+    st = run_batch(gctx, cases[:1], G.CLIP_NORMALIZE | G.CLIP_NO_CLEANUP)
+    _, res = gctx.clip_results(st)
+    A = O.area_boundary(P, Q)
+    allxy = np.concatenate([P, Q])
+    ox, oy = 0.5 * (allxy[:, 0].min() + allxy[:, 0].max()), 0.5 * (
+        allxy[:, 1].min() + allxy[:, 1].max())
+    assert abs(O.comps_area(res[0], ox, oy) - A) <= AREA_RTOL * A
 
 
 def test_clip_area_oracle(gctx):
@@ -281,27 +321,30 @@ def test_clip_long_runs(gctx):
         ref = O.phase1(Po, Qo, 1)
         side = "psP2" if name == "longP" else "psQ2"
         assert int(np.diff(ref[side]).max()) - 1 > (2048 if name == "longQ3290" else 32), name
-    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
     check_phase1(gctx, st, cases, 1)
+    check_results(gctx, st, cases, 1, gpu_route=True)
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
     check_results(gctx, st, cases, 1)
     for p in gctx.clip_get("label_path", len(cases)):
         assert p in (G.LP_SEGMENTS, G.LP_SEG_CLEANUP)
 
 
-def test_clip_batched_rare_pairs(gctx):
+@pytest.mark.parametrize("gpu_route", [0, 1])
+def test_clip_batched_rare_pairs(gctx, gpu_route):
     """Pairs from the batched workload that reach rare paths (a trace that
     does not close, BOUNCING intersections among thousands of crossings)
-    equal the GPU-route oracle, status included, and the BOUNCING pairs stay
-    on the parallel labelling path."""
+    equal the oracle of the route, status included, and the BOUNCING pairs
+    stay on the parallel labelling path."""
     z = np.load(GOLDEN + "/batched_pairs.npz")
     names = ["p4596", "p156581", "p73779", "p176719"]
     cases = [(n, z[n + "_P"], z[n + "_Q"]) for n in names]
-    st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE | (G.CLIP_GPU_ROUTE if gpu_route else 0))
     status, results = gctx.clip_results(st)
     path = gctx.clip_get("label_path", len(cases))
     for k, (name, P0, Q0) in enumerate(cases):
         P, Q = oriented(P0, Q0)
-        s_ref, comps, _ = O.clip_gpu_route(P, Q, 1)
+        s_ref, comps, _ = (O.clip_gpu_route if gpu_route else O.clip_cpu)(P, Q, 1)
         assert status[k] == s_ref, (name, status[k], s_ref)
         assert [len(c) for c in results[k]] == [len(c) for c in comps], name
         for a, b in zip(results[k], comps):

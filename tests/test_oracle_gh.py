@@ -214,3 +214,74 @@ def test_phase1_invariants():
     for e in range(len(P)):
         a = t["alphaP"][t["psP2"][e] + 1:t["psP2"][e + 1]]
         assert np.all(np.diff(a) >= 0)
+
+
+def _prefilter_cases():
+    """Pairs that stress the CPU-route edge-pair prefilter of the batched
+    clip: general position, grid-snapped (shared vertices, collinear edges),
+    near-duplicate vertices (edges down to 1e-12), shared and identical
+    rings, perpendicular tiny edges far apart along one line (where the
+    collinear branch of intersect() reaches far), a huge ring around a tiny
+    one (large normalised coordinates), and the large-pair shape."""
+    rs = np.random.RandomState(0)
+    out = []
+    for s in range(6):
+        out.append(("rand%d" % s, G.gen_single_pair(0, s, int(rs.randint(8, 300)),
+                                                   int(rs.randint(8, 300))), 1))
+    for grid in (1e-2, 1e-3, 1e-4):
+        out.append(("snap%g" % grid, G.gen_single_pair(1, 50, 1200, 300, snap=grid), 1))
+
+    def dups(P, scale):
+        keep = []
+        for v in P:
+            keep.append(v)
+            if rs.rand() < 0.3:
+                keep.append(v + rs.randn(2) * scale)
+        return np.array(keep)
+
+    for scale in (1e-12, 1e-9, 1e-6):
+        P, Q = G.gen_single_pair(0, 7, 300, 150)
+        for norm in (1, 0):
+            out.append(("dups%g_%d" % (scale, norm), (dups(P, scale), dups(Q, scale)), norm))
+    P, _ = G.gen_single_pair(0, 11, 240, 8)
+    out.append(("shared_sub", (P, P[::3].copy()), 1))
+    mid = 0.5 * (P[:-1] + P[1:])
+    out.append(("shared_mid", (P, np.concatenate([P[:120], mid[120:]])), 1))
+    out.append(("identical", (P, P.copy()), 1))
+    P = np.array([[0, 0], [0, 1e-3], [-1, 1], [-1, -1]], float)
+    Q = np.array([[1000, 5e-4], [1000 + 1e-5, 5e-4], [1000, 2], [-0.5, 2], [-0.5, -2],
+                  [1000, -2]], float)
+    out.append(("perp_far_raw", (P, Q), 0))
+    out.append(("perp_far", (P, Q), 1))
+    P, Q = G.gen_single_pair(0, 3, 300, 200)
+    out.append(("huge_vs_tiny", (P * 1e4, Q * 1e-2 + 0.3), 1))
+    out.append(("wavy", G.gen_single_pair(1, 0, 3000, 800), 1))
+    out.append(("wavy_snap", G.gen_single_pair(1, 0, 3000, 800, snap=1e-3), 1))
+    return out
+
+
+@pytest.mark.parametrize("name,pq,norm", _prefilter_cases(), ids=lambda v: v if isinstance(
+    v, str) else "")
+def test_cpu_route_prefilter_is_sound(name, pq, norm):
+    """The batched clip's CPU route skips an edge pair only when intersect()
+    returns 0 for it, and drops an edge only when it has no nonzero pair:
+    checked against intersect() on every pair of edges."""
+    P, Q = pq
+    if len(P) < len(Q):
+        P, Q = Q, P
+    rc, d = O.prefilter_check(P, Q, norm)
+    assert rc == 0, name
+    assert d["bad_skips"] == 0 and d["bad_drops"] == 0, (name, d)
+    if name.startswith(("rand", "snap", "wavy", "identical")):
+        assert d["skipped"] > 0.9 * d["pairs"], (name, d)  # the filter does its job
+
+
+def test_prefilter_check_catches_a_wrong_rule():
+    """The check itself is not vacuous: on the perpendicular far-apart edges
+    the collinear branch of intersect() classifies a pair whose boxes are
+    1000 apart, which a plain closed-box test (the GPU route's lsmf) skips."""
+    P = np.array([[0, 0], [0, 1e-3], [-1, 1], [-1, -1]], float)
+    Q = np.array([[1000, 5e-4], [1000 + 1e-5, 5e-4], [1000, 2], [-0.5, 2], [-0.5, -2],
+                  [1000, -2]], float)
+    t, al, be = O.intersect_type(P[0], P[1], Q[0], Q[1])
+    assert t == 7 and 0 < al < 1  # Q1's foot lies on P; Q1 itself is 1000 away
