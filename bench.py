@@ -109,10 +109,21 @@ def main():
     ap.add_argument("--n", type=int, default=20000, help="polygons per layer")
     ap.add_argument("--cpu-sample", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["join-clip", "batched"], default="join-clip",
+    ap.add_argument("--workload", choices=["join-clip", "batched", "large"],
+                    default="join-clip",
                     help="join-clip: the two-layer join + clip (default); batched: "
-                         "independent uneven pairs clipped in one pass")
+                         "independent uneven pairs clipped in one pass; large: one pair of "
+                         "1e6 x 1e5 vertices")
+    ap.add_argument("--large-n", type=int, nargs=2, default=[1000000, 100000],
+                    help="ring sizes for --workload large")
+    ap.add_argument("--snap", type=float, default=0.0,
+                    help="grid for --workload large (1e-3: the degenerate variant)")
+    ap.add_argument("--reference-eps", action="store_true",
+                    help="--workload large without GCMF_CLIP_SCALE_EDGES (the reference's "
+                         "absolute epsilon in the CMBR frame)")
     ap.add_argument("--pairs", type=int, default=200000, help="pairs for --workload batched")
+    ap.add_argument("--per-pair-sample", type=int, default=2000,
+                    help="pairs clipped one call each for the per-pair comparison (batched)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one plain 256 MiB device-to-device copy and exit (the reference "
                          "reading for the HBM byte counters FETCH_SIZE / WRITE_SIZE)")
@@ -125,6 +136,8 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         return batched(args)
+    if args.workload == "large":
+        return large(args)
 
     import gcmf_amd as G
     print("generating layers ...", file=sys.stderr, flush=True)
@@ -237,6 +250,34 @@ def batched(args):
     step()
     ctx.synchronize()
     ctx.set_timing(False)
+    kernel_ms = {k: round(ctx.timer(k)[0], 4) for k in (
+        "gh_pair_setup", "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own", "gh_calc",
+        "gh_sort_p", "gh_sort_runs", "gh_sort_q", "gh_init_label", "gh_cpu_fill",
+        "gh_cpu_sort", "gh_cpu_link", "gh_label_trace", "gh_gather_out", "sort_hist",
+        "sort_scatter")}
+    # the plain per-pair path (one clip call per pair, as the reference's
+    # ghcuda loop runs GH_CUDA per pair) on an evenly strided sample of the
+    # same pairs; each pair's result must match its result in the batch
+    nverts_batch = ctx.clip_get("nverts", args.pairs)
+    status_batch = ctx.clip_get("status", args.pairs)
+    stride = max(1, args.pairs // max(1, args.per_pair_sample))
+    sample = list(range(0, args.pairs, stride))[:args.per_pair_sample]
+    mism = 0
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in sample:
+        st = G.ClipStats()
+        ctx._check(L.gcmf_clip(ctx.h, C.byref(dP.desc), C.byref(dQ.desc),
+                               C.c_void_p(d_pairs.ptr.value + 8 * i), 1, G.CLIP_NORMALIZE,
+                               C.byref(st)), "gcmf_clip")
+        s1 = int(ctx.clip_get("status", 1)[0])
+        if s1 != int(status_batch[i]) or (s1 == 0 and st.total_verts != int(nverts_batch[i])):
+            mism += 1
+    ctx.synchronize()
+    t_pp = time.perf_counter() - t0
+    per_pair_rate = len(sample) / t_pp
+    print("per-pair path: %d pairs in %.2f s, %d mismatches" % (len(sample), t_pp, mism),
+          file=sys.stderr, flush=True)
     kernels = ["gh_pair_setup", "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own",
                "gh_calc", "gh_sort_p", "gh_sort_runs", "gh_sort_q", "gh_init_label",
                "gh_cpu_fill", "gh_cpu_sort", "gh_cpu_link",
@@ -253,12 +294,83 @@ def batched(args):
                        "result_vertices": cs["total_verts"]},
             "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 4),
             "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 4),
-            "kernel_ms_per_step": {k: round(ctx.timer(k)[0], 4) for k in kernels},
+            "kernel_ms_per_step": kernel_ms,
+            "per_pair_path": {"pairs_per_s": round(per_pair_rate, 2),
+                              "sample": "%d pairs (every %d-th), one gcmf_clip call each"
+                                        % (len(sample), stride),
+                              "result_mismatches_vs_batch": mism,
+                              "batched_speedup": round(args.pairs / (elapsed / args.steps)
+                                                       / per_pair_rate, 2)},
             "roofline": roofline_line(roof_ms, roof_n, cs["n_eval"],
                                       traffic_bytes("batched", args))}
     d_pairs.free()
     dP.free()
     dQ.free()
+    ctx.close()
+    print(json.dumps(line))
+
+
+def large(args):
+    """One large pair: a base ring of 1e6 and a clip ring of 1e5 vertices,
+    radii 1 + 0.2 (1/6) sum_k sin(2^k t + phi_k) + 0.02 U(-1, 1) about (0, 0)
+    and (0.3, 0.2), optionally snapped to a grid (consecutive duplicates
+    dropped).  One step = one clip of the pair."""
+    import gcmf_amd as G
+    nP, nQ = args.large_n
+    P, Q = G.gen_single_pair(1, 0, nP, nQ, snap=args.snap)
+    ctx = G.Context(0)
+    base, over = G.Layer.from_rings([P]), G.Layer.from_rings([Q])
+    db, do = G.DeviceLayer(ctx, base), G.DeviceLayer(ctx, over)
+    pairs = np.array([[0, 0]], np.int32)
+    flags = G.CLIP_NORMALIZE | (0 if args.reference_eps else G.CLIP_SCALE_EDGES)
+    for _ in range(args.warmup):
+        cs = ctx.clip(db, do, pairs, flags)
+    ctx.timer_reset()
+    ctx.set_timing(True, only=ROOF_KERNEL)
+    lat = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        s0 = time.perf_counter()
+        cs = ctx.clip(db, do, pairs, flags)
+        lat.append(time.perf_counter() - s0)
+        print("step %d: %.2f ms" % (i, 1e3 * lat[-1]), file=sys.stderr, flush=True)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    roof_ms, roof_n = ctx.timer(ROOF_KERNEL)
+    ctx.timer_reset()
+    ctx.set_timing(True)
+    ctx.clip(db, do, pairs, flags)
+    ctx.synchronize()
+    ctx.set_timing(False)
+    kernel_ms = {k: round(ctx.timer(k)[0], 4) for k in (
+        "gh_pair_setup", "gh_pair_fill", "gh_count", "gh_cpu_fill", "gh_cpu_sort",
+        "gh_sort_runs", "gh_cpu_link", "gh_label_trace", "gh_gather_out", "sort_hist",
+        "sort_scatter", "scan_reduce", "scan_downsweep")}
+    path = int(ctx.clip_get("label_path", 1)[0])
+    status = int(ctx.clip_get("status", 1)[0])
+    line = {"metric": METRIC, "value": round(args.steps / elapsed, 4), "unit": "pairs/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded generator, no dataset)",
+            "config": {"workload": "one pair of %d x %d vertices (six-sinusoid radii, "
+                                   "snap %g), FP64, seed 0" % (len(P), len(Q), args.snap),
+                       "predicate_frame": "CMBR-normalised" + (
+                           "" if args.reference_eps else ", edge-scaled (GCMF_CLIP_SCALE_EDGES)"),
+                       "status": status, "label_path": path,
+                       "result_components": cs["total_comps"],
+                       "result_vertices": cs["total_verts"],
+                       "intersection_pairs": cs["n_intersections"],
+                       "n_eval_build": cs["n_eval"],
+                       "n_eval_reference_algorithm": len(P) * len(Q)},
+            "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "kernel_ms_per_step": kernel_ms,
+            "roofline": roofline_line(roof_ms, roof_n, cs["n_eval"], None)}
+    db.free()
+    do.free()
     ctx.close()
     print(json.dumps(line))
 

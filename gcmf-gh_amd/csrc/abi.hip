@@ -103,6 +103,7 @@ void run_clip(gcmf_ctx *ctx, const gcmf::JoinInputs &in, const int *d_pairs, lon
   ctx->clip_valid = false;
   gcmf::ClipOptions opt;
   opt.normalize = (flags & GCMF_CLIP_NORMALIZE) ? 1 : 0;
+  if ((flags & GCMF_CLIP_NORMALIZE) && (flags & GCMF_CLIP_SCALE_EDGES)) opt.normalize |= 4;
   opt.cleanup = (flags & GCMF_CLIP_NO_CLEANUP) ? 0 : 1;
   opt.label_mode = (flags & GCMF_CLIP_SEQ_LABEL) ? 1 : 0;
   opt.route = (flags & GCMF_CLIP_GPU_ROUTE) ? 0 : 1;

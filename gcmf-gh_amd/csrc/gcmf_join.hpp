@@ -61,7 +61,7 @@ void launch_segjoin_short(Context &c, long nList, const int *list, const int *pa
 enum { GH_OK = 0, GH_INVALID = 1, GH_DISJOINT = 2, GH_TRACE_FAIL = -1, GH_OVERFLOW = -5 };
 
 struct ClipOptions {
-  int normalize = 1;  // per-pair CMBR normalisation of every predicate
+  int normalize = 1;  // 1: per-pair CMBR normalisation of every predicate, | 4: edge-scaled
   int cleanup = 1;    // run cleanUpResult
   int noswap = 0;     // keep P = base ring even when it is the smaller one
   int label_mode = 0; // 0: wave per pair (sequential fallback), 1: thread per pair

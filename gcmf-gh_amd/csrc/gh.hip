@@ -154,12 +154,20 @@ __global__ void __launch_bounds__(256)
   cm[2] = m[1][2] < m[0][2] ? m[1][2] : m[0][2];
   cm[3] = m[1][3] < m[0][3] ? m[1][3] : m[0][3];
   double cx = 0.0, cy = 0.0, s = 1.0;
-  if (normalize) {
+  if (normalize & 1) {
     double w = cm[2] - cm[0], h = cm[3] - cm[1];
     double e = w > h ? w : h;
     cx = cm[0];
     cy = cm[1];
     s = e > 0.0 ? 1.0 / e : 1.0;
+    if (normalize & 4) {
+      // edge-scaled: times the power of two f >= sqrt(nP nQ) / 1024, so that
+      // fine rings keep edge lengths well above the absolute epsilon
+      const double r = sqrt((double)nP * (double)nQ) / 1024.0;
+      double f = 1.0;
+      while (f < r) f *= 2.0;
+      s *= f;
+    }
   }
   pm.startP[k] = sP;
   pm.startQ[k] = sQ;
@@ -253,14 +261,15 @@ constexpr long kCullMin = 4096;
 
 struct RunBoxes {
   const double *b1, *b2, *b3;  // [4 * runs] x0, y0, x1, y1
+  const double *l1, *l2, *l3;  // [runs] shortest edge of the run
 };
 
 __global__ void k_run_boxes(long edges, const int *map, const long long *off, const double *ring,
-                            double *box1) {
+                            double *box1, double *len1) {
   const long G = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long e0 = G * kRun1;
   if (e0 >= edges) return;
-  double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+  double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, lmin = INFINITY;
   for (long e = e0; e < e0 + kRun1 && e < edges; e++) {
     const int k = map[e];
     const long long b = off[k], n = off[k + 1] - b;
@@ -269,49 +278,64 @@ __global__ void k_run_boxes(long edges, const int *map, const long long *off, co
     y0 = fmin(y0, fmin(p.y, q.y));
     x1 = fmax(x1, fmax(p.x, q.x));
     y1 = fmax(y1, fmax(p.y, q.y));
+    const double dx = q.x - p.x, dy = q.y - p.y;
+    lmin = fmin(lmin, sqrt(dx * dx + dy * dy));
   }
   box1[4 * G] = x0;
   box1[4 * G + 1] = y0;
   box1[4 * G + 2] = x1;
   box1[4 * G + 3] = y1;
+  len1[G] = lmin;
 }
 
 // boxes of the next level: 16 boxes merged
-__global__ void k_run_boxes_merge(long runsIn, const double *in, double *out) {
+__global__ void k_run_boxes_merge(long runsIn, const double *in, const double *lenIn,
+                                  double *out, double *lenOut) {
   const long S = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long g0 = S * 16;
   if (g0 >= runsIn) return;
-  double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+  double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, lmin = INFINITY;
   for (long g = g0; g < g0 + 16 && g < runsIn; g++) {
     x0 = fmin(x0, in[4 * g]);
     y0 = fmin(y0, in[4 * g + 1]);
     x1 = fmax(x1, in[4 * g + 2]);
     y1 = fmax(y1, in[4 * g + 3]);
+    lmin = fmin(lmin, lenIn[g]);
   }
   out[4 * S] = x0;
   out[4 * S + 1] = y0;
   out[4 * S + 2] = x1;
   out[4 * S + 3] = y1;
+  lenOut[S] = lmin;
 }
 
 __device__ __forceinline__ bool box_hit(const double *b, const EdgeBox &e) {
   return !(e.x0 > b[2] || e.x1 < b[0] || e.y0 > b[3] || e.y1 < b[1]);
 }
 
+// A run is visited when its box meets ab, or when one of its edges may form
+// a pair shorter than the tiny threshold T with an edge of length la (T = 0
+// for box culling alone)
+__device__ __forceinline__ bool run_hit(const double *b, const double *lmin, const EdgeBox &e,
+                                        double la, double T) {
+  return box_hit(b, e) || (T > 0.0 && la * *lmin < T);
+}
+
 // f(q, B_q, B_q+1) over the edges of the ring at global edges [e0, e1)
-// (vertices V) whose runs' boxes meet ab, in index order
+// (vertices V) whose runs pass run_hit, in index order
 template <class F>
 __device__ __forceinline__ void culled_edges(const double *V, long long e0, long long e1,
-                                             const RunBoxes &rb, const EdgeBox &ab, F &&f) {
+                                             const RunBoxes &rb, const EdgeBox &ab, double la,
+                                             double T, F &&f) {
   const long long n = e1 - e0;
   for (long long S3 = e0 / kRun3; S3 * kRun3 < e1; S3++) {
-    if (!box_hit(rb.b3 + 4 * S3, ab)) continue;
+    if (!run_hit(rb.b3 + 4 * S3, rb.l3 + S3, ab, la, T)) continue;
     const long long a2 = max(S3 * 16, e0 / kRun2), z2 = min(S3 * 16 + 16, (e1 - 1) / kRun2 + 1);
     for (long long S2 = a2; S2 < z2; S2++) {
-      if (!box_hit(rb.b2 + 4 * S2, ab)) continue;
+      if (!run_hit(rb.b2 + 4 * S2, rb.l2 + S2, ab, la, T)) continue;
       const long long a1 = max(S2 * 16, e0 / kRun1), z1 = min(S2 * 16 + 16, (e1 - 1) / kRun1 + 1);
       for (long long S1 = a1; S1 < z1; S1++) {
-        if (!box_hit(rb.b1 + 4 * S1, ab)) continue;
+        if (!run_hit(rb.b1 + 4 * S1, rb.l1 + S1, ab, la, T)) continue;
         const long long a = max(S1 * kRun1, e0), z = min(S1 * kRun1 + kRun1, e1);
         for (long long e = a; e < z; e++) {
           const long long q = e - e0;
@@ -386,7 +410,7 @@ __global__ void __launch_bounds__(256)
       c2 += adds_vertex(t);
     };
     if (CULL && bN > kCullMin)
-      culled_edges(Bv, bBase, bBase + bN, sideP ? boxQ : boxP, ab, visit);
+      culled_edges(Bv, bBase, bBase + bN, sideP ? boxQ : boxP, ab, 0.0, 0.0, visit);
     else
       ring_edges(Bv, (int)bN, visit);
   }
@@ -1031,9 +1055,9 @@ __global__ void __launch_bounds__(256)
 // Hence the Chebyshev gap of the closed edge boxes of a nonzero pair is at
 // most mP + mQ, mP = kA L_P + c0, mQ = kA L_Q + kB / L_Q + c0, with kA = 1.5
 // (eps + eA), kB = 1.5 * 2.2 (eps + 2 dl), c0 = 16 u D, unless L_P L_Q < T,
-// in which case the pair is always classified.  Both views take the same
-// decision for a pair (the expressions are symmetric), so their partner sets
-// agree.
+// in which case the pair is classified unless |A(P1,Q1,Q2)| >= eps rules the
+// collinear branch out.  Both views take the same decision for a pair (the
+// expressions are symmetric), so their partner sets agree.
 
 // per-pair filter constants (cf[16 k + i])
 enum { CF_BOXP = 0, CF_BOXQ = 4, CF_MAXMP = 8, CF_MAXMQ = 9, CF_MINLP = 10, CF_MINLQ = 11,
@@ -1124,16 +1148,23 @@ __device__ __forceinline__ double edge_gap(pt a1, pt a2, pt b1, pt b2) {
 // Per-edge view of the filter: an edge A of one ring against the edges of
 // the other ring (B).  skip() is the pair decision; the ring-level bound G
 // (A's margin plus the largest margin of B) rejects without loading B's
-// length and margin when no pair of A can be tiny.
+// length and margin when no pair of A can be tiny.  A tiny pair whose boxes
+// are apart is still skipped when |A(P1, Q1, Q2)| >= eps: intersect() then
+// cannot take its collinear branch (the value is computed as intersect()
+// computes it), and the other types are bounded by the margins.
 struct CpuFilter {
   double la, ma, G, T;
-  bool tiny;  // some pair of A may have L_A L_B < T
+  bool tiny;   // some pair of A may have L_A L_B < T
+  bool sideP;  // A is a P edge
   const double *lenB, *mrgB;
   __device__ __forceinline__ bool skip(int q, pt A1, pt A2, pt B1, pt B2) const {
     const double gap = edge_gap(A1, A2, B1, B2);
     if (!(gap > 0.0)) return false;
     if (!tiny && gap > G) return true;
-    return gap > ma + mrgB[q] && !(la * lenB[q] < T);
+    if (!(gap > ma + mrgB[q])) return false;
+    if (!(la * lenB[q] < T)) return true;
+    const double AP1 = sideP ? triA(A1, B1, B2) : triA(B1, A1, A2);
+    return !(fabs(AP1) < kGhEps);
   }
 };
 
@@ -1148,6 +1179,7 @@ __device__ __forceinline__ CpuFilter cpu_filter(const double *cf, int k, bool si
   f.T = c[CF_TINY];
   f.G = f.ma + (sideP ? c[CF_MAXMQ] : c[CF_MAXMP]);
   f.tiny = f.la * (sideP ? c[CF_MINLQ] : c[CF_MINLP]) < f.T;
+  f.sideP = sideP;
   f.lenB = (sideP ? lenQ : lenP) + bBase;
   f.mrgB = (sideP ? mrgQ : mrgP) + bBase;
   return f;
@@ -1208,10 +1240,12 @@ __global__ void __launch_bounds__(256)
       }
       c2 += (sideP ? ins_on_p(t) : ins_on_q(t)) ? 1 : 0;
     };
-    if (CULL && !f.tiny && bN > kCullMin) {
+    if (CULL && bN > kCullMin) {
+      // runs apart from the box widened by G and without a possible tiny
+      // pair hold no pair the filter keeps
       const double G = f.G * (1.0 + 1e-12) + 1e-300;
       const EdgeBox wide{ab.x0 - G, ab.y0 - G, ab.x1 + G, ab.y1 + G};
-      culled_edges(Bv, bBase, bBase + bN, sideP ? boxQ : boxP, wide, visit);
+      culled_edges(Bv, bBase, bBase + bN, sideP ? boxQ : boxP, wide, f.la, f.T, visit);
     } else {
       ring_edges(Bv, (int)bN, visit);
     }
@@ -1266,7 +1300,7 @@ __global__ void __launch_bounds__(64)
         n += __popcll(m);
       }
     };
-    if (rb.b1 == nullptr || f.tiny || bN <= kCullMin) {
+    if (rb.b1 == nullptr || bN <= kCullMin) {
       scan(0, bN);
     } else {
       const double G = f.G * (1.0 + 1e-12) + 1e-300;
@@ -1274,11 +1308,11 @@ __global__ void __launch_bounds__(64)
       const EdgeBox ab{ab0.x0 - G, ab0.y0 - G, ab0.x1 + G, ab0.y1 + G};
       const long long e0 = bBase, e1 = bBase + bN;
       for (long long S3 = e0 / kRun3; S3 * kRun3 < e1; S3++) {
-        if (!box_hit(rb.b3 + 4 * S3, ab)) continue;
+        if (!run_hit(rb.b3 + 4 * S3, rb.l3 + S3, ab, f.la, f.T)) continue;
         const long long a2 = max(S3 * 16, e0 / kRun2),
                         z2 = min(S3 * 16 + 16, (e1 - 1) / kRun2 + 1);
         for (long long S2 = a2; S2 < z2; S2++)
-          if (box_hit(rb.b2 + 4 * S2, ab))
+          if (run_hit(rb.b2 + 4 * S2, rb.l2 + S2, ab, f.la, f.T))
             scan(max(S2 * kRun2, e0) - e0, min(S2 * kRun2 + kRun2, e1) - e0);
       }
     }
@@ -3040,24 +3074,31 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   // run boxes of a side, when some ring of it is long enough to be culled
   auto run_boxes = [&](bool qside, long edges, const int *map, const long long *off,
                        const double *ring) {
-    RunBoxes r{nullptr, nullptr, nullptr};
+    RunBoxes r{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     const long r1 = (edges + kRun1 - 1) / kRun1, r2 = (r1 + 15) / 16, r3 = (r2 + 15) / 16;
     double *b1 = wsT<double>(c, qside ? "gh_rb1Q" : "gh_rb1P", 4 * r1);
     double *b2 = wsT<double>(c, qside ? "gh_rb2Q" : "gh_rb2P", 4 * r2);
     double *b3 = wsT<double>(c, qside ? "gh_rb3Q" : "gh_rb3P", 4 * r3);
+    double *l1 = wsT<double>(c, qside ? "gh_rl1Q" : "gh_rl1P", r1);
+    double *l2 = wsT<double>(c, qside ? "gh_rl2Q" : "gh_rl2P", r2);
+    double *l3 = wsT<double>(c, qside ? "gh_rl3Q" : "gh_rl3P", r3);
     hipLaunchKernelGGL(k_run_boxes, dim3(grid_for(r1, 256)), dim3(256), 0, c.stream, edges, map,
-                       off, ring, b1);
+                       off, ring, b1, l1);
     hipLaunchKernelGGL(k_run_boxes_merge, dim3(grid_for(r2, 256)), dim3(256), 0, c.stream, r1,
-                       (const double *)b1, b2);
+                       (const double *)b1, (const double *)l1, b2, l2);
     hipLaunchKernelGGL(k_run_boxes_merge, dim3(grid_for(r3, 256)), dim3(256), 0, c.stream, r2,
-                       (const double *)b2, b3);
+                       (const double *)b2, (const double *)l2, b3, l3);
     HIP_CHECK(hipGetLastError());
     r.b1 = b1;
     r.b2 = b2;
     r.b3 = b3;
+    r.l1 = l1;
+    r.l2 = l2;
+    r.l3 = l3;
     return r;
   };
-  RunBoxes boxP{nullptr, nullptr, nullptr}, boxQ{nullptr, nullptr, nullptr};
+  RunBoxes boxP{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  RunBoxes boxQ{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   if (maxRingP > kCullMin && edgesP > 0)
     boxP = run_boxes(false, edgesP, mapP, (const long long *)offP, (const double *)ringP);
   if (maxRingQ > kCullMin && edgesQ > 0)

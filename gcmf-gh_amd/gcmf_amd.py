@@ -215,6 +215,7 @@ CLIP_NORMALIZE = 1
 CLIP_NO_CLEANUP = 2
 CLIP_SEQ_LABEL = 4
 CLIP_GPU_ROUTE = 8
+CLIP_SCALE_EDGES = 16
 
 
 def hip_lib():

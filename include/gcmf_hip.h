@@ -110,6 +110,12 @@ int gcmf_join_get(gcmf_ctx *ctx, int field, void *dst, long count);
  * polyclip.h:188-281).  The two agree in general position; on degenerate
  * inputs only the CPU route is well defined. */
 #define GCMF_CLIP_GPU_ROUTE 8
+/* With GCMF_CLIP_NORMALIZE: scale each pair's normalised frame further by
+ * the power of two f >= sqrt(nP nQ) / 1024 (1 for rings of up to 1024
+ * vertices).  The reference's absolute epsilon (1e-7, constants.h:4) makes
+ * nearby edges of million-vertex rings "collinear" and their trace fail;
+ * this keeps their edges long against it.  A deviation from the reference. */
+#define GCMF_CLIP_SCALE_EDGES 16
 
 typedef struct {
   long pairs, ok, invalid, disjoint, failed;

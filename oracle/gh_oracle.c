@@ -137,7 +137,16 @@ static void ring_mbr(const double *c, int n, double *m) {
   m[0] = minX; m[1] = minY; m[2] = maxX; m[3] = maxY;
 }
 
-/* returns 0 when the MBRs are disjoint (the reference exits the process) */
+/* the power of two f >= sqrt(nP nQ) / 1024 (1 for rings of up to 1024
+ * vertices): the extra scale of the edge-scaled normalisation */
+static double edge_scale(int nP, int nQ) {
+  double r = sqrt((double)nP * (double)nQ) / 1024.0, f = 1.0;
+  while (f < r) f *= 2.0;
+  return f;
+}
+
+/* returns 0 when the MBRs are disjoint (the reference exits the process);
+ * normalize & 1: per-pair CMBR normalisation, & 4: edge-scaled */
 static int pair_frame(const double *P, int nP, const double *Q, int nQ,
                       int normalize, double *cmbr, double *norm) {
   double a[4], b[4];
@@ -149,12 +158,13 @@ static int pair_frame(const double *P, int nP, const double *Q, int nQ,
   cmbr[1] = a[1] < b[1] ? b[1] : a[1];
   cmbr[2] = b[2] < a[2] ? b[2] : a[2];
   cmbr[3] = b[3] < a[3] ? b[3] : a[3];
-  if (normalize) {
+  if (normalize & 1) {
     double w = cmbr[2] - cmbr[0], h = cmbr[3] - cmbr[1];
     double e = w > h ? w : h;
     norm[0] = cmbr[0];
     norm[1] = cmbr[1];
     norm[2] = e > 0.0 ? 1.0 / e : 1.0;
+    if (normalize & 4) norm[2] *= edge_scale(nP, nQ);
   } else {
     norm[0] = 0.0; norm[1] = 0.0; norm[2] = 1.0;
   }
@@ -682,7 +692,7 @@ static void result_empty(orc_result_t *out, int status) {
 int orc_clip_cpu(const double *P, int nP, const double *Q, int nQ, int normalize,
                  orc_result_t *out) {
   double cmbr[4], norm[3];
-  if (!pair_frame(P, nP, Q, nQ, normalize & 1, cmbr, norm)) {
+  if (!pair_frame(P, nP, Q, nQ, normalize & 5, cmbr, norm)) {
     result_empty(out, 2);
     return 2;
   }
@@ -1139,7 +1149,7 @@ int orc_phase1(const double *Pc, int nP, const double *Qc, int nQ, int normalize
 int orc_clip_gpu_route(const double *P, int nP, const double *Q, int nQ,
                        int normalize, orc_result_t *out) {
   orc_phase1_t t;
-  int rc = orc_phase1(P, nP, Q, nQ, normalize & 1, &t);
+  int rc = orc_phase1(P, nP, Q, nQ, normalize & 5, &t);
   if (rc == 2 || rc == 1) {
     result_empty(out, rc);
     out->n_inter = t.psP1 ? t.psP1[nP] : 0;
@@ -1440,7 +1450,8 @@ int orc_prefilter_check(const double *P, int nP, const double *Q, int nQ, int no
     for (int q = 0; q < nQ; q++) {
       pt Q1 = r[1][q], Q2 = r[1][q + 1 == nQ ? 0 : q + 1];
       double gap = edge_gap_c(P1, P2, Q1, Q2);
-      int skip = gap > 0.0 && gap > m[0][p] + m[1][q] && !(L[0][p] * L[1][q] < T);
+      int skip = gap > 0.0 && gap > m[0][p] + m[1][q] &&
+                 (!(L[0][p] * L[1][q] < T) || !(fabs(triA(P1, Q1, Q2)) < GH_EPS));
       double al, be;
       int t = itype(P1, P2, Q1, Q2, &al, &be);
       out[0]++;

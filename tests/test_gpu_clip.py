@@ -231,6 +231,26 @@ def test_clip_degenerate_wavy(gctx):
     assert abs(O.comps_area(res[0], ox, oy) - A) <= AREA_RTOL * A
 
 
+def test_clip_edge_scaled_fine_pair(gctx):
+    """A 50000 x 5000 pair of the large-pair shape in the edge-scaled frame
+    (GCMF_CLIP_SCALE_EDGES, mode bit 4 of the oracle): identical to the CPU
+    route in the same frame, and its traced area equals the boundary
+    integral within 1e-12."""
+    P, Q = G.gen_single_pair(1, 0, 50000, 5000)
+    cases = [("fine", P, Q)]
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_SCALE_EDGES)
+    assert st["ok"] == 1
+    check_results(gctx, st, cases, 5)
+    st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_SCALE_EDGES | G.CLIP_NO_CLEANUP)
+    check_results(gctx, st, cases, 7)
+    _, res = gctx.clip_results(st)
+    A = O.area_boundary(P, Q)
+    allxy = np.concatenate([P, Q])
+    ox, oy = 0.5 * (allxy[:, 0].min() + allxy[:, 0].max()), 0.5 * (
+        allxy[:, 1].min() + allxy[:, 1].max())
+    assert abs(O.comps_area(res[0], ox, oy) - A) <= AREA_RTOL * A
+
+
 def test_clip_area_oracle(gctx):
     cases = random_cases(40, seed0=300, nmax=400)
     P, Q = G.gen_single_pair(0, 0, 2000, 1500)
