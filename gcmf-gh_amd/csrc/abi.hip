@@ -160,6 +160,12 @@ void gcmf_shutdown(gcmf_ctx *ctx) {
   }
   for (auto e : ctx->c.event_pool) (void)hipEventDestroy(e);
   if (ctx->c.hbuf) (void)hipHostFree(ctx->c.hbuf);
+  if (ctx->c.side) {
+    (void)hipStreamSynchronize(ctx->c.side);
+    (void)hipStreamDestroy(ctx->c.side);
+    (void)hipEventDestroy(ctx->c.ev_fork);
+    (void)hipEventDestroy(ctx->c.ev_join);
+  }
   (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
 }

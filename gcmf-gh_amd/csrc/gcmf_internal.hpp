@@ -64,6 +64,10 @@ struct Context {
   std::string last_error;
   int cu_count = 256;
   long long *hbuf = nullptr;  // pinned host staging for small read-backs (kHostSlots)
+  // a second stream (default priority) for work that overlaps the main one,
+  // with the events that fork it from and join it to `stream`
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   hipEvent_t take_event();
   void flush_timers();  // synchronises on pending events and accumulates

@@ -2153,10 +2153,71 @@ __device__ __forceinline__ int lt_count(unsigned long long m) {
   return __popcll(m & ((1ULL << __lane_id()) - 1ULL));
 }
 
+// Group-wide reductions and scans.  A group is the W threads of one block:
+// one wave (W = 64, pairs of ordinary size) or sixteen (W = 1024, the large
+// pairs).  Every thread of the group must call them, with uniform control
+// flow; the LDS slots are guarded by a barrier on entry.
+template <int W>
+__device__ __forceinline__ int grp_sum(int v) {
+  v = wave_sum(v);
+  if (W == 64) return v;
+  __shared__ int red[W / 64];
+  __syncthreads();
+  if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = 0;
+  for (int i = 0; i < W / 64; i++) t += red[i];
+  return t;
+}
+template <int W>
+__device__ __forceinline__ int grp_min(int v) {
+  v = wave_min(v);
+  if (W == 64) return v;
+  __shared__ int red[W / 64];
+  __syncthreads();
+  if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = red[0];
+  for (int i = 1; i < W / 64; i++) t = min(t, red[i]);
+  return t;
+}
+template <int W>
+__device__ __forceinline__ bool grp_any(bool b) {
+  if (W == 64) return __ballot(b) != 0ULL;
+  return grp_sum<W>(__ballot(b) != 0ULL ? 1 : 0) != 0;
+}
+// exclusive prefix of v over the group in thread order; total: the sum
+template <int W>
+__device__ __forceinline__ int grp_scan(int v, int &total) {
+  const int lane = __lane_id();
+  int incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (W == 64) {
+    total = __shfl(incl, 63, 64);
+    return incl - v;
+  }
+  __shared__ int part[W / 64];
+  __syncthreads();
+  if (lane == 63) part[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  int before = 0, tot = 0;
+  for (int i = 0; i < W / 64; i++) {
+    if (i < w) before += part[i];
+    tot += part[i];
+  }
+  total = tot;
+  return before + incl - v;
+}
+
 // parity of the winding number of R over the ring [r0, r0+n) in index order
-__device__ bool wave_pip(const Nodes &N, int r0, int n, pt R) {
+template <int W>
+__device__ bool grp_pip(const Nodes &N, int r0, int n, pt R) {
   int w = 0;
-  for (int a = __lane_id(); a < n; a += 64) {
+  for (int a = threadIdx.x; a < n; a += W) {
     int ia = r0 + a, ib = r0 + (a + 1 == n ? 0 : a + 1);
     pt P0 = P_(N, ia), P1 = P_(N, ib);
     if ((P0.y < R.y) != (P1.y < R.y)) {
@@ -2168,15 +2229,16 @@ __device__ bool wave_pip(const Nodes &N, int r0, int n, pt R) {
       }
     }
   }
-  return (wave_sum(w) & 1) != 0;
+  return (grp_sum<W>(w) & 1) != 0;
 }
 
 // index of the first node of [r0, r0+n) without the intersection flag, or -1
-__device__ int wave_first_nonint(const Nodes &N, int r0, int n) {
+template <int W>
+__device__ int grp_first_nonint(const Nodes &N, int r0, int n) {
   int best = 0x7fffffff;
-  for (int a = __lane_id(); a < n; a += 64)
+  for (int a = threadIdx.x; a < n; a += W)
     if (!isint(N, r0 + a)) best = min(best, a);
-  best = wave_min(best);
+  best = grp_min<W>(best);
   return best == 0x7fffffff ? -1 : r0 + best;
 }
 
@@ -2188,23 +2250,25 @@ __device__ __forceinline__ bool iscross(const Nodes &N, int v) {
 // the status at a vertex is the start status toggled once per crossing
 // before it; crossings take it as enex, bouncing vertices met with status
 // EXIT are flagged for splitting
-__device__ void wave_enex(Nodes &N, int r0, int n, int V, int status0) {
+template <int W>
+__device__ void grp_enex(Nodes &N, int r0, int n, int V, int status0) {
   int carry = 0;
   const int v0 = V - r0;
-  for (int base = 1; base < n; base += 64) {
-    const int j = base + __lane_id();
+  for (int base = 1; base < n; base += W) {
+    const int j = base + threadIdx.x;
     int u = v0 + j;
     u = r0 + (u >= n ? u - n : u);
     const bool in = j < n && isint(N, u);
     const bool cr = in && N.lab[u] == L_CROSSING;
-    const unsigned long long m = __ballot(cr);
+    int tot;
+    const int pre = grp_scan<W>(cr ? 1 : 0, tot);
     if (in) {
-      const int par = (carry + lt_count(m)) & 1;
+      const int par = (carry + pre) & 1;
       const int status = par ? (status0 == ENTRY ? EXIT : ENTRY) : status0;
       if (cr) N.enex[u] = (signed char)status;
       else if (status == EXIT) N.fl[u] |= F_SPLIT;
     }
-    carry += __popcll(m);
+    carry += tot;
   }
 }
 
@@ -2281,9 +2345,10 @@ enum { SEG_DONE = 0, SEG_SLOW = 2, SEG_OVERFLOW = 3 };
 // O.pn, rg[0, n) mapping output positions to components) in the sequential
 // order.  Returns SEG_DONE, SEG_SLOW when the pair needs the sequential
 // trace, or SEG_OVERFLOW.
+template <int W>
 __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, int *rg, int nP,
                              int nQ) {
-  const int lane = __lane_id();
+  const int lane = threadIdx.x;
   const int n = nP + nQ;
   int *rk = rg, *dn = rg + n;
   int2 *vc[2] = {(int2 *)(rg + 2 * n), (int2 *)(rg + 4 * n)};
@@ -2293,21 +2358,17 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   int carry = 0, XP = 0;
   for (int ring = 0; ring < 2; ring++) {
     const int rb = ring ? nP : 0, nR = ring ? nQ : nP;
-    for (int b = 0; b < nR; b += 64) {
+    for (int b = 0; b < nR; b += W) {
       const int j = b + lane, u = rb + j;
       const int mu = j < nR ? dmult(N, u) : 0;
-      int incl = mu;
-      for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-      }
-      const int r = carry + incl - mu;
+      int tot;
+      const int r = carry + grp_scan<W>(mu, tot);
       if (mu > 0 && r + mu <= n) {
         rk[u] = r;
         dn[r] = 2 * u;
         if (mu == 2) dn[r + 1] = 2 * u + 1;
       }
-      carry += __shfl(incl, 63, 64);
+      carry += tot;
     }
     if (ring == 0) XP = carry;
   }
@@ -2318,7 +2379,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   // arrival; the arrival must carry the status the inner loop stops on.
   // Key .x: the P departure of the departure's intersection point.
   bool bad = false;
-  for (int d = lane; d < D; d += 64) {
+  for (int d = lane; d < D; d += W) {
     const Departure p = departure(N, dn, d, nP, nQ, XP, XQ);
     const bool onP = d < XP;
     const int ua = dn[p.arr] >> 1, u = p.code >> 1;
@@ -2338,7 +2399,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     vc[0][d] = make_int2(key, d);
     off[d] = p.dist;
   }
-  if (__ballot(bad)) return SEG_SLOW;
+  if (grp_any<W>(bad)) return SEG_SLOW;
   // pointer jumping: smallest P key (.x) and smallest departure (.y) of
   // every cycle; four independent chains per lane in flight
   int rounds = 0;
@@ -2350,12 +2411,12 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     const int *__restrict__ ns = nx[cur];
     int2 *__restrict__ vd = vc[cur ^ 1];
     int *__restrict__ nd = nx[cur ^ 1];
-    for (int d0 = lane; d0 < D; d0 += 256) {
+    for (int d0 = lane; d0 < D; d0 += 4 * W) {
       int x[4];
       int2 a[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int d = d0 + 64 * k;
+        const int d = d0 + W * k;
         x[k] = d < D ? ns[d] : 0;
         a[k] = d < D ? vs[d] : make_int2(0, 0);
       }
@@ -2368,7 +2429,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       }
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int d = d0 + 64 * k;
+        const int d = d0 + W * k;
         if (d < D) {
           vd[d] = make_int2(min(a[k].x, b[k].x), min(a[k].y, b[k].y));
           nd[d] = y[k];
@@ -2382,7 +2443,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   // a cycle is traced when it holds the departure of its smallest P key; its
   // list starts there.  S: segment lengths, then suffix sums
   int *S[2] = {(int *)vc[cur ^ 1], (int *)vc[cur ^ 1] + n};
-  for (int d = lane; d < D; d += 64) {
+  for (int d = lane; d < D; d += W) {
     const int2 c = mc[d];
     const int sd = c.x;
     const bool sel = mc[sd].y == c.y;
@@ -2398,9 +2459,9 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   }
   __syncthreads();
   // each intersection point departs exactly once over the traced cycles
-  for (int d = lane; d < XP; d += 64)
+  for (int d = lane; d < XP; d += W)
     if (S[1][d] == S[1][dep_partner(N, rk, dn[d], nP, nQ)]) bad = true;
-  if (__ballot(bad)) return SEG_SLOW;
+  if (grp_any<W>(bad)) return SEG_SLOW;
   cur = 0;
   for (int t = 0; t < rounds; t++) {
     __syncthreads();
@@ -2408,11 +2469,11 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     const int *__restrict__ ns = nx[cur];
     int *__restrict__ sd = S[cur ^ 1];
     int *__restrict__ nd = nx[cur ^ 1];
-    for (int d0 = lane; d0 < D; d0 += 256) {
+    for (int d0 = lane; d0 < D; d0 += 4 * W) {
       int x[4], a[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int d = d0 + 64 * k;
+        const int d = d0 + W * k;
         x[k] = d < D ? ns[d] : -1;
         a[k] = d < D ? ss[d] : 0;
       }
@@ -2424,7 +2485,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       }
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int d = d0 + 64 * k;
+        const int d = d0 + W * k;
         if (d < D) {
           sd[d] = a[k] + b[k];
           nd[d] = y[k];
@@ -2438,18 +2499,15 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   // components in order of their start departure
   int *cix = nx[0], *ciD = nx[1];
   int vcarry = 0, ccarry = 0;
-  for (int b = 0; b < XP; b += 64) {
+  for (int b = 0; b < XP; b += W) {
     const int d = b + lane;
     const bool start = d < XP && mc[d].x == d;
-    const unsigned long long m = __ballot(start);
     const int len = start ? Sf[d] : 0;
-    int incl = len;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int u = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += u;
-    }
+    int ctot, vtot;
+    const int cpre = grp_scan<W>(start ? 1 : 0, ctot);
+    const int vpre = grp_scan<W>(len, vtot);
     if (start) {
-      const int ci = ccarry + lt_count(m), base = vcarry + incl - len;
+      const int ci = ccarry + cpre, base = vcarry + vpre;
       cbase[d] = base;
       cix[d] = ci;
       if (ci < O.capComp) {
@@ -2457,8 +2515,8 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
         clen[ci] = len;
       }
     }
-    vcarry += __shfl(incl, 63, 64);
-    ccarry += __popcll(m);
+    vcarry += vtot;
+    ccarry += ctot;
   }
   if (vcarry > O.cap || ccarry > O.capComp) return SEG_OVERFLOW;
   // the output position map holds n entries
@@ -2466,7 +2524,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   O.n = vcarry;
   O.ncomp = ccarry;
   __syncthreads();
-  for (int d = lane; d < D; d += 64) {
+  for (int d = lane; d < D; d += W) {
     const int2 c = mc[d];
     const int sd = c.x;
     if (mc[sd].y == c.y) {
@@ -2486,16 +2544,13 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     const int rb = ring ? nP : 0, nR = ring ? nQ : nP, base = ring ? XP : 0;
     const int XR = ring ? XQ : XP;
     carry = 0;
-    for (int b = 0; b < nR; b += 64) {
+    for (int b = 0; b < nR; b += W) {
       const int j = b + lane, u = rb + j;
       const int mu = j < nR ? dmult(N, u) : 0;
-      int incl = mu;
-      for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-      }
+      int tot;
+      const int pre = grp_scan<W>(mu, tot);
       if (j < nR) {
-        const int r = carry + incl - mu, ri = r + mu;
+        const int r = carry + pre, ri = r + mu;
         const int df = base + (r == 0 ? XR - 1 : r - 1), db = base + (ri == XR ? 0 : ri);
         for (int side = 0; side < 2; side++) {
           const int dd = side ? db : df;
@@ -2511,7 +2566,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
           }
         }
       }
-      carry += __shfl(incl, 63, 64);
+      carry += tot;
     }
   }
   __syncthreads();
@@ -2519,9 +2574,10 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
 }
 
 // a whole node ring as output component c at [off, off + n)
+template <int W>
 __device__ void copy_ring_out(OutBuf &O, double *xy, int *cw, const Nodes &N, int r0, int n,
                               int off, int c) {
-  for (int j = __lane_id(); j < n; j += 64) {
+  for (int j = threadIdx.x; j < n; j += W) {
     st(O.pn, off + j, P_(N, r0 + j));
     st(xy, off + j, ld(N.po, r0 + j));
     cw[off + j] = c;
@@ -2621,41 +2677,83 @@ __device__ int clean_component(const double *pn, int *keep, int b, int L) {
   return kept;
 }
 
+// the same clean-up of one component by one thread, in order: the root
+// search (A), then the pass (B)
+__device__ int lane_clean(const double *pn, int *keep, int b, int L) {
+  int r = -1;
+  if (L >= 3) {
+    const pt Pl = ld(pn, b + L - 1);
+    for (int i = 0; i <= L - 3; i++)
+      if (!(fabs(triA(Pl, ld(pn, b + i), ld(pn, b + i + 1))) < kGhEps)) {
+        r = i;
+        break;
+      }
+  }
+  for (int j = 0; j < (r < 0 ? L : r); j++) keep[b + j] = 0;
+  if (r < 0) return 0;
+  keep[b + r] = 1;
+  const int x0 = b + r, m = L - 1 - r;
+  pt last = ld(pn, x0);
+  int kept = 1;
+  for (int j = 1; j <= m; j++) {
+    const pt cur = ld(pn, x0 + j), nxt = ld(pn, j == m ? x0 : x0 + j + 1);
+    const bool kp = !(fabs(triA(last, cur, nxt)) < kGhEps);
+    keep[x0 + j] = kp ? 1 : 0;
+    if (kp) {
+      last = cur;
+      kept++;
+    }
+  }
+  return kept;
+}
+
+// components up to this length are cleaned by one thread each, longer ones
+// by one wave each
+constexpr int kLaneClean = 48;
+
 // clean-up of every component of O (component c at [O.compRoot[c], +clen[c])
 // in list order, cw: position -> component), then compaction of xy and clen
 // over the kept vertices and non-empty components.  Scratch: rg[n, 5n).
 // Returns true when some vertex was dropped; O.n / O.ncomp are updated.
-__device__ bool wave_cleanup_compact(OutBuf &O, double *xy, int *clen, const int *cw, int *rg,
-                                     int n) {
-  const int lane = __lane_id();
+template <int W>
+__device__ bool grp_cleanup_compact(OutBuf &O, double *xy, int *clen, const int *cw, int *rg,
+                                    int n) {
+  const int lane = threadIdx.x;
   const int T = O.n, NC = O.ncomp;
   int *dirty = rg + n, *dl = rg + 2 * n, *keep = rg + 3 * n, *nlen = rg + 4 * n;
-  for (int c = lane; c < NC; c += 64) dirty[c] = 0;
+  for (int c = lane; c < NC; c += W) dirty[c] = 0;
   __syncthreads();
-  for (int w = lane; w < T; w += 64) {
+  for (int w = lane; w < T; w += W) {
     const int c = min(max(cw[w], 0), NC - 1), b = O.compRoot[c], L = clen[c];
     const int pw = w == b ? b + L - 1 : w - 1, nw = w == b + L - 1 ? b : w + 1;
     if (fabs(triA(ld(O.pn, pw), ld(O.pn, w), ld(O.pn, nw))) < kGhEps) dirty[c] = 1;
   }
   __syncthreads();
   int nd = 0;
-  for (int base = 0; base < NC; base += 64) {
+  for (int base = 0; base < NC; base += W) {
     const int c = base + lane;
     const bool f = c < NC && dirty[c] != 0;
-    const unsigned long long m = __ballot(f);
-    if (f) dl[nd + lt_count(m)] = c;
-    nd += __popcll(m);
+    int tot;
+    const int pre = grp_scan<W>(f ? 1 : 0, tot);
+    if (f) dl[nd + pre] = c;
+    nd += tot;
   }
   if (nd == 0) return false;
   __syncthreads();
-  for (int i = 0; i < nd; i++) {
+  for (int i = lane; i < nd; i += W) {
     const int c = dl[i];
-    const int kept = clean_component(O.pn, keep, O.compRoot[c], clen[c]);
-    if (lane == 0) nlen[c] = kept;
+    if (clen[c] <= kLaneClean) nlen[c] = lane_clean(O.pn, keep, O.compRoot[c], clen[c]);
+  }
+  for (int i = lane >> 6; i < nd; i += W / 64) {
+    const int c = dl[i];
+    if (clen[c] > kLaneClean) {
+      const int kept = clean_component(O.pn, keep, O.compRoot[c], clen[c]);
+      if (__lane_id() == 0) nlen[c] = kept;
+    }
   }
   __syncthreads();
   int vout = 0;
-  for (int base = 0; base < T; base += 64) {
+  for (int base = 0; base < T; base += W) {
     const int w = base + lane;
     bool kp = false;
     pt v;
@@ -2664,18 +2762,20 @@ __device__ bool wave_cleanup_compact(OutBuf &O, double *xy, int *clen, const int
       kp = dirty[c] ? keep[w] != 0 : true;
       if (kp) v = ld(xy, w);
     }
-    const unsigned long long m = __ballot(kp);
-    if (kp) st(xy, vout + lt_count(m), v);
-    vout += __popcll(m);
+    int tot;
+    const int pre = grp_scan<W>(kp ? 1 : 0, tot);
+    if (kp) st(xy, vout + pre, v);
+    vout += tot;
   }
   int cout = 0;
-  for (int base = 0; base < NC; base += 64) {
+  for (int base = 0; base < NC; base += W) {
     const int c = base + lane;
     int len = 0;
     if (c < NC) len = dirty[c] ? nlen[c] : clen[c];
-    const unsigned long long m = __ballot(len > 0);
-    if (len > 0) clen[cout + lt_count(m)] = len;
-    cout += __popcll(m);
+    int tot;
+    const int pre = grp_scan<W>(len > 0 ? 1 : 0, tot);
+    if (len > 0) clen[cout + pre] = len;
+    cout += tot;
   }
   O.n = vout;
   O.ncomp = cout;
@@ -2694,17 +2794,19 @@ enum {
 
 // true when some vertex of the ring [r0, r0+n) of O (cyclic) meets
 // cleanUpResult's removal test
-__device__ bool wave_any_cleanup(const OutBuf &O, int r0, int n) {
+template <int W>
+__device__ bool grp_any_cleanup(const OutBuf &O, int r0, int n) {
   bool cand = false;
-  for (int j = __lane_id(); j < n; j += 64) {
+  for (int j = threadIdx.x; j < n; j += W) {
     const int w = r0 + j, pw = r0 + (j == 0 ? n - 1 : j - 1), nw = r0 + (j + 1 == n ? 0 : j + 1);
     if (fabs(triA(ld(O.pn, pw), ld(O.pn, w), ld(O.pn, nw))) < kGhEps) cand = true;
   }
-  return __ballot(cand) != 0ULL;
+  return grp_any<W>(cand);
 }
 
-__device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const TraceArgs &a,
-                               int *aux) {
+template <int W>
+__device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceArgs &a,
+                          int *aux) {
   const int lane = threadIdx.x;
   if (status[k] != GH_OK) {
     if (lane == 0) {
@@ -2735,7 +2837,7 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
   // slot is first taken as an intersection vertex, then the first slot of
   // each edge is re-flagged as its parent vertex
   const bool xl = a.explicitLinks != 0;
-  for (int j = lane; j < nP; j += 64) {
+  for (int j = lane; j < nP; j += W) {
     const long long sl = sbP + j;
     st(N.pn, j, ld(s.iP, sl));
     st(N.po, j, ld(s.oP, sl));
@@ -2751,7 +2853,7 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
     N.enex[j] = NEITHER;
     N.fl[j] = F_INT;
   }
-  for (int j = lane; j < nQ; j += 64) {
+  for (int j = lane; j < nQ; j += W) {
     const long long sl = sbQ + j;
     const int v = nP + j;
     st(N.pn, v, ld(s.iQ, sl));
@@ -2766,12 +2868,12 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
     N.fl[v] = F_INT;
   }
   __syncthreads();
-  for (long long e = pE0 + lane; e < pE1; e += 64) {
+  for (long long e = pE0 + lane; e < pE1; e += W) {
     const long long sl = a.psP2[e];
     const bool linked = xl ? s.nbP[sl] > 0 : s.aP[sl] != kNone;
     N.fl[sl - sbP] = (unsigned char)(F_SRC | (linked ? F_INT : 0));
   }
-  for (long long e = qE0 + lane; e < qE1; e += 64) {
+  for (long long e = qE0 + lane; e < qE1; e += W) {
     const long long sl = a.psQ2[e];
     const bool linked = s.aQ[sl] != kNone || s.nbQ[sl] > 0;
     N.fl[nP + (sl - sbQ)] = (unsigned char)(F_SRC | (linked ? F_INT : 0));
@@ -2779,16 +2881,16 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
   __syncthreads();
   if (!xl) {
     // a P node linked from several Q slots keeps the last one (link() order)
-    for (int j = lane; j < nQ; j += 64)
+    for (int j = lane; j < nQ; j += W)
       if (N.nb[nP + j] >= 0) atomicMax(&N.nb[N.nb[nP + j]], nP + j);
     __syncthreads();
-    for (int j = lane; j < nP; j += 64)
+    for (int j = lane; j < nP; j += W)
       if (N.nb[j] >= 0) N.fl[j] |= F_INT;
     __syncthreads();
   }
   // -------- step 1 (restored) + step 3, one lane per P intersection vertex
   bool slow = false;
-  for (int I = lane; I < nP; I += 64) {
+  for (int I = lane; I < nP; I += W) {
     bool bad = false;
     if (isint(N, I)) {
       const int q = N.nb[I];
@@ -2816,16 +2918,16 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
   __syncthreads();
   // links must be mutual on both rings (several P sources may point at one
   // Q vertex when links are explicit)
-  for (int j = lane; j < nQ; j += 64) {
+  for (int j = lane; j < nQ; j += W) {
     const int v = nP + j;
     if (isint(N, v)) {
       const int q = N.nb[v];
       if (q < 0 || N.nb[q] != v) slow = true;
     }
   }
-  for (int j = lane; j < nP; j += 64)
+  for (int j = lane; j < nP; j += W)
     if (isint(N, j) && (N.nb[j] < 0 || N.nb[N.nb[j]] != j)) slow = true;
-  slow = __ballot(slow) != 0ULL;
+  slow = grp_any<W>(slow);
   // -------- output regions
   OutBuf O;
   O.pn = a.opn + 2 * nb0;
@@ -2839,12 +2941,12 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
   O.capComp = (int)(a.compOff[k + 1] - a.compOff[k]);
   // crossings on P (the links are mutual here, so Q has as many)
   int nCrossP = 0;
-  for (int j = lane; j < nP; j += 64) nCrossP += iscross(N, j) ? 1 : 0;
-  nCrossP = wave_sum(nCrossP);
+  for (int j = lane; j < nP; j += W) nCrossP += iscross(N, j) ? 1 : 0;
+  nCrossP = grp_sum<W>(nCrossP);
   int VP = -1, VQ = -1;
   if (!slow) {
-    VP = wave_first_nonint(N, 0, nP);
-    VQ = wave_first_nonint(N, nP, nQ);
+    VP = grp_first_nonint<W>(N, 0, nP);
+    VQ = grp_first_nonint<W>(N, nP, nQ);
     if (VP < 0 || VQ < 0) slow = true;
   }
   double *xy = a.xyTmp + 2 * nb0;
@@ -2855,12 +2957,12 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
       // step 3.5: no crossings on either ring -> containment by PiP of
       // each ring's first non-intersection vertex
       path = LP_NO_CROSSING;
-      const bool pin = wave_pip(N, nP, nQ, P_(N, VP)), qin = wave_pip(N, 0, nP, P_(N, VQ));
+      const bool pin = grp_pip<W>(N, nP, nQ, P_(N, VP)), qin = grp_pip<W>(N, 0, nP, P_(N, VQ));
       if ((pin ? nP : 0) + (qin ? nQ : 0) > O.cap || (pin ? 1 : 0) + (qin ? 1 : 0) > O.capComp) {
         slow = true;
       } else {
         if (pin) {
-          copy_ring_out(O, xy, rg, N, 0, nP, 0, 0);
+          copy_ring_out<W>(O, xy, rg, N, 0, nP, 0, 0);
           if (lane == 0) {
             O.compRoot[0] = 0;
             clen[0] = nP;
@@ -2869,7 +2971,7 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
           O.n = nP;
         }
         if (qin) {
-          copy_ring_out(O, xy, rg, N, nP, nQ, O.n, O.ncomp);
+          copy_ring_out<W>(O, xy, rg, N, nP, nQ, O.n, O.ncomp);
           if (lane == 0) {
             O.compRoot[O.ncomp] = O.n;
             clen[O.ncomp] = nQ;
@@ -2880,10 +2982,12 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
       }
     } else {
       // step 4
-      wave_enex(N, 0, nP, VP, wave_pip(N, nP, nQ, P_(N, VP)) ? EXIT : ENTRY);
-      wave_enex(N, nP, nQ, VQ, wave_pip(N, 0, nP, P_(N, VQ)) ? EXIT : ENTRY);
+      const bool pinP = grp_pip<W>(N, nP, nQ, P_(N, VP));
+      grp_enex<W>(N, 0, nP, VP, pinP ? EXIT : ENTRY);
+      const bool pinQ = grp_pip<W>(N, 0, nP, P_(N, VQ));
+      grp_enex<W>(N, nP, nQ, VQ, pinQ ? EXIT : ENTRY);
       __syncthreads();
-      const int rc = segment_trace(N, O, xy, clen, rg, nP, nQ);
+      const int rc = segment_trace<W>(N, O, xy, clen, rg, nP, nQ);
       if (rc == SEG_OVERFLOW) {
         if (lane == 0) {
           status[k] = GH_OVERFLOW;
@@ -2926,13 +3030,23 @@ __device__ int label_wave_pair(long k, int *status, const SlotArrays &s, const T
     return LP_SEQUENTIAL;
   }
   __syncthreads();
-  if (a.cleanup && wave_cleanup_compact(O, xy, clen, rg, rg, nP + nQ) && path == LP_SEGMENTS)
+  if (a.cleanup && grp_cleanup_compact<W>(O, xy, clen, rg, rg, nP + nQ) && path == LP_SEGMENTS)
     path = LP_SEG_CLEANUP;
   if (lane == 0) {
     a.outNComp[k] = O.ncomp;
     a.outNVerts[k] = O.n;
   }
   return path;
+}
+
+// Pairs with at least kBigNodes nodes are labelled by a block of kBigGroup
+// threads (k_label_block), the others by one wave each (k_label_wave).
+constexpr long long kBigNodes = 8192;
+constexpr int kBigGroup = 1024;
+
+__device__ __forceinline__ long long pair_nodes(const TraceArgs &a, long k) {
+  return a.psP2[a.offP[k + 1]] - a.psP2[a.offP[k]] + a.psQ2[a.offQ[k + 1]] -
+         a.psQ2[a.offQ[k]];
 }
 
 // One wave per pair; records the path each pair took and its duration in
@@ -2942,11 +3056,42 @@ __global__ void __launch_bounds__(64)
                  long long *lticks) {
   const long k = blockIdx.x;
   if (k >= M) return;
+  if (pair_nodes(a, k) >= kBigNodes) return;  // k_label_block's
   const long long t0 = wall_clock64();
-  const int path = label_wave_pair(k, status, s, a, aux);
+  const int path = label_pair<64>(k, status, s, a, aux);
   if (threadIdx.x == 0 && lpath) {
     lpath[k] = path;
     lticks[k] = wall_clock64() - t0;
+  }
+}
+
+// the pairs k_label_block takes (by size alone, whatever their status, so
+// that each pair is k_label_wave's or k_label_block's), appended to list;
+// cnt[0] their number
+__global__ void __launch_bounds__(1024)
+    k_big_pairs(long M, TraceArgs a, int *list, unsigned long long *cnt) {
+  __shared__ unsigned long long lds[17];
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool big = k < M && pair_nodes(a, k) >= kBigNodes;
+  const long i = block_append(cnt, big, lds);
+  if (i >= 0) list[i] = (int)k;
+}
+
+// One block of kBigGroup threads per listed pair, the blocks taking the list
+// in turn
+__global__ void __launch_bounds__(kBigGroup)
+    k_label_block(const int *list, const unsigned long long *cnt, int *status, SlotArrays s,
+                  TraceArgs a, int *aux, int *lpath, long long *lticks) {
+  const long n = (long)*cnt;
+  for (long i = blockIdx.x; i < n; i += gridDim.x) {
+    const long k = list[i];
+    const long long t0 = wall_clock64();
+    const int path = label_pair<kBigGroup>(k, status, s, a, aux);
+    if (threadIdx.x == 0 && lpath) {
+      lpath[k] = path;
+      lticks[k] = wall_clock64() - t0;
+    }
+    __syncthreads();
   }
 }
 
@@ -3526,10 +3671,31 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     int *aux = wsT<int>(c, "gh_aux", 5 * nodes);
     res.labelPath = wsT<int>(c, "gh_lpath", M);
     res.labelTicks = wsT<long long>(c, "gh_lticks", M);
+    int *bigList = wsT<int>(c, "gh_bigList", M);
+    unsigned long long *bigCnt = wsT<unsigned long long>(c, "gh_bigCnt", 1);
+    fill_i32(c, (int *)bigCnt, 0, 2);
+    hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M, ta,
+                       bigList, bigCnt);
+    HIP_CHECK(hipGetLastError());
+    if (!c.side) {
+      HIP_CHECK(hipStreamCreate(&c.side));
+      HIP_CHECK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+    }
     LaunchTimer t(c, "gh_label_trace");
+    // the large pairs on the side stream, concurrently with one wave per
+    // remaining pair on the main one (disjoint pairs, disjoint regions)
+    HIP_CHECK(hipEventRecord(c.ev_fork, c.stream));
+    HIP_CHECK(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+    hipLaunchKernelGGL(k_label_block, dim3(256), dim3(kBigGroup), 0, c.side,
+                       (const int *)bigList, (const unsigned long long *)bigCnt, pm.status, s, ta,
+                       aux, res.labelPath, res.labelTicks);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(c.ev_join, c.side));
     hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), 0, c.stream, M, pm.status, s,
                        ta, aux, res.labelPath, res.labelTicks);
     HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamWaitEvent(c.stream, c.ev_join, 0));
   }
   res.vertOff = wsT<long long>(c, "gh_vOff", M + 1);
   res.compOff = wsT<long long>(c, "gh_cOff", M + 1);
