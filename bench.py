@@ -109,11 +109,16 @@ def main():
     ap.add_argument("--n", type=int, default=20000, help="polygons per layer")
     ap.add_argument("--cpu-sample", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["join-clip", "batched", "large"],
+    ap.add_argument("--workload", choices=["join-clip", "batched", "large", "serve"],
                     default="join-clip",
                     help="join-clip: the two-layer join + clip (default); batched: "
                          "independent uneven pairs clipped in one pass; large: one pair of "
-                         "1e6 x 1e5 vertices")
+                         "1e6 x 1e5 vertices; serve: Poisson stream of clip and join requests")
+    ap.add_argument("--rate", type=float, default=2000.0, help="requests/s for --workload serve")
+    ap.add_argument("--duration", type=float, default=20.0,
+                    help="seconds of arrivals for --workload serve")
+    ap.add_argument("--hbm-budget", type=float, default=200e9,
+                    help="free-HBM admission budget in bytes for --workload serve")
     ap.add_argument("--large-n", type=int, nargs=2, default=[1000000, 100000],
                     help="ring sizes for --workload large")
     ap.add_argument("--snap", type=float, default=0.0,
@@ -138,6 +143,8 @@ def main():
         return batched(args)
     if args.workload == "large":
         return large(args)
+    if args.workload == "serve":
+        return serve(args)
 
     import gcmf_amd as G
     print("generating layers ...", file=sys.stderr, flush=True)
@@ -372,6 +379,61 @@ def large(args):
     db.free()
     do.free()
     ctx.close()
+    print(json.dumps(line))
+
+
+def serve(args):
+    """Served stream: Poisson arrivals at --rate requests/s for --duration s;
+    90 % clips of one pair drawn from a resident pool of the batched
+    workload's pairs, 10 % joins of two resident layers drawn from four layer
+    sets of the join workload's generator at 1000, 5000, 10000 and 20000
+    polygons; admission by a free-HBM budget."""
+    import gcmf_amd as G
+    import serve as S
+    print("generating the pool ...", file=sys.stderr, flush=True)
+    P, Q = G.gen_pair_layers(args.pairs, seed=0)
+    sets = [G.gen_join_layers(n, n, seed=10 + i) for i, n in enumerate((1000, 5000, 10000,
+                                                                        20000))]
+    srv = S.Server(P, Q, sets, hbm_budget=args.hbm_budget)
+    # warm-up: one batch and one join of each set, untimed
+    warm = [S.Request(0, "clip", 0.0, 0)] + [S.Request(1 + i, "join", 0.0, i) for i in range(4)]
+    srv.run(warm)
+    srv.batches.clear()
+    srv.deferred = 0
+    reqs = S.poisson_requests(args.rate, args.duration, 0.9, args.pairs, len(sets), seed=0)
+    print("serving %d requests over %.0f s ..." % (len(reqs), args.duration), file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    srv.run(reqs)
+    wall = time.perf_counter() - t0
+    lat = S.latency_summary(reqs)
+    clips = [r for r in reqs if r.kind == "clip"]
+    joins = [r for r in reqs if r.kind == "join"]
+    makespan = max(r.t_done for r in reqs)
+    sizes = [b[2] for b in srv.batches]
+    line = {"metric": METRIC, "value": round(len(clips) / makespan, 2), "unit": "pairs/s",
+            "n_gpus": 1, "steps": len(srv.batches), "warmup": 1,
+            "ms_per_step": round(1e3 * float(np.mean([b[1] - b[0] for b in srv.batches])), 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded generators, no dataset)",
+            "config": {"workload": "served stream: Poisson %g req/s for %g s, 90%% clips of "
+                                   "one pair from a resident pool of %d log-normal pairs, 10%% "
+                                   "joins of resident layer pairs of 1k/5k/10k/20k polygons, "
+                                   "free-HBM budget %.0f GB" % (args.rate, args.duration,
+                                                                args.pairs,
+                                                                args.hbm_budget / 1e9),
+                       "requests": len(reqs), "clips": len(clips), "joins": len(joins),
+                       "clip_batches": len(srv.batches),
+                       "batch_size_mean": round(float(np.mean(sizes)), 2),
+                       "batch_size_max": int(max(sizes)),
+                       "deferred_by_budget": srv.deferred,
+                       "failed_clips": sum(1 for r in clips if r.status not in (0, 2))},
+            "latency_ms_p50": lat["all"]["p50_ms"], "latency_ms_p99": lat["all"]["p99_ms"],
+            "latency": lat,
+            "join_candidates_per_s": round(sum(r.summary[0] for r in joins) / makespan, 2),
+            "requests_per_s": round(len(reqs) / makespan, 2),
+            "wall_s": round(wall, 3)}
+    srv.close()
     print(json.dumps(line))
 
 

@@ -251,6 +251,31 @@ def test_clip_edge_scaled_fine_pair(gctx):
     assert abs(O.comps_area(res[0], ox, oy) - A) <= AREA_RTOL * A
 
 
+def test_clip_large_pair_fixture(gctx):
+    """The 200000 x 20000 pair of the large-pair shape in the edge-scaled
+    frame, against the oracle's result committed in
+    tests/golden/large_pair_200k.json (made by make_large_pair.py, 2.5 min on
+    one core): the same status and component sizes, bit-identical vertices
+    (SHA-256 of the float64 coordinates).  The committed result is cleaned
+    (cleanUpResult drops vertices whose doubled turn area is below the
+    absolute 1e-7), which moves its area by 3.5e-10 relative from the
+    boundary integral; the traced area without clean-up is checked at 1e-12
+    on smaller pairs."""
+    import hashlib
+    import json
+    with open(GOLDEN + "/large_pair_200k.json") as f:
+        ref = json.load(f)
+    assert abs(ref["area_result"] - ref["area_boundary"]) <= 1e-9 * ref["area_boundary"]
+    P, Q = G.gen_single_pair(1, 0, 200000, 20000)
+    st = run_batch(gctx, [("large", P, Q)], G.CLIP_NORMALIZE | G.CLIP_SCALE_EDGES)
+    status, res = gctx.clip_results(st)
+    assert status[0] == ref["status"] == 0
+    assert st["n_intersections"] == ref["n_inter"]
+    assert [len(c) for c in res[0]] == ref["comp_sizes"]
+    xy = np.ascontiguousarray(np.concatenate(res[0]), np.float64)
+    assert hashlib.sha256(xy.tobytes()).hexdigest() == ref["xy_sha256"]
+
+
 def test_clip_area_oracle(gctx):
     cases = random_cases(40, seed0=300, nmax=400)
     P, Q = G.gen_single_pair(0, 0, 2000, 1500)
