@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--n", type=int, default=20000, help="polygons per layer")
     ap.add_argument("--cpu-sample", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["join-clip", "batched", "large", "serve"],
+    ap.add_argument("--workload", choices=["join-clip", "batched", "large", "serve", "prims"],
                     default="join-clip",
                     help="join-clip: the two-layer join + clip (default); batched: "
                          "independent uneven pairs clipped in one pass; large: one pair of "
@@ -145,6 +145,8 @@ def main():
         return large(args)
     if args.workload == "serve":
         return serve(args)
+    if args.workload == "prims":
+        return prims(args)
 
     import gcmf_amd as G
     print("generating layers ...", file=sys.stderr, flush=True)
@@ -380,6 +382,88 @@ def large(args):
     do.free()
     ctx.close()
     print(json.dumps(line))
+
+
+def prims(args):
+    """HBM sweep of the device primitives and the MBR filter at 2^20..2^27
+    elements: achieved algorithmic GB/s and the fraction of MI355X's 8 TB/s.
+    Algorithmic bytes: sort P passes x (8 + 4 read + 8 + 4 written) per key
+    (P = bits / 8); scan 4 read + 8 written per element; compaction 4 read
+    per flag + 4 written per kept index; MBR filter (B + O) x 32 read + K x 8
+    written (the candidate pairs)."""
+    import gcmf_amd as G
+    ctx = G.Context(0)
+    rs = np.random.RandomState(0)
+    out = []
+
+    def timed(fn, reps):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    for lg in range(20, 28):
+        n = 1 << lg
+        keys = rs.randint(0, 1 << 62, size=n, dtype=np.int64).astype(np.uint64)
+        bufs = [ctx.upload(keys), ctx.upload(np.arange(n, dtype=np.int32)),
+                ctx.upload(np.zeros(n, np.uint64)), ctx.upload(np.zeros(n, np.int32))]
+        for bits in (32, 64):
+            t = timed(lambda: ctx.radix_sort(bufs[0], bufs[1], bufs[2], bufs[3], n, bits), 3)
+            gb = (bits // 8) * n * 24 / t / 1e9
+            out.append({"prim": "radix_sort", "n": n, "bits": bits, "ms": round(1e3 * t, 4),
+                        "GB/s": round(gb, 1), "frac": round(gb / HBM_PEAK_GBS, 4)})
+        for b in bufs:
+            b.free()
+        a = rs.randint(0, 100, size=n).astype(np.int32)
+        d_in, d_out = ctx.upload(a), ctx.upload(np.zeros(n + 1, np.int64))
+        t = timed(lambda: ctx.scan(d_in, d_out, n), 5)
+        gb = 12 * n / t / 1e9
+        out.append({"prim": "scan", "n": n, "ms": round(1e3 * t, 4), "GB/s": round(gb, 1),
+                    "frac": round(gb / HBM_PEAK_GBS, 4)})
+        d_idx = ctx.upload(np.zeros(n, np.int32))
+        flags = (a < 50).astype(np.int32)
+        d_f = ctx.upload(flags)
+        kept = int(flags.sum())
+        t = timed(lambda: ctx.compact(d_f, d_idx, n), 5)
+        gb = (4 * n + 4 * kept) / t / 1e9
+        out.append({"prim": "compact", "n": n, "ms": round(1e3 * t, 4), "GB/s": round(gb, 1),
+                    "frac": round(gb / HBM_PEAK_GBS, 4)})
+        for b in (d_in, d_out, d_idx, d_f):
+            b.free()
+        print(json.dumps(out[-4:]), file=sys.stderr, flush=True)
+    # MBR filter (endpoint sort + sweep) on layers of 8-vertex polygons with
+    # 2 (B + O) = 2^20 .. 2^24 endpoint keys
+    for lg in range(20, 25):
+        npoly = (1 << lg) // 4
+        r = 1.0 / np.sqrt(npoly)  # a few candidates per polygon at every size
+        base, over = G.gen_join_layers(npoly, npoly, seed=lg, rmin=0.2 * r, rmax=0.5 * r, nmin=8,
+                                       nmax=8)
+        db, do = G.DeviceLayer(ctx, base), G.DeviceLayer(ctx, over)
+        js = ctx.join(db, do)
+        ctx.timer_reset()
+        ctx.set_timing(True)
+        ctx.join(db, do)
+        ctx.synchronize()
+        ctx.set_timing(False)
+        names = ["mbr_endpoints", "sort_hist", "sort_scatter", "scan_tile", "scan_reduce",
+                 "scan_downsweep", "mbr_sweep_prep", "mbr_sweep_count", "mbr_sweep_write"]
+        ms = sum(ctx.timer(k)[0] for k in names)
+        sweep_ms = sum(ctx.timer(k)[0] for k in ("mbr_sweep_prep", "mbr_sweep_count",
+                                                  "mbr_sweep_write"))
+        K = js["candidates"]
+        gb = (2 * npoly * 32 + K * 8) / (ms * 1e-3) / 1e9
+        out.append({"prim": "mbr_filter", "keys": 4 * npoly, "candidates": K,
+                    "ms": round(ms, 4), "sweep_ms": round(sweep_ms, 4), "GB/s": round(gb, 1),
+                    "frac": round(gb / HBM_PEAK_GBS, 4)})
+        db.free()
+        do.free()
+        print(json.dumps(out[-1]), file=sys.stderr, flush=True)
+    ctx.close()
+    print(json.dumps({"workload": "primitive HBM sweep", "peak_GBs": HBM_PEAK_GBS,
+                      "results": out}))
 
 
 def serve(args):

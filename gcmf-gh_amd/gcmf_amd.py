@@ -248,6 +248,9 @@ def hip_lib():
         L.gcmf_join_clip.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), C.c_int,
                                      C.POINTER(JoinStats), C.POINTER(ClipStats)]
         L.gcmf_clip_get.argtypes = [vp, C.c_int, vp, C.c_long]
+        L.gcmf_radix_sort_u64.argtypes = [vp, vp, vp, vp, vp, C.c_long, C.c_int]
+        L.gcmf_scan_i32_i64.argtypes = [vp, vp, vp, C.c_long]
+        L.gcmf_compact_flags.argtypes = [vp, vp, C.c_long, vp, C.POINTER(C.c_long)]
         _hip = L
     return _hip
 
@@ -335,6 +338,24 @@ class Context:
         self._check(hip_lib().gcmf_timer_get(self.h, kernel.encode(), C.byref(ms), C.byref(n)),
                     "timer_get")
         return ms.value, n.value
+
+    # ------------------------------------------------------- primitives
+    def radix_sort(self, d_keys_in, d_vals_in, d_keys_out, d_vals_out, n, bits):
+        """Stable sort of n u64 keys (device buffers) by their low `bits` bits."""
+        self._check(hip_lib().gcmf_radix_sort_u64(self.h, d_keys_in.ptr, d_vals_in.ptr,
+                                                  d_keys_out.ptr, d_vals_out.ptr, int(n),
+                                                  int(bits)), "radix_sort")
+
+    def scan(self, d_in, d_out, n):
+        """Exclusive scan of n i32 into n + 1 i64 (device buffers)."""
+        self._check(hip_lib().gcmf_scan_i32_i64(self.h, d_in.ptr, d_out.ptr, int(n)), "scan")
+
+    def compact(self, d_flags, d_idx, n):
+        """Indices of the nonzero flags; returns their number."""
+        cnt = C.c_long()
+        self._check(hip_lib().gcmf_compact_flags(self.h, d_flags.ptr, int(n), d_idx.ptr,
+                                                 C.byref(cnt)), "compact")
+        return cnt.value
 
     # ------------------------------------------------------------- join
     def join(self, base, over):

@@ -63,6 +63,24 @@ int gcmf_set_timing_only(gcmf_ctx *ctx, const char *kernel);
 int gcmf_timer_reset(gcmf_ctx *ctx);
 int gcmf_timer_get(gcmf_ctx *ctx, const char *kernel, double *total_ms, long *launches);
 
+/* ------------------------------------------------------- primitives ---- */
+/* Device primitives on caller-owned device buffers, ordered on the context's
+ * stream and finished on return. */
+
+/* Stable LSD radix sort of n u64 keys carrying i32 values over their low
+ * `bits` bits (8 bits per pass, bits <= 64); replaces RadixSort
+ * (GPU_Utility.h:242-289) with its digit-bucket kernels
+ * (GPU_RadixSort.h:27-146, 16 decimal-digit passes there). */
+int gcmf_radix_sort_u64(gcmf_ctx *ctx, const uint64_t *d_keys_in, const int *d_vals_in,
+                        uint64_t *d_keys_out, int *d_vals_out, long n, int bits);
+/* Exclusive prefix sum of n i32 into n + 1 i64 (d_out[n] = the total, so
+ * d_out[1..n] is the inclusive scan); replaces PrefixSum int->long
+ * (GPU_Utility.h:128-154, GPU_PrefixSum.h:355-458). */
+int gcmf_scan_i32_i64(gcmf_ctx *ctx, const int *d_in, long long *d_out, long n);
+/* Indices of the nonzero flags in increasing order and their number;
+ * replaces Make_Filtered_Index_List (GPU_Vector.h:108-120). */
+int gcmf_compact_flags(gcmf_ctx *ctx, const int *d_flags, long n, int *d_idx, long *count);
+
 /* ------------------------------------------------------------ join ----- */
 typedef struct {
   long candidates;      /* K: MBR-filter pairs (SortBaseMBROverlap return) */

@@ -106,3 +106,43 @@ def test_join_then_clip_pipeline(gctx):
         assert [len(c) for c in res[k]] == [len(c) for c in comps]
         for a, c in zip(res[k], comps):
             np.testing.assert_array_equal(a, c)
+
+
+@pytest.mark.parametrize("n,bits", [(1, 8), (5000, 13), (300000, 32), (1 << 20, 64)])
+def test_radix_sort_entry_point(gctx, n, bits):
+    """gcmf_radix_sort_u64 equals numpy's stable argsort on the low `bits`
+    bits (the closed form of a stable LSD radix sort)."""
+    rs = np.random.RandomState(n)
+    keys = rs.randint(0, 1 << 62, size=n, dtype=np.int64).astype(np.uint64)
+    keys[: n // 3] = keys[n // 3: 2 * (n // 3)]  # ties
+    mask = np.uint64((1 << bits) - 1) if bits < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    keys &= mask
+    vals = np.arange(n, dtype=np.int32)
+    bufs = [gctx.upload(keys), gctx.upload(vals), gctx.upload(np.zeros(n, np.uint64)),
+            gctx.upload(np.zeros(n, np.int32))]
+    gctx.radix_sort(bufs[0], bufs[1], bufs[2], bufs[3], n, bits)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(gctx.download(bufs[3], np.int32, n), order)
+    np.testing.assert_array_equal(gctx.download(bufs[2], np.uint64, n), keys[order])
+    for b in bufs:
+        b.free()
+
+
+@pytest.mark.parametrize("n", [0, 1, 4097, 3_000_000])
+def test_scan_and_compact_entry_points(gctx, n):
+    """gcmf_scan_i32_i64 is the exclusive cumulative sum with the total last;
+    gcmf_compact_flags lists the nonzero flags in order."""
+    rs = np.random.RandomState(7)
+    a = rs.randint(-3, 1000, size=n).astype(np.int32)
+    d_in, d_out = gctx.upload(a), gctx.upload(np.zeros(n + 1, np.int64))
+    gctx.scan(d_in, d_out, n)
+    ref = np.concatenate([[0], np.cumsum(a.astype(np.int64))])
+    np.testing.assert_array_equal(gctx.download(d_out, np.int64, n + 1), ref)
+    flags = (rs.rand(n) < 0.3).astype(np.int32) * rs.randint(1, 5, size=n).astype(np.int32)
+    d_f, d_idx = gctx.upload(flags), gctx.upload(np.zeros(max(n, 1), np.int32))
+    cnt = gctx.compact(d_f, d_idx, n)
+    want = np.nonzero(flags)[0]
+    assert cnt == len(want)
+    np.testing.assert_array_equal(gctx.download(d_idx, np.int32, cnt), want)
+    for b in (d_in, d_out, d_f, d_idx):
+        b.free()
