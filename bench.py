@@ -40,7 +40,7 @@ def traffic_bytes(workload, args):
     """HBM bytes per launch of the roofline kernel measured with rocprofv3
     FETCH_SIZE / WRITE_SIZE (scaled by a plain copy's reading) for this
     workload at its default size, from profiles/; None otherwise."""
-    fn = os.path.join(PROFILES, "k_count_traffic.json")
+    fn = os.path.join(PROFILES, "k_count_cpu_traffic.json")
     if not os.path.exists(fn):
         return None
     with open(fn) as f:
@@ -59,7 +59,7 @@ def roofline_line(total_ms, launches, n_eval, traffic):
     achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     return {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS,
             "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFS, 6), "traffic": traffic,
-            "kernel": "k_count", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+            "kernel": "k_count_cpu", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
             "work": "%d classified edge pairs x %d FP64 flops" % (n_eval, FLOPS_PER_EVAL)}
 
 

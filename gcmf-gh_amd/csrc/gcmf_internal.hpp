@@ -109,7 +109,7 @@ T read_scalar(Context &c, const T *dptr) {
 
 // copies up to kHostSlots int64 values back to the host in one transfer
 // (synchronises); null sources read as 0
-constexpr int kHostSlots = 16;
+constexpr int kHostSlots = 24;
 void read_scalars(Context &c, std::initializer_list<const long long *> src, long long *out);
 
 inline unsigned grid_for(long n, int block) { return (unsigned)((n + block - 1) / block); }

@@ -3040,13 +3040,18 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
 }
 
 // Pairs with at least kBigNodes nodes are labelled by a block of kBigGroup
-// threads (k_label_block), the others by one wave each (k_label_wave).
+// threads (k_label_block; 512 keeps the sequential fallback's registers
+// without spills), the others by one wave each (k_label_wave).
 constexpr long long kBigNodes = 8192;
-constexpr int kBigGroup = 1024;
+constexpr int kBigGroup = 512;
 
+__device__ __forceinline__ long long pair_nodes(const long long *offP, const long long *offQ,
+                                                const long long *psP2, const long long *psQ2,
+                                                long k) {
+  return psP2[offP[k + 1]] - psP2[offP[k]] + psQ2[offQ[k + 1]] - psQ2[offQ[k]];
+}
 __device__ __forceinline__ long long pair_nodes(const TraceArgs &a, long k) {
-  return a.psP2[a.offP[k + 1]] - a.psP2[a.offP[k]] + a.psQ2[a.offQ[k + 1]] -
-         a.psQ2[a.offQ[k]];
+  return pair_nodes(a.offP, a.offQ, a.psP2, a.psQ2, k);
 }
 
 // One wave per pair; records the path each pair took and its duration in
@@ -3069,10 +3074,11 @@ __global__ void __launch_bounds__(64)
 // that each pair is k_label_wave's or k_label_block's), appended to list;
 // cnt[0] their number
 __global__ void __launch_bounds__(1024)
-    k_big_pairs(long M, TraceArgs a, int *list, unsigned long long *cnt) {
+    k_big_pairs(long M, const long long *offP, const long long *offQ, const long long *psP2,
+                const long long *psQ2, int *list, unsigned long long *cnt) {
   __shared__ unsigned long long lds[17];
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool big = k < M && pair_nodes(a, k) >= kBigNodes;
+  const bool big = k < M && pair_nodes(offP, offQ, psP2, psQ2, k) >= kBigNodes;
   const long i = block_append(cnt, big, lds);
   if (i >= 0) list[i] = (int)k;
 }
@@ -3302,13 +3308,22 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   int *actP = wsT<int>(c, "gh_actP", edgesP), *actQ = wsT<int>(c, "gh_actQ", edgesQ);
   const long long *nActPd = compact_flags_dev(c, cnt1P, edgesP, actP, "actP");
   const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
+  // the pairs labelled by a block (their number comes back with the totals)
+  int *bigList = wsT<int>(c, "gh_bigList", M);
+  unsigned long long *bigCnt = wsT<unsigned long long>(c, "gh_bigCnt", 1);
+  fill_i32(c, (int *)bigCnt, 0, 2);
+  hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M,
+                     (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
+                     (const long long *)psQ2, bigList, bigCnt);
+  HIP_CHECK(hipGetLastError());
   {
     const long long *rc = (const long long *)runCnt;
     read_scalars(c, {psP2 + edgesP, psQ2 + edgesQ, rc, rc + 2, rc + 3, rc + 8, rc + 10, rc + 11,
                      nActPd, nActQd, psP1 + edgesP, psQ1 + edgesQ, rc + 1, rc + 9, rc + 4,
-                     rc + 12},
+                     rc + 12, (const long long *)bigCnt},
                  rb);
   }
+  const long nBig = (long)rb[16];
   const long nActP = (long)rb[8], nActQ = (long)rb[9];
   const long nShortP = (long)rb[12], nShortQ = (long)rb[13];
   const long nOverP = (long)rb[14], nOverQ = (long)rb[15];
@@ -3671,13 +3686,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     int *aux = wsT<int>(c, "gh_aux", 5 * nodes);
     res.labelPath = wsT<int>(c, "gh_lpath", M);
     res.labelTicks = wsT<long long>(c, "gh_lticks", M);
-    int *bigList = wsT<int>(c, "gh_bigList", M);
-    unsigned long long *bigCnt = wsT<unsigned long long>(c, "gh_bigCnt", 1);
-    fill_i32(c, (int *)bigCnt, 0, 2);
-    hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M, ta,
-                       bigList, bigCnt);
-    HIP_CHECK(hipGetLastError());
-    if (!c.side) {
+    if (nBig > 0 && !c.side) {
       HIP_CHECK(hipStreamCreate(&c.side));
       HIP_CHECK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
       HIP_CHECK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
@@ -3685,17 +3694,20 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     LaunchTimer t(c, "gh_label_trace");
     // the large pairs on the side stream, concurrently with one wave per
     // remaining pair on the main one (disjoint pairs, disjoint regions)
-    HIP_CHECK(hipEventRecord(c.ev_fork, c.stream));
-    HIP_CHECK(hipStreamWaitEvent(c.side, c.ev_fork, 0));
-    hipLaunchKernelGGL(k_label_block, dim3(256), dim3(kBigGroup), 0, c.side,
-                       (const int *)bigList, (const unsigned long long *)bigCnt, pm.status, s, ta,
-                       aux, res.labelPath, res.labelTicks);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(c.ev_join, c.side));
+    if (nBig > 0) {
+      HIP_CHECK(hipEventRecord(c.ev_fork, c.stream));
+      HIP_CHECK(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+      hipLaunchKernelGGL(k_label_block, dim3((unsigned)(nBig < 128 ? nBig : 128)),
+                         dim3(kBigGroup), 0, c.side, (const int *)bigList,
+                         (const unsigned long long *)bigCnt, pm.status, s, ta, aux,
+                         res.labelPath, res.labelTicks);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipEventRecord(c.ev_join, c.side));
+    }
     hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), 0, c.stream, M, pm.status, s,
                        ta, aux, res.labelPath, res.labelTicks);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+    if (nBig > 0) HIP_CHECK(hipStreamWaitEvent(c.stream, c.ev_join, 0));
   }
   res.vertOff = wsT<long long>(c, "gh_vOff", M + 1);
   res.compOff = wsT<long long>(c, "gh_cOff", M + 1);
