@@ -257,7 +257,7 @@ __device__ __forceinline__ void ring_edges(const double *V, int n, F &&f) {
 // run whose box misses the edge's box holds no edge that lsmf could accept,
 // and the edges of the other runs are still visited in index order.
 constexpr int kRun1 = 16, kRun2 = 256, kRun3 = 4096;
-constexpr long kCullMin = 4096;
+constexpr long kCullMin = 512;
 
 struct RunBoxes {
   const double *b1, *b2, *b3;  // [4 * runs] x0, y0, x1, y1
@@ -3040,10 +3040,12 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
 }
 
 // Pairs with at least kBigNodes nodes are labelled by a block of kBigGroup
-// threads (k_label_block; 512 keeps the sequential fallback's registers
-// without spills), the others by one wave each (k_label_wave).
+// threads (k_label_block), the others by one wave each (k_label_wave).  At
+// 1024 threads the sequential fallback spills a few registers; the extra
+// parallelism of the block is worth more (the 1e6 x 1e5 pair labels in 113
+// ms at 1024 threads, 167 ms at 512).
 constexpr long long kBigNodes = 8192;
-constexpr int kBigGroup = 512;
+constexpr int kBigGroup = 1024;
 
 __device__ __forceinline__ long long pair_nodes(const long long *offP, const long long *offQ,
                                                 const long long *psP2, const long long *psQ2,
