@@ -255,6 +255,109 @@ __global__ void k_copy_total(TO *dst, const long long *src) {
   if (threadIdx.x == 0) *dst = (TO)*src;
 }
 
+// ---- i32 -> i64 scans of large arrays: 16-byte loads and stores.
+// Thread t of a tile holds the four int4 groups at tile offsets k * 1024 +
+// 4 t (k = 0..3), so each load instruction of a wave reads 1 KiB
+// contiguously; the tile is scanned group by group with a carry, and every
+// thread writes its four results of a group as two 16-byte stores.
+constexpr int kVecTile = 4096;
+
+__device__ __forceinline__ int4 ld_i4(const int *in, long long base, int k) {
+  return *(const int4 *)(in + base + k * 1024 + 4 * threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_vscan_reduce(const int *in, long n, long long *partials) {
+  __shared__ long long lds[4];
+  const long long base = (long long)blockIdx.x * kVecTile;
+  long long s = 0;
+  if (base + kVecTile <= n) {
+    int4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = ld_i4(in, base, k);
+#pragma unroll
+    for (int k = 0; k < 4; k++) s += (long long)v[k].x + v[k].y + v[k].z + v[k].w;
+  } else {
+    for (int k = 0; k < 4; k++)
+      for (int j = 0; j < 4; j++) {
+        const long long i = base + k * 1024 + 4 * threadIdx.x + j;
+        if (i < n) s += in[i];
+      }
+  }
+  long long tot;
+  block_excl_scan<long long>(s, lds, tot);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// offsets[b] = the sum of tiles before b
+__global__ void __launch_bounds__(256)
+    k_vscan_down(const int *in, long n, long long *out, const long long *offsets) {
+  __shared__ long long lds[4];
+  const long long base = (long long)blockIdx.x * kVecTile;
+  long long carry = offsets[blockIdx.x];
+  const bool full = base + kVecTile <= n;
+  int4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (full) {
+      v[k] = ld_i4(in, base, k);
+    } else {
+      const long long i = base + k * 1024 + 4 * threadIdx.x;
+      v[k].x = i < n ? in[i] : 0;
+      v[k].y = i + 1 < n ? in[i + 1] : 0;
+      v[k].z = i + 2 < n ? in[i + 2] : 0;
+      v[k].w = i + 3 < n ? in[i + 3] : 0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const long long s = (long long)v[k].x + v[k].y + v[k].z + v[k].w;
+    long long tot;
+    const long long e = carry + block_excl_scan<long long>(s, lds, tot);
+    const long long o0 = e, o1 = o0 + v[k].x, o2 = o1 + v[k].y, o3 = o2 + v[k].z;
+    const long long i = base + k * 1024 + 4 * threadIdx.x;
+    if (full) {
+      longlong2 *d = (longlong2 *)(out + i);
+      d[0] = make_longlong2(o0, o1);
+      d[1] = make_longlong2(o2, o3);
+    } else {
+      if (i < n) out[i] = o0;
+      if (i + 1 < n) out[i + 1] = o1;
+      if (i + 2 < n) out[i + 2] = o2;
+      if (i + 3 < n) out[i + 3] = o3;
+    }
+    carry += tot;
+  }
+}
+
+template <typename TI, typename TO>
+static void scan_impl(Context &c, const TI *in, TO *out, long n, int depth);
+
+// the vector path: both pointers 16-byte aligned and n of at least a few
+// tiles
+static bool vscan(Context &c, const int *in, long long *out, long n) {
+  if (n < 4L * kVecTile || ((uintptr_t)in & 15) || ((uintptr_t)out & 15)) return false;
+  const long tiles = (n + kVecTile - 1) / kVecTile;
+  long long *partials = (long long *)c.ws.get("vscan_partials", sizeof(long long) * (tiles + 1));
+  {
+    LaunchTimer t(c, "scan_reduce");
+    hipLaunchKernelGGL(k_vscan_reduce, dim3((unsigned)tiles), dim3(256), 0, c.stream, in, n,
+                       partials);
+    HIP_CHECK(hipGetLastError());
+  }
+  // exclusive scan of the tile sums in place, the total at partials[tiles]
+  scan_impl<long long, long long>(c, partials, partials, tiles, 1);
+  {
+    LaunchTimer t(c, "scan_downsweep");
+    hipLaunchKernelGGL(k_vscan_down, dim3((unsigned)tiles), dim3(256), 0, c.stream, in, n, out,
+                       (const long long *)partials);
+    HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL((k_copy_total<long long>), dim3(1), dim3(64), 0, c.stream, out + n,
+                     (const long long *)(partials + tiles));
+  HIP_CHECK(hipGetLastError());
+  return true;
+}
+
 template <typename TI, typename TO>
 static void scan_impl(Context &c, const TI *in, TO *out, long n, int depth) {
   // out has n+1 entries; out[n] = total.
@@ -311,6 +414,7 @@ void scan_i32_to_i64(Context &c, const int *in, long long *out, long n) {
     HIP_CHECK(hipGetLastError());
     return;
   }
+  if (vscan(c, in, out, n)) return;
   scan_impl<int, long long>(c, in, out, n, 0);
 }
 
@@ -334,8 +438,81 @@ __global__ void k_compact(const int *flag, const long long *pos, long n, int *ou
   if (i < n && flag[i] != 0) out[pos[i]] = (int)i;
 }
 
+// Two-pass compaction of large flag arrays, with the vector scan's tile
+// layout: nonzero counts per tile, their scan, then every tile rescans its
+// flags and writes the indices of the nonzero ones.
+__global__ void __launch_bounds__(256) k_vcompact_count(const int *flag, long n, long long *cnt) {
+  __shared__ long long lds[4];
+  const long long base = (long long)blockIdx.x * kVecTile;
+  long long s = 0;
+  if (base + kVecTile <= n) {
+    int4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = ld_i4(flag, base, k);
+#pragma unroll
+    for (int k = 0; k < 4; k++) s += (v[k].x != 0) + (v[k].y != 0) + (v[k].z != 0) + (v[k].w != 0);
+  } else {
+    for (int k = 0; k < 4; k++)
+      for (int j = 0; j < 4; j++) {
+        const long long i = base + k * 1024 + 4 * threadIdx.x + j;
+        if (i < n) s += flag[i] != 0;
+      }
+  }
+  long long tot;
+  block_excl_scan<long long>(s, lds, tot);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256)
+    k_vcompact_write(const int *flag, long n, const long long *offsets, int *out) {
+  __shared__ long long lds[4];
+  const long long base = (long long)blockIdx.x * kVecTile;
+  long long carry = offsets[blockIdx.x];
+  const bool full = base + kVecTile <= n;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const long long i = base + k * 1024 + 4 * threadIdx.x;
+    int4 v;
+    if (full) {
+      v = ld_i4(flag, base, k);
+    } else {
+      v.x = i < n ? flag[i] : 0;
+      v.y = i + 1 < n ? flag[i + 1] : 0;
+      v.z = i + 2 < n ? flag[i + 2] : 0;
+      v.w = i + 3 < n ? flag[i + 3] : 0;
+    }
+    const int f0 = v.x != 0, f1 = v.y != 0, f2 = v.z != 0, f3 = v.w != 0;
+    long long tot;
+    long long p = carry + block_excl_scan<long long>(f0 + f1 + f2 + f3, lds, tot);
+    if (f0) out[p++] = (int)i;
+    if (f1) out[p++] = (int)(i + 1);
+    if (f2) out[p++] = (int)(i + 2);
+    if (f3) out[p++] = (int)(i + 3);
+    carry += tot;
+  }
+}
+
+// the vector path of compaction; returns the device address of the count, or
+// nullptr when the arrays do not suit it
+static const long long *vcompact(Context &c, const int *flag, long n, int *out_idx,
+                                 const std::string &t) {
+  if (n < 4L * kVecTile || ((uintptr_t)flag & 15)) return nullptr;
+  const long tiles = (n + kVecTile - 1) / kVecTile;
+  long long *cnt = (long long *)c.ws.get(t + "_vc", sizeof(long long) * (tiles + 1));
+  hipLaunchKernelGGL(k_vcompact_count, dim3((unsigned)tiles), dim3(256), 0, c.stream, flag, n,
+                     cnt);
+  HIP_CHECK(hipGetLastError());
+  scan_impl<long long, long long>(c, cnt, cnt, tiles, 1);
+  hipLaunchKernelGGL(k_vcompact_write, dim3((unsigned)tiles), dim3(256), 0, c.stream, flag, n,
+                     (const long long *)cnt, out_idx);
+  HIP_CHECK(hipGetLastError());
+  return cnt + tiles;
+}
+
 long compact_flags(Context &c, const int *flag, long n, int *out_idx, const char *tag) {
   if (n <= 0) return 0;
+  if (const long long *d = vcompact(c, flag, n, out_idx, std::string("compact_") + tag))
+    return (long)read_scalar(c, d);
   std::string t = std::string("compact_") + tag;
   int *f01 = (int *)c.ws.get(t + "_f", sizeof(int) * (size_t)n);
   long long *pos = (long long *)c.ws.get(t + "_p", sizeof(long long) * (size_t)(n + 1));
@@ -357,6 +534,7 @@ const long long *compact_flags_dev(Context &c, const int *flag, long n, int *out
     HIP_CHECK(hipGetLastError());
     return pos;
   }
+  if (const long long *d = vcompact(c, flag, n, out_idx, t)) return d;
   int *f01 = (int *)c.ws.get(t + "_f", sizeof(int) * (size_t)n);
   hipLaunchKernelGGL(k_flags01, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, flag, f01, n);
   HIP_CHECK(hipGetLastError());
