@@ -127,6 +127,8 @@ def main():
                     help="--workload large without GCMF_CLIP_SCALE_EDGES (the reference's "
                          "absolute epsilon in the CMBR frame)")
     ap.add_argument("--pairs", type=int, default=200000, help="pairs for --workload batched")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batched: keep the default shapes instead of autotuning them")
     ap.add_argument("--per-pair-sample", type=int, default=2000,
                     help="pairs clipped one call each for the per-pair comparison (batched)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -238,6 +240,12 @@ def batched(args):
                                args.pairs, G.CLIP_NORMALIZE, C.byref(st)), "gcmf_clip")
         return st.as_dict()
 
+    # launch shapes chosen by the autotuner on this batch (untimed)
+    t0 = time.perf_counter()
+    tuned = ctx.tune_clip(dP, dQ, d_pairs, args.pairs) if not args.no_tune else ctx.tuning()
+    tune_s = time.perf_counter() - t0
+    print("tuning: %s (%.1f s)" % (tuned, tune_s), file=sys.stderr, flush=True)
+
     for _ in range(args.warmup):
         cs = step()
     ctx.timer_reset()
@@ -300,7 +308,9 @@ def batched(args):
                                    "counts (median 64, sigma 1.5, 8..50000), 70%% "
                                    "overlapping, FP64, seed 0" % args.pairs,
                        "ok": cs["ok"], "disjoint": cs["disjoint"], "failed": cs["failed"],
-                       "result_vertices": cs["total_verts"]},
+                       "result_vertices": cs["total_verts"],
+                       "autotuned": None if args.no_tune else tuned,
+                       "tuning_s": round(tune_s, 2)},
             "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 4),
             "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 4),
             "kernel_ms_per_step": kernel_ms,

@@ -299,6 +299,90 @@ int gcmf_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
   });
 }
 
+int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out) {
+  return guarded(ctx, [&] {
+    if (!out) throw gcmf::Error{GCMF_ERR_ARG, "null tuning"};
+    out->cull_min = ctx->c.tune.cullMin;
+    out->big_nodes = ctx->c.tune.bigNodes;
+    out->count_block = ctx->c.tune.countBlock;
+  });
+}
+
+int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
+  return guarded(ctx, [&] {
+    if (!t || t->cull_min < 1 || t->big_nodes < 1 ||
+        (t->count_block != 64 && t->count_block != 128 && t->count_block != 256))
+      throw gcmf::Error{GCMF_ERR_ARG, "bad tuning"};
+    ctx->c.tune.cullMin = t->cull_min;
+    ctx->c.tune.bigNodes = t->big_nodes;
+    ctx->c.tune.countBlock = t->count_block;
+  });
+}
+
+int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+                   const int *d_pairs, long M, int flags, gcmf_tuning *out) {
+  return guarded(ctx, [&] {
+    if (!base || !over || (M > 0 && !d_pairs) || M < 0)
+      throw gcmf::Error{GCMF_ERR_ARG, "bad tuning arguments"};
+    gcmf::Context &c = ctx->c;
+    const gcmf::JoinInputs in = to_inputs(base, over);
+    const bool timing0 = c.timing;
+    const std::string only0 = c.timing_only;
+    c.flush_timers();
+    const auto saved = c.timers;
+    gcmf_clip_stats st;
+    // milliseconds of the launches named `kernel` in one clip
+    auto measure = [&](const char *kernel) {
+      c.flush_timers();
+      c.timers.clear();
+      c.timing = true;
+      c.timing_only = kernel;
+      run_clip(ctx, in, d_pairs, M, flags, &st);
+      c.flush_timers();
+      auto it = c.timers.find(kernel);
+      return it == c.timers.end() ? 0.0 : it->second.total_ms;
+    };
+    gcmf::Tuning best = c.tune;
+    measure("gh_count");  // warm-up: workspace growth
+    {
+      double bt = 1e300;
+      for (long v : {128L, 256L, 512L, 1024L, 4096L}) {
+        c.tune = best;
+        c.tune.cullMin = v;
+        const double t = measure("gh_count");
+        if (t < bt) bt = t, best.cullMin = v;
+      }
+    }
+    {
+      double bt = 1e300;
+      for (int v : {64, 128, 256}) {
+        c.tune = best;
+        c.tune.countBlock = v;
+        const double t = measure("gh_count");
+        if (t < bt) bt = t, best.countBlock = v;
+      }
+    }
+    {
+      double bt = 1e300;
+      for (long long v : {2048LL, 4096LL, 8192LL, 16384LL, 65536LL}) {
+        c.tune = best;
+        c.tune.bigNodes = v;
+        const double t = measure("gh_label_trace");
+        if (t < bt) bt = t, best.bigNodes = v;
+      }
+    }
+    c.tune = best;
+    c.timers = saved;
+    c.timing = timing0;
+    c.timing_only = only0;
+    if (out) {
+      out->cull_min = best.cullMin;
+      out->big_nodes = best.bigNodes;
+      out->count_block = best.countBlock;
+    }
+  });
+}
+
 int gcmf_join_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over, int flags,
                    gcmf_join_stats *jstats, gcmf_clip_stats *cstats) {
   return guarded(ctx, [&] {

@@ -52,8 +52,18 @@ struct KernelTimer {
   long launches = 0;
 };
 
+// Launch shapes and thresholds of the batched clip that the tuner
+// (gcmf_tune_clip) chooses by measurement; the defaults suit the join
+// workload's pairs.
+struct Tuning {
+  long cullMin = 512;       // rings longer than this are walked through run boxes
+  long long bigNodes = 8192;  // pairs with this many nodes are labelled by a block
+  int countBlock = 256;     // threads per block of the count kernel (64, 128, 256)
+};
+
 struct Context {
   int device = 0;
+  Tuning tune;
   hipStream_t stream = nullptr;
   Workspace ws;
   bool timing = false;

@@ -253,15 +253,16 @@ __device__ __forceinline__ void ring_edges(const double *V, int n, F &&f) {
 // ----------------------------------------------------------- run boxes ----
 // Boxes of runs of kRun1, kRun2 = 16 kRun1 and kRun3 = 16 kRun2 consecutive
 // edges in global edge order (a run spanning two pairs covers both, which
-// only loosens it).  Rings longer than kCullMin are walked through them: a
+// only loosens it).  Rings longer than the culling threshold (RunBoxes::
+// cullMin, set from Tuning::cullMin) are walked through them: a
 // run whose box misses the edge's box holds no edge that lsmf could accept,
 // and the edges of the other runs are still visited in index order.
 constexpr int kRun1 = 16, kRun2 = 256, kRun3 = 4096;
-constexpr long kCullMin = 512;
 
 struct RunBoxes {
   const double *b1, *b2, *b3;  // [4 * runs] x0, y0, x1, y1
   const double *l1, *l2, *l3;  // [runs] shortest edge of the run
+  long cullMin;                // rings longer than this are walked through the runs
 };
 
 __global__ void k_run_boxes(long edges, const int *map, const long long *off, const double *ring,
@@ -367,7 +368,8 @@ __device__ __forceinline__ void listed_edges(const double *V, int n, const int *
 // gpuCMBRFilter (:382-415) + gpuCountIntersections (:463-548), both views in
 // one launch: threads [0, edgesP) count P edges against Q, the rest count Q
 // edges against P.  CULL selects the run-box traversal for rings longer than
-// kCullMin; the plain variant keeps the register budget of the short-ring walk.
+// the culling threshold; the plain variant keeps the register budget of the
+// short-ring walk.
 template <bool CULL>
 __global__ void __launch_bounds__(256)
     k_count(long edgesP, long edgesQ, const int *mapP, const int *mapQ, const long long *offP,
@@ -409,7 +411,7 @@ __global__ void __launch_bounds__(256)
       }
       c2 += adds_vertex(t);
     };
-    if (CULL && bN > kCullMin)
+    if (CULL && bN > boxP.cullMin)
       culled_edges(Bv, bBase, bBase + bN, sideP ? boxQ : boxP, ab, 0.0, 0.0, visit);
     else
       ring_edges(Bv, (int)bN, visit);
@@ -484,7 +486,7 @@ __global__ void __launch_bounds__(64)
         n += __popcll(m);
       }
     };
-    if (bN <= kCullMin) {
+    if (bN <= rb.cullMin) {
       scan(0, bN);
     } else {
       const long long e0 = bBase, e1 = bBase + bN;
@@ -1056,8 +1058,9 @@ __global__ void __launch_bounds__(256)
 // most mP + mQ, mP = kA L_P + c0, mQ = kA L_Q + kB / L_Q + c0, with kA = 1.5
 // (eps + eA), kB = 1.5 * 2.2 (eps + 2 dl), c0 = 16 u D, unless L_P L_Q < T,
 // in which case the pair is classified unless |A(P1,Q1,Q2)| >= eps rules the
-// collinear branch out.  Both views take the same decision for a pair (the
-// expressions are symmetric), so their partner sets agree.
+// collinear branch out.  Every reject, including the cheaper ring-level ones
+// below, is of a pair whose type is provably 0, so both views find exactly
+// the pairs with a nonzero type and their partner sets agree.
 
 // per-pair filter constants (cf[16 k + i])
 enum { CF_BOXP = 0, CF_BOXQ = 4, CF_MAXMP = 8, CF_MAXMQ = 9, CF_MINLP = 10, CF_MINLQ = 11,
@@ -1156,8 +1159,12 @@ struct CpuFilter {
   double la, ma, G, T;
   bool tiny;   // some pair of A may have L_A L_B < T
   bool sideP;  // A is a P edge
+  EdgeBox wide;  // A's box widened by G (outward), for the plain-compare reject
   const double *lenB, *mrgB;
   __device__ __forceinline__ bool skip(int q, pt A1, pt A2, pt B1, pt B2) const {
+    // both of B's endpoints beyond one side of the widened box: the gap
+    // exceeds G, and no pair of A is tiny
+    if (!tiny && box_misses(wide, B1, B2)) return true;
     const double gap = edge_gap(A1, A2, B1, B2);
     if (!(gap > 0.0)) return false;
     if (!tiny && gap > G) return true;
@@ -1180,6 +1187,7 @@ __device__ __forceinline__ CpuFilter cpu_filter(const double *cf, int k, bool si
   f.G = f.ma + (sideP ? c[CF_MAXMQ] : c[CF_MAXMP]);
   f.tiny = f.la * (sideP ? c[CF_MINLQ] : c[CF_MINLP]) < f.T;
   f.sideP = sideP;
+  f.wide = EdgeBox{0.0, 0.0, 0.0, 0.0};
   f.lenB = (sideP ? lenQ : lenP) + bBase;
   f.mrgB = (sideP ? mrgQ : mrgP) + bBase;
   return f;
@@ -1220,9 +1228,15 @@ __global__ void __launch_bounds__(256)
   const double *Bv = (sideP ? ringQ : ringP) + 2 * bBase;
   const long long i = ga - aBase;
   const pt A1 = ld(A, ga), A2 = ld(A, i == aN - 1 ? aBase : ga + 1);
-  const CpuFilter f = cpu_filter(cf, k, sideP, ga, bBase, lenP, mrgP, lenQ, mrgQ);
+  CpuFilter f = cpu_filter(cf, k, sideP, ga, bBase, lenP, mrgP, lenQ, mrgQ);
   const double *ob = cf + (long long)CF_N * k + (sideP ? CF_BOXQ : CF_BOXP);
   const EdgeBox ab = edge_box(A1, A2);
+  {
+    // G widened by a relative 1e-12 so that rounding of the box sums cannot
+    // turn a gap of exactly G into a reject
+    const double Gw = f.G * (1.0 + 1e-12) + 1e-300;
+    f.wide = EdgeBox{ab.x0 - Gw, ab.y0 - Gw, ab.x1 + Gw, ab.y1 + Gw};
+  }
   const double ogap = fmax(fmax(ob[0] - ab.x1, ab.x0 - ob[2]), fmax(ob[1] - ab.y1, ab.y0 - ob[3]));
   const bool flag = f.tiny || !(ogap > f.G);
   int c1 = 0, c2 = 0;
@@ -1240,7 +1254,7 @@ __global__ void __launch_bounds__(256)
       }
       c2 += (sideP ? ins_on_p(t) : ins_on_q(t)) ? 1 : 0;
     };
-    if (CULL && bN > kCullMin) {
+    if (CULL && bN > boxP.cullMin) {
       // runs apart from the box widened by G and without a possible tiny
       // pair hold no pair the filter keeps
       const double G = f.G * (1.0 + 1e-12) + 1e-300;
@@ -1280,7 +1294,12 @@ __global__ void __launch_bounds__(64)
     const long long bBase = offB[k], bN = offB[k + 1] - bBase;
     const long long i = g - aBase;
     const pt A1 = ld(ringA, g), A2 = ld(ringA, i == aN - 1 ? aBase : g + 1);
-    const CpuFilter f = cpu_filter(cf, k, sideP, g, bBase, lenP, mrgP, lenQ, mrgQ);
+    CpuFilter f = cpu_filter(cf, k, sideP, g, bBase, lenP, mrgP, lenQ, mrgQ);
+    {
+      const EdgeBox ab = edge_box(A1, A2);
+      const double Gw = f.G * (1.0 + 1e-12) + 1e-300;
+      f.wide = EdgeBox{ab.x0 - Gw, ab.y0 - Gw, ab.x1 + Gw, ab.y1 + Gw};
+    }
     const double *V = ringB + 2 * bBase;
     long long n = 0;
     auto scan = [&](long long a, long long z) {
@@ -1300,7 +1319,7 @@ __global__ void __launch_bounds__(64)
         n += __popcll(m);
       }
     };
-    if (rb.b1 == nullptr || bN <= kCullMin) {
+    if (rb.b1 == nullptr || bN <= rb.cullMin) {
       scan(0, bN);
     } else {
       const double G = f.G * (1.0 + 1e-12) + 1e-300;
@@ -2010,6 +2029,7 @@ struct TraceArgs {
   int *compLen;  // [comp capacity] final component sizes
   double *xyTmp; // [node capacity] compacted result vertices per pair
   int cleanup;
+  long long bigNodes;  // pairs with this many nodes go to k_label_block
   // links given on both rings (nbP and nbQ, the CPU route); otherwise only
   // the Q slots' links are read and applied with link() (the GPU route)
   int explicitLinks;
@@ -3039,12 +3059,12 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
   return path;
 }
 
-// Pairs with at least kBigNodes nodes are labelled by a block of kBigGroup
+// Pairs with at least TraceArgs::bigNodes nodes (Tuning::bigNodes) are
+// labelled by a block of kBigGroup
 // threads (k_label_block), the others by one wave each (k_label_wave).  At
 // 1024 threads the sequential fallback spills a few registers; the extra
 // parallelism of the block is worth more (the 1e6 x 1e5 pair labels in 113
 // ms at 1024 threads, 167 ms at 512).
-constexpr long long kBigNodes = 8192;
 constexpr int kBigGroup = 1024;
 
 __device__ __forceinline__ long long pair_nodes(const long long *offP, const long long *offQ,
@@ -3063,7 +3083,7 @@ __global__ void __launch_bounds__(64)
                  long long *lticks) {
   const long k = blockIdx.x;
   if (k >= M) return;
-  if (pair_nodes(a, k) >= kBigNodes) return;  // k_label_block's
+  if (pair_nodes(a, k) >= a.bigNodes) return;  // k_label_block's
   const long long t0 = wall_clock64();
   const int path = label_pair<64>(k, status, s, a, aux);
   if (threadIdx.x == 0 && lpath) {
@@ -3077,10 +3097,10 @@ __global__ void __launch_bounds__(64)
 // cnt[0] their number
 __global__ void __launch_bounds__(1024)
     k_big_pairs(long M, const long long *offP, const long long *offQ, const long long *psP2,
-                const long long *psQ2, int *list, unsigned long long *cnt) {
+                const long long *psQ2, long long bigNodes, int *list, unsigned long long *cnt) {
   __shared__ unsigned long long lds[17];
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool big = k < M && pair_nodes(offP, offQ, psP2, psQ2, k) >= kBigNodes;
+  const bool big = k < M && pair_nodes(offP, offQ, psP2, psQ2, k) >= bigNodes;
   const long i = block_append(cnt, big, lds);
   if (i >= 0) list[i] = (int)k;
 }
@@ -3227,7 +3247,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   // run boxes of a side, when some ring of it is long enough to be culled
   auto run_boxes = [&](bool qside, long edges, const int *map, const long long *off,
                        const double *ring) {
-    RunBoxes r{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    RunBoxes r{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c.tune.cullMin};
     const long r1 = (edges + kRun1 - 1) / kRun1, r2 = (r1 + 15) / 16, r3 = (r2 + 15) / 16;
     double *b1 = wsT<double>(c, qside ? "gh_rb1Q" : "gh_rb1P", 4 * r1);
     double *b2 = wsT<double>(c, qside ? "gh_rb2Q" : "gh_rb2P", 4 * r2);
@@ -3250,18 +3270,19 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     r.l3 = l3;
     return r;
   };
-  RunBoxes boxP{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  RunBoxes boxQ{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  if (maxRingP > kCullMin && edgesP > 0)
+  RunBoxes boxP{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c.tune.cullMin};
+  RunBoxes boxQ{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c.tune.cullMin};
+  if (maxRingP > c.tune.cullMin && edgesP > 0)
     boxP = run_boxes(false, edgesP, mapP, (const long long *)offP, (const double *)ringP);
-  if (maxRingQ > kCullMin && edgesQ > 0)
+  if (maxRingQ > c.tune.cullMin && edgesQ > 0)
     boxQ = run_boxes(true, edgesQ, mapQ, (const long long *)offQ, (const double *)ringQ);
   if (edgesP + edgesQ > 0) {
     LaunchTimer t(c, "gh_count");
     const bool cull = boxP.b1 != nullptr || boxQ.b1 != nullptr;
+    const int cb = c.tune.countBlock;
     if (cpuRoute)
       hipLaunchKernelGGL(cull ? k_count_cpu<true> : k_count_cpu<false>,
-                         dim3(grid_for(edgesP + edgesQ, 256)), dim3(256), 0, c.stream, edgesP,
+                         dim3(grid_for(edgesP + edgesQ, cb)), dim3(cb), 0, c.stream, edgesP,
                          edgesQ, mapP, mapQ, (const long long *)offP, (const long long *)offQ,
                          ringP, ringQ, (const double *)elenP, (const double *)emrgP,
                          (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
@@ -3269,7 +3290,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          boxQ);
     else
       hipLaunchKernelGGL(cull ? k_count<true> : k_count<false>,
-                         dim3(grid_for(edgesP + edgesQ, 256)), dim3(256), 0, c.stream, edgesP,
+                         dim3(grid_for(edgesP + edgesQ, cb)), dim3(cb), 0, c.stream, edgesP,
                          edgesQ, mapP, mapQ, (const long long *)offP, (const long long *)offQ,
                          ringP, ringQ, pm.cmbrN, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ,
                          stats, lstP, lstQ, boxP, boxQ);
@@ -3316,7 +3337,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   fill_i32(c, (int *)bigCnt, 0, 2);
   hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M,
                      (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
-                     (const long long *)psQ2, bigList, bigCnt);
+                     (const long long *)psQ2, (long long)c.tune.bigNodes, bigList, bigCnt);
   HIP_CHECK(hipGetLastError());
   {
     const long long *rc = (const long long *)runCnt;
@@ -3677,6 +3698,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   ta.xyTmp = wsT<double>(c, "gh_xyTmp", 2 * nodes);
   ta.cleanup = opt.cleanup;
   ta.explicitLinks = cpuRoute ? 1 : 0;
+  ta.bigNodes = c.tune.bigNodes;
   if (opt.label_mode == 1) {
     LaunchTimer t(c, "gh_label_trace");
     hipLaunchKernelGGL(k_label_trace, dim3(grid_for(M, 64)), dim3(64), 0, c.stream, M,

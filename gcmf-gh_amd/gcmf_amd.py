@@ -185,6 +185,13 @@ class ClipStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("cull_min", C.c_long), ("big_nodes", C.c_longlong), ("count_block", C.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class DLayer(C.Structure):
     _fields_ = [("npoly", C.c_long), ("d_coords", C.c_void_p), ("d_vnum", C.c_void_p),
                 ("d_vpsnum", C.c_void_p), ("d_mbr", C.c_void_p), ("d_mbr2", C.c_void_p)]
@@ -251,6 +258,10 @@ def hip_lib():
         L.gcmf_radix_sort_u64.argtypes = [vp, vp, vp, vp, vp, C.c_long, C.c_int]
         L.gcmf_scan_i32_i64.argtypes = [vp, vp, vp, C.c_long]
         L.gcmf_compact_flags.argtypes = [vp, vp, C.c_long, vp, C.POINTER(C.c_long)]
+        L.gcmf_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.gcmf_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.gcmf_tune_clip.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), vp, C.c_long,
+                                     C.c_int, C.POINTER(Tuning)]
         _hip = L
     return _hip
 
@@ -338,6 +349,25 @@ class Context:
         self._check(hip_lib().gcmf_timer_get(self.h, kernel.encode(), C.byref(ms), C.byref(n)),
                     "timer_get")
         return ms.value, n.value
+
+    # ------------------------------------------------------------ tuning
+    def tuning(self):
+        t = Tuning()
+        self._check(hip_lib().gcmf_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, cull_min, big_nodes, count_block):
+        t = Tuning(cull_min, big_nodes, count_block)
+        self._check(hip_lib().gcmf_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def tune_clip(self, base, over, d_pairs, M, flags=CLIP_NORMALIZE):
+        """Autotunes the batched clip's shapes on one batch (device pairs
+        buffer `d_pairs` of M pairs); returns the chosen values."""
+        t = Tuning()
+        self._check(hip_lib().gcmf_tune_clip(self.h, C.byref(base.desc), C.byref(over.desc),
+                                             d_pairs.ptr, int(M), int(flags), C.byref(t)),
+                    "tune_clip")
+        return t.as_dict()
 
     # ------------------------------------------------------- primitives
     def radix_sort(self, d_keys_in, d_vals_in, d_keys_out, d_vals_out, n, bits):

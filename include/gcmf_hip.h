@@ -150,6 +150,23 @@ typedef struct {
 int gcmf_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
               const int *d_pairs, long M, int flags, gcmf_clip_stats *stats);
 
+/* Launch shapes and thresholds of the batched clip, per context. */
+typedef struct {
+  long cull_min;       /* rings longer than this are walked through run boxes (512) */
+  long long big_nodes; /* pairs with this many nodes are labelled by a
+                          1024-thread block instead of one wave (8192) */
+  int count_block;     /* threads per block of the count kernel: 64, 128, 256 (256) */
+} gcmf_tuning;
+int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out);
+int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t);
+/* Chooses the tuning by measurement on one batch (the autotuner): clips the
+ * pairs once per candidate value of each parameter in turn, times the
+ * launches the parameter affects with HIP events, and keeps the fastest in
+ * the context.  The chosen values are returned in *out.  No reference
+ * counterpart. */
+int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+                   const int *d_pairs, long M, int flags, gcmf_tuning *out);
+
 /* Join then clip the joined pairs without leaving the device. */
 int gcmf_join_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
                    int flags, gcmf_join_stats *jstats, gcmf_clip_stats *cstats);

@@ -397,3 +397,34 @@ def test_clip_batched_rare_pairs(gctx, gpu_route):
     assert status[0] == -1 and path[0] == G.LP_FAILED
     for k in (1, 2, 3):  # split and unsplit BOUNCING pairs
         assert path[k] in (G.LP_SEGMENTS, G.LP_SEG_CLEANUP), (names[k], path[k])
+
+
+def test_clip_autotuner_keeps_results(gctx):
+    """gcmf_tune_clip picks launch shapes by measurement; results are the
+    same under the tuned and the default shapes, and set_tuning rejects
+    invalid shapes."""
+    P, Q = G.gen_pair_layers(300, seed=9, nmax=6000)
+    dP, dQ = G.DeviceLayer(gctx, P), G.DeviceLayer(gctx, Q)
+    pairs = np.stack([np.arange(300), np.arange(300)], axis=1).astype(np.int32)
+    default = gctx.tuning()
+    st0 = gctx.clip(dP, dQ, pairs)
+    _, r0 = gctx.clip_results(st0)
+    d_pairs = gctx.upload(pairs)
+    try:
+        tuned = gctx.tune_clip(dP, dQ, d_pairs, 300)
+        assert tuned["count_block"] in (64, 128, 256)
+        assert tuned == gctx.tuning()
+        st1 = gctx.clip(dP, dQ, pairs)
+        _, r1 = gctx.clip_results(st1)
+        assert len(r0) == len(r1)
+        for a, b in zip(r0, r1):
+            assert [len(c) for c in a] == [len(c) for c in b]
+            for x, y in zip(a, b):
+                np.testing.assert_array_equal(x, y)
+        with pytest.raises(G.GcmfError):
+            gctx.set_tuning(512, 8192, 100)
+    finally:
+        gctx.set_tuning(default["cull_min"], default["big_nodes"], default["count_block"])
+        d_pairs.free()
+        dP.free()
+        dQ.free()
