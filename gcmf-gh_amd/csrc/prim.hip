@@ -606,7 +606,9 @@ __global__ void __launch_bounds__(kSortThreads)
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
-  // exclusive over waves, per digit
+  // exclusive over waves, per digit, and the tile's own digit starts
+  __shared__ int dstart[256];
+  __shared__ long long lds_scan[kSortThreads / 64];
   {
     int d = threadIdx.x;  // 256 threads == 256 digits
     int s = 0;
@@ -615,15 +617,32 @@ __global__ void __launch_bounds__(kSortThreads)
       wcnt[w][d] = s;
       s += v;
     }
+    long long tot;
+    dstart[d] = (int)block_excl_scan<long long>(s, lds_scan, tot);
   }
   __syncthreads();
+  // the tile sorted by digit in LDS, stably ...
+  __shared__ uint64_t sk[kSortTile];
+  __shared__ int sv[kSortTile];
 #pragma unroll
   for (int r = 0; r < kSortItems; r++) {
     if (pos[r] < 0) continue;
-    int d = (int)((kk[r] >> shift) & 255);
-    long long dst = off[(long)d * tiles + blockIdx.x] + wcnt[wave][d] + pos[r];
-    keys_out[dst] = kk[r];
-    vals_out[dst] = vv[r];
+    const int d = (int)((kk[r] >> shift) & 255);
+    const int lp = dstart[d] + wcnt[wave][d] + pos[r];
+    sk[lp] = kk[r];
+    sv[lp] = vv[r];
+  }
+  __syncthreads();
+  // ... then written out in that order: consecutive threads store
+  // consecutive addresses inside each digit's run
+  const long tbase = (long)blockIdx.x * kSortTile;
+  const int tn = (int)min((long)kSortTile, n - tbase);
+  for (int i = threadIdx.x; i < tn; i += kSortThreads) {
+    const uint64_t key = sk[i];
+    const int d = (int)((key >> shift) & 255);
+    const long long dst = off[(long)d * tiles + blockIdx.x] + (i - dstart[d]);
+    keys_out[dst] = key;
+    vals_out[dst] = sv[i];
   }
 }
 
