@@ -82,37 +82,82 @@ struct PairMeta {
   double *cmbrN;  // [4M] normalised CMBR
 };
 
-// one wave per pair: sizes, swap (|P| >= |Q|, ghcuda.cpp:64-91), ring MBRs
+// The per-pair setup kernels run one group of W threads per pair: four waves
+// of a 256-thread block (W = 64) for batches, one 1024-thread block (W =
+// 1024) when a few pairs are large.  Only exact min / max reductions cross
+// lanes, so both widths give identical results.
+template <int W>
+__device__ __forceinline__ long pgrp_pair() {
+  return W == 64 ? (long)blockIdx.x * 4 + (threadIdx.x >> 6) : (long)blockIdx.x;
+}
+template <int W>
+__device__ __forceinline__ int pgrp_lane() {
+  return W == 64 ? (int)__lane_id() : (int)threadIdx.x;
+}
+// (min of a, max of b) over the group
+template <int W>
+__device__ __forceinline__ void pgrp_minmax(double &a, double &b) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a = fmin(a, __shfl_xor(a, off, 64));
+    b = fmax(b, __shfl_xor(b, off, 64));
+  }
+  if (W == 64) return;
+  __shared__ double sa[W / 64], sb[W / 64];
+  __syncthreads();
+  if (__lane_id() == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  a = sa[0];
+  b = sb[0];
+  for (int i = 1; i < W / 64; i++) {
+    a = fmin(a, sa[i]);
+    b = fmax(b, sb[i]);
+  }
+}
+// threads per block and blocks for M pairs at group width W
+inline unsigned pgrp_grid(long M, int W) { return W == 64 ? grid_for(M, 4) : (unsigned)M; }
+inline int pgrp_block(int W) { return W == 64 ? 256 : W; }
+
+// one group per pair: sizes, swap (|P| >= |Q|, ghcuda.cpp:64-91), ring MBRs
 // (readShapefile.h:304-334), CMBR and disjointness (ghcuda.cpp:94-125)
-__global__ void __launch_bounds__(256)
+template <int W>
+__global__ void __launch_bounds__(W == 64 ? 256 : W)
     k_pair_setup(long M, const int *pairs, const double *bCoords, const int *bVNum,
                  const long *bVPSNum, const double *oCoords, const int *oVNum, const long *oVPSNum,
                  int normalize, int noswap, PairMeta pm, int *eP, int *eQ,
                  unsigned long long *maxRing) {
-  __shared__ int lmax[2][4];
-  const long k = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (threadIdx.x < 8) lmax[threadIdx.x >> 2][threadIdx.x & 3] = 0;
-  __syncthreads();
-  if (k < M) {
-    const int b = pairs[2 * k], o = pairs[2 * k + 1];
-    const int nb = bVNum[b] - 1, no = oVNum[o] - 1;
-    if (__lane_id() == 0) {
-      lmax[0][threadIdx.x >> 6] = nb > no ? nb : no;
-      lmax[1][threadIdx.x >> 6] = nb > no ? no : nb;
+  const long k = pgrp_pair<W>();
+  if (W == 64) {
+    __shared__ int lmax[2][4];
+    if (threadIdx.x < 8) lmax[threadIdx.x >> 2][threadIdx.x & 3] = 0;
+    __syncthreads();
+    if (k < M) {
+      const int b = pairs[2 * k], o = pairs[2 * k + 1];
+      const int nb = bVNum[b] - 1, no = oVNum[o] - 1;
+      if (__lane_id() == 0) {
+        lmax[0][threadIdx.x >> 6] = nb > no ? nb : no;
+        lmax[1][threadIdx.x >> 6] = nb > no ? no : nb;
+      }
     }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int a = 0, q = 0;
-    for (int w = 0; w < 4; w++) {
-      a = max(a, lmax[0][w]);
-      q = max(q, lmax[1][w]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int a = 0, q = 0;
+      for (int w = 0; w < 4; w++) {
+        a = max(a, lmax[0][w]);
+        q = max(q, lmax[1][w]);
+      }
+      if (a) atomicMax(maxRing, (unsigned long long)a);
+      if (q) atomicMax(maxRing + 1, (unsigned long long)q);
     }
-    if (a) atomicMax(maxRing, (unsigned long long)a);
-    if (q) atomicMax(maxRing + 1, (unsigned long long)q);
+  } else if (threadIdx.x == 0 && k < M) {
+    const int nb = bVNum[pairs[2 * k]] - 1, no = oVNum[pairs[2 * k + 1]] - 1;
+    if (max(nb, no) > 0) atomicMax(maxRing, (unsigned long long)max(nb, no));
+    if (min(nb, no) > 0) atomicMax(maxRing + 1, (unsigned long long)min(nb, no));
   }
   if (k >= M) return;
-  const int lane = __lane_id();
+  const int lane = pgrp_lane<W>();
   const int b = pairs[2 * k], o = pairs[2 * k + 1];
   const int nb = bVNum[b] - 1, no = oVNum[o] - 1;
   const long sb = bVPSNum[b] - bVNum[b], so = oVPSNum[o] - oVNum[o];
@@ -125,19 +170,15 @@ __global__ void __launch_bounds__(256)
     const double *c = side == 0 ? cP + 2 * sP : cQ + 2 * sQ;
     const int n = side == 0 ? nP : nQ;
     double mnx = c[0], mny = c[1], mxx = c[0], mxy = c[1];
-    for (int i = lane; i < n; i += 64) {
+    for (int i = lane; i < n; i += W) {
       double x = c[2 * i], y = c[2 * i + 1];
       mnx = fmin(mnx, x);
       mny = fmin(mny, y);
       mxx = fmax(mxx, x);
       mxy = fmax(mxy, y);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      mnx = fmin(mnx, __shfl_xor(mnx, off, 64));
-      mny = fmin(mny, __shfl_xor(mny, off, 64));
-      mxx = fmax(mxx, __shfl_xor(mxx, off, 64));
-      mxy = fmax(mxy, __shfl_xor(mxy, off, 64));
-    }
+    pgrp_minmax<W>(mnx, mxx);
+    pgrp_minmax<W>(mny, mxy);
     m[side][0] = mnx;
     m[side][1] = mny;
     m[side][2] = mxx;
@@ -186,14 +227,15 @@ __global__ void __launch_bounds__(256)
   eQ[k] = status == GH_OK ? nQ : 0;
 }
 
-// one wave per pair: edge -> pair maps and normalised rings
-__global__ void __launch_bounds__(256)
+// one group per pair: edge -> pair maps and normalised rings
+template <int W>
+__global__ void __launch_bounds__(W == 64 ? 256 : W)
     k_pair_fill(long M, const int *pairs, const double *bCoords, const double *oCoords,
                 PairMeta pm, const long long *offP, const long long *offQ, int *mapP, int *mapQ,
                 double *nP_, double *nQ_) {
-  const long k = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long k = pgrp_pair<W>();
   if (k >= M) return;
-  const int lane = __lane_id();
+  const int lane = pgrp_lane<W>();
   const int sw = pm.swapped[k];
   const double cx = pm.norm[3 * k], cy = pm.norm[3 * k + 1], s = pm.norm[3 * k + 2];
   for (int side = 0; side < 2; side++) {
@@ -203,7 +245,7 @@ __global__ void __launch_bounds__(256)
     const long start = side == 0 ? pm.startP[k] : pm.startQ[k];
     int *map = side == 0 ? mapP : mapQ;
     double *dst = side == 0 ? nP_ : nQ_;
-    for (long long i = lane; i < n; i += 64) {
+    for (long long i = lane; i < n; i += W) {
       double x = c[2 * (start + i)], y = c[2 * (start + i) + 1];
       dst[2 * (base + i)] = (x - cx) * s;
       dst[2 * (base + i) + 1] = (y - cy) * s;
@@ -1066,32 +1108,29 @@ __global__ void __launch_bounds__(256)
 enum { CF_BOXP = 0, CF_BOXQ = 4, CF_MAXMP = 8, CF_MAXMQ = 9, CF_MINLP = 10, CF_MINLQ = 11,
        CF_TINY = 12, CF_N = 16 };
 
-// one wave per pair: ring boxes, edge lengths and margins
-__global__ void __launch_bounds__(256)
+// one group per pair: ring boxes, edge lengths and margins
+template <int W>
+__global__ void __launch_bounds__(W == 64 ? 256 : W)
     k_cpu_filter_setup(long M, const int *status, const long long *offP, const long long *offQ,
                        const double *ringP, const double *ringQ, double *lenP, double *mrgP,
                        double *lenQ, double *mrgQ, double *cf) {
-  const long k = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long k = pgrp_pair<W>();
   if (k >= M || status[k] != GH_OK) return;
-  const int lane = __lane_id();
+  const int lane = pgrp_lane<W>();
   double bx[2][4];
   for (int side = 0; side < 2; side++) {
     const long long b = side ? offQ[k] : offP[k], n = (side ? offQ[k + 1] : offP[k + 1]) - b;
     const double *V = side ? ringQ : ringP;
     double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
-    for (long long i = lane; i < n; i += 64) {
+    for (long long i = lane; i < n; i += W) {
       const pt p = ld(V, b + i);
       x0 = fmin(x0, p.x);
       y0 = fmin(y0, p.y);
       x1 = fmax(x1, p.x);
       y1 = fmax(y1, p.y);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      x0 = fmin(x0, __shfl_xor(x0, off, 64));
-      y0 = fmin(y0, __shfl_xor(y0, off, 64));
-      x1 = fmax(x1, __shfl_xor(x1, off, 64));
-      y1 = fmax(y1, __shfl_xor(y1, off, 64));
-    }
+    pgrp_minmax<W>(x0, x1);
+    pgrp_minmax<W>(y0, y1);
     bx[side][0] = x0;
     bx[side][1] = y0;
     bx[side][2] = x1;
@@ -1109,7 +1148,7 @@ __global__ void __launch_bounds__(256)
     const double *V = side ? ringQ : ringP;
     double *len = side ? lenQ : lenP, *mrg = side ? mrgQ : mrgP;
     double mn = INFINITY, mx = 0.0;
-    for (long long i = lane; i < n; i += 64) {
+    for (long long i = lane; i < n; i += W) {
       const pt p = ld(V, b + i), q = ld(V, b + (i + 1 == n ? 0 : i + 1));
       const double dx = q.x - p.x, dy = q.y - p.y;
       const double L = sqrt(dx * dx + dy * dy);
@@ -1119,10 +1158,7 @@ __global__ void __launch_bounds__(256)
       mn = fmin(mn, L);
       mx = fmax(mx, m);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      mn = fmin(mn, __shfl_xor(mn, off, 64));
-      mx = fmax(mx, __shfl_xor(mx, off, 64));
-    }
+    pgrp_minmax<W>(mn, mx);
     minL[side] = mn;
     maxM[side] = mx;
   }
@@ -3156,19 +3192,20 @@ __global__ void k_capacity(long M, const long long *offP, const long long *offQ,
 }
 
 // final dense output: result vertices and component sizes in pair order
-__global__ void __launch_bounds__(256)
+template <int W>
+__global__ void __launch_bounds__(W == 64 ? 256 : W)
     k_gather_out(long M, const long long *nodeOff, const long long *compCapOff,
                  const long long *vOff, const long long *cOff, const double *xyTmp,
                  const int *compLen, double *xy, int *compSizes) {
-  const long k = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long k = pgrp_pair<W>();
   if (k >= M) return;
-  const int lane = __lane_id();
+  const int lane = pgrp_lane<W>();
   const long long nv = vOff[k + 1] - vOff[k], nc = cOff[k + 1] - cOff[k];
-  for (long long i = lane; i < nv; i += 64) {
+  for (long long i = lane; i < nv; i += W) {
     xy[2 * (vOff[k] + i)] = xyTmp[2 * (nodeOff[k] + i)];
     xy[2 * (vOff[k] + i) + 1] = xyTmp[2 * (nodeOff[k] + i) + 1];
   }
-  for (long long i = lane; i < nc; i += 64) compSizes[cOff[k] + i] = compLen[compCapOff[k] + i];
+  for (long long i = lane; i < nc; i += W) compSizes[cOff[k] + i] = compLen[compCapOff[k] + i];
 }
 
 template <typename T>
@@ -3181,6 +3218,9 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   res = ClipResult();
   res.M = M;
   if (M <= 0) return;
+  // group width of the per-pair setup kernels: a 1024-thread block per pair
+  // for a few pairs, four waves per block otherwise
+  const int pw = M <= 32 ? 1024 : 64;
   PairMeta pm;
   pm.startP = wsT<long>(c, "gh_startP", M);
   pm.startQ = wsT<long>(c, "gh_startQ", M);
@@ -3197,7 +3237,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   fill_i32(c, (int *)maxRing, 0, 4);
   {
     LaunchTimer t(c, "gh_pair_setup");
-    hipLaunchKernelGGL(k_pair_setup, dim3(grid_for(M, 4)), dim3(256), 0, c.stream, M, pairs,
+    hipLaunchKernelGGL(pw == 64 ? k_pair_setup<64> : k_pair_setup<1024>, dim3(pgrp_grid(M, pw)),
+                       dim3(pgrp_block(pw)), 0, c.stream, M, pairs,
                        in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum, in.oVPSNum,
                        opt.normalize, opt.noswap, pm, eP, eQ, maxRing);
     HIP_CHECK(hipGetLastError());
@@ -3214,7 +3255,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   double *ringQ = wsT<double>(c, "gh_ringQ", 2 * edgesQ);
   {
     LaunchTimer t(c, "gh_pair_fill");
-    hipLaunchKernelGGL(k_pair_fill, dim3(grid_for(M, 4)), dim3(256), 0, c.stream, M, pairs,
+    hipLaunchKernelGGL(pw == 64 ? k_pair_fill<64> : k_pair_fill<1024>, dim3(pgrp_grid(M, pw)),
+                       dim3(pgrp_block(pw)), 0, c.stream, M, pairs,
                        in.bCoords, in.oCoords, pm, (const long long *)offP,
                        (const long long *)offQ, mapP, mapQ, ringP, ringQ);
     HIP_CHECK(hipGetLastError());
@@ -3231,7 +3273,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     elenQ = wsT<double>(c, "gh_elenQ", edgesQ);
     emrgQ = wsT<double>(c, "gh_emrgQ", edgesQ);
     cf = wsT<double>(c, "gh_cf", (long long)CF_N * M);
-    hipLaunchKernelGGL(k_cpu_filter_setup, dim3(grid_for(M, 4)), dim3(256), 0, c.stream, M,
+    hipLaunchKernelGGL(pw == 64 ? k_cpu_filter_setup<64> : k_cpu_filter_setup<1024>, dim3(pgrp_grid(M, pw)),
+                       dim3(pgrp_block(pw)), 0, c.stream, M,
                        (const int *)pm.status, (const long long *)offP, (const long long *)offQ,
                        (const double *)ringP, (const double *)ringQ, elenP, emrgP, elenQ, emrgQ,
                        cf);
@@ -3743,7 +3786,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   res.compSizes = wsT<int>(c, "gh_out_comp", comps);
   {
     LaunchTimer t(c, "gh_gather_out");
-    hipLaunchKernelGGL(k_gather_out, dim3(grid_for(M, 4)), dim3(256), 0, c.stream, M,
+    hipLaunchKernelGGL(pw == 64 ? k_gather_out<64> : k_gather_out<1024>, dim3(pgrp_grid(M, pw)),
+                       dim3(pgrp_block(pw)), 0, c.stream, M,
                        (const long long *)nodeOff, (const long long *)compCapOff,
                        (const long long *)res.vertOff, (const long long *)res.compOff, ta.xyTmp,
                        ta.compLen, res.xy, res.compSizes);
