@@ -472,21 +472,16 @@ __global__ void __launch_bounds__(256)
   if (__lane_id() == 0 && ev) atomicAdd(nEval, ev);
 }
 
-// Complete partner lists at the count1 offsets: the count pass's first
-// kPartners entries, or, for an edge with more, a rescan of the other ring
-// with the count pass's tests (same order, same set).
-__global__ void k_partner_lists(long nAct, const int *act, bool sideP, const int *mapA,
-                                const long long *offA, const long long *offB, const double *ringA,
-                                const double *ringB, const long long *psA1, const int *lstFixed,
-                                int *lst) {
+// Partner lists at the count1 offsets for the active edges with at most
+// kPartners partners: the entries the count pass recorded (the others are
+// rescanned by k_partner_rescan*, in the same order).
+__global__ void k_partner_lists(long nAct, const int *act, const long long *psA1,
+                                const int *lstFixed, int *lst) {
   const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (ia >= nAct) return;
   const long g = act[ia];
-  const int k = mapA[g];
   const long long c1 = psA1[g + 1] - psA1[g];
   int *dst = lst + psA1[g];
-  (void)k;
-  (void)sideP;
   if (c1 <= kPartners)
     for (int i = 0; i < (int)c1; i++) dst[i] = lstFixed[(long long)g * kPartners + i];
 }
@@ -2457,7 +2452,10 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   }
   if (grp_any<W>(bad)) return SEG_SLOW;
   // pointer jumping: smallest P key (.x) and smallest departure (.y) of
-  // every cycle; four independent chains per lane in flight
+  // every cycle; four independent chains per lane in flight.  After round t
+  // every value is the minimum over the next 2^t departures of its cycle; a
+  // round that changes no value means every window already spans its cycle's
+  // minimum, so the loop ends there instead of after ceil(log2 D) rounds.
   int rounds = 0;
   while ((1 << rounds) < D) rounds++;
   int cur = 0;
@@ -2467,6 +2465,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     const int *__restrict__ ns = nx[cur];
     int2 *__restrict__ vd = vc[cur ^ 1];
     int *__restrict__ nd = nx[cur ^ 1];
+    bool changed = false;
     for (int d0 = lane; d0 < D; d0 += 4 * W) {
       int x[4];
       int2 a[4];
@@ -2487,12 +2486,15 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       for (int k = 0; k < 4; k++) {
         const int d = d0 + W * k;
         if (d < D) {
-          vd[d] = make_int2(min(a[k].x, b[k].x), min(a[k].y, b[k].y));
+          const int2 m = make_int2(min(a[k].x, b[k].x), min(a[k].y, b[k].y));
+          changed |= m.x != a[k].x || m.y != a[k].y;
+          vd[d] = m;
           nd[d] = y[k];
         }
       }
     }
     cur ^= 1;
+    if (!grp_any<W>(changed)) break;
   }
   __syncthreads();
   const int2 *mc = vc[cur];
@@ -2518,6 +2520,8 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   for (int d = lane; d < XP; d += W)
     if (S[1][d] == S[1][dep_partner(N, rk, dn[d], nP, nQ)]) bad = true;
   if (grp_any<W>(bad)) return SEG_SLOW;
+  // list ranking of the traced lists (nxt = -1 ends a list): the loop ends
+  // once no element still points anywhere
   cur = 0;
   for (int t = 0; t < rounds; t++) {
     __syncthreads();
@@ -2525,6 +2529,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     const int *__restrict__ ns = nx[cur];
     int *__restrict__ sd = S[cur ^ 1];
     int *__restrict__ nd = nx[cur ^ 1];
+    bool live = false;
     for (int d0 = lane; d0 < D; d0 += 4 * W) {
       int x[4], a[4];
 #pragma unroll
@@ -2545,10 +2550,12 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
         if (d < D) {
           sd[d] = a[k] + b[k];
           nd[d] = y[k];
+          live |= y[k] >= 0;
         }
       }
     }
     cur ^= 1;
+    if (!grp_any<W>(live)) break;
   }
   __syncthreads();
   const int *Sf = S[cur];
@@ -3397,14 +3404,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   int *lstAllP = wsT<int>(c, "gh_lstAllP", rb[10]), *lstAllQ = wsT<int>(c, "gh_lstAllQ", rb[11]);
   if (nActP > 0)
     hipLaunchKernelGGL(k_partner_lists, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream, nActP,
-                       (const int *)actP, true, mapP, (const long long *)offP,
-                       (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
-                       (const long long *)psP1, (const int *)lstP, lstAllP);
+                       (const int *)actP, (const long long *)psP1, (const int *)lstP, lstAllP);
   if (nActQ > 0)
     hipLaunchKernelGGL(k_partner_lists, dim3(grid_for(nActQ, 256)), dim3(256), 0, c.stream, nActQ,
-                       (const int *)actQ, false, mapQ, (const long long *)offQ,
-                       (const long long *)offP, (const double *)ringQ, (const double *)ringP,
-                       (const long long *)psQ1, (const int *)lstQ, lstAllQ);
+                       (const int *)actQ, (const long long *)psQ1, (const int *)lstQ, lstAllQ);
   if (nOverP > 0 && cpuRoute)
     hipLaunchKernelGGL(k_partner_rescan_cpu, dim3(nOverP < 65536 ? nOverP : 65536), dim3(64), 0,
                        c.stream, nOverP, (const int *)listOP, true, mapP,
