@@ -1370,6 +1370,58 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// The Q side of the CPU route from the P side's complete partner lists.
+// Every pair (P edge, Q edge) with a nonzero type, taken in P-edge order,
+// becomes a record keyed by its global Q edge, and the Q edge's counts are
+// added from the same pass (the type recomputed as intersect(P edge, Q
+// edge), as every pass computes it).  A stable sort of the records by Q edge
+// then lists each Q edge's partners in increasing P order: exactly the list
+// a walk of that Q edge over the P ring would build.
+__global__ void __launch_bounds__(256)
+    k_cpu_transpose(long nActP, const int *actP, const int *mapP, const long long *offP,
+                    const long long *offQ, const double *ringP, const double *ringQ,
+                    const long long *psP1, const int *lstAllP, int *cnt1Q, int *cnt2Q,
+                    uint64_t *keys, int *vals) {
+  const long ia = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= nActP) return;
+  const long g = actP[ia];
+  const int k = mapP[g];
+  const long long pBase = offP[k], pN = offP[k + 1] - pBase;
+  const long long qBase = offQ[k], qN = offQ[k + 1] - qBase;
+  const int pid = (int)(g - pBase);
+  const pt A1 = ld(ringP, g), A2 = ld(ringP, pid == pN - 1 ? pBase : g + 1);
+  for (long long j = psP1[g]; j < psP1[g + 1]; j++) {
+    const int q = lstAllP[j];
+    const long long qg = qBase + q;
+    const pt B1 = ld(ringQ, qg), B2 = ld(ringQ, q + 1 == qN ? qBase : qg + 1);
+    double al, be;
+    const int t = itype(A1, A2, B1, B2, al, be);
+    atomicAdd(&cnt1Q[qg], 1);
+    if (ins_on_q(t)) atomicAdd(&cnt2Q[qg], 1);
+    keys[j] = (uint64_t)qg;
+    vals[j] = pid;
+  }
+}
+
+// The Q edges' prefilter flags of the CPU route (the flag the count kernel
+// records for a P edge: a possible tiny pair, or the edge's box within G of
+// the P ring's box), for a batch whose Q side is built by transposition.
+__global__ void __launch_bounds__(256)
+    k_cpu_flags_q(long edgesQ, const int *mapQ, const long long *offP, const long long *offQ,
+                  const double *ringQ, const double *lenP, const double *mrgP,
+                  const double *lenQ, const double *mrgQ, const double *cf, int *boolQ) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= edgesQ) return;
+  const int k = mapQ[g];
+  const long long aBase = offQ[k], aN = offQ[k + 1] - aBase;
+  const pt A1 = ld(ringQ, g), A2 = ld(ringQ, g - aBase == aN - 1 ? aBase : g + 1);
+  const CpuFilter f = cpu_filter(cf, k, false, g, offP[k], lenP, mrgP, lenQ, mrgQ);
+  const double *ob = cf + (long long)CF_N * k + CF_BOXP;
+  const EdgeBox ab = edge_box(A1, A2);
+  const double ogap = fmax(fmax(ob[0] - ab.x1, ab.x0 - ob[2]), fmax(ob[1] - ab.y1, ab.y0 - ob[3]));
+  boolQ[g] = (f.tiny || !(ogap > f.G)) ? 1 : 0;
+}
+
 // CPU-route slot records
 struct CpuSlots {
   uint64_t *kP2, *kQ2;       // exact alpha / beta (IEEE bits) of the unsorted slots
@@ -3326,113 +3378,228 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     boxP = run_boxes(false, edgesP, mapP, (const long long *)offP, (const double *)ringP);
   if (maxRingQ > c.tune.cullMin && edgesQ > 0)
     boxQ = run_boxes(true, edgesQ, mapQ, (const long long *)offQ, (const double *)ringQ);
-  if (edgesP + edgesQ > 0) {
-    LaunchTimer t(c, "gh_count");
-    const bool cull = boxP.b1 != nullptr || boxQ.b1 != nullptr;
-    const int cb = c.tune.countBlock;
-    if (cpuRoute)
-      hipLaunchKernelGGL(cull ? k_count_cpu<true> : k_count_cpu<false>,
-                         dim3(grid_for(edgesP + edgesQ, cb)), dim3(cb), 0, c.stream, edgesP,
-                         edgesQ, mapP, mapQ, (const long long *)offP, (const long long *)offQ,
-                         ringP, ringQ, (const double *)elenP, (const double *)emrgP,
-                         (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
-                         cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, stats, lstP, lstQ, boxP,
-                         boxQ);
-    else
-      hipLaunchKernelGGL(cull ? k_count<true> : k_count<false>,
-                         dim3(grid_for(edgesP + edgesQ, cb)), dim3(cb), 0, c.stream, edgesP,
-                         edgesQ, mapP, mapQ, (const long long *)offP, (const long long *)offQ,
-                         ringP, ringQ, pm.cmbrN, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ,
-                         stats, lstP, lstQ, boxP, boxQ);
-    HIP_CHECK(hipGetLastError());
-  }
   long long *psP1 = wsT<long long>(c, "gh_psP1", edgesP + 1);
   long long *psP2 = wsT<long long>(c, "gh_psP2", edgesP + 1);
   long long *psQ1 = wsT<long long>(c, "gh_psQ1", edgesQ + 1);
   long long *psQ2 = wsT<long long>(c, "gh_psQ2", edgesQ + 1);
-  scan_i32_to_i64(c, cnt1P, psP1, edgesP);
-  scan_i32_to_i64(c, cnt2P, psP2, edgesP);
-  scan_i32_to_i64(c, cnt1Q, psQ1, edgesQ);
-  scan_i32_to_i64(c, cnt2Q, psQ2, edgesQ);
-  hipLaunchKernelGGL(k_pair_check, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
-                     (const long long *)offP, (const long long *)offQ, (const long long *)psP1,
-                     (const long long *)psQ1, (const long long *)psP2, (const long long *)psQ2,
-                     pm.status, stats + 1);
-  HIP_CHECK(hipGetLastError());
-  // edges whose alpha runs go through the segmented sort (statuses are final
-  // here), read back with the slot totals
   unsigned long long *runCnt = wsT<unsigned long long>(c, "gh_runCnt", 16);
   int *listMP = wsT<int>(c, "gh_runListMP", edgesP), *listMQ = wsT<int>(c, "gh_runListMQ", edgesQ);
   int *listSP = wsT<int>(c, "gh_runListSP", edgesP), *listSQ = wsT<int>(c, "gh_runListSQ", edgesQ);
   int *listOP = wsT<int>(c, "gh_runListOP", edgesP), *listOQ = wsT<int>(c, "gh_runListOQ", edgesQ);
   int *listP = wsT<int>(c, "gh_runListP", edgesP), *lenP = wsT<int>(c, "gh_runLenP", edgesP);
   int *listQ = wsT<int>(c, "gh_runListQ", edgesQ), *lenQ = wsT<int>(c, "gh_runLenQ", edgesQ);
-  fill_i32(c, (int *)runCnt, 0, 32);  // 8 counters per side
-  if (edgesP > 0)
-    hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesP, 1024)), dim3(1024), 0, c.stream,
-                       edgesP, mapP, (const long long *)psP2, (const long long *)psP1, pm.status,
-                       listSP, listMP, listP, lenP, listOP, runCnt);
-  if (edgesQ > 0)
-    hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesQ, 1024)), dim3(1024), 0, c.stream,
-                       edgesQ, mapQ, (const long long *)psQ2, (const long long *)psQ1, pm.status,
-                       listSQ, listMQ, listQ, lenQ, listOQ, runCnt + 8);
-  HIP_CHECK(hipGetLastError());
-  // edges with intersections (count1 > 0): the later passes run over these
   int *actP = wsT<int>(c, "gh_actP", edgesP), *actQ = wsT<int>(c, "gh_actQ", edgesQ);
-  const long long *nActPd = compact_flags_dev(c, cnt1P, edgesP, actP, "actP");
-  const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
-  // the pairs labelled by a block (their number comes back with the totals)
   int *bigList = wsT<int>(c, "gh_bigList", M);
   unsigned long long *bigCnt = wsT<unsigned long long>(c, "gh_bigCnt", 1);
-  fill_i32(c, (int *)bigCnt, 0, 2);
-  hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M,
-                     (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
-                     (const long long *)psQ2, (long long)c.tune.bigNodes, bigList, bigCnt);
-  HIP_CHECK(hipGetLastError());
-  {
+  int *lstAllP = nullptr, *lstAllQ = nullptr;
+  if (cpuRoute) {
+    // -- the P view counts, lists and partner lists
+    if (edgesP > 0) {
+      LaunchTimer t(c, "gh_count");
+      const bool cull = boxQ.b1 != nullptr;
+      const int cb = c.tune.countBlock;
+      hipLaunchKernelGGL(cull ? k_count_cpu<true> : k_count_cpu<false>,
+                         dim3(grid_for(edgesP, cb)), dim3(cb), 0, c.stream, edgesP, 0L, mapP,
+                         mapQ, (const long long *)offP, (const long long *)offQ, ringP, ringQ,
+                         (const double *)elenP, (const double *)emrgP, (const double *)elenQ,
+                         (const double *)emrgQ, (const double *)cf, cnt1P, cnt2P, cnt1Q, cnt2Q,
+                         boolP, boolQ, stats, lstP, lstQ, boxP, boxQ);
+      HIP_CHECK(hipGetLastError());
+    }
+    scan_i32_to_i64(c, cnt1P, psP1, edgesP);
+    scan_i32_to_i64(c, cnt2P, psP2, edgesP);
+    fill_i32(c, (int *)runCnt, 0, 32);  // 8 counters per side
+    if (edgesP > 0)
+      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesP, 1024)), dim3(1024), 0,
+                         c.stream, edgesP, mapP, (const long long *)psP2,
+                         (const long long *)psP1, pm.status, listSP, listMP, listP, lenP, listOP,
+                         runCnt);
+    HIP_CHECK(hipGetLastError());
+    const long long *nActPd = compact_flags_dev(c, cnt1P, edgesP, actP, "actP");
     const long long *rc = (const long long *)runCnt;
-    read_scalars(c, {psP2 + edgesP, psQ2 + edgesQ, rc, rc + 2, rc + 3, rc + 8, rc + 10, rc + 11,
-                     nActPd, nActQd, psP1 + edgesP, psQ1 + edgesQ, rc + 1, rc + 9, rc + 4,
-                     rc + 12, (const long long *)bigCnt},
-                 rb);
+    long long ra[8];
+    read_scalars(c, {psP2 + edgesP, psP1 + edgesP, rc, rc + 2, rc + 3, nActPd, rc + 1, rc + 4},
+                 ra);
+    const long H = (long)ra[1];  // pairs with a nonzero type
+    if (H >= 0x7fffffffL)
+      throw Error{GCMF_ERR_OVERFLOW, "clip batch exceeds 2^31 intersecting edge pairs"};
+    rb[0] = ra[0];
+    rb[2] = ra[2];
+    rb[3] = ra[3];
+    rb[4] = ra[4];
+    rb[8] = ra[5];
+    rb[10] = H;
+    rb[11] = H;
+    rb[12] = ra[6];
+    rb[14] = ra[7];
+    rb[15] = 0;  // the Q lists come complete from the transposition
+    const long nActP0 = (long)ra[5], nOverP0 = (long)ra[7];
+    lstAllP = wsT<int>(c, "gh_lstAllP", H);
+    if (nActP0 > 0)
+      hipLaunchKernelGGL(k_partner_lists, dim3(grid_for(nActP0, 256)), dim3(256), 0, c.stream,
+                         nActP0, (const int *)actP, (const long long *)psP1, (const int *)lstP,
+                         lstAllP);
+    if (nOverP0 > 0)
+      hipLaunchKernelGGL(k_partner_rescan_cpu, dim3(nOverP0 < 65536 ? nOverP0 : 65536),
+                         dim3(64), 0, c.stream, nOverP0, (const int *)listOP, true, mapP,
+                         (const long long *)offP, (const long long *)offQ,
+                         (const double *)ringP, (const double *)ringQ, (const long long *)psP1,
+                         lstAllP, (const double *)elenP, (const double *)emrgP,
+                         (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
+                         boxQ);
+    HIP_CHECK(hipGetLastError());
+    // -- the Q side by transposition
+    fill_i32(c, cnt1Q, 0, edgesQ);
+    fill_i32(c, cnt2Q, 1, edgesQ);
+    if (edgesQ > 0)
+      hipLaunchKernelGGL(k_cpu_flags_q, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
+                         edgesQ, mapQ, (const long long *)offP, (const long long *)offQ,
+                         (const double *)ringQ, (const double *)elenP, (const double *)emrgP,
+                         (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
+                         boolQ);
+    uint64_t *tk = wsT<uint64_t>(c, "gh_trK", H), *tk2 = wsT<uint64_t>(c, "gh_trK2", H);
+    int *tv = wsT<int>(c, "gh_trV", H);
+    lstAllQ = wsT<int>(c, "gh_lstAllQ", H);
+    if (nActP0 > 0) {
+      LaunchTimer t(c, "gh_transpose");
+      hipLaunchKernelGGL(k_cpu_transpose, dim3(grid_for(nActP0, 256)), dim3(256), 0, c.stream,
+                         nActP0, (const int *)actP, mapP, (const long long *)offP,
+                         (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
+                         (const long long *)psP1, (const int *)lstAllP, cnt1Q, cnt2Q, tk, tv);
+      HIP_CHECK(hipGetLastError());
+    }
+    if (H > 0) {
+      int bits = 0;
+      for (long x = edgesQ; x > 0; x >>= 1) bits++;
+      radix_sort_u64(c, tk, tv, tk2, lstAllQ, H, bits);
+    }
+    scan_i32_to_i64(c, cnt1Q, psQ1, edgesQ);
+    scan_i32_to_i64(c, cnt2Q, psQ2, edgesQ);
+    hipLaunchKernelGGL(k_pair_check, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
+                       (const long long *)offP, (const long long *)offQ,
+                       (const long long *)psP1, (const long long *)psQ1,
+                       (const long long *)psP2, (const long long *)psQ2, pm.status, stats + 1);
+    if (edgesQ > 0)
+      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesQ, 1024)), dim3(1024), 0,
+                         c.stream, edgesQ, mapQ, (const long long *)psQ2,
+                         (const long long *)psQ1, pm.status, listSQ, listMQ, listQ, lenQ, listOQ,
+                         runCnt + 8);
+    HIP_CHECK(hipGetLastError());
+    const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
+    fill_i32(c, (int *)bigCnt, 0, 2);
+    hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M,
+                       (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
+                       (const long long *)psQ2, (long long)c.tune.bigNodes, bigList, bigCnt);
+    HIP_CHECK(hipGetLastError());
+    long long rq[7];
+    read_scalars(c, {psQ2 + edgesQ, rc + 8, rc + 10, rc + 11, nActQd, rc + 9,
+                     (const long long *)bigCnt},
+                 rq);
+    rb[1] = rq[0];
+    rb[5] = rq[1];
+    rb[6] = rq[2];
+    rb[7] = rq[3];
+    rb[9] = rq[4];
+    rb[13] = rq[5];
+    rb[16] = rq[6];
+  } else {
+    if (edgesP + edgesQ > 0) {
+      LaunchTimer t(c, "gh_count");
+      const bool cull = boxP.b1 != nullptr || boxQ.b1 != nullptr;
+      const int cb = c.tune.countBlock;
+      if (cpuRoute)
+        hipLaunchKernelGGL(cull ? k_count_cpu<true> : k_count_cpu<false>,
+                           dim3(grid_for(edgesP + edgesQ, cb)), dim3(cb), 0, c.stream, edgesP,
+                           edgesQ, mapP, mapQ, (const long long *)offP, (const long long *)offQ,
+                           ringP, ringQ, (const double *)elenP, (const double *)emrgP,
+                           (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
+                           cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, stats, lstP, lstQ, boxP,
+                           boxQ);
+      else
+        hipLaunchKernelGGL(cull ? k_count<true> : k_count<false>,
+                           dim3(grid_for(edgesP + edgesQ, cb)), dim3(cb), 0, c.stream, edgesP,
+                           edgesQ, mapP, mapQ, (const long long *)offP, (const long long *)offQ,
+                           ringP, ringQ, pm.cmbrN, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ,
+                           stats, lstP, lstQ, boxP, boxQ);
+      HIP_CHECK(hipGetLastError());
+    }
+    scan_i32_to_i64(c, cnt1P, psP1, edgesP);
+    scan_i32_to_i64(c, cnt2P, psP2, edgesP);
+    scan_i32_to_i64(c, cnt1Q, psQ1, edgesQ);
+    scan_i32_to_i64(c, cnt2Q, psQ2, edgesQ);
+    hipLaunchKernelGGL(k_pair_check, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
+                       (const long long *)offP, (const long long *)offQ, (const long long *)psP1,
+                       (const long long *)psQ1, (const long long *)psP2, (const long long *)psQ2,
+                       pm.status, stats + 1);
+    HIP_CHECK(hipGetLastError());
+    // edges whose alpha runs go through the segmented sort (statuses are final
+    // here), read back with the slot totals
+    fill_i32(c, (int *)runCnt, 0, 32);  // 8 counters per side
+    if (edgesP > 0)
+      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesP, 1024)), dim3(1024), 0, c.stream,
+                         edgesP, mapP, (const long long *)psP2, (const long long *)psP1, pm.status,
+                         listSP, listMP, listP, lenP, listOP, runCnt);
+    if (edgesQ > 0)
+      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesQ, 1024)), dim3(1024), 0, c.stream,
+                         edgesQ, mapQ, (const long long *)psQ2, (const long long *)psQ1, pm.status,
+                         listSQ, listMQ, listQ, lenQ, listOQ, runCnt + 8);
+    HIP_CHECK(hipGetLastError());
+    // edges with intersections (count1 > 0): the later passes run over these
+    const long long *nActPd = compact_flags_dev(c, cnt1P, edgesP, actP, "actP");
+    const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
+    // the pairs labelled by a block (their number comes back with the totals)
+    fill_i32(c, (int *)bigCnt, 0, 2);
+    hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M,
+                       (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
+                       (const long long *)psQ2, (long long)c.tune.bigNodes, bigList, bigCnt);
+    HIP_CHECK(hipGetLastError());
+    {
+      const long long *rc = (const long long *)runCnt;
+      read_scalars(c, {psP2 + edgesP, psQ2 + edgesQ, rc, rc + 2, rc + 3, rc + 8, rc + 10, rc + 11,
+                       nActPd, nActQd, psP1 + edgesP, psQ1 + edgesQ, rc + 1, rc + 9, rc + 4,
+                       rc + 12, (const long long *)bigCnt},
+                   rb);
+    }
+    const long nActP = (long)rb[8], nActQ = (long)rb[9];
+    const long nOverP = (long)rb[14], nOverQ = (long)rb[15];
+    // every active edge's partners at its count1 offset
+    lstAllP = wsT<int>(c, "gh_lstAllP", rb[10]);
+    lstAllQ = wsT<int>(c, "gh_lstAllQ", rb[11]);
+    if (nActP > 0)
+      hipLaunchKernelGGL(k_partner_lists, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream, nActP,
+                         (const int *)actP, (const long long *)psP1, (const int *)lstP, lstAllP);
+    if (nActQ > 0)
+      hipLaunchKernelGGL(k_partner_lists, dim3(grid_for(nActQ, 256)), dim3(256), 0, c.stream, nActQ,
+                         (const int *)actQ, (const long long *)psQ1, (const int *)lstQ, lstAllQ);
+    if (nOverP > 0 && cpuRoute)
+      hipLaunchKernelGGL(k_partner_rescan_cpu, dim3(nOverP < 65536 ? nOverP : 65536), dim3(64), 0,
+                         c.stream, nOverP, (const int *)listOP, true, mapP,
+                         (const long long *)offP, (const long long *)offQ, (const double *)ringP,
+                         (const double *)ringQ, (const long long *)psP1, lstAllP,
+                         (const double *)elenP, (const double *)emrgP, (const double *)elenQ,
+                         (const double *)emrgQ, (const double *)cf, boxQ);
+    else if (nOverP > 0)
+      hipLaunchKernelGGL(k_partner_rescan, dim3(nOverP < 65536 ? nOverP : 65536), dim3(64), 0,
+                         c.stream, nOverP, (const int *)listOP, mapP, (const long long *)offP,
+                         (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
+                         (const long long *)psP1, lstAllP, boxQ);
+    if (nOverQ > 0 && cpuRoute)
+      hipLaunchKernelGGL(k_partner_rescan_cpu, dim3(nOverQ < 65536 ? nOverQ : 65536), dim3(64), 0,
+                         c.stream, nOverQ, (const int *)listOQ, false, mapQ,
+                         (const long long *)offQ, (const long long *)offP, (const double *)ringQ,
+                         (const double *)ringP, (const long long *)psQ1, lstAllQ,
+                         (const double *)elenP, (const double *)emrgP, (const double *)elenQ,
+                         (const double *)emrgQ, (const double *)cf, boxP);
+    else if (nOverQ > 0)
+      hipLaunchKernelGGL(k_partner_rescan, dim3(nOverQ < 65536 ? nOverQ : 65536), dim3(64), 0,
+                         c.stream, nOverQ, (const int *)listOQ, mapQ, (const long long *)offQ,
+                         (const long long *)offP, (const double *)ringQ, (const double *)ringP,
+                         (const long long *)psQ1, lstAllQ, boxP);
+    HIP_CHECK(hipGetLastError());
   }
   const long nBig = (long)rb[16];
   const long nActP = (long)rb[8], nActQ = (long)rb[9];
   const long nShortP = (long)rb[12], nShortQ = (long)rb[13];
-  const long nOverP = (long)rb[14], nOverQ = (long)rb[15];
-  // every active edge's partners at its count1 offset
-  int *lstAllP = wsT<int>(c, "gh_lstAllP", rb[10]), *lstAllQ = wsT<int>(c, "gh_lstAllQ", rb[11]);
-  if (nActP > 0)
-    hipLaunchKernelGGL(k_partner_lists, dim3(grid_for(nActP, 256)), dim3(256), 0, c.stream, nActP,
-                       (const int *)actP, (const long long *)psP1, (const int *)lstP, lstAllP);
-  if (nActQ > 0)
-    hipLaunchKernelGGL(k_partner_lists, dim3(grid_for(nActQ, 256)), dim3(256), 0, c.stream, nActQ,
-                       (const int *)actQ, (const long long *)psQ1, (const int *)lstQ, lstAllQ);
-  if (nOverP > 0 && cpuRoute)
-    hipLaunchKernelGGL(k_partner_rescan_cpu, dim3(nOverP < 65536 ? nOverP : 65536), dim3(64), 0,
-                       c.stream, nOverP, (const int *)listOP, true, mapP,
-                       (const long long *)offP, (const long long *)offQ, (const double *)ringP,
-                       (const double *)ringQ, (const long long *)psP1, lstAllP,
-                       (const double *)elenP, (const double *)emrgP, (const double *)elenQ,
-                       (const double *)emrgQ, (const double *)cf, boxQ);
-  else if (nOverP > 0)
-    hipLaunchKernelGGL(k_partner_rescan, dim3(nOverP < 65536 ? nOverP : 65536), dim3(64), 0,
-                       c.stream, nOverP, (const int *)listOP, mapP, (const long long *)offP,
-                       (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
-                       (const long long *)psP1, lstAllP, boxQ);
-  if (nOverQ > 0 && cpuRoute)
-    hipLaunchKernelGGL(k_partner_rescan_cpu, dim3(nOverQ < 65536 ? nOverQ : 65536), dim3(64), 0,
-                       c.stream, nOverQ, (const int *)listOQ, false, mapQ,
-                       (const long long *)offQ, (const long long *)offP, (const double *)ringQ,
-                       (const double *)ringP, (const long long *)psQ1, lstAllQ,
-                       (const double *)elenP, (const double *)emrgP, (const double *)elenQ,
-                       (const double *)emrgQ, (const double *)cf, boxP);
-  else if (nOverQ > 0)
-    hipLaunchKernelGGL(k_partner_rescan, dim3(nOverQ < 65536 ? nOverQ : 65536), dim3(64), 0,
-                       c.stream, nOverQ, (const int *)listOQ, mapQ, (const long long *)offQ,
-                       (const long long *)offP, (const double *)ringQ, (const double *)ringP,
-                       (const long long *)psQ1, lstAllQ, boxP);
-  HIP_CHECK(hipGetLastError());
   const long slotsP = (long)rb[0], slotsQ = (long)rb[1];
   const int nMedP = (int)rb[2], nRunsP = (int)rb[3], nMedQ = (int)rb[5], nRunsQ = (int)rb[6];
   const long runTotP = (long)rb[4], runTotQ = (long)rb[7];
