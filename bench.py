@@ -459,7 +459,8 @@ def prims(args):
         ctx.synchronize()
         ctx.set_timing(False)
         names = ["mbr_endpoints", "sort_hist", "sort_scatter", "scan_tile", "scan_reduce",
-                 "scan_downsweep", "mbr_sweep_prep", "mbr_sweep_count", "mbr_sweep_write"]
+                 "scan_downsweep", "scan_lookback", "mbr_sweep_prep", "mbr_sweep_count",
+                 "mbr_sweep_write"]
         ms = sum(ctx.timer(k)[0] for k in names)
         sweep_ms = sum(ctx.timer(k)[0] for k in ("mbr_sweep_prep", "mbr_sweep_count",
                                                   "mbr_sweep_write"))

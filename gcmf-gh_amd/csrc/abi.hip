@@ -199,7 +199,7 @@ int gcmf_radix_sort_u64(gcmf_ctx *ctx, const uint64_t *d_keys_in, const int *d_v
       throw gcmf::Error{GCMF_ERR_ARG, "bad radix sort arguments"};
     if (n >= 0x7fffffffL) throw gcmf::Error{GCMF_ERR_ARG, "radix sort of 2^31 keys or more"};
     gcmf::radix_sort_u64(ctx->c, d_keys_in, d_vals_in, d_keys_out, d_vals_out, n, bits);
-    HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    gcmf::check_lookback(ctx->c);
   });
 }
 
@@ -207,7 +207,7 @@ int gcmf_scan_i32_i64(gcmf_ctx *ctx, const int *d_in, long long *d_out, long n) 
   return guarded(ctx, [&] {
     if (n < 0 || !d_out || (n > 0 && !d_in)) throw gcmf::Error{GCMF_ERR_ARG, "bad scan arguments"};
     gcmf::scan_i32_to_i64(ctx->c, d_in, d_out, n);
-    HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    gcmf::check_lookback(ctx->c);
   });
 }
 
@@ -217,6 +217,7 @@ int gcmf_compact_flags(gcmf_ctx *ctx, const int *d_flags, long n, int *d_idx, lo
       throw gcmf::Error{GCMF_ERR_ARG, "bad compaction arguments"};
     if (n >= 0x7fffffffL) throw gcmf::Error{GCMF_ERR_ARG, "compaction of 2^31 flags or more"};
     *count = gcmf::compact_flags(ctx->c, d_flags, n, d_idx, "api");
+    gcmf::check_lookback(ctx->c);
   });
 }
 

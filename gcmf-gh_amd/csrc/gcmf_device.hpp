@@ -113,6 +113,18 @@ __device__ __forceinline__ int initial_label(int m, int p) {
 }
 
 // wavefront helpers (wave64)
+// MI355X deals the blocks of a launch to its 8 XCDs round-robin (block b to
+// XCD b % 8), and each XCD has its own L2.  Kernels whose neighbouring tiles
+// write into shared cache lines (scatters, compactions) process tile
+// xcd_tile(b) instead of tile b: every XCD then owns one contiguous range of
+// tiles, and the partly written lines at tile boundaries are merged in one
+// L2 instead of being written back piecewise from two.  The map is a
+// bijection of [0, tiles) for any XCD count; only its speed assumes 8.
+__device__ __forceinline__ long xcd_tile(long b, long tiles) {
+  const long q = tiles >> 3, r = tiles & 7, x = b & 7, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ unsigned long long ballot(bool b) { return __ballot(b); }
 __device__ __forceinline__ int popc64(unsigned long long m) { return __popcll(m); }

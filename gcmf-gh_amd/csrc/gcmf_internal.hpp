@@ -78,6 +78,10 @@ struct Context {
   // with the events that fork it from and join it to `stream`
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // device word set when a single-pass scan's look-back wait ran past its
+  // bound (never in a correct run); read back and cleared by read_scalars
+  // and check_lookback
+  long long *lb_err = nullptr;
 
   hipEvent_t take_event();
   void flush_timers();  // synchronises on pending events and accumulates
@@ -108,6 +112,9 @@ void radix_sort_u64(Context &c, const uint64_t *keys_in, const int *vals_in, uin
                     int *vals_out, long n, int bits);
 // fills
 void fill_i32(Context &c, int *p, int v, long n);
+// raises an Error when a single-pass scan's look-back wait ran past its
+// bound since the last check (synchronises)
+void check_lookback(Context &c);
 // copies one value back to the host (synchronises)
 template <typename T>
 T read_scalar(Context &c, const T *dptr) {

@@ -8,7 +8,11 @@
 // wave64 shuffles inside a wave, LDS across the 4 waves of a block.
 // Radix sort: 8-bit digits, LDS digit histograms per 4096-key tile, a
 // digit-major scan of the histograms, and a stable scatter ranked with
-// 8 wave ballots per 64 keys (peer mask = keys with the same digit).
+// 8 wave ballots per 64 keys (peer mask = keys with the same digit).  The
+// scatter and the compaction's write pass take their tiles through
+// xcd_tile, so that neighbouring tiles, whose outputs share cache lines at
+// every digit run's ends, run on one XCD (measured on MI355X at 2^27 keys:
+// 1.57 -> 0.75 ms per scatter pass).
 #include "gcmf_device.hpp"
 #include "gcmf_internal.hpp"
 
@@ -84,20 +88,31 @@ __global__ void k_gather_scalars(ScalarSrc s, int n, long long *dst) {
   if (i < n) dst[i] = s.p[i] ? *s.p[i] : 0;
 }
 
+// the look-back error word rides along as one more slot
 void read_scalars(Context &c, std::initializer_list<const long long *> src, long long *out) {
   const int n = (int)src.size();
-  if (n > kHostSlots) throw Error{GCMF_ERR_ARG, "too many scalars in one read-back"};
+  if (n + 1 > kHostSlots) throw Error{GCMF_ERR_ARG, "too many scalars in one read-back"};
   ScalarSrc s{};
   int i = 0;
   for (const long long *p : src) s.p[i++] = p;
+  s.p[n] = c.lb_err;
   long long *d = (long long *)c.ws.get("read_scalars", sizeof(long long) * kHostSlots);
-  hipLaunchKernelGGL(k_gather_scalars, dim3(1), dim3(64), 0, c.stream, s, n, d);
+  hipLaunchKernelGGL(k_gather_scalars, dim3(1), dim3(64), 0, c.stream, s, n + 1, d);
   HIP_CHECK(hipGetLastError());
   if (!c.hbuf) HIP_CHECK(hipHostMalloc((void **)&c.hbuf, sizeof(long long) * kHostSlots));
-  HIP_CHECK(hipMemcpyAsync(c.hbuf, d, sizeof(long long) * (size_t)n, hipMemcpyDeviceToHost,
+  HIP_CHECK(hipMemcpyAsync(c.hbuf, d, sizeof(long long) * (size_t)(n + 1), hipMemcpyDeviceToHost,
                            c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   for (int j = 0; j < n; j++) out[j] = c.hbuf[j];
+  if (c.hbuf[n] != 0) {
+    fill_i32(c, (int *)c.lb_err, 0, 2);
+    throw Error{GCMF_ERR_HIP, "a scan's look-back wait exceeded its bound"};
+  }
+}
+
+void check_lookback(Context &c) {
+  long long v;
+  read_scalars(c, {}, &v);
 }
 
 // --------------------------------------------------------------- fill ----
@@ -329,6 +344,267 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---- single-pass scans with decoupled look-back (Merrill and Garland,
+// "Single-pass Parallel Prefix Scan with Decoupled Look-back", NVIDIA
+// technical report NVR-2016-002), over the same 4096-element tiles.
+//
+// A block takes its tile from an atomic counter, so every tile it waits for
+// belongs to a block that is already running, and that block publishes its
+// tile's sum before it waits for anything: every wait ends.  The wait is
+// still bounded (kLbSpinLimit sleeps); past the bound the block records an
+// error in the context's error word, which the next read-back raises.
+//
+// A tile descriptor is one 64-bit word, value * 4 + status (kLbAgg: the
+// tile's own sum; kLbIncl: the sum of all tiles through this one), stored
+// and loaded with device-scope atomics.  The value is the only data passed
+// between blocks, so no other memory needs ordering across the XCDs' L2s.
+// Values must stay inside [-2^61, 2^61), which holds for fewer than 2^30
+// int32 inputs.
+//
+// Each poll is a device-scope load that bypasses the XCD's L2, and tile t
+// reaches the nearest published inclusive prefix only after the tiles
+// before it have published theirs, so the chain of look-backs, not HBM,
+// bounds this path on long arrays: measured on MI355X, 0.67 ms for 2^27
+// elements against 0.44 ms for the reduce-then-scan path.  It is used up
+// to 2^20 elements (256 tiles), where it replaces that path's four
+// launches by two.
+constexpr unsigned long long kLbAgg = 1, kLbIncl = 2;
+constexpr long kLbMaxN = 1L << 20;
+constexpr int kLbSpinLimit = 1 << 22;
+
+__device__ __forceinline__ void lb_publish(unsigned long long *d, long long v,
+                                           unsigned long long st) {
+  __hip_atomic_store(d, ((unsigned long long)v << 2) | st, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The sum of all tiles before tile t (t > 0), by one wave: lane l reads the
+// descriptor of tile j - l, 64 tiles a window, until the nearest inclusive
+// descriptor with every descriptor after it published.
+__device__ long long lb_lookback(const unsigned long long *desc, long t, long long *err) {
+  const int lane = __lane_id();
+  long long prefix = 0;
+  long j = t - 1;
+  int spins = 0;
+  while (true) {
+    const long idx = j - lane;
+    unsigned long long d;
+    unsigned long long ready, incl;
+    while (true) {
+      d = idx >= 0 ? __hip_atomic_load(desc + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                   : kLbIncl;
+      const unsigned long long notReady = __ballot((d & 3) == 0);
+      incl = __ballot((d & 3) == kLbIncl);
+      // lanes before the first unpublished one can be used
+      ready = notReady ? ((1ULL << (__ffsll((long long)notReady) - 1)) - 1ULL) : ~0ULL;
+      if ((incl & ready) || !notReady) break;
+      if (++spins > kLbSpinLimit) {
+        if (lane == 0) atomicExch((unsigned long long *)err, 1ULL);
+        incl = 1;  // give up: treat the nearest tile as the end
+        ready = 1;
+        d = lane == 0 ? (d & ~3ULL) | kLbIncl : 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    const unsigned long long hit = incl & ready;
+    const int stop = hit ? __ffsll((long long)hit) - 1 : 63;
+    long long v = lane <= stop ? ((long long)d >> 2) : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    prefix += v;
+    if (hit) return prefix;
+    j -= 64;
+  }
+}
+
+// Exclusive scan of four values per thread over a 256-thread block, in
+// group-major order (every thread's s[0], then every thread's s[1], ...).
+__device__ __forceinline__ void block_excl_scan4(const long long s[4], long long e[4],
+                                                 long long &total, long long (*lds)[4]) {
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  long long inc[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) inc[k] = wave_incl_scan(s[k]);
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < 4; k++) lds[wave][k] = inc[k];
+  __syncthreads();
+  long long run = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    long long wb = 0, gt = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      const long long x = lds[w][k];
+      wb += w < wave ? x : 0;
+      gt += x;
+    }
+    e[k] = run + wb + inc[k] - s[k];
+    run += gt;
+  }
+  total = run;
+}
+
+__device__ __forceinline__ void ld_tile4(const int *in, long n, long long base, bool full,
+                                         int4 v[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (full) {
+      v[k] = ld_i4(in, base, k);
+    } else {
+      const long long i = base + k * 1024 + 4 * threadIdx.x;
+      v[k].x = i < n ? in[i] : 0;
+      v[k].y = i + 1 < n ? in[i + 1] : 0;
+      v[k].z = i + 2 < n ? in[i + 2] : 0;
+      v[k].w = i + 3 < n ? in[i + 3] : 0;
+    }
+  }
+}
+
+// Takes the block's tile, publishes its sum, and returns the sum of the
+// tiles before it (every thread).
+__device__ __forceinline__ long long lb_tile_prefix(unsigned long long *desc, long t,
+                                                    long long tot, long long *err,
+                                                    long long *s_pre) {
+  if (threadIdx.x == 0) lb_publish(desc + t, tot, t == 0 ? kLbIncl : kLbAgg);
+  if (t > 0) {
+    if (threadIdx.x < 64) {
+      const long long pre = lb_lookback(desc, t, err);
+      if (threadIdx.x == 0) {
+        lb_publish(desc + t, pre + tot, kLbIncl);
+        *s_pre = pre;
+      }
+    }
+  } else if (threadIdx.x == 0) {
+    *s_pre = 0;
+  }
+  __syncthreads();
+  return *s_pre;
+}
+
+__global__ void k_lb_clear(unsigned long long *desc, long tiles) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= tiles) desc[i] = 0;  // desc[tiles] holds the tile counter
+}
+
+__global__ void __launch_bounds__(256)
+    k_lb_scan(const int *in, long n, long long *out, unsigned long long *desc, long tiles,
+              long long *err) {
+  __shared__ long long lds[4][4];
+  __shared__ long long s_pre;
+  __shared__ long s_tile;
+  if (threadIdx.x == 0) s_tile = (long)atomicAdd(desc + tiles, 1ULL);
+  __syncthreads();
+  const long t = s_tile;
+  const long long base = (long long)t * kVecTile;
+  const bool full = base + kVecTile <= n;
+  int4 v[4];
+  ld_tile4(in, n, base, full, v);
+  long long s[4], e[4], tot;
+#pragma unroll
+  for (int k = 0; k < 4; k++) s[k] = (long long)v[k].x + v[k].y + v[k].z + v[k].w;
+  block_excl_scan4(s, e, tot, lds);
+  const long long pre = lb_tile_prefix(desc, t, tot, err, &s_pre);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const long long o0 = pre + e[k], o1 = o0 + v[k].x, o2 = o1 + v[k].y, o3 = o2 + v[k].z;
+    const long long i = base + k * 1024 + 4 * threadIdx.x;
+    if (full) {
+      longlong2 *d = (longlong2 *)(out + i);
+      d[0] = make_longlong2(o0, o1);
+      d[1] = make_longlong2(o2, o3);
+    } else {
+      if (i < n) out[i] = o0;
+      if (i + 1 < n) out[i + 1] = o1;
+      if (i + 2 < n) out[i + 2] = o2;
+      if (i + 3 < n) out[i + 3] = o3;
+    }
+  }
+  if (t == tiles - 1 && threadIdx.x == 0) out[n] = pre + tot;
+}
+
+__global__ void __launch_bounds__(256)
+    k_lb_compact(const int *flag, long n, int *out, long long *count, unsigned long long *desc,
+                 long tiles, long long *err) {
+  __shared__ long long lds[4][4];
+  __shared__ long long s_pre;
+  __shared__ long s_tile;
+  if (threadIdx.x == 0) s_tile = (long)atomicAdd(desc + tiles, 1ULL);
+  __syncthreads();
+  const long t = s_tile;
+  const long long base = (long long)t * kVecTile;
+  const bool full = base + kVecTile <= n;
+  int4 v[4];
+  ld_tile4(flag, n, base, full, v);
+  long long s[4], e[4], tot;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    s[k] = (v[k].x != 0) + (v[k].y != 0) + (v[k].z != 0) + (v[k].w != 0);
+  block_excl_scan4(s, e, tot, lds);
+  const long long pre = lb_tile_prefix(desc, t, tot, err, &s_pre);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const long long i = base + k * 1024 + 4 * threadIdx.x;
+    long long p = pre + e[k];
+    if (v[k].x != 0) out[p++] = (int)i;
+    if (v[k].y != 0) out[p++] = (int)(i + 1);
+    if (v[k].z != 0) out[p++] = (int)(i + 2);
+    if (v[k].w != 0) out[p++] = (int)(i + 3);
+  }
+  if (t == tiles - 1 && threadIdx.x == 0) *count = pre + tot;
+}
+
+static long long *lb_err_word(Context &c) {
+  if (!c.lb_err) {
+    c.lb_err = (long long *)c.ws.get("lookback_err", 256);
+    fill_i32(c, (int *)c.lb_err, 0, 2);
+  }
+  return c.lb_err;
+}
+
+// the tile descriptors and the tile counter, cleared for one launch
+static unsigned long long *lb_desc(Context &c, long tiles) {
+  unsigned long long *desc =
+      (unsigned long long *)c.ws.get("lookback_desc", sizeof(unsigned long long) * (tiles + 1));
+  hipLaunchKernelGGL(k_lb_clear, dim3(grid_for(tiles + 1, 256)), dim3(256), 0, c.stream, desc,
+                     tiles);
+  HIP_CHECK(hipGetLastError());
+  return desc;
+}
+
+static bool lb_fits(const void *a, const void *b, long n) {
+  return n > 0 && n <= kLbMaxN && !((uintptr_t)a & 15) && !((uintptr_t)b & 15);
+}
+
+static bool lb_scan(Context &c, const int *in, long long *out, long n) {
+  if (!lb_fits(in, out, n)) return false;
+  const long tiles = (n + kVecTile - 1) / kVecTile;
+  long long *err = lb_err_word(c);
+  unsigned long long *desc = lb_desc(c, tiles);
+  LaunchTimer t(c, "scan_lookback");
+  hipLaunchKernelGGL(k_lb_scan, dim3((unsigned)tiles), dim3(256), 0, c.stream, in, n, out, desc,
+                     tiles, err);
+  HIP_CHECK(hipGetLastError());
+  return true;
+}
+
+// returns the device address of the count, or nullptr when the arrays do
+// not suit the single-pass path
+static const long long *lb_compact(Context &c, const int *flag, long n, int *out_idx,
+                                   const std::string &t) {
+  if (!lb_fits(flag, flag, n)) return nullptr;
+  const long tiles = (n + kVecTile - 1) / kVecTile;
+  long long *count = (long long *)c.ws.get(t + "_n", sizeof(long long));
+  long long *err = lb_err_word(c);
+  unsigned long long *desc = lb_desc(c, tiles);
+  LaunchTimer tm(c, "compact_lookback");
+  hipLaunchKernelGGL(k_lb_compact, dim3((unsigned)tiles), dim3(256), 0, c.stream, flag, n,
+                     out_idx, count, desc, tiles, err);
+  HIP_CHECK(hipGetLastError());
+  return count;
+}
+
 template <typename TI, typename TO>
 static void scan_impl(Context &c, const TI *in, TO *out, long n, int depth);
 
@@ -414,6 +690,7 @@ void scan_i32_to_i64(Context &c, const int *in, long long *out, long n) {
     HIP_CHECK(hipGetLastError());
     return;
   }
+  if (lb_scan(c, in, out, n)) return;
   if (vscan(c, in, out, n)) return;
   scan_impl<int, long long>(c, in, out, n, 0);
 }
@@ -464,10 +741,11 @@ __global__ void __launch_bounds__(256) k_vcompact_count(const int *flag, long n,
 }
 
 __global__ void __launch_bounds__(256)
-    k_vcompact_write(const int *flag, long n, const long long *offsets, int *out) {
+    k_vcompact_write(const int *flag, long n, const long long *offsets, long tiles, int *out) {
   __shared__ long long lds[4];
-  const long long base = (long long)blockIdx.x * kVecTile;
-  long long carry = offsets[blockIdx.x];
+  const long T = xcd_tile(blockIdx.x, tiles);
+  const long long base = (long long)T * kVecTile;
+  long long carry = offsets[T];
   const bool full = base + kVecTile <= n;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -504,13 +782,15 @@ static const long long *vcompact(Context &c, const int *flag, long n, int *out_i
   HIP_CHECK(hipGetLastError());
   scan_impl<long long, long long>(c, cnt, cnt, tiles, 1);
   hipLaunchKernelGGL(k_vcompact_write, dim3((unsigned)tiles), dim3(256), 0, c.stream, flag, n,
-                     (const long long *)cnt, out_idx);
+                     (const long long *)cnt, tiles, out_idx);
   HIP_CHECK(hipGetLastError());
   return cnt + tiles;
 }
 
 long compact_flags(Context &c, const int *flag, long n, int *out_idx, const char *tag) {
   if (n <= 0) return 0;
+  if (const long long *d = lb_compact(c, flag, n, out_idx, std::string("compact_") + tag))
+    return (long)read_scalar(c, d);
   if (const long long *d = vcompact(c, flag, n, out_idx, std::string("compact_") + tag))
     return (long)read_scalar(c, d);
   std::string t = std::string("compact_") + tag;
@@ -534,6 +814,7 @@ const long long *compact_flags_dev(Context &c, const int *flag, long n, int *out
     HIP_CHECK(hipGetLastError());
     return pos;
   }
+  if (const long long *d = lb_compact(c, flag, n, out_idx, t)) return d;
   if (const long long *d = vcompact(c, flag, n, out_idx, t)) return d;
   int *f01 = (int *)c.ws.get(t + "_f", sizeof(int) * (size_t)n);
   hipLaunchKernelGGL(k_flags01, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, flag, f01, n);
@@ -558,10 +839,17 @@ __global__ void __launch_bounds__(kSortThreads)
   h[threadIdx.x] = 0;
   __syncthreads();
   const long base = (long)blockIdx.x * kSortTile;
-#pragma unroll 4
+  // every load issued before the first LDS update
+  uint64_t kk[kSortItems];
+#pragma unroll
   for (int k = 0; k < kSortItems; k++) {
-    long i = base + (long)k * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & 255)], 1);
+    const long i = base + (long)k * kSortThreads + threadIdx.x;
+    kk[k] = i < n ? keys[i] : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < kSortItems; k++) {
+    const long i = base + (long)k * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(int)((kk[k] >> shift) & 255)], 1);
   }
   __syncthreads();
   hist[(long)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
@@ -574,18 +862,29 @@ __global__ void __launch_bounds__(kSortThreads)
                    const long long *off, long tiles, uint64_t *keys_out, int *vals_out) {
   __shared__ int wcnt[kSortThreads / 64][256];
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const long T = xcd_tile(blockIdx.x, tiles);
+  // this tile's output start for digit threadIdx.x (256 threads, 256
+  // digits), fetched now so that its latency hides behind the ranking
+  const long long dOff = off[(long)threadIdx.x * tiles + T];
   for (int d = threadIdx.x; d < 256 * (kSortThreads / 64); d += kSortThreads)
     (&wcnt[0][0])[d] = 0;
-  __syncthreads();
-  const long wbase = (long)blockIdx.x * kSortTile + (long)wave * kWaveKeys;
+  const long wbase = T * kSortTile + (long)wave * kWaveKeys;
   uint64_t kk[kSortItems];
   int vv[kSortItems];
   int pos[kSortItems];
+  // all of the wave's loads in flight at once, before the ranking rounds
+#pragma unroll
+  for (int r = 0; r < kSortItems; r++) {
+    const long i = wbase + (long)r * 64 + lane;
+    kk[r] = i < n ? keys[i] : 0;
+    vv[r] = i < n ? vals[i] : 0;
+  }
+  __syncthreads();
 #pragma unroll
   for (int r = 0; r < kSortItems; r++) {
     long i = wbase + (long)r * 64 + lane;
     bool valid = i < n;
-    uint64_t key = valid ? keys[i] : 0;
+    uint64_t key = kk[r];
     int d = (int)((key >> shift) & 255);
     unsigned long long peers = __ballot(valid);
 #pragma unroll
@@ -595,8 +894,6 @@ __global__ void __launch_bounds__(kSortThreads)
     }
     int before = valid ? wcnt[wave][d] : 0;
     int rk = __popcll(peers & ((1ULL << lane) - 1ULL));
-    kk[r] = key;
-    vv[r] = valid ? vals[i] : 0;
     pos[r] = valid ? before + rk : -1;
     // the lowest lane of each peer group advances the wave's digit counter
     bool leader = valid && rk == 0;
@@ -633,14 +930,20 @@ __global__ void __launch_bounds__(kSortThreads)
     sv[lp] = vv[r];
   }
   __syncthreads();
+  // the per-wave counters are dead now: their LDS holds each digit's
+  // output start minus its start inside the tile
+  static_assert(sizeof(wcnt) >= 256 * sizeof(long long), "digit bases do not fit");
+  long long *gbase = (long long *)&wcnt[0][0];
+  gbase[threadIdx.x] = dOff - dstart[threadIdx.x];
+  __syncthreads();
   // ... then written out in that order: consecutive threads store
   // consecutive addresses inside each digit's run
-  const long tbase = (long)blockIdx.x * kSortTile;
+  const long tbase = T * kSortTile;
   const int tn = (int)min((long)kSortTile, n - tbase);
   for (int i = threadIdx.x; i < tn; i += kSortThreads) {
     const uint64_t key = sk[i];
     const int d = (int)((key >> shift) & 255);
-    const long long dst = off[(long)d * tiles + blockIdx.x] + (i - dstart[d]);
+    const long long dst = gbase[d] + i;
     keys_out[dst] = key;
     vals_out[dst] = sv[i];
   }

@@ -128,7 +128,9 @@ def test_radix_sort_entry_point(gctx, n, bits):
         b.free()
 
 
-@pytest.mark.parametrize("n", [0, 1, 4097, 3_000_000])
+# sizes around the single-pass scan's 4096-element tiles and its 64-tile
+# look-back windows
+@pytest.mark.parametrize("n", [0, 1, 4096, 4097, 64 * 4096 + 5, 3_000_000, 20_000_003])
 def test_scan_and_compact_entry_points(gctx, n):
     """gcmf_scan_i32_i64 is the exclusive cumulative sum with the total last;
     gcmf_compact_flags lists the nonzero flags in order."""
@@ -146,3 +148,18 @@ def test_scan_and_compact_entry_points(gctx, n):
     np.testing.assert_array_equal(gctx.download(d_idx, np.int32, cnt), want)
     for b in (d_in, d_out, d_f, d_idx):
         b.free()
+
+
+def test_scan_extreme_values(gctx):
+    """int32 extremes of both signs over 2^22 + 7 elements: the running sum
+    leaves the int32 range many times and every tile's carry is 64-bit."""
+    n = (1 << 22) + 7
+    rs = np.random.RandomState(11)
+    a = np.where(rs.rand(n) < 0.7, np.int32(2**31 - 1), np.int32(-2**31)).astype(np.int32)
+    d_in, d_out = gctx.upload(a), gctx.upload(np.zeros(n + 1, np.int64))
+    for _ in range(2):  # the second call reuses the tile descriptors
+        gctx.scan(d_in, d_out, n)
+        ref = np.concatenate([[0], np.cumsum(a.astype(np.int64))])
+        np.testing.assert_array_equal(gctx.download(d_out, np.int64, n + 1), ref)
+    d_in.free()
+    d_out.free()
