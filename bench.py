@@ -30,6 +30,9 @@ FP64_PEAK_TFS = 78.6
 # algorithmic FP64 operations per classified edge pair: four orientation
 # determinants (7 flops each), two subtractions and two divisions
 FLOPS_PER_EVAL = 32
+# per edge pair the count kernel visits: the eight FP64 compares of the
+# box-reject test (a lower bound: pairs it does not reject cost more)
+FLOPS_PER_TEST = 8
 # the dominant kernel, timed live with HIP events inside the timed region
 ROOF_KERNEL = "gh_count"
 BREAKDOWN_STEPS = 3
@@ -51,16 +54,21 @@ def traffic_bytes(workload, args):
     return rec["bytes_per_launch"]
 
 
-def roofline_line(total_ms, launches, n_eval, traffic):
-    """FP64 work of the count kernel: 32 flops per classified edge pair
-    (four 7-flop orientation determinants, two subtractions, two divisions)."""
+def roofline_line(total_ms, launches, cs, traffic):
+    """FP64 work of the count kernel: 8 compares per edge pair it visits
+    (the box-reject test) plus 32 flops per edge pair it classifies (four
+    7-flop orientation determinants, two subtractions, two divisions).  The
+    kernel has no matrix-shaped work: the peak is MI355X's FP64 rate, which
+    its vector units share with the matrix cores."""
+    n_eval, n_tested = cs["n_eval"], cs["n_tested"]
     avg_ms = total_ms / max(launches, 1)
-    flops = FLOPS_PER_EVAL * n_eval
+    flops = FLOPS_PER_EVAL * n_eval + FLOPS_PER_TEST * n_tested
     achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     return {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS,
             "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFS, 6), "traffic": traffic,
             "kernel": "k_count_cpu", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
-            "work": "%d classified edge pairs x %d FP64 flops" % (n_eval, FLOPS_PER_EVAL)}
+            "work": "%d edge pairs tested x %d FP64 compares + %d classified x %d FP64 flops"
+                    % (n_tested, FLOPS_PER_TEST, n_eval, FLOPS_PER_EVAL)}
 
 
 def cpu_baseline(base, over, joined, sample_pairs):
@@ -183,17 +191,17 @@ def main():
     ctx.set_timing(False)
 
     kernels = ["mbr_endpoints", "sort_hist", "sort_scatter", "scan_tile", "scan_reduce",
-               "scan_downsweep", "mbr_sweep_prep", "mbr_sweep_count", "mbr_sweep_write",
-               "get_cmbr", "pip", "cmf_count", "cmf_apply", "segment_join", "gh_pair_setup",
-               "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own", "gh_calc", "gh_sort_p",
-               "gh_sort_runs", "gh_sort_q", "gh_init_label", "gh_cpu_fill", "gh_cpu_sort",
-               "gh_cpu_link", "gh_label_trace", "gh_gather_out",
-               "fill_i32"]
+               "scan_downsweep", "scan_lookback", "compact_lookback", "mbr_sweep_prep",
+               "mbr_sweep_count", "mbr_sweep_write", "get_cmbr", "pip", "cmf_count", "cmf_apply",
+               "segment_join", "gh_pair_setup", "gh_pair_fill", "gh_count", "gh_transpose",
+               "gh_nmap", "gh_calc_own", "gh_calc", "gh_sort_p", "gh_sort_runs", "gh_sort_q",
+               "gh_init_label", "gh_cpu_fill", "gh_cpu_sort", "gh_cpu_link", "gh_label_trace",
+               "gh_gather_out", "fill_i32"]
     times = {k: ctx.timer(k) for k in kernels}
     ms_step = 1e3 * elapsed / args.steps
     pairs = cs["pairs"]
     value = pairs / (elapsed / args.steps)
-    roofline = roofline_line(roof_ms, roof_n, cs["n_eval"], traffic_bytes("join-clip", args))
+    roofline = roofline_line(roof_ms, roof_n, cs, traffic_bytes("join-clip", args))
     line = {"metric": METRIC, "value": round(value, 2), "unit": "pairs/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -268,7 +276,8 @@ def batched(args):
     ctx.synchronize()
     ctx.set_timing(False)
     kernel_ms = {k: round(ctx.timer(k)[0], 4) for k in (
-        "gh_pair_setup", "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own", "gh_calc",
+        "gh_pair_setup", "gh_pair_fill", "gh_count", "gh_transpose", "gh_nmap", "gh_calc_own",
+        "gh_calc",
         "gh_sort_p", "gh_sort_runs", "gh_sort_q", "gh_init_label", "gh_cpu_fill",
         "gh_cpu_sort", "gh_cpu_link", "gh_label_trace", "gh_gather_out", "sort_hist",
         "sort_scatter")}
@@ -295,8 +304,8 @@ def batched(args):
     per_pair_rate = len(sample) / t_pp
     print("per-pair path: %d pairs in %.2f s, %d mismatches" % (len(sample), t_pp, mism),
           file=sys.stderr, flush=True)
-    kernels = ["gh_pair_setup", "gh_pair_fill", "gh_count", "gh_nmap", "gh_calc_own",
-               "gh_calc", "gh_sort_p", "gh_sort_runs", "gh_sort_q", "gh_init_label",
+    kernels = ["gh_pair_setup", "gh_pair_fill", "gh_count", "gh_transpose", "gh_nmap",
+               "gh_calc_own", "gh_calc", "gh_sort_p", "gh_sort_runs", "gh_sort_q", "gh_init_label",
                "gh_cpu_fill", "gh_cpu_sort", "gh_cpu_link",
                "gh_label_trace", "gh_gather_out", "sort_hist", "sort_scatter"]
     line = {"metric": METRIC, "value": round(args.pairs / (elapsed / args.steps), 2),
@@ -320,7 +329,7 @@ def batched(args):
                               "result_mismatches_vs_batch": mism,
                               "batched_speedup": round(args.pairs / (elapsed / args.steps)
                                                        / per_pair_rate, 2)},
-            "roofline": roofline_line(roof_ms, roof_n, cs["n_eval"],
+            "roofline": roofline_line(roof_ms, roof_n, cs,
                                       traffic_bytes("batched", args))}
     d_pairs.free()
     dP.free()
@@ -364,7 +373,8 @@ def large(args):
     ctx.synchronize()
     ctx.set_timing(False)
     kernel_ms = {k: round(ctx.timer(k)[0], 4) for k in (
-        "gh_pair_setup", "gh_pair_fill", "gh_count", "gh_cpu_fill", "gh_cpu_sort",
+        "gh_pair_setup", "gh_pair_fill", "gh_count", "gh_transpose", "gh_cpu_fill",
+        "gh_cpu_sort",
         "gh_sort_runs", "gh_cpu_link", "gh_label_trace", "gh_gather_out", "sort_hist",
         "sort_scatter", "scan_reduce", "scan_downsweep")}
     path = int(ctx.clip_get("label_path", 1)[0])
@@ -387,7 +397,7 @@ def large(args):
             "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 3),
             "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 3),
             "kernel_ms_per_step": kernel_ms,
-            "roofline": roofline_line(roof_ms, roof_n, cs["n_eval"], None)}
+            "roofline": roofline_line(roof_ms, roof_n, cs, None)}
     db.free()
     do.free()
     ctx.close()

@@ -92,6 +92,7 @@ void fill_clip_stats(gcmf_ctx *ctx, gcmf_clip_stats *st) {
   st->slots_q = ctx->p1.slotsQ;
   st->n_eval = ctx->cr.nEvalP;
   st->n_intersections = ctx->cr.totalIntersections;
+  st->n_tested = ctx->cr.nTested;
   st->ok = ctx->cr.nOk;
   st->invalid = ctx->cr.nInvalid;
   st->disjoint = ctx->cr.nDisjoint;

@@ -82,6 +82,10 @@ struct Context {
   // bound (never in a correct run); read back and cleared by read_scalars
   // and check_lookback
   long long *lb_err = nullptr;
+  // the two halves of the single-pass scans' tile descriptors: each launch
+  // uses one and clears the other for the next launch
+  unsigned long long *lb_desc = nullptr;
+  int lb_half = 0;
 
   hipEvent_t take_event();
   void flush_timers();  // synchronises on pending events and accumulates

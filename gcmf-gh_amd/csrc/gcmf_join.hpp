@@ -82,6 +82,7 @@ struct ClipResult {
   long long *labelTicks = nullptr;
   // phase-1 statistics
   long long nEvalP = 0, totalIntersections = 0;
+  long long nTested = 0;  // edge pairs the count kernel tested (box compare at least)
   long long nOk = 0, nInvalid = 0, nDisjoint = 0, nFailed = 0;  // pairs by status
 };
 

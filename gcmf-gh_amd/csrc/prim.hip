@@ -371,6 +371,16 @@ __global__ void __launch_bounds__(256)
 constexpr unsigned long long kLbAgg = 1, kLbIncl = 2;
 constexpr long kLbMaxN = 1L << 20;
 constexpr int kLbSpinLimit = 1 << 22;
+// descriptors per half: the tiles of kLbMaxN elements and the tile counter
+constexpr long kLbSlots = 512;
+static_assert(kLbMaxN / 4096 + 1 <= kLbSlots, "descriptor half too small");
+
+// every launch clears the other half, which the next launch uses
+__device__ __forceinline__ void lb_clear_next(unsigned long long *next) {
+  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < kLbSlots;
+       j += (long)gridDim.x * blockDim.x)
+    next[j] = 0;
+}
 
 __device__ __forceinline__ void lb_publish(unsigned long long *d, long long v,
                                            unsigned long long st) {
@@ -483,17 +493,18 @@ __device__ __forceinline__ long long lb_tile_prefix(unsigned long long *desc, lo
   return *s_pre;
 }
 
-__global__ void k_lb_clear(unsigned long long *desc, long tiles) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i <= tiles) desc[i] = 0;  // desc[tiles] holds the tile counter
+__global__ void k_lb_clear_all(unsigned long long *desc) {
+  for (long j = threadIdx.x; j < 2 * kLbSlots; j += blockDim.x) desc[j] = 0;
 }
 
+// desc[tiles] holds the tile counter
 __global__ void __launch_bounds__(256)
-    k_lb_scan(const int *in, long n, long long *out, unsigned long long *desc, long tiles,
-              long long *err) {
+    k_lb_scan(const int *in, long n, long long *out, unsigned long long *desc,
+              unsigned long long *next, long tiles, long long *err) {
   __shared__ long long lds[4][4];
   __shared__ long long s_pre;
   __shared__ long s_tile;
+  lb_clear_next(next);
   if (threadIdx.x == 0) s_tile = (long)atomicAdd(desc + tiles, 1ULL);
   __syncthreads();
   const long t = s_tile;
@@ -526,10 +537,11 @@ __global__ void __launch_bounds__(256)
 
 __global__ void __launch_bounds__(256)
     k_lb_compact(const int *flag, long n, int *out, long long *count, unsigned long long *desc,
-                 long tiles, long long *err) {
+                 unsigned long long *next, long tiles, long long *err) {
   __shared__ long long lds[4][4];
   __shared__ long long s_pre;
   __shared__ long s_tile;
+  lb_clear_next(next);
   if (threadIdx.x == 0) s_tile = (long)atomicAdd(desc + tiles, 1ULL);
   __syncthreads();
   const long t = s_tile;
@@ -563,15 +575,22 @@ static long long *lb_err_word(Context &c) {
   return c.lb_err;
 }
 
-// the tile descriptors and the tile counter, cleared for one launch
-static unsigned long long *lb_desc(Context &c, long tiles) {
-  unsigned long long *desc =
-      (unsigned long long *)c.ws.get("lookback_desc", sizeof(unsigned long long) * (tiles + 1));
-  hipLaunchKernelGGL(k_lb_clear, dim3(grid_for(tiles + 1, 256)), dim3(256), 0, c.stream, desc,
-                     tiles);
-  HIP_CHECK(hipGetLastError());
-  return desc;
+// the cleared descriptor half for this launch and the half it clears for
+// the next; lb_flip switches them once the launch is queued
+static unsigned long long *lb_halves(Context &c, unsigned long long **next) {
+  if (!c.lb_desc) {
+    unsigned long long *d = (unsigned long long *)c.ws.get(
+        "lookback_desc", sizeof(unsigned long long) * 2 * kLbSlots);
+    hipLaunchKernelGGL(k_lb_clear_all, dim3(1), dim3(256), 0, c.stream, d);
+    HIP_CHECK(hipGetLastError());
+    c.lb_desc = d;
+    c.lb_half = 0;
+  }
+  *next = c.lb_desc + (1 - c.lb_half) * kLbSlots;
+  return c.lb_desc + c.lb_half * kLbSlots;
 }
+
+static void lb_flip(Context &c) { c.lb_half = 1 - c.lb_half; }
 
 static bool lb_fits(const void *a, const void *b, long n) {
   return n > 0 && n <= kLbMaxN && !((uintptr_t)a & 15) && !((uintptr_t)b & 15);
@@ -581,11 +600,13 @@ static bool lb_scan(Context &c, const int *in, long long *out, long n) {
   if (!lb_fits(in, out, n)) return false;
   const long tiles = (n + kVecTile - 1) / kVecTile;
   long long *err = lb_err_word(c);
-  unsigned long long *desc = lb_desc(c, tiles);
+  unsigned long long *next;
+  unsigned long long *desc = lb_halves(c, &next);
   LaunchTimer t(c, "scan_lookback");
   hipLaunchKernelGGL(k_lb_scan, dim3((unsigned)tiles), dim3(256), 0, c.stream, in, n, out, desc,
-                     tiles, err);
+                     next, tiles, err);
   HIP_CHECK(hipGetLastError());
+  lb_flip(c);
   return true;
 }
 
@@ -597,11 +618,13 @@ static const long long *lb_compact(Context &c, const int *flag, long n, int *out
   const long tiles = (n + kVecTile - 1) / kVecTile;
   long long *count = (long long *)c.ws.get(t + "_n", sizeof(long long));
   long long *err = lb_err_word(c);
-  unsigned long long *desc = lb_desc(c, tiles);
+  unsigned long long *next;
+  unsigned long long *desc = lb_halves(c, &next);
   LaunchTimer tm(c, "compact_lookback");
   hipLaunchKernelGGL(k_lb_compact, dim3((unsigned)tiles), dim3(256), 0, c.stream, flag, n,
-                     out_idx, count, desc, tiles, err);
+                     out_idx, count, desc, next, tiles, err);
   HIP_CHECK(hipGetLastError());
+  lb_flip(c);
   return count;
 }
 

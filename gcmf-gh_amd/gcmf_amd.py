@@ -179,7 +179,8 @@ class ClipStats(C.Structure):
     _fields_ = [(n, C.c_long) for n in ("pairs", "ok", "invalid", "disjoint", "failed",
                                         "total_comps", "total_verts", "edges_p", "edges_q",
                                         "slots_p", "slots_q")] + \
-               [("n_eval", C.c_longlong), ("n_intersections", C.c_longlong)]
+               [("n_eval", C.c_longlong), ("n_intersections", C.c_longlong),
+                ("n_tested", C.c_longlong)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

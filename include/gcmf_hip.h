@@ -141,6 +141,7 @@ typedef struct {
   long edges_p, edges_q, slots_p, slots_q;
   long long n_eval;          /* edge pairs classified by the count kernel */
   long long n_intersections; /* intersection records over valid pairs */
+  long long n_tested;        /* edge pairs the count kernel visited and box-tested */
 } gcmf_clip_stats;
 
 /* Batched Greiner-Hormann intersection of M pairs (d_pairs = 2M i32 of
