@@ -1859,13 +1859,16 @@ __device__ bool out_copy_ring(OutBuf &O, const Nodes &N, int root) {
 }
 
 // steps 1..6 of labelIntersections, createResult; returns status
-__device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
+// labelsDone: step 1 and step 3 have been applied already (by the group
+// path, which computes the same labels in parallel before it falls back)
+__device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O, bool labelsDone) {
   // step 1 (polyclip.h:343-397, restored) then step 3 (copy labels to Q)
-  {
+  if (!labelsDone) {
     int I = rootP;
     do {
+      const int In = N.next[I];  // loaded with I's fields (no step moves nodes)
       if (isint(N, I) && N.nb[I] >= 0) {
-        int Pm = N.prev[I], Pp = N.next[I], Qm = N.prev[N.nb[I]], Qp = N.next[N.nb[I]];
+        int Pm = N.prev[I], Pp = In, Qm = N.prev[N.nb[I]], Qp = N.next[N.nb[I]];
         int tm = node_rel(N, Qm, Pm, I, Pp), tp = node_rel(N, Qp, Pm, I, Pp);
         int l = N.lab[I];
         if ((tm == 0 && tp == 1) || (tm == 1 && tp == 0)) l = L_CROSSING;
@@ -1877,12 +1880,13 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
         if ((tm == 2 && tp == 0) || (tp == 2 && tm == 0)) l = L_ON_RIGHT;
         N.lab[I] = (signed char)l;
       }
-      I = N.next[I];
+      I = In;
     } while (I != rootP);
     I = rootP;
     do {
+      const int In = N.next[I];
       if (isint(N, I) && N.nb[I] >= 0) N.lab[N.nb[I]] = N.lab[I];
-      I = N.next[I];
+      I = In;
     } while (I != rootP);
   }
   // step 2: intersection chains (:398-451)
@@ -2017,8 +2021,12 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
     do {
       I = N.next[I];
     } while (!isint(N, I) && I != V);
+    // the walk loads each vertex's flags, label and successor together (one
+    // wait per vertex); the stores touch only the current vertex's enex,
+    // lab and fl
+    int lI = N.lab[I], In = N.next[I];
     while (I != V) {
-      int l = N.lab[I];
+      int l = lI;
       if (l == L_CROSSING) {
         N.enex[I] = (signed char)status;
         toggle(status);
@@ -2046,9 +2054,16 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
         }
         toggle(status);
       }
-      do {
-        I = N.next[I];
-      } while (!isint(N, I) && I != V);
+      I = In;
+      unsigned char fI = N.fl[I];
+      lI = N.lab[I];
+      In = N.next[I];
+      while (!(fI & F_INT) && I != V) {
+        I = In;
+        fI = N.fl[I];
+        lI = N.lab[I];
+        In = N.next[I];
+      }
     }
   }
   // step 5: split bouncing vertex pairs (:661-714); step 6 (:722-734)
@@ -2108,7 +2123,8 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
     bool full = false;
     while (have) {
       if (O.ncomp >= O.capComp) return GH_OVERFLOW;
-      int R = -1, V = I;
+      // tail: the component's last vertex (O.prev[R]), kept in a register
+      int R = -1, tail = -1, V = I;
       N.fl[V] &= ~F_INT;
       long long steps = 0;
       // (V, status) after a move fixes the rest of the walk, so a repeated
@@ -2119,14 +2135,33 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
       do {
         int status = N.enex[V];
         toggle(status);
+        // the walk is a chain of dependent loads: every load that depends on
+        // the current vertex alone (its status, its move, its coordinates,
+        // its flags) is issued before the step's stores, so that a step
+        // waits on memory once
+        const int *mv = status == EXIT ? N.next : N.prev;
+        int Vn = mv[V], enV;
         do {
-          V = status == EXIT ? N.next[V] : N.prev[V];
-          N.fl[V] &= ~F_INT;
+          V = Vn;
+          enV = N.enex[V];
+          Vn = mv[V];
+          const pt pv = P_(N, V), ov = ld(N.po, V);
+          const unsigned char fv = N.fl[V];
+          N.fl[V] = (unsigned char)(fv & ~F_INT);
           if (!full) {
-            int w = out_new(O, P_(N, V), ld(N.po, V));
-            if (w < 0) full = true;
-            else if (R < 0) R = w;
-            else out_append(O, R, w);
+            int w = out_new(O, pv, ov);
+            if (w < 0) {
+              full = true;
+            } else if (R < 0) {
+              R = tail = w;
+            } else {
+              // out_append(O, R, w) with O.prev[R] == tail
+              O.prev[w] = tail;
+              O.next[w] = R;
+              O.next[tail] = w;
+              O.prev[R] = w;
+              tail = w;
+            }
           }
           if (++steps > guard) return GH_TRACE_FAIL;
           if (V == savedV && status == savedS) return GH_TRACE_FAIL;
@@ -2137,7 +2172,7 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O) {
             lam = 0;
           }
           lam++;
-        } while (!(N.enex[V] == status) && V != I);
+        } while (!(enV == status) && V != I);
         if (V != I) {
           V = N.nb[V];
           if (V < 0) return GH_TRACE_FAIL;
@@ -2279,7 +2314,7 @@ __global__ void __launch_bounds__(64)
   O.compRoot = a.ocompRoot + a.compOff[k];
   O.ncomp = 0;
   O.capComp = (int)(a.compOff[k + 1] - a.compOff[k]);
-  int rc = label_and_trace(N, 0, (int)cP, O);
+  int rc = label_and_trace(N, 0, (int)cP, O, false);
   if (rc != GH_OK) {
     status[k] = rc;
     return;
@@ -2306,6 +2341,8 @@ __global__ void __launch_bounds__(64)
 }
 
 // ---------------------------------------- wave-parallel labelling path ----
+// nodes per lane whose loads a staged pass issues before its stores
+constexpr int kStage = 4;
 // One wave64 per pair.  The lists are built in parallel; labelling step 1/3
 // runs per vertex; when every intersection vertex is a plain CROSSING (no
 // bouncing, chain or overlap labels, mutual links, a non-intersection vertex
@@ -3032,48 +3069,106 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
   int *rg = aux + 5 * nb0;  // 10 * (nP + nQ) ints of segment scratch
   // -------- build (ghcuda.cpp:239-322 semantics, see k_label_trace): every
   // slot is first taken as an intersection vertex, then the first slot of
-  // each edge is re-flagged as its parent vertex
+  // each edge is re-flagged as its parent vertex.  The loops below (and
+  // step 1) take kStage nodes per lane at a time and issue all their loads
+  // before the first store, so that a pass waits on memory once per
+  // kStage * W nodes rather than once per W.
   const bool xl = a.explicitLinks != 0;
-  for (int j = lane; j < nP; j += W) {
-    const long long sl = sbP + j;
-    st(N.pn, j, ld(s.iP, sl));
-    st(N.po, j, ld(s.oP, sl));
-    N.prev[j] = j == 0 ? nP - 1 : j - 1;
-    N.next[j] = j == nP - 1 ? 0 : j + 1;
-    if (xl) {
-      const long long kq = (long long)s.nbP[sl] - 1 - sbQ;
-      N.nb[j] = (kq >= 0 && kq < cQ) ? nP + (int)kq : -1;
-    } else {
-      N.nb[j] = -1;
+  for (int j0 = lane; j0 < nP; j0 += kStage * W) {
+    pt ip[kStage], op[kStage];
+    int nbv[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const int j = j0 + u * W;
+      if (j < nP) {
+        const long long sl = sbP + j;
+        ip[u] = ld(s.iP, sl);
+        op[u] = ld(s.oP, sl);
+        nbv[u] = xl ? s.nbP[sl] : 0;
+      }
     }
-    N.lab[j] = L_NONE;
-    N.enex[j] = NEITHER;
-    N.fl[j] = F_INT;
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const int j = j0 + u * W;
+      if (j < nP) {
+        st(N.pn, j, ip[u]);
+        st(N.po, j, op[u]);
+        N.prev[j] = j == 0 ? nP - 1 : j - 1;
+        N.next[j] = j == nP - 1 ? 0 : j + 1;
+        if (xl) {
+          const long long kq = (long long)nbv[u] - 1 - sbQ;
+          N.nb[j] = (kq >= 0 && kq < cQ) ? nP + (int)kq : -1;
+        } else {
+          N.nb[j] = -1;
+        }
+        N.lab[j] = L_NONE;
+        N.enex[j] = NEITHER;
+        N.fl[j] = F_INT;
+      }
+    }
   }
-  for (int j = lane; j < nQ; j += W) {
-    const long long sl = sbQ + j;
-    const int v = nP + j;
-    st(N.pn, v, ld(s.iQ, sl));
-    st(N.po, v, ld(s.oQ, sl));
-    N.prev[v] = j == 0 ? nP + nQ - 1 : v - 1;
-    N.next[v] = j == nQ - 1 ? nP : v + 1;
-    N.lab[v] = L_NONE;
-    N.enex[v] = NEITHER;
-    // an unlinked parent has nbQ <= 0, which lands outside [0, cP) as well
-    const long long kp = (long long)s.nbQ[sl] - 1 - sbP;
-    N.nb[v] = (kp >= 0 && kp < cP) ? (int)kp : -1;
-    N.fl[v] = F_INT;
+  for (int j0 = lane; j0 < nQ; j0 += kStage * W) {
+    pt ip[kStage], op[kStage];
+    int nbv[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const int j = j0 + u * W;
+      if (j < nQ) {
+        const long long sl = sbQ + j;
+        ip[u] = ld(s.iQ, sl);
+        op[u] = ld(s.oQ, sl);
+        nbv[u] = s.nbQ[sl];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const int j = j0 + u * W;
+      if (j < nQ) {
+        const int v = nP + j;
+        st(N.pn, v, ip[u]);
+        st(N.po, v, op[u]);
+        N.prev[v] = j == 0 ? nP + nQ - 1 : v - 1;
+        N.next[v] = j == nQ - 1 ? nP : v + 1;
+        N.lab[v] = L_NONE;
+        N.enex[v] = NEITHER;
+        // an unlinked parent has nbQ <= 0, which lands outside [0, cP) as well
+        const long long kp = (long long)nbv[u] - 1 - sbP;
+        N.nb[v] = (kp >= 0 && kp < cP) ? (int)kp : -1;
+        N.fl[v] = F_INT;
+      }
+    }
   }
   __syncthreads();
-  for (long long e = pE0 + lane; e < pE1; e += W) {
-    const long long sl = a.psP2[e];
-    const bool linked = xl ? s.nbP[sl] > 0 : s.aP[sl] != kNone;
-    N.fl[sl - sbP] = (unsigned char)(F_SRC | (linked ? F_INT : 0));
+  for (long long e0 = pE0 + lane; e0 < pE1; e0 += kStage * W) {
+    long long sl[kStage];
+    bool linked[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const long long e = e0 + u * W;
+      sl[u] = e < pE1 ? a.psP2[e] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; u++)
+      linked[u] = sl[u] >= 0 && (xl ? s.nbP[sl[u]] > 0 : s.aP[sl[u]] != kNone);
+#pragma unroll
+    for (int u = 0; u < kStage; u++)
+      if (sl[u] >= 0) N.fl[sl[u] - sbP] = (unsigned char)(F_SRC | (linked[u] ? F_INT : 0));
   }
-  for (long long e = qE0 + lane; e < qE1; e += W) {
-    const long long sl = a.psQ2[e];
-    const bool linked = s.aQ[sl] != kNone || s.nbQ[sl] > 0;
-    N.fl[nP + (sl - sbQ)] = (unsigned char)(F_SRC | (linked ? F_INT : 0));
+  for (long long e0 = qE0 + lane; e0 < qE1; e0 += kStage * W) {
+    long long sl[kStage];
+    bool linked[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const long long e = e0 + u * W;
+      sl[u] = e < qE1 ? a.psQ2[e] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; u++)
+      linked[u] = sl[u] >= 0 && (s.aQ[sl[u]] != kNone || s.nbQ[sl[u]] > 0);
+#pragma unroll
+    for (int u = 0; u < kStage; u++)
+      if (sl[u] >= 0)
+        N.fl[nP + (sl[u] - sbQ)] = (unsigned char)(F_SRC | (linked[u] ? F_INT : 0));
   }
   __syncthreads();
   if (!xl) {
@@ -3086,35 +3181,52 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
     __syncthreads();
   }
   // -------- step 1 (restored) + step 3, one lane per P intersection vertex
+  // The labels are computed for kStage vertices per lane from loads only
+  // and stored afterwards.  The P ring's mutual-link check rides along: a
+  // P intersection vertex I must have nb[nb[I]] == I.
   bool slow = false;
-  for (int I = lane; I < nP; I += W) {
-    bool bad = false;
-    if (isint(N, I)) {
-      const int q = N.nb[I];
-      if (q < 0) {
-        bad = true;
-      } else {
-        const int Pm = I == 0 ? nP - 1 : I - 1, Pp = I == nP - 1 ? 0 : I + 1;
-        const int Qm = N.prev[q], Qp = N.next[q];
-        const int tm = node_rel(N, Qm, Pm, I, Pp), tp = node_rel(N, Qp, Pm, I, Pp);
-        int l = L_NONE;
-        if ((tm == 0 && tp == 1) || (tm == 1 && tp == 0)) l = L_CROSSING;
-        if ((tm == 0 && tp == 0) || (tm == 1 && tp == 1)) l = L_BOUNCING;
-        if ((tp == 3 && tm == 1) || (tm == 3 && tp == 1)) l = L_LEFT_ON;
-        if ((tp == 3 && tm == 0) || (tm == 3 && tp == 0)) l = L_RIGHT_ON;
-        if ((tp == 3 && tm == 2) || (tm == 3 && tp == 2)) l = L_ON_ON;
-        if ((tm == 2 && tp == 1) || (tp == 2 && tm == 1)) l = L_ON_LEFT;
-        if ((tm == 2 && tp == 0) || (tp == 2 && tm == 0)) l = L_ON_RIGHT;
-        N.lab[I] = (signed char)l;
-        N.lab[q] = (signed char)l;
-        bad = l != L_CROSSING && l != L_BOUNCING;
-      }
+  for (int I0 = lane; I0 < nP; I0 += kStage * W) {
+    int q[kStage], l[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const int I = I0 + u * W;
+      q[u] = I < nP && isint(N, I) ? N.nb[I] : -2;
+      l[u] = L_NONE;
     }
-    slow |= bad;
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const int I = I0 + u * W;
+      if (q[u] == -2) continue;
+      if (q[u] < 0) {
+        slow = true;
+        continue;
+      }
+      if (N.nb[q[u]] != I) slow = true;
+      const int Pm = I == 0 ? nP - 1 : I - 1, Pp = I == nP - 1 ? 0 : I + 1;
+      const int Qm = N.prev[q[u]], Qp = N.next[q[u]];
+      const int tm = node_rel(N, Qm, Pm, I, Pp), tp = node_rel(N, Qp, Pm, I, Pp);
+      int lb = L_NONE;
+      if ((tm == 0 && tp == 1) || (tm == 1 && tp == 0)) lb = L_CROSSING;
+      if ((tm == 0 && tp == 0) || (tm == 1 && tp == 1)) lb = L_BOUNCING;
+      if ((tp == 3 && tm == 1) || (tm == 3 && tp == 1)) lb = L_LEFT_ON;
+      if ((tp == 3 && tm == 0) || (tm == 3 && tp == 0)) lb = L_RIGHT_ON;
+      if ((tp == 3 && tm == 2) || (tm == 3 && tp == 2)) lb = L_ON_ON;
+      if ((tm == 2 && tp == 1) || (tp == 2 && tm == 1)) lb = L_ON_LEFT;
+      if ((tm == 2 && tp == 0) || (tp == 2 && tm == 0)) lb = L_ON_RIGHT;
+      l[u] = lb;
+      if (lb != L_CROSSING && lb != L_BOUNCING) slow = true;
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      if (q[u] < 0) continue;
+      N.lab[I0 + u * W] = (signed char)l[u];
+      N.lab[q[u]] = (signed char)l[u];
+    }
   }
   __syncthreads();
-  // links must be mutual on both rings (several P sources may point at one
-  // Q vertex when links are explicit)
+  // links must be mutual on the Q ring too (several P sources may point at
+  // one Q vertex when links are explicit)
+#pragma unroll 4
   for (int j = lane; j < nQ; j += W) {
     const int v = nP + j;
     if (isint(N, v)) {
@@ -3122,8 +3234,6 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
       if (q < 0 || N.nb[q] != v) slow = true;
     }
   }
-  for (int j = lane; j < nP; j += W)
-    if (isint(N, j) && (N.nb[j] < 0 || N.nb[N.nb[j]] != j)) slow = true;
   slow = grp_any<W>(slow);
   // -------- output regions
   OutBuf O;
@@ -3197,11 +3307,28 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
     }
   }
   if (slow) {
-    // sequential path for degenerate pairs (labels are recomputed there)
+    // sequential path for degenerate pairs, from step 2 on.  Step 1's P
+    // labels above are the sequential step 1's (the same function of the
+    // same nodes, the P ring still in index order).  Its step 3 copies each
+    // P intersection vertex's label to its neighbour in ring order, so a Q
+    // vertex linked from several P vertices ends with the last one's label:
+    // the largest such P index, found by an atomic maximum.
+    int *owner = rg;  // nQ entries of the segment scratch (10 (nP + nQ) ints)
+    __syncthreads();
+    for (int j = lane; j < nQ; j += W) owner[j] = -1;
+    __syncthreads();
+    for (int I = lane; I < nP; I += W) {
+      const int q = isint(N, I) ? N.nb[I] : -1;
+      if (q >= nP && q < nP + nQ) atomicMax(&owner[q - nP], I);
+    }
+    __syncthreads();
+    for (int j = lane; j < nQ; j += W)
+      if (owner[j] >= 0) N.lab[nP + j] = N.lab[owner[j]];
+    __syncthreads();
     if (lane != 0) return LP_SEQUENTIAL;
     O.n = 0;
     O.ncomp = 0;
-    int rc = label_and_trace(N, 0, nP, O);
+    int rc = label_and_trace(N, 0, nP, O, true);
     if (rc != GH_OK) {
       status[k] = rc;
       a.outNComp[k] = 0;
@@ -3243,6 +3370,8 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
 // parallelism of the block is worth more (the 1e6 x 1e5 pair labels in 113
 // ms at 1024 threads, 167 ms at 512).
 constexpr int kBigGroup = 1024;
+// batches of at least this many pairs are labelled largest first
+constexpr long kLptMinPairs = 1024;
 
 __device__ __forceinline__ long long pair_nodes(const long long *offP, const long long *offQ,
                                                 const long long *psP2, const long long *psQ2,
@@ -3253,13 +3382,28 @@ __device__ __forceinline__ long long pair_nodes(const TraceArgs &a, long k) {
   return pair_nodes(a.offP, a.offQ, a.psP2, a.psQ2, k);
 }
 
-// One wave per pair; records the path each pair took and its duration in
-// ticks of the constant-rate wall clock (both optional).
-__global__ void __launch_bounds__(64)
-    k_label_wave(long M, int *status, SlotArrays s, TraceArgs a, int *aux, int *lpath,
-                 long long *lticks) {
-  const long k = blockIdx.x;
+// Largest pairs first (longest-processing-time order): keys
+// 2^31 - 1 - nodes (nodes clamped), values the pair indices, for a stable
+// radix sort on 31 bits.  Blocks are dispatched in index order, so the
+// pairs that take longest start first instead of forming the launch's tail.
+__global__ void k_label_order_keys(long M, const long long *offP, const long long *offQ,
+                                   const long long *psP2, const long long *psQ2, uint64_t *keys,
+                                   int *vals) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= M) return;
+  const long long n = pair_nodes(offP, offQ, psP2, psQ2, k);
+  keys[k] = (uint64_t)(0x7fffffffLL - (n < 0x7fffffffLL ? n : 0x7fffffffLL));
+  vals[k] = (int)k;
+}
+
+// One wave per pair, the pairs taken in `order` (null: index order);
+// records the path each pair took and its duration in ticks of the
+// constant-rate wall clock (both optional).
+__global__ void __launch_bounds__(64)
+    k_label_wave(long M, const int *order, int *status, SlotArrays s, TraceArgs a, int *aux,
+                 int *lpath, long long *lticks) {
+  if ((long)blockIdx.x >= M) return;
+  const long k = order ? order[blockIdx.x] : (long)blockIdx.x;
   if (pair_nodes(a, k) >= a.bigNodes) return;  // k_label_block's
   const long long t0 = wall_clock64();
   const int path = label_pair<64>(k, status, s, a, aux);
@@ -4013,6 +4157,20 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       HIP_CHECK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
       HIP_CHECK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
     }
+    // batches of many pairs are labelled largest first; the sorted order's
+    // first nBig entries are then the block path's pairs, largest first
+    int *order = nullptr;
+    if (M >= kLptMinPairs) {
+      LaunchTimer t(c, "gh_label_order");
+      uint64_t *ok = wsT<uint64_t>(c, "gh_lpt_keys", M), *ok2 = wsT<uint64_t>(c, "gh_lpt_keys2", M);
+      int *ov = wsT<int>(c, "gh_lpt_vals", M);
+      order = wsT<int>(c, "gh_lpt_order", M);
+      hipLaunchKernelGGL(k_label_order_keys, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
+                         (const long long *)offP, (const long long *)offQ,
+                         (const long long *)psP2, (const long long *)psQ2, ok, ov);
+      HIP_CHECK(hipGetLastError());
+      radix_sort_u64(c, ok, ov, ok2, order, M, 31);
+    }
     LaunchTimer t(c, "gh_label_trace");
     // the large pairs on the side stream, concurrently with one wave per
     // remaining pair on the main one (disjoint pairs, disjoint regions)
@@ -4020,14 +4178,15 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       HIP_CHECK(hipEventRecord(c.ev_fork, c.stream));
       HIP_CHECK(hipStreamWaitEvent(c.side, c.ev_fork, 0));
       hipLaunchKernelGGL(k_label_block, dim3((unsigned)(nBig < 128 ? nBig : 128)),
-                         dim3(kBigGroup), 0, c.side, (const int *)bigList,
+                         dim3(kBigGroup), 0, c.side, (const int *)(order ? order : bigList),
                          (const unsigned long long *)bigCnt, pm.status, s, ta, aux,
                          res.labelPath, res.labelTicks);
       HIP_CHECK(hipGetLastError());
       HIP_CHECK(hipEventRecord(c.ev_join, c.side));
     }
-    hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), 0, c.stream, M, pm.status, s,
-                       ta, aux, res.labelPath, res.labelTicks);
+    hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), 0, c.stream, M,
+                       (const int *)order, pm.status, s, ta, aux, res.labelPath,
+                       res.labelTicks);
     HIP_CHECK(hipGetLastError());
     if (nBig > 0) HIP_CHECK(hipStreamWaitEvent(c.stream, c.ev_join, 0));
   }
