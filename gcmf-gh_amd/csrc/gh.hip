@@ -1860,8 +1860,12 @@ __device__ bool out_copy_ring(OutBuf &O, const Nodes &N, int root) {
 
 // steps 1..6 of labelIntersections, createResult; returns status
 // labelsDone: step 1 and step 3 have been applied already (by the group
-// path, which computes the same labels in parallel before it falls back)
-__device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O, bool labelsDone) {
+// path, which computes the same labels in parallel before it falls back).
+// pinHint (or null): {V_P, pip(V_P in Q), V_Q, pip(V_Q in P)}, step 4's
+// point-in-polygon results for the vertices it names, computed in parallel;
+// step 4 uses one only when its own start vertex is the one named.
+__device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O, bool labelsDone,
+                               const int *pinHint) {
   // step 1 (polyclip.h:343-397, restored) then step 3 (copy labels to Q)
   if (!labelsDone) {
     int I = rootP;
@@ -2015,7 +2019,9 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O, bool l
       if (V < 0) return GH_TRACE_FAIL;
     }
     int status = ENTRY;
-    if (node_pip(N, Qr, P_(N, V))) toggle(status);
+    const bool pin = (pinHint && pinHint[2 * i] == V) ? pinHint[2 * i + 1] != 0
+                                                      : node_pip(N, Qr, P_(N, V));
+    if (pin) toggle(status);
     bool first = true;
     int I = V;
     do {
@@ -2314,7 +2320,7 @@ __global__ void __launch_bounds__(64)
   O.compRoot = a.ocompRoot + a.compOff[k];
   O.ncomp = 0;
   O.capComp = (int)(a.compOff[k + 1] - a.compOff[k]);
-  int rc = label_and_trace(N, 0, (int)cP, O, false);
+  int rc = label_and_trace(N, 0, (int)cP, O, false, nullptr);
   if (rc != GH_OK) {
     status[k] = rc;
     return;
@@ -3325,10 +3331,23 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
     for (int j = lane; j < nQ; j += W)
       if (owner[j] >= 0) N.lab[nP + j] = N.lab[owner[j]];
     __syncthreads();
+    // step 4's point-in-polygon tests of each ring's first non-intersection
+    // vertex (the rings are still in index order; the flags do not change
+    // before step 4), in parallel
+    int pinHint[4] = {-1, 0, -1, 0};
+    {
+      const int vp = grp_first_nonint<W>(N, 0, nP), vq = grp_first_nonint<W>(N, nP, nQ);
+      if (vp >= 0 && vq >= 0) {
+        pinHint[0] = vp;
+        pinHint[1] = grp_pip<W>(N, nP, nQ, P_(N, vp)) ? 1 : 0;
+        pinHint[2] = vq;
+        pinHint[3] = grp_pip<W>(N, 0, nP, P_(N, vq)) ? 1 : 0;
+      }
+    }
     if (lane != 0) return LP_SEQUENTIAL;
     O.n = 0;
     O.ncomp = 0;
-    int rc = label_and_trace(N, 0, nP, O, true);
+    int rc = label_and_trace(N, 0, nP, O, true, pinHint);
     if (rc != GH_OK) {
       status[k] = rc;
       a.outNComp[k] = 0;
