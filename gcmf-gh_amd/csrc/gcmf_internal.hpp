@@ -58,7 +58,7 @@ struct KernelTimer {
 struct Tuning {
   long cullMin = 512;       // rings longer than this are walked through run boxes
   long long bigNodes = 8192;  // pairs with this many nodes are labelled by a block
-  int countBlock = 256;     // threads per block of the count kernel (64, 128, 256)
+  int countBlock = 128;     // threads per block of the count kernel (64, 128, 256)
 };
 
 struct Context {

@@ -2349,6 +2349,7 @@ __global__ void __launch_bounds__(64)
 // ---------------------------------------- wave-parallel labelling path ----
 // nodes per lane whose loads a staged pass issues before its stores
 constexpr int kStage = 4;
+constexpr int kRingStage = 2;  // the same for the ring writes of segment_trace
 // One wave64 per pair.  The lists are built in parallel; labelling step 1/3
 // runs per vertex; when every intersection vertex is a plain CROSSING (no
 // bouncing, chain or overlap labels, mutual links, a non-intersection vertex
@@ -2779,34 +2780,75 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   // and on the backward segment of the departure after it; of these two,
   // which are reverses of each other, at most one is traced, except at a
   // split vertex, where the two arrive at its two copies
+  // (kRingStage chunks of W vertices at a time: each level of loads, the
+  // vertices' multiplicities, then their two departures, then those
+  // departures' statuses, components and the vertices' coordinates, is
+  // issued for all the chunks before it is used; the stores come last)
   int *cw = rk;  // output position -> component
   for (int ring = 0; ring < 2; ring++) {
     const int rb = ring ? nP : 0, nR = ring ? nQ : nP, base = ring ? XP : 0;
     const int XR = ring ? XQ : XP;
     carry = 0;
-    for (int b = 0; b < nR; b += W) {
-      const int j = b + lane, u = rb + j;
-      const int mu = j < nR ? dmult(N, u) : 0;
-      int tot;
-      const int pre = grp_scan<W>(mu, tot);
-      if (j < nR) {
-        const int r = carry + pre, ri = r + mu;
-        const int df = base + (r == 0 ? XR - 1 : r - 1), db = base + (ri == XR ? 0 : ri);
+    for (int b0 = 0; b0 < nR; b0 += kRingStage * W) {
+      int mu[kRingStage], r[kRingStage];
+#pragma unroll
+      for (int t = 0; t < kRingStage; t++) {
+        const int j = b0 + t * W + lane;
+        mu[t] = j < nR ? dmult(N, rb + j) : 0;
+      }
+#pragma unroll
+      for (int t = 0; t < kRingStage; t++) {
+        int tot;
+        r[t] = carry + grp_scan<W>(mu[t], tot);
+        carry += tot;
+      }
+      // the departures before and after each vertex: [t][0] forward, [t][1]
+      // backward
+      int dd[kRingStage][2], of[kRingStage][2], cd[kRingStage][2];
+#pragma unroll
+      for (int t = 0; t < kRingStage; t++) {
+        const int j = b0 + t * W + lane, ri = r[t] + mu[t];
+        dd[t][0] = base + (r[t] == 0 ? XR - 1 : r[t] - 1);
+        dd[t][1] = base + (ri == XR ? 0 : ri);
+#pragma unroll
         for (int side = 0; side < 2; side++) {
-          const int dd = side ? db : df;
-          if (off[dd] < 0 || dep_enex(N, dn[dd]) != (side ? EXIT : ENTRY)) continue;
-          const int ud = dn[dd] >> 1;
+          of[t][side] = j < nR ? off[dd[t][side]] : -1;
+          cd[t][side] = j < nR ? dn[dd[t][side]] : 0;
+        }
+      }
+      bool on[kRingStage][2];
+      int ci[kRingStage][2];
+      pt pu[kRingStage], ou[kRingStage];
+#pragma unroll
+      for (int t = 0; t < kRingStage; t++) {
+        const int j = b0 + t * W + lane;
+        if (j < nR) {
+          pu[t] = P_(N, rb + j);
+          ou[t] = ld(N.po, rb + j);
+        }
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+          on[t][side] = of[t][side] >= 0 && dep_enex(N, cd[t][side]) == (side ? EXIT : ENTRY);
+          ci[t][side] = on[t][side] ? ciD[dd[t][side]] : 0;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < kRingStage; t++) {
+        const int u = rb + b0 + t * W + lane;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+          if (!on[t][side]) continue;
+          const int ud = cd[t][side] >> 1;
           int dist = side ? ud - u : u - ud;
           if (dist <= 0) dist += nR;
-          const int w = off[dd] + dist - 1;
+          const int w = of[t][side] + dist - 1;
           if (w >= 0 && w < O.n) {
-            st(O.pn, w, P_(N, u));
-            st(xy, w, ld(N.po, u));
-            cw[w] = ciD[dd];
+            st(O.pn, w, pu[t]);
+            st(xy, w, ou[t]);
+            cw[w] = ci[t][side];
           }
         }
       }
-      carry += tot;
     }
   }
   __syncthreads();

@@ -156,7 +156,7 @@ typedef struct {
   long cull_min;       /* rings longer than this are walked through run boxes (512) */
   long long big_nodes; /* pairs with this many nodes are labelled by a
                           1024-thread block instead of one wave (8192) */
-  int count_block;     /* threads per block of the count kernel: 64, 128, 256 (256) */
+  int count_block;     /* threads per block of the count kernel: 64, 128, 256 (128) */
 } gcmf_tuning;
 int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out);
 int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t);

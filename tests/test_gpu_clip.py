@@ -316,6 +316,9 @@ def test_clip_uneven_batch(gctx):
     dP.free()
     dQ.free()
     assert st["failed"] == 0 and st["invalid"] == 0
+    # every classified edge pair was visited first; a batch of 3000 pairs is
+    # labelled largest first
+    assert st["n_tested"] >= st["n_eval"] > 0
     for k in range(0, 3000, 37):
         a, b = oriented(P.ring(k), Q.ring(k))
         s_ref, comps, _ = O.clip_cpu(a, b, 1)
