@@ -1897,6 +1897,7 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O, bool l
   {
     int it = rootP;
     do {
+      const int itn = N.next[it];  // the chain below changes labels only
       if (isint(N, it) && (N.lab[it] == L_LEFT_ON || N.lab[it] == L_RIGHT_ON)) {
         int I = it;
         int x = N.lab[I] == L_LEFT_ON ? 0 : 1;
@@ -1913,7 +1914,7 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O, bool l
         if (N.nb[X] >= 0) N.lab[N.nb[X]] = ct;
         if (N.nb[I] >= 0) N.lab[N.nb[I]] = ct;
       }
-      it = N.next[it];
+      it = itn;
     } while (it != rootP);
   }
   // step 3.5: components without crossings (:474-543)
@@ -2105,11 +2106,12 @@ __device__ int label_and_trace(Nodes &N, int rootP, int rootQ, OutBuf &O, bool l
     (void)nNodes;
     I = rootP;
     do {
+      const int In = N.next[I];  // labels only change below
       if ((N.fl[I] & F_CROSS) && N.nb[I] >= 0 && (N.fl[N.nb[I]] & F_CROSS)) {
         N.lab[I] = L_CROSSING;
         N.lab[N.nb[I]] = L_CROSSING;
       }
-      I = N.next[I];
+      I = In;
     } while (I != rootP);
   }
   // createResult (:743-783)
