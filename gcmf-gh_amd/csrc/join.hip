@@ -344,6 +344,8 @@ __device__ __forceinline__ bool seg_hit(double bX1, double bY1, double bX2, doub
   return true;
 }
 
+constexpr int kSegStage = 4;
+
 // counts come from bCount/oCount, or (when null) from the offsets; the flag
 // goes to segFlag[k] or, with flagByList, to segFlag[w] (the reference's
 // per-survivor layout).
@@ -368,30 +370,59 @@ __global__ void __launch_bounds__(256)
   const double *bc = bCoords + 2 * bBase, *oc = oCoords + 2 * oBase;
   const long long total = (long long)bE * oE;
   int flag = 0;
-  for (long long t0 = 0; t0 < total && !flag; t0 += 64) {
-    long long t = t0 + lane;
+  // edge pair t's four vertices; false when the base edge is degenerate
+  auto pair_at = [&](long long t, double *v) {
+    const int bi = be[t / oE], oi = oe[t % oE];
+    const int b2 = bi < bV - 1 ? bi + 1 : 0, o2 = oi < oV - 1 ? oi + 1 : 0;
+    v[0] = bc[2 * bi], v[1] = bc[2 * bi + 1], v[2] = bc[2 * b2], v[3] = bc[2 * b2 + 1];
+    v[4] = oc[2 * oi], v[5] = oc[2 * oi + 1], v[6] = oc[2 * o2], v[7] = oc[2 * o2 + 1];
+    return !(v[0] == v[2] && v[1] == v[3]);
+  };
+  if (total >= 0x7fffffffLL) {
+    // lists of 2^31 pairs or more (rings of tens of thousands of listed
+    // edges): one pair per lane a round, 64-bit indices
+    for (long long t0 = 0; t0 < total && !flag; t0 += 64) {
+      const long long t = t0 + lane;
+      double v[8];
+      const bool hit = t < total && pair_at(t, v) &&
+                       seg_hit(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
+      flag = __ballot(hit) != 0ULL;
+    }
+    if (lane == 0) segFlag[flagByList ? w : k] = flag;
+    return;
+  }
+  // kSegStage edge pairs per lane a round, all their loads issued before
+  // the tests; the flag is "some pair hits", which testing pairs past the
+  // first hit cannot change
+  const unsigned tot = (unsigned)total, uoE = (unsigned)(oE > 0 ? oE : 1);
+  for (unsigned t0 = 0; t0 < tot && !flag; t0 += 64 * kSegStage) {
+    int bi[kSegStage], oi[kSegStage];
+#pragma unroll
+    for (int u = 0; u < kSegStage; u++) {
+      const unsigned t = t0 + 64 * u + lane;
+      bi[u] = t < tot ? (int)be[t / uoE] : -1;
+      oi[u] = t < tot ? (int)oe[t % uoE] : -1;
+    }
+    double s[kSegStage][8];
+#pragma unroll
+    for (int u = 0; u < kSegStage; u++) {
+      if (bi[u] < 0) continue;
+      const int b2 = bi[u] < bV - 1 ? bi[u] + 1 : 0, o2 = oi[u] < oV - 1 ? oi[u] + 1 : 0;
+      s[u][0] = bc[2 * bi[u]];
+      s[u][1] = bc[2 * bi[u] + 1];
+      s[u][2] = bc[2 * b2];
+      s[u][3] = bc[2 * b2 + 1];
+      s[u][4] = oc[2 * oi[u]];
+      s[u][5] = oc[2 * oi[u] + 1];
+      s[u][6] = oc[2 * o2];
+      s[u][7] = oc[2 * o2 + 1];
+    }
     bool hit = false;
-    if (t < total) {
-      int bi = be[t / oE], oi = oe[t % oE];
-      double bX1 = bc[2 * bi], bY1 = bc[2 * bi + 1], bX2, bY2;
-      if (bi < bV - 1) {
-        bX2 = bc[2 * (bi + 1)];
-        bY2 = bc[2 * (bi + 1) + 1];
-      } else {
-        bX2 = bc[0];
-        bY2 = bc[1];
-      }
-      if (!(bX1 == bX2 && bY1 == bY2)) {
-        double oX1 = oc[2 * oi], oY1 = oc[2 * oi + 1], oX2, oY2;
-        if (oi < oV - 1) {
-          oX2 = oc[2 * (oi + 1)];
-          oY2 = oc[2 * (oi + 1) + 1];
-        } else {
-          oX2 = oc[0];
-          oY2 = oc[1];
-        }
-        hit = seg_hit(bX1, bY1, bX2, bY2, oX1, oY1, oX2, oY2);
-      }
+#pragma unroll
+    for (int u = 0; u < kSegStage; u++) {
+      if (bi[u] < 0) continue;
+      if (!(s[u][0] == s[u][2] && s[u][1] == s[u][3]))
+        hit |= seg_hit(s[u][0], s[u][1], s[u][2], s[u][3], s[u][4], s[u][5], s[u][6], s[u][7]);
     }
     flag = __ballot(hit) != 0ULL;
   }
