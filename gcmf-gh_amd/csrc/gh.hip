@@ -393,12 +393,30 @@ __device__ __forceinline__ long long culled_edges(const double *V, long long e0,
 constexpr int kPartners = 16;
 
 // f(q, B_q, B_q+1) for the listed edges lst[0, c) of ring V (n vertices)
+constexpr int kListStage = 4;
 template <class F>
 __device__ __forceinline__ void listed_edges(const double *V, int n, const int *lst, int c,
                                              F &&f) {
-  for (int i = 0; i < c; i++) {
-    const int q = lst[i];
-    f(q, ld(V, q), ld(V, q + 1 == n ? 0 : q + 1));
+  // kListStage entries a round: their indices, then their vertices, are
+  // loaded before the round's visits (which write neither V nor lst), so a
+  // round waits on memory twice rather than twice per entry; the visits
+  // keep the list's order
+  constexpr int U = kListStage;
+  for (int i0 = 0; i0 < c; i0 += U) {
+    int q[U];
+    pt b1[U], b2[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) q[u] = i0 + u < c ? lst[i0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (i0 + u < c) {
+        b1[u] = ld(V, q[u]);
+        b2[u] = ld(V, q[u] + 1 == n ? 0 : q[u] + 1);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (i0 + u < c) f(q[u], b1[u], b2[u]);
   }
 }
 
