@@ -1490,17 +1490,19 @@ __global__ void __launch_bounds__(256)
   const long long qBase = offQ[k], qN = offQ[k + 1] - qBase;
   const int pid = (int)(g - pBase);
   const pt A1 = ld(ringP, g), A2 = ld(ringP, pid == pN - 1 ? pBase : g + 1);
-  for (long long j = psP1[g]; j < psP1[g + 1]; j++) {
-    const int q = lstAllP[j];
+  const long long j0 = psP1[g];
+  long long j = j0;
+  auto visit = [&](int q, pt B1, pt B2) {
     const long long qg = qBase + q;
-    const pt B1 = ld(ringQ, qg), B2 = ld(ringQ, q + 1 == qN ? qBase : qg + 1);
     double al, be;
     const int t = itype(A1, A2, B1, B2, al, be);
     atomicAdd(&cnt1Q[qg], 1);
     if (ins_on_q(t)) atomicAdd(&cnt2Q[qg], 1);
     keys[j] = (uint64_t)qg;
     vals[j] = pid;
-  }
+    j++;
+  };
+  listed_edges(ringQ + 2 * qBase, (int)qN, lstAllP + j0, (int)(psP1[g + 1] - j0), visit);
 }
 
 // The Q edges' prefilter flags of the CPU route (the flag the count kernel
@@ -1747,11 +1749,32 @@ __global__ void __launch_bounds__(256)
     return u >= 0 ? (long long)dB[u] : -1;
   };
   const long long base = ps2A[g], hi = ps2A[g + 1];
-  for (long long d = base + 1; d < hi; d++) {
-    const int rec = rA[d], q = rec >> 4, t = rec & 15;
-    const long long nb = t == 1 ? counterpart(q) : ps2B[bBase + q];
-    if (nb < 0) bad = true;
-    else nbA[d] = (int)(nb + 1);
+  // kListStage slots a round: their records, then the partner edges' slot
+  // ranges, are loaded before the round's searches and stores (which write
+  // only nbA)
+  for (long long d0 = base + 1; d0 < hi; d0 += kListStage) {
+    int rec[kListStage];
+    long long lo[kListStage], up[kListStage];
+#pragma unroll
+    for (int u = 0; u < kListStage; u++) rec[u] = d0 + u < hi ? rA[d0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < kListStage; u++) {
+      if (d0 + u >= hi) continue;
+      const long long qg = bBase + (rec[u] >> 4);
+      lo[u] = ps2B[qg];
+      up[u] = (rec[u] & 15) == 1 ? ps2B[qg + 1] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kListStage; u++) {
+      if (d0 + u >= hi) continue;
+      long long nb = lo[u];
+      if ((rec[u] & 15) == 1) {
+        const long long w = find_partner(rB2, lo[u] + 1, up[u], ia);
+        nb = w >= 0 ? (long long)dB[w] : -1;
+      }
+      if (nb < 0) bad = true;
+      else nbA[d0 + u] = (int)(nb + 1);
+    }
   }
   const int src = (sideP ? cs.srcP : cs.srcQ)[g];
   if (src >= 0) {
