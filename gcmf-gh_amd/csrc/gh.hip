@@ -115,12 +115,40 @@ inline int pgrp_block(int W) { return W == 64 ? 256 : W; }
 
 // one group per pair: sizes, swap (|P| >= |Q|, ghcuda.cpp:64-91), ring MBRs
 // (readShapefile.h:304-334), CMBR and disjointness (ghcuda.cpp:94-125)
+// out[0] = max of bm[2b], out[1] = max of bm[2b + 1] over b < n (out zeroed
+// beforehand); a grid of up to 64 blocks, one atomic per block and value
+__global__ void __launch_bounds__(256)
+    k_max_block_pairs(const unsigned long long *bm, long n, unsigned long long *out) {
+  __shared__ unsigned long long s0[4], s1[4];
+  unsigned long long a = 0, q = 0;
+  for (long b = (long)blockIdx.x * blockDim.x + threadIdx.x; b < n;
+       b += (long)gridDim.x * blockDim.x) {
+    a = max(a, bm[2 * b]);
+    q = max(q, bm[2 * b + 1]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a = max(a, (unsigned long long)__shfl_xor(a, off, 64));
+    q = max(q, (unsigned long long)__shfl_xor(q, off, 64));
+  }
+  if (__lane_id() == 0) {
+    s0[threadIdx.x >> 6] = a;
+    s1[threadIdx.x >> 6] = q;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = max(max(s0[0], s0[1]), max(s0[2], s0[3]));
+    q = max(max(s1[0], s1[1]), max(s1[2], s1[3]));
+    if (a) atomicMax(out, a);
+    if (q) atomicMax(out + 1, q);
+  }
+}
+
 template <int W>
 __global__ void __launch_bounds__(W == 64 ? 256 : W)
     k_pair_setup(long M, const int *pairs, const double *bCoords, const int *bVNum,
                  const long *bVPSNum, const double *oCoords, const int *oVNum, const long *oVPSNum,
                  int normalize, int noswap, PairMeta pm, int *eP, int *eQ,
-                 unsigned long long *maxRing) {
+                 unsigned long long *blockMax) {
   const long k = pgrp_pair<W>();
   if (W == 64) {
     __shared__ int lmax[2][4];
@@ -135,19 +163,26 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
       }
     }
     __syncthreads();
+    // the block's largest rings, stored plainly (k_max_block_pairs folds
+    // them): an atomic per block on one address serialises across the chip
     if (threadIdx.x == 0) {
       int a = 0, q = 0;
       for (int w = 0; w < 4; w++) {
         a = max(a, lmax[0][w]);
         q = max(q, lmax[1][w]);
       }
-      if (a) atomicMax(maxRing, (unsigned long long)a);
-      if (q) atomicMax(maxRing + 1, (unsigned long long)q);
+      blockMax[2 * blockIdx.x] = (unsigned long long)a;
+      blockMax[2 * blockIdx.x + 1] = (unsigned long long)q;
     }
-  } else if (threadIdx.x == 0 && k < M) {
-    const int nb = bVNum[pairs[2 * k]] - 1, no = oVNum[pairs[2 * k + 1]] - 1;
-    if (max(nb, no) > 0) atomicMax(maxRing, (unsigned long long)max(nb, no));
-    if (min(nb, no) > 0) atomicMax(maxRing + 1, (unsigned long long)min(nb, no));
+  } else if (threadIdx.x == 0) {
+    int a = 0, q = 0;
+    if (k < M) {
+      const int nb = bVNum[pairs[2 * k]] - 1, no = oVNum[pairs[2 * k + 1]] - 1;
+      a = max(max(nb, no), 0);
+      q = max(min(nb, no), 0);
+    }
+    blockMax[2 * blockIdx.x] = (unsigned long long)a;
+    blockMax[2 * blockIdx.x + 1] = (unsigned long long)q;
   }
   if (k >= M) return;
   const int lane = pgrp_lane<W>();
@@ -3625,11 +3660,19 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   unsigned long long *maxRing = wsT<unsigned long long>(c, "gh_maxRing", 2);
   fill_i32(c, (int *)maxRing, 0, 4);
   {
-    LaunchTimer t(c, "gh_pair_setup");
-    hipLaunchKernelGGL(pw == 64 ? k_pair_setup<64> : k_pair_setup<1024>, dim3(pgrp_grid(M, pw)),
-                       dim3(pgrp_block(pw)), 0, c.stream, M, pairs,
-                       in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum, in.oVPSNum,
-                       opt.normalize, opt.noswap, pm, eP, eQ, maxRing);
+    const long nb = pgrp_grid(M, pw);
+    unsigned long long *blockMax = wsT<unsigned long long>(c, "gh_setup_bmax", 2 * nb);
+    {
+      LaunchTimer t(c, "gh_pair_setup");
+      hipLaunchKernelGGL(pw == 64 ? k_pair_setup<64> : k_pair_setup<1024>, dim3((unsigned)nb),
+                         dim3(pgrp_block(pw)), 0, c.stream, M, pairs, in.bCoords, in.bVNum,
+                         in.bVPSNum, in.oCoords, in.oVNum, in.oVPSNum, opt.normalize,
+                         opt.noswap, pm, eP, eQ, blockMax);
+      HIP_CHECK(hipGetLastError());
+    }
+    const long rblocks = std::min<long>(64, (nb + 255) / 256);
+    hipLaunchKernelGGL(k_max_block_pairs, dim3((unsigned)rblocks), dim3(256), 0, c.stream,
+                       (const unsigned long long *)blockMax, nb, maxRing);
     HIP_CHECK(hipGetLastError());
   }
   scan_i32_to_i64(c, eP, offP, M);
