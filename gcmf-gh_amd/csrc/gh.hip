@@ -665,13 +665,23 @@ __global__ void k_pair_check(long M, const long long *offP, const long long *off
                              const long long *psP1, const long long *psQ1,
                              const long long *psP2, const long long *psQ2, int *status,
                              unsigned long long *nInter) {
-  long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= M) return;
-  if (status[k] != GH_OK) return;
-  long long c1P = psP1[offP[k + 1]] - psP1[offP[k]];
-  long long c1Q = psQ1[offQ[k + 1]] - psQ1[offQ[k]];
-  if (c1P != c1Q) status[k] = GH_INVALID;
-  else atomicAdd(nInter, (unsigned long long)c1P);
+  __shared__ unsigned long long part[4];
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long n = 0;
+  if (k < M && status[k] == GH_OK) {
+    const long long c1P = psP1[offP[k + 1]] - psP1[offP[k]];
+    const long long c1Q = psQ1[offQ[k + 1]] - psQ1[offQ[k]];
+    if (c1P != c1Q) status[k] = GH_INVALID;
+    else n = (unsigned long long)c1P;
+  }
+  // one atomic per block (256 threads)
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  if (__lane_id() == 0) part[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(nInter, t);
+  }
 }
 
 // ----------------------------------------------------- neighbour map ----
@@ -3584,16 +3594,28 @@ __global__ void __launch_bounds__(kBigGroup)
 }
 
 // pairs by final status: OK, invalid, disjoint, failed
-__global__ void k_status_counts(long M, const int *status, unsigned long long *cnt) {
-  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int which = -1;
-  if (k < M) {
+// (a grid of at most kStatBlocks 256-thread blocks striding over the pairs,
+// one atomic per block and status: atomics per wave on one cache line
+// serialise across the chip)
+__global__ void __launch_bounds__(256)
+    k_status_counts(long M, const int *status, unsigned long long *cnt) {
+  __shared__ unsigned long long part[4][4];
+  unsigned long long c[4] = {0, 0, 0, 0};
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < M;
+       k += (long)gridDim.x * blockDim.x) {
     const int s = status[k];
-    which = s == GH_OK ? 0 : s == GH_INVALID ? 1 : s == GH_DISJOINT ? 2 : 3;
+    c[s == GH_OK ? 0 : s == GH_INVALID ? 1 : s == GH_DISJOINT ? 2 : 3]++;
   }
   for (int w = 0; w < 4; w++) {
-    const unsigned long long m = __ballot(which == w);
-    if (__lane_id() == 0 && m) atomicAdd(cnt + w, (unsigned long long)__popcll(m));
+    unsigned long long v = c[w];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (__lane_id() == 0) part[threadIdx.x >> 6][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const unsigned long long v = part[0][threadIdx.x] + part[1][threadIdx.x] +
+                                 part[2][threadIdx.x] + part[3][threadIdx.x];
+    if (v) atomicAdd(cnt + threadIdx.x, v);
   }
 }
 
@@ -4360,8 +4382,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   res.nverts = ta.outNVerts;
   unsigned long long *sc = wsT<unsigned long long>(c, "gh_status_counts", 4);
   fill_i32(c, (int *)sc, 0, 8);
-  hipLaunchKernelGGL(k_status_counts, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
-                     (const int *)pm.status, sc);
+  hipLaunchKernelGGL(k_status_counts, dim3(std::min<unsigned>(kStatBlocks, grid_for(M, 256))),
+                     dim3(256), 0, c.stream, M, (const int *)pm.status, sc);
   HIP_CHECK(hipGetLastError());
   {
     const long long *st = (const long long *)stats, *sl = (const long long *)sc;

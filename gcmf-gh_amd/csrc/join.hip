@@ -41,13 +41,24 @@ __global__ void k_endpoints(long B, long O, const uint64_t *bMBR, const uint64_t
     lo = x1 < x2 ? x1 : x2;
     hi = x1 < x2 ? x2 : x1;
   }
-  // wave min/max, then one atomic per wave
+  // wave min/max, block min/max, then one atomic per block and value
+  // (atomics per wave on one cache line serialise across the chip)
+  __shared__ uint64_t slo[4], shi[4];
   for (int off = 32; off > 0; off >>= 1) {
     uint64_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
     lo = l2 < lo ? l2 : lo;
     hi = h2 > hi ? h2 : hi;
   }
   if (__lane_id() == 0) {
+    slo[threadIdx.x >> 6] = lo;
+    shi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      lo = slo[w] < lo ? slo[w] : lo;
+      hi = shi[w] > hi ? shi[w] : hi;
+    }
     atomicMin(mn, (unsigned long long)lo);
     atomicMax(mx, (unsigned long long)hi);
   }
