@@ -1362,10 +1362,12 @@ __device__ __forceinline__ bool src_on_q(int t) {
   return t == 3 || t == 4 || t == 5 || t == 7 || t == 8;
 }
 
-// Count pass of the CPU route, both views in one launch: count1 = partners
-// with a nonzero type, count2 = 1 + the vertices the pairs add on this
-// edge's ring.  The Q view evaluates intersect(P edge, Q edge) with the same
-// argument order as the P view.
+// Count pass of the CPU route: count1 = partners with a nonzero type,
+// count2 = 1 + the vertices the pairs add on this edge's ring.  Threads
+// [0, edgesP) take P edges, the rest Q edges; the Q view evaluates
+// intersect(P edge, Q edge) with the same argument order as the P view.
+// clip_pairs launches it over P's edges only (edgesQ = 0) and builds the Q
+// side by transposition (k_cpu_transpose).
 template <bool CULL>
 __device__ __forceinline__ void count_edge_cpu(long g, long edgesP, long edgesQ, const int *mapP,
                                                const int *mapQ, const long long *offP,
