@@ -3533,17 +3533,20 @@ __device__ __forceinline__ long long pair_nodes(const TraceArgs &a, long k) {
   return pair_nodes(a.offP, a.offQ, a.psP2, a.psQ2, k);
 }
 
-// Largest pairs first (longest-processing-time order): keys
-// 2^31 - 1 - nodes (nodes clamped), values the pair indices, for a stable
-// radix sort on 31 bits.  Blocks are dispatched in index order, so the
-// pairs that take longest start first instead of forming the launch's tail.
+// Largest pairs first (longest-processing-time order, by power-of-two size
+// class), values the pair indices, for one stable 8-bit radix pass: key
+// 64 * (pair below bigNodes) + 63 - floor(log2 nodes).  The block path's
+// pairs (nodes >= bigNodes) are then exactly the first nBig entries, for
+// any threshold.  Blocks are dispatched in index order, so the pairs that
+// take longest start first instead of forming the launch's tail.
 __global__ void k_label_order_keys(long M, const long long *offP, const long long *offQ,
-                                   const long long *psP2, const long long *psQ2, uint64_t *keys,
-                                   int *vals) {
+                                   const long long *psP2, const long long *psQ2,
+                                   long long bigNodes, uint64_t *keys, int *vals) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= M) return;
   const long long n = pair_nodes(offP, offQ, psP2, psQ2, k);
-  keys[k] = (uint64_t)(0x7fffffffLL - (n < 0x7fffffffLL ? n : 0x7fffffffLL));
+  const int cls = n > 0 ? 63 - __clzll((unsigned long long)n) : 0;
+  keys[k] = (uint64_t)((n >= bigNodes ? 0 : 64) + 63 - cls);
   vals[k] = (int)k;
 }
 
@@ -4338,9 +4341,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       order = wsT<int>(c, "gh_lpt_order", M);
       hipLaunchKernelGGL(k_label_order_keys, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
                          (const long long *)offP, (const long long *)offQ,
-                         (const long long *)psP2, (const long long *)psQ2, ok, ov);
+                         (const long long *)psP2, (const long long *)psQ2,
+                         (long long)c.tune.bigNodes, ok, ov);
       HIP_CHECK(hipGetLastError());
-      radix_sort_u64(c, ok, ov, ok2, order, M, 31);
+      radix_sort_u64(c, ok, ov, ok2, order, M, 8);
     }
     LaunchTimer t(c, "gh_label_trace");
     // the large pairs on the side stream, concurrently with one wave per

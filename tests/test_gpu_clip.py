@@ -431,3 +431,31 @@ def test_clip_autotuner_keeps_results(gctx):
         d_pairs.free()
         dP.free()
         dQ.free()
+
+
+@pytest.mark.parametrize("big_nodes", [3000, 1 << 40])
+def test_clip_block_threshold_keeps_results(gctx, big_nodes):
+    """A batch big enough to be labelled largest first, split between the
+    wave path and the block path at a node threshold that is not a power of
+    two (or with no block path at all): every pair is labelled once, and the
+    results equal the default shapes' bit for bit."""
+    P, Q = G.gen_pair_layers(1500, seed=4, nmax=6000)
+    dP, dQ = G.DeviceLayer(gctx, P), G.DeviceLayer(gctx, Q)
+    pairs = np.stack([np.arange(1500), np.arange(1500)], axis=1).astype(np.int32)
+    default = gctx.tuning()
+    try:
+        st0 = gctx.clip(dP, dQ, pairs)
+        s0, r0 = gctx.clip_results(st0)
+        gctx.set_tuning(default["cull_min"], big_nodes, default["count_block"])
+        st1 = gctx.clip(dP, dQ, pairs)
+        s1, r1 = gctx.clip_results(st1)
+        np.testing.assert_array_equal(s0, s1)
+        assert len(r0) == len(r1)
+        for a, b in zip(r0, r1):
+            assert [len(c) for c in a] == [len(c) for c in b]
+            for x, y in zip(a, b):
+                np.testing.assert_array_equal(x, y)
+    finally:
+        gctx.set_tuning(default["cull_min"], default["big_nodes"], default["count_block"])
+        dP.free()
+        dQ.free()
