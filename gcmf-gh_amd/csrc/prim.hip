@@ -740,7 +740,11 @@ __global__ void k_compact(const int *flag, const long long *pos, long n, int *ou
 
 // Two-pass compaction of large flag arrays, with the vector scan's tile
 // layout: nonzero counts per tile, their scan, then every tile rescans its
-// flags and writes the indices of the nonzero ones.
+// flags and writes the indices of the nonzero ones.  The write pass takes
+// the four groups' counts through ONE block scan, as 16-bit fields of one
+// 64-bit word (a group holds 1024 flags, so no field carries into the next),
+// places the tile's indices in LDS in index order and stores them as one
+// contiguous run.
 __global__ void __launch_bounds__(256) k_vcompact_count(const int *flag, long n, long long *cnt) {
   __shared__ long long lds[4];
   const long long base = (long long)blockIdx.x * kVecTile;
@@ -766,31 +770,44 @@ __global__ void __launch_bounds__(256) k_vcompact_count(const int *flag, long n,
 __global__ void __launch_bounds__(256)
     k_vcompact_write(const int *flag, long n, const long long *offsets, long tiles, int *out) {
   __shared__ long long lds[4];
+  __shared__ int stage[kVecTile];
   const long T = xcd_tile(blockIdx.x, tiles);
   const long long base = (long long)T * kVecTile;
-  long long carry = offsets[T];
+  const long long carry = offsets[T];
   const bool full = base + kVecTile <= n;
+  int4 v[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const long long i = base + k * 1024 + 4 * threadIdx.x;
-    int4 v;
     if (full) {
-      v = ld_i4(flag, base, k);
+      v[k] = ld_i4(flag, base, k);
     } else {
-      v.x = i < n ? flag[i] : 0;
-      v.y = i + 1 < n ? flag[i + 1] : 0;
-      v.z = i + 2 < n ? flag[i + 2] : 0;
-      v.w = i + 3 < n ? flag[i + 3] : 0;
+      const long long i = base + k * 1024 + 4 * threadIdx.x;
+      v[k].x = i < n ? flag[i] : 0;
+      v[k].y = i + 1 < n ? flag[i + 1] : 0;
+      v[k].z = i + 2 < n ? flag[i + 2] : 0;
+      v[k].w = i + 3 < n ? flag[i + 3] : 0;
     }
-    const int f0 = v.x != 0, f1 = v.y != 0, f2 = v.z != 0, f3 = v.w != 0;
-    long long tot;
-    long long p = carry + block_excl_scan<long long>(f0 + f1 + f2 + f3, lds, tot);
-    if (f0) out[p++] = (int)i;
-    if (f1) out[p++] = (int)(i + 1);
-    if (f2) out[p++] = (int)(i + 2);
-    if (f3) out[p++] = (int)(i + 3);
-    carry += tot;
   }
+  long long packed = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    packed |= (long long)((v[k].x != 0) + (v[k].y != 0) + (v[k].z != 0) + (v[k].w != 0))
+              << (16 * k);
+  long long tot;
+  const long long pre = block_excl_scan<long long>(packed, lds, tot);
+  int start = 0;  // LDS position of group k's first index
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    int p = start + (int)((pre >> (16 * k)) & 0xffff);
+    const int i = k * 1024 + 4 * threadIdx.x;  // offset in the tile
+    if (v[k].x != 0) stage[p++] = i;
+    if (v[k].y != 0) stage[p++] = i + 1;
+    if (v[k].z != 0) stage[p++] = i + 2;
+    if (v[k].w != 0) stage[p++] = i + 3;
+    start += (int)((tot >> (16 * k)) & 0xffff);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < start; j += 256) out[carry + j] = (int)base + stage[j];
 }
 
 // the vector path of compaction; returns the device address of the count, or
