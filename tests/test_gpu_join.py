@@ -150,6 +150,34 @@ def test_scan_and_compact_entry_points(gctx, n):
         b.free()
 
 
+@pytest.mark.parametrize("n", [(1 << 20) - 3, (1 << 21) + 4099])
+def test_compact_dense_and_empty_tiles(gctx, n):
+    """Compaction over tiles of 4096 flags that are all set (every group of
+    1024 full), all clear, set only at a tile's first and last flag, or
+    random, with negative flag values counting as set and a partial last
+    tile that is all set; single-pass (up to 2^20 flags) and two-pass forms."""
+    rs = np.random.RandomState(3)
+    tiles = (n + 4095) // 4096
+    flags = np.zeros(n, np.int32)
+    for t in range(tiles):
+        lo, hi = 4096 * t, min(4096 * (t + 1), n)
+        kind = t % 4 if t < tiles - 1 else 0
+        if kind == 0:
+            flags[lo:hi] = rs.choice(np.array([-7, 1, 2**31 - 1], np.int32), size=hi - lo)
+        elif kind == 2:
+            flags[lo] = flags[hi - 1] = -1
+        elif kind == 3:
+            flags[lo:hi] = (rs.rand(hi - lo) < 0.5).astype(np.int32)
+    d_f, d_idx = gctx.upload(flags), gctx.upload(np.zeros(n, np.int32))
+    want = np.nonzero(flags)[0]
+    for _ in range(2):  # the second call reuses the workspace
+        cnt = gctx.compact(d_f, d_idx, n)
+        assert cnt == len(want)
+        np.testing.assert_array_equal(gctx.download(d_idx, np.int32, cnt), want)
+    d_f.free()
+    d_idx.free()
+
+
 def test_scan_extreme_values(gctx):
     """int32 extremes of both signs over 2^22 + 7 elements: the running sum
     leaves the int32 range many times and every tile's carry is 64-bit."""
