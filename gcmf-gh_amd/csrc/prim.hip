@@ -878,21 +878,37 @@ __global__ void __launch_bounds__(kSortThreads)
   __shared__ int h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
-  const long base = (long)blockIdx.x * kSortTile;
-  // every load issued before the first LDS update
+  // tiles in XCD order, so that the neighbouring tiles whose counts share a
+  // cache line of every digit's row are counted under one L2
+  const long T = xcd_tile(blockIdx.x, tiles);
+  const long base = T * kSortTile;
+  // every load issued before the first LDS update; a full tile of 16-byte
+  // aligned keys is read as 16-byte pairs (the counts do not depend on which
+  // thread holds which key)
   uint64_t kk[kSortItems];
+  const bool vec = base + kSortTile <= n && ((uintptr_t)keys & 15) == 0;
+  if (vec) {
+    const ulonglong2 *k2 = (const ulonglong2 *)(keys + base);
 #pragma unroll
-  for (int k = 0; k < kSortItems; k++) {
-    const long i = base + (long)k * kSortThreads + threadIdx.x;
-    kk[k] = i < n ? keys[i] : 0;
+    for (int k = 0; k < kSortItems / 2; k++) {
+      const ulonglong2 v = k2[k * kSortThreads + threadIdx.x];
+      kk[2 * k] = v.x;
+      kk[2 * k + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kSortItems; k++) {
+      const long i = base + (long)k * kSortThreads + threadIdx.x;
+      kk[k] = i < n ? keys[i] : 0;
+    }
   }
 #pragma unroll
   for (int k = 0; k < kSortItems; k++) {
     const long i = base + (long)k * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(int)((kk[k] >> shift) & 255)], 1);
+    if (vec || i < n) atomicAdd(&h[(int)((kk[k] >> shift) & 255)], 1);
   }
   __syncthreads();
-  hist[(long)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+  hist[(long)threadIdx.x * tiles + T] = h[threadIdx.x];
 }
 
 // Stable scatter: wave w ranks its 1024 keys in 16 rounds of 64 in order;
