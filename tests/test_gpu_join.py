@@ -128,6 +128,37 @@ def test_radix_sort_entry_point(gctx, n, bits):
         b.free()
 
 
+class _At:
+    """A device address `off` bytes into a buffer, for entry points that
+    take caller-owned pointers."""
+
+    def __init__(self, buf, off):
+        import ctypes as C
+        self.ptr = C.c_void_p(buf.ptr.value + off)
+
+
+def test_radix_sort_unaligned_buffers(gctx):
+    """Keys and outputs one element past a 16-byte boundary: the histogram
+    reads such keys one by one instead of in 16-byte pairs; the sort still
+    equals the stable argsort."""
+    n, bits = 3 * 4096 + 5, 64
+    rs = np.random.RandomState(9)
+    keys = rs.randint(0, 1 << 62, size=n, dtype=np.int64).astype(np.uint64)
+    keys[: n // 4] = keys[n // 4: 2 * (n // 4)]  # ties
+    vals = np.arange(n, dtype=np.int32)
+    pad_k = np.concatenate([np.zeros(1, np.uint64), keys])
+    pad_v = np.concatenate([np.zeros(1, np.int32), vals])
+    bk, bv = gctx.upload(pad_k), gctx.upload(pad_v)
+    ok, ov = gctx.upload(np.zeros(n + 1, np.uint64)), gctx.upload(np.zeros(n + 1, np.int32))
+    assert bk.ptr.value % 16 == 0 and ok.ptr.value % 16 == 0
+    gctx.radix_sort(_At(bk, 8), _At(bv, 4), _At(ok, 8), _At(ov, 4), n, bits)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(gctx.download(ov, np.int32, n + 1)[1:], order)
+    np.testing.assert_array_equal(gctx.download(ok, np.uint64, n + 1)[1:], keys[order])
+    for b in (bk, bv, ok, ov):
+        b.free()
+
+
 # sizes around the single-pass scan's 4096-element tiles and its 64-tile
 # look-back windows
 @pytest.mark.parametrize("n", [0, 1, 4096, 4097, 64 * 4096 + 5, 3_000_000, 20_000_003])
