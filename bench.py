@@ -88,7 +88,7 @@ def cpu_baseline(base, over, joined, sample_pairs):
         P, Q = base.ring(b), over.ring(o)
         if len(P) < len(Q):
             P, Q = Q, P
-        O.clip_cpu(P, Q, 1)
+        O.clip_cpu(P, Q, 5)
     t_clip = time.perf_counter() - t0
     per_pair = t_clip / len(idx)
     t_step = t_join + per_pair * len(joined)

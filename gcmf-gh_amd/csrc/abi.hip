@@ -8,19 +8,6 @@
 #include "gcmf_internal.hpp"
 #include "gcmf_join.hpp"
 
-struct gcmf_ctx {
-  gcmf::Context c;
-  bool join_valid = false;
-  gcmf::JoinInputs jin;
-  long K = 0;
-  int *pairs = nullptr, *counts = nullptr;
-  gcmf::MbrSortView sv;
-  gcmf::JoinRefine jr;
-  bool clip_valid = false;
-  gcmf::ClipResult cr;
-  gcmf::Phase1View p1;
-};
-
 namespace {
 
 template <typename F>
@@ -103,8 +90,8 @@ void run_clip(gcmf_ctx *ctx, const gcmf::JoinInputs &in, const int *d_pairs, lon
               gcmf_clip_stats *st) {
   ctx->clip_valid = false;
   gcmf::ClipOptions opt;
-  opt.normalize = (flags & GCMF_CLIP_NORMALIZE) ? 1 : 0;
-  if ((flags & GCMF_CLIP_NORMALIZE) && (flags & GCMF_CLIP_SCALE_EDGES)) opt.normalize |= 4;
+  // frame bits: 1 CMBR normalisation, 4 edge scale (unless the unit frame is asked for)
+  opt.normalize = (flags & GCMF_CLIP_NORMALIZE) ? ((flags & GCMF_CLIP_UNIT_FRAME) ? 1 : 5) : 0;
   opt.cleanup = (flags & GCMF_CLIP_NO_CLEANUP) ? 0 : 1;
   opt.label_mode = (flags & GCMF_CLIP_SEQ_LABEL) ? 1 : 0;
   opt.route = (flags & GCMF_CLIP_GPU_ROUTE) ? 0 : 1;
@@ -438,6 +425,14 @@ int gcmf_clip_get(gcmf_ctx *ctx, int field, void *dst, long count) {
     }
     if (count > maxc) throw gcmf::Error{GCMF_ERR_ARG, "count exceeds field size"};
     copy_out(ctx->c, dst, src, elem * (size_t)count);
+  });
+}
+
+int gcmf_clip_last_stats(gcmf_ctx *ctx, gcmf_clip_stats *stats) {
+  return guarded(ctx, [&] {
+    if (!stats) throw gcmf::Error{GCMF_ERR_ARG, "null stats"};
+    if (!ctx->clip_valid) throw gcmf::Error{GCMF_ERR_ARG, "no clip result on this context"};
+    fill_clip_stats(ctx, stats);
   });
 }
 

@@ -107,3 +107,18 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                 const ClipOptions &opt, ClipResult &res, Phase1View *p1);
 
 }  // namespace gcmf
+
+// The state behind one gcmf_ctx handle of the C ABI: the context and the
+// results of its last join and clip, kept on the device for the *_get calls.
+struct gcmf_ctx {
+  gcmf::Context c;
+  bool join_valid = false;
+  gcmf::JoinInputs jin;
+  long K = 0;
+  int *pairs = nullptr, *counts = nullptr;
+  gcmf::MbrSortView sv;
+  gcmf::JoinRefine jr;
+  bool clip_valid = false;
+  gcmf::ClipResult cr;
+  gcmf::Phase1View p1;
+};

@@ -8,40 +8,56 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/gcmf_hip.h"
+#include "../../include/gcmf_host.h"
 #include "gcmf_internal.hpp"
 #include "gcmf_join.hpp"
 
 namespace {
 
 std::mutex g_mu;
-gcmf::Context *g_ctx = nullptr;
+// The process-default context of the reference-named entry points (the
+// reference keeps global state instead, F7); created on first use on the
+// device current at that time.
+gcmf_ctx *g_ctx = nullptr;
 
-gcmf::Context &dctx() {
+gcmf_ctx &dctx() {
   if (!g_ctx) {
-    g_ctx = new gcmf::Context();
     int dev = 0;
-    (void)hipGetDevice(&dev);
-    g_ctx->device = dev;
-    HIP_CHECK(hipStreamCreate(&g_ctx->stream));
+    HIP_CHECK(hipGetDevice(&dev));
+    gcmf_ctx *x = nullptr;
+    const int rc = gcmf_init(dev, &x);
+    if (rc != GCMF_OK) throw gcmf::Error{rc, "cannot create the default context"};
+    g_ctx = x;
   }
-  HIP_CHECK(hipSetDevice(g_ctx->device));
+  HIP_CHECK(hipSetDevice(g_ctx->c.device));
   return *g_ctx;
 }
 
+// Runs f on the default context; library errors (and allocation failures of
+// the host containers) are printed and reported as false, never thrown
+// across the C boundary.
 template <typename F>
 bool shim(const char *name, F &&f) {
   std::lock_guard<std::mutex> lk(g_mu);
+  std::string msg;
   try {
     f(dctx());
     return true;
   } catch (const gcmf::Error &e) {
-    std::fprintf(stderr, "%s: %s\n", name, e.what.c_str());
-    return false;
+    msg = e.what;
+  } catch (const std::exception &e) {
+    msg = e.what();
   }
+  std::fprintf(stderr, "%s: %s\n", name, msg.c_str());
+  if (g_ctx) g_ctx->c.last_error = msg;
+  return false;
 }
 
 template <typename T>
@@ -75,9 +91,24 @@ __global__ void k_eq1(long n, const int *f, int *o) {
   if (i < n) o[i] = f[i] == 1;
 }
 
-// state of the last ghcuda / GH_CUDA call (results stay on the device)
-gcmf::ClipResult g_last_clip;
+// Device arrays freed at scope exit unless `keep` is set (they reached the
+// caller).
+struct OwnedDev {
+  std::vector<void *> p;
+  bool keep = false;
+  ~OwnedDev() {
+    if (!keep)
+      for (void *q : p) (void)hipFree(q);
+  }
+  template <typename T>
+  T *add(T *q) {
+    p.push_back(q);
+    return q;
+  }
+};
 
+// savePolygon's format (polygon.h:575-584) at full precision: one vertex
+// per line, "x y," inside a component and "x y;" at its end.
 void write_results(gcmf::Context &c, const gcmf::ClipResult &r) {
   const char *path = std::getenv("GCMF_RESULT_FILE");
   if (!path || r.totalVerts <= 0) return;
@@ -89,8 +120,7 @@ void write_results(gcmf::Context &c, const gcmf::ClipResult &r) {
                            hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   FILE *f = std::fopen(path, "w");
-  if (!f) return;
-  // savePolygon format (polygon.h:575-584) at full precision
+  if (!f) throw gcmf::Error{GCMF_ERR_ARG, std::string("cannot write ") + path};
   long v = 0;
   for (int comp : cs) {
     for (int i = 0; i < comp; i++, v++)
@@ -99,9 +129,58 @@ void write_results(gcmf::Context &c, const gcmf::ClipResult &r) {
   std::fclose(f);
 }
 
+// Predicate frame of ghcuda / GH_CUDA: the normalised, edge-scaled frame
+// (frame bits 1 | 4 of ClipOptions::normalize) unless GCMF_NORMALIZE=0 asks
+// for the reference's raw coordinates.
 int normalize_default() {
   const char *e = std::getenv("GCMF_NORMALIZE");
-  return (e && e[0] == '0') ? 0 : 1;
+  return (e && e[0] == '0') ? 0 : 5;
+}
+
+// Clips M pairs on the default context and keeps the results there, where
+// gcmf_clip_get / gcmf_clip_last_stats on gcmf_ref_context() read them.
+void ref_clip(gcmf_ctx &x, const gcmf::JoinInputs &in, const int *d_pairs, long M,
+              const gcmf::ClipOptions &opt) {
+  x.clip_valid = false;
+  gcmf::clip_pairs(x.c, in, d_pairs, M, opt, x.cr, &x.p1);
+  x.c.flush_timers();
+  x.clip_valid = true;
+}
+
+// Host layers to the device: the join of ST_Intersect (ST_Intersect.cu:18-212)
+// with the fixed hand-off; the joined pairs stay in the context workspace.
+gcmf::JoinRefine &st_intersect(gcmf_ctx &x, long bPolNum, long oPolNum, coord_t *baseCoords,
+                               coord_t *overlayCoords, int *bVNum, int *oVNum, long *bVPSNum,
+                               long *oVPSNum, mbr_t *seqMBR, mbr_t *seqOMBR, coord_t *seqMBR2,
+                               coord_t *seqOMBR2) {
+  gcmf::Context &c = x.c;
+  if (bPolNum < 0 || oPolNum < 0) throw gcmf::Error{GCMF_ERR_ARG, "negative polygon count"};
+  const long nb = bPolNum > 0 ? bVPSNum[bPolNum - 1] : 0;
+  const long no = oPolNum > 0 ? oVPSNum[oPolNum - 1] : 0;
+  auto up = [&](const char *tag, const void *src, size_t bytes) {
+    void *d = c.ws.get(std::string("ref_st_") + tag, bytes);
+    if (bytes) HIP_CHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, c.stream));
+    return d;
+  };
+  gcmf::JoinInputs in;
+  in.B = bPolNum;
+  in.O = oPolNum;
+  in.bCoords = (const double *)up("bc", baseCoords, 16 * (size_t)nb);
+  in.oCoords = (const double *)up("oc", overlayCoords, 16 * (size_t)no);
+  in.bVNum = (const int *)up("bv", bVNum, 4 * (size_t)bPolNum);
+  in.oVNum = (const int *)up("ov", oVNum, 4 * (size_t)oPolNum);
+  in.bVPSNum = (const long *)up("bp", bVPSNum, 8 * (size_t)bPolNum);
+  in.oVPSNum = (const long *)up("op", oVPSNum, 8 * (size_t)oPolNum);
+  in.bMBR = (const uint64_t *)up("bm", seqMBR, 32 * (size_t)bPolNum);
+  in.oMBR = (const uint64_t *)up("om", seqOMBR, 32 * (size_t)oPolNum);
+  in.bMBR2 = (const double *)up("bm2", seqMBR2, 32 * (size_t)bPolNum);
+  in.oMBR2 = (const double *)up("om2", seqOMBR2, 32 * (size_t)oPolNum);
+  x.join_valid = false;
+  x.jin = in;
+  x.K = gcmf::mbr_filter(c, bPolNum, oPolNum, in.bMBR, in.oMBR, &x.counts, &x.pairs, &x.sv);
+  gcmf::join_refine(c, in, x.K, x.pairs, x.jr);
+  x.join_valid = true;
+  return x.jr;
 }
 
 }  // namespace
@@ -113,7 +192,8 @@ long SortBaseMBROverlap(long bPolNum, long oPolNum, mbr_t *dbMBR, mbr_t *doMBR, 
   long K = -1;
   *djxyCounter = nullptr;
   *djxyVector = nullptr;
-  shim("SortBaseMBROverlap", [&](gcmf::Context &c) {
+  shim("SortBaseMBROverlap", [&](gcmf_ctx &x) {
+    gcmf::Context &c = x.c;
     const uint64_t *b = (const uint64_t *)dbMBR, *o = (const uint64_t *)doMBR;
     if (dimSort == 1 && dimSelect == 1) {  // InitializeData with x/y exchanged (Join.h:34)
       uint64_t *sb = (uint64_t *)c.ws.get("ref_swap_b", 32 * (size_t)(bPolNum + 1));
@@ -144,7 +224,8 @@ long SortBaseMBROverlap(long bPolNum, long oPolNum, mbr_t *dbMBR, mbr_t *doMBR, 
 
 void GetCMBR(long pairNum, int *jxyVector, coord_t *bMBR, coord_t *oMBR, coord_t **cMBR,
              int **jpipIndexList, int **pipFlag, char **pipType, int **joinFlag, long *pipNum) {
-  shim("GetCMBR", [&](gcmf::Context &c) {
+  shim("GetCMBR", [&](gcmf_ctx &x) {
+    gcmf::Context &c = x.c;
     *cMBR = dalloc<double>(4 * pairNum);
     *pipFlag = dalloc<int>(pairNum);
     *pipType = dalloc<char>(pairNum);
@@ -166,7 +247,8 @@ long PointInPolygonTest(coord_t *bCoords, coord_t *oCoords, long pairNum, long p
                         int *jxyVector, int *jPiPIndexList, char *pipType, long *bVPSNum,
                         long *oVPSNum, int *pipFlag, int *joinFlag) {
   long within = -1;
-  shim("PointInPolygonTest", [&](gcmf::Context &c) {
+  shim("PointInPolygonTest", [&](gcmf_ctx &x) {
+    gcmf::Context &c = x.c;
     gcmf::launch_pip(c, pipNum, jPiPIndexList, jxyVector, (const signed char *)pipType, bCoords,
                      bVPSNum, oCoords, oVPSNum, pipFlag, joinFlag);
     int *tmp = (int *)c.ws.get("ref_within_idx", 4 * (size_t)(pairNum + 1));
@@ -180,7 +262,8 @@ void CountCMF(coord_t *bCoords, coord_t *oCoords, long pairNum, int *jxyVector, 
               long **bEdgePSCounter, long **oEdgePSCounter, long **workLoadPSCounter,
               int **jxy2IndexList, poly_size_t **bEdgeList, poly_size_t **oEdgeList, long *eiNum,
               long *workLoadNum) {
-  shim("CountCMF", [&](gcmf::Context &c) {
+  shim("CountCMF", [&](gcmf_ctx &x) {
+    gcmf::Context &c = x.c;
     gcmf::JoinInputs in;
     in.bCoords = bCoords;
     in.oCoords = oCoords;
@@ -238,7 +321,8 @@ void ApplyCMF(coord_t *bCoords, coord_t *oCoords, long pairNum, int *jxyVector, 
               coord_t *cMBR, long *bEdgePSCounter, long *oEdgePSCounter, poly_size_t *bEdgeList,
               poly_size_t *oEdgeList) {
   (void)pairNum;
-  shim("ApplyCMF", [&](gcmf::Context &c) {
+  shim("ApplyCMF", [&](gcmf_ctx &x) {
+    gcmf::Context &c = x.c;
     gcmf::JoinInputs in;
     in.bCoords = bCoords;
     in.oCoords = oCoords;
@@ -259,7 +343,8 @@ long SegmentIntersectJoin(coord_t *bCoords, coord_t *oCoords, long pairNum, int 
                           poly_size_t *oEdgeList, int **segIntersectJoinFlag) {
   long hits = -1;
   *segIntersectJoinFlag = nullptr;
-  shim("SegmentIntersectJoin", [&](gcmf::Context &c) {
+  shim("SegmentIntersectJoin", [&](gcmf_ctx &x) {
+    gcmf::Context &c = x.c;
     gcmf::JoinInputs in;
     in.bCoords = bCoords;
     in.oCoords = oCoords;
@@ -287,7 +372,8 @@ void calculateIntersections(coord_t *baseCoord, coord_t *overlayCoords, int size
   (void)sizeQ;
   (void)cmbr;  // recomputed from the rings exactly as getCMBR does
   *invalid = 1;
-  shim("calculateIntersections", [&](gcmf::Context &c) {
+  shim("calculateIntersections", [&](gcmf_ctx &x) {
+    gcmf::Context &c = x.c;
     gcmf::JoinInputs in;
     in.bCoords = baseCoord;
     in.oCoords = overlayCoords;
@@ -302,9 +388,8 @@ void calculateIntersections(coord_t *baseCoord, coord_t *overlayCoords, int size
     opt.normalize = 0;
     opt.noswap = 1;
     opt.route = 0;  // the phase-1 arrays of calculateIntersections
-    gcmf::ClipResult r;
-    gcmf::Phase1View p;
-    gcmf::clip_pairs(c, in, dp, 1, opt, r, &p);
+    ref_clip(x, in, dp, 1, opt);
+    const gcmf::Phase1View &p = x.p1;
     int st;
     HIP_CHECK(hipMemcpy(&st, p.status, 4, hipMemcpyDeviceToHost));
     if (st != gcmf::GH_OK && st < 0) st = gcmf::GH_OK;  // phase 1 itself succeeded
@@ -334,33 +419,39 @@ void calculateIntersections(coord_t *baseCoord, coord_t *overlayCoords, int size
 
 int GH_CUDA(coord_t *bCoords, coord_t *oCoords, int *pBVNum, long *pBVPSNum, int *pOVNum,
             long *pOVPSNum, int bPID, int oPID, int swapped) {
-  (void)swapped;  // the swap is decided from the ring sizes (ghcuda.cpp:64-91)
   int invalid = 1;
-  shim("GH_CUDA", [&](gcmf::Context &c) {
+  shim("GH_CUDA", [&](gcmf_ctx &x) {
+    gcmf::Context &c = x.c;
+    // swapped = 1: P is overlay ring oPID and Q base ring bPID
+    // (regularPolygonHandler, ghcuda.cpp:196-219); the caller decides, as
+    // loadPolygonDataFromLayer does for ghcuda (ghcuda.cpp:64-91)
     gcmf::JoinInputs in;
-    in.bCoords = bCoords;
-    in.oCoords = oCoords;
-    in.bVNum = pBVNum;
-    in.oVNum = pOVNum;
-    in.bVPSNum = pBVPSNum;
-    in.oVPSNum = pOVPSNum;
+    in.bCoords = swapped ? oCoords : bCoords;
+    in.oCoords = swapped ? bCoords : oCoords;
+    in.bVNum = swapped ? pOVNum : pBVNum;
+    in.oVNum = swapped ? pBVNum : pOVNum;
+    in.bVPSNum = swapped ? pOVPSNum : pBVPSNum;
+    in.oVPSNum = swapped ? pBVPSNum : pOVPSNum;
     int *dp = (int *)c.ws.get("ref_gh_pair", 8);
-    int hp[2] = {bPID, oPID};
+    int hp[2] = {swapped ? oPID : bPID, swapped ? bPID : oPID};
     HIP_CHECK(hipMemcpyAsync(dp, hp, 8, hipMemcpyHostToDevice, c.stream));
     gcmf::ClipOptions opt;
     opt.normalize = normalize_default();
-    gcmf::clip_pairs(c, in, dp, 1, opt, g_last_clip, nullptr);
+    opt.noswap = 1;
+    ref_clip(x, in, dp, 1, opt);
     int st;
-    HIP_CHECK(hipMemcpy(&st, g_last_clip.status, 4, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpyAsync(&st, x.cr.status, 4, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
     invalid = st == gcmf::GH_INVALID ? 1 : 0;
-    write_results(c, g_last_clip);
+    write_results(c, x.cr);
   });
   return invalid;
 }
 
 int ghcuda(int pIDList[], int qIDList[], int totalNumPairs, coord_t *bCoords, coord_t *oCoords,
            int *pBVNum, long *pBVPSNum, int *pOVNum, long *pOVPSNum) {
-  shim("ghcuda", [&](gcmf::Context &c) {
+  shim("ghcuda", [&](gcmf_ctx &x) {
+    gcmf::Context &c = x.c;
     gcmf::JoinInputs in;
     in.bCoords = bCoords;
     in.oCoords = oCoords;
@@ -378,49 +469,187 @@ int ghcuda(int pIDList[], int qIDList[], int totalNumPairs, coord_t *bCoords, co
     HIP_CHECK(hipMemcpyAsync(dp, hp.data(), 8 * (size_t)M, hipMemcpyHostToDevice, c.stream));
     gcmf::ClipOptions opt;
     opt.normalize = normalize_default();
-    gcmf::clip_pairs(c, in, dp, M, opt, g_last_clip, nullptr);
-    write_results(c, g_last_clip);
+    ref_clip(x, in, dp, M, opt);
+    write_results(c, x.cr);
   });
   return 0;
 }
 
 int ST_Intersect(long bPolNum, long oPolNum, coord_t *baseCoords, coord_t *overlayCoords,
                  int *bVNum, int *oVNum, long *bVPSNum, long *oVPSNum, mbr_t *seqMBR,
-                 mbr_t *seqOMBR, coord_t *seqMBR2, coord_t *seqOMBR2, int **jPairs) {
+                 mbr_t *seqOMBR, coord_t *seqMBR2, coord_t *seqOMBR2, int *jPairs) {
+  // jPairs is passed by value in the reference and never written through
+  // (ST_Intersect.cu:191 assigns a fresh malloc to the local copy), so the
+  // caller's buffer is left untouched here too
+  (void)jPairs;
   int ret = -1;
+  shim("ST_Intersect", [&](gcmf_ctx &x) {
+    gcmf::JoinRefine &r = st_intersect(x, bPolNum, oPolNum, baseCoords, overlayCoords, bVNum,
+                                       oVNum, bVPSNum, oVPSNum, seqMBR, seqOMBR, seqMBR2,
+                                       seqOMBR2);
+    ret = (int)r.nJoined;
+  });
+  return ret;
+}
+
+int gcmf_st_intersect_pairs(long bPolNum, long oPolNum, const double *baseCoords,
+                            const double *overlayCoords, const int *bVNum, const int *oVNum,
+                            const long *bVPSNum, const long *oVPSNum, const uint64_t *seqMBR,
+                            const uint64_t *seqOMBR, const double *seqMBR2,
+                            const double *seqOMBR2, int **jPairs) {
+  if (!jPairs) return GCMF_ERR_ARG;
   *jPairs = nullptr;
-  shim("ST_Intersect", [&](gcmf::Context &c) {
-    const long nb = bPolNum > 0 ? bVPSNum[bPolNum - 1] : 0;
-    const long no = oPolNum > 0 ? oVPSNum[oPolNum - 1] : 0;
-    auto up = [&](const char *tag, const void *src, size_t bytes) {
-      void *d = c.ws.get(std::string("ref_st_") + tag, bytes);
-      if (bytes) HIP_CHECK(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, c.stream));
-      return d;
-    };
-    gcmf::JoinInputs in;
-    in.B = bPolNum;
-    in.O = oPolNum;
-    in.bCoords = (const double *)up("bc", baseCoords, 16 * (size_t)nb);
-    in.oCoords = (const double *)up("oc", overlayCoords, 16 * (size_t)no);
-    in.bVNum = (const int *)up("bv", bVNum, 4 * (size_t)bPolNum);
-    in.oVNum = (const int *)up("ov", oVNum, 4 * (size_t)oPolNum);
-    in.bVPSNum = (const long *)up("bp", bVPSNum, 8 * (size_t)bPolNum);
-    in.oVPSNum = (const long *)up("op", oVPSNum, 8 * (size_t)oPolNum);
-    in.bMBR = (const uint64_t *)up("bm", seqMBR, 32 * (size_t)bPolNum);
-    in.oMBR = (const uint64_t *)up("om", seqOMBR, 32 * (size_t)oPolNum);
-    in.bMBR2 = (const double *)up("bm2", seqMBR2, 32 * (size_t)bPolNum);
-    in.oMBR2 = (const double *)up("om2", seqOMBR2, 32 * (size_t)oPolNum);
-    int *pairs = nullptr;
-    long K = gcmf::mbr_filter(c, bPolNum, oPolNum, in.bMBR, in.oMBR, nullptr, &pairs, nullptr);
-    gcmf::JoinRefine r;
-    gcmf::join_refine(c, in, K, pairs, r);
+  int ret = GCMF_ERR_HIP;
+  shim("gcmf_st_intersect_pairs", [&](gcmf_ctx &x) {
+    gcmf::JoinRefine &r = st_intersect(
+        x, bPolNum, oPolNum, const_cast<double *>(baseCoords), const_cast<double *>(overlayCoords),
+        const_cast<int *>(bVNum), const_cast<int *>(oVNum), const_cast<long *>(bVPSNum),
+        const_cast<long *>(oVPSNum), (mbr_t *)seqMBR, (mbr_t *)seqOMBR,
+        const_cast<double *>(seqMBR2), const_cast<double *>(seqOMBR2));
     int *h = (int *)std::malloc(8 * (size_t)(r.nJoined ? r.nJoined : 1));
-    if (r.nJoined > 0)
-      HIP_CHECK(hipMemcpy(h, r.joinedPairs, 8 * (size_t)r.nJoined, hipMemcpyDeviceToHost));
+    if (!h) throw gcmf::Error{GCMF_ERR_ALLOC, "host allocation of the joined pairs"};
+    if (r.nJoined > 0) {
+      HIP_CHECK(hipMemcpyAsync(h, r.joinedPairs, 8 * (size_t)r.nJoined, hipMemcpyDeviceToHost,
+                               x.c.stream));
+      HIP_CHECK(hipStreamSynchronize(x.c.stream));
+    }
     *jPairs = h;
     ret = (int)r.nJoined;
   });
   return ret;
+}
+
+int spatialJoin(int argc, char *argv[], int **pIDList, int **qIDList, long *totalPairNum,
+                coord_t **pBaseCoords, coord_t **pOverlayCoords, int **pBVNum, long **pBVPSNum,
+                int **pOVNum, long **pOVPSNum) {
+  int rc = -1;
+  shim("spatialJoin", [&](gcmf_ctx &x) {
+    gcmf::Context &c = x.c;
+    // console arguments as spatialJoin3.cu:52-69: dimSort 1 sorts one
+    // dimension (dimSelect 'y' picks y), 2 both (here: x)
+    int dimSort = 1, dimSelect = 1;
+    if (argc < 2) {
+      dimSelect = 0;
+    } else if (argc < 3) {
+      dimSort = argv[1][0] == '2' ? 2 : 1;
+      dimSelect = 0;
+    } else if (argc < 4) {
+      dimSelect = argv[2][0] == 'y' ? 1 : 0;
+      if (argv[1][0] == '2') {
+        dimSort = 2;
+        dimSelect = 0;
+      }
+    }
+    // layer files: the reference compiles their names in (DATASET,
+    // spatialJoin3.cu:88-117); here they come from the environment
+    const char *bf = std::getenv("GCMF_BASE_FILE"), *of = std::getenv("GCMF_OVERLAY_FILE");
+    if (!bf || !of)
+      throw gcmf::Error{GCMF_ERR_ARG, "set GCMF_BASE_FILE and GCMF_OVERLAY_FILE to the layers"};
+    std::unique_ptr<gcmf_layer, void (*)(gcmf_layer *)> bl(gcmf_layer_read_text(bf, -1),
+                                                           gcmf_layer_free);
+    std::unique_ptr<gcmf_layer, void (*)(gcmf_layer *)> ol(gcmf_layer_read_text(of, -1),
+                                                           gcmf_layer_free);
+    if (!bl || !ol) throw gcmf::Error{GCMF_ERR_ARG, "cannot read the layer files"};
+    const long B = gcmf_layer_npoly(bl.get()), O = gcmf_layer_npoly(ol.get());
+    const long nb = gcmf_layer_nverts(bl.get()), no = gcmf_layer_nverts(ol.get());
+    // resident layers, handed to the caller as the reference hands its
+    // device arrays to ghcuda (spatialJoin3.cu:397-402)
+    OwnedDev own;
+    double *bc = own.add(dalloc<double>(2 * nb)), *oc = own.add(dalloc<double>(2 * no));
+    int *bv = own.add(dalloc<int>(B)), *ov = own.add(dalloc<int>(O));
+    long *bp = own.add(dalloc<long>(B)), *op = own.add(dalloc<long>(O));
+    auto up = [&](void *d, const void *h, size_t bytes) {
+      if (bytes) HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c.stream));
+    };
+    up(bc, gcmf_layer_coords(bl.get()), 16 * (size_t)nb);
+    up(oc, gcmf_layer_coords(ol.get()), 16 * (size_t)no);
+    up(bv, gcmf_layer_vnum(bl.get()), 4 * (size_t)B);
+    up(ov, gcmf_layer_vnum(ol.get()), 4 * (size_t)O);
+    up(bp, gcmf_layer_vpsnum(bl.get()), 8 * (size_t)B);
+    up(op, gcmf_layer_vpsnum(ol.get()), 8 * (size_t)O);
+    gcmf::JoinInputs in;
+    in.B = B;
+    in.O = O;
+    in.bCoords = bc;
+    in.oCoords = oc;
+    in.bVNum = bv;
+    in.oVNum = ov;
+    in.bVPSNum = bp;
+    in.oVPSNum = op;
+    in.bMBR2 = (const double *)c.ws.get("ref_sj_bm2", 32 * (size_t)(B + 1));
+    in.oMBR2 = (const double *)c.ws.get("ref_sj_om2", 32 * (size_t)(O + 1));
+    up((void *)in.bMBR2, gcmf_layer_mbr2(bl.get()), 32 * (size_t)B);
+    up((void *)in.oMBR2, gcmf_layer_mbr2(ol.get()), 32 * (size_t)O);
+    // MBR keys in the dimension the arguments select
+    std::vector<uint64_t> km[2];
+    const gcmf_layer *ls[2] = {bl.get(), ol.get()};
+    const long ns[2] = {B, O};
+    const char *tags[2] = {"ref_sj_bm", "ref_sj_om"};
+    const uint64_t *dk[2];
+    for (int s = 0; s < 2; s++) {
+      const uint64_t *m = gcmf_layer_mbr(ls[s]);
+      km[s].assign(m, m + 4 * ns[s]);
+      if (dimSort == 1 && dimSelect == 1)
+        for (long i = 0; i < ns[s]; i++) {
+          std::swap(km[s][4 * i], km[s][4 * i + 1]);
+          std::swap(km[s][4 * i + 2], km[s][4 * i + 3]);
+        }
+      uint64_t *d = (uint64_t *)c.ws.get(tags[s], 32 * (size_t)(ns[s] + 1));
+      up(d, km[s].data(), 32 * (size_t)ns[s]);
+      dk[s] = d;
+    }
+    in.bMBR = dk[0];
+    in.oMBR = dk[1];
+    x.join_valid = false;
+    x.jin = in;
+    x.K = gcmf::mbr_filter(c, B, O, in.bMBR, in.oMBR, &x.counts, &x.pairs, &x.sv);
+    gcmf::join_refine(c, in, x.K, x.pairs, x.jr);
+    x.join_valid = true;
+    // the joined pairs (within U intersecting; the reference hands over the
+    // within pairs only, spatialJoin3.cu:407)
+    const long n = x.jr.nJoined;
+    std::vector<int> jp(2 * (size_t)(n ? n : 1));
+    if (n > 0) {
+      HIP_CHECK(hipMemcpyAsync(jp.data(), x.jr.joinedPairs, 8 * (size_t)n, hipMemcpyDeviceToHost,
+                               c.stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    int *pl = (int *)std::malloc(4 * (size_t)(n ? n : 1));
+    int *ql = (int *)std::malloc(4 * (size_t)(n ? n : 1));
+    if (!pl || !ql) {
+      std::free(pl);
+      std::free(ql);
+      throw gcmf::Error{GCMF_ERR_ALLOC, "host allocation of the pair lists"};
+    }
+    for (long k = 0; k < n; k++) {
+      pl[k] = jp[2 * k];
+      ql[k] = jp[2 * k + 1];
+    }
+    *pIDList = pl;
+    *qIDList = ql;
+    *totalPairNum = n;
+    *pBaseCoords = bc;
+    *pOverlayCoords = oc;
+    *pBVNum = bv;
+    *pBVPSNum = bp;
+    *pOVNum = ov;
+    *pOVPSNum = op;
+    own.keep = true;
+    rc = 0;
+  });
+  return rc;
+}
+
+gcmf_ctx *gcmf_ref_context(void) {
+  gcmf_ctx *r = nullptr;
+  shim("gcmf_ref_context", [&](gcmf_ctx &x) { r = &x; });
+  return r;
+}
+
+void gcmf_ref_release(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_ctx) gcmf_shutdown(g_ctx);
+  g_ctx = nullptr;
 }
 
 }  // extern "C"

@@ -223,7 +223,8 @@ CLIP_NORMALIZE = 1
 CLIP_NO_CLEANUP = 2
 CLIP_SEQ_LABEL = 4
 CLIP_GPU_ROUTE = 8
-CLIP_SCALE_EDGES = 16
+CLIP_SCALE_EDGES = 16  # accepted, no effect: the edge scale is part of CLIP_NORMALIZE
+CLIP_UNIT_FRAME = 32
 
 
 def hip_lib():
@@ -263,8 +264,49 @@ def hip_lib():
         L.gcmf_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.gcmf_tune_clip.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), vp, C.c_long,
                                      C.c_int, C.POINTER(Tuning)]
+        L.gcmf_clip_last_stats.argtypes = [vp, C.POINTER(ClipStats)]
+        _ref_prototypes(L)
         _hip = L
     return _hip
+
+
+def _ref_prototypes(L):
+    """ctypes prototypes of the reference-named entry points (include/gcmf_hip.h),
+    as a caller of Join.h / ghcuda.h / main.h / ST_Intersect.cu binds them."""
+    vp, lp, ip, dp = C.c_void_p, C.c_long, C.POINTER(C.c_int), C.POINTER(C.c_double)
+    P = C.POINTER
+    L.SortBaseMBROverlap.argtypes = [lp, lp, vp, vp, P(vp), P(vp), C.c_char, C.c_char]
+    L.SortBaseMBROverlap.restype = lp
+    L.GetCMBR.argtypes = [lp, vp, vp, vp, P(vp), P(vp), P(vp), P(vp), P(vp), P(lp)]
+    L.GetCMBR.restype = None
+    L.PointInPolygonTest.argtypes = [vp, vp, lp, lp, vp, vp, vp, vp, vp, vp, vp]
+    L.PointInPolygonTest.restype = lp
+    L.CountCMF.argtypes = [vp, vp, lp, vp, vp, vp, vp, vp, vp, vp, P(vp), P(vp), P(vp), P(vp),
+                           P(vp), P(vp), P(lp), P(lp)]
+    L.CountCMF.restype = None
+    L.ApplyCMF.argtypes = [vp, vp, lp, vp, lp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.ApplyCMF.restype = None
+    L.SegmentIntersectJoin.argtypes = [vp, vp, lp, vp, vp, vp, vp, vp, vp, vp, vp, P(vp)]
+    L.SegmentIntersectJoin.restype = lp
+    L.calculateIntersections.argtypes = [vp, vp, C.c_int, C.c_int, dp, vp, vp, vp, vp, C.c_int,
+                                         C.c_int, ip, ip, P(dp), P(dp), P(ip), P(ip), P(ip),
+                                         P(ip), P(ip), P(ip), ip]
+    L.calculateIntersections.restype = None
+    L.GH_CUDA.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
+    L.GH_CUDA.restype = C.c_int
+    L.ghcuda.argtypes = [ip, ip, C.c_int, vp, vp, vp, vp, vp, vp]
+    L.ghcuda.restype = C.c_int
+    L.ST_Intersect.argtypes = [lp, lp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.ST_Intersect.restype = C.c_int
+    L.gcmf_st_intersect_pairs.argtypes = [lp, lp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, P(ip)]
+    L.gcmf_st_intersect_pairs.restype = C.c_int
+    L.spatialJoin.argtypes = [C.c_int, P(C.c_char_p), P(ip), P(ip), P(lp), P(dp), P(dp), P(ip),
+                              P(P(C.c_long)), P(ip), P(P(C.c_long))]
+    L.spatialJoin.restype = C.c_int
+    L.gcmf_ref_context.argtypes = []
+    L.gcmf_ref_context.restype = vp
+    L.gcmf_ref_release.argtypes = []
+    L.gcmf_ref_release.restype = None
 
 
 class DeviceBuffer:
@@ -447,3 +489,22 @@ class Context:
                 v += n
             out.append(comps)
         return status, out
+
+
+class RefContext(Context):
+    """The process-default context of the reference-named entry points
+    (gcmf_ref_context): reads the device results of their last calls."""
+
+    def __init__(self):
+        h = hip_lib().gcmf_ref_context()
+        if not h:
+            raise GcmfError("no default context (no HIP device)")
+        self.h = C.c_void_p(h)
+
+    def close(self):
+        self.h = None
+
+    def last_clip_stats(self):
+        st = ClipStats()
+        self._check(hip_lib().gcmf_clip_last_stats(self.h, C.byref(st)), "clip_last_stats")
+        return st.as_dict()

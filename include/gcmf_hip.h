@@ -120,7 +120,16 @@ enum {
 int gcmf_join_get(gcmf_ctx *ctx, int field, void *dst, long count);
 
 /* ------------------------------------------------------------ clip ----- */
-#define GCMF_CLIP_NORMALIZE 1 /* per-pair CMBR normalisation of predicates */
+/* Per-pair predicate frame.  The reference evaluates every predicate on raw
+ * coordinates against an absolute epsilon (1e-7, constants.h:4), which
+ * classifies nearly every edge pair of small polygons as collinear.  With
+ * GCMF_CLIP_NORMALIZE each pair's predicates run on ((x - cx) s, (y - cy) s),
+ * (cx, cy) the CMBR corner and s = f / max(CMBR width, height), where f is
+ * the power of two f >= sqrt(nP nQ) / 1024 (1 for rings of up to 1024
+ * vertices), so that the edges of long rings stay long against the epsilon.
+ * Output vertices are computed in the original coordinates.  Without the
+ * flag the predicates use raw coordinates, as the reference does. */
+#define GCMF_CLIP_NORMALIZE 1
 #define GCMF_CLIP_NO_CLEANUP 2 /* skip cleanUpResult (polyclip.h:791)      */
 #define GCMF_CLIP_SEQ_LABEL 4  /* label/trace one thread per pair (checking) */
 /* Phase 1 as the reference's GPU route (calculateIntersections,
@@ -128,12 +137,13 @@ int gcmf_join_get(gcmf_ctx *ctx, int field, void *dst, long count);
  * polyclip.h:188-281).  The two agree in general position; on degenerate
  * inputs only the CPU route is well defined. */
 #define GCMF_CLIP_GPU_ROUTE 8
-/* With GCMF_CLIP_NORMALIZE: scale each pair's normalised frame further by
- * the power of two f >= sqrt(nP nQ) / 1024 (1 for rings of up to 1024
- * vertices).  The reference's absolute epsilon (1e-7, constants.h:4) makes
- * nearby edges of million-vertex rings "collinear" and their trace fail;
- * this keeps their edges long against it.  A deviation from the reference. */
+/* Accepted for compatibility: the edge scale f is now part of
+ * GCMF_CLIP_NORMALIZE's frame, so this flag changes nothing. */
 #define GCMF_CLIP_SCALE_EDGES 16
+/* With GCMF_CLIP_NORMALIZE: keep f = 1 (the plain unit CMBR frame).  Rings
+ * of 10^4-scale integer coordinates with 10^5 vertices (the reference's
+ * s.txt x c.txt pair) fail their trace in that frame. */
+#define GCMF_CLIP_UNIT_FRAME 32
 
 typedef struct {
   long pairs, ok, invalid, disjoint, failed;
@@ -265,16 +275,56 @@ int GH_CUDA(coord_t *bCoords, coord_t *oCoords, int *pBVNum, long *pBVPSNum, int
             long *pOVPSNum, int bPID, int oPID, int swapped);
 
 /* main.h:8-11 / ghcuda.cpp:385 -- clips every listed pair of the resident
- * layers; returns 0.  Results are kept for gcmf_ghcuda_result. */
+ * layers (each swapped so |P| >= |Q|, ghcuda.cpp:64-91); returns 0.  The
+ * results stay on the process-default context: read them with
+ * gcmf_clip_last_stats / gcmf_clip_get on gcmf_ref_context(), or set
+ * GCMF_RESULT_FILE to have them written in savePolygon's format.
+ * GCMF_NORMALIZE=0 selects the raw predicate frame (the reference's). */
 int ghcuda(int pIDList[], int qIDList[], int totalNumPairs, coord_t *bCoords, coord_t *oCoords,
            int *pBVNum, long *pBVPSNum, int *pOVNum, long *pOVPSNum);
 
-/* ST_Intersect.cu:18 (fixed variant): host layers in; *jPairs (host,
- * malloc'ed, caller frees) receives the 2*n joined pairs; returns n =
- * #within + #intersecting. */
+/* main.h:5-7 / spatialJoin3.cu:40-412 -- reads the two layer files named by
+ * GCMF_BASE_FILE and GCMF_OVERLAY_FILE (the reference compiles its dataset
+ * paths in, spatialJoin3.cu:88-117; argv picks the sort dimension as there),
+ * joins them on the GPU and returns: the joined pairs as host malloc'ed
+ * lists (*pIDList base ids, *qIDList overlay ids, *totalPairNum of them;
+ * within U segment-intersecting, where the reference hands over the within
+ * pairs only, spatialJoin3.cu:407), and the layers resident in device memory
+ * (hipMalloc'ed, owned by the caller), ready for ghcuda.  Returns 0, or -1
+ * with every output untouched. */
+int spatialJoin(int argc, char *argv[], int **pIDList, int **qIDList, long *totalPairNum,
+                coord_t **pBaseCoords, coord_t **pOverlayCoords, int **pBVNum, long **pBVPSNum,
+                int **pOVNum, long **pOVPSNum);
+
+/* ST_Intersect.cu:18-19 -- the reference's prototype: host layers in;
+ * returns #within + #intersecting.  jPairs is passed by value there and the
+ * pair list never reaches the caller, so this entry point does not write
+ * through it either; use gcmf_st_intersect_pairs for the pairs.  Returns -1
+ * on failure. */
 int ST_Intersect(long bPolNum, long oPolNum, coord_t *baseCoords, coord_t *overlayCoords,
                  int *bVNum, int *oVNum, long *bVPSNum, long *oVPSNum, mbr_t *seqMBR,
-                 mbr_t *seqOMBR, coord_t *seqMBR2, coord_t *seqOMBR2, int **jPairs);
+                 mbr_t *seqOMBR, coord_t *seqMBR2, coord_t *seqOMBR2, int *jPairs);
+
+/* ST_Intersect with its output fixed (ST_Intersect.cu:186-212): *jPairs
+ * (host, malloc'ed, caller frees) receives the 2n joined (base, overlay)
+ * pairs, within pairs and intersecting pairs in candidate order; returns n,
+ * or a negative status with *jPairs NULL. */
+int gcmf_st_intersect_pairs(long bPolNum, long oPolNum, const double *baseCoords,
+                            const double *overlayCoords, const int *bVNum, const int *oVNum,
+                            const long *bVPSNum, const long *oVPSNum, const uint64_t *seqMBR,
+                            const uint64_t *seqOMBR, const double *seqMBR2,
+                            const double *seqOMBR2, int **jPairs);
+
+/* The process-default context the reference-named entry points run on
+ * (created on first use on the current device), so that their device
+ * results can be read with gcmf_clip_get / gcmf_join_get; NULL when no
+ * device is usable.  No reference counterpart (it keeps global state, F7). */
+gcmf_ctx *gcmf_ref_context(void);
+/* Releases the process-default context and its device memory; the next
+ * reference-named call creates a new one. */
+void gcmf_ref_release(void);
+/* Statistics of the last clip on a context. */
+int gcmf_clip_last_stats(gcmf_ctx *ctx, gcmf_clip_stats *stats);
 
 #ifdef __cplusplus
 }

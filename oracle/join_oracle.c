@@ -13,6 +13,11 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Planted mistakes for the tests that show the oracle's pins are not
+ * vacuous (each pin must fail under its mutant).  0, the default, is the
+ * oracle itself; nothing but those tests sets it. */
+int orc_mutant = 0;
+
 /* CoordToMBR(type 1), SEQ_Overlay.h:69-97: keep at most 12 fractional
  * digits of the decimal token, drop '.', zero-pad, atoll, add 9e15. */
 uint64_t orc_coord_to_mbr(const char *ct) {
@@ -164,8 +169,16 @@ long long orc_pip(long long K, const int *pairs, const signed char *pipType,
     int c = 0;
     for (long long i = 0, j = vNum - 1; i < vNum; j = i++) {
       double Y1 = poly[2 * i + 1], Y2 = poly[2 * j + 1], X1 = poly[2 * i], X2 = poly[2 * j];
-      if ((Y1 > pY) == (Y2 > pY)) continue;
-      if (pX < (X2 - X1) * (pY - Y1) / (Y2 - Y1) + X1) c = !c;
+      if (orc_mutant == ORC_MUT_PIP_BOTH_ENDS) {
+        /* counts an edge with an endpoint on the ray from both sides */
+        if ((Y1 > pY && Y2 > pY) || (Y1 < pY && Y2 < pY) || Y1 == Y2) continue;
+      } else if (orc_mutant == ORC_MUT_PIP_CLOSED_Y) {
+        if ((Y1 >= pY) == (Y2 >= pY)) continue;
+      } else if ((Y1 > pY) == (Y2 > pY)) {
+        continue;
+      }
+      double xc = (X2 - X1) * (pY - Y1) / (Y2 - Y1) + X1;
+      if (orc_mutant == ORC_MUT_PIP_LE ? pX <= xc : pX < xc) c = !c;
     }
     pipFlag[k] = c;
     if (c == 1) { joinFlag[k] = 0; within++; }
@@ -180,6 +193,12 @@ static int cmf_edge(const double *c, int vNum, int i, const double *cm) {
   if (i < vNum - 1) { x2 = c[2 * (i + 1)]; y2 = c[2 * (i + 1) + 1]; }
   else { x2 = c[0]; y2 = c[1]; }
   double cX1 = cm[0], cY1 = cm[1], cX2 = cm[2], cY2 = cm[3];
+  if (orc_mutant == ORC_MUT_CMF_NO_CLOSING && i == vNum - 1) return 0;
+  if (orc_mutant == ORC_MUT_CMF_OPEN) {
+    if ((cX1 >= x1 && cX1 >= x2) || (cX2 <= x1 && cX2 <= x2)) return 0;
+    if ((cY1 >= y1 && cY1 >= y2) || (cY2 <= y1 && cY2 <= y2)) return 0;
+    return 1;
+  }
   if ((cX1 > x1 && cX1 > x2) || (cX2 < x1 && cX2 < x2)) return 0;
   if ((cY1 > y1 && cY1 > y2) || (cY2 < y1 && cY2 < y2)) return 0;
   return 1;
@@ -220,7 +239,10 @@ static int seg_hit(double bX1, double bY1, double bX2, double bY2,
   if (((bY2 > oY1) && (bY2 > oY2)) && ((bY1 > oY1) && (bY1 > oY2))) return 0;
   double denom = ((bX1 - bX2) * (oY1 - oY2)) - ((bY1 - bY2) * (oX1 - oX2));
   if (denom == 0) {
-    if ((oX1 == oX2 && oX1 == bX1) || (bX1 == bX2 && bX1 == oX1)) return 1;
+    if (orc_mutant == ORC_MUT_SEG_NO_PARALLEL) return 0;
+    if (orc_mutant != ORC_MUT_SEG_NO_VERTICAL &&
+        ((oX1 == oX2 && oX1 == bX1) || (bX1 == bX2 && bX1 == oX1)))
+      return 1;
     double oM = (oY2 - oY1) / (oX2 - oX1);
     double bM = (bY2 - bY1) / (bX2 - bX1);
     double oB = oY1 - oM * oX1;
@@ -230,6 +252,10 @@ static int seg_hit(double bX1, double bY1, double bX2, double bY2,
   double nomX = (((bX1 * bY2) - (bY1 * bX2)) * (oX1 - oX2)) - ((bX1 - bX2) * ((oX1 * oY2) - (oY1 * oX2)));
   double nomY = ((((bX1 * bY2) - (bY1 * bX2)) * (oY1 - oY2)) - (((bY1 - bY2) * ((oX1 * oY2) - (oY1 * oX2)))));
   double x0 = nomX / denom, y0 = nomY / denom;
+  if (orc_mutant == ORC_MUT_SEG_OPEN &&
+      ((x0 == bX1 && y0 == bY1) || (x0 == bX2 && y0 == bY2) || (x0 == oX1 && y0 == oY1) ||
+       (x0 == oX2 && y0 == oY2)))
+    return 0; /* touching at an endpoint does not count */
   if ((x0 > bX1 && x0 > bX2) || (x0 < bX1 && x0 < bX2)) return 0;
   if ((x0 > oX1 && x0 > oX2) || (x0 < oX1 && x0 < oX2)) return 0;
   if ((y0 > bY1 && y0 > bY2) || (y0 < bY1 && y0 < bY2)) return 0;

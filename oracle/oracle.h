@@ -94,6 +94,20 @@ int orc_initial_label(int qMType, int qPType);
 
 /* ------------------------------------------------------------- join ---- */
 
+/* Planted mistakes (test infrastructure: the tests that show each pin of
+ * the join oracle fails when the oracle is wrong).  0 = the oracle. */
+extern int orc_mutant;
+enum {
+  ORC_MUT_SEG_OPEN = 1,        /* segment join: touching endpoints miss      */
+  ORC_MUT_SEG_NO_PARALLEL = 2, /* segment join: parallel branch always 0     */
+  ORC_MUT_SEG_NO_VERTICAL = 3, /* segment join: vertical collinear case lost */
+  ORC_MUT_PIP_BOTH_ENDS = 4,   /* PiP: edge ending on the ray counted twice  */
+  ORC_MUT_PIP_CLOSED_Y = 5,    /* PiP: y test (y0, y1] instead of [y0, y1)   */
+  ORC_MUT_PIP_LE = 6,          /* PiP: pX <= crossing instead of <           */
+  ORC_MUT_CMF_OPEN = 7,        /* CMF: open box test, touching edges miss    */
+  ORC_MUT_CMF_NO_CLOSING = 8   /* CMF: the closing edge v[n-1] -> v[0] lost  */
+};
+
 /* CoordToMBR type 1 (SEQ_Overlay.h:69-97) on one decimal token. */
 uint64_t orc_coord_to_mbr(const char *tok);
 

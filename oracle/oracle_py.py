@@ -271,3 +271,74 @@ def join_refine(pairs, base, over):
             "join_flag_after_pip": jf[:K], "join_flag": jf2[:K], "E": E,
             "b_count": bc[:K], "o_count": oc[:K], "b_edges": be[:int(bc.sum())],
             "o_edges": oe[:int(oc.sum())], "seg_flag": seg[:K], "hits": hits}
+
+
+# ------------------------------------------------------ planted mistakes ----
+MUT_SEG_OPEN, MUT_SEG_NO_PARALLEL, MUT_SEG_NO_VERTICAL = 1, 2, 3
+MUT_PIP_BOTH_ENDS, MUT_PIP_CLOSED_Y, MUT_PIP_LE = 4, 5, 6
+MUT_CMF_OPEN, MUT_CMF_NO_CLOSING = 7, 8
+
+
+class mutant:
+    """Plants mistake m in the join oracle for the duration of a with-block
+    (orc_mutant), so a test can show that a pin fails on a wrong oracle."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        C.c_int.in_dll(lib(), "orc_mutant").value = self.m
+        return self
+
+    def __exit__(self, *exc):
+        C.c_int.in_dll(lib(), "orc_mutant").value = 0
+        return False
+
+
+def pip_flags(pairs, pip_type, base, over):
+    """orc_pip on explicit pip types: (flags, within)."""
+    L = lib()
+    pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+    K = len(pr) // 2
+    pt = np.ascontiguousarray(pip_type, dtype=np.int8)
+    flag = np.zeros(max(K, 1), dtype=np.int32)
+    jf = np.ones(max(K, 1), dtype=np.int32)
+    w = L.orc_pip(K, pr, pt, base["coords"], base["vpsnum"], over["coords"], over["vpsnum"],
+                  flag, jf)
+    return flag[:K], w
+
+
+def cmf(pairs, cmbr, base, over):
+    """orc_cmf on explicit CMBRs with every join flag set: (flags, b_count,
+    o_count, b_edges, o_edges, E)."""
+    L = lib()
+    pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+    K = len(pr) // 2
+    cm = np.ascontiguousarray(cmbr, dtype=np.float64).reshape(-1)
+    jf = np.ones(max(K, 1), dtype=np.int32)
+    bc = np.zeros(max(K, 1), dtype=np.int32)
+    oc = np.zeros(max(K, 1), dtype=np.int32)
+    L.orc_cmf(K, pr, cm, base["coords"], base["vnum"], base["vpsnum"], over["coords"],
+              over["vnum"], over["vpsnum"], jf.copy(), bc, oc, None, None)
+    be = np.zeros(max(int(bc.sum()), 1), dtype=np.int16)
+    oe = np.zeros(max(int(oc.sum()), 1), dtype=np.int16)
+    E = L.orc_cmf(K, pr, cm, base["coords"], base["vnum"], base["vpsnum"], over["coords"],
+                  over["vnum"], over["vpsnum"], jf, bc, oc, be.ctypes.data_as(C.c_void_p),
+                  oe.ctypes.data_as(C.c_void_p))
+    return jf[:K], bc[:K], oc[:K], be[:int(bc.sum())], oe[:int(oc.sum())], E
+
+
+def segment_join(pairs, base, over, b_count, o_count, b_edges, o_edges):
+    """orc_segment_join with every join flag set: (seg_flag, hits)."""
+    L = lib()
+    pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+    K = len(pr) // 2
+    jf = np.ones(max(K, 1), dtype=np.int32)
+    seg = np.zeros(max(K, 1), dtype=np.int32)
+    hits = L.orc_segment_join(K, pr, jf, base["coords"], base["vpsnum"], over["coords"],
+                              over["vpsnum"], np.ascontiguousarray(b_count, np.int32),
+                              np.ascontiguousarray(o_count, np.int32),
+                              np.ascontiguousarray(b_edges, np.int16),
+                              np.ascontiguousarray(o_edges, np.int16), seg)
+    return seg[:K], hits
+

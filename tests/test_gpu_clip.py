@@ -20,6 +20,9 @@ from conftest import GOLDEN, load_poly
 pytestmark = pytest.mark.gpu
 
 AREA_RTOL = 1e-12
+# the oracle's frame mask for GCMF_CLIP_NORMALIZE: bit 0 CMBR normalisation,
+# bit 2 the edge scale (GCMF_CLIP_UNIT_FRAME drops bit 2)
+NORM = 5
 
 
 def fig_cases():
@@ -146,22 +149,22 @@ def check_results(gctx, st, cases, mode, gpu_route=False):
 def test_phase1_fig_fixtures(gctx, normalize):
     cases = fig_cases()
     st = run_batch(gctx, cases, (G.CLIP_NORMALIZE if normalize else 0) | G.CLIP_GPU_ROUTE)
-    check_phase1(gctx, st, cases, normalize)
+    check_phase1(gctx, st, cases, NORM if normalize else 0)
 
 
 def test_phase1_random_batch(gctx):
     cases = random_cases(64)
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
     assert st["ok"] == 64
-    check_phase1(gctx, st, cases, 1)
+    check_phase1(gctx, st, cases, NORM)
 
 
 def test_phase1_c1_pair(gctx):
     P, Q = G.gen_single_pair(0, 0, 2000, 1500)
     cases = [("c1", P, Q)]
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
-    check_phase1(gctx, st, cases, 1)
-    assert st["n_intersections"] == O.phase1(P, Q, 1)["psP1"][-1]
+    check_phase1(gctx, st, cases, NORM)
+    assert st["n_intersections"] == O.phase1(P, Q, NORM)["psP1"][-1]
 
 
 def test_phase1_grid_snapped(gctx):
@@ -171,8 +174,8 @@ def test_phase1_grid_snapped(gctx):
         P, Q = G.gen_single_pair(1, 50 + s, 3000, 400, snap=1e-2)
         cases.append(("snap%d" % s, P, Q))
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
-    check_phase1(gctx, st, cases, 1)
-    check_results(gctx, st, cases, 1, gpu_route=True)
+    check_phase1(gctx, st, cases, NORM)
+    check_results(gctx, st, cases, NORM, gpu_route=True)
 
 
 def snapped_cases(seed0, n=4, grid=1e-2):
@@ -192,7 +195,7 @@ def test_clip_matches_cpu_route(gctx, mode, seq):
     flags = (G.CLIP_NORMALIZE if mode & 1 else 0) | (G.CLIP_NO_CLEANUP if mode & 2 else 0)
     flags |= G.CLIP_SEQ_LABEL if seq else 0
     st = run_batch(gctx, cases, flags)
-    check_results(gctx, st, cases, mode)
+    check_results(gctx, st, cases, mode | (4 if mode & 1 else 0))
     if not seq:
         # pairs in general position take the parallel paths, not the
         # sequential fallback
@@ -208,7 +211,7 @@ def test_clip_gpu_route_results(gctx):
     route's in general position), through the wave path."""
     cases = fig_cases() + random_cases(48, seed0=77) + snapped_cases(90)
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
-    check_results(gctx, st, cases, 1, gpu_route=True)
+    check_results(gctx, st, cases, NORM, gpu_route=True)
 
 
 def test_clip_degenerate_wavy(gctx):
@@ -221,7 +224,7 @@ def test_clip_degenerate_wavy(gctx):
     cases = [("wavy", P, Q), ("wavy_snap", Ps, Qs)]
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
     assert st["ok"] == 2
-    check_results(gctx, st, cases, 1)
+    check_results(gctx, st, cases, NORM)
     st = run_batch(gctx, cases[:1], G.CLIP_NORMALIZE | G.CLIP_NO_CLEANUP)
     _, res = gctx.clip_results(st)
     A = O.area_boundary(P, Q)
@@ -321,7 +324,7 @@ def test_clip_uneven_batch(gctx):
     assert st["n_tested"] >= st["n_eval"] > 0
     for k in range(0, 3000, 37):
         a, b = oriented(P.ring(k), Q.ring(k))
-        s_ref, comps, _ = O.clip_cpu(a, b, 1)
+        s_ref, comps, _ = O.clip_cpu(a, b, NORM)
         assert status[k] == (2 if s_ref == 2 else 0)
         assert [len(c) for c in res[k]] == [len(c) for c in comps]
         for x, y in zip(res[k], comps):
@@ -336,10 +339,10 @@ def test_clip_disjoint_and_nested(gctx):
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
     status, res = gctx.clip_results(st)
     assert status[0] == 2 and res[0] == []
-    s_ref, comps, _ = O.clip_cpu(P, inner, 1)
+    s_ref, comps, _ = O.clip_cpu(P, inner, NORM)
     assert status[1] == 0 and len(res[1]) == len(comps) == 1
     np.testing.assert_array_equal(res[1][0], comps[0])
-    s_ref, comps, _ = O.clip_cpu(P, P.copy(), 1)
+    s_ref, comps, _ = O.clip_cpu(P, P.copy(), NORM)
     assert [len(c) for c in res[2]] == [len(c) for c in comps]
 
 
@@ -366,14 +369,14 @@ def test_clip_long_runs(gctx):
     cases = long_run_cases()
     for name, P, Q in cases:
         Po, Qo = oriented(P, Q)
-        ref = O.phase1(Po, Qo, 1)
+        ref = O.phase1(Po, Qo, NORM)
         side = "psP2" if name == "longP" else "psQ2"
         assert int(np.diff(ref[side]).max()) - 1 > (2048 if name == "longQ3290" else 32), name
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
-    check_phase1(gctx, st, cases, 1)
-    check_results(gctx, st, cases, 1, gpu_route=True)
+    check_phase1(gctx, st, cases, NORM)
+    check_results(gctx, st, cases, NORM, gpu_route=True)
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
-    check_results(gctx, st, cases, 1)
+    check_results(gctx, st, cases, NORM)
     for p in gctx.clip_get("label_path", len(cases)):
         assert p in (G.LP_SEGMENTS, G.LP_SEG_CLEANUP)
 
@@ -392,7 +395,7 @@ def test_clip_batched_rare_pairs(gctx, gpu_route):
     path = gctx.clip_get("label_path", len(cases))
     for k, (name, P0, Q0) in enumerate(cases):
         P, Q = oriented(P0, Q0)
-        s_ref, comps, _ = (O.clip_gpu_route if gpu_route else O.clip_cpu)(P, Q, 1)
+        s_ref, comps, _ = (O.clip_gpu_route if gpu_route else O.clip_cpu)(P, Q, NORM)
         assert status[k] == s_ref, (name, status[k], s_ref)
         assert [len(c) for c in results[k]] == [len(c) for c in comps], name
         for a, b in zip(results[k], comps):
@@ -459,3 +462,20 @@ def test_clip_block_threshold_keeps_results(gctx, big_nodes):
         gctx.set_tuning(default["cull_min"], default["big_nodes"], default["count_block"])
         dP.free()
         dQ.free()
+
+
+@pytest.mark.parametrize("flags", [G.CLIP_NORMALIZE, 0])
+def test_clip_reference_large_fixture(gctx, flags):
+    """The reference's own large pair (s.txt, 101 242 integer-coordinate
+    vertices, x c.txt, 72 997) through gcmf_clip, in the default frame and in
+    the raw frame: status 0, 47 intersections and the 11 components of
+    GH-CUDA/results/Fig-large-R.poly (sizes as GH-CUDA/ReadMe.txt:59), every
+    vertex equal to the file at its printed precision."""
+    from conftest import large_fixture, printed
+    S, Cq, R = large_fixture()
+    st = run_batch(gctx, [("large", S, Cq)], flags)
+    status, res = gctx.clip_results(st)
+    assert status[0] == 0 and st["n_intersections"] == 47
+    assert [len(c) for c in res[0]] == [len(c) for c in R]
+    for a, b in zip(res[0], R):
+        np.testing.assert_array_equal(printed(a), b)

@@ -98,7 +98,7 @@ def test_join_then_clip_pipeline(gctx):
         P, Q = base.ring(b), over.ring(o)
         if len(P) < len(Q):
             P, Q = Q, P
-        s_ref, comps, _ = O.clip_cpu(P, Q, 1)
+        s_ref, comps, _ = O.clip_cpu(P, Q, 5)
         if s_ref == 2:
             assert status[k] == 2
             continue

@@ -34,8 +34,24 @@ def test_reference_entry_points_exported():
     L = C.CDLL(os.path.join(ROOT, "gcmf-gh_amd", "libgcmf_hip.so"))
     for n in ("SortBaseMBROverlap", "GetCMBR", "PointInPolygonTest", "CountCMF", "ApplyCMF",
               "SegmentIntersectJoin", "calculateIntersections", "GH_CUDA", "ghcuda",
-              "ST_Intersect"):
+              "ST_Intersect", "spatialJoin", "gcmf_st_intersect_pairs", "gcmf_ref_context"):
         assert hasattr(L, n), n
+
+
+def test_reference_cxx_caller_links(tmp_path):
+    """A C++ caller that declares the reference's entry points as the
+    reference's headers do (C++ linkage, tests/ref_link/caller.cpp) links
+    against libgcmf_hip.so: every mangled name is exported."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = os.path.join(ROOT, "tests", "ref_link", "caller.cpp")
+    out = str(tmp_path / "caller")
+    r = subprocess.run(["g++", "-std=c++11", src, "-L" + os.path.join(ROOT, "gcmf-gh_amd"),
+                        "-lgcmf_hip", "-Wl,--allow-shlib-undefined", "-o", out],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
 
 
 def test_generators_deterministic():

@@ -285,3 +285,22 @@ def test_prefilter_check_catches_a_wrong_rule():
                   [1000, -2]], float)
     t, al, be = O.intersect_type(P[0], P[1], Q[0], Q[1])
     assert t == 7 and 0 < al < 1  # Q1's foot lies on P; Q1 itself is 1000 away
+
+
+def test_oracle_reference_large_fixture():
+    """The reference's own large pair (GH-CUDA/data/readPolygon/s.txt, 101 242
+    integer-coordinate vertices, x c.txt, 72 997) against its committed result
+    GH-CUDA/results/Fig-large-R.poly: in the normalised, edge-scaled frame
+    (mode 5, the clip's default) the oracle gives the 11 components with the
+    sizes of GH-CUDA/ReadMe.txt:59 and every vertex equal to the file at its
+    printed precision.  About 80 s: the oracle classifies all 7.4e9 edge
+    pairs."""
+    from conftest import large_fixture, printed
+    S, Cq, R = large_fixture()
+    assert [len(c) for c in R] == [5289, 5792, 1247, 1953, 1579, 4808, 4300, 3441, 6188,
+                                   9739, 5976]
+    st, comps, stats = O.clip_cpu(S, Cq, 5)
+    assert st == 0 and stats["n_inter"] == 47
+    assert [len(c) for c in comps] == [len(c) for c in R]
+    for a, b in zip(comps, R):
+        np.testing.assert_array_equal(printed(a), b)
