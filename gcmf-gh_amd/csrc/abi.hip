@@ -133,6 +133,13 @@ int gcmf_init(int device, gcmf_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess)
     ctx->c.cu_count = prop.multiProcessorCount;
+  try {
+    ctx->c.scratch = gcmf::noalloc_scratch_address();
+  } catch (const gcmf::Error &) {
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+    return GCMF_ERR_HIP;
+  }
   *out = ctx;
   return GCMF_OK;
 }
@@ -188,6 +195,16 @@ int gcmf_radix_sort_u64(gcmf_ctx *ctx, const uint64_t *d_keys_in, const int *d_v
     if (n >= 0x7fffffffL) throw gcmf::Error{GCMF_ERR_ARG, "radix sort of 2^31 keys or more"};
     gcmf::radix_sort_u64(ctx->c, d_keys_in, d_vals_in, d_keys_out, d_vals_out, n, bits);
     gcmf::check_lookback(ctx->c);
+  });
+}
+
+int gcmf_radix_sort_u64_noalloc(gcmf_ctx *ctx, const uint64_t *d_keys_in, const int *d_vals_in,
+                                uint64_t *d_keys_out, int *d_vals_out, long n, int bits) {
+  return guarded(ctx, [&] {
+    if (n < 0 || bits < 0 || bits > 64 || (n > 0 && (!d_keys_in || !d_vals_in || !d_keys_out ||
+                                                      !d_vals_out)))
+      throw gcmf::Error{GCMF_ERR_ARG, "bad radix sort arguments"};
+    gcmf::radix_sort_u64_noalloc(ctx->c, d_keys_in, d_vals_in, d_keys_out, d_vals_out, n, bits);
   });
 }
 

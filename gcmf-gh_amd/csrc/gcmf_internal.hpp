@@ -74,6 +74,9 @@ struct Context {
   std::string last_error;
   int cu_count = 256;
   long long *hbuf = nullptr;  // pinned host staging for small read-backs (kHostSlots)
+  // device address of the allocation-free sort's static scratch on this
+  // context's device (noalloc_scratch_address, looked up by gcmf_init)
+  void *scratch = nullptr;
   // a second stream (default priority) for work that overlaps the main one,
   // with the events that fork it from and join it to `stream`
   hipStream_t side = nullptr;
@@ -114,6 +117,14 @@ const long long *compact_flags_dev(Context &c, const int *flag, long n, int *out
 // significant low bits of every key.  Results land in keys_out / vals_out.
 void radix_sort_u64(Context &c, const uint64_t *keys_in, const int *vals_in, uint64_t *keys_out,
                     int *vals_out, long n, int bits);
+// The same sort with its scratch in a static device array (no allocation;
+// n <= noalloc_sort_max(); stream-ordered, not synchronised).
+void radix_sort_u64_noalloc(Context &c, const uint64_t *keys_in, const int *vals_in,
+                            uint64_t *keys_out, int *vals_out, long n, int bits);
+long noalloc_sort_max();
+size_t noalloc_scratch_bytes();
+// the scratch array's device address on the current device
+void *noalloc_scratch_address();
 // fills
 void fill_i32(Context &c, int *p, int v, long n);
 // raises an Error when a single-pass scan's look-back wait ran past its

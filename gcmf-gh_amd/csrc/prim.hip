@@ -14,6 +14,10 @@
 // every digit run's ends, run on one XCD (measured on MI355X at 2^27 keys:
 // 1.57 -> 0.75 ms per scatter pass).
 #include "gcmf_device.hpp"
+#include <cstring>
+#include <mutex>
+
+#include "../../include/gcmf_hip.h"
 #include "gcmf_internal.hpp"
 
 namespace gcmf {
@@ -1041,6 +1045,101 @@ void radix_sort_u64(Context &c, const uint64_t *keys_in, const int *vals_in, uin
                          src_k, src_v, n, 8 * p, off, tiles, dk, dv);
       HIP_CHECK(hipGetLastError());
     }
+    src_k = dk;
+    src_v = dv;
+  }
+}
+
+// ---------------------------------------------- sort without allocation ----
+// gcmf_radix_sort_u64_noalloc keeps its scratch -- the ping-pong key and value
+// buffers, the per-tile digit histograms, their scanned offsets and the
+// scan's tile partials -- in one static device array per GPU, so that the
+// call allocates nothing.  Every region is fully written before it is read
+// in every pass, so no clearing is needed; calls that overlap on one GPU
+// race on it.
+constexpr long kNoAllocSortMax = 1L << 21;                    // keys per call
+constexpr long kNoAllocTiles = kNoAllocSortMax / kSortTile;  // 512
+constexpr long kNoAllocHist = 256 * kNoAllocTiles;
+constexpr long kNoAllocParts = kNoAllocHist / kScanTile + 1;
+constexpr size_t noalloc_round(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNoAllocKeys = 0;
+constexpr size_t kNoAllocVals = kNoAllocKeys + noalloc_round(8 * kNoAllocSortMax);
+constexpr size_t kNoAllocHistAt = kNoAllocVals + noalloc_round(4 * kNoAllocSortMax);
+constexpr size_t kNoAllocOffAt = kNoAllocHistAt + noalloc_round(4 * kNoAllocHist);
+constexpr size_t kNoAllocPartAt = kNoAllocOffAt + noalloc_round(8 * (kNoAllocHist + 1));
+constexpr size_t kNoAllocBytes = kNoAllocPartAt + noalloc_round(8 * kNoAllocParts);
+static_assert(kNoAllocHist / kScanTile + 1 <= kScanPrefixTiles, "histogram scan too long");
+
+static_assert(kNoAllocBytes == (size_t)GCMF_NOALLOC_SCRATCH_BYTES, "header disagrees");
+static_assert(kNoAllocSortMax == GCMF_NOALLOC_SORT_MAX, "header disagrees");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+long noalloc_sort_max() { return kNoAllocSortMax; }
+size_t noalloc_scratch_bytes() { return kNoAllocBytes; }
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is no device address).
+void *noalloc_scratch_address() {
+  static std::mutex mu;
+  static void *addr[64] = {};
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw Error{GCMF_ERR_ARG, "device index beyond 63"};
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    addr[dev] = p;
+  }
+  return addr[dev];
+}
+
+void radix_sort_u64_noalloc(Context &c, const uint64_t *keys_in, const int *vals_in,
+                            uint64_t *keys_out, int *vals_out, long n, int bits) {
+  if (n > kNoAllocSortMax)
+    throw Error{GCMF_ERR_ARG, "gcmf_radix_sort_u64_noalloc takes at most 2^21 keys"};
+  if (!c.scratch) throw Error{GCMF_ERR_ARG, "context has no scratch address"};
+  if (n <= 0) return;
+  const int passes = (bits + 7) / 8;
+  if (passes == 0) {
+    HIP_CHECK(hipMemcpyAsync(keys_out, keys_in, sizeof(uint64_t) * (size_t)n,
+                             hipMemcpyDeviceToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(vals_out, vals_in, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice,
+                             c.stream));
+    return;
+  }
+  unsigned char *s = (unsigned char *)c.scratch;
+  uint64_t *kb = (uint64_t *)(s + kNoAllocKeys);
+  int *vb = (int *)(s + kNoAllocVals);
+  int *hist = (int *)(s + kNoAllocHistAt);
+  long long *off = (long long *)(s + kNoAllocOffAt);
+  long long *parts = (long long *)(s + kNoAllocPartAt);
+  const long tiles = (n + kSortTile - 1) / kSortTile;
+  const long nh = 256 * tiles, st = (nh + kScanTile - 1) / kScanTile;
+  const uint64_t *src_k = keys_in;
+  const int *src_v = vals_in;
+  for (int p = 0; p < passes; p++) {
+    const bool to_out = ((passes - 1 - p) % 2) == 0;
+    uint64_t *dk = to_out ? keys_out : kb;
+    int *dv = to_out ? vals_out : vb;
+    hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)tiles), dim3(kSortThreads), 0, c.stream,
+                       src_k, n, 8 * p, hist, tiles);
+    // exclusive scan of the histograms into off (the total at off[nh])
+    if (st == 1) {
+      hipLaunchKernelGGL((k_tile_scan<int, long long, long long>), dim3(1), dim3(kScanThreads),
+                         0, c.stream, (const int *)hist, nh, off, (const long long *)nullptr,
+                         off + nh);
+    } else {
+      hipLaunchKernelGGL((k_tile_reduce<int, long long>), dim3((unsigned)st), dim3(kScanThreads),
+                         0, c.stream, (const int *)hist, nh, parts);
+      hipLaunchKernelGGL((k_tile_scan_prefix<int, long long>), dim3((unsigned)st),
+                         dim3(kScanThreads), 0, c.stream, (const int *)hist, nh, off,
+                         (const long long *)parts, st);
+    }
+    hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)tiles), dim3(kSortThreads), 0, c.stream,
+                       src_k, src_v, n, 8 * p, (const long long *)off, tiles, dk, dv);
+    HIP_CHECK(hipGetLastError());
     src_k = dk;
     src_v = dv;
   }

@@ -225,6 +225,7 @@ CLIP_SEQ_LABEL = 4
 CLIP_GPU_ROUTE = 8
 CLIP_SCALE_EDGES = 16  # accepted, no effect: the edge scale is part of CLIP_NORMALIZE
 CLIP_UNIT_FRAME = 32
+NOALLOC_SORT_MAX = 1 << 21
 
 
 def hip_lib():
@@ -258,6 +259,7 @@ def hip_lib():
                                      C.POINTER(JoinStats), C.POINTER(ClipStats)]
         L.gcmf_clip_get.argtypes = [vp, C.c_int, vp, C.c_long]
         L.gcmf_radix_sort_u64.argtypes = [vp, vp, vp, vp, vp, C.c_long, C.c_int]
+        L.gcmf_radix_sort_u64_noalloc.argtypes = [vp, vp, vp, vp, vp, C.c_long, C.c_int]
         L.gcmf_scan_i32_i64.argtypes = [vp, vp, vp, C.c_long]
         L.gcmf_compact_flags.argtypes = [vp, vp, C.c_long, vp, C.POINTER(C.c_long)]
         L.gcmf_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
@@ -418,6 +420,13 @@ class Context:
         self._check(hip_lib().gcmf_radix_sort_u64(self.h, d_keys_in.ptr, d_vals_in.ptr,
                                                   d_keys_out.ptr, d_vals_out.ptr, int(n),
                                                   int(bits)), "radix_sort")
+
+    def radix_sort_noalloc(self, d_keys_in, d_vals_in, d_keys_out, d_vals_out, n, bits):
+        """The same sort with its scratch in a static device array: allocates
+        nothing, n <= NOALLOC_SORT_MAX, stream-ordered (not synchronised)."""
+        self._check(hip_lib().gcmf_radix_sort_u64_noalloc(self.h, d_keys_in.ptr, d_vals_in.ptr,
+                                                          d_keys_out.ptr, d_vals_out.ptr, int(n),
+                                                          int(bits)), "radix_sort_noalloc")
 
     def scan(self, d_in, d_out, n):
         """Exclusive scan of n i32 into n + 1 i64 (device buffers)."""

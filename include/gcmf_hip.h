@@ -73,6 +73,18 @@ int gcmf_timer_get(gcmf_ctx *ctx, const char *kernel, double *total_ms, long *la
  * (GPU_RadixSort.h:27-146, 16 decimal-digit passes there). */
 int gcmf_radix_sort_u64(gcmf_ctx *ctx, const uint64_t *d_keys_in, const int *d_vals_in,
                         uint64_t *d_keys_out, int *d_vals_out, long n, int bits);
+/* No reference counterpart.  gcmf_radix_sort_u64 for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops): the same
+ * sort, bit for bit, with its scratch (ping-pong buffers, digit histograms,
+ * offsets) in one static device array of GCMF_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends; it calls no allocating routine.
+ * n <= GCMF_NOALLOC_SORT_MAX, else GCMF_ERR_ARG.  Stream-ordered on the
+ * context's stream and not synchronised (gcmf_synchronize waits).  Calls
+ * that overlap on one GPU, from any context, race on the shared array. */
+#define GCMF_NOALLOC_SORT_MAX (1L << 21)
+#define GCMF_NOALLOC_SCRATCH_BYTES 26739456L
+int gcmf_radix_sort_u64_noalloc(gcmf_ctx *ctx, const uint64_t *d_keys_in, const int *d_vals_in,
+                                uint64_t *d_keys_out, int *d_vals_out, long n, int bits);
 /* Exclusive prefix sum of n i32 into n + 1 i64 (d_out[n] = the total, so
  * d_out[1..n] is the inclusive scan); replaces PrefixSum int->long
  * (GPU_Utility.h:128-154, GPU_PrefixSum.h:355-458). */
