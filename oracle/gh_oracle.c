@@ -301,9 +301,17 @@ static int rel_oracle(const vpool *pl, int Q, int P1, int P2, int P3) {
 
 typedef struct {
   vpool pl;
-  int rootP, rootQ;
+  int rootP, rootQ;       /* single-ring callers: the one component of each */
+  int *rP, *rQ, nRP, nRQ; /* the components (rings) of P and Q */
+  int unionOp;            /* UNION instead of intersection (polyclip.h:42) */
   int *res_roots; int nres, capres;
 } ghstate;
+
+/* a single ring on each side: its root is the only component */
+static void single_roots(ghstate *g) {
+  g->rP = &g->rootP; g->nRP = 1;
+  g->rQ = &g->rootQ; g->nRQ = 1;
+}
 
 static void res_push(ghstate *g, int root) {
   if (g->nres == g->capres) {
@@ -317,7 +325,8 @@ static void res_push(ghstate *g, int root) {
  * intersection vertices, then step 3 (polyclip.h:461-465): copy to Q */
 static void label_step1_and_3(ghstate *g) {
   vtx *v = g->pl.v;
-  int root = g->rootP, I = root;
+  for (int c = 0; c < g->nRP; c++) {
+  int root = g->rP[c], I = root;
   do {
     if (v[I].inter && v[I].nb >= 0) {
       int P_m = v[I].prev, P_p = v[I].next;
@@ -334,21 +343,48 @@ static void label_step1_and_3(ghstate *g) {
     }
     I = v[I].next;
   } while (I != root);
-  I = root;
-  do {
-    if (v[I].inter && v[I].nb >= 0) v[v[I].nb].label = v[I].label;
-    I = v[I].next;
-  } while (I != root);
+  }
+  for (int c = 0; c < g->nRP; c++) {
+    int root = g->rP[c], I = root;
+    do {
+      if (v[I].inter && v[I].nb >= 0) v[v[I].nb].label = v[I].label;
+      I = v[I].next;
+    } while (I != root);
+  }
 }
 
-static int no_crossing_vertex(const vpool *pl, int root) {
+/* polygon::noCrossingVertex, polygon.h:465-475 (bounces count as
+ * crossings for the union) */
+static int no_crossing_vertex(const vpool *pl, int root, int union_case) {
   const vtx *v = pl->v;
   int V = root;
   do {
-    if (v[V].inter && (v[V].label == L_CROSSING || v[V].label == L_DELAYED_CROSSING)) return 0;
+    if (v[V].inter) {
+      if (v[V].label == L_CROSSING || v[V].label == L_DELAYED_CROSSING) return 0;
+      if (union_case && (v[V].label == L_BOUNCING || v[V].label == L_DELAYED_BOUNCING)) return 0;
+    }
     V = v[V].next;
   } while (V != root);
   return 1;
+}
+
+/* even-odd containment in a multi-component polygon: toggled once per
+ * component that contains R (polyclip.h:505-507, 563-565) */
+static int point_in_comps(const vpool *pl, const int *roots, int n, pt R) {
+  int in = 0;
+  for (int c = 0; c < n; c++)
+    if (point_in_poly(pl, roots[c], R)) in = !in;
+  return in;
+}
+
+/* is vertex V on the ring rooted at root? */
+static int on_ring(const vpool *pl, int root, int V) {
+  int W = root;
+  do {
+    if (W == V) return 1;
+    W = pl->v[W].next;
+  } while (W != root);
+  return 0;
 }
 
 static int all_on_on(const vpool *pl, int root) {
@@ -410,10 +446,11 @@ static int non_intersection_vertex(vpool *pl, int root) {
  * createResult (:743-783), cleanUpResult (:791-807) */
 static int label_trace_cleanup(ghstate *g, int do_cleanup) {
   vpool *pl = &g->pl;
+  const int U = g->unionOp ? 1 : 0;
   int root;
-  /* 2) intersection chains */
-  root = g->rootP;
-  {
+  /* 2) intersection chains, over the components of P */
+  for (int c = 0; c < g->nRP; c++) {
+    root = g->rP[c];
     int it = root;
     do {
       vtx *v = pl->v;
@@ -437,45 +474,60 @@ static int label_trace_cleanup(ghstate *g, int do_cleanup) {
     } while (it != root);
   }
 
-  /* 3.5) components without crossings */
-  int noInter[2] = {0, 0}, ident[2] = {0, 0};
-  for (int i = 0; i < 2; i++) {
-    int Pr = i == 0 ? g->rootP : g->rootQ;
-    int Qr = i == 0 ? g->rootQ : g->rootP;
-    if (no_crossing_vertex(pl, Pr)) {
-      noInter[i] = 1;
-      if (all_on_on(pl, Pr)) {
-        ident[i] = 1;
-      } else {
-        pt p;
-        if (!non_intersection_point(pl, Pr, &p)) return -1;
-        int inside = point_in_poly(pl, Qr, p);
-        if (inside) res_push(g, Pr);
-      }
-    }
-  }
-  if (ident[0] && ident[1]) {
-    /* single components, neither is a hole: P is added when Q contains the
-     * vertex linked to P's root (polyclip.h:517-541) */
-    int V = g->rootQ, target = pl->v[g->rootP].nb, found = 0;
-    do {
-      if (V == target) { found = 1; break; }
-      V = pl->v[V].next;
-    } while (V != g->rootQ);
-    if (found) res_push(g, g->rootP);
-  }
-
-  /* 4) entry/exit flags */
+  /* 3.5) components without crossings (polyclip.h:478-541) */
+  const int *roots[2] = {g->rP, g->rQ};
+  const int nr[2] = {g->nRP, g->nRQ};
+  int *noInter[2], *ident[2];
+  int rc = 0;
   int *splitList[2] = {NULL, NULL}, nsplit[2] = {0, 0};
   int *crossList[2] = {NULL, NULL}, ncross[2] = {0, 0};
   for (int i = 0; i < 2; i++) {
-    int Pr = i == 0 ? g->rootP : g->rootQ;
-    int Qr = i == 0 ? g->rootQ : g->rootP;
-    if (noInter[i]) continue;
+    noInter[i] = (int *)calloc((size_t)nr[i] + 1, sizeof(int));
+    ident[i] = (int *)calloc((size_t)nr[i] + 1, sizeof(int));
+  }
+  for (int i = 0; i < 2; i++) {
+    for (int c = 0; c < nr[i]; c++) {
+      int Pr = roots[i][c];
+      if (no_crossing_vertex(pl, Pr, U)) {
+        noInter[i][c] = 1;
+        if (all_on_on(pl, Pr)) {
+          ident[i][c] = 1;
+        } else {
+          pt p;
+          if (!non_intersection_point(pl, Pr, &p)) { rc = -1; goto done; }
+          int inside = point_in_comps(pl, roots[1 - i], nr[1 - i], p);
+          if (inside ^ U) res_push(g, Pr);
+        }
+      }
+    }
+  }
+  /* components of P identical to one of Q: added when both or neither are
+   * holes (polyclip.h:517-541) */
+  for (int c = 0; c < nr[0]; c++) {
+    if (!ident[0][c]) continue;
+    int Pr = roots[0][c];
+    int P_isHole = 0;
+    for (int c2 = 0; c2 < nr[0]; c2++)
+      if (roots[0][c2] != Pr && point_in_poly(pl, roots[0][c2], pl->v[Pr].p)) P_isHole = !P_isHole;
+    for (int q = 0; q < nr[1]; q++) {
+      if (!ident[1][q] || !on_ring(pl, roots[1][q], pl->v[Pr].nb)) continue;
+      int Qr = roots[1][q], Q_isHole = 0;
+      for (int q2 = 0; q2 < nr[1]; q2++)
+        if (roots[1][q2] != Qr && point_in_poly(pl, roots[1][q2], pl->v[Qr].p)) Q_isHole = !Q_isHole;
+      if (P_isHole == Q_isHole) res_push(g, Pr);
+      break;
+    }
+  }
+
+  /* 4) entry/exit flags (polyclip.h:546-660) */
+  for (int i = 0; i < 2; i++) {
+    for (int c = 0; c < nr[i]; c++) {
+    if (noInter[i][c]) continue;
+    int Pr = roots[i][c];
     int V = non_intersection_vertex(pl, Pr);
-    if (V < 0) return -1;
+    if (V < 0) { rc = -1; goto done; }
     int status = ENTRY;
-    if (point_in_poly(pl, Qr, pl->v[V].p)) toggle(&status);
+    if (point_in_comps(pl, roots[1 - i], nr[1 - i], pl->v[V].p)) toggle(&status);
     int first_chain_vertex = 1;
     int I = V;
     int started = 0;
@@ -492,7 +544,7 @@ static int label_trace_cleanup(ghstate *g, int do_cleanup) {
         v[I].enex = status;
         toggle(&status);
       }
-      if (v[I].label == L_BOUNCING && status == EXIT) {
+      if (v[I].label == L_BOUNCING && ((status == EXIT) ^ U)) {
         if (!v[I].split) {
           v[I].split = 1;
           splitList[i] = (int *)realloc(splitList[i], sizeof(int) * (size_t)(nsplit[i] + 1));
@@ -502,10 +554,10 @@ static int label_trace_cleanup(ghstate *g, int do_cleanup) {
       if (v[I].label == L_DELAYED_CROSSING) {
         v[I].enex = status;
         if (first_chain_vertex) {
-          if (status == EXIT) v[I].label = L_CROSSING;
+          if ((status == EXIT) ^ U) v[I].label = L_CROSSING;
           first_chain_vertex = 0;
         } else {
-          if (status == ENTRY) v[I].label = L_CROSSING;
+          if ((status == ENTRY) ^ U) v[I].label = L_CROSSING;
           first_chain_vertex = 1;
           toggle(&status);
         }
@@ -513,14 +565,14 @@ static int label_trace_cleanup(ghstate *g, int do_cleanup) {
       if (v[I].label == L_DELAYED_BOUNCING) {
         v[I].enex = status;
         if (first_chain_vertex) {
-          if (status == EXIT && !v[I].cross) {
+          if (((status == EXIT) ^ U) && !v[I].cross) {
             v[I].cross = 1;
             crossList[i] = (int *)realloc(crossList[i], sizeof(int) * (size_t)(ncross[i] + 1));
             crossList[i][ncross[i]++] = I;
           }
           first_chain_vertex = 0;
         } else {
-          if (status == ENTRY && !v[I].cross) {
+          if (((status == ENTRY) ^ U) && !v[I].cross) {
             v[I].cross = 1;
             crossList[i] = (int *)realloc(crossList[i], sizeof(int) * (size_t)(ncross[i] + 1));
             crossList[i][ncross[i]++] = I;
@@ -531,6 +583,7 @@ static int label_trace_cleanup(ghstate *g, int do_cleanup) {
       }
       do { I = pl->v[I].next; } while (!pl->v[I].inter && I != V);
       if (I == V) break;
+    }
     }
   }
 
@@ -555,8 +608,9 @@ static int label_trace_cleanup(ghstate *g, int do_cleanup) {
     insert_vertex(pl, V_P, I_P, -1.0);
     insert_vertex(pl, V_Q, I_Q, -1.0);
     v = pl->v;
-    v[I_P].enex = EXIT; v[V_P].enex = ENTRY;
-    v[I_Q].enex = EXIT; v[V_Q].enex = ENTRY;
+    const int ex = U ? ENTRY : EXIT, en = U ? EXIT : ENTRY;
+    v[I_P].enex = ex; v[V_P].enex = en;
+    v[I_Q].enex = ex; v[V_Q].enex = en;
     v[I_P].label = L_CROSSING; v[V_P].label = L_CROSSING;
     v[I_Q].label = L_CROSSING; v[V_Q].label = L_CROSSING;
   }
@@ -571,13 +625,11 @@ static int label_trace_cleanup(ghstate *g, int do_cleanup) {
       pl->v[I_Q].label = L_CROSSING;
     }
   }
-  free(splitList[0]); free(splitList[1]);
-  free(crossList[0]); free(crossList[1]);
 
-  /* createResult, polyclip.h:743-783 */
+  /* createResult, polyclip.h:743-783, over the components of P */
   long long guard = 4LL * (long long)pl->n + 64;
-  root = g->rootP;
-  {
+  for (int c = 0; c < nr[0]; c++) {
+    root = roots[0][c];
 #define IS_CI(k) (pl->v[(k)].inter && pl->v[(k)].label == L_CROSSING)
     int I = root;
     int have = IS_CI(I);
@@ -594,17 +646,17 @@ static int label_trace_cleanup(ghstate *g, int do_cleanup) {
         int status = pl->v[V].enex;
         toggle(&status);
         do {
-          if (status == EXIT) V = pl->v[V].next;
+          if ((status == EXIT) ^ U) V = pl->v[V].next;
           else V = pl->v[V].prev;
           pl->v[V].inter = 0;
           pt p = pl->v[V].p, o = pl->v[V].o;
           int nv = vnew(pl, p, o, -1.0);
           poly_append(pl, &R, nv);
-          if (++steps > guard) return -1;
+          if (++steps > guard) { rc = -1; goto done; }
         } while (!(pl->v[V].enex == status) && (V != I));
         if (V != I) {
           V = pl->v[V].nb;
-          if (V < 0) return -1;
+          if (V < 0) { rc = -1; goto done; }
           pl->v[V].inter = 0;
         }
       } while (V != I);
@@ -614,6 +666,11 @@ static int label_trace_cleanup(ghstate *g, int do_cleanup) {
     }
 #undef IS_CI
   }
+done:
+  free(splitList[0]); free(splitList[1]);
+  free(crossList[0]); free(crossList[1]);
+  for (int i = 0; i < 2; i++) { free(noInter[i]); free(ident[i]); }
+  if (rc < 0) return rc;
 
   /* cleanUpResult, polyclip.h:791-807 */
   for (int k = 0; do_cleanup && k < g->nres; k++) {
@@ -689,36 +746,82 @@ static void result_empty(orc_result_t *out, int status) {
 /* ----------------------------------------------------------------------
  * CPU route: computeIntersections, polyclip.h:188-281
  * -------------------------------------------------------------------- */
-int orc_clip_cpu(const double *P, int nP, const double *Q, int nQ, int normalize,
-                 orc_result_t *out) {
+/* the frame of a union whose MBRs are disjoint: the joint MBR's corner and
+ * extent take the CMBR's place */
+static void joint_frame(const double *P, int nP, const double *Q, int nQ, int normalize,
+                        double *norm) {
+  double a[4], b[4];
+  ring_mbr(P, nP, a);
+  ring_mbr(Q, nQ, b);
+  double x0 = a[0] < b[0] ? a[0] : b[0], y0 = a[1] < b[1] ? a[1] : b[1];
+  double x1 = a[2] > b[2] ? a[2] : b[2], y1 = a[3] > b[3] ? a[3] : b[3];
+  norm[0] = 0.0; norm[1] = 0.0; norm[2] = 1.0;
+  if (normalize & 1) {
+    double e = (x1 - x0) > (y1 - y0) ? (x1 - x0) : (y1 - y0);
+    norm[0] = x0;
+    norm[1] = y0;
+    norm[2] = e > 0.0 ? 1.0 / e : 1.0;
+    if (normalize & 4) norm[2] *= edge_scale(nP, nQ);
+  }
+}
+
+/* CPU route over complex polygons: computeIntersections (polyclip.h:188-281,
+ * every source edge of every component of P against every source edge of
+ * every component of Q), labelIntersections with steps 1 and 3 restored,
+ * createResult, cleanUpResult.  mode: 1 normalise, 2 no clean-up, 4 edge
+ * scale, 8 union (polyclip.h:42). */
+int orc_clip_multi(const double *P, const int *pSizes, int nPR, const double *Q,
+                   const int *qSizes, int nQR, int mode, orc_result_t *out) {
+  int nP = 0, nQ = 0;
+  for (int r = 0; r < nPR; r++) nP += pSizes[r];
+  for (int r = 0; r < nQR; r++) nQ += qSizes[r];
   double cmbr[4], norm[3];
-  if (!pair_frame(P, nP, Q, nQ, normalize & 5, cmbr, norm)) {
+  if (nP < 1 || nQ < 1) {
     result_empty(out, 2);
     return 2;
+  }
+  if (!pair_frame(P, nP, Q, nQ, mode & 5, cmbr, norm)) {
+    if (!(mode & 8)) {
+      result_empty(out, 2);
+      return 2;
+    }
+    joint_frame(P, nP, Q, nQ, mode & 5, norm);
   }
   ghstate g;
   memset(&g, 0, sizeof(g));
   g.rootP = g.rootQ = -1;
-  for (int i = 0; i < nP; i++) {
-    pt o = {P[2 * i], P[2 * i + 1]};
-    int V = vnew(&g.pl, tf(o.x, o.y, norm), o, -1.0);
-    g.pl.v[V].source = 1;
-    poly_append(&g.pl, &g.rootP, V);
-  }
-  for (int i = 0; i < nQ; i++) {
-    pt o = {Q[2 * i], Q[2 * i + 1]};
-    int V = vnew(&g.pl, tf(o.x, o.y, norm), o, -1.0);
-    g.pl.v[V].source = 1;
-    poly_append(&g.pl, &g.rootQ, V);
+  g.unionOp = (mode & 8) ? 1 : 0;
+  g.rP = (int *)malloc(sizeof(int) * (size_t)(nPR + 1));
+  g.rQ = (int *)malloc(sizeof(int) * (size_t)(nQR + 1));
+  const double *src[2] = {P, Q};
+  const int *sizes[2] = {pSizes, qSizes}, nrs[2] = {nPR, nQR};
+  int *rts[2] = {g.rP, g.rQ}, *cnt[2] = {&g.nRP, &g.nRQ};
+  for (int side = 0; side < 2; side++) {
+    int off = 0;
+    for (int r = 0; r < nrs[side]; r++) {
+      int root = -1;
+      for (int i = 0; i < sizes[side][r]; i++) {
+        pt o = {src[side][2 * (off + i)], src[side][2 * (off + i) + 1]};
+        int V = vnew(&g.pl, tf(o.x, o.y, norm), o, -1.0);
+        g.pl.v[V].source = 1;
+        poly_append(&g.pl, &root, V);
+      }
+      off += sizes[side][r];
+      if (root >= 0) rts[side][(*cnt[side])++] = root;
+    }
   }
   long long n_eval = 0, n_inter = 0;
   /* source edges of P x source edges of Q; inserted vertices are skipped by
    * walking source vertices only (edgeIterator with SOURCE) */
-  int one = g.rootP;
+  for (int cp = 0; cp < g.nRP; cp++) {
+  int rootP = g.rP[cp];
+  int one = rootP;
   do {
     int two = g.pl.v[one].next;
     while (!g.pl.v[two].source) two = g.pl.v[two].next;
-    int q1 = g.rootQ;
+    for (int cq = 0; cq < g.nRQ; cq++) {
+    int rootQ = g.rQ[cq];
+    int q1 = rootQ;
     do {
       int q2 = g.pl.v[q1].next;
       while (!g.pl.v[q2].source) q2 = g.pl.v[q2].next;
@@ -767,12 +870,14 @@ int orc_clip_cpu(const double *P, int nP, const double *Q, int nQ, int normalize
         break;
       }
       q1 = q2;
-    } while (q1 != g.rootQ);
+    } while (q1 != rootQ);
+    }
     one = two;
-  } while (one != g.rootP);
+  } while (one != rootP);
+  }
 
   label_step1_and_3(&g);
-  int rc = label_trace_cleanup(&g, !(normalize & 2));
+  int rc = label_trace_cleanup(&g, !(mode & 2));
   memset(out, 0, sizeof(*out));
   if (rc < 0) {
     result_empty(out, -1);
@@ -781,8 +886,15 @@ int orc_clip_cpu(const double *P, int nP, const double *Q, int nQ, int normalize
   }
   out->n_eval = n_eval;
   out->n_inter = n_inter;
+  free(g.rP);
+  free(g.rQ);
   gh_free(&g);
   return out->status;
+}
+
+int orc_clip_cpu(const double *P, int nP, const double *Q, int nQ, int normalize,
+                 orc_result_t *out) {
+  return orc_clip_multi(P, &nP, 1, Q, &nQ, 1, normalize & 7, out);
 }
 
 /* ----------------------------------------------------------------------
@@ -1159,6 +1271,7 @@ int orc_clip_gpu_route(const double *P, int nP, const double *Q, int nQ,
   ghstate g;
   memset(&g, 0, sizeof(g));
   g.rootP = g.rootQ = -1;
+  single_roots(&g);
   int *mapP = (int *)malloc(sizeof(int) * (size_t)(t.cP ? t.cP : 1));
   int pe = 0;
   for (int s = 0; s < t.cP; s++) {
@@ -1321,6 +1434,7 @@ int orc_debug_lists(const double *P, int nP, const double *Q, int nQ, int normal
   ghstate g;
   memset(&g, 0, sizeof(g));
   g.rootP = g.rootQ = -1;
+  single_roots(&g);
   double cmbr[4], norm[3];
   if (!pair_frame(P, nP, Q, nQ, normalize, cmbr, norm)) return 2;
   if (route == 0) {

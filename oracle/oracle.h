@@ -66,6 +66,14 @@ int orc_phase1(const double *P, int nP, const double *Q, int nQ, int normalize,
 int orc_clip_cpu(const double *P, int nP, const double *Q, int nQ, int normalize,
                  orc_result_t *out);
 
+/* CPU route over complex polygons (several rings per side, polyclip.h's
+ * vector<polygon>): P packs nPR rings of pSizes[r] vertices, Q likewise.
+ * mode: bit 0 normalise, bit 1 skip cleanUpResult, bit 2 edge scale,
+ * bit 3 union instead of intersection (UNION, polyclip.h:42).  A union of
+ * polygons with disjoint MBRs is normalised by the joint MBR. */
+int orc_clip_multi(const double *P, const int *pSizes, int nPR, const double *Q,
+                   const int *qSizes, int nQR, int mode, orc_result_t *out);
+
 /* GPU route: orc_phase1 arrays spliced into lists (ghcuda.cpp:183-339), then
  * the same labelling / tracing / clean-up as the CPU route. */
 int orc_clip_gpu_route(const double *P, int nP, const double *Q, int nQ,

@@ -52,6 +52,8 @@ def lib():
         L = C.CDLL(path)
         L.orc_clip_cpu.argtypes = [f64p, C.c_int, f64p, C.c_int, C.c_int, C.POINTER(Result)]
         L.orc_clip_gpu_route.argtypes = L.orc_clip_cpu.argtypes
+        L.orc_clip_multi.argtypes = [f64p, i32p, C.c_int, f64p, i32p, C.c_int, C.c_int,
+                                     C.POINTER(Result)]
         L.orc_result_free.argtypes = [C.POINTER(Result)]
         L.orc_phase1.argtypes = [f64p, C.c_int, f64p, C.c_int, C.c_int, C.POINTER(Phase1)]
         L.orc_phase1_free.argtypes = [C.POINTER(Phase1)]
@@ -91,6 +93,9 @@ def ref_lib():
         L = C.CDLL(path)
         L.ref_clip.argtypes = [f64p, C.c_int, f64p, C.c_int, f64p, C.c_int, i32p, C.c_int,
                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.ref_clip_multi.argtypes = [f64p, i32p, C.c_int, f64p, i32p, C.c_int, C.c_int, f64p,
+                                     C.c_int, i32p, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]
         _ref = L
     return _ref
 
@@ -119,6 +124,52 @@ def clip_cpu(P, Q, normalize=True):
     out = (r.status, _unpack(r), {"n_eval": r.n_eval, "n_inter": r.n_inter})
     L.orc_result_free(C.byref(r))
     return out
+
+
+def _rings(rings):
+    rings = [np.asarray(r, dtype=np.float64).reshape(-1, 2) for r in rings]
+    xy = np.ascontiguousarray(np.concatenate(rings).reshape(-1)) if rings else np.zeros(0)
+    return xy, np.array([len(r) for r in rings], dtype=np.int32)
+
+
+UNION = 8  # mode bit of clip_multi
+
+
+def clip_multi(Ps, Qs, mode=5):
+    """CPU route over complex polygons (lists of rings).  mode bits: 1
+    normalise, 2 no clean-up, 4 edge scale, 8 union.  Returns (status,
+    [component arrays], stats)."""
+    L = lib()
+    p, ps = _rings(Ps)
+    q, qs = _rings(Qs)
+    r = Result()
+    L.orc_clip_multi(p, ps, len(ps), q, qs, len(qs), int(mode), C.byref(r))
+    out = (r.status, _unpack(r), {"n_eval": r.n_eval, "n_inter": r.n_inter})
+    L.orc_result_free(C.byref(r))
+    return out
+
+
+def ref_clip_multi(Ps, Qs, union=False):
+    """The reference's CPU GH over complex polygons (raw coordinates)."""
+    L = ref_lib()
+    if L is None:
+        return None
+    p, ps = _rings(Ps)
+    q, qs = _rings(Qs)
+    cap = 8 * (len(p) + len(q)) + 64
+    xy = np.zeros(2 * cap, dtype=np.float64)
+    sizes = np.zeros(cap, dtype=np.int32)
+    nc, nv = C.c_int(), C.c_int()
+    rc = L.ref_clip_multi(p, ps, len(ps), q, qs, len(qs), int(union), xy, cap, sizes, cap,
+                          C.byref(nc), C.byref(nv))
+    if rc != 0:
+        raise RuntimeError("reference clip output overflow")
+    comps, off = [], 0
+    for c in range(nc.value):
+        n = int(sizes[c])
+        comps.append(xy[2 * off:2 * (off + n)].reshape(-1, 2).copy())
+        off += n
+    return comps
 
 
 def clip_gpu_route(P, Q, normalize=True):

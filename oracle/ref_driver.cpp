@@ -48,25 +48,33 @@ void restored_step3() {
 }
 }  // namespace
 
-extern "C" int ref_clip(const double *P, int nP, const double *Q, int nQ,
-                        double *out_xy, int out_cap, int *comp_sizes,
-                        int comp_cap, int *ncomp, int *nverts) {
+// Complex polygons: P has nPR components of pSizes[r] vertices each (packed
+// in P), Q likewise; union_op selects the union (UNION, polyclip.h:42).
+extern "C" int ref_clip_multi(const double *P, const int *pSizes, int nPR, const double *Q,
+                              const int *qSizes, int nQR, int union_op, double *out_xy,
+                              int out_cap, int *comp_sizes, int comp_cap, int *ncomp,
+                              int *nverts) {
   NullBuf nb;
   std::streambuf *old = std::cout.rdbuf(&nb);
   pPolygon.clear();
   qPolygon.clear();
   resultPolygon.clear();
-  polygon p, q;
-  for (int i = 0; i < nP; i++) {
-    point2D v(P[2 * i], P[2 * i + 1]);
-    p.newVertex(v, true);
+  UNION = union_op != 0;
+  const double *src[2] = {P, Q};
+  const int *sizes[2] = {pSizes, qSizes};
+  const int nr[2] = {nPR, nQR};
+  for (int side = 0; side < 2; side++) {
+    int off = 0;
+    for (int r = 0; r < nr[side]; r++) {
+      polygon p;
+      for (int i = 0; i < sizes[side][r]; i++) {
+        point2D v(src[side][2 * (off + i)], src[side][2 * (off + i) + 1]);
+        p.newVertex(v, true);
+      }
+      off += sizes[side][r];
+      (side == 0 ? pPolygon : qPolygon).push_back(p);
+    }
   }
-  for (int i = 0; i < nQ; i++) {
-    point2D v(Q[2 * i], Q[2 * i + 1]);
-    q.newVertex(v, true);
-  }
-  pPolygon.push_back(p);
-  qPolygon.push_back(q);
   computeIntersections();
   restored_step1();
   restored_step3();
@@ -96,6 +104,14 @@ extern "C" int ref_clip(const double *P, int nP, const double *Q, int nQ,
   resultPolygon.clear();
   pPolygon.clear();
   qPolygon.clear();
+  UNION = false;
   std::cout.rdbuf(old);
   return rc;
+}
+
+extern "C" int ref_clip(const double *P, int nP, const double *Q, int nQ,
+                        double *out_xy, int out_cap, int *comp_sizes,
+                        int comp_cap, int *ncomp, int *nverts) {
+  return ref_clip_multi(P, &nP, 1, Q, &nQ, 1, 0, out_xy, out_cap, comp_sizes, comp_cap, ncomp,
+                        nverts);
 }
