@@ -27,75 +27,182 @@ import numpy as np  # noqa: E402
 METRIC = "clipped polygon pairs/s + join candidates/s at ≤1e-9 err; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
-# algorithmic FP64 operations per classified edge pair: four orientation
-# determinants (7 flops each), two subtractions and two divisions
+# FP64 work of the count kernel per edge pair it classifies: four
+# orientation determinants (7 flops each), two subtractions and two
+# divisions; per intersection point it emits, 7 flops for (1-a)P1 + aP2
 FLOPS_PER_EVAL = 32
-# per edge pair the count kernel visits: the eight FP64 compares of the
-# box-reject test (a lower bound: pairs it does not reject cost more)
-FLOPS_PER_TEST = 8
-# the dominant kernel, timed live with HIP events inside the timed region
-ROOF_KERNEL = "gh_count"
+FLOPS_PER_POINT = 7
+# the two largest kernels of the clip, timed live with HIP events inside the
+# timed region
+COUNT_KERNEL = "gh_count"
+LABEL_KERNEL = "gh_label_trace"
+LIVE_KERNELS = COUNT_KERNEL + "," + LABEL_KERNEL
 BREAKDOWN_STEPS = 3
 PROFILES = os.path.join(ROOT, "profiles")
 
 
-def traffic_bytes(workload, args):
-    """HBM bytes per launch of the roofline kernel measured with rocprofv3
-    FETCH_SIZE / WRITE_SIZE (scaled by a plain copy's reading) for this
-    workload at its default size, from profiles/; None otherwise."""
-    fn = os.path.join(PROFILES, "k_count_cpu_traffic.json")
+def traffic_bytes(kernel, workload, size):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (each scaled by a plain copy's reading) for this workload at
+    this size, from profiles/<kernel>_traffic.json; None otherwise."""
+    fn = os.path.join(PROFILES, "%s_traffic.json" % kernel)
     if not os.path.exists(fn):
         return None
     with open(fn) as f:
         t = json.load(f)
     rec = t.get(workload)
-    if not rec or rec.get("size") != (args.n if workload == "join-clip" else args.pairs):
+    if not rec or rec.get("size") != size:
         return None
     return rec["bytes_per_launch"]
 
 
-def roofline_line(total_ms, launches, cs, traffic):
-    """FP64 work of the count kernel: 8 compares per edge pair it visits
-    (the box-reject test) plus 32 flops per edge pair it classifies (four
-    7-flop orientation determinants, two subtractions, two divisions).  The
-    kernel has no matrix-shaped work: the peak is MI355X's FP64 rate, which
-    its vector units share with the matrix cores."""
-    n_eval, n_tested = cs["n_eval"], cs["n_tested"]
+def count_roofline(total_ms, launches, cs, traffic):
+    """The count kernel (k_count_cpu_lds / k_count_cpu) priced as FP64 vector
+    work: 32 flops per edge pair it classifies plus 7 per intersection
+    point, against MI355X's 78.6 TFLOP/s FP64 peak.  Box rejects are not
+    counted: they are the kernel's overhead, not the algorithm's work."""
     avg_ms = total_ms / max(launches, 1)
-    flops = FLOPS_PER_EVAL * n_eval + FLOPS_PER_TEST * n_tested
+    flops = FLOPS_PER_EVAL * cs["n_eval"] + FLOPS_PER_POINT * cs["n_intersections"]
     achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
-    return {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS,
+    return {"bound": "fp64-valu", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFS,
             "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFS, 6), "traffic": traffic,
-            "kernel": "k_count_cpu", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
-            "work": "%d edge pairs tested x %d FP64 compares + %d classified x %d FP64 flops"
-                    % (n_tested, FLOPS_PER_TEST, n_eval, FLOPS_PER_EVAL)}
+            "kernel": "k_count_cpu_lds", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+            "work": "%d classified edge pairs x %d + %d intersection points x %d FP64 flops; "
+                    "%d edge pairs visited" % (cs["n_eval"], FLOPS_PER_EVAL,
+                                                cs["n_intersections"], FLOPS_PER_POINT,
+                                                cs["n_tested"])}
 
 
-def cpu_baseline(base, over, joined, sample_pairs):
-    """Oracle (C restatement of the reference CPU path) on one core: the full
-    join of both layers plus the clip of `sample_pairs` joined pairs, scaled
-    to pairs/s by the clip share."""
+def label_bytes(cs):
+    """Algorithmic HBM bytes of labelling and tracing one batch: every slot
+    (ring vertex or inserted intersection) read once -- its normalised point
+    (16 B), original point (16 B) and neighbour link (4 B) --, every edge's
+    first slot (8 B) to flag the source vertices, and the result written
+    once (16 B per vertex, 4 B per component) with 8 B of counts per pair."""
+    return (36 * (cs["slots_p"] + cs["slots_q"]) + 8 * (cs["edges_p"] + cs["edges_q"]) +
+            16 * cs["total_verts"] + 4 * cs["total_comps"] + 8 * cs["pairs"])
+
+
+def label_roofline(total_ms, launches, cs, traffic):
+    avg_ms = total_ms / max(launches, 1)
+    b = label_bytes(cs)
+    achieved = b / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+            "kernel": "k_label_wave", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+            "work": "%d algorithmic bytes per launch: 36 B x %d slots + 8 B x %d edges + "
+                    "16 B x %d result vertices + 4 B x %d components + 8 B x %d pairs"
+                    % (b, cs["slots_p"] + cs["slots_q"], cs["edges_p"] + cs["edges_q"],
+                       cs["total_verts"], cs["total_comps"], cs["pairs"])}
+
+
+# ------------------------------------------------ CPU path (the oracle) ----
+_W = {}
+
+
+def _oracle_worker_init(n, seed):
+    """Pool worker: the same layers, regenerated from the seed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import gcmf_amd as G
+    _W["layers"] = G.gen_join_layers(n, n, seed=seed)
+
+
+def _clip_pairs(base, over, pairs):
+    import oracle_py as O
+    out = []
+    for b, o in pairs:
+        P, Q = base.ring(int(b)), over.ring(int(o))
+        if len(P) < len(Q):
+            P, Q = Q, P
+        st, comps, _ = O.clip_cpu(P, Q, 5)  # the GCMF_CLIP_NORMALIZE frame
+        out.append((st, comps))
+    return out
+
+
+def _digest(st, comps):
+    import hashlib
+    h = hashlib.sha1()
+    for c in comps:
+        h.update(np.ascontiguousarray(c, np.float64).tobytes())
+    return st, tuple(len(c) for c in comps), h.hexdigest()
+
+
+def _oracle_worker_clip(pairs):
+    """Clips a chunk; returns digests, so that the timed region is the
+    clipping and not the transfer of the results."""
+    base, over = _W["layers"]
+    return [_digest(st, comps) for st, comps in _clip_pairs(base, over, pairs)]
+
+
+def cpu_reference(base, over, n, seed):
+    """The CPU path (the repository's C restatement of the reference's CPU
+    route) over the whole workload: the join (MBR sweep, CMBR, PiP, CMF,
+    segment refine) and the clip of every joined pair, on one core and then
+    with the clips spread over a pool of worker processes (the join stays on
+    one core).  Returns the baseline record and the oracle's outputs."""
+    import multiprocessing as mp
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
     t0 = time.perf_counter()
-    _, pairs = O.mbr_sweep(base.mbr, over.mbr)
-    O.join_refine(pairs, base.as_dict(), over.as_dict())
+    _, cand = O.mbr_sweep(base.mbr, over.mbr)
+    ref = O.join_refine(cand, base.as_dict(), over.as_dict())
+    joined = cand[(ref["pip_flag"] == 1) | (ref["seg_flag"] == 1)]
     t_join = time.perf_counter() - t0
-    idx = np.linspace(0, len(joined) - 1, min(sample_pairs, len(joined))).astype(int)
+    print("cpu: join %.2f s, %d joined pairs" % (t_join, len(joined)), file=sys.stderr,
+          flush=True)
     t0 = time.perf_counter()
-    for k in idx:
-        b, o = joined[k]
-        P, Q = base.ring(b), over.ring(o)
-        if len(P) < len(Q):
-            P, Q = Q, P
-        O.clip_cpu(P, Q, 5)
-    t_clip = time.perf_counter() - t0
-    per_pair = t_clip / len(idx)
-    t_step = t_join + per_pair * len(joined)
-    return {"value": len(joined) / t_step, "unit": "pairs/s", "cores": 1, "kind": "port",
-            "sample": "full join of both layers (%.2f s) + clip of %d of %d joined pairs "
-                      "(%.2f s), clip time scaled to all pairs" % (t_join, len(idx), len(joined),
-                                                                   t_clip)}
+    results = _clip_pairs(base, over, joined)
+    t_clip1 = time.perf_counter() - t0
+    print("cpu: clip on one core %.2f s" % t_clip1, file=sys.stderr, flush=True)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    W = max(1, min(16, cores))
+    chunks = [joined[i::4 * W] for i in range(4 * W)]
+    with mp.get_context("spawn").Pool(W, initializer=_oracle_worker_init,
+                                       initargs=(n, seed)) as pool:
+        pool.map(len, [[0]] * W)  # workers up
+        t0 = time.perf_counter()
+        parts = pool.map(_oracle_worker_clip, chunks)
+        t_clipW = time.perf_counter() - t0
+    # the pool's results are the single-core run's (interleaved back)
+    same = True
+    for i, part in enumerate(parts):
+        for j, d in enumerate(part):
+            same &= d == _digest(*results[i + 4 * W * j])
+    print("cpu: clip on %d processes %.2f s" % (W, t_clipW), file=sys.stderr, flush=True)
+    rec = {"value": round(len(joined) / (t_join + t_clipW), 2), "unit": "pairs/s", "cores": W,
+           "kind": "port",
+           "sample": "the whole workload, no sampling: the join of both layers (%.2f s, one "
+                     "core) and the clip of all %d joined pairs (%.2f s over %d processes, "
+                     "%.2f s on one core)" % (t_join, len(joined), t_clipW, W, t_clip1),
+           "one_core": {"value": round(len(joined) / (t_join + t_clip1), 2), "unit": "pairs/s",
+                        "cores": 1, "seconds": round(t_join + t_clip1, 3)},
+           "pool_matches_one_core": bool(same)}
+    return rec, cand, joined, results
+
+
+def parity(ctx, js, cs, cand, joined, results):
+    """The GPU outputs of the last timed step against the oracle's: the
+    candidate pair list (order included), the joined list, and every
+    clipped pair's status, component sizes and vertices, bit for bit."""
+    g_cand = ctx.join_get("pairs", 2 * js["candidates"]).reshape(-1, 2)
+    g_join = ctx.join_get("joined", 2 * js["joined"]).reshape(-1, 2)
+    status, res = ctx.clip_results(cs)
+    mism = 0
+    if len(res) == len(results):
+        for k, (st, comps) in enumerate(results):
+            ok = int(status[k]) == st and len(res[k]) == len(comps) and all(
+                np.array_equal(a, b) for a, b in zip(res[k], comps))
+            mism += 0 if ok else 1
+    else:
+        mism = max(len(res), len(results))
+    return {"candidates_equal": bool(np.array_equal(g_cand, cand)),
+            "joined_equal": bool(np.array_equal(g_join, joined)),
+            "pairs_compared": len(results), "pairs_mismatched": mism,
+            "checked": "candidate list, joined list and every clipped pair (status, component "
+                       "sizes, vertex coordinates) bit-exact against the oracle"}
 
 
 def copy_calibration():
@@ -115,7 +222,6 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=20000, help="polygons per layer")
-    ap.add_argument("--cpu-sample", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["join-clip", "batched", "large", "serve", "prims"],
                     default="join-clip",
@@ -132,8 +238,8 @@ def main():
     ap.add_argument("--snap", type=float, default=0.0,
                     help="grid for --workload large (1e-3: the degenerate variant)")
     ap.add_argument("--reference-eps", action="store_true",
-                    help="--workload large without GCMF_CLIP_SCALE_EDGES (the reference's "
-                         "absolute epsilon in the CMBR frame)")
+                    help="--workload large in the unit CMBR frame (GCMF_CLIP_UNIT_FRAME: the "
+                         "reference's absolute epsilon without the edge scale)")
     ap.add_argument("--pairs", type=int, default=200000, help="pairs for --workload batched")
     ap.add_argument("--no-tune", action="store_true",
                     help="batched: keep the default shapes instead of autotuning them")
@@ -161,15 +267,19 @@ def main():
     import gcmf_amd as G
     print("generating layers ...", file=sys.stderr, flush=True)
     base, over = G.gen_join_layers(args.n, args.n, seed=0)
+    cpu = None
+    if not args.no_cpu:
+        # before the GPU is touched: the pool's processes start clean
+        cpu = cpu_reference(base, over, args.n, 0)
     ctx = G.Context(0)
     db, do = G.DeviceLayer(ctx, base), G.DeviceLayer(ctx, over)
     flags = G.CLIP_NORMALIZE
     for _ in range(args.warmup):
         js, cs = ctx.join_clip(db, do, flags)
     ctx.synchronize()
-    # timed region: only the roofline kernel carries events
+    # timed region: only the count and label kernels carry events
     ctx.timer_reset()
-    ctx.set_timing(True, only=ROOF_KERNEL)
+    ctx.set_timing(True, only=LIVE_KERNELS)
     lat = []
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -181,7 +291,9 @@ def main():
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     ctx.set_timing(False)
-    roof_ms, roof_n = ctx.timer(ROOF_KERNEL)
+    cnt_ms, cnt_n = ctx.timer(COUNT_KERNEL)
+    lab_ms, lab_n = ctx.timer(LABEL_KERNEL)
+    check = parity(ctx, js, cs, *cpu[1:]) if cpu else None
     # per-kernel breakdown from separate, fully timed steps
     ctx.timer_reset()
     ctx.set_timing(True)
@@ -201,7 +313,11 @@ def main():
     ms_step = 1e3 * elapsed / args.steps
     pairs = cs["pairs"]
     value = pairs / (elapsed / args.steps)
-    roofline = roofline_line(roof_ms, roof_n, cs, traffic_bytes("join-clip", args))
+    size = args.n
+    roofs = [count_roofline(cnt_ms, cnt_n, cs, traffic_bytes("k_count_cpu_lds", "join-clip",
+                                                              size)),
+             label_roofline(lab_ms, lab_n, cs, traffic_bytes("k_label_wave", "join-clip", size))]
+    roofs.sort(key=lambda r: -r["avg_launch_ms"])  # the dominant kernel first
     line = {"metric": METRIC, "value": round(value, 2), "unit": "pairs/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -219,10 +335,10 @@ def main():
             "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 4),
             "kernel_ms_per_step": {k: round(v[0] / BREAKDOWN_STEPS, 4)
                                    for k, v in times.items() if v[1]},
-            "roofline": roofline}
-    if not args.no_cpu:
-        joined = ctx.join_get("joined", 2 * js["joined"]).reshape(-1, 2)
-        line["cpu_baseline"] = cpu_baseline(base, over, joined, args.cpu_sample)
+            "roofline": roofs[0], "roofline_next": roofs[1]}
+    if cpu:
+        line["cpu_baseline"] = cpu[0]
+        line["parity"] = check
     db.free()
     do.free()
     ctx.close()
@@ -257,7 +373,7 @@ def batched(args):
     for _ in range(args.warmup):
         cs = step()
     ctx.timer_reset()
-    ctx.set_timing(True, only=ROOF_KERNEL)
+    ctx.set_timing(True, only=LIVE_KERNELS)
     lat = []
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -269,7 +385,8 @@ def batched(args):
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     ctx.set_timing(False)
-    roof_ms, roof_n = ctx.timer(ROOF_KERNEL)
+    cnt_ms, cnt_n = ctx.timer(COUNT_KERNEL)
+    lab_ms, lab_n = ctx.timer(LABEL_KERNEL)
     ctx.timer_reset()
     ctx.set_timing(True)
     step()
@@ -329,8 +446,8 @@ def batched(args):
                               "result_mismatches_vs_batch": mism,
                               "batched_speedup": round(args.pairs / (elapsed / args.steps)
                                                        / per_pair_rate, 2)},
-            "roofline": roofline_line(roof_ms, roof_n, cs,
-                                      traffic_bytes("batched", args))}
+            "roofline": label_roofline(lab_ms, lab_n, cs, None),
+            "roofline_next": count_roofline(cnt_ms, cnt_n, cs, None)}
     d_pairs.free()
     dP.free()
     dQ.free()
@@ -350,11 +467,11 @@ def large(args):
     base, over = G.Layer.from_rings([P]), G.Layer.from_rings([Q])
     db, do = G.DeviceLayer(ctx, base), G.DeviceLayer(ctx, over)
     pairs = np.array([[0, 0]], np.int32)
-    flags = G.CLIP_NORMALIZE | (0 if args.reference_eps else G.CLIP_SCALE_EDGES)
+    flags = G.CLIP_NORMALIZE | (G.CLIP_UNIT_FRAME if args.reference_eps else 0)
     for _ in range(args.warmup):
         cs = ctx.clip(db, do, pairs, flags)
     ctx.timer_reset()
-    ctx.set_timing(True, only=ROOF_KERNEL)
+    ctx.set_timing(True, only=LIVE_KERNELS)
     lat = []
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -366,7 +483,8 @@ def large(args):
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     ctx.set_timing(False)
-    roof_ms, roof_n = ctx.timer(ROOF_KERNEL)
+    cnt_ms, cnt_n = ctx.timer(COUNT_KERNEL)
+    lab_ms, lab_n = ctx.timer(LABEL_KERNEL)
     ctx.timer_reset()
     ctx.set_timing(True)
     ctx.clip(db, do, pairs, flags)
@@ -387,7 +505,8 @@ def large(args):
             "config": {"workload": "one pair of %d x %d vertices (six-sinusoid radii, "
                                    "snap %g), FP64, seed 0" % (len(P), len(Q), args.snap),
                        "predicate_frame": "CMBR-normalised" + (
-                           "" if args.reference_eps else ", edge-scaled (GCMF_CLIP_SCALE_EDGES)"),
+                           ", unit frame (GCMF_CLIP_UNIT_FRAME)" if args.reference_eps
+                           else ", edge-scaled (the default)"),
                        "status": status, "label_path": path,
                        "result_components": cs["total_comps"],
                        "result_vertices": cs["total_verts"],
@@ -397,7 +516,8 @@ def large(args):
             "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 3),
             "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 3),
             "kernel_ms_per_step": kernel_ms,
-            "roofline": roofline_line(roof_ms, roof_n, cs, None)}
+            "roofline": label_roofline(lab_ms, lab_n, cs, None),
+            "roofline_next": count_roofline(cnt_ms, cnt_n, cs, None)}
     db.free()
     do.free()
     ctx.close()

@@ -305,23 +305,45 @@ int gcmf_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
   });
 }
 
+int gcmf_clip_complex(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+                      const int *d_parts, long M, int flags, gcmf_clip_stats *stats) {
+  return guarded(ctx, [&] {
+    if (!base || !over || (M > 0 && !d_parts) || M < 0)
+      throw gcmf::Error{GCMF_ERR_ARG, "bad clip arguments"};
+    ctx->clip_valid = false;
+    gcmf::ClipOptions opt;
+    opt.normalize = (flags & GCMF_CLIP_NORMALIZE) ? ((flags & GCMF_CLIP_UNIT_FRAME) ? 1 : 5) : 0;
+    opt.cleanup = (flags & GCMF_CLIP_NO_CLEANUP) ? 0 : 1;
+    gcmf::clip_complex(ctx->c, to_inputs(base, over), d_parts, M, opt,
+                       (flags & GCMF_CLIP_UNION) != 0, ctx->cr);
+    ctx->p1 = gcmf::Phase1View();
+    ctx->p1.M = M;
+    ctx->c.flush_timers();
+    ctx->clip_valid = true;
+    fill_clip_stats(ctx, stats);
+  });
+}
+
 int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out) {
   return guarded(ctx, [&] {
     if (!out) throw gcmf::Error{GCMF_ERR_ARG, "null tuning"};
     out->cull_min = ctx->c.tune.cullMin;
     out->big_nodes = ctx->c.tune.bigNodes;
     out->count_block = ctx->c.tune.countBlock;
+    out->lds_max_q = ctx->c.tune.ldsMaxQ;
   });
 }
 
 int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
   return guarded(ctx, [&] {
     if (!t || t->cull_min < 1 || t->big_nodes < 1 ||
-        (t->count_block != 64 && t->count_block != 128 && t->count_block != 256))
+        (t->count_block != 64 && t->count_block != 128 && t->count_block != 256) ||
+        t->lds_max_q < 0 || t->lds_max_q > 1024)
       throw gcmf::Error{GCMF_ERR_ARG, "bad tuning"};
     ctx->c.tune.cullMin = t->cull_min;
     ctx->c.tune.bigNodes = t->big_nodes;
     ctx->c.tune.countBlock = t->count_block;
+    ctx->c.tune.ldsMaxQ = t->lds_max_q;
   });
 }
 
@@ -370,6 +392,15 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
     }
     {
       double bt = 1e300;
+      for (int v : {0, 1024}) {
+        c.tune = best;
+        c.tune.ldsMaxQ = v;
+        const double t = measure("gh_count");
+        if (t < bt) bt = t, best.ldsMaxQ = v;
+      }
+    }
+    {
+      double bt = 1e300;
       for (long long v : {2048LL, 4096LL, 8192LL, 16384LL, 65536LL}) {
         c.tune = best;
         c.tune.bigNodes = v;
@@ -385,6 +416,7 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
       out->cull_min = best.cullMin;
       out->big_nodes = best.bigNodes;
       out->count_block = best.countBlock;
+      out->lds_max_q = best.ldsMaxQ;
     }
   });
 }

@@ -106,6 +106,12 @@ struct Phase1View {
 void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                 const ClipOptions &opt, ClipResult &res, Phase1View *p1);
 
+// Clips M pairs of complex polygons: parts[4k..4k+3] = first base ring,
+// base ring count, first overlay ring, overlay ring count (at most 64 rings
+// a side); the union instead of the intersection when unionOp.
+void clip_complex(Context &c, const JoinInputs &in, const int *parts, long M,
+                  const ClipOptions &opt, bool unionOp, ClipResult &res);
+
 }  // namespace gcmf
 
 // The state behind one gcmf_ctx handle of the C ABI: the context and the

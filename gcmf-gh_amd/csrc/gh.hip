@@ -155,11 +155,14 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
     if (threadIdx.x < 8) lmax[threadIdx.x >> 2][threadIdx.x & 3] = 0;
     __syncthreads();
     if (k < M) {
+      // the largest P and Q rings as the pairs are oriented below (P the
+      // longer ring unless noswap keeps P the base ring)
       const int b = pairs[2 * k], o = pairs[2 * k + 1];
       const int nb = bVNum[b] - 1, no = oVNum[o] - 1;
+      const bool sw = !noswap && nb < no;
       if (__lane_id() == 0) {
-        lmax[0][threadIdx.x >> 6] = nb > no ? nb : no;
-        lmax[1][threadIdx.x >> 6] = nb > no ? no : nb;
+        lmax[0][threadIdx.x >> 6] = max(sw ? no : nb, 0);
+        lmax[1][threadIdx.x >> 6] = max(sw ? nb : no, 0);
       }
     }
     __syncthreads();
@@ -178,8 +181,9 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
     int a = 0, q = 0;
     if (k < M) {
       const int nb = bVNum[pairs[2 * k]] - 1, no = oVNum[pairs[2 * k + 1]] - 1;
-      a = max(max(nb, no), 0);
-      q = max(min(nb, no), 0);
+      const bool sw = !noswap && nb < no;
+      a = max(sw ? no : nb, 0);
+      q = max(sw ? nb : no, 0);
     }
     blockMax[2 * blockIdx.x] = (unsigned long long)a;
     blockMax[2 * blockIdx.x + 1] = (unsigned long long)q;
@@ -1378,7 +1382,8 @@ __device__ __forceinline__ void count_edge_cpu(long g, long edgesP, long edgesQ,
                                                int *cnt2P, int *cnt1Q, int *cnt2Q, int *boolP,
                                                int *boolQ, int *lstP, int *lstQ,
                                                const RunBoxes &boxP, const RunBoxes &boxQ,
-                                               unsigned long long &ev, unsigned long long &tv) {
+                                               int ldsQ, unsigned long long &ev,
+                                               unsigned long long &tv) {
   const bool sideP = g < edgesP;
   const long ga = sideP ? g : g - edgesP;
   const int k = sideP ? mapP[ga] : mapQ[ga];
@@ -1386,6 +1391,8 @@ __device__ __forceinline__ void count_edge_cpu(long g, long edgesP, long edgesQ,
   const long long aN = (sideP ? offP[k + 1] : offQ[k + 1]) - aBase;
   const long long bBase = sideP ? offQ[k] : offP[k];
   const long long bN = (sideP ? offQ[k + 1] : offP[k + 1]) - bBase;
+  // P edges of pairs whose Q ring the LDS pass (k_count_cpu_lds) takes
+  if (sideP && bN > boxP.cullMin && bN <= ldsQ) return;
   const double *A = sideP ? ringP : ringQ;
   const double *Bv = (sideP ? ringQ : ringP) + 2 * bBase;
   const long long i = ga - aBase;
@@ -1444,15 +1451,239 @@ __global__ void __launch_bounds__(256)
                 const double *ringQ, const double *lenP, const double *mrgP, const double *lenQ,
                 const double *mrgQ, const double *cf, int *cnt1P, int *cnt2P, int *cnt1Q,
                 int *cnt2Q, int *boolP, int *boolQ, unsigned long long *wstat, int *lstP,
-                int *lstQ, RunBoxes boxP, RunBoxes boxQ) {
+                int *lstQ, RunBoxes boxP, RunBoxes boxQ, int ldsQ) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long ev = 0, tv = 0;
   if (g < edgesP + edgesQ)
     count_edge_cpu<CULL>(g, edgesP, edgesQ, mapP, mapQ, offP, offQ, ringP, ringQ, lenP, mrgP,
                          lenQ, mrgQ, cf, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, lstP, lstQ,
-                         boxP, boxQ, ev, tv);
+                         boxP, boxQ, ldsQ, ev, tv);
   // every lane of every wave reaches the statistics
   wave_stats(wstat, g, ev, tv);
+}
+
+// ---------------------------------------- count pass, Q binned in LDS ----
+// The CPU route's count pass for the pairs whose Q ring is longer than the
+// culling threshold and fits in LDS (Tuning::cullMin < nQ <= Tuning::ldsMaxQ
+// <= kLdsQ): one block per pair.  (Shorter rings keep the lockstep walk of
+// k_count_cpu, whose loads every lane of a wave shares: on the join
+// workload, rings of at most 512 vertices, this pass visited 9.4 times
+// fewer edge pairs but took 0.81 ms against the walk's 0.43 ms on MI355X,
+// the per-pair staging and binning and the lanes' divergent walks costing
+// more than the visits saved.)  The block stages
+// the Q ring in LDS and bins its edges in a g x g grid over Q's box
+// (g = ceil(sqrt(nQ / 2)), at most kLdsG); an edge whose box spans more
+// than kLdsSpan cells goes to a list that every P edge walks.  A P edge then
+// visits only the edges binned in the cells its box widened by G covers,
+// each edge once (from the first covered cell of its own cell range),
+// instead of the whole ring.  Cell indices are a monotone function of the
+// coordinate, so an edge outside those cells lies beyond the widened box in
+// x or y, and CpuFilter::skip rejects it by its first test: counts, flags and
+// partner sets are the ring walk's.  Partners come out of index order; a
+// list of at most kPartners is sorted before it is stored (longer lists are
+// rebuilt in index order by k_partner_rescan_cpu).  A P edge whose filter
+// admits tiny pairs walks the whole ring from LDS, in index order.
+constexpr int kLdsQ = 1024;   // largest Q ring the LDS pass takes
+constexpr int kLdsG = 23;     // largest grid side: ceil(sqrt(kLdsQ / 2))
+constexpr int kLdsSpan = 4;   // edges spanning more cells go to the long list
+constexpr int kLdsBlock = 128;
+static_assert(kLdsBlock == 128, "the cell scan below adds the totals of exactly two waves");
+
+struct LdsGrid {
+  int g;
+  double x0, y0, sx, sy;  // grid corner and cells per unit
+  __device__ __forceinline__ int cx(double x) const {
+    // (x - x0) * sx is monotone in x; the clamp maps NaN to 0
+    return (int)fmin(fmax((x - x0) * sx, 0.0), (double)(g - 1));
+  }
+  __device__ __forceinline__ int cy(double y) const {
+    return (int)fmin(fmax((y - y0) * sy, 0.0), (double)(g - 1));
+  }
+};
+
+// dynamic LDS of the pass for Q rings of up to capQ vertices
+__host__ __device__ constexpr size_t lds_count_bytes(int capQ) {
+  return 16 * (size_t)capQ                      // Q vertices
+         + 4 * (size_t)(kLdsG * kLdsG + 1)      // cell offsets
+         + 4 * (size_t)(kLdsG * kLdsG)          // cell cursors
+         + 2 * (size_t)kLdsSpan * capQ          // binned edge indices
+         + 2 * (size_t)capQ                     // long edges
+         + 2 * (size_t)kLdsBlock * kPartners    // each thread's partners
+         + 64;                                  // counters and scan scratch
+}
+
+__global__ void __launch_bounds__(kLdsBlock)
+    k_count_cpu_lds(long M, int minQ, int capQ, const long long *offP, const long long *offQ,
+                    const double *ringP, const double *ringQ, const double *lenP,
+                    const double *mrgP, const double *lenQ, const double *mrgQ, const double *cf,
+                    int *cnt1P, int *cnt2P, int *boolP, int *lstP, unsigned long long *bstat) {
+  extern __shared__ double lds_count[];
+  pt *sq = (pt *)lds_count;
+  int *cellOff = (int *)(sq + capQ);
+  int *cellCur = cellOff + (kLdsG * kLdsG + 1);
+  short *ent = (short *)(cellCur + kLdsG * kLdsG);
+  short *longL = ent + kLdsSpan * capQ;
+  short *hl = longL + capQ;
+  int *misc = (int *)(((uintptr_t)(hl + kLdsBlock * kPartners) + 15) & ~(uintptr_t)15);
+  // misc[0] long-list length, misc[2..3] the two waves' sums of the cell scan
+  const long k = blockIdx.x;
+  unsigned long long ev = 0, tv = 0;
+  const long long pBase = offP[k], nP = offP[k + 1] - pBase;
+  const long long qBase = offQ[k], nQ = offQ[k + 1] - qBase;
+  const bool mine = nP > 0 && nQ > minQ && nQ <= capQ;
+  if (mine) {
+    // -- stage Q and bin its edges
+    const double *c = cf + (long long)CF_N * k + CF_BOXQ;
+    LdsGrid gr;
+    gr.g = min(kLdsG, max(1, (int)ceil(sqrt(0.5 * (double)nQ))));
+    gr.x0 = c[0];
+    gr.y0 = c[1];
+    const double w = c[2] - c[0], h = c[3] - c[1];
+    gr.sx = w > 0.0 ? (double)gr.g / w : 0.0;
+    gr.sy = h > 0.0 ? (double)gr.g / h : 0.0;
+    const int cells = gr.g * gr.g;
+    for (long long q = threadIdx.x; q < nQ; q += kLdsBlock) sq[q] = ld(ringQ, qBase + q);
+    for (int i = threadIdx.x; i < cells; i += kLdsBlock) cellCur[i] = 0;
+    if (threadIdx.x == 0) misc[0] = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < (int)nQ; q += kLdsBlock) {
+      const pt B1 = sq[q], B2 = sq[q + 1 == nQ ? 0 : q + 1];
+      const int i0 = gr.cx(fmin(B1.x, B2.x)), i1 = gr.cx(fmax(B1.x, B2.x));
+      const int j0 = gr.cy(fmin(B1.y, B2.y)), j1 = gr.cy(fmax(B1.y, B2.y));
+      if ((i1 - i0 + 1) * (j1 - j0 + 1) > kLdsSpan) {
+        longL[atomicAdd(&misc[0], 1)] = (short)q;
+      } else {
+        for (int i = i0; i <= i1; i++)
+          for (int j = j0; j <= j1; j++) atomicAdd(&cellCur[i * gr.g + j], 1);
+      }
+    }
+    __syncthreads();
+    // exclusive scan of the cell counts: each thread sums a run of cells
+    {
+      const int per = (cells + kLdsBlock - 1) / kLdsBlock;
+      const int a = min(cells, (int)threadIdx.x * per), z = min(cells, a + per);
+      int sum = 0;
+      for (int i = a; i < z; i++) sum += cellCur[i];
+      int x = sum;  // inclusive scan over the wave
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (__lane_id() >= off) x += y;
+      }
+      if (__lane_id() == 63) misc[2 + (threadIdx.x >> 6)] = x;
+      __syncthreads();
+      int run = x - sum + ((threadIdx.x >> 6) ? misc[2] : 0);
+      for (int i = a; i < z; i++) {
+        const int v = cellCur[i];
+        cellOff[i] = run;
+        cellCur[i] = run;
+        run += v;
+      }
+      if (threadIdx.x == kLdsBlock - 1) cellOff[cells] = run;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < (int)nQ; q += kLdsBlock) {
+      const pt B1 = sq[q], B2 = sq[q + 1 == nQ ? 0 : q + 1];
+      const int i0 = gr.cx(fmin(B1.x, B2.x)), i1 = gr.cx(fmax(B1.x, B2.x));
+      const int j0 = gr.cy(fmin(B1.y, B2.y)), j1 = gr.cy(fmax(B1.y, B2.y));
+      if ((i1 - i0 + 1) * (j1 - j0 + 1) <= kLdsSpan)
+        for (int i = i0; i <= i1; i++)
+          for (int j = j0; j <= j1; j++) ent[atomicAdd(&cellCur[i * gr.g + j], 1)] = (short)q;
+    }
+    __syncthreads();
+    const int nLong = misc[0];
+    // -- the P edges
+    short *mine_hl = hl + threadIdx.x * kPartners;
+    for (long long i = threadIdx.x; i < nP; i += kLdsBlock) {
+      const long long ga = pBase + i;
+      const pt A1 = ld(ringP, ga), A2 = ld(ringP, i == nP - 1 ? pBase : ga + 1);
+      CpuFilter f = cpu_filter(cf, (int)k, true, ga, qBase, lenP, mrgP, lenQ, mrgQ);
+      const EdgeBox ab = edge_box(A1, A2);
+      {
+        const double Gw = f.G * (1.0 + 1e-12) + 1e-300;
+        f.wide = EdgeBox{ab.x0 - Gw, ab.y0 - Gw, ab.x1 + Gw, ab.y1 + Gw};
+      }
+      const double ogap = fmax(fmax(c[0] - ab.x1, ab.x0 - c[2]), fmax(c[1] - ab.y1, ab.y0 - c[3]));
+      const bool flag = f.tiny || !(ogap > f.G);
+      int c1 = 0, c2 = 0;
+      if (flag) {
+        auto visit = [&](int q, pt B1, pt B2) {
+          if (f.skip(q, A1, A2, B1, B2)) return;
+          double al, be;
+          ev++;
+          const int t = itype(A1, A2, B1, B2, al, be);
+          if (t != 0) {
+            if (c1 < kPartners) mine_hl[c1] = (short)q;
+            c1++;
+          }
+          c2 += ins_on_p(t) ? 1 : 0;
+        };
+        if (f.tiny) {
+          for (int q = 0; q < (int)nQ; q++) visit(q, sq[q], sq[q + 1 == nQ ? 0 : q + 1]);
+          tv += nQ;
+        } else {
+          const int qi0 = gr.cx(f.wide.x0), qi1 = gr.cx(f.wide.x1);
+          const int qj0 = gr.cy(f.wide.y0), qj1 = gr.cy(f.wide.y1);
+          for (int ci = qi0; ci <= qi1; ci++) {
+            for (int cj = qj0; cj <= qj1; cj++) {
+              const int cell = ci * gr.g + cj;
+              for (int e = cellOff[cell]; e < cellOff[cell + 1]; e++) {
+                const int q = ent[e];
+                const pt B1 = sq[q], B2 = sq[q + 1 == nQ ? 0 : q + 1];
+                // visited from the first cell its range shares with the query
+                if (max(gr.cx(fmin(B1.x, B2.x)), qi0) != ci ||
+                    max(gr.cy(fmin(B1.y, B2.y)), qj0) != cj)
+                  continue;
+                tv++;
+                visit(q, B1, B2);
+              }
+            }
+          }
+          for (int e = 0; e < nLong; e++) {
+            const int q = longL[e];
+            tv++;
+            visit(q, sq[q], sq[q + 1 == nQ ? 0 : q + 1]);
+          }
+        }
+      }
+      cnt1P[ga] = c1;
+      cnt2P[ga] = c2 + 1;
+      boolP[ga] = flag;
+      if (c1 <= kPartners) {
+        // the partners in index order (insertion sort of at most 16)
+        for (int a = 1; a < c1; a++) {
+          const short v = mine_hl[a];
+          int b = a - 1;
+          while (b >= 0 && mine_hl[b] > v) {
+            mine_hl[b + 1] = mine_hl[b];
+            b--;
+          }
+          mine_hl[b + 1] = v;
+        }
+        int *lst = lstP + ga * kPartners;
+        for (int a = 0; a < c1; a++) lst[a] = mine_hl[a];
+      }
+    }
+  }
+  // the block's statistics, one plain store per block (k_sum_wave_stats)
+  for (int off = 32; off > 0; off >>= 1) {
+    ev += __shfl_xor(ev, off, 64);
+    tv += __shfl_xor(tv, off, 64);
+  }
+  __shared__ unsigned long long wsum[2][kLdsBlock / 64];
+  if (__lane_id() == 0) {
+    wsum[0][threadIdx.x >> 6] = ev;
+    wsum[1][threadIdx.x >> 6] = tv;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long e = 0, t = 0;
+    for (int w = 0; w < kLdsBlock / 64; w++) {
+      e += wsum[0][w];
+      t += wsum[1][w];
+    }
+    bstat[2 * k] = e;
+    bstat[2 * k + 1] = t;
+  }
 }
 
 // Partner lists of the edges with more than kPartners partners, CPU route:
@@ -3794,24 +4025,45 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   unsigned long long *bigCnt = wsT<unsigned long long>(c, "gh_bigCnt", 1);
   int *lstAllP = nullptr, *lstAllQ = nullptr;
   if (cpuRoute) {
-    // -- the P view counts, lists and partner lists
+    // -- the P view counts, lists and partner lists: pairs whose Q ring
+    // fits in LDS by the binned pass, the others by the ring walk
     if (edgesP > 0) {
-      const bool cull = boxQ.b1 != nullptr;
-      const int cb = c.tune.countBlock;
-      const unsigned grid = grid_for(edgesP, cb);
-      const long waves = (long)grid * cb / 64;
-      unsigned long long *wstat = wsT<unsigned long long>(c, "gh_wave_stats", 2 * waves);
-      {
-        LaunchTimer t(c, "gh_count");
-        hipLaunchKernelGGL(cull ? k_count_cpu<true> : k_count_cpu<false>, dim3(grid), dim3(cb), 0,
-                           c.stream, edgesP, 0L, mapP, mapQ, (const long long *)offP,
-                           (const long long *)offQ, ringP, ringQ, (const double *)elenP,
-                           (const double *)emrgP, (const double *)elenQ, (const double *)emrgQ,
-                           (const double *)cf, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, wstat,
-                           lstP, lstQ, boxP, boxQ);
-        HIP_CHECK(hipGetLastError());
+      const int capQ = (int)std::min<long>(maxRingQ, (long)std::min(c.tune.ldsMaxQ, kLdsQ));
+      const int minQ = (int)std::min<long>(c.tune.cullMin, 0x7fffffffL);
+      if (capQ > minQ) {
+        unsigned long long *bstat = wsT<unsigned long long>(c, "gh_lds_stats", 2 * M);
+        {
+          LaunchTimer t(c, "gh_count");
+          hipLaunchKernelGGL(k_count_cpu_lds, dim3((unsigned)M), dim3(kLdsBlock),
+                             lds_count_bytes(capQ), c.stream, M, minQ, capQ,
+                             (const long long *)offP,
+                             (const long long *)offQ, (const double *)ringP,
+                             (const double *)ringQ, (const double *)elenP, (const double *)emrgP,
+                             (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
+                             cnt1P, cnt2P, boolP, lstP, bstat);
+          HIP_CHECK(hipGetLastError());
+        }
+        sum_wave_stats(c, bstat, M, stats);
       }
-      sum_wave_stats(c, wstat, waves, stats);
+      {  // the ring walk: every pair the LDS pass does not take
+        const bool cull = boxQ.b1 != nullptr;
+        const int cb = c.tune.countBlock;
+        const unsigned grid = grid_for(edgesP, cb);
+        const long waves = (long)grid * cb / 64;
+        unsigned long long *wstat = wsT<unsigned long long>(c, "gh_wave_stats", 2 * waves);
+        {
+          LaunchTimer t(c, "gh_count");
+          hipLaunchKernelGGL(cull ? k_count_cpu<true> : k_count_cpu<false>, dim3(grid), dim3(cb),
+                             0, c.stream, edgesP, 0L, mapP, mapQ, (const long long *)offP,
+                             (const long long *)offQ, ringP, ringQ, (const double *)elenP,
+                             (const double *)emrgP, (const double *)elenQ,
+                             (const double *)emrgQ, (const double *)cf, cnt1P, cnt2P, cnt1Q,
+                             cnt2Q, boolP, boolQ, wstat, lstP, lstQ, boxP, boxQ,
+                             capQ > minQ ? capQ : 0);
+          HIP_CHECK(hipGetLastError());
+        }
+        sum_wave_stats(c, wstat, waves, stats);
+      }
     }
     scan_i32_to_i64(c, cnt1P, psP1, edgesP);
     scan_i32_to_i64(c, cnt2P, psP2, edgesP);
@@ -4435,5 +4687,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     p1->norm = pm.norm;
   }
 }
+
+// complex polygons (several rings per side) and the union
+#include "gh_complex.inl"
 
 }  // namespace gcmf

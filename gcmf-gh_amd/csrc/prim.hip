@@ -68,8 +68,23 @@ void Context::flush_timers() {
   pending.clear();
 }
 
+// true when launches named n are timed: every name when the filter is
+// empty, else the names listed in it (comma-separated)
+static bool timed_kernel(const std::string &only, const char *n) {
+  if (only.empty()) return true;
+  const size_t len = std::strlen(n);
+  size_t a = 0;
+  while (a <= only.size()) {
+    size_t z = only.find(',', a);
+    if (z == std::string::npos) z = only.size();
+    if (z - a == len && only.compare(a, len, n) == 0) return true;
+    a = z + 1;
+  }
+  return false;
+}
+
 LaunchTimer::LaunchTimer(Context &ctx, const char *n) : c(ctx), name(n) {
-  if (c.timing && (c.timing_only.empty() || c.timing_only == n)) {
+  if (c.timing && timed_kernel(c.timing_only, n)) {
     a = c.take_event();
     b = c.take_event();
     HIP_CHECK(hipEventRecord(a, c.stream));

@@ -187,7 +187,8 @@ class ClipStats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("cull_min", C.c_long), ("big_nodes", C.c_longlong), ("count_block", C.c_int)]
+    _fields_ = [("cull_min", C.c_long), ("big_nodes", C.c_longlong), ("count_block", C.c_int),
+                ("lds_max_q", C.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -225,6 +226,7 @@ CLIP_SEQ_LABEL = 4
 CLIP_GPU_ROUTE = 8
 CLIP_SCALE_EDGES = 16  # accepted, no effect: the edge scale is part of CLIP_NORMALIZE
 CLIP_UNIT_FRAME = 32
+CLIP_UNION = 64
 NOALLOC_SORT_MAX = 1 << 21
 
 
@@ -267,6 +269,8 @@ def hip_lib():
         L.gcmf_tune_clip.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), vp, C.c_long,
                                      C.c_int, C.POINTER(Tuning)]
         L.gcmf_clip_last_stats.argtypes = [vp, C.POINTER(ClipStats)]
+        L.gcmf_clip_complex.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), vp, C.c_long,
+                                        C.c_int, C.POINTER(ClipStats)]
         _ref_prototypes(L)
         _hip = L
     return _hip
@@ -401,8 +405,10 @@ class Context:
         self._check(hip_lib().gcmf_get_tuning(self.h, C.byref(t)), "get_tuning")
         return t.as_dict()
 
-    def set_tuning(self, cull_min, big_nodes, count_block):
-        t = Tuning(cull_min, big_nodes, count_block)
+    def set_tuning(self, cull_min, big_nodes, count_block, lds_max_q=None):
+        if lds_max_q is None:
+            lds_max_q = self.tuning()["lds_max_q"]
+        t = Tuning(cull_min, big_nodes, count_block, lds_max_q)
         self._check(hip_lib().gcmf_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def tune_clip(self, base, over, d_pairs, M, flags=CLIP_NORMALIZE):
@@ -463,6 +469,22 @@ class Context:
             self._check(hip_lib().gcmf_clip(self.h, C.byref(base.desc), C.byref(over.desc),
                                             dp.ptr, len(pairs) // 2, flags, C.byref(st)),
                         "gcmf_clip")
+        finally:
+            dp.free()
+        return st.as_dict()
+
+    def clip_complex(self, base, over, parts, flags=CLIP_NORMALIZE):
+        """Complex polygons: parts = (M, 4) of (first base ring, base ring
+        count, first overlay ring, overlay ring count); CLIP_UNION for the
+        union.  Read the results with clip_results."""
+        parts = np.ascontiguousarray(parts, dtype=np.int32).reshape(-1)
+        dp = self.upload(parts)
+        st = ClipStats()
+        try:
+            self._check(hip_lib().gcmf_clip_complex(self.h, C.byref(base.desc),
+                                                    C.byref(over.desc), dp.ptr,
+                                                    len(parts) // 4, flags, C.byref(st)),
+                        "gcmf_clip_complex")
         finally:
             dp.free()
         return st.as_dict()

@@ -57,8 +57,9 @@ int gcmf_memcpy_d2h(gcmf_ctx *ctx, void *dst, const void *d_src, size_t bytes);
 int gcmf_synchronize(gcmf_ctx *ctx);
 /* Kernel timing with HIP events recorded on the context stream. */
 int gcmf_set_timing(gcmf_ctx *ctx, int on);
-/* Restricts timing to the launches named `kernel` (NULL or "" times all),
- * so that one kernel can be timed inside an otherwise untimed region. */
+/* Restricts timing to the launches named `kernel`, or to a comma-separated
+ * list of names (NULL or "" times all), so that a few kernels can be timed
+ * inside an otherwise untimed region. */
 int gcmf_set_timing_only(gcmf_ctx *ctx, const char *kernel);
 int gcmf_timer_reset(gcmf_ctx *ctx);
 int gcmf_timer_get(gcmf_ctx *ctx, const char *kernel, double *total_ms, long *launches);
@@ -173,12 +174,31 @@ typedef struct {
 int gcmf_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
               const int *d_pairs, long M, int flags, gcmf_clip_stats *stats);
 
+/* Complex polygons: each side of a pair is a run of consecutive rings of
+ * its layer (outer rings, holes and islands, even-odd), as the reference's
+ * multi-polygon program clips them (GH-CUDA/ghcuda-multiPolygon.cpp:259-330,
+ * gppolyclip_mDIV512-multiPolygon.cu, the component loops of
+ * GH-CUDA/lib/polyclip.h).  d_parts = 4M i32: (first base ring, base ring
+ * count, first overlay ring, overlay ring count), at most 64 rings a side
+ * (pairs with more, or with a ring of no vertex, get status 2); parts
+ * outside the layers fail the call with GCMF_ERR_ARG.  P is the base side (no swap).  With
+ * GCMF_CLIP_UNION the result is the union (polyclip.h:42, UNION); a union of
+ * polygons with disjoint MBRs is normalised by their joint MBR.  Results
+ * are read with gcmf_clip_get (the phase-1 fields are not available). */
+#define GCMF_CLIP_UNION 64
+int gcmf_clip_complex(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+                      const int *d_parts, long M, int flags, gcmf_clip_stats *stats);
+
 /* Launch shapes and thresholds of the batched clip, per context. */
 typedef struct {
   long cull_min;       /* rings longer than this are walked through run boxes (512) */
   long long big_nodes; /* pairs with this many nodes are labelled by a
                           1024-thread block instead of one wave (8192) */
   int count_block;     /* threads per block of the count kernel: 64, 128, 256 (128) */
+  int lds_max_q;       /* pairs whose Q ring is longer than cull_min and has at
+                          most this many vertices are counted with Q binned in
+                          LDS, one block per pair; the others walk Q's ring
+                          (0..1024, 1024; 0 or <= cull_min: no LDS pass) */
 } gcmf_tuning;
 int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out);
 int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t);

@@ -479,3 +479,42 @@ def test_clip_reference_large_fixture(gctx, flags):
     assert [len(c) for c in res[0]] == [len(c) for c in R]
     for a, b in zip(res[0], R):
         np.testing.assert_array_equal(printed(a), b)
+
+
+def test_count_lds_pass_matches_ring_walk(gctx):
+    """The CPU route's count pass with the Q ring binned in LDS (pairs whose
+    Q ring is longer than cull_min and at most lds_max_q vertices) gives the
+    ring walk's phase-1 arrays and results bit for bit: the Fig fixtures,
+    random pairs of the join workload's sizes, grid-snapped pairs (tiny and
+    collinear edges), the long-run cases, and an uneven batch whose Q rings
+    above 1024 vertices still walk, all in one launch.  Three runs: LDS
+    pass from 64 vertices up, the culled walk from 64 vertices up, and the
+    full walk; the LDS pass classifies the same edge pairs as the full walk
+    and visits far fewer."""
+    cases = fig_cases() + random_cases(64, seed0=900, nmax=520) + snapped_cases(200) + \
+        long_run_cases()
+    P, Q = G.gen_pair_layers(600, seed=12, nmax=3000)
+    cases += [("u%d" % k, P.ring(k), Q.ring(k)) for k in range(600)]
+    default = gctx.tuning()
+    out = []
+    try:
+        for cull, lds in ((64, 1024), (64, 0), (1 << 40, 0)):
+            gctx.set_tuning(cull, default["big_nodes"], default["count_block"], lds)
+            st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+            n = {"psP1": st["edges_p"] + 1, "psP2": st["edges_p"] + 1,
+                 "psQ1": st["edges_q"] + 1, "psQ2": st["edges_q"] + 1,
+                 "boolP": st["edges_p"], "boolQ": st["edges_q"], "iP": 2 * st["slots_p"],
+                 "iQ": 2 * st["slots_q"], "oP": 2 * st["slots_p"], "oQ": 2 * st["slots_q"],
+                 "nbP": st["slots_p"], "nbQ": st["slots_q"], "status": len(cases),
+                 "comp_sizes": st["total_comps"], "xy": 2 * st["total_verts"]}
+            out.append((st, {f: gctx.clip_get(f, c) for f, c in n.items()}))
+    finally:
+        gctx.set_tuning(default["cull_min"], default["big_nodes"], default["count_block"],
+                        default["lds_max_q"])
+    (s1, a1), (s2, a2), (s0, a0) = out
+    assert s1["n_eval"] == s0["n_eval"] == s2["n_eval"]
+    assert s1["n_intersections"] == s0["n_intersections"] == s2["n_intersections"]
+    assert s1["n_tested"] * 3 < s0["n_tested"], (s1["n_tested"], s0["n_tested"])
+    for f in a1:
+        np.testing.assert_array_equal(a1[f], a0[f], err_msg=f)
+        np.testing.assert_array_equal(a2[f], a0[f], err_msg=f)

@@ -113,18 +113,31 @@ def bench():
     return B
 
 
-def test_roofline_line_arithmetic(bench):
-    cs = {"n_eval": 3_000_000, "n_tested": 50_000_000}
-    r = bench.roofline_line(4.0, 8, cs, 123456)
-    flops = 32 * 3_000_000 + 8 * 50_000_000
+def test_count_roofline_arithmetic(bench):
+    """The count kernel priced as 32 flops per classified edge pair plus 7
+    per intersection point (box rejects are not work), against 78.6 TFLOP/s."""
+    cs = {"n_eval": 3_000_000, "n_tested": 50_000_000, "n_intersections": 400_000}
+    r = bench.count_roofline(4.0, 8, cs, 123456)
+    flops = 32 * 3_000_000 + 7 * 400_000
     achieved = flops / 0.5e-3 / 1e12               # 8 launches in 4 ms
     assert r["avg_launch_ms"] == 0.5 and r["launches"] == 8
     assert r["achieved"] == pytest.approx(achieved, abs=1e-4)
-    assert r["peak"] == 78.6 and r["unit"] == "TFLOP/s"
+    assert r["bound"] == "fp64-valu" and r["peak"] == 78.6 and r["unit"] == "TFLOP/s"
     assert r["frac"] == pytest.approx(achieved / 78.6, abs=1e-6)
     assert r["traffic"] == 123456
-    zero = bench.roofline_line(0.0, 0, cs, None)
+    zero = bench.count_roofline(0.0, 0, cs, None)
     assert zero["achieved"] == 0.0 and zero["frac"] == 0.0 and zero["traffic"] is None
+
+
+def test_label_roofline_arithmetic(bench):
+    """The label kernel priced by its algorithmic bytes against 8 TB/s."""
+    cs = {"slots_p": 1000, "slots_q": 500, "edges_p": 800, "edges_q": 400,
+          "total_verts": 300, "total_comps": 7, "pairs": 3}
+    b = 36 * 1500 + 8 * 1200 + 16 * 300 + 4 * 7 + 8 * 3
+    assert bench.label_bytes(cs) == b
+    r = bench.label_roofline(2.0, 4, cs, None)
+    assert r["bound"] == "hbm" and r["unit"] == "GB/s" and r["peak"] == 8000.0
+    assert r["achieved"] == pytest.approx(b / 0.5e-3 / 1e9, abs=1e-2)
 
 
 def test_bench_refuses_more_than_one_gpu():
