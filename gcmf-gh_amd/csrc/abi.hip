@@ -331,6 +331,7 @@ int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out) {
     out->big_nodes = ctx->c.tune.bigNodes;
     out->count_block = ctx->c.tune.countBlock;
     out->lds_max_q = ctx->c.tune.ldsMaxQ;
+    out->label_lds = ctx->c.tune.labelLds;
   });
 }
 
@@ -338,12 +339,14 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
   return guarded(ctx, [&] {
     if (!t || t->cull_min < 1 || t->big_nodes < 1 ||
         (t->count_block != 64 && t->count_block != 128 && t->count_block != 256) ||
-        t->lds_max_q < 0 || t->lds_max_q > 1024)
+        t->lds_max_q < 0 || t->lds_max_q > 1024 || t->label_lds < 0 || t->label_lds > 65536 ||
+        t->label_lds % 16 != 0)
       throw gcmf::Error{GCMF_ERR_ARG, "bad tuning"};
     ctx->c.tune.cullMin = t->cull_min;
     ctx->c.tune.bigNodes = t->big_nodes;
     ctx->c.tune.countBlock = t->count_block;
     ctx->c.tune.ldsMaxQ = t->lds_max_q;
+    ctx->c.tune.labelLds = t->label_lds;
   });
 }
 
@@ -408,6 +411,15 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
         if (t < bt) bt = t, best.bigNodes = v;
       }
     }
+    {
+      double bt = 1e300;
+      for (int v : {0, 16384, 24576, 32768, 49152, 65536}) {
+        c.tune = best;
+        c.tune.labelLds = v;
+        const double t = measure("gh_label_trace");
+        if (t < bt) bt = t, best.labelLds = v;
+      }
+    }
     c.tune = best;
     c.timers = saved;
     c.timing = timing0;
@@ -417,6 +429,7 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
       out->big_nodes = best.bigNodes;
       out->count_block = best.countBlock;
       out->lds_max_q = best.ldsMaxQ;
+      out->label_lds = best.labelLds;
     }
   });
 }

@@ -11,7 +11,10 @@
 //
 // Phases 2-3 (labelling, tracing, clean-up; polyclip.h:335-807 with the
 // commented step 1 restored, fed as ghcuda.cpp:183-339 feeds it) run on the
-// GPU, one thread per pair over array-linked node lists.
+// GPU over array-linked node lists: one wave per pair (k_label_wave), one
+// 1024-thread block per pair of Tuning::bigNodes nodes or more
+// (k_label_block), and the sequential steps on one lane for the degenerate
+// pairs the parallel steps do not cover.
 //
 // Every predicate runs on per-pair normalised coordinates
 // ((x-cx)*s, (y-cy)*s), (cx,cy) = CMBR corner, s = 1/max CMBR extent, and
@@ -2068,7 +2071,9 @@ __global__ void __launch_bounds__(256)
 enum { L_NONE = 0, L_CROSSING, L_BOUNCING, L_LEFT_ON, L_RIGHT_ON, L_ON_ON, L_ON_LEFT,
        L_ON_RIGHT, L_DELAYED_CROSSING, L_DELAYED_BOUNCING };
 enum { EXIT = 0, ENTRY = 1, NEITHER = 2 };
-enum { F_SRC = 1, F_INT = 2, F_SPLIT = 4, F_CROSS = 8 };
+// F_LINK: set by the wave build on every slot whose phase-1 link is set
+// (only the parents' is read; the bit means nothing afterwards)
+enum { F_SRC = 1, F_INT = 2, F_SPLIT = 4, F_CROSS = 8, F_LINK = 16 };
 
 struct Nodes {  // one pair's node region, local indices
   double *pn, *po;
@@ -2076,10 +2081,18 @@ struct Nodes {  // one pair's node region, local indices
   unsigned char *fl;
   signed char *lab, *enex;
   int n, cap;
+  // po == nullptr: the original coordinates are read from the pair's slots,
+  // nodes [0, nP0) at oP, the others at oQ (the parallel steps, which keep
+  // the nodes in slot order)
+  const double *oP = nullptr, *oQ = nullptr;
+  int nP0 = 0;
 };
 
 __device__ __forceinline__ bool isint(const Nodes &N, int v) { return N.fl[v] & F_INT; }
 __device__ __forceinline__ pt P_(const Nodes &N, int v) { return ld(N.pn, v); }
+__device__ __forceinline__ pt P_orig(const Nodes &N, int v) {
+  return N.po ? ld(N.po, v) : (v < N.nP0 ? ld(N.oP, v) : ld(N.oQ, v - N.nP0));
+}
 
 __device__ int node_new(Nodes &N, pt p, pt o) {
   if (N.n >= N.cap) return -1;
@@ -2670,7 +2683,9 @@ __global__ void __launch_bounds__(64)
 // ---------------------------------------- wave-parallel labelling path ----
 // nodes per lane whose loads a staged pass issues before its stores
 constexpr int kStage = 4;
+constexpr int kStageWave = 8;  // the list build's, for one-wave groups
 constexpr int kRingStage = 2;  // the same for the ring writes of segment_trace
+constexpr int kRingStageWave = 2;
 // One wave64 per pair.  The lists are built in parallel; labelling step 1/3
 // runs per vertex; when every intersection vertex is a plain CROSSING (no
 // bouncing, chain or overlap labels, mutual links, a non-intersection vertex
@@ -2887,23 +2902,54 @@ __device__ __forceinline__ Departure departure(const Nodes &N, const int *dn, in
 
 enum { SEG_DONE = 0, SEG_SLOW = 2, SEG_OVERFLOW = 3 };
 
+__host__ __device__ constexpr int even_up(int v) { return (v + 1) & ~1; }
+// ints of segment scratch for a pair of n nodes and at most Dc departures
+// (Dc even): rg[0, n) and Dc-sized departure arrays for the trace, and for
+// the clean-up after it the position map rg[0, n) plus 3 ints per component
+// (at most Dc / 2 + 2) and one per output position (at most n).  Within the
+// 10 n + 20 ints of a pair's global region whenever Dc <= even_up(n).
+__host__ __device__ constexpr int seg_scratch_ints(int n, int Dc) {
+  return even_up(n) + (9 * Dc > even_up(n) + 2 * Dc + 16 ? 9 * Dc : even_up(n) + 2 * Dc + 16);
+}
+
+// A bump allocator over a wave's LDS: the arrays a pair's passes revisit
+// are placed in it while they fit, 16-byte aligned; take() returns the
+// array's place in the pair's global region once the LDS is full.  Every
+// lane makes the same calls, so the placement is uniform.
+struct Arena {
+  char *p;
+  int left;
+  // reserve: bytes that must stay free for later arrays
+  template <typename T>
+  __device__ T *take(long count, T *global, long reserve = 0) {
+    const long b = ((long)sizeof(T) * count + 15) & ~15L;
+    if (!p || b + reserve > left) return global;
+    T *r = (T *)p;
+    p += b;
+    left -= (int)b;
+    return r;
+  }
+};
+
 // createResult for a pair whose intersection vertices are CROSSING or
 // BOUNCING (passed over like ordinary vertices unless split), with mutual
-// links and enex alternating over the departures of both rings.  `rg` holds
-// 10 * (nP + nQ) ints of scratch.  Writes the components (roots in
-// O.compRoot, lengths in clen, result vertices in xy, normalised copies in
-// O.pn, rg[0, n) mapping output positions to components) in the sequential
-// order.  Returns SEG_DONE, SEG_SLOW when the pair needs the sequential
-// trace, or SEG_OVERFLOW.
+// links and enex alternating over the departures of both rings.  Dc: an
+// even bound on the number of departures (2 per intersection vertex at
+// most); `rg` holds seg_scratch_ints(n, Dc) ints, 16-byte aligned.  Writes
+// the components (roots in O.compRoot, lengths in clen, result vertices in
+// xy; omap and rg[0, n) map output positions to their nodes and
+// components) in the sequential order.  Returns SEG_DONE, SEG_SLOW when the
+// pair needs the sequential trace, or SEG_OVERFLOW.
 template <int W>
 __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, int *rg, int nP,
-                             int nQ) {
+                             int nQ, int Dc, int *omap) {
   const int lane = threadIdx.x;
-  const int n = nP + nQ;
-  int *rk = rg, *dn = rg + n;
-  int2 *vc[2] = {(int2 *)(rg + 2 * n), (int2 *)(rg + 4 * n)};
-  int *nx[2] = {rg + 6 * n, rg + 7 * n};
-  int *cbase = rg + 8 * n, *off = rg + 9 * n;
+  const int n = nP + nQ, nE = even_up(n);
+  // rk: per node; the rest per departure
+  int *rk = rg, *dn = rg + nE;
+  int2 *vc[2] = {(int2 *)(rg + nE + Dc), (int2 *)(rg + nE + 3 * Dc)};
+  int *nx[2] = {rg + nE + 5 * Dc, rg + nE + 6 * Dc};
+  int *cbase = rg + nE + 7 * Dc, *off = rg + nE + 8 * Dc;
   // departures by ring order
   int carry = 0, XP = 0;
   for (int ring = 0; ring < 2; ring++) {
@@ -2913,7 +2959,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       const int mu = j < nR ? dmult(N, u) : 0;
       int tot;
       const int r = carry + grp_scan<W>(mu, tot);
-      if (mu > 0 && r + mu <= n) {
+      if (mu > 0 && r + mu <= Dc) {
         rk[u] = r;
         dn[r] = 2 * u;
         if (mu == 2) dn[r + 1] = 2 * u + 1;
@@ -2923,7 +2969,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     if (ring == 0) XP = carry;
   }
   const int XQ = carry - XP, D = carry;
-  if (XP != XQ || XP == 0 || D > n) return SEG_SLOW;
+  if (XP != XQ || XP == 0 || D > Dc) return SEG_SLOW;
   __syncthreads();
   // successor of each departure: the departure from the neighbour of its
   // arrival; the arrival must carry the status the inner loop stops on.
@@ -2999,7 +3045,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   const int2 *mc = vc[cur];
   // a cycle is traced when it holds the departure of its smallest P key; its
   // list starts there.  S: segment lengths, then suffix sums
-  int *S[2] = {(int *)vc[cur ^ 1], (int *)vc[cur ^ 1] + n};
+  int *S[2] = {(int *)vc[cur ^ 1], (int *)vc[cur ^ 1] + Dc};
   for (int d = lane; d < D; d += W) {
     const int2 c = mc[d];
     const int sd = c.x;
@@ -3101,33 +3147,34 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   // and on the backward segment of the departure after it; of these two,
   // which are reverses of each other, at most one is traced, except at a
   // split vertex, where the two arrive at its two copies
-  // (kRingStage chunks of W vertices at a time: each level of loads, the
+  // (kRS chunks of W vertices at a time: each level of loads, the
   // vertices' multiplicities, then their two departures, then those
   // departures' statuses, components and the vertices' coordinates, is
   // issued for all the chunks before it is used; the stores come last)
   int *cw = rk;  // output position -> component
+  constexpr int kRS = W == 64 ? kRingStageWave : kRingStage;
   for (int ring = 0; ring < 2; ring++) {
     const int rb = ring ? nP : 0, nR = ring ? nQ : nP, base = ring ? XP : 0;
     const int XR = ring ? XQ : XP;
     carry = 0;
-    for (int b0 = 0; b0 < nR; b0 += kRingStage * W) {
-      int mu[kRingStage], r[kRingStage];
+    for (int b0 = 0; b0 < nR; b0 += kRS * W) {
+      int mu[kRS], r[kRS];
 #pragma unroll
-      for (int t = 0; t < kRingStage; t++) {
+      for (int t = 0; t < kRS; t++) {
         const int j = b0 + t * W + lane;
         mu[t] = j < nR ? dmult(N, rb + j) : 0;
       }
 #pragma unroll
-      for (int t = 0; t < kRingStage; t++) {
+      for (int t = 0; t < kRS; t++) {
         int tot;
         r[t] = carry + grp_scan<W>(mu[t], tot);
         carry += tot;
       }
       // the departures before and after each vertex: [t][0] forward, [t][1]
       // backward
-      int dd[kRingStage][2], of[kRingStage][2], cd[kRingStage][2];
+      int dd[kRS][2], of[kRS][2], cd[kRS][2];
 #pragma unroll
-      for (int t = 0; t < kRingStage; t++) {
+      for (int t = 0; t < kRS; t++) {
         const int j = b0 + t * W + lane, ri = r[t] + mu[t];
         dd[t][0] = base + (r[t] == 0 ? XR - 1 : r[t] - 1);
         dd[t][1] = base + (ri == XR ? 0 : ri);
@@ -3137,16 +3184,13 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
           cd[t][side] = j < nR ? dn[dd[t][side]] : 0;
         }
       }
-      bool on[kRingStage][2];
-      int ci[kRingStage][2];
-      pt pu[kRingStage], ou[kRingStage];
+      bool on[kRS][2];
+      int ci[kRS][2];
+      pt ou[kRS];
 #pragma unroll
-      for (int t = 0; t < kRingStage; t++) {
+      for (int t = 0; t < kRS; t++) {
         const int j = b0 + t * W + lane;
-        if (j < nR) {
-          pu[t] = P_(N, rb + j);
-          ou[t] = ld(N.po, rb + j);
-        }
+        if (j < nR) ou[t] = P_orig(N, rb + j);
 #pragma unroll
         for (int side = 0; side < 2; side++) {
           on[t][side] = of[t][side] >= 0 && dep_enex(N, cd[t][side]) == (side ? EXIT : ENTRY);
@@ -3154,7 +3198,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
         }
       }
 #pragma unroll
-      for (int t = 0; t < kRingStage; t++) {
+      for (int t = 0; t < kRS; t++) {
         const int u = rb + b0 + t * W + lane;
 #pragma unroll
         for (int side = 0; side < 2; side++) {
@@ -3164,9 +3208,9 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
           if (dist <= 0) dist += nR;
           const int w = of[t][side] + dist - 1;
           if (w >= 0 && w < O.n) {
-            st(O.pn, w, pu[t]);
             st(xy, w, ou[t]);
             cw[w] = ci[t][side];
+            omap[w] = u;
           }
         }
       }
@@ -3176,14 +3220,15 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   return SEG_DONE;
 }
 
-// a whole node ring as output component c at [off, off + n)
+// a whole node ring as output component c at [off, off + n); omap: output
+// position -> node
 template <int W>
-__device__ void copy_ring_out(OutBuf &O, double *xy, int *cw, const Nodes &N, int r0, int n,
+__device__ void copy_ring_out(double *xy, int *cw, int *omap, const Nodes &N, int r0, int n,
                               int off, int c) {
   for (int j = threadIdx.x; j < n; j += W) {
-    st(O.pn, off + j, P_(N, r0 + j));
-    st(xy, off + j, ld(N.po, r0 + j));
+    st(xy, off + j, P_orig(N, r0 + j));
     cw[off + j] = c;
+    omap[off + j] = r0 + j;
   }
 }
 
@@ -3209,16 +3254,21 @@ __device__ __forceinline__ unsigned compose(unsigned g, unsigned f) {
   return h;
 }
 
+// the normalised point at output position w: node omap[w] of pn
+__device__ __forceinline__ pt out_pt(const double *pn, const int *omap, int w) {
+  return ld(pn, omap[w]);
+}
+
 // writes keep[b..b+L) (0/1) and returns the number of vertices kept
-__device__ int clean_component(const double *pn, int *keep, int b, int L) {
+__device__ int clean_component(const double *pn, const int *omap, int *keep, int b, int L) {
   const int lane = __lane_id();
   int r = -1;
   if (L >= 3) {
-    const pt Pl = ld(pn, b + L - 1);
+    const pt Pl = out_pt(pn, omap, b + L - 1);
     for (int base = 0; base <= L - 3 && r < 0; base += 64) {
       const int i = base + lane;
       bool stays = false;
-      if (i <= L - 3) stays = !(fabs(triA(Pl, ld(pn, b + i), ld(pn, b + i + 1))) < kGhEps);
+      if (i <= L - 3) stays = !(fabs(triA(Pl, out_pt(pn, omap, b + i), out_pt(pn, omap, b + i + 1))) < kGhEps);
       const unsigned long long m = __ballot(stays);
       if (m) r = base + __ffsll((long long)m) - 1;
     }
@@ -3233,9 +3283,9 @@ __device__ int clean_component(const double *pn, int *keep, int b, int L) {
     const int j = base + lane;
     unsigned K = 0;
     if (j <= m) {
-      const pt cur = ld(pn, x0 + j), nxt = ld(pn, j == m ? x0 : x0 + j + 1);
+      const pt cur = out_pt(pn, omap, x0 + j), nxt = out_pt(pn, omap, j == m ? x0 : x0 + j + 1);
       for (int d = 1; d <= kCleanW && d <= j; d++)
-        if (!(fabs(triA(ld(pn, x0 + j - d), cur, nxt)) < kGhEps)) K |= 1u << (d - 1);
+        if (!(fabs(triA(out_pt(pn, omap, x0 + j - d), cur, nxt)) < kGhEps)) K |= 1u << (d - 1);
     }
     unsigned f = 0x76543210u;  // identity past the end
     if (j <= m) {
@@ -3263,10 +3313,10 @@ __device__ int clean_component(const double *pn, int *keep, int b, int L) {
   }
   if (__ballot(ovf)) {
     if (lane == 0) {
-      pt last = ld(pn, x0);
+      pt last = out_pt(pn, omap, x0);
       kept = 1;
       for (int j = 1; j <= m; j++) {
-        const pt cur = ld(pn, x0 + j), nxt = ld(pn, j == m ? x0 : x0 + j + 1);
+        const pt cur = out_pt(pn, omap, x0 + j), nxt = out_pt(pn, omap, j == m ? x0 : x0 + j + 1);
         const bool kp = !(fabs(triA(last, cur, nxt)) < kGhEps);
         keep[x0 + j] = kp ? 1 : 0;
         if (kp) {
@@ -3282,12 +3332,12 @@ __device__ int clean_component(const double *pn, int *keep, int b, int L) {
 
 // the same clean-up of one component by one thread, in order: the root
 // search (A), then the pass (B)
-__device__ int lane_clean(const double *pn, int *keep, int b, int L) {
+__device__ int lane_clean(const double *pn, const int *omap, int *keep, int b, int L) {
   int r = -1;
   if (L >= 3) {
-    const pt Pl = ld(pn, b + L - 1);
+    const pt Pl = out_pt(pn, omap, b + L - 1);
     for (int i = 0; i <= L - 3; i++)
-      if (!(fabs(triA(Pl, ld(pn, b + i), ld(pn, b + i + 1))) < kGhEps)) {
+      if (!(fabs(triA(Pl, out_pt(pn, omap, b + i), out_pt(pn, omap, b + i + 1))) < kGhEps)) {
         r = i;
         break;
       }
@@ -3296,10 +3346,10 @@ __device__ int lane_clean(const double *pn, int *keep, int b, int L) {
   if (r < 0) return 0;
   keep[b + r] = 1;
   const int x0 = b + r, m = L - 1 - r;
-  pt last = ld(pn, x0);
+  pt last = out_pt(pn, omap, x0);
   int kept = 1;
   for (int j = 1; j <= m; j++) {
-    const pt cur = ld(pn, x0 + j), nxt = ld(pn, j == m ? x0 : x0 + j + 1);
+    const pt cur = out_pt(pn, omap, x0 + j), nxt = out_pt(pn, omap, j == m ? x0 : x0 + j + 1);
     const bool kp = !(fabs(triA(last, cur, nxt)) < kGhEps);
     keep[x0 + j] = kp ? 1 : 0;
     if (kp) {
@@ -3316,20 +3366,23 @@ constexpr int kLaneClean = 48;
 
 // clean-up of every component of O (component c at [O.compRoot[c], +clen[c])
 // in list order, cw: position -> component), then compaction of xy and clen
-// over the kept vertices and non-empty components.  Scratch: rg[n, 5n).
-// Returns true when some vertex was dropped; O.n / O.ncomp are updated.
+// over the kept vertices and non-empty components; the normalised point at
+// output position w is node omap[w] of pn.  Scratch: 3 O.ncomp + O.n ints
+// at sc.  Returns true when some vertex was dropped; O.n / O.ncomp are
+// updated.
 template <int W>
-__device__ bool grp_cleanup_compact(OutBuf &O, double *xy, int *clen, const int *cw, int *rg,
-                                    int n) {
+__device__ bool grp_cleanup_compact(OutBuf &O, double *xy, int *clen, const int *cw, int *sc,
+                                    const double *pn, const int *omap) {
   const int lane = threadIdx.x;
   const int T = O.n, NC = O.ncomp;
-  int *dirty = rg + n, *dl = rg + 2 * n, *keep = rg + 3 * n, *nlen = rg + 4 * n;
+  int *dirty = sc, *dl = sc + NC, *nlen = sc + 2 * NC, *keep = sc + 3 * NC;
   for (int c = lane; c < NC; c += W) dirty[c] = 0;
   __syncthreads();
   for (int w = lane; w < T; w += W) {
     const int c = min(max(cw[w], 0), NC - 1), b = O.compRoot[c], L = clen[c];
     const int pw = w == b ? b + L - 1 : w - 1, nw = w == b + L - 1 ? b : w + 1;
-    if (fabs(triA(ld(O.pn, pw), ld(O.pn, w), ld(O.pn, nw))) < kGhEps) dirty[c] = 1;
+    if (fabs(triA(out_pt(pn, omap, pw), out_pt(pn, omap, w), out_pt(pn, omap, nw))) < kGhEps)
+      dirty[c] = 1;
   }
   __syncthreads();
   int nd = 0;
@@ -3345,12 +3398,12 @@ __device__ bool grp_cleanup_compact(OutBuf &O, double *xy, int *clen, const int 
   __syncthreads();
   for (int i = lane; i < nd; i += W) {
     const int c = dl[i];
-    if (clen[c] <= kLaneClean) nlen[c] = lane_clean(O.pn, keep, O.compRoot[c], clen[c]);
+    if (clen[c] <= kLaneClean) nlen[c] = lane_clean(pn, omap, keep, O.compRoot[c], clen[c]);
   }
   for (int i = lane >> 6; i < nd; i += W / 64) {
     const int c = dl[i];
     if (clen[c] > kLaneClean) {
-      const int kept = clean_component(O.pn, keep, O.compRoot[c], clen[c]);
+      const int kept = clean_component(pn, omap, keep, O.compRoot[c], clen[c]);
       if (__lane_id() == 0) nlen[c] = kept;
     }
   }
@@ -3395,21 +3448,36 @@ enum {
   LP_FAILED = 5        // overflow or failed trace
 };
 
-// true when some vertex of the ring [r0, r0+n) of O (cyclic) meets
-// cleanUpResult's removal test
+// gives the sequential steps, which insert nodes, the pair's nodes in its
+// global region G: copies the arrays of N that lie in the LDS arena there,
+// writes the ring links and the original coordinates (which the parallel
+// steps derive from the slot order) and makes N the global region; every
+// lane must call it
 template <int W>
-__device__ bool grp_any_cleanup(const OutBuf &O, int r0, int n) {
-  bool cand = false;
+__device__ void spill_nodes(Nodes &N, const Nodes &G, int nP, int nQ) {
+  const int n = nP + nQ;
+  __syncthreads();
   for (int j = threadIdx.x; j < n; j += W) {
-    const int w = r0 + j, pw = r0 + (j == 0 ? n - 1 : j - 1), nw = r0 + (j + 1 == n ? 0 : j + 1);
-    if (fabs(triA(ld(O.pn, pw), ld(O.pn, w), ld(O.pn, nw))) < kGhEps) cand = true;
+    if (N.pn != G.pn) st(G.pn, j, ld(N.pn, j));
+    st(G.po, j, P_orig(N, j));
+    G.next[j] = j == nP - 1 ? 0 : (j == n - 1 ? nP : j + 1);
+    G.prev[j] = j == 0 ? nP - 1 : (j == nP ? n - 1 : j - 1);
+    if (N.nb != G.nb) G.nb[j] = N.nb[j];
+    if (N.fl != G.fl) G.fl[j] = N.fl[j];
+    if (N.lab != G.lab) G.lab[j] = N.lab[j];
+    if (N.enex != G.enex) G.enex[j] = N.enex[j];
   }
-  return grp_any<W>(cand);
+  __syncthreads();
+  const int nn = N.n, cap = N.cap;
+  N = G;
+  N.n = nn;
+  N.cap = cap;
 }
 
+// arena / arenaBytes: the group's LDS for the pair's arrays (null: none)
 template <int W>
 __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceArgs &a,
-                          int *aux) {
+                          int *aux, char *arena, int arenaBytes) {
   const int lane = threadIdx.x;
   if (status[k] != GH_OK) {
     if (lane == 0) {
@@ -3421,49 +3489,67 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
   const long long sbP = a.psP2[a.offP[k]], cP = a.psP2[a.offP[k + 1]] - sbP;
   const long long sbQ = a.psQ2[a.offQ[k]], cQ = a.psQ2[a.offQ[k + 1]] - sbQ;
   const long long nb0 = a.nodeOff[k], cap = a.nodeOff[k + 1] - nb0;
-  Nodes N;
-  N.pn = a.npn + 2 * nb0;
-  N.po = a.npo + 2 * nb0;
-  N.next = a.nnext + nb0;
-  N.prev = a.nprev + nb0;
-  N.nb = a.nnb + nb0;
-  N.fl = a.nfl + nb0;
-  N.lab = a.nlab + nb0;
-  N.enex = a.nenex + nb0;
-  N.n = (int)(cP + cQ);
-  N.cap = (int)cap;
+  // the pair's global region (node capacity cap >= 2 (nP + nQ) + 4)
+  Nodes G;
+  G.pn = a.npn + 2 * nb0;
+  G.po = a.npo + 2 * nb0;
+  G.next = a.nnext + nb0;
+  G.prev = a.nprev + nb0;
+  G.nb = a.nnb + nb0;
+  G.fl = a.nfl + nb0;
+  G.lab = a.nlab + nb0;
+  G.enex = a.nenex + nb0;
+  G.n = (int)(cP + cQ);
+  G.cap = (int)cap;
   const long long pE0 = a.offP[k], pE1 = a.offP[k + 1];
   const long long qE0 = a.offQ[k], qE1 = a.offQ[k + 1];
-  const int nP = (int)cP, nQ = (int)cQ;
-  int *rg = aux + 5 * nb0;  // 10 * (nP + nQ) ints of segment scratch
+  const int nP = (int)cP, nQ = (int)cQ, n = nP + nQ;
+  // the parallel steps keep the n nodes in LDS as far as they fit, the
+  // arrays read most often first; the sequential steps, which insert
+  // nodes, run on the global region (spill_nodes)
+  Arena ar{arena, arenaBytes};
+  // Ring links follow the slot order (node j + 1 after node j on each
+  // ring) and the original coordinates stay in the slots, so neither is
+  // materialised here.
+  Nodes N = G;
+  N.pn = ar.take<double>(2 * n, G.pn);
+  N.nb = ar.take<int>(n, G.nb);
+  N.fl = ar.take<unsigned char>(n, G.fl);
+  N.lab = ar.take<signed char>(n, G.lab);
+  N.enex = ar.take<signed char>(n, G.enex);
+  N.next = N.prev = nullptr;
+  N.po = nullptr;
+  N.oP = s.oP + 2 * sbP;
+  N.oQ = s.oQ + 2 * sbQ;
+  N.nP0 = nP;
+  int *rg = aux + 5 * nb0;  // segment scratch: 10 n + 20 ints in the global region
   // -------- build (ghcuda.cpp:239-322 semantics, see k_label_trace): every
-  // slot is first taken as an intersection vertex, then the first slot of
-  // each edge is re-flagged as its parent vertex.  The loops below (and
-  // step 1) take kStage nodes per lane at a time and issue all their loads
-  // before the first store, so that a pass waits on memory once per
-  // kStage * W nodes rather than once per W.
+  // slot is first taken as an intersection vertex, with its link as loaded
+  // kept in F_LINK; then the first slot of each edge is re-flagged as its
+  // parent vertex, linked when F_LINK is set.  Each pass takes kS nodes per
+  // lane at a time and issues all their loads before the first store, so
+  // that it waits on memory once per kS * W nodes.
   const bool xl = a.explicitLinks != 0;
-  for (int j0 = lane; j0 < nP; j0 += kStage * W) {
-    pt ip[kStage], op[kStage];
-    int nbv[kStage];
+  constexpr int kS = W == 64 ? kStageWave : kStage;
+  for (int j0 = lane; j0 < nP; j0 += kS * W) {
+    pt ip[kS];
+    int nbv[kS];
+    bool lk[kS];
 #pragma unroll
-    for (int u = 0; u < kStage; u++) {
+    for (int u = 0; u < kS; u++) {
       const int j = j0 + u * W;
       if (j < nP) {
         const long long sl = sbP + j;
         ip[u] = ld(s.iP, sl);
-        op[u] = ld(s.oP, sl);
         nbv[u] = xl ? s.nbP[sl] : 0;
+        lk[u] = xl ? nbv[u] > 0 : s.aP[sl] != kNone;
       }
     }
 #pragma unroll
-    for (int u = 0; u < kStage; u++) {
+    for (int u = 0; u < kS; u++) {
       const int j = j0 + u * W;
       if (j < nP) {
         st(N.pn, j, ip[u]);
-        st(N.po, j, op[u]);
-        N.prev[j] = j == 0 ? nP - 1 : j - 1;
-        N.next[j] = j == nP - 1 ? 0 : j + 1;
         if (xl) {
           const long long kq = (long long)nbv[u] - 1 - sbQ;
           N.nb[j] = (kq >= 0 && kq < cQ) ? nP + (int)kq : -1;
@@ -3472,72 +3558,64 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
         }
         N.lab[j] = L_NONE;
         N.enex[j] = NEITHER;
-        N.fl[j] = F_INT;
+        N.fl[j] = (unsigned char)(F_INT | (lk[u] ? F_LINK : 0));
       }
     }
   }
-  for (int j0 = lane; j0 < nQ; j0 += kStage * W) {
-    pt ip[kStage], op[kStage];
-    int nbv[kStage];
+  for (int j0 = lane; j0 < nQ; j0 += kS * W) {
+    pt ip[kS];
+    int nbv[kS];
+    bool lk[kS];
 #pragma unroll
-    for (int u = 0; u < kStage; u++) {
+    for (int u = 0; u < kS; u++) {
       const int j = j0 + u * W;
       if (j < nQ) {
         const long long sl = sbQ + j;
         ip[u] = ld(s.iQ, sl);
-        op[u] = ld(s.oQ, sl);
         nbv[u] = s.nbQ[sl];
+        lk[u] = s.aQ[sl] != kNone || nbv[u] > 0;
       }
     }
 #pragma unroll
-    for (int u = 0; u < kStage; u++) {
+    for (int u = 0; u < kS; u++) {
       const int j = j0 + u * W;
       if (j < nQ) {
         const int v = nP + j;
         st(N.pn, v, ip[u]);
-        st(N.po, v, op[u]);
-        N.prev[v] = j == 0 ? nP + nQ - 1 : v - 1;
-        N.next[v] = j == nQ - 1 ? nP : v + 1;
         N.lab[v] = L_NONE;
         N.enex[v] = NEITHER;
         // an unlinked parent has nbQ <= 0, which lands outside [0, cP) as well
         const long long kp = (long long)nbv[u] - 1 - sbP;
         N.nb[v] = (kp >= 0 && kp < cP) ? (int)kp : -1;
-        N.fl[v] = F_INT;
+        N.fl[v] = (unsigned char)(F_INT | (lk[u] ? F_LINK : 0));
       }
     }
   }
   __syncthreads();
-  for (long long e0 = pE0 + lane; e0 < pE1; e0 += kStage * W) {
-    long long sl[kStage];
-    bool linked[kStage];
+  // parents: the edges' first slots (one edge each)
+  for (long long e0 = pE0 + lane; e0 < pE1; e0 += kS * W) {
+    long long sl[kS];
 #pragma unroll
-    for (int u = 0; u < kStage; u++) {
+    for (int u = 0; u < kS; u++) {
       const long long e = e0 + u * W;
-      sl[u] = e < pE1 ? a.psP2[e] : -1;
+      sl[u] = e < pE1 ? a.psP2[e] - sbP : -1;
     }
 #pragma unroll
-    for (int u = 0; u < kStage; u++)
-      linked[u] = sl[u] >= 0 && (xl ? s.nbP[sl[u]] > 0 : s.aP[sl[u]] != kNone);
-#pragma unroll
-    for (int u = 0; u < kStage; u++)
-      if (sl[u] >= 0) N.fl[sl[u] - sbP] = (unsigned char)(F_SRC | (linked[u] ? F_INT : 0));
-  }
-  for (long long e0 = qE0 + lane; e0 < qE1; e0 += kStage * W) {
-    long long sl[kStage];
-    bool linked[kStage];
-#pragma unroll
-    for (int u = 0; u < kStage; u++) {
-      const long long e = e0 + u * W;
-      sl[u] = e < qE1 ? a.psQ2[e] : -1;
-    }
-#pragma unroll
-    for (int u = 0; u < kStage; u++)
-      linked[u] = sl[u] >= 0 && (s.aQ[sl[u]] != kNone || s.nbQ[sl[u]] > 0);
-#pragma unroll
-    for (int u = 0; u < kStage; u++)
+    for (int u = 0; u < kS; u++)
       if (sl[u] >= 0)
-        N.fl[nP + (sl[u] - sbQ)] = (unsigned char)(F_SRC | (linked[u] ? F_INT : 0));
+        N.fl[sl[u]] = (unsigned char)(F_SRC | ((N.fl[sl[u]] & F_LINK) ? F_INT : 0));
+  }
+  for (long long e0 = qE0 + lane; e0 < qE1; e0 += kS * W) {
+    long long sl[kS];
+#pragma unroll
+    for (int u = 0; u < kS; u++) {
+      const long long e = e0 + u * W;
+      sl[u] = e < qE1 ? nP + (a.psQ2[e] - sbQ) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kS; u++)
+      if (sl[u] >= 0)
+        N.fl[sl[u]] = (unsigned char)(F_SRC | ((N.fl[sl[u]] & F_LINK) ? F_INT : 0));
   }
   __syncthreads();
   if (!xl) {
@@ -3572,7 +3650,7 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
       }
       if (N.nb[q[u]] != I) slow = true;
       const int Pm = I == 0 ? nP - 1 : I - 1, Pp = I == nP - 1 ? 0 : I + 1;
-      const int Qm = N.prev[q[u]], Qp = N.next[q[u]];
+      const int Qm = q[u] == nP ? n - 1 : q[u] - 1, Qp = q[u] == n - 1 ? nP : q[u] + 1;
       const int tm = node_rel(N, Qm, Pm, I, Pp), tp = node_rel(N, Qp, Pm, I, Pp);
       int lb = L_NONE;
       if ((tm == 0 && tp == 1) || (tm == 1 && tp == 0)) lb = L_CROSSING;
@@ -3615,10 +3693,19 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
   O.compRoot = a.ocompRoot + a.compOff[k];
   O.ncomp = 0;
   O.capComp = (int)(a.compOff[k + 1] - a.compOff[k]);
-  // crossings on P (the links are mutual here, so Q has as many)
-  int nCrossP = 0;
-  for (int j = lane; j < nP; j += W) nCrossP += iscross(N, j) ? 1 : 0;
+  // crossings on P (the links are mutual here, so Q has as many), and the
+  // intersection vertices of both rings, each of which departs at most twice
+  int nCrossP = 0, nInt = 0;
+  for (int j = lane; j < n; j += W) {
+    nInt += isint(N, j) ? 1 : 0;
+    if (j < nP) nCrossP += iscross(N, j) ? 1 : 0;
+  }
   nCrossP = grp_sum<W>(nCrossP);
+  nInt = grp_sum<W>(nInt);
+  const int Dc = even_up(min(2 * nInt, n));
+  rg = ar.take<int>(seg_scratch_ints(n, Dc), rg);
+  // output position -> node (the clean-up reads the node's normalised point)
+  int *omap = ar.take<int>(n, (int *)O.pn);
   int VP = -1, VQ = -1;
   if (!slow) {
     VP = grp_first_nonint<W>(N, 0, nP);
@@ -3638,7 +3725,7 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
         slow = true;
       } else {
         if (pin) {
-          copy_ring_out<W>(O, xy, rg, N, 0, nP, 0, 0);
+          copy_ring_out<W>(xy, rg, omap, N, 0, nP, 0, 0);
           if (lane == 0) {
             O.compRoot[0] = 0;
             clen[0] = nP;
@@ -3647,7 +3734,7 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
           O.n = nP;
         }
         if (qin) {
-          copy_ring_out<W>(O, xy, rg, N, nP, nQ, O.n, O.ncomp);
+          copy_ring_out<W>(xy, rg, omap, N, nP, nQ, O.n, O.ncomp);
           if (lane == 0) {
             O.compRoot[O.ncomp] = O.n;
             clen[O.ncomp] = nQ;
@@ -3663,7 +3750,7 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
       const bool pinQ = grp_pip<W>(N, 0, nP, P_(N, VQ));
       grp_enex<W>(N, nP, nQ, VQ, pinQ ? EXIT : ENTRY);
       __syncthreads();
-      const int rc = segment_trace<W>(N, O, xy, clen, rg, nP, nQ);
+      const int rc = segment_trace<W>(N, O, xy, clen, rg, nP, nQ, Dc, omap);
       if (rc == SEG_OVERFLOW) {
         if (lane == 0) {
           status[k] = GH_OVERFLOW;
@@ -3682,8 +3769,9 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
     // P intersection vertex's label to its neighbour in ring order, so a Q
     // vertex linked from several P vertices ends with the last one's label:
     // the largest such P index, found by an atomic maximum.
-    int *owner = rg;  // nQ entries of the segment scratch (10 (nP + nQ) ints)
-    __syncthreads();
+    spill_nodes<W>(N, G, nP, nQ);
+    O.pn = a.opn + 2 * nb0;
+    int *owner = rg;  // nQ entries of the segment scratch
     for (int j = lane; j < nQ; j += W) owner[j] = -1;
     __syncthreads();
     for (int I = lane; I < nP; I += W) {
@@ -3736,7 +3824,8 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
     return LP_SEQUENTIAL;
   }
   __syncthreads();
-  if (a.cleanup && grp_cleanup_compact<W>(O, xy, clen, rg, rg, nP + nQ) && path == LP_SEGMENTS)
+  if (a.cleanup && grp_cleanup_compact<W>(O, xy, clen, rg, rg + even_up(n), N.pn, omap) &&
+      path == LP_SEGMENTS)
     path = LP_SEG_CLEANUP;
   if (lane == 0) {
     a.outNComp[k] = O.ncomp;
@@ -3786,12 +3875,14 @@ __global__ void k_label_order_keys(long M, const long long *offP, const long lon
 // constant-rate wall clock (both optional).
 __global__ void __launch_bounds__(64)
     k_label_wave(long M, const int *order, int *status, SlotArrays s, TraceArgs a, int *aux,
-                 int *lpath, long long *lticks) {
+                 int *lpath, long long *lticks, int arenaBytes) {
   if ((long)blockIdx.x >= M) return;
   const long k = order ? order[blockIdx.x] : (long)blockIdx.x;
   if (pair_nodes(a, k) >= a.bigNodes) return;  // k_label_block's
   const long long t0 = wall_clock64();
-  const int path = label_pair<64>(k, status, s, a, aux);
+  extern __shared__ __align__(16) char label_lds[];
+  const int path = label_pair<64>(k, status, s, a, aux, arenaBytes > 0 ? label_lds : nullptr,
+                                  arenaBytes);
   if (threadIdx.x == 0 && lpath) {
     lpath[k] = path;
     lticks[k] = wall_clock64() - t0;
@@ -3820,7 +3911,7 @@ __global__ void __launch_bounds__(kBigGroup)
   for (long i = blockIdx.x; i < n; i += gridDim.x) {
     const long k = list[i];
     const long long t0 = wall_clock64();
-    const int path = label_pair<kBigGroup>(k, status, s, a, aux);
+    const int path = label_pair<kBigGroup>(k, status, s, a, aux, nullptr, 0);
     if (threadIdx.x == 0 && lpath) {
       lpath[k] = path;
       lticks[k] = wall_clock64() - t0;
@@ -4611,9 +4702,9 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       HIP_CHECK(hipGetLastError());
       HIP_CHECK(hipEventRecord(c.ev_join, c.side));
     }
-    hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), 0, c.stream, M,
-                       (const int *)order, pm.status, s, ta, aux, res.labelPath,
-                       res.labelTicks);
+    hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), (size_t)c.tune.labelLds,
+                       c.stream, M, (const int *)order, pm.status, s, ta, aux, res.labelPath,
+                       res.labelTicks, c.tune.labelLds);
     HIP_CHECK(hipGetLastError());
     if (nBig > 0) HIP_CHECK(hipStreamWaitEvent(c.stream, c.ev_join, 0));
   }

@@ -188,7 +188,7 @@ class ClipStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("cull_min", C.c_long), ("big_nodes", C.c_longlong), ("count_block", C.c_int),
-                ("lds_max_q", C.c_int)]
+                ("lds_max_q", C.c_int), ("label_lds", C.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -405,10 +405,13 @@ class Context:
         self._check(hip_lib().gcmf_get_tuning(self.h, C.byref(t)), "get_tuning")
         return t.as_dict()
 
-    def set_tuning(self, cull_min, big_nodes, count_block, lds_max_q=None):
+    def set_tuning(self, cull_min, big_nodes, count_block, lds_max_q=None, label_lds=None):
+        cur = self.tuning()
         if lds_max_q is None:
-            lds_max_q = self.tuning()["lds_max_q"]
-        t = Tuning(cull_min, big_nodes, count_block, lds_max_q)
+            lds_max_q = cur["lds_max_q"]
+        if label_lds is None:
+            label_lds = cur["label_lds"]
+        t = Tuning(cull_min, big_nodes, count_block, lds_max_q, label_lds)
         self._check(hip_lib().gcmf_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def tune_clip(self, base, over, d_pairs, M, flags=CLIP_NORMALIZE):

@@ -199,6 +199,9 @@ typedef struct {
                           most this many vertices are counted with Q binned in
                           LDS, one block per pair; the others walk Q's ring
                           (0..1024, 1024; 0 or <= cull_min: no LDS pass) */
+  int label_lds;       /* bytes of LDS each labelling wave keeps a pair's node
+                          arrays and scratch in; what does not fit stays in
+                          HBM (0..65536, a multiple of 16; 20480) */
 } gcmf_tuning;
 int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out);
 int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t);

@@ -430,7 +430,7 @@ def test_clip_autotuner_keeps_results(gctx):
         with pytest.raises(G.GcmfError):
             gctx.set_tuning(512, 8192, 100)
     finally:
-        gctx.set_tuning(default["cull_min"], default["big_nodes"], default["count_block"])
+        gctx.set_tuning(**default)
         d_pairs.free()
         dP.free()
         dQ.free()
@@ -459,7 +459,7 @@ def test_clip_block_threshold_keeps_results(gctx, big_nodes):
             for x, y in zip(a, b):
                 np.testing.assert_array_equal(x, y)
     finally:
-        gctx.set_tuning(default["cull_min"], default["big_nodes"], default["count_block"])
+        gctx.set_tuning(**default)
         dP.free()
         dQ.free()
 
@@ -509,8 +509,7 @@ def test_count_lds_pass_matches_ring_walk(gctx):
                  "comp_sizes": st["total_comps"], "xy": 2 * st["total_verts"]}
             out.append((st, {f: gctx.clip_get(f, c) for f, c in n.items()}))
     finally:
-        gctx.set_tuning(default["cull_min"], default["big_nodes"], default["count_block"],
-                        default["lds_max_q"])
+        gctx.set_tuning(**default)
     (s1, a1), (s2, a2), (s0, a0) = out
     assert s1["n_eval"] == s0["n_eval"] == s2["n_eval"]
     assert s1["n_intersections"] == s0["n_intersections"] == s2["n_intersections"]
@@ -518,3 +517,34 @@ def test_count_lds_pass_matches_ring_walk(gctx):
     for f in a1:
         np.testing.assert_array_equal(a1[f], a0[f], err_msg=f)
         np.testing.assert_array_equal(a2[f], a0[f], err_msg=f)
+
+
+@pytest.mark.parametrize("label_lds", [0, 2048, 65536])
+def test_label_lds_arena_keeps_results(gctx, label_lds):
+    """The wave labelling kernel keeps a pair's node arrays, segment scratch
+    and normalised output in LDS as far as label_lds bytes allow, the rest
+    in HBM: with none (0), with room for the first arrays of small pairs
+    only (2048) and with room for whole pairs (65536), every pair equals the
+    oracle, through the segment path, the clean-up, the no-crossing path
+    and the sequential fallback (which moves the nodes out of LDS first)."""
+    cases = fig_cases() + random_cases(48, seed0=77) + snapped_cases(90)
+    rs = np.random.RandomState(5)
+    for s in range(6):  # nested and disjoint pairs: the no-crossing path
+        P, _ = G.gen_single_pair(0, 500 + s, int(rs.randint(8, 300)), 8)
+        c = P.mean(axis=0)
+        cases.append(("nest%d" % s, P, c + 0.3 * (P - c)))
+    z = np.load(GOLDEN + "/batched_pairs.npz")
+    cases += [(n, z[n + "_P"], z[n + "_Q"]) for n in ["p156581", "p73779", "p176719"]]
+    default = gctx.tuning()
+    try:
+        gctx.set_tuning(**dict(default, label_lds=label_lds))
+        for flags, mode in ((G.CLIP_NORMALIZE, NORM), (G.CLIP_NORMALIZE | G.CLIP_NO_CLEANUP,
+                                                       NORM | 2)):
+            st = run_batch(gctx, cases, flags)
+            check_results(gctx, st, cases, mode)
+            paths = set(gctx.clip_get("label_path", len(cases)).tolist())
+            assert {G.LP_SEGMENTS, G.LP_NO_CROSSING, G.LP_SEQUENTIAL} <= paths, paths
+            if flags == G.CLIP_NORMALIZE:
+                assert G.LP_SEG_CLEANUP in paths, paths
+    finally:
+        gctx.set_tuning(**default)
