@@ -4,7 +4,10 @@
 #include <cstring>
 #include <string>
 
+#include <vector>
+
 #include "../../include/gcmf_hip.h"
+#include "../../include/gcmf_host.h"
 #include "gcmf_internal.hpp"
 #include "gcmf_join.hpp"
 
@@ -98,6 +101,7 @@ void run_clip(gcmf_ctx *ctx, const gcmf::JoinInputs &in, const int *d_pairs, lon
   gcmf::clip_pairs(ctx->c, in, d_pairs, M, opt, ctx->cr, &ctx->p1);
   ctx->c.flush_timers();
   ctx->clip_valid = true;
+  ctx->clip_nested = false;
   fill_clip_stats(ctx, st);
 }
 
@@ -230,6 +234,19 @@ int gcmf_synchronize(gcmf_ctx *ctx) {
   return guarded(ctx, [&] { HIP_CHECK(hipStreamSynchronize(ctx->c.stream)); });
 }
 
+int gcmf_mem_info(gcmf_ctx *ctx, size_t *free_bytes, size_t *total_bytes,
+                  size_t *workspace_bytes) {
+  return guarded(ctx, [&] {
+    size_t f = 0, t = 0;
+    HIP_CHECK(hipMemGetInfo(&f, &t));
+    size_t w = 0;
+    for (const auto &kv : ctx->c.ws.slots) w += kv.second.bytes;
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    if (workspace_bytes) *workspace_bytes = w;
+  });
+}
+
 int gcmf_set_timing(gcmf_ctx *ctx, int on) {
   return guarded(ctx, [&] {
     ctx->c.flush_timers();
@@ -320,6 +337,7 @@ int gcmf_clip_complex(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer 
     ctx->p1.M = M;
     ctx->c.flush_timers();
     ctx->clip_valid = true;
+    ctx->clip_nested = true;
     fill_clip_stats(ctx, stats);
   });
 }
@@ -439,6 +457,45 @@ int gcmf_join_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
   return guarded(ctx, [&] {
     run_join(ctx, base, over, jstats);
     run_clip(ctx, ctx->jin, ctx->jr.joinedPairs, ctx->jr.nJoined, flags, cstats);
+  });
+}
+
+int gcmf_clip_areas(gcmf_ctx *ctx, double *d_out) {
+  return guarded(ctx, [&] {
+    if (!ctx->clip_valid) throw gcmf::Error{GCMF_ERR_ARG, "no clip result on this context"};
+    if (ctx->cr.totalComps > 0 && !d_out) throw gcmf::Error{GCMF_ERR_ARG, "null output"};
+    gcmf::comp_areas(ctx->c, ctx->cr, d_out);
+    HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int gcmf_clip_write(gcmf_ctx *ctx, const char *path, int format) {
+  return guarded(ctx, [&] {
+    if (!ctx->clip_valid) throw gcmf::Error{GCMF_ERR_ARG, "no clip result on this context"};
+    if (!path || (format != GCMF_WRITE_POLY && format != GCMF_WRITE_WKT))
+      throw gcmf::Error{GCMF_ERR_ARG, "bad write arguments"};
+    const auto &r = ctx->cr;
+    std::vector<double> xy(2 * (size_t)r.totalVerts);
+    std::vector<int> cs((size_t)r.totalComps), nc((size_t)r.M), st((size_t)r.M);
+    gcmf::Context &c = ctx->c;
+    if (r.totalVerts > 0)
+      HIP_CHECK(hipMemcpyAsync(xy.data(), r.xy, 16 * (size_t)r.totalVerts,
+                               hipMemcpyDeviceToHost, c.stream));
+    if (r.totalComps > 0)
+      HIP_CHECK(hipMemcpyAsync(cs.data(), r.compSizes, 4 * (size_t)r.totalComps,
+                               hipMemcpyDeviceToHost, c.stream));
+    if (r.M > 0) {
+      HIP_CHECK(hipMemcpyAsync(nc.data(), r.ncomp, 4 * (size_t)r.M, hipMemcpyDeviceToHost,
+                               c.stream));
+      HIP_CHECK(hipMemcpyAsync(st.data(), r.status, 4 * (size_t)r.M, hipMemcpyDeviceToHost,
+                               c.stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    const int rc = format == GCMF_WRITE_POLY
+                       ? gcmf_result_write_poly(path, xy.data(), cs.data(), r.totalComps)
+                       : gcmf_result_write_wkt(path, xy.data(), cs.data(), nc.data(),
+                                               st.data(), r.M, ctx->clip_nested ? 1 : 0);
+    if (rc != 0) throw gcmf::Error{GCMF_ERR_ARG, std::string("cannot write ") + path};
   });
 }
 

@@ -112,6 +112,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
 void clip_complex(Context &c, const JoinInputs &in, const int *parts, long M,
                   const ClipOptions &opt, bool unionOp, ClipResult &res);
 
+// Signed area of every component of res into out[res.totalComps] (device),
+// summed as gcmf_ring_area sums it on the host.
+void comp_areas(Context &c, const ClipResult &res, double *out);
+
 }  // namespace gcmf
 
 // The state behind one gcmf_ctx handle of the C ABI: the context and the
@@ -125,6 +129,7 @@ struct gcmf_ctx {
   gcmf::MbrSortView sv;
   gcmf::JoinRefine jr;
   bool clip_valid = false;
+  bool clip_nested = false;  // the last clip was of complex polygons (or the union)
   gcmf::ClipResult cr;
   gcmf::Phase1View p1;
 };

@@ -3986,6 +3986,42 @@ static T *wsT(Context &c, const char *name, long long count) {
   return (T *)c.ws.get(name, sizeof(T) * (size_t)(count > 0 ? count : 1));
 }
 
+// Signed area of each result component, one thread per component: the fan
+// from the component's first vertex, summed in vertex order exactly as the
+// host's gcmf_ring_area (both built without contraction), so the two agree
+// bit for bit.
+__global__ void __launch_bounds__(256)
+    k_comp_area(long nc, const int *sizes, const long long *voff, const double *xy,
+                double *out) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const long long b = voff[c];
+  const int n = sizes[c];
+  double a = 0.0;
+  if (n >= 3) {
+    const double x0 = xy[2 * b], y0 = xy[2 * b + 1];
+    double s = 0.0;
+    for (int i = 1; i + 1 < n; i++) {
+      const double ax = xy[2 * (b + i)] - x0, ay = xy[2 * (b + i) + 1] - y0;
+      const double bx = xy[2 * (b + i + 1)] - x0, by = xy[2 * (b + i + 1) + 1] - y0;
+      s += ax * by - bx * ay;
+    }
+    a = 0.5 * s;
+  }
+  out[c] = a;
+}
+
+void comp_areas(Context &c, const ClipResult &res, double *out) {
+  const long nc = res.totalComps;
+  if (nc <= 0) return;
+  long long *voff = wsT<long long>(c, "gh_area_voff", nc + 1);
+  scan_i32_to_i64(c, res.compSizes, voff, nc);
+  hipLaunchKernelGGL(k_comp_area, dim3(grid_for(nc, 256)), dim3(256), 0, c.stream, nc,
+                     (const int *)res.compSizes, (const long long *)voff,
+                     (const double *)res.xy, out);
+  HIP_CHECK(hipGetLastError());
+}
+
 void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                 const ClipOptions &opt, ClipResult &res, Phase1View *p1) {
   res = ClipResult();

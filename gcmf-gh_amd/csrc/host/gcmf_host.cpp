@@ -363,4 +363,132 @@ gcmf_layer *gcmf_layer_read_text(const char *path, long max_poly) {
   return L;
 }
 
+// ------------------------------------------------------------ results ----
+double gcmf_ring_area(const double *xy, long n) {
+  if (n < 3) return 0.0;
+  const double x0 = xy[0], y0 = xy[1];
+  double s = 0.0;
+  for (long i = 1; i + 1 < n; i++) {
+    const double ax = xy[2 * i] - x0, ay = xy[2 * i + 1] - y0;
+    const double bx = xy[2 * i + 2] - x0, by = xy[2 * i + 3] - y0;
+    s += ax * by - bx * ay;
+  }
+  return 0.5 * s;
+}
+
+int gcmf_result_areas(const double *xy, const int *comp_sizes, long ncomps, double *out) {
+  if (ncomps < 0 || (ncomps > 0 && (!xy || !comp_sizes || !out))) return -1;
+  long v = 0;
+  for (long c = 0; c < ncomps; c++) {
+    if (comp_sizes[c] < 0) return -1;
+    out[c] = gcmf_ring_area(xy + 2 * v, comp_sizes[c]);
+    v += comp_sizes[c];
+  }
+  return 0;
+}
+
+int gcmf_result_write_poly(const char *path, const double *xy, const int *comp_sizes,
+                           long ncomps) {
+  if (!path || ncomps < 0 || (ncomps > 0 && (!xy || !comp_sizes))) return -1;
+  FILE *f = std::fopen(path, "w");
+  if (!f) return -1;
+  long v = 0;
+  for (long c = 0; c < ncomps; c++)
+    for (int i = 0; i < comp_sizes[c]; i++, v++)
+      std::fprintf(f, "%.17g %.17g%s\n", xy[2 * v], xy[2 * v + 1],
+                   i + 1 < comp_sizes[c] ? "," : ";");
+  return std::fclose(f) == 0 ? 0 : -1;
+}
+
+namespace {
+// crossing-number point-in-ring test (the rule of polygon::pointInPoly's
+// parity without its on-edge cases)
+bool ring_contains(const double *r, int n, double x, double y) {
+  bool in = false;
+  for (int i = 0, j = n - 1; i < n; j = i++) {
+    const double xi = r[2 * i], yi = r[2 * i + 1], xj = r[2 * j], yj = r[2 * j + 1];
+    if ((yi > y) != (yj > y) && x < (xj - xi) * (y - yi) / (yj - yi) + xi) in = !in;
+  }
+  return in;
+}
+
+void wkt_ring(FILE *f, const double *r, int n) {
+  std::fputc('(', f);
+  for (int i = 0; i <= n; i++) {
+    const int k = i < n ? i : 0;  // WKT repeats the first vertex
+    std::fprintf(f, "%s%.17g %.17g", i ? ", " : "", r[2 * k], r[2 * k + 1]);
+  }
+  std::fputc(')', f);
+}
+}  // namespace
+
+int gcmf_result_write_wkt(const char *path, const double *xy, const int *comp_sizes,
+                          const int *pair_ncomp, const int *pair_status, long npairs,
+                          int nest) {
+  if (!path || npairs < 0 || (npairs > 0 && (!pair_ncomp || !pair_status))) return -1;
+  FILE *f = std::fopen(path, "w");
+  if (!f) return -1;
+  long v = 0, c = 0;
+  std::vector<const double *> ring;
+  std::vector<int> len, depth, parent;
+  for (long k = 0; k < npairs; k++) {
+    const int nc = pair_ncomp[k];
+    ring.assign(nc, nullptr);
+    len.assign(nc, 0);
+    for (int i = 0; i < nc; i++, c++) {
+      ring[i] = xy + 2 * v;
+      len[i] = comp_sizes[c];
+      v += comp_sizes[c];
+    }
+    // even-odd nesting: a ring inside an even number of the others bounds
+    // a polygon, one inside an odd number is a hole of the innermost ring
+    // of one less depth that holds it (the depth is taken at the midpoint
+    // of the ring's first edge)
+    depth.assign(nc, 0);
+    parent.assign(nc, -1);
+    if (nest && nc > 1) {
+      for (int i = 0; i < nc; i++) {
+        if (len[i] < 2) continue;
+        const double mx = 0.5 * (ring[i][0] + ring[i][2]), my = 0.5 * (ring[i][1] + ring[i][3]);
+        std::vector<int> in;
+        for (int j = 0; j < nc; j++)
+          if (j != i && len[j] >= 3 && ring_contains(ring[j], len[j], mx, my)) in.push_back(j);
+        depth[i] = (int)in.size();
+        if (depth[i] % 2 == 1) {
+          for (int j : in) {
+            int dj = 0;  // depth of j: the rings of `in` that hold j's point
+            const double px = 0.5 * (ring[j][0] + ring[j][2]), py = 0.5 * (ring[j][1] + ring[j][3]);
+            for (int h : in)
+              if (h != j && ring_contains(ring[h], len[h], px, py)) dj++;
+            if (dj == depth[i] - 1) parent[i] = j;
+          }
+        }
+      }
+    }
+    std::fprintf(f, "%ld\t%d\t", k, pair_status[k]);
+    int npoly = 0;
+    for (int i = 0; i < nc; i++)
+      if (parent[i] < 0 && depth[i] % 2 == 0) npoly++;
+    if (npoly == 0) {
+      std::fputs("MULTIPOLYGON EMPTY\n", f);
+      continue;
+    }
+    std::fputs("MULTIPOLYGON (", f);
+    int w = 0;
+    for (int i = 0; i < nc; i++) {
+      if (parent[i] >= 0 || depth[i] % 2 != 0) continue;
+      std::fputs(w++ ? ", (" : "(", f);
+      wkt_ring(f, ring[i], len[i]);
+      for (int h = 0; h < nc; h++)
+        if (parent[h] == i) {
+          std::fputs(", ", f);
+          wkt_ring(f, ring[h], len[h]);
+        }
+      std::fputc(')', f);
+    }
+    std::fputs(")\n", f);
+  }
+  return std::fclose(f) == 0 ? 0 : -1;
+}
+
 }  // extern "C"

@@ -119,14 +119,8 @@ void write_results(gcmf::Context &c, const gcmf::ClipResult &r) {
   HIP_CHECK(hipMemcpyAsync(cs.data(), r.compSizes, 4 * (size_t)r.totalComps,
                            hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
-  FILE *f = std::fopen(path, "w");
-  if (!f) throw gcmf::Error{GCMF_ERR_ARG, std::string("cannot write ") + path};
-  long v = 0;
-  for (int comp : cs) {
-    for (int i = 0; i < comp; i++, v++)
-      std::fprintf(f, "%.17g %.17g%s\n", xy[2 * v], xy[2 * v + 1], i + 1 < comp ? "," : ";");
-  }
-  std::fclose(f);
+  if (gcmf_result_write_poly(path, xy.data(), cs.data(), r.totalComps) != 0)
+    throw gcmf::Error{GCMF_ERR_ARG, std::string("cannot write ") + path};
 }
 
 // Predicate frame of ghcuda / GH_CUDA: the normalised, edge-scaled frame

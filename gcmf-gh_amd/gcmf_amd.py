@@ -58,6 +58,12 @@ def host_lib():
                                            C.POINTER(vp)]
         L.gcmf_gen_single_pair.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_double,
                                            C.POINTER(vp), C.POINTER(vp)]
+        L.gcmf_ring_area.argtypes = [f64p, C.c_long]
+        L.gcmf_ring_area.restype = C.c_double
+        L.gcmf_result_areas.argtypes = [f64p, i32p, C.c_long, f64p]
+        L.gcmf_result_write_poly.argtypes = [C.c_char_p, f64p, i32p, C.c_long]
+        L.gcmf_result_write_wkt.argtypes = [C.c_char_p, f64p, i32p, i32p, i32p, C.c_long,
+                                            C.c_int]
         _host = L
     return _host
 
@@ -227,6 +233,7 @@ CLIP_GPU_ROUTE = 8
 CLIP_SCALE_EDGES = 16  # accepted, no effect: the edge scale is part of CLIP_NORMALIZE
 CLIP_UNIT_FRAME = 32
 CLIP_UNION = 64
+WRITE_POLY, WRITE_WKT = 0, 1
 NOALLOC_SORT_MAX = 1 << 21
 
 
@@ -249,6 +256,8 @@ def hip_lib():
         L.gcmf_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
         L.gcmf_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
         L.gcmf_synchronize.argtypes = [vp]
+        L.gcmf_mem_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                                    C.POINTER(C.c_size_t)]
         L.gcmf_set_timing.argtypes = [vp, C.c_int]
         L.gcmf_set_timing_only.argtypes = [vp, C.c_char_p]
         L.gcmf_timer_reset.argtypes = [vp]
@@ -260,6 +269,8 @@ def hip_lib():
         L.gcmf_join_clip.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), C.c_int,
                                      C.POINTER(JoinStats), C.POINTER(ClipStats)]
         L.gcmf_clip_get.argtypes = [vp, C.c_int, vp, C.c_long]
+        L.gcmf_clip_areas.argtypes = [vp, vp]
+        L.gcmf_clip_write.argtypes = [vp, C.c_char_p, C.c_int]
         L.gcmf_radix_sort_u64.argtypes = [vp, vp, vp, vp, vp, C.c_long, C.c_int]
         L.gcmf_radix_sort_u64_noalloc.argtypes = [vp, vp, vp, vp, vp, C.c_long, C.c_int]
         L.gcmf_scan_i32_i64.argtypes = [vp, vp, vp, C.c_long]
@@ -382,6 +393,14 @@ class Context:
 
     def synchronize(self):
         self._check(hip_lib().gcmf_synchronize(self.h), "synchronize")
+
+    def mem_info(self):
+        """(free, total) device bytes of the context's GPU and the bytes its
+        grow-only workspace holds."""
+        f, t, w = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        self._check(hip_lib().gcmf_mem_info(self.h, C.byref(f), C.byref(t), C.byref(w)),
+                    "mem_info")
+        return f.value, t.value, w.value
 
     def set_timing(self, on, only=None):
         """Kernel timing on/off; `only` restricts it to one kernel name."""
@@ -523,6 +542,62 @@ class Context:
                 v += n
             out.append(comps)
         return status, out
+
+
+    def clip_areas(self, stats):
+        """Signed area of every component of the last clip, computed on the
+        device (counter-clockwise positive)."""
+        n = int(stats["total_comps"])
+        d = DeviceBuffer(self, 8 * max(n, 1))
+        try:
+            self._check(hip_lib().gcmf_clip_areas(self.h, d.ptr), "clip_areas")
+            return self.download(d, np.float64, n)
+        finally:
+            d.free()
+
+    def clip_write(self, path, fmt=WRITE_POLY):
+        """Writes the last clip's results (.poly or one WKT line per pair)."""
+        self._check(hip_lib().gcmf_clip_write(self.h, path.encode(), int(fmt)), "clip_write")
+
+
+def _results_arrays(results):
+    """(xy, comp_sizes, ncomp per pair) of per-pair lists of (n, 2) arrays."""
+    comps = [c for r in results for c in r]
+    xy = (np.concatenate([np.asarray(c, np.float64).reshape(-1) for c in comps])
+          if comps else np.zeros(0))
+    sizes = np.array([len(c) for c in comps], np.int32)
+    ncomp = np.array([len(r) for r in results], np.int32)
+    return np.ascontiguousarray(xy), sizes, ncomp
+
+
+def ring_area(ring):
+    """Signed area of one ring (gcmf_ring_area)."""
+    r = np.ascontiguousarray(ring, np.float64)
+    return host_lib().gcmf_ring_area(r, len(r))
+
+
+def result_areas(results):
+    """gcmf_result_areas over per-pair lists of components."""
+    xy, sizes, _ = _results_arrays(results)
+    out = np.zeros(len(sizes))
+    rc = host_lib().gcmf_result_areas(xy, sizes, len(sizes), out)
+    if rc != 0:
+        raise GcmfError("gcmf_result_areas failed")
+    return out
+
+
+def result_write(path, results, status, fmt=WRITE_POLY, nest=False):
+    """Writes per-pair lists of components (gcmf_result_write_poly / _wkt)."""
+    xy, sizes, ncomp = _results_arrays(results)
+    st = np.ascontiguousarray(status, np.int32)
+    L = host_lib()
+    if fmt == WRITE_POLY:
+        rc = L.gcmf_result_write_poly(path.encode(), xy, sizes, len(sizes))
+    else:
+        rc = L.gcmf_result_write_wkt(path.encode(), xy, sizes, ncomp, st, len(ncomp),
+                                     1 if nest else 0)
+    if rc != 0:
+        raise GcmfError("cannot write " + path)
 
 
 class RefContext(Context):

@@ -10,10 +10,14 @@ HIP stream) in a thread of their own, so that joins and clip batches overlap
 on the device.  Each request's latency runs from its arrival to the moment
 its result is ready on the host side of the loop.
 
-The budget is charged with an estimate of the device memory a request needs
-(workspace bytes per vertex of its pair, or per polygon of its layers) while
-the request is admitted and released when it completes; a request that does
-not fit waits for the next batch.
+Admission is by measured device memory.  The budget is the free HBM that
+hipMemGetInfo reports once the pools are resident (gcmf_mem_info; at most
+`hbm_budget`), less a reserve.  A request is charged the device memory it
+needs while it is admitted and released when it completes: bytes per vertex
+of its pair (clips) or per polygon of its layers (joins), both measured
+from the workspace growth of calibration runs on the server's own contexts
+(the largest pool pairs clipped as one batch; every layer set joined once).
+A request that does not fit waits for the next batch.
 
 The reference runs pairs one by one (ghcuda.cpp:385-425) and has no serving
 loop; this module has no reference counterpart.
@@ -27,15 +31,18 @@ import numpy as np
 
 import gcmf_amd as G
 
-# device workspace of the batched clip per vertex of a pair (ring copies,
-# counts, scans, slots, node and output regions), and of the join per polygon
-CLIP_BYTES_PER_VERTEX = 2048
-JOIN_BYTES_PER_POLYGON = 4096
+# floors of the per-vertex / per-polygon charges (the measured values are
+# used when larger), the share of free HBM left unbudgeted, and the number
+# of largest pool pairs in the clip calibration batch
+CLIP_BYTES_PER_VERTEX = 256
+JOIN_BYTES_PER_POLYGON = 256
+HBM_RESERVE = 0.1
+CALIBRATION_PAIRS = 256
 
 
 class Request:
     __slots__ = ("rid", "kind", "arrival", "key", "cost", "t_start", "t_done", "batch",
-                 "status", "summary", "digest")
+                 "status", "summary", "digest", "result")
 
     def __init__(self, rid, kind, arrival, key):
         self.rid, self.kind, self.arrival, self.key = rid, kind, arrival, key
@@ -45,6 +52,7 @@ class Request:
         self.status = None
         self.summary = None
         self.digest = None
+        self.result = None
 
     @property
     def latency(self):
@@ -87,15 +95,19 @@ class Server:
         self.dQ = G.DeviceLayer(self.cctx, pair_Q)
         self.joins = [(G.DeviceLayer(self.jctx, b), G.DeviceLayer(self.jctx, o))
                       for b, o in join_sets]
-        self.join_cost = [JOIN_BYTES_PER_POLYGON * (b.npoly + o.npoly) for b, o in join_sets]
-        nv = (pair_P.vnum.astype(np.int64) - 1) + (pair_Q.vnum.astype(np.int64) - 1)
-        self.pair_cost = CLIP_BYTES_PER_VERTEX * nv
-        self.budget = float(hbm_budget)
         self.max_batch = int(max_batch)
         self.flags = flags
         self.keep = keep_results
         self.d_pairs = self.cctx.upload(np.zeros(2 * self.max_batch, np.int32))
         self.h_pairs = np.zeros(2 * self.max_batch, np.int32)
+        self.pair_verts = (pair_P.vnum.astype(np.int64) - 1) + (pair_Q.vnum.astype(np.int64) - 1)
+        self.join_polys = [b.npoly + o.npoly for b, o in join_sets]
+        self._calibrate()
+        self.pair_cost = self.clip_bytes_per_vertex * self.pair_verts
+        self.join_cost = [self.join_bytes_per_polygon * n for n in self.join_polys]
+        free, total, _ = self.cctx.mem_info()
+        self.hbm_free = free
+        self.budget = min(float(hbm_budget), (1.0 - HBM_RESERVE) * free)
         self._lock = threading.Lock()
         self._inflight = 0.0
         self.batches = []     # (t_start, t_done, size)
@@ -111,6 +123,28 @@ class Server:
             o.free()
         self.cctx.close()
         self.jctx.close()
+
+    # ------------------------------------------------------- calibration
+    def _calibrate(self):
+        """Measured charges: the clip context's workspace growth over one
+        batch of the largest pool pairs, per vertex, and the join context's
+        over one join of every layer set, per polygon of the largest set."""
+        n = len(self.pair_verts)
+        k = min(CALIBRATION_PAIRS, n, self.max_batch)
+        keys = np.argsort(-self.pair_verts, kind="stable")[:k].astype(np.int32)
+        w0 = self.cctx.mem_info()[2]
+        self.cctx.clip(self.dP, self.dQ, np.stack([keys, keys], axis=1), self.flags)
+        w1 = self.cctx.mem_info()[2]
+        self.clip_bytes_per_vertex = max(CLIP_BYTES_PER_VERTEX,
+                                         int(np.ceil(w1 / max(1, int(self.pair_verts[keys].sum())))))
+        self.calibration_clip_bytes = w1 - w0
+        w0 = self.jctx.mem_info()[2]
+        for b, o in self.joins:
+            self.jctx.join(b, o)
+        w1 = self.jctx.mem_info()[2]
+        big = max(self.join_polys) if self.join_polys else 1
+        self.join_bytes_per_polygon = max(JOIN_BYTES_PER_POLYGON, int(np.ceil(w1 / big)))
+        self.calibration_join_bytes = w1 - w0
 
     # ------------------------------------------------------------ budget
     def _admit(self, cost):
@@ -189,6 +223,7 @@ class Server:
                     comps = results[j]
                     r.digest = _digest(np.array([len(c) for c in comps], np.int64),
                                        *[np.asarray(c, np.float64) for c in comps])
+                    r.result = comps
                 self._release(r.cost)
 
     def _join_loop(self, reqs, t0):
@@ -209,8 +244,10 @@ class Server:
                 r.status = 0
                 r.summary = (js["candidates"], js["joined"])
                 if self.keep:
-                    r.digest = _digest(self.jctx.join_get("pairs", 2 * js["candidates"]),
-                                       self.jctx.join_get("joined", 2 * js["joined"]))
+                    cand = self.jctx.join_get("pairs", 2 * js["candidates"])
+                    joined = self.jctx.join_get("joined", 2 * js["joined"])
+                    r.digest = _digest(cand, joined)
+                    r.result = (cand.reshape(-1, 2), joined.reshape(-1, 2))
                 r.t_done = time.perf_counter() - t0
                 self._release(r.cost)
         except Exception as e:  # surfaced by run()

@@ -55,6 +55,11 @@ int gcmf_free(gcmf_ctx *ctx, void *d_ptr);
 int gcmf_memcpy_h2d(gcmf_ctx *ctx, void *d_dst, const void *src, size_t bytes);
 int gcmf_memcpy_d2h(gcmf_ctx *ctx, void *dst, const void *d_src, size_t bytes);
 int gcmf_synchronize(gcmf_ctx *ctx);
+/* Free and total device memory of the context's GPU (hipMemGetInfo) and the
+ * bytes the context's grow-only workspace holds (reused by later calls);
+ * any pointer may be null. */
+int gcmf_mem_info(gcmf_ctx *ctx, size_t *free_bytes, size_t *total_bytes,
+                  size_t *workspace_bytes);
 /* Kernel timing with HIP events recorded on the context stream. */
 int gcmf_set_timing(gcmf_ctx *ctx, int on);
 /* Restricts timing to the launches named `kernel`, or to a comma-separated
@@ -188,6 +193,19 @@ int gcmf_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
 #define GCMF_CLIP_UNION 64
 int gcmf_clip_complex(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
                       const int *d_parts, long M, int flags, gcmf_clip_stats *stats);
+
+/* Results of the context's last clip (gcmf_clip, gcmf_clip_complex):
+ * the signed area of every component (counter-clockwise positive; the sum of
+ * gcmf_ring_area, bit for bit) into d_out[total_comps] on the device, and
+ * the results as a file: GCMF_WRITE_POLY, the .poly text of savePolygon
+ * (polyclip.h:859-865) at %.17g, or GCMF_WRITE_WKT, one line per pair
+ * "pair<TAB>status<TAB>MULTIPOLYGON (...)", the components of a complex
+ * clip or a union nested by the even-odd rule (gcmf_result_write_wkt in
+ * gcmf_host.h).  The reference has neither area nor WKT. */
+#define GCMF_WRITE_POLY 0
+#define GCMF_WRITE_WKT 1
+int gcmf_clip_areas(gcmf_ctx *ctx, double *d_out);
+int gcmf_clip_write(gcmf_ctx *ctx, const char *path, int format);
 
 /* Launch shapes and thresholds of the batched clip, per context. */
 typedef struct {

@@ -64,6 +64,32 @@ int gcmf_gen_pair_layers(uint64_t seed, long nPairs, double median, double sigma
 int gcmf_gen_single_pair(int kind, uint64_t seed, int nP, int nQ, double snap_grid,
                          gcmf_layer **P, gcmf_layer **Q);
 
+/* ------------------------------------------------------------ results ----
+ * A batch's results as host arrays: xy (x, y of every result vertex, the
+ * components one after another, each vertex once), comp_sizes (vertices per
+ * component), pair_ncomp / pair_status (per pair, in pair order). */
+/* Signed area of one ring of n vertices (counter-clockwise positive):
+ * 0.5 * sum over the fan from vertex 0 of the cross products of
+ * (v_i - v_0, v_i+1 - v_0).  The reference has no area routine. */
+double gcmf_ring_area(const double *xy, long n);
+/* gcmf_ring_area of every component into out[ncomps]. */
+int gcmf_result_areas(const double *xy, const int *comp_sizes, long ncomps, double *out);
+/* savePolygon's .poly text (polyclip.h:859-865, polygon.h:575-584): one
+ * "x y," line per vertex, the component's last vertex ending in ";", at
+ * %.17g, so that every double round-trips. */
+int gcmf_result_write_poly(const char *path, const double *xy, const int *comp_sizes,
+                           long ncomps);
+/* One line per pair: "pair<TAB>status<TAB>" and the pair's result as a WKT
+ * MULTIPOLYGON (each ring closed by repeating its first vertex, %.17g;
+ * MULTIPOLYGON EMPTY when there is none).  nest = 0: every component is a
+ * polygon of its own (the intersection of two simple rings has no holes);
+ * nest = 1: components are nested by the even-odd rule, a ring inside an
+ * odd number of others becoming a hole of the innermost ring holding it
+ * (complex polygons, union). */
+int gcmf_result_write_wkt(const char *path, const double *xy, const int *comp_sizes,
+                          const int *pair_ncomp, const int *pair_status, long npairs,
+                          int nest);
+
 #ifdef __cplusplus
 }
 #endif
