@@ -357,6 +357,29 @@ __device__ __forceinline__ bool seg_hit(double bX1, double bY1, double bX2, doub
 
 constexpr int kSegStage = 4;
 
+__device__ __forceinline__ int gcd_int(int a, int b) {
+  while (b) {
+    const int t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// The order in which a pair's base edges are taken: i -> (i * S) mod bE,
+// S coprime to bE and near 0.618 bE, a permutation that spreads successive
+// rounds over the whole base list.  The flag is "some edge pair hits", so
+// the order changes only how soon a pair that hits stops: its hits cluster
+// where the rings cross, which index order reaches after half the list on
+// average and this order after about bE / (number of base edges with hits).
+__device__ __forceinline__ int seg_stride(int bE) {
+  if (bE <= 2) return 1;
+  int S = (int)(0.6180339887498949 * bE);
+  if (S < 1) S = 1;
+  while (gcd_int(S, bE) != 1) S++;
+  return S;
+}
+
 // counts come from bCount/oCount, or (when null) from the offsets; the flag
 // goes to segFlag[k] or, with flagByList, to segFlag[w] (the reference's
 // per-survivor layout).
@@ -406,12 +429,13 @@ __global__ void __launch_bounds__(256)
   // the tests; the flag is "some pair hits", which testing pairs past the
   // first hit cannot change
   const unsigned tot = (unsigned)total, uoE = (unsigned)(oE > 0 ? oE : 1);
+  const unsigned long long S = (unsigned long long)seg_stride(bE);
   for (unsigned t0 = 0; t0 < tot && !flag; t0 += 64 * kSegStage) {
     int bi[kSegStage], oi[kSegStage];
 #pragma unroll
     for (int u = 0; u < kSegStage; u++) {
       const unsigned t = t0 + 64 * u + lane;
-      bi[u] = t < tot ? (int)be[t / uoE] : -1;
+      bi[u] = t < tot ? (int)be[(unsigned)((t / uoE) * S % (unsigned long long)bE)] : -1;
       oi[u] = t < tot ? (int)oe[t % uoE] : -1;
     }
     double s[kSegStage][8];

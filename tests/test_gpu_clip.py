@@ -165,6 +165,20 @@ def test_phase1_c1_pair(gctx):
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE | G.CLIP_GPU_ROUTE)
     check_phase1(gctx, st, cases, NORM)
     assert st["n_intersections"] == O.phase1(P, Q, NORM)["psP1"][-1]
+    check_results(gctx, st, cases, NORM, gpu_route=True)
+
+
+@pytest.mark.parametrize("flags,mode", [(G.CLIP_NORMALIZE, NORM), (0, 0),
+                                        (G.CLIP_NORMALIZE | G.CLIP_NO_CLEANUP, NORM | 2)])
+def test_clip_c1_pair_results(gctx, flags, mode):
+    """The single 2000 x 1500 pair: the final clipped polygon (status,
+    component sizes, every vertex) equals the CPU route's bit for bit, in
+    the default frame, the raw frame and without clean-up."""
+    P, Q = G.gen_single_pair(0, 0, 2000, 1500)
+    cases = [("c1", P, Q)]
+    st = run_batch(gctx, cases, flags)
+    assert st["ok"] == 1 and st["total_comps"] > 0
+    check_results(gctx, st, cases, mode)
 
 
 def test_phase1_grid_snapped(gctx):
