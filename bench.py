@@ -86,13 +86,13 @@ def label_bytes(cs):
             16 * cs["total_verts"] + 4 * cs["total_comps"] + 8 * cs["pairs"])
 
 
-def label_roofline(total_ms, launches, cs, traffic):
+def label_roofline(total_ms, launches, cs, traffic, kernel="k_label_wave"):
     avg_ms = total_ms / max(launches, 1)
     b = label_bytes(cs)
     achieved = b / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-            "kernel": "k_label_wave", "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+            "kernel": kernel, "avg_launch_ms": round(avg_ms, 5), "launches": launches,
             "work": "%d algorithmic bytes per launch: 36 B x %d slots + 8 B x %d edges + "
                     "16 B x %d result vertices + 4 B x %d components + 8 B x %d pairs"
                     % (b, cs["slots_p"] + cs["slots_q"], cs["edges_p"] + cs["edges_q"],
@@ -502,6 +502,12 @@ def large(args):
         "sort_scatter", "scan_reduce", "scan_downsweep")}
     path = int(ctx.clip_get("label_path", 1)[0])
     status = int(ctx.clip_get("status", 1)[0])
+    # the kernel that labels the pair: by its node count (ring vertices and
+    # intersection copies) against the context's thresholds
+    tune = ctx.tuning()
+    nodes = cs["slots_p"] + cs["slots_q"]
+    lab_kernel = ("k_label_grid" if nodes >= tune["grid_nodes"] else
+                  "k_label_block" if nodes >= tune["big_nodes"] else "k_label_wave")
     line = {"metric": METRIC, "value": round(args.steps / elapsed, 4), "unit": "pairs/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
@@ -521,7 +527,7 @@ def large(args):
             "latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 3),
             "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 3),
             "kernel_ms_per_step": kernel_ms,
-            "roofline": label_roofline(lab_ms, lab_n, cs, None),
+            "roofline": label_roofline(lab_ms, lab_n, cs, None, lab_kernel),
             "roofline_next": count_roofline(cnt_ms, cnt_n, cs, None)}
     db.free()
     do.free()

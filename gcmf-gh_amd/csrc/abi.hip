@@ -350,6 +350,8 @@ int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out) {
     out->count_block = ctx->c.tune.countBlock;
     out->lds_max_q = ctx->c.tune.ldsMaxQ;
     out->label_lds = ctx->c.tune.labelLds;
+    out->grid_nodes = ctx->c.tune.gridNodes;
+    out->grid_clean = ctx->c.tune.gridClean;
   });
 }
 
@@ -358,13 +360,15 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
     if (!t || t->cull_min < 1 || t->big_nodes < 1 ||
         (t->count_block != 64 && t->count_block != 128 && t->count_block != 256) ||
         t->lds_max_q < 0 || t->lds_max_q > 1024 || t->label_lds < 0 || t->label_lds > 65536 ||
-        t->label_lds % 16 != 0)
+        t->label_lds % 16 != 0 || t->grid_nodes < 1 || t->grid_clean < 49)
       throw gcmf::Error{GCMF_ERR_ARG, "bad tuning"};
     ctx->c.tune.cullMin = t->cull_min;
     ctx->c.tune.bigNodes = t->big_nodes;
     ctx->c.tune.countBlock = t->count_block;
     ctx->c.tune.ldsMaxQ = t->lds_max_q;
     ctx->c.tune.labelLds = t->label_lds;
+    ctx->c.tune.gridNodes = t->grid_nodes;
+    ctx->c.tune.gridClean = t->grid_clean;
   });
 }
 
@@ -448,6 +452,8 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
       out->count_block = best.countBlock;
       out->lds_max_q = best.ldsMaxQ;
       out->label_lds = best.labelLds;
+      out->grid_nodes = best.gridNodes;
+      out->grid_clean = best.gridClean;
     }
   });
 }

@@ -21,6 +21,8 @@
 // every output vertex is produced in original coordinates from the same alpha.
 #include <functional>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "gcmf_device.hpp"
 #include "gcmf_internal.hpp"
 #include "gcmf_join.hpp"
@@ -2574,6 +2576,7 @@ struct TraceArgs {
   double *xyTmp; // [node capacity] compacted result vertices per pair
   int cleanup;
   long long bigNodes;  // pairs with this many nodes go to k_label_block
+  int gridClean;       // components longer than this in a grid-labelled pair: grid clean-up
   // links given on both rings (nbP and nbQ, the CPU route); otherwise only
   // the Q slots' links are read and applied with link() (the GPU route)
   int explicitLinks;
@@ -2722,40 +2725,133 @@ __device__ __forceinline__ int lt_count(unsigned long long m) {
   return __popcll(m & ((1ULL << __lane_id()) - 1ULL));
 }
 
-// Group-wide reductions and scans.  A group is the W threads of one block:
-// one wave (W = 64, pairs of ordinary size) or sixteen (W = 1024, the large
-// pairs).  Every thread of the group must call them, with uniform control
-// flow; the LDS slots are guarded by a barrier on entry.
+// Group-wide reductions and scans.  A group is the W threads of one block,
+// one wave (W = 64, pairs of ordinary size) or sixteen (W = 1024, large
+// pairs), or, for W = kGrid, every thread of a cooperative launch (pairs of
+// a million nodes and more): grp_rank / grp_size / grp_sync stand for the
+// thread's index, the group's size and its barrier.  Every thread of the
+// group must call them, with control flow uniform over the group; the LDS
+// slots are guarded by a barrier on entry.  A grid-wide reduction leaves
+// one partial per block in the partials buffer (grid_part, two halves used
+// in turn, so a block never overwrites a half another block still reads),
+// syncs the grid, and lets every block combine the partials it needs.
+constexpr int kGrid = 0;
+
+__shared__ long long *grid_part;  // [2 * gridDim.x], set by the grid kernel
+__shared__ int grid_half;         // the half the next grid reduction uses
+
 template <int W>
-__device__ __forceinline__ int grp_sum(int v) {
-  v = wave_sum(v);
-  if (W == 64) return v;
-  __shared__ int red[W / 64];
+__device__ __forceinline__ int grp_rank() {
+  return W == kGrid ? (int)(blockIdx.x * blockDim.x + threadIdx.x) : (int)threadIdx.x;
+}
+template <int W>
+__device__ __forceinline__ int grp_size() {
+  return W == kGrid ? (int)(gridDim.x * blockDim.x) : W;
+}
+template <int W>
+__device__ __forceinline__ void grp_sync() {
+  if constexpr (W == kGrid) cooperative_groups::this_grid().sync();
+  else __syncthreads();
+}
+
+// block-wide sum of v (blockDim.x <= 1024); every thread of the block calls it
+__device__ __forceinline__ long long block_sum_ll(long long v) {
+  __shared__ long long red[16];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   __syncthreads();
   if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  int t = 0;
-  for (int i = 0; i < W / 64; i++) t += red[i];
+  long long t = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += red[i];
   return t;
+}
+
+// the grid-wide combination of one value per block (its block total b):
+// returns the sum over blocks before this one and sets total; op 0 sums,
+// op 1 takes the minimum (before then unused)
+__device__ long long grid_combine(long long b, int op, long long &total) {
+  long long *part = grid_part + (long long)grid_half * gridDim.x;
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+  cooperative_groups::this_grid().sync();
+  long long before = 0, all = op == 0 ? 0 : 0x7fffffffffffffffLL;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+    const long long x = part[i];
+    if (op == 0) {
+      all += x;
+      if (i < (int)blockIdx.x) before += x;
+    } else {
+      all = min(all, x);
+    }
+  }
+  if (op == 0) {
+    before = block_sum_ll(before);
+    all = block_sum_ll(all);
+  } else {
+    __shared__ long long red[16];
+    for (int off = 32; off > 0; off >>= 1) all = min(all, __shfl_xor(all, off, 64));
+    __syncthreads();
+    if (__lane_id() == 0) red[threadIdx.x >> 6] = all;
+    __syncthreads();
+    all = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) all = min(all, red[i]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) grid_half ^= 1;
+  __syncthreads();
+  total = all;
+  return before;
+}
+
+template <int W>
+__device__ __forceinline__ int grp_sum(int v) {
+  v = wave_sum(v);
+  if constexpr (W == 64) {
+    return v;
+  } else if constexpr (W == kGrid) {
+    long long total;
+    grid_combine(block_sum_ll(__lane_id() == 0 ? v : 0), 0, total);
+    return (int)total;
+  } else {
+    __shared__ int red[W / 64];
+    __syncthreads();
+    if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int t = 0;
+    for (int i = 0; i < W / 64; i++) t += red[i];
+    return t;
+  }
 }
 template <int W>
 __device__ __forceinline__ int grp_min(int v) {
   v = wave_min(v);
-  if (W == 64) return v;
-  __shared__ int red[W / 64];
-  __syncthreads();
-  if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  int t = red[0];
-  for (int i = 1; i < W / 64; i++) t = min(t, red[i]);
-  return t;
+  if constexpr (W == 64) {
+    return v;
+  } else if constexpr (W == kGrid) {
+    __shared__ int red[16];
+    __syncthreads();
+    if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int t = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) t = min(t, red[i]);
+    long long total;
+    grid_combine(t, 1, total);
+    return (int)total;
+  } else {
+    __shared__ int red[W / 64];
+    __syncthreads();
+    if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int t = red[0];
+    for (int i = 1; i < W / 64; i++) t = min(t, red[i]);
+    return t;
+  }
 }
 template <int W>
 __device__ __forceinline__ bool grp_any(bool b) {
   if (W == 64) return __ballot(b) != 0ULL;
-  return grp_sum<W>(__ballot(b) != 0ULL ? 1 : 0) != 0;
+  return grp_sum<W>(__ballot(b) != 0ULL && __lane_id() == 0 ? 1 : 0) != 0;
 }
-// exclusive prefix of v over the group in thread order; total: the sum
+// exclusive prefix of v over the group in rank order; total: the sum
 template <int W>
 __device__ __forceinline__ int grp_scan(int v, int &total) {
   const int lane = __lane_id();
@@ -2764,17 +2860,49 @@ __device__ __forceinline__ int grp_scan(int v, int &total) {
     const int t = __shfl_up(incl, o, 64);
     if (lane >= o) incl += t;
   }
-  if (W == 64) {
+  if constexpr (W == 64) {
     total = __shfl(incl, 63, 64);
     return incl - v;
+  } else {
+    constexpr int kParts = W == kGrid ? 16 : W / 64;
+    const int nw = W == kGrid ? (int)(blockDim.x >> 6) : W / 64;
+    __shared__ int part[kParts];
+    __syncthreads();
+    if (lane == 63) part[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    int before = 0, tot = 0;
+    for (int i = 0; i < nw; i++) {
+      if (i < w) before += part[i];
+      tot += part[i];
+    }
+    if constexpr (W == kGrid) {
+      long long all;
+      const long long pre = grid_combine(tot, 0, all);
+      total = (int)all;
+      return (int)pre + before + incl - v;
+    } else {
+      total = tot;
+      return before + incl - v;
+    }
   }
-  __shared__ int part[W / 64];
+}
+
+// exclusive prefix of v over the block in thread order; total: the block's sum
+__device__ __forceinline__ long long block_scan_ll(long long v, long long &total) {
+  const int lane = __lane_id();
+  long long incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  __shared__ long long part[16];
   __syncthreads();
   if (lane == 63) part[threadIdx.x >> 6] = incl;
   __syncthreads();
   const int w = threadIdx.x >> 6;
-  int before = 0, tot = 0;
-  for (int i = 0; i < W / 64; i++) {
+  long long before = 0, tot = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
     if (i < w) before += part[i];
     tot += part[i];
   }
@@ -2782,11 +2910,77 @@ __device__ __forceinline__ int grp_scan(int v, int &total) {
   return before + incl - v;
 }
 
+// grp_scan for int or long long values (a wave or one block)
+template <int W, typename T>
+__device__ __forceinline__ T grp_scan_t(T v, T &total) {
+  if constexpr (sizeof(T) == sizeof(int)) {
+    int tot;
+    const int pre = grp_scan<W>((int)v, tot);
+    total = tot;
+    return pre;
+  } else if constexpr (W == 64) {
+    const int lane = __lane_id();
+    long long incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    total = __shfl(incl, 63, 64);
+    return incl - v;
+  } else {
+    long long tot;
+    const long long pre = block_scan_ll(v, tot);
+    total = tot;
+    return pre;
+  }
+}
+
+// The range [0, n) in rank order with the exclusive running sum of v: calls
+// f(j, prefix, x) for every j, where x = v(j) and prefix is the sum of v over
+// [0, j); returns the sum over [0, n).  A wave or block walks the range in
+// steps of its size.  The whole grid gives each block one contiguous tile:
+// v is summed over every tile first (one grid-wide combination), then
+// evaluated again as each block walks its tile in blockDim steps with
+// block-wide scans, so the grid needs one grid barrier where stepping by
+// its size needs one per step.  v must not read what f writes in another
+// tile, and f's stores must not reach another tile's reads.
+template <int W, typename T, class V, class F>
+__device__ T grp_scan_range(int n, V v, F f) {
+  T carry = 0;
+  if constexpr (W == kGrid) {
+    const int tile = (int)(((long long)n + gridDim.x - 1) / gridDim.x);
+    const int t0 = (int)min((long long)blockIdx.x * tile, (long long)n), t1 = min(t0 + tile, n);
+    long long s = 0;
+    for (int j = t0 + (int)threadIdx.x; j < t1; j += blockDim.x) s += (long long)v(j);
+    long long total;
+    carry = (T)grid_combine(block_sum_ll(s), 0, total);
+    for (int b = t0; b < t1; b += blockDim.x) {
+      const int j = b + (int)threadIdx.x;
+      const T x = j < t1 ? v(j) : T(0);
+      long long tot;
+      const T pre = (T)block_scan_ll((long long)x, tot);
+      if (j < t1) f(j, (T)(carry + pre), x);
+      carry += (T)tot;
+    }
+    return (T)total;
+  } else {
+    for (int b = 0; b < n; b += W) {
+      const int j = b + grp_rank<W>();
+      const T x = j < n ? v(j) : T(0);
+      T tot;
+      const T pre = grp_scan_t<W, T>(x, tot);
+      if (j < n) f(j, (T)(carry + pre), x);
+      carry += tot;
+    }
+    return carry;
+  }
+}
+
 // parity of the winding number of R over the ring [r0, r0+n) in index order
 template <int W>
 __device__ bool grp_pip(const Nodes &N, int r0, int n, pt R) {
   int w = 0;
-  for (int a = threadIdx.x; a < n; a += W) {
+  for (int a = grp_rank<W>(); a < n; a += grp_size<W>()) {
     int ia = r0 + a, ib = r0 + (a + 1 == n ? 0 : a + 1);
     pt P0 = P_(N, ia), P1 = P_(N, ib);
     if ((P0.y < R.y) != (P1.y < R.y)) {
@@ -2805,7 +2999,7 @@ __device__ bool grp_pip(const Nodes &N, int r0, int n, pt R) {
 template <int W>
 __device__ int grp_first_nonint(const Nodes &N, int r0, int n) {
   int best = 0x7fffffff;
-  for (int a = threadIdx.x; a < n; a += W)
+  for (int a = grp_rank<W>(); a < n; a += grp_size<W>())
     if (!isint(N, r0 + a)) best = min(best, a);
   best = grp_min<W>(best);
   return best == 0x7fffffff ? -1 : r0 + best;
@@ -2821,24 +3015,25 @@ __device__ __forceinline__ bool iscross(const Nodes &N, int v) {
 // EXIT are flagged for splitting
 template <int W>
 __device__ void grp_enex(Nodes &N, int r0, int n, int V, int status0) {
-  int carry = 0;
+  // i-th vertex after V in ring order, i = 1..n-1
   const int v0 = V - r0;
-  for (int base = 1; base < n; base += W) {
-    const int j = base + threadIdx.x;
-    int u = v0 + j;
-    u = r0 + (u >= n ? u - n : u);
-    const bool in = j < n && isint(N, u);
-    const bool cr = in && N.lab[u] == L_CROSSING;
-    int tot;
-    const int pre = grp_scan<W>(cr ? 1 : 0, tot);
-    if (in) {
-      const int par = (carry + pre) & 1;
-      const int status = par ? (status0 == ENTRY ? EXIT : ENTRY) : status0;
-      if (cr) N.enex[u] = (signed char)status;
-      else if (status == EXIT) N.fl[u] |= F_SPLIT;
-    }
-    carry += tot;
-  }
+  auto node = [&](int i) {
+    const int u = v0 + i;
+    return r0 + (u >= n ? u - n : u);
+  };
+  grp_scan_range<W, int>(
+      n - 1,
+      [&](int j) {
+        const int u = node(j + 1);
+        return isint(N, u) && N.lab[u] == L_CROSSING ? 1 : 0;
+      },
+      [&](int j, int pre, int cr) {
+        const int u = node(j + 1);
+        if (!isint(N, u)) return;
+        const int status = (pre & 1) ? (status0 == ENTRY ? EXIT : ENTRY) : status0;
+        if (cr) N.enex[u] = (signed char)status;
+        else if (status == EXIT) N.fl[u] |= F_SPLIT;
+      });
 }
 
 // Departures of createResult (polyclip.h:743-783).  Every CROSSING vertex
@@ -2909,7 +3104,7 @@ enum { SEG_DONE = 0, SEG_SLOW = 2, SEG_OVERFLOW = 3 };
 __host__ __device__ constexpr int even_up(int v) { return (v + 1) & ~1; }
 // ints of segment scratch for a pair of n nodes and at most Dc departures
 // (Dc even): rg[0, n) and Dc-sized departure arrays for the trace, and for
-// the clean-up after it the position map rg[0, n) plus 3 ints per component
+// the clean-up after it the position map rg[0, n) plus 4 ints per component
 // (at most Dc / 2 + 2) and one per output position (at most n).  Within the
 // 10 n + 20 ints of a pair's global region whenever Dc <= even_up(n).
 __host__ __device__ constexpr int seg_scratch_ints(int n, int Dc) {
@@ -2947,7 +3142,7 @@ struct Arena {
 template <int W>
 __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, int *rg, int nP,
                              int nQ, int Dc, int *omap) {
-  const int lane = threadIdx.x;
+  const int lane = grp_rank<W>();
   const int n = nP + nQ, nE = even_up(n);
   // rk: per node; the rest per departure
   int *rk = rg, *dn = rg + nE;
@@ -2955,31 +3150,30 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   int *nx[2] = {rg + nE + 5 * Dc, rg + nE + 6 * Dc};
   int *cbase = rg + nE + 7 * Dc, *off = rg + nE + 8 * Dc;
   // departures by ring order
-  int carry = 0, XP = 0;
+  int XP = 0, XQ = 0;
   for (int ring = 0; ring < 2; ring++) {
-    const int rb = ring ? nP : 0, nR = ring ? nQ : nP;
-    for (int b = 0; b < nR; b += W) {
-      const int j = b + lane, u = rb + j;
-      const int mu = j < nR ? dmult(N, u) : 0;
-      int tot;
-      const int r = carry + grp_scan<W>(mu, tot);
-      if (mu > 0 && r + mu <= Dc) {
-        rk[u] = r;
-        dn[r] = 2 * u;
-        if (mu == 2) dn[r + 1] = 2 * u + 1;
-      }
-      carry += tot;
-    }
-    if (ring == 0) XP = carry;
+    const int rb = ring ? nP : 0, nR = ring ? nQ : nP, r0 = ring ? XP : 0;
+    const int tot = grp_scan_range<W, int>(
+        nR, [&](int j) { return dmult(N, rb + j); },
+        [&](int j, int pre, int mu) {
+          const int u = rb + j, r = r0 + pre;
+          if (mu > 0 && r + mu <= Dc) {
+            rk[u] = r;
+            dn[r] = 2 * u;
+            if (mu == 2) dn[r + 1] = 2 * u + 1;
+          }
+        });
+    if (ring == 0) XP = tot;
+    else XQ = tot;
   }
-  const int XQ = carry - XP, D = carry;
+  const int D = XP + XQ;
   if (XP != XQ || XP == 0 || D > Dc) return SEG_SLOW;
-  __syncthreads();
+  grp_sync<W>();
   // successor of each departure: the departure from the neighbour of its
   // arrival; the arrival must carry the status the inner loop stops on.
   // Key .x: the P departure of the departure's intersection point.
   bool bad = false;
-  for (int d = lane; d < D; d += W) {
+  for (int d = lane; d < D; d += grp_size<W>()) {
     const Departure p = departure(N, dn, d, nP, nQ, XP, XQ);
     const bool onP = d < XP;
     const int ua = dn[p.arr] >> 1, u = p.code >> 1;
@@ -3009,18 +3203,20 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   while ((1 << rounds) < D) rounds++;
   int cur = 0;
   for (int t = 0; t < rounds; t++) {
-    __syncthreads();
+    // for the whole grid, the grid barrier of the grp_any before orders the
+    // last round's stores
+    if constexpr (W != kGrid) grp_sync<W>();
     const int2 *__restrict__ vs = vc[cur];
     const int *__restrict__ ns = nx[cur];
     int2 *__restrict__ vd = vc[cur ^ 1];
     int *__restrict__ nd = nx[cur ^ 1];
     bool changed = false;
-    for (int d0 = lane; d0 < D; d0 += 4 * W) {
+    for (int d0 = lane; d0 < D; d0 += 4 * grp_size<W>()) {
       int x[4];
       int2 a[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int d = d0 + W * k;
+        const int d = d0 + grp_size<W>() * k;
         x[k] = d < D ? ns[d] : 0;
         a[k] = d < D ? vs[d] : make_int2(0, 0);
       }
@@ -3033,7 +3229,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       }
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int d = d0 + W * k;
+        const int d = d0 + grp_size<W>() * k;
         if (d < D) {
           const int2 m = make_int2(min(a[k].x, b[k].x), min(a[k].y, b[k].y));
           changed |= m.x != a[k].x || m.y != a[k].y;
@@ -3045,12 +3241,12 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     cur ^= 1;
     if (!grp_any<W>(changed)) break;
   }
-  __syncthreads();
+  grp_sync<W>();
   const int2 *mc = vc[cur];
   // a cycle is traced when it holds the departure of its smallest P key; its
   // list starts there.  S: segment lengths, then suffix sums
   int *S[2] = {(int *)vc[cur ^ 1], (int *)vc[cur ^ 1] + Dc};
-  for (int d = lane; d < D; d += W) {
+  for (int d = lane; d < D; d += grp_size<W>()) {
     const int2 c = mc[d];
     const int sd = c.x;
     const bool sel = mc[sd].y == c.y;
@@ -3064,26 +3260,26 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     S[0][d] = off[d];
     nx[0][d] = nxt;
   }
-  __syncthreads();
+  grp_sync<W>();
   // each intersection point departs exactly once over the traced cycles
-  for (int d = lane; d < XP; d += W)
+  for (int d = lane; d < XP; d += grp_size<W>())
     if (S[1][d] == S[1][dep_partner(N, rk, dn[d], nP, nQ)]) bad = true;
   if (grp_any<W>(bad)) return SEG_SLOW;
   // list ranking of the traced lists (nxt = -1 ends a list): the loop ends
   // once no element still points anywhere
   cur = 0;
   for (int t = 0; t < rounds; t++) {
-    __syncthreads();
+    if constexpr (W != kGrid) grp_sync<W>();
     const int *__restrict__ ss = S[cur];
     const int *__restrict__ ns = nx[cur];
     int *__restrict__ sd = S[cur ^ 1];
     int *__restrict__ nd = nx[cur ^ 1];
     bool live = false;
-    for (int d0 = lane; d0 < D; d0 += 4 * W) {
+    for (int d0 = lane; d0 < D; d0 += 4 * grp_size<W>()) {
       int x[4], a[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int d = d0 + W * k;
+        const int d = d0 + grp_size<W>() * k;
         x[k] = d < D ? ns[d] : -1;
         a[k] = d < D ? ss[d] : 0;
       }
@@ -3095,7 +3291,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       }
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int d = d0 + W * k;
+        const int d = d0 + grp_size<W>() * k;
         if (d < D) {
           sd[d] = a[k] + b[k];
           nd[d] = y[k];
@@ -3106,37 +3302,31 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
     cur ^= 1;
     if (!grp_any<W>(live)) break;
   }
-  __syncthreads();
+  grp_sync<W>();
   const int *Sf = S[cur];
   // components in order of their start departure
   int *cix = nx[0], *ciD = nx[1];
-  int vcarry = 0, ccarry = 0;
-  for (int b = 0; b < XP; b += W) {
-    const int d = b + lane;
-    const bool start = d < XP && mc[d].x == d;
-    const int len = start ? Sf[d] : 0;
-    int ctot, vtot;
-    const int cpre = grp_scan<W>(start ? 1 : 0, ctot);
-    const int vpre = grp_scan<W>(len, vtot);
-    if (start) {
-      const int ci = ccarry + cpre, base = vcarry + vpre;
-      cbase[d] = base;
-      cix[d] = ci;
-      if (ci < O.capComp) {
-        O.compRoot[ci] = base;
-        clen[ci] = len;
-      }
-    }
-    vcarry += vtot;
-    ccarry += ctot;
-  }
+  // one scan of (components << 32) + vertices
+  const long long vc_all = grp_scan_range<W, long long>(
+      XP, [&](int d) { return mc[d].x == d ? (1LL << 32) + Sf[d] : 0LL; },
+      [&](int d, long long pre, long long x) {
+        if (x == 0) return;
+        const int ci = (int)(pre >> 32), base = (int)(pre & 0xffffffffLL);
+        cbase[d] = base;
+        cix[d] = ci;
+        if (ci < O.capComp) {
+          O.compRoot[ci] = base;
+          clen[ci] = (int)(x & 0xffffffffLL);
+        }
+      });
+  const int vcarry = (int)(vc_all & 0xffffffffLL), ccarry = (int)(vc_all >> 32);
   if (vcarry > O.cap || ccarry > O.capComp) return SEG_OVERFLOW;
   // the output position map holds n entries
   if (vcarry > n) return SEG_SLOW;
   O.n = vcarry;
   O.ncomp = ccarry;
-  __syncthreads();
-  for (int d = lane; d < D; d += W) {
+  grp_sync<W>();
+  for (int d = lane; d < D; d += grp_size<W>()) {
     const int2 c = mc[d];
     const int sd = c.x;
     if (mc[sd].y == c.y) {
@@ -3146,7 +3336,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       off[d] = -1;
     }
   }
-  __syncthreads();
+  grp_sync<W>();
   // a ring vertex lies on the forward segment of the departure before it
   // and on the backward segment of the departure after it; of these two,
   // which are reverses of each other, at most one is traced, except at a
@@ -3156,16 +3346,45 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
   // departures' statuses, components and the vertices' coordinates, is
   // issued for all the chunks before it is used; the stores come last)
   int *cw = rk;  // output position -> component
+  if constexpr (W == kGrid) {
+    // the whole grid: one vertex per thread at a time, in block tiles
+    for (int ring = 0; ring < 2; ring++) {
+      const int rb = ring ? nP : 0, nR = ring ? nQ : nP, base = ring ? XP : 0;
+      const int XR = ring ? XQ : XP;
+      grp_scan_range<W, int>(
+          nR, [&](int j) { return dmult(N, rb + j); },
+          [&](int j, int r, int mu) {
+            const int u = rb + j, ri = r + mu;
+            const pt ou = P_orig(N, u);
+            for (int side = 0; side < 2; side++) {
+              const int dd = base + (side ? (ri == XR ? 0 : ri) : (r == 0 ? XR - 1 : r - 1));
+              const int of = off[dd], cd = dn[dd];
+              if (of < 0 || dep_enex(N, cd) != (side ? EXIT : ENTRY)) continue;
+              const int ud = cd >> 1;
+              int dist = side ? ud - u : u - ud;
+              if (dist <= 0) dist += nR;
+              const int w = of + dist - 1;
+              if (w >= 0 && w < O.n) {
+                st(xy, w, ou);
+                cw[w] = ciD[dd];
+                omap[w] = u;
+              }
+            }
+          });
+    }
+    grp_sync<W>();
+    return SEG_DONE;
+  }
   constexpr int kRS = W == 64 ? kRingStageWave : kRingStage;
   for (int ring = 0; ring < 2; ring++) {
     const int rb = ring ? nP : 0, nR = ring ? nQ : nP, base = ring ? XP : 0;
     const int XR = ring ? XQ : XP;
-    carry = 0;
-    for (int b0 = 0; b0 < nR; b0 += kRS * W) {
+    int carry = 0;
+    for (int b0 = 0; b0 < nR; b0 += kRS * grp_size<W>()) {
       int mu[kRS], r[kRS];
 #pragma unroll
       for (int t = 0; t < kRS; t++) {
-        const int j = b0 + t * W + lane;
+        const int j = b0 + t * grp_size<W>() + lane;
         mu[t] = j < nR ? dmult(N, rb + j) : 0;
       }
 #pragma unroll
@@ -3179,7 +3398,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       int dd[kRS][2], of[kRS][2], cd[kRS][2];
 #pragma unroll
       for (int t = 0; t < kRS; t++) {
-        const int j = b0 + t * W + lane, ri = r[t] + mu[t];
+        const int j = b0 + t * grp_size<W>() + lane, ri = r[t] + mu[t];
         dd[t][0] = base + (r[t] == 0 ? XR - 1 : r[t] - 1);
         dd[t][1] = base + (ri == XR ? 0 : ri);
 #pragma unroll
@@ -3193,7 +3412,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       pt ou[kRS];
 #pragma unroll
       for (int t = 0; t < kRS; t++) {
-        const int j = b0 + t * W + lane;
+        const int j = b0 + t * grp_size<W>() + lane;
         if (j < nR) ou[t] = P_orig(N, rb + j);
 #pragma unroll
         for (int side = 0; side < 2; side++) {
@@ -3203,7 +3422,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       }
 #pragma unroll
       for (int t = 0; t < kRS; t++) {
-        const int u = rb + b0 + t * W + lane;
+        const int u = rb + b0 + t * grp_size<W>() + lane;
 #pragma unroll
         for (int side = 0; side < 2; side++) {
           if (!on[t][side]) continue;
@@ -3220,7 +3439,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
       }
     }
   }
-  __syncthreads();
+  grp_sync<W>();
   return SEG_DONE;
 }
 
@@ -3229,7 +3448,7 @@ __device__ int segment_trace(const Nodes &N, OutBuf &O, double *xy, int *clen, i
 template <int W>
 __device__ void copy_ring_out(double *xy, int *cw, int *omap, const Nodes &N, int r0, int n,
                               int off, int c) {
-  for (int j = threadIdx.x; j < n; j += W) {
+  for (int j = grp_rank<W>(); j < n; j += grp_size<W>()) {
     st(xy, off + j, P_orig(N, r0 + j));
     cw[off + j] = c;
     omap[off + j] = r0 + j;
@@ -3247,7 +3466,9 @@ __device__ void copy_ring_out(double *xy, int *cw, int *omap, const Nodes &N, in
 // each lane evaluates its vertex for d = 1..kCleanW, which makes it a map on
 // the states {1..kCleanW, overflow}, and a wave prefix of map compositions
 // gives every lane its state.  A run of more than kCleanW dropped vertices
-// ends in the overflow state, and the component then repeats (B) on one lane.
+// ends in the overflow state: that chunk of 64 vertices then runs on one
+// lane with the last kept vertex itself, and the map prefix resumes with the
+// first chunk that starts within kCleanW of a kept vertex.
 constexpr int kCleanW = 7;
 
 __device__ __forceinline__ unsigned nib(unsigned f, unsigned s) { return (f >> (4 * s)) & 15u; }
@@ -3281,55 +3502,71 @@ __device__ int clean_component(const double *pn, const int *omap, int *keep, int
   if (r < 0) return 0;
   if (lane == 0) keep[b + r] = 1;
   const int x0 = b + r, m = L - 1 - r;
-  int s = 1, kept = 1;
-  bool ovf = false;
+  // chunks of 64 vertices in order, the state carried between them as the
+  // position of the last kept vertex (x0 + jl); a chunk whose first vertex
+  // lies within kCleanW of it runs the map prefix, and a chunk that reaches
+  // the overflow state, or starts too far from it, runs on lane 0 with the
+  // kept vertex itself (a long run of dropped vertices costs one chunk at a
+  // time, not the component)
+  int kept = 1, jl = 0;
   for (int base = 1; base <= m; base += 64) {
-    const int j = base + lane;
-    unsigned K = 0;
-    if (j <= m) {
-      const pt cur = out_pt(pn, omap, x0 + j), nxt = out_pt(pn, omap, j == m ? x0 : x0 + j + 1);
-      for (int d = 1; d <= kCleanW && d <= j; d++)
-        if (!(fabs(triA(out_pt(pn, omap, x0 + j - d), cur, nxt)) < kGhEps)) K |= 1u << (d - 1);
-    }
-    unsigned f = 0x76543210u;  // identity past the end
-    if (j <= m) {
-      f = 0;  // state 0 (overflow) stays 0
-      for (unsigned t = 1; t <= kCleanW; t++) {
-        const unsigned to = ((K >> (t - 1)) & 1u) ? 1u : (t < kCleanW ? t + 1 : 0u);
-        f |= to << (4 * t);
-      }
-    }
-    unsigned F = f;
-    for (int o = 1; o < 64; o <<= 1) {
-      const unsigned G = __shfl_up(F, o, 64);
-      if (lane >= o) F = compose(F, G);
-    }
-    const unsigned Fex = __shfl_up(F, 1, 64);
-    const unsigned sj = lane == 0 ? (unsigned)s : nib(Fex, (unsigned)s);
-    bool kp = false;
-    if (j <= m) {
-      if (sj == 0) ovf = true;
-      else kp = ((K >> (sj - 1)) & 1u) != 0;
-      keep[x0 + j] = kp ? 1 : 0;
-    }
-    kept += __popcll(__ballot(kp));
-    s = (int)nib(__shfl(F, 63, 64), (unsigned)s);
-  }
-  if (__ballot(ovf)) {
-    if (lane == 0) {
-      pt last = out_pt(pn, omap, x0);
-      kept = 1;
-      for (int j = 1; j <= m; j++) {
+    const int j = base + lane, s = base - jl;
+    bool seq = s > kCleanW;
+    if (!seq) {
+      unsigned K = 0;
+      if (j <= m) {
         const pt cur = out_pt(pn, omap, x0 + j), nxt = out_pt(pn, omap, j == m ? x0 : x0 + j + 1);
-        const bool kp = !(fabs(triA(last, cur, nxt)) < kGhEps);
-        keep[x0 + j] = kp ? 1 : 0;
-        if (kp) {
-          last = cur;
-          kept++;
+        for (int d = 1; d <= kCleanW && d <= j; d++)
+          if (!(fabs(triA(out_pt(pn, omap, x0 + j - d), cur, nxt)) < kGhEps)) K |= 1u << (d - 1);
+      }
+      unsigned f = 0x76543210u;  // identity past the end
+      if (j <= m) {
+        f = 0;  // state 0 (overflow) stays 0
+        for (unsigned t = 1; t <= kCleanW; t++) {
+          const unsigned to = ((K >> (t - 1)) & 1u) ? 1u : (t < kCleanW ? t + 1 : 0u);
+          f |= to << (4 * t);
         }
       }
+      unsigned F = f;
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned G = __shfl_up(F, o, 64);
+        if (lane >= o) F = compose(F, G);
+      }
+      const unsigned Fex = __shfl_up(F, 1, 64);
+      const unsigned sj = lane == 0 ? (unsigned)s : nib(Fex, (unsigned)s);
+      bool kp = false, ovf = false;
+      if (j <= m) {
+        if (sj == 0) ovf = true;
+        else kp = ((K >> (sj - 1)) & 1u) != 0;
+      }
+      if (__ballot(ovf)) {
+        seq = true;
+      } else {
+        if (j <= m) keep[x0 + j] = kp ? 1 : 0;
+        const unsigned long long km = __ballot(kp);
+        kept += __popcll(km);
+        if (km) jl = base + 63 - __clzll((long long)km);
+      }
     }
-    kept = __shfl(kept, 0, 64);
+    if (seq) {
+      if (lane == 0) {
+        pt last = out_pt(pn, omap, x0 + jl);
+        const int e = base + 63 < m ? base + 63 : m;
+        for (int jj = base; jj <= e; jj++) {
+          const pt cur = out_pt(pn, omap, x0 + jj),
+                   nxt = out_pt(pn, omap, jj == m ? x0 : x0 + jj + 1);
+          const bool kp = !(fabs(triA(last, cur, nxt)) < kGhEps);
+          keep[x0 + jj] = kp ? 1 : 0;
+          if (kp) {
+            last = cur;
+            jl = jj;
+            kept++;
+          }
+        }
+      }
+      kept = __shfl(kept, 0, 64);
+      jl = __shfl(jl, 0, 64);
+    }
   }
   return kept;
 }
@@ -3365,55 +3602,217 @@ __device__ int lane_clean(const double *pn, const int *omap, int *keep, int b, i
 }
 
 // components up to this length are cleaned by one thread each, longer ones
-// by one wave each
+// by one wave each, and, in a pair labelled by the whole grid, those longer
+// than TraceArgs::gridClean by the whole grid
 constexpr int kLaneClean = 48;
+
+constexpr unsigned kMapId = 0x76543210u;  // the identity on the 8 states
+
+// the map on the states {overflow, 1..kCleanW} of a vertex whose
+// |A(vertex d back, it, next)| >= eps for the d with bit d - 1 of K set
+__device__ __forceinline__ unsigned clean_map(unsigned K) {
+  unsigned f = 0;  // state 0 (overflow) stays 0
+  for (unsigned t = 1; t <= kCleanW; t++) {
+    const unsigned to = ((K >> (t - 1)) & 1u) ? 1u : (t < kCleanW ? t + 1 : 0u);
+    f |= to << (4 * t);
+  }
+  return f;
+}
+
+// Exclusive composition, over every thread of a cooperative launch in rank
+// order, of one map per thread: returns the composition of the maps of the
+// threads before this one (the earliest applied first).  Within a wave by
+// shuffles, over the block's waves in LDS, over the blocks through the
+// partials buffer: each block composes the aggregates of the blocks before
+// it, a contiguous run per thread, then an ordered tree over the block.
+__device__ unsigned grid_compose_scan(unsigned f) {
+  const int lane = __lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  unsigned F = f;
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned G = __shfl_up(F, o, 64);
+    if (lane >= o) F = compose(F, G);
+  }
+  unsigned Fex = __shfl_up(F, 1, 64);
+  if (lane == 0) Fex = kMapId;
+  __shared__ unsigned wagg[16], wred[16];
+  __syncthreads();
+  if (lane == 63) wagg[w] = F;
+  __syncthreads();
+  unsigned before = kMapId, blk = kMapId;
+  for (int i = 0; i < nw; i++) {
+    if (i < w) before = compose(wagg[i], before);
+    blk = compose(wagg[i], blk);
+  }
+  long long *part = grid_part + (long long)grid_half * gridDim.x;
+  if (threadIdx.x == 0) part[blockIdx.x] = (long long)blk;
+  cooperative_groups::this_grid().sync();
+  const int nb = (int)blockIdx.x, per = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int i0 = (int)threadIdx.x * per, i1 = min(nb, i0 + per);
+  unsigned g = kMapId;
+  for (int i = i0; i < i1; i++) g = compose((unsigned)part[i], g);
+  // lane l of a wave then holds the runs of lanes [l, l + 2o) after step o
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned h = __shfl_down(g, o, 64);
+    if (lane + o < 64) g = compose(h, g);
+  }
+  if (lane == 0) wred[w] = g;
+  __syncthreads();
+  unsigned gb = kMapId;
+  for (int i = 0; i < nw; i++) gb = compose(wred[i], gb);
+  if (threadIdx.x == 0) grid_half ^= 1;
+  __syncthreads();
+  return compose(Fex, compose(before, gb));
+}
+
+// clean_component by every thread of a cooperative launch, for the long
+// components of a pair labelled grid-wide (L >= 3): the root search a grid-size slice at a time; in
+// the pass each thread takes a contiguous run of vertices, stores each one's
+// bits K in keep, composes their maps, and after the grid-wide exclusive
+// composition walks its run from its own starting state.  A run that
+// reaches the overflow state sends the component to clean_component on the
+// first wave.  Writes keep[b..b+L) and returns the number of vertices kept
+// (the same on every thread); the caller syncs the grid before keep is read.
+__device__ int grid_clean_component(const double *pn, const int *omap, int *keep, int b, int L) {
+  const int g = grp_rank<kGrid>(), G = grp_size<kGrid>();
+  constexpr int kNone = 0x7fffffff;
+  int r = -1;
+  const pt Pl = out_pt(pn, omap, b + L - 1);
+  for (int base = 0; base <= L - 3; base += G) {
+    const int i = base + g;
+    int cand = kNone;
+    if (i <= L - 3 &&
+        !(fabs(triA(Pl, out_pt(pn, omap, b + i), out_pt(pn, omap, b + i + 1))) < kGhEps))
+      cand = i;
+    const int mn = grp_min<kGrid>(cand);
+    if (mn != kNone) {
+      r = mn;
+      break;
+    }
+  }
+  for (int j = g; j < (r < 0 ? L : r); j += G) keep[b + j] = 0;
+  if (r < 0) return 0;
+  if (g == 0) keep[b + r] = 1;
+  const int x0 = b + r, m = L - 1 - r;
+  const int per = (m + G - 1) / G;
+  const int j0 = 1 + (int)min((long long)g * per, (long long)m), j1 = min(j0 + per, m + 1);
+  unsigned agg = kMapId;
+  for (int j = j0; j < j1; j++) {
+    const pt cur = out_pt(pn, omap, x0 + j), nxt = out_pt(pn, omap, j == m ? x0 : x0 + j + 1);
+    unsigned K = 0;
+    for (int d = 1; d <= kCleanW && d <= j; d++)
+      if (!(fabs(triA(out_pt(pn, omap, x0 + j - d), cur, nxt)) < kGhEps)) K |= 1u << (d - 1);
+    keep[x0 + j] = (int)K;
+    agg = compose(clean_map(K), agg);
+  }
+  unsigned s = nib(grid_compose_scan(agg), 1);
+  long long kept = 0;
+  bool ovf = false;
+  for (int j = j0; j < j1; j++) {
+    if (s == 0) {
+      ovf = true;
+      break;
+    }
+    const bool kp = (((unsigned)keep[x0 + j] >> (s - 1)) & 1u) != 0;
+    keep[x0 + j] = kp ? 1 : 0;
+    kept += kp ? 1 : 0;
+    s = kp ? 1u : (s < kCleanW ? s + 1 : 0u);
+  }
+  // the kept count and, above bit 40, the threads that overflowed
+  long long total;
+  grid_combine(block_sum_ll(kept + (ovf ? 1LL << 40 : 0LL)), 0, total);
+  if ((total >> 40) == 0) return 1 + (int)total;
+  int k2 = 0;
+  if (g < 64) {
+    k2 = clean_component(pn, omap, keep, b, L);
+    if (g != 0) k2 = 0;
+  }
+  return grp_sum<kGrid>(k2);
+}
 
 // clean-up of every component of O (component c at [O.compRoot[c], +clen[c])
 // in list order, cw: position -> component), then compaction of xy and clen
 // over the kept vertices and non-empty components; the normalised point at
 // output position w is node omap[w] of pn.  Scratch: 3 O.ncomp + O.n ints
-// at sc.  Returns true when some vertex was dropped; O.n / O.ncomp are
+// at sc (4 O.ncomp + O.n for the whole grid).  Returns true when some vertex was dropped; O.n / O.ncomp are
 // updated.
 template <int W>
 __device__ bool grp_cleanup_compact(OutBuf &O, double *xy, int *clen, const int *cw, int *sc,
-                                    const double *pn, const int *omap) {
-  const int lane = threadIdx.x;
+                                    const double *pn, const int *omap, int gridClean) {
+  const int lane = grp_rank<W>();
   const int T = O.n, NC = O.ncomp;
   int *dirty = sc, *dl = sc + NC, *nlen = sc + 2 * NC, *keep = sc + 3 * NC;
-  for (int c = lane; c < NC; c += W) dirty[c] = 0;
-  __syncthreads();
-  for (int w = lane; w < T; w += W) {
+  for (int c = lane; c < NC; c += grp_size<W>()) dirty[c] = 0;
+  grp_sync<W>();
+  for (int w = lane; w < T; w += grp_size<W>()) {
     const int c = min(max(cw[w], 0), NC - 1), b = O.compRoot[c], L = clen[c];
     const int pw = w == b ? b + L - 1 : w - 1, nw = w == b + L - 1 ? b : w + 1;
     if (fabs(triA(out_pt(pn, omap, pw), out_pt(pn, omap, w), out_pt(pn, omap, nw))) < kGhEps)
       dirty[c] = 1;
   }
-  __syncthreads();
-  int nd = 0;
-  for (int base = 0; base < NC; base += W) {
-    const int c = base + lane;
-    const bool f = c < NC && dirty[c] != 0;
-    int tot;
-    const int pre = grp_scan<W>(f ? 1 : 0, tot);
-    if (f) dl[nd + pre] = c;
-    nd += tot;
-  }
+  grp_sync<W>();
+  const int nd = grp_scan_range<W, int>(
+      NC, [&](int c) { return dirty[c] != 0 ? 1 : 0; },
+      [&](int c, int pre, int f) {
+        if (f) dl[pre] = c;
+      });
   if (nd == 0) return false;
-  __syncthreads();
-  for (int i = lane; i < nd; i += W) {
+  grp_sync<W>();
+  for (int i = lane; i < nd; i += grp_size<W>()) {
     const int c = dl[i];
     if (clen[c] <= kLaneClean) nlen[c] = lane_clean(pn, omap, keep, O.compRoot[c], clen[c]);
   }
-  for (int i = lane >> 6; i < nd; i += W / 64) {
+  for (int i = lane >> 6; i < nd; i += grp_size<W>() / 64) {
     const int c = dl[i];
-    if (clen[c] > kLaneClean) {
+    if (clen[c] > kLaneClean && (W != kGrid || clen[c] <= gridClean)) {
       const int kept = clean_component(pn, omap, keep, O.compRoot[c], clen[c]);
       if (__lane_id() == 0) nlen[c] = kept;
     }
   }
-  __syncthreads();
+  if constexpr (W == kGrid) {
+    // the components longer than gridClean (> kLaneClean), listed in order at
+    // gl (at most NC ints past keep), one after another by the whole grid
+    int *gl = keep + T;
+    const int ng = grp_scan_range<W, int>(
+        nd, [&](int i) { return clen[dl[i]] > gridClean ? 1 : 0; },
+        [&](int i, int pre, int f) {
+          if (f) gl[pre] = dl[i];
+        });
+    grp_sync<W>();
+    for (int i = 0; i < ng; i++) {
+      const int c = gl[i];
+      const int kept = grid_clean_component(pn, omap, keep, O.compRoot[c], clen[c]);
+      if (lane == 0) nlen[c] = kept;
+    }
+  }
+  grp_sync<W>();
+  if constexpr (W == kGrid) {
+    // compaction out of place, since a block's stores may reach tiles other
+    // blocks have yet to read: the kept vertices to O.po (unused by the
+    // segment path) and the lengths to dl, then back
+    double *xo = O.po;
+    const int vout = grp_scan_range<W, int>(
+        T,
+        [&](int w) {
+          const int c = min(max(cw[w], 0), NC - 1);
+          return dirty[c] ? (keep[w] != 0 ? 1 : 0) : 1;
+        },
+        [&](int w, int pre, int kp) {
+          if (kp) st(xo, pre, ld(xy, w));
+        });
+    const int cout = grp_scan_range<W, int>(
+        NC, [&](int c) { return (dirty[c] ? nlen[c] : clen[c]) > 0 ? 1 : 0; },
+        [&](int c, int pre, int f) {
+          if (f) dl[pre] = dirty[c] ? nlen[c] : clen[c];
+        });
+    grp_sync<W>();
+    for (int i = lane; i < vout; i += grp_size<W>()) st(xy, i, ld(xo, i));
+    for (int i = lane; i < cout; i += grp_size<W>()) clen[i] = dl[i];
+    O.n = vout;
+    O.ncomp = cout;
+    return true;
+  }
   int vout = 0;
-  for (int base = 0; base < T; base += W) {
+  for (int base = 0; base < T; base += grp_size<W>()) {
     const int w = base + lane;
     bool kp = false;
     pt v;
@@ -3428,7 +3827,7 @@ __device__ bool grp_cleanup_compact(OutBuf &O, double *xy, int *clen, const int 
     vout += tot;
   }
   int cout = 0;
-  for (int base = 0; base < NC; base += W) {
+  for (int base = 0; base < NC; base += grp_size<W>()) {
     const int c = base + lane;
     int len = 0;
     if (c < NC) len = dirty[c] ? nlen[c] : clen[c];
@@ -3460,8 +3859,8 @@ enum {
 template <int W>
 __device__ void spill_nodes(Nodes &N, const Nodes &G, int nP, int nQ) {
   const int n = nP + nQ;
-  __syncthreads();
-  for (int j = threadIdx.x; j < n; j += W) {
+  grp_sync<W>();
+  for (int j = grp_rank<W>(); j < n; j += grp_size<W>()) {
     if (N.pn != G.pn) st(G.pn, j, ld(N.pn, j));
     st(G.po, j, P_orig(N, j));
     G.next[j] = j == nP - 1 ? 0 : (j == n - 1 ? nP : j + 1);
@@ -3471,7 +3870,7 @@ __device__ void spill_nodes(Nodes &N, const Nodes &G, int nP, int nQ) {
     if (N.lab != G.lab) G.lab[j] = N.lab[j];
     if (N.enex != G.enex) G.enex[j] = N.enex[j];
   }
-  __syncthreads();
+  grp_sync<W>();
   const int nn = N.n, cap = N.cap;
   N = G;
   N.n = nn;
@@ -3482,7 +3881,7 @@ __device__ void spill_nodes(Nodes &N, const Nodes &G, int nP, int nQ) {
 template <int W>
 __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceArgs &a,
                           int *aux, char *arena, int arenaBytes) {
-  const int lane = threadIdx.x;
+  const int lane = grp_rank<W>();
   if (status[k] != GH_OK) {
     if (lane == 0) {
       a.outNComp[k] = 0;
@@ -3535,13 +3934,13 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
   // that it waits on memory once per kS * W nodes.
   const bool xl = a.explicitLinks != 0;
   constexpr int kS = W == 64 ? kStageWave : kStage;
-  for (int j0 = lane; j0 < nP; j0 += kS * W) {
+  for (int j0 = lane; j0 < nP; j0 += kS * grp_size<W>()) {
     pt ip[kS];
     int nbv[kS];
     bool lk[kS];
 #pragma unroll
     for (int u = 0; u < kS; u++) {
-      const int j = j0 + u * W;
+      const int j = j0 + u * grp_size<W>();
       if (j < nP) {
         const long long sl = sbP + j;
         ip[u] = ld(s.iP, sl);
@@ -3551,7 +3950,7 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
     }
 #pragma unroll
     for (int u = 0; u < kS; u++) {
-      const int j = j0 + u * W;
+      const int j = j0 + u * grp_size<W>();
       if (j < nP) {
         st(N.pn, j, ip[u]);
         if (xl) {
@@ -3566,13 +3965,13 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
       }
     }
   }
-  for (int j0 = lane; j0 < nQ; j0 += kS * W) {
+  for (int j0 = lane; j0 < nQ; j0 += kS * grp_size<W>()) {
     pt ip[kS];
     int nbv[kS];
     bool lk[kS];
 #pragma unroll
     for (int u = 0; u < kS; u++) {
-      const int j = j0 + u * W;
+      const int j = j0 + u * grp_size<W>();
       if (j < nQ) {
         const long long sl = sbQ + j;
         ip[u] = ld(s.iQ, sl);
@@ -3582,7 +3981,7 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
     }
 #pragma unroll
     for (int u = 0; u < kS; u++) {
-      const int j = j0 + u * W;
+      const int j = j0 + u * grp_size<W>();
       if (j < nQ) {
         const int v = nP + j;
         st(N.pn, v, ip[u]);
@@ -3595,13 +3994,13 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
       }
     }
   }
-  __syncthreads();
+  grp_sync<W>();
   // parents: the edges' first slots (one edge each)
-  for (long long e0 = pE0 + lane; e0 < pE1; e0 += kS * W) {
+  for (long long e0 = pE0 + lane; e0 < pE1; e0 += kS * grp_size<W>()) {
     long long sl[kS];
 #pragma unroll
     for (int u = 0; u < kS; u++) {
-      const long long e = e0 + u * W;
+      const long long e = e0 + u * grp_size<W>();
       sl[u] = e < pE1 ? a.psP2[e] - sbP : -1;
     }
 #pragma unroll
@@ -3609,11 +4008,11 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
       if (sl[u] >= 0)
         N.fl[sl[u]] = (unsigned char)(F_SRC | ((N.fl[sl[u]] & F_LINK) ? F_INT : 0));
   }
-  for (long long e0 = qE0 + lane; e0 < qE1; e0 += kS * W) {
+  for (long long e0 = qE0 + lane; e0 < qE1; e0 += kS * grp_size<W>()) {
     long long sl[kS];
 #pragma unroll
     for (int u = 0; u < kS; u++) {
-      const long long e = e0 + u * W;
+      const long long e = e0 + u * grp_size<W>();
       sl[u] = e < qE1 ? nP + (a.psQ2[e] - sbQ) : -1;
     }
 #pragma unroll
@@ -3621,32 +4020,32 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
       if (sl[u] >= 0)
         N.fl[sl[u]] = (unsigned char)(F_SRC | ((N.fl[sl[u]] & F_LINK) ? F_INT : 0));
   }
-  __syncthreads();
+  grp_sync<W>();
   if (!xl) {
     // a P node linked from several Q slots keeps the last one (link() order)
-    for (int j = lane; j < nQ; j += W)
+    for (int j = lane; j < nQ; j += grp_size<W>())
       if (N.nb[nP + j] >= 0) atomicMax(&N.nb[N.nb[nP + j]], nP + j);
-    __syncthreads();
-    for (int j = lane; j < nP; j += W)
+    grp_sync<W>();
+    for (int j = lane; j < nP; j += grp_size<W>())
       if (N.nb[j] >= 0) N.fl[j] |= F_INT;
-    __syncthreads();
+    grp_sync<W>();
   }
   // -------- step 1 (restored) + step 3, one lane per P intersection vertex
   // The labels are computed for kStage vertices per lane from loads only
   // and stored afterwards.  The P ring's mutual-link check rides along: a
   // P intersection vertex I must have nb[nb[I]] == I.
   bool slow = false;
-  for (int I0 = lane; I0 < nP; I0 += kStage * W) {
+  for (int I0 = lane; I0 < nP; I0 += kStage * grp_size<W>()) {
     int q[kStage], l[kStage];
 #pragma unroll
     for (int u = 0; u < kStage; u++) {
-      const int I = I0 + u * W;
+      const int I = I0 + u * grp_size<W>();
       q[u] = I < nP && isint(N, I) ? N.nb[I] : -2;
       l[u] = L_NONE;
     }
 #pragma unroll
     for (int u = 0; u < kStage; u++) {
-      const int I = I0 + u * W;
+      const int I = I0 + u * grp_size<W>();
       if (q[u] == -2) continue;
       if (q[u] < 0) {
         slow = true;
@@ -3670,15 +4069,15 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
 #pragma unroll
     for (int u = 0; u < kStage; u++) {
       if (q[u] < 0) continue;
-      N.lab[I0 + u * W] = (signed char)l[u];
+      N.lab[I0 + u * grp_size<W>()] = (signed char)l[u];
       N.lab[q[u]] = (signed char)l[u];
     }
   }
-  __syncthreads();
+  grp_sync<W>();
   // links must be mutual on the Q ring too (several P sources may point at
   // one Q vertex when links are explicit)
 #pragma unroll 4
-  for (int j = lane; j < nQ; j += W) {
+  for (int j = lane; j < nQ; j += grp_size<W>()) {
     const int v = nP + j;
     if (isint(N, v)) {
       const int q = N.nb[v];
@@ -3700,7 +4099,7 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
   // crossings on P (the links are mutual here, so Q has as many), and the
   // intersection vertices of both rings, each of which departs at most twice
   int nCrossP = 0, nInt = 0;
-  for (int j = lane; j < n; j += W) {
+  for (int j = lane; j < n; j += grp_size<W>()) {
     nInt += isint(N, j) ? 1 : 0;
     if (j < nP) nCrossP += iscross(N, j) ? 1 : 0;
   }
@@ -3753,7 +4152,7 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
       grp_enex<W>(N, 0, nP, VP, pinP ? EXIT : ENTRY);
       const bool pinQ = grp_pip<W>(N, 0, nP, P_(N, VQ));
       grp_enex<W>(N, nP, nQ, VQ, pinQ ? EXIT : ENTRY);
-      __syncthreads();
+      grp_sync<W>();
       const int rc = segment_trace<W>(N, O, xy, clen, rg, nP, nQ, Dc, omap);
       if (rc == SEG_OVERFLOW) {
         if (lane == 0) {
@@ -3776,16 +4175,16 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
     spill_nodes<W>(N, G, nP, nQ);
     O.pn = a.opn + 2 * nb0;
     int *owner = rg;  // nQ entries of the segment scratch
-    for (int j = lane; j < nQ; j += W) owner[j] = -1;
-    __syncthreads();
-    for (int I = lane; I < nP; I += W) {
+    for (int j = lane; j < nQ; j += grp_size<W>()) owner[j] = -1;
+    grp_sync<W>();
+    for (int I = lane; I < nP; I += grp_size<W>()) {
       const int q = isint(N, I) ? N.nb[I] : -1;
       if (q >= nP && q < nP + nQ) atomicMax(&owner[q - nP], I);
     }
-    __syncthreads();
-    for (int j = lane; j < nQ; j += W)
+    grp_sync<W>();
+    for (int j = lane; j < nQ; j += grp_size<W>())
       if (owner[j] >= 0) N.lab[nP + j] = N.lab[owner[j]];
-    __syncthreads();
+    grp_sync<W>();
     // step 4's point-in-polygon tests of each ring's first non-intersection
     // vertex (the rings are still in index order; the flags do not change
     // before step 4), in parallel
@@ -3827,8 +4226,8 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
     a.outNVerts[k] = nv;
     return LP_SEQUENTIAL;
   }
-  __syncthreads();
-  if (a.cleanup && grp_cleanup_compact<W>(O, xy, clen, rg, rg + even_up(n), N.pn, omap) &&
+  grp_sync<W>();
+  if (a.cleanup && grp_cleanup_compact<W>(O, xy, clen, rg, rg + even_up(n), N.pn, omap, a.gridClean) &&
       path == LP_SEGMENTS)
     path = LP_SEG_CLEANUP;
   if (lane == 0) {
@@ -3845,6 +4244,9 @@ __device__ int label_pair(long k, int *status, const SlotArrays &s, const TraceA
 // parallelism of the block is worth more (the 1e6 x 1e5 pair labels in 113
 // ms at 1024 threads, 167 ms at 512).
 constexpr int kBigGroup = 1024;
+// Pairs with at least Tuning::gridNodes nodes are labelled by the whole GPU,
+// one cooperative launch of kGridBlock-thread blocks each (k_label_grid).
+constexpr int kGridBlock = 256;
 // batches of at least this many pairs are labelled largest first
 constexpr long kLptMinPairs = 1024;
 
@@ -3859,18 +4261,21 @@ __device__ __forceinline__ long long pair_nodes(const TraceArgs &a, long k) {
 
 // Largest pairs first (longest-processing-time order, by power-of-two size
 // class), values the pair indices, for one stable 8-bit radix pass: key
-// 64 * (pair below bigNodes) + 63 - floor(log2 nodes).  The block path's
-// pairs (nodes >= bigNodes) are then exactly the first nBig entries, for
-// any threshold.  Blocks are dispatched in index order, so the pairs that
-// take longest start first instead of forming the launch's tail.
+// 128 * (pair at or above gridNodes) + 64 * (pair below bigNodes) + 63 -
+// floor(log2 nodes).  The block path's pairs (bigNodes <= nodes <
+// gridNodes) are then exactly the first nBig entries, for any thresholds,
+// and the grid path's come last (the waves skip them).  Blocks are
+// dispatched in index order, so the pairs that take longest start first
+// instead of forming the launch's tail.
 __global__ void k_label_order_keys(long M, const long long *offP, const long long *offQ,
                                    const long long *psP2, const long long *psQ2,
-                                   long long bigNodes, uint64_t *keys, int *vals) {
+                                   long long bigNodes, long long gridNodes, uint64_t *keys,
+                                   int *vals) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= M) return;
   const long long n = pair_nodes(offP, offQ, psP2, psQ2, k);
   const int cls = n > 0 ? 63 - __clzll((unsigned long long)n) : 0;
-  keys[k] = (uint64_t)((n >= bigNodes ? 0 : 64) + 63 - cls);
+  keys[k] = (uint64_t)((n >= gridNodes ? 128 : 0) + (n >= bigNodes ? 0 : 64) + 63 - cls);
   vals[k] = (int)k;
 }
 
@@ -3893,17 +4298,43 @@ __global__ void __launch_bounds__(64)
   }
 }
 
-// the pairs k_label_block takes (by size alone, whatever their status, so
-// that each pair is k_label_wave's or k_label_block's), appended to list;
-// cnt[0] their number
+// the pairs k_label_block takes (bigNodes <= nodes < gridNodes) appended to
+// list, cnt[0] their number, and those k_label_grid takes (nodes >=
+// gridNodes) to glist, cnt[1] their number; by size alone, whatever their
+// status, so that each pair is labelled by exactly one of the three kernels
 __global__ void __launch_bounds__(1024)
     k_big_pairs(long M, const long long *offP, const long long *offQ, const long long *psP2,
-                const long long *psQ2, long long bigNodes, int *list, unsigned long long *cnt) {
+                const long long *psQ2, long long bigNodes, long long gridNodes, int *list,
+                int *glist, unsigned long long *cnt) {
   __shared__ unsigned long long lds[17];
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool big = k < M && pair_nodes(offP, offQ, psP2, psQ2, k) >= bigNodes;
+  const long long n = k < M ? pair_nodes(offP, offQ, psP2, psQ2, k) : 0;
+  const bool big = k < M && n >= bigNodes && n < gridNodes;
   const long i = block_append(cnt, big, lds);
   if (i >= 0) list[i] = (int)k;
+  __syncthreads();
+  const bool huge = k < M && n >= gridNodes;
+  const long j = block_append(cnt + 1, huge, lds);
+  if (j >= 0) glist[j] = (int)k;
+}
+
+// One pair labelled by every thread of a cooperative launch (the pairs of
+// Tuning::gridNodes nodes or more, one launch each); part: the grid
+// reductions' partials, 2 * gridDim.x entries
+__global__ void __launch_bounds__(kGridBlock)
+    k_label_grid(long k, int *status, SlotArrays s, TraceArgs a, int *aux, long long *part,
+                 int *lpath, long long *lticks) {
+  if (threadIdx.x == 0) {
+    grid_part = part;
+    grid_half = 0;
+  }
+  __syncthreads();
+  const long long t0 = wall_clock64();
+  const int path = label_pair<kGrid>(k, status, s, a, aux, nullptr, 0);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && lpath) {
+    lpath[k] = path;
+    lticks[k] = wall_clock64() - t0;
+  }
 }
 
 // One block of kBigGroup threads per listed pair, the blocks taking the list
@@ -4153,7 +4584,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   int *listQ = wsT<int>(c, "gh_runListQ", edgesQ), *lenQ = wsT<int>(c, "gh_runLenQ", edgesQ);
   int *actP = wsT<int>(c, "gh_actP", edgesP), *actQ = wsT<int>(c, "gh_actQ", edgesQ);
   int *bigList = wsT<int>(c, "gh_bigList", M);
-  unsigned long long *bigCnt = wsT<unsigned long long>(c, "gh_bigCnt", 1);
+  unsigned long long *bigCnt = wsT<unsigned long long>(c, "gh_bigCnt", 2);
+  int *gridList = wsT<int>(c, "gh_gridList", M);
   int *lstAllP = nullptr, *lstAllQ = nullptr;
   if (cpuRoute) {
     // -- the P view counts, lists and partner lists: pairs whose Q ring
@@ -4276,14 +4708,15 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          runCnt + 8);
     HIP_CHECK(hipGetLastError());
     const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
-    fill_i32(c, (int *)bigCnt, 0, 2);
+    fill_i32(c, (int *)bigCnt, 0, 4);
     hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M,
                        (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
-                       (const long long *)psQ2, (long long)c.tune.bigNodes, bigList, bigCnt);
+                       (const long long *)psQ2, (long long)c.tune.bigNodes,
+                       (long long)c.tune.gridNodes, bigList, gridList, bigCnt);
     HIP_CHECK(hipGetLastError());
-    long long rq[7];
+    long long rq[8];
     read_scalars(c, {psQ2 + edgesQ, rc + 8, rc + 10, rc + 11, nActQd, rc + 9,
-                     (const long long *)bigCnt},
+                     (const long long *)bigCnt, (const long long *)bigCnt + 1},
                  rq);
     rb[1] = rq[0];
     rb[5] = rq[1];
@@ -4292,6 +4725,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     rb[9] = rq[4];
     rb[13] = rq[5];
     rb[16] = rq[6];
+    rb[17] = rq[7];
   } else {
     if (edgesP + edgesQ > 0) {
       const bool cull = boxP.b1 != nullptr || boxQ.b1 != nullptr;
@@ -4334,16 +4768,17 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     const long long *nActPd = compact_flags_dev(c, cnt1P, edgesP, actP, "actP");
     const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
     // the pairs labelled by a block (their number comes back with the totals)
-    fill_i32(c, (int *)bigCnt, 0, 2);
+    fill_i32(c, (int *)bigCnt, 0, 4);
     hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M,
                        (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
-                       (const long long *)psQ2, (long long)c.tune.bigNodes, bigList, bigCnt);
+                       (const long long *)psQ2, (long long)c.tune.bigNodes,
+                       (long long)c.tune.gridNodes, bigList, gridList, bigCnt);
     HIP_CHECK(hipGetLastError());
     {
       const long long *rc = (const long long *)runCnt;
       read_scalars(c, {psP2 + edgesP, psQ2 + edgesQ, rc, rc + 2, rc + 3, rc + 8, rc + 10, rc + 11,
                        nActPd, nActQd, psP1 + edgesP, psQ1 + edgesQ, rc + 1, rc + 9, rc + 4,
-                       rc + 12, (const long long *)bigCnt},
+                       rc + 12, (const long long *)bigCnt, (const long long *)bigCnt + 1},
                    rb);
     }
     const long nActP = (long)rb[8], nActQ = (long)rb[9];
@@ -4383,7 +4818,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          (const long long *)psQ1, lstAllQ, boxP);
     HIP_CHECK(hipGetLastError());
   }
-  const long nBig = (long)rb[16];
+  const long nBig = (long)rb[16], nGrid = (long)rb[17];
   const long nActP = (long)rb[8], nActQ = (long)rb[9];
   const long nShortP = (long)rb[12], nShortQ = (long)rb[13];
   const long slotsP = (long)rb[0], slotsQ = (long)rb[1];
@@ -4698,6 +5133,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   ta.cleanup = opt.cleanup;
   ta.explicitLinks = cpuRoute ? 1 : 0;
   ta.bigNodes = c.tune.bigNodes;
+  ta.gridClean = c.tune.gridClean;
   if (opt.label_mode == 1) {
     LaunchTimer t(c, "gh_label_trace");
     hipLaunchKernelGGL(k_label_trace, dim3(grid_for(M, 64)), dim3(64), 0, c.stream, M,
@@ -4725,7 +5161,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       hipLaunchKernelGGL(k_label_order_keys, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
                          (const long long *)offP, (const long long *)offQ,
                          (const long long *)psP2, (const long long *)psQ2,
-                         (long long)c.tune.bigNodes, ok, ov);
+                         (long long)c.tune.bigNodes, (long long)c.tune.gridNodes, ok, ov);
       HIP_CHECK(hipGetLastError());
       radix_sort_u64(c, ok, ov, ok2, order, M, 8);
     }
@@ -4747,6 +5183,27 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                        res.labelTicks, c.tune.labelLds);
     HIP_CHECK(hipGetLastError());
     if (nBig > 0) HIP_CHECK(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+    if (nGrid > 0) {
+      // the pairs of gridNodes nodes or more, one cooperative launch each over
+      // as many blocks as the GPU holds at once, after the other labelling
+      std::vector<int> gl((size_t)nGrid);
+      HIP_CHECK(hipMemcpyAsync(gl.data(), gridList, sizeof(int) * (size_t)nGrid,
+                               hipMemcpyDeviceToHost, c.stream));
+      HIP_CHECK(hipStreamSynchronize(c.stream));
+      int perCu = 0;
+      HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_label_grid, kGridBlock,
+                                                             0));
+      const int blocks = (perCu > 0 ? perCu : 1) * c.cu_count;
+      long long *part = wsT<long long>(c, "gh_grid_part", 2L * blocks);
+      for (int kp : gl) {
+        long kk = kp;
+        int *st = pm.status, *lp = res.labelPath;
+        long long *lt = res.labelTicks;
+        void *args[] = {&kk, &st, &s, &ta, &aux, &part, &lp, &lt};
+        HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_label_grid, dim3((unsigned)blocks),
+                                             dim3(kGridBlock), args, 0, c.stream));
+      }
+    }
   }
   res.vertOff = wsT<long long>(c, "gh_vOff", M + 1);
   res.compOff = wsT<long long>(c, "gh_cOff", M + 1);

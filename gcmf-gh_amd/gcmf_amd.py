@@ -194,7 +194,8 @@ class ClipStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("cull_min", C.c_long), ("big_nodes", C.c_longlong), ("count_block", C.c_int),
-                ("lds_max_q", C.c_int), ("label_lds", C.c_int)]
+                ("lds_max_q", C.c_int), ("label_lds", C.c_int), ("grid_nodes", C.c_longlong),
+                ("grid_clean", C.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -424,13 +425,19 @@ class Context:
         self._check(hip_lib().gcmf_get_tuning(self.h, C.byref(t)), "get_tuning")
         return t.as_dict()
 
-    def set_tuning(self, cull_min, big_nodes, count_block, lds_max_q=None, label_lds=None):
+    def set_tuning(self, cull_min, big_nodes, count_block, lds_max_q=None, label_lds=None,
+                   grid_nodes=None, grid_clean=None):
         cur = self.tuning()
         if lds_max_q is None:
             lds_max_q = cur["lds_max_q"]
         if label_lds is None:
             label_lds = cur["label_lds"]
-        t = Tuning(cull_min, big_nodes, count_block, lds_max_q, label_lds)
+        if grid_nodes is None:
+            grid_nodes = cur["grid_nodes"]
+        if grid_clean is None:
+            grid_clean = cur["grid_clean"]
+        t = Tuning(cull_min, big_nodes, count_block, lds_max_q, label_lds, grid_nodes,
+                   grid_clean)
         self._check(hip_lib().gcmf_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def tune_clip(self, base, over, d_pairs, M, flags=CLIP_NORMALIZE):

@@ -220,6 +220,11 @@ typedef struct {
   int label_lds;       /* bytes of LDS each labelling wave keeps a pair's node
                           arrays and scratch in; what does not fit stays in
                           HBM (0..65536, a multiple of 16; 20480) */
+  long long grid_nodes; /* pairs with this many nodes are labelled by the whole
+                          GPU, one cooperative launch each (2^20) */
+  int grid_clean;       /* in such a pair, result components longer than this
+                          are cleaned up by the whole GPU, the others by one
+                          wave or thread each (49.., 16384) */
 } gcmf_tuning;
 int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out);
 int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t);

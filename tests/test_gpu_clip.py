@@ -562,3 +562,77 @@ def test_label_lds_arena_keeps_results(gctx, label_lds):
                 assert G.LP_SEG_CLEANUP in paths, paths
     finally:
         gctx.set_tuning(**default)
+
+
+def test_clip_grid_labelling_large_pair(gctx):
+    """The 200000 x 20000 pair labelled by the whole GPU (one cooperative
+    launch, grid_nodes below its node count) gives the committed oracle
+    result bit for bit, as the 1024-thread block does."""
+    import hashlib
+    import json
+    with open(GOLDEN + "/large_pair_200k.json") as f:
+        ref = json.load(f)
+    P, Q = G.gen_single_pair(1, 0, 200000, 20000)
+    default = gctx.tuning()
+    try:
+        gctx.set_tuning(**dict(default, grid_nodes=1 << 17))
+        st = run_batch(gctx, [("large", P, Q)], G.CLIP_NORMALIZE)
+        status, res = gctx.clip_results(st)
+        assert gctx.clip_get("label_path", 1)[0] in (G.LP_SEGMENTS, G.LP_SEG_CLEANUP)
+    finally:
+        gctx.set_tuning(**default)
+    assert status[0] == ref["status"] == 0
+    assert [len(c) for c in res[0]] == ref["comp_sizes"]
+    xy = np.ascontiguousarray(np.concatenate(res[0]), np.float64)
+    assert hashlib.sha256(xy.tobytes()).hexdigest() == ref["xy_sha256"]
+
+
+def dense_square_cases():
+    """A square with 1500 vertices on each side against a diamond and a
+    7-point star crossing it: the result components carry runs of hundreds
+    of collinear vertices that the clean-up drops (runs far beyond the
+    map states of the parallel clean-up, so its overflow path is taken)."""
+    t = np.linspace(0.0, 1.0, 1500, endpoint=False)
+    z, o = np.zeros_like(t), np.ones_like(t)
+    sq = np.concatenate([np.stack([t, z], 1), np.stack([o, t], 1), np.stack([1 - t, o], 1),
+                         np.stack([z, 1 - t], 1)])
+    a = np.arange(4) * np.pi / 2 + 0.1
+    rot = np.array([0.5, 0.5]) + 0.62 * np.stack([np.cos(a), np.sin(a)], 1)
+    b = np.pi / 2 + np.arange(14) * np.pi / 7
+    r = np.where(np.arange(14) % 2 == 0, 0.8, 0.45)
+    star = np.array([0.5, 0.45]) + np.stack([r * np.cos(b), r * np.sin(b)], 1)
+    return [("dense_rot", sq, rot), ("dense_star", sq, star)]
+
+
+def test_clip_grid_labelling_batch(gctx):
+    """A batch split between the wave, block and grid paths (grid_nodes low
+    enough that several pairs take cooperative launches one after another,
+    among them the batched workload's rare pairs, a trace that does not
+    close included): every pair equals the oracle of the route taken and the
+    default shapes' results, status included.  Run twice: with the grid
+    clean-up for components above 16384 vertices only, and for every
+    component above 48 (grid_clean 49), where the dense squares' collinear
+    runs send it to its overflow path."""
+    z = np.load(GOLDEN + "/batched_pairs.npz")
+    cases = [(n, z[n + "_P"], z[n + "_Q"]) for n in ["p4596", "p156581", "p73779", "p176719"]]
+    cases += random_cases(30, seed0=61, nmax=300) + snapped_cases(300, n=3)
+    cases += [("wavy%d" % s,) + tuple(G.gen_single_pair(1, 70 + s, 6000, 1500)) for s in range(3)]
+    cases += dense_square_cases()
+    st0 = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+    s0, r0 = gctx.clip_results(st0)
+    assert G.LP_SEG_CLEANUP in set(gctx.clip_get("label_path", len(cases)).tolist())
+    default = gctx.tuning()
+    for grid_clean in (default["grid_clean"], 49):
+        try:
+            gctx.set_tuning(**dict(default, big_nodes=2000, grid_nodes=6000,
+                                   grid_clean=grid_clean))
+            st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+            s1, r1 = gctx.clip_results(st)
+        finally:
+            gctx.set_tuning(**default)
+        np.testing.assert_array_equal(s1, s0)
+        for a, b in zip(r1, r0):
+            assert [len(c) for c in a] == [len(c) for c in b]
+            for x, y in zip(a, b):
+                np.testing.assert_array_equal(x, y)
+        check_results(gctx, st, cases, NORM)
