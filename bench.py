@@ -601,10 +601,11 @@ def prims(args):
         ctx.set_timing(False)
         names = ["mbr_endpoints", "sort_hist", "sort_scatter", "scan_tile", "scan_reduce",
                  "scan_downsweep", "scan_lookback", "mbr_sweep_prep", "mbr_sweep_count",
-                 "mbr_sweep_write"]
+                 "mbr_sweep_write", "mbr_strip_build", "mbr_strip_count", "mbr_strip_write"]
         ms = sum(ctx.timer(k)[0] for k in names)
         sweep_ms = sum(ctx.timer(k)[0] for k in ("mbr_sweep_prep", "mbr_sweep_count",
-                                                  "mbr_sweep_write"))
+                                                  "mbr_sweep_write", "mbr_strip_build",
+                                                  "mbr_strip_count", "mbr_strip_write"))
         K = js["candidates"]
         gb = (2 * npoly * 32 + K * 8) / (ms * 1e-3) / 1e9
         out.append({"prim": "mbr_filter", "keys": 4 * npoly, "candidates": K,

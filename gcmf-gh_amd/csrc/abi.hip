@@ -352,6 +352,7 @@ int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out) {
     out->label_lds = ctx->c.tune.labelLds;
     out->grid_nodes = ctx->c.tune.gridNodes;
     out->grid_clean = ctx->c.tune.gridClean;
+    out->strip_min = ctx->c.tune.stripMin;
   });
 }
 
@@ -360,7 +361,8 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
     if (!t || t->cull_min < 1 || t->big_nodes < 1 ||
         (t->count_block != 64 && t->count_block != 128 && t->count_block != 256) ||
         t->lds_max_q < 0 || t->lds_max_q > 1024 || t->label_lds < 0 || t->label_lds > 65536 ||
-        t->label_lds % 16 != 0 || t->grid_nodes < 1 || t->grid_clean < 49)
+        t->label_lds % 16 != 0 || t->grid_nodes < 1 || t->grid_clean < 49 ||
+        t->strip_min < 0)
       throw gcmf::Error{GCMF_ERR_ARG, "bad tuning"};
     ctx->c.tune.cullMin = t->cull_min;
     ctx->c.tune.bigNodes = t->big_nodes;
@@ -369,6 +371,7 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
     ctx->c.tune.labelLds = t->label_lds;
     ctx->c.tune.gridNodes = t->grid_nodes;
     ctx->c.tune.gridClean = t->grid_clean;
+    ctx->c.tune.stripMin = t->strip_min;
   });
 }
 
@@ -454,6 +457,7 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
       out->label_lds = best.labelLds;
       out->grid_nodes = best.gridNodes;
       out->grid_clean = best.gridClean;
+      out->strip_min = best.stripMin;
     }
   });
 }

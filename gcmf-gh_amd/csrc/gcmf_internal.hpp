@@ -60,6 +60,7 @@ struct Tuning {
   long long bigNodes = 8192;  // pairs with this many nodes are labelled by a block
   long long gridNodes = 1LL << 20;  // ... and with this many by the whole GPU
   int gridClean = 16384;    // there, components longer than this are cleaned by the grid
+  long long stripMin = 1LL << 17;  // joins of this many polygons sweep the MBRs in y strips
   int countBlock = 128;     // threads per block of the count kernel (64, 128, 256)
   int labelLds = 20480;     // LDS bytes per labelling wave for a pair's arrays (0: none)
   int ldsMaxQ = 1024;       // pairs with Q rings above cullMin and up to this size are

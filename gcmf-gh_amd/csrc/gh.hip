@@ -117,6 +117,18 @@ __device__ __forceinline__ void pgrp_minmax(double &a, double &b) {
 // threads per block and blocks for M pairs at group width W
 inline unsigned pgrp_grid(long M, int W) { return W == 64 ? grid_for(M, 4) : (unsigned)M; }
 inline int pgrp_block(int W) { return W == 64 ? 256 : W; }
+// Kernels without cross-lane work (copies, per-vertex maps) split each pair
+// of the 1024-thread width over `split` blocks, so that a few large pairs
+// still fill the GPU: block b takes pair b / split, part b % split.
+inline int pgrp_split(long M, int W) { return W == 64 ? 1 : (int)std::max<long>(1, 512 / M); }
+template <int W>
+__device__ __forceinline__ long pgrp_split_pair(int split) {
+  return W == 64 ? pgrp_pair<W>() : (long)(blockIdx.x / (unsigned)split);
+}
+template <int W>
+__device__ __forceinline__ int pgrp_split_part(int split) {
+  return W == 64 ? 0 : (int)(blockIdx.x % (unsigned)split);
+}
 
 // one group per pair: sizes, swap (|P| >= |Q|, ghcuda.cpp:64-91), ring MBRs
 // (readShapefile.h:304-334), CMBR and disjointness (ghcuda.cpp:94-125)
@@ -264,15 +276,17 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
   eQ[k] = status == GH_OK ? nQ : 0;
 }
 
-// one group per pair: edge -> pair maps and normalised rings
+// one group per pair (split over several blocks at width 1024): edge ->
+// pair maps and normalised rings
 template <int W>
 __global__ void __launch_bounds__(W == 64 ? 256 : W)
-    k_pair_fill(long M, const int *pairs, const double *bCoords, const double *oCoords,
-                PairMeta pm, const long long *offP, const long long *offQ, int *mapP, int *mapQ,
-                double *nP_, double *nQ_) {
-  const long k = pgrp_pair<W>();
+    k_pair_fill(long M, int split, const int *pairs, const double *bCoords,
+                const double *oCoords, PairMeta pm, const long long *offP, const long long *offQ,
+                int *mapP, int *mapQ, double *nP_, double *nQ_) {
+  const long k = pgrp_split_pair<W>(split);
   if (k >= M) return;
-  const int lane = pgrp_lane<W>();
+  const int lane = pgrp_lane<W>() + W * pgrp_split_part<W>(split);
+  const long long stride = (long long)W * split;
   const int sw = pm.swapped[k];
   const double cx = pm.norm[3 * k], cy = pm.norm[3 * k + 1], s = pm.norm[3 * k + 2];
   for (int side = 0; side < 2; side++) {
@@ -282,7 +296,7 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
     const long start = side == 0 ? pm.startP[k] : pm.startQ[k];
     int *map = side == 0 ? mapP : mapQ;
     double *dst = side == 0 ? nP_ : nQ_;
-    for (long long i = lane; i < n; i += W) {
+    for (long long i = lane; i < n; i += stride) {
       double x = c[2 * (start + i)], y = c[2 * (start + i) + 1];
       dst[2 * (base + i)] = (x - cx) * s;
       dst[2 * (base + i) + 1] = (y - cy) * s;
@@ -4400,20 +4414,23 @@ __global__ void k_capacity(long M, const long long *offP, const long long *offQ,
 }
 
 // final dense output: result vertices and component sizes in pair order
+// (each pair split over several blocks at width 1024)
 template <int W>
 __global__ void __launch_bounds__(W == 64 ? 256 : W)
-    k_gather_out(long M, const long long *nodeOff, const long long *compCapOff,
+    k_gather_out(long M, int split, const long long *nodeOff, const long long *compCapOff,
                  const long long *vOff, const long long *cOff, const double *xyTmp,
                  const int *compLen, double *xy, int *compSizes) {
-  const long k = pgrp_pair<W>();
+  const long k = pgrp_split_pair<W>(split);
   if (k >= M) return;
-  const int lane = pgrp_lane<W>();
+  const int lane = pgrp_lane<W>() + W * pgrp_split_part<W>(split);
+  const long long stride = (long long)W * split;
   const long long nv = vOff[k + 1] - vOff[k], nc = cOff[k + 1] - cOff[k];
-  for (long long i = lane; i < nv; i += W) {
+  for (long long i = lane; i < nv; i += stride) {
     xy[2 * (vOff[k] + i)] = xyTmp[2 * (nodeOff[k] + i)];
     xy[2 * (vOff[k] + i) + 1] = xyTmp[2 * (nodeOff[k] + i) + 1];
   }
-  for (long long i = lane; i < nc; i += W) compSizes[cOff[k] + i] = compLen[compCapOff[k] + i];
+  for (long long i = lane; i < nc; i += stride)
+    compSizes[cOff[k] + i] = compLen[compCapOff[k] + i];
 }
 
 template <typename T>
@@ -4507,8 +4524,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   double *ringQ = wsT<double>(c, "gh_ringQ", 2 * edgesQ);
   {
     LaunchTimer t(c, "gh_pair_fill");
-    hipLaunchKernelGGL(pw == 64 ? k_pair_fill<64> : k_pair_fill<1024>, dim3(pgrp_grid(M, pw)),
-                       dim3(pgrp_block(pw)), 0, c.stream, M, pairs,
+    const int split = pgrp_split(M, pw);
+    hipLaunchKernelGGL(pw == 64 ? k_pair_fill<64> : k_pair_fill<1024>,
+                       dim3(pgrp_grid(M, pw) * (unsigned)split), dim3(pgrp_block(pw)), 0,
+                       c.stream, M, split, pairs,
                        in.bCoords, in.oCoords, pm, (const long long *)offP,
                        (const long long *)offQ, mapP, mapQ, ringP, ringQ);
     HIP_CHECK(hipGetLastError());
@@ -5215,9 +5234,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   res.compSizes = wsT<int>(c, "gh_out_comp", comps);
   {
     LaunchTimer t(c, "gh_gather_out");
-    hipLaunchKernelGGL(pw == 64 ? k_gather_out<64> : k_gather_out<1024>, dim3(pgrp_grid(M, pw)),
-                       dim3(pgrp_block(pw)), 0, c.stream, M,
-                       (const long long *)nodeOff, (const long long *)compCapOff,
+    const int split = pgrp_split(M, pw);
+    hipLaunchKernelGGL(pw == 64 ? k_gather_out<64> : k_gather_out<1024>,
+                       dim3(pgrp_grid(M, pw) * (unsigned)split), dim3(pgrp_block(pw)), 0,
+                       c.stream, M, split, (const long long *)nodeOff, (const long long *)compCapOff,
                        (const long long *)res.vertOff, (const long long *)res.compOff, ta.xyTmp,
                        ta.compLen, res.xy, res.compSizes);
     HIP_CHECK(hipGetLastError());

@@ -896,7 +896,7 @@ void clip_complex(Context &c, const JoinInputs &in, const int *parts, long M,
   res.xy = wsT<double>(c, "cx_out_xy", 2 * nodes);
   res.compSizes = wsT<int>(c, "cx_out_comp", comps);
   hipLaunchKernelGGL(k_gather_out<64>, dim3(pgrp_grid(M, 64)), dim3(pgrp_block(64)), 0, c.stream,
-                     M, (const long long *)a.nodeOff, (const long long *)a.compOff,
+                     M, 1, (const long long *)a.nodeOff, (const long long *)a.compOff,
                      (const long long *)res.vertOff, (const long long *)res.compOff, w.xyTmp,
                      w.compLen, res.xy, res.compSizes);
   HIP_CHECK(hipGetLastError());

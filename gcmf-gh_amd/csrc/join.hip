@@ -15,6 +15,9 @@
 //      edge lists from ballot ranks [Join.h:157-287, GPU_Join.h:511-735].
 //   5. Segment-intersection refine, one wave per CMF survivor, wave-uniform
 //      early exit [Join.h:356-393, GPU_Join.h:326-456].
+#include <algorithm>
+#include <vector>
+
 #include "gcmf_device.hpp"
 #include "gcmf_internal.hpp"
 #include "gcmf_join.hpp"
@@ -24,51 +27,62 @@ namespace gcmf {
 constexpr uint64_t kKeyMod = 10000000000000000ULL;  // 16 decimal digits
 
 // ------------------------------------------------------------ stage 1 ----
+// mm: x key min, x key max, y key min, y key max (over both corners); a
+// grid of at most kEndpointBlocks blocks striding over the polygons
+constexpr unsigned kEndpointBlocks = 1024;
 __global__ void k_endpoints(long B, long O, const uint64_t *bMBR, const uint64_t *oMBR,
-                            uint64_t *keys, int *idx, uint64_t *Y, unsigned long long *mn,
-                            unsigned long long *mx) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t lo = ~0ULL, hi = 0;
-  if (e < B + O) {
+                            uint64_t *keys, int *idx, uint64_t *Y, unsigned long long *mm) {
+  uint64_t lo = ~0ULL, hi = 0, ylo = ~0ULL, yhi = 0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < B + O;
+       e += (long)gridDim.x * blockDim.x) {
     const uint64_t *m = e < B ? bMBR + 4 * e : oMBR + 4 * (e - B);
-    uint64_t x1 = m[0] % kKeyMod, x2 = m[2] % kKeyMod;
+    const uint64_t x1 = m[0] % kKeyMod, x2 = m[2] % kKeyMod, y1 = m[1], y2 = m[3];
     keys[2 * e] = x1;
     keys[2 * e + 1] = x2;
     idx[2 * e] = (int)(2 * e);
     idx[2 * e + 1] = (int)(2 * e + 1);
-    Y[2 * e] = m[1];
-    Y[2 * e + 1] = m[3];
-    lo = x1 < x2 ? x1 : x2;
-    hi = x1 < x2 ? x2 : x1;
+    Y[2 * e] = y1;
+    Y[2 * e + 1] = y2;
+    lo = min(lo, min(x1, x2));
+    hi = max(hi, max(x1, x2));
+    ylo = min(ylo, min(y1, y2));
+    yhi = max(yhi, max(y1, y2));
   }
   // wave min/max, block min/max, then one atomic per block and value
-  // (atomics per wave on one cache line serialise across the chip)
-  __shared__ uint64_t slo[4], shi[4];
+  // (atomics on one cache line serialise across the chip, so the grid is
+  // capped)
+  __shared__ uint64_t slo[4], shi[4], sylo[4], syhi[4];
   for (int off = 32; off > 0; off >>= 1) {
     uint64_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
+    uint64_t yl2 = __shfl_xor(ylo, off, 64), yh2 = __shfl_xor(yhi, off, 64);
     lo = l2 < lo ? l2 : lo;
     hi = h2 > hi ? h2 : hi;
+    ylo = yl2 < ylo ? yl2 : ylo;
+    yhi = yh2 > yhi ? yh2 : yhi;
   }
   if (__lane_id() == 0) {
     slo[threadIdx.x >> 6] = lo;
     shi[threadIdx.x >> 6] = hi;
+    sylo[threadIdx.x >> 6] = ylo;
+    syhi[threadIdx.x >> 6] = yhi;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
       lo = slo[w] < lo ? slo[w] : lo;
       hi = shi[w] > hi ? shi[w] : hi;
+      ylo = sylo[w] < ylo ? sylo[w] : ylo;
+      yhi = syhi[w] > yhi ? syhi[w] : yhi;
     }
-    atomicMin(mn, (unsigned long long)lo);
-    atomicMax(mx, (unsigned long long)hi);
+    atomicMin(mm, (unsigned long long)lo);
+    atomicMax(mm + 1, (unsigned long long)hi);
+    atomicMin(mm + 2, (unsigned long long)ylo);
+    atomicMax(mm + 3, (unsigned long long)yhi);
   }
 }
 
-__global__ void k_init_minmax(unsigned long long *mn, unsigned long long *mx) {
-  if (threadIdx.x == 0) {
-    *mn = ~0ULL;
-    *mx = 0ULL;
-  }
+__global__ void k_init_minmax(unsigned long long *mm) {
+  if (threadIdx.x < 4) mm[threadIdx.x] = (threadIdx.x & 1) ? 0ULL : ~0ULL;
 }
 
 __global__ void k_sub_key(uint64_t *keys, long n, const unsigned long long *mn) {
@@ -89,13 +103,19 @@ __global__ void k_sweep_prep(const int *perm, long n, const uint64_t *Y, int *ra
   rec_yhi[i] = Y[m + 1];
 }
 
-// one wave per polygon element; write == nullptr -> count only
+// one wave per polygon element (or per polygon of list[0, nList)); pairs ==
+// nullptr -> count only
 __global__ void __launch_bounds__(256)
     k_sweep(long B, long O, const int *rank, const int *rec_id, const uint64_t *rec_ylo,
             const uint64_t *rec_yhi, const uint64_t *Y, int *counts, const long long *offs,
-            int *pairs) {
-  const long e = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (e >= B + O) return;
+            int *pairs, const int *list, long nList) {
+  long e = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (list) {
+    if (e >= nList) return;
+    e = list[e];
+  } else if (e >= B + O) {
+    return;
+  }
   const int lane = __lane_id();
   const int lo = rank[2 * e], hi = rank[2 * e + 1];
   const uint64_t ey1 = Y[2 * e], ey2 = Y[2 * e + 1];
@@ -129,6 +149,352 @@ __global__ void __launch_bounds__(256)
   if (!pairs && lane == 0) counts[e] = cnt;
 }
 
+// ------------------------------------------------ stage 1, y strips ----
+// The sweep above walks a polygon's whole rank range, so its work grows with
+// the sum of the ranges (about N^1.5 for N evenly spread polygons), not with
+// the candidates.  For layers of Tuning::stripMin polygons or more the y
+// range of the MBR keys is cut into S strips of height 2^shift (the power of
+// two at or above twice the median MBR height of 1024 sampled polygons), and
+// every left-endpoint record is listed, per layer and in rank order, in each
+// strip its y interval touches.  A polygon then visits, in each strip its
+// own y interval touches, the other layer's records whose rank lies
+// strictly between its endpoints' ranks (a binary search for the first,
+// then a walk), applies the sweep's y test, and takes a record only in the
+// strip of the larger of the two intervals' lower ends, where both
+// intervals are listed, so each candidate is found once.  A polygon's
+// candidates are merged over its strips in rank order, the sweep's order.
+// Polygons touching more than kStripSpan strips, or with more than
+// kStripScan records to visit, are counted and written by the sweep itself
+// (one wave each), so the candidate list is the sweep's, entry for entry.
+constexpr int kStripSpan = 4;
+constexpr int kStripScan = 256;
+constexpr int kStripSample = 1024;
+constexpr long kStripMax = 1L << 22;
+
+struct Strips {
+  uint64_t y0;
+  int shift, S;
+};
+__device__ __forceinline__ int strip_of(const Strips &g, uint64_t y) {
+  const uint64_t s = (y - g.y0) >> g.shift;
+  return s >= (uint64_t)g.S ? g.S - 1 : (int)s;
+}
+
+// |ymax - ymin| of polygons j N / ns, j < ns
+__global__ void k_height_sample(long N, const uint64_t *Y, int ns, uint64_t *out) {
+  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= ns) return;
+  const long e = (long)((long long)j * N / ns);
+  const uint64_t a = Y[2 * e], b = Y[2 * e + 1];
+  out[j] = a > b ? a - b : b - a;
+}
+
+// the number of strips each record (left endpoint at rank i) is listed in
+__global__ void k_strip_rec_count(long n, const int *rec_id, const uint64_t *rec_ylo,
+                                  const uint64_t *rec_yhi, Strips g, int *cnt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = 0;
+  if (!(rec_id[i] & 1)) {
+    const uint64_t a = rec_ylo[i], b = rec_yhi[i];
+    c = strip_of(g, a < b ? b : a) - strip_of(g, a < b ? a : b) + 1;
+  }
+  cnt[i] = c;
+}
+
+// (layer * S + strip, rank) for every listing, in rank order; layer 1 is
+// the overlay (endpoint ids from 2B)
+__global__ void k_strip_rec_emit(long n, long B, const int *rec_id, const uint64_t *rec_ylo,
+                                 const uint64_t *rec_yhi, Strips g, const long long *off,
+                                 uint64_t *keys, int *vals) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (rec_id[i] & 1)) return;
+  const uint64_t a = rec_ylo[i], b = rec_yhi[i];
+  const int s0 = strip_of(g, a < b ? a : b), s1 = strip_of(g, a < b ? b : a);
+  const uint64_t side = rec_id[i] < 2 * B ? 0 : 1;
+  long long o = off[i];
+  for (int s = s0; s <= s1; s++, o++) {
+    keys[o] = side * (uint64_t)g.S + (uint64_t)s;
+    vals[o] = (int)i;
+  }
+}
+
+// sOff[k] = the first listing of key k in the sorted keys, k <= 2S (each
+// entry written once: the keys in (key[i - 1], key[i]] start at i)
+__global__ void k_strip_bounds(long R, const uint64_t *skeys, long nk, long long *sOff) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > R) return;
+  const long cur = i < R ? (long)skeys[i] : nk;
+  const long prev = i > 0 ? (long)skeys[i - 1] : -1;
+  for (long k = prev + 1; k <= cur; k++) sOff[k] = i;
+}
+
+// what a search needs of the left-endpoint record at rank r: its endpoint id
+// (2 e), the rank of polygon e's right endpoint and the y interval, in one
+// record so that a random visit touches one cache line
+struct StripRec {
+  int id, hi;
+  uint64_t y1, y2;
+};
+__global__ void k_strip_recs(long n, const int *rec_id, const uint64_t *rec_ylo,
+                             const uint64_t *rec_yhi, const int *rank, StripRec *qrec) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int m = rec_id[i];
+  if (!(m & 1)) qrec[i] = StripRec{m, rank[m + 1], rec_ylo[i], rec_yhi[i]};
+}
+
+// first index of [l, r) whose rank is above v
+__device__ __forceinline__ long first_above(const int *a, long l, long r, int v) {
+  while (l < r) {
+    const long m = (l + r) >> 1;
+    if (a[m] > v) r = m;
+    else l = m + 1;
+  }
+  return l;
+}
+
+// One polygon's strip ranges: returns false when the sweep takes it (more
+// than kStripSpan strips or kStripScan records).
+struct StripQuery {
+  int sa, ns;
+  long p0[kStripSpan], p1[kStripSpan];
+};
+__device__ __forceinline__ bool strip_query(const Strips &g, const long long *sOff,
+                                            const int *list, int side, int lo, int hi,
+                                            uint64_t ya, uint64_t yb, StripQuery &q) {
+  q.sa = strip_of(g, ya);
+  q.ns = strip_of(g, yb) - q.sa + 1;
+  if (q.ns > kStripSpan) return false;
+  long left = kStripScan;
+#pragma unroll
+  for (int t = 0; t < kStripSpan; t++) {
+    q.p0[t] = q.p1[t] = 0;
+    if (t < q.ns) {
+      const long k = (long)side * g.S + q.sa + t;
+      const long l = sOff[k], r = sOff[k + 1];
+      long p = first_above(list, l, r, lo);
+      q.p0[t] = p;
+      while (p < r && list[p] < hi) {
+        if (left-- == 0) return false;
+        p++;
+      }
+      q.p1[t] = p;
+    }
+  }
+  return true;
+}
+
+// the sweep's y test and the strip that owns the pair
+__device__ __forceinline__ bool strip_hit(const Strips &g, int s, uint64_t ey1, uint64_t ey2,
+                                          uint64_t ya, uint64_t ry1, uint64_t ry2) {
+  if (ey1 > ry2 || ry1 > ey2) return false;
+  const uint64_t rlo = ry1 < ry2 ? ry1 : ry2;
+  return strip_of(g, ya > rlo ? ya : rlo) == s;
+}
+
+// Count (pairs == nullptr) or write.  The count pass runs one thread per
+// listing (R of them, skeys / list): a polygon is taken at the listing of
+// its own record in the first strip it touches, so every polygon once, in
+// (layer, strip, rank) order -- neighbouring threads then search the same
+// strip's list near each other.  It keeps the first kStripInline
+// candidates' endpoint ids in inl (rank order), so the write pass (one
+// thread per polygon) copies them and searches again only for polygons with
+// more; a polygon the sweep takes is marked inl[e, 0] = -1 and appended to
+// heavy (heavy[0] counts them, the list follows).
+constexpr int kStripInline = 8;
+__global__ void __launch_bounds__(256)
+    k_strip_query(long B, long O, Strips g, const long long *sOff, const int *list,
+                  const uint64_t *skeys, long R, const int *rank, const StripRec *qrec,
+                  const uint64_t *Y, int *counts, const long long *offs, int *pairs,
+                  int *heavy, int *inl) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long e = t;
+  int lo, hi;
+  uint64_t ey1, ey2;
+  if (pairs) {
+    if (e >= B + O) return;
+    lo = rank[2 * e];
+    hi = rank[2 * e + 1];
+    ey1 = Y[2 * e];
+    ey2 = Y[2 * e + 1];
+  } else {
+    if (t >= R) return;
+    lo = list[t];
+    const StripRec me = qrec[lo];
+    e = me.id / 2;
+    hi = me.hi;
+    ey1 = me.y1;
+    ey2 = me.y2;
+    const long key = (long)skeys[t];
+    if ((key >= g.S ? key - g.S : key) != strip_of(g, ey1 < ey2 ? ey1 : ey2)) return;
+  }
+  const bool overlay = e >= B;
+  int *mine = inl + (long long)kStripInline * e;
+  long long pos = 0;
+  int cnt = 0;
+  if (pairs) {
+    cnt = counts[e];
+    if (cnt == 0 || mine[0] < 0) return;
+    pos = offs[e];
+    if (cnt <= kStripInline) {
+      for (int j = 0; j < cnt; j++) {
+        const int m = mine[j];
+        pairs[2 * (pos + j)] = overlay ? m / 2 : (int)e;
+        pairs[2 * (pos + j) + 1] = overlay ? (int)(e - B) : m / 2 - (int)B;
+      }
+      return;
+    }
+  }
+  const uint64_t ya = ey1 < ey2 ? ey1 : ey2, yb = ey1 < ey2 ? ey2 : ey1;
+  StripQuery q;
+  if (!strip_query(g, sOff, list, overlay ? 0 : 1, lo, hi, ya, yb, q)) {
+    counts[e] = 0;
+    mine[0] = -1;
+    heavy[1 + atomicAdd(heavy, 1)] = (int)e;
+    return;
+  }
+  // merge the strips' lists by rank
+  cnt = 0;
+  for (;;) {
+    int best = -1, br = 0x7fffffff;
+#pragma unroll
+    for (int u = 0; u < kStripSpan; u++)
+      if (q.p0[u] < q.p1[u]) {
+        const int r = list[q.p0[u]];
+        if (r < br) {
+          br = r;
+          best = u;
+        }
+      }
+    if (best < 0) break;
+#pragma unroll
+    for (int u = 0; u < kStripSpan; u++)
+      if (u == best) q.p0[u]++;
+    const StripRec x = qrec[br];
+    if (!strip_hit(g, q.sa + best, ey1, ey2, ya, x.y1, x.y2)) continue;
+    const int m = x.id;
+    if (pairs) {
+      pairs[2 * pos] = overlay ? m / 2 : (int)e;
+      pairs[2 * pos + 1] = overlay ? (int)(e - B) : m / 2 - (int)B;
+      pos++;
+    } else {
+      if (cnt < kStripInline) mine[cnt] = m;
+      cnt++;
+    }
+  }
+  if (!pairs) counts[e] = cnt;
+}
+
+// The strip form of the sweep: fills counts[N] and offs[N + 1] and returns
+// the candidate count, with the pairs written to *pairs_out; returns -1
+// (nothing written) when the records would be listed more than four times
+// each on average, where the plain sweep is used instead.
+static long strip_sweep(Context &c, long B, long O, const int *rank, const int *rec_id,
+                        const uint64_t *rec_ylo, const uint64_t *rec_yhi, const uint64_t *Y,
+                        uint64_t y0, uint64_t y1, int *counts, long long *offs,
+                        int **pairs_out) {
+  const long N = B + O, n = 2 * N;
+  // strip height: the power of two at or above twice the median sampled MBR
+  // height, at most kStripMax strips
+  const int ns = (int)std::min<long>(kStripSample, N);
+  uint64_t *samp = (uint64_t *)c.ws.get("strip_sample", 8 * (size_t)ns);
+  hipLaunchKernelGGL(k_height_sample, dim3(grid_for(ns, 256)), dim3(256), 0, c.stream, N, Y, ns,
+                     samp);
+  HIP_CHECK(hipGetLastError());
+  std::vector<uint64_t> hs((size_t)ns);
+  HIP_CHECK(hipMemcpyAsync(hs.data(), samp, 8 * (size_t)ns, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  std::nth_element(hs.begin(), hs.begin() + ns / 2, hs.end());
+  const uint64_t range = y1 - y0, h2 = 2 * hs[(size_t)ns / 2];
+  int shift = 0;
+  while (shift < 63 && ((1ULL << shift) < h2 || (range >> shift) + 1 > (uint64_t)kStripMax))
+    shift++;
+  Strips g{y0, shift, (int)((range >> shift) + 1)};
+  // listings, in rank order, then stably sorted by (layer, strip)
+  int *cnt = (int *)c.ws.get("strip_cnt", 4 * (size_t)n);
+  long long *off = (long long *)c.ws.get("strip_off", 8 * (size_t)(n + 1));
+  {
+    LaunchTimer t(c, "mbr_strip_build");
+    hipLaunchKernelGGL(k_strip_rec_count, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n,
+                       rec_id, rec_ylo, rec_yhi, g, cnt);
+    HIP_CHECK(hipGetLastError());
+  }
+  scan_i32_to_i64(c, cnt, off, n);
+  const long R = (long)read_scalar(c, off + n);
+  if (R > 4 * N) return -1;
+  const long nk = 2L * g.S;
+  const size_t R1 = (size_t)std::max(R, 1L);
+  uint64_t *keys = (uint64_t *)c.ws.get("strip_keys", 8 * R1);
+  int *vals = (int *)c.ws.get("strip_vals", 4 * R1);
+  uint64_t *skeys = (uint64_t *)c.ws.get("strip_skeys", 8 * R1);
+  int *list = (int *)c.ws.get("strip_list", 4 * R1);
+  long long *sOff = (long long *)c.ws.get("strip_soff", 8 * (size_t)(nk + 1));
+  StripRec *qrec = (StripRec *)c.ws.get("strip_recs", sizeof(StripRec) * (size_t)n);
+  {
+    LaunchTimer t(c, "mbr_strip_build");
+    hipLaunchKernelGGL(k_strip_rec_emit, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, B,
+                       rec_id, rec_ylo, rec_yhi, g, (const long long *)off, keys, vals);
+    HIP_CHECK(hipGetLastError());
+  }
+  int kbits = 0;
+  while (kbits < 64 && ((uint64_t)(nk - 1) >> kbits) != 0) kbits++;
+  radix_sort_u64(c, keys, vals, skeys, list, R, kbits);
+  {
+    LaunchTimer t(c, "mbr_strip_build");
+    hipLaunchKernelGGL(k_strip_bounds, dim3(grid_for(R + 1, 256)), dim3(256), 0, c.stream, R,
+                       (const uint64_t *)skeys, nk, sOff);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_strip_recs, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, rec_id,
+                       rec_ylo, rec_yhi, rank, qrec);
+    HIP_CHECK(hipGetLastError());
+  }
+  // counts; the polygons the strips leave to the sweep
+  int *heavy = (int *)c.ws.get("strip_heavy", 4 * (size_t)(N + 1));
+  int *inl = (int *)c.ws.get("strip_inline", 4 * (size_t)kStripInline * (size_t)N);
+  fill_i32(c, heavy, 0, 1);
+  {
+    LaunchTimer t(c, "mbr_strip_count");
+    hipLaunchKernelGGL(k_strip_query, dim3(grid_for(R, 256)), dim3(256), 0, c.stream, B, O, g,
+                       (const long long *)sOff, (const int *)list, (const uint64_t *)skeys, R,
+                       rank, (const StripRec *)qrec, Y, counts, (const long long *)nullptr,
+                       (int *)nullptr, heavy, inl);
+    HIP_CHECK(hipGetLastError());
+  }
+  int nHeavy = 0;
+  HIP_CHECK(hipMemcpyAsync(&nHeavy, heavy, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (nHeavy > 0) {
+    LaunchTimer t(c, "mbr_sweep_count");
+    hipLaunchKernelGGL(k_sweep, dim3(grid_for(nHeavy, 4)), dim3(256), 0, c.stream, B, O, rank,
+                       rec_id, rec_ylo, rec_yhi, Y, counts, (const long long *)nullptr,
+                       (int *)nullptr, (const int *)(heavy + 1), (long)nHeavy);
+    HIP_CHECK(hipGetLastError());
+  }
+  scan_i32_to_i64(c, counts, offs, N);
+  const long K = (long)read_scalar(c, offs + N);
+  int *pairs = (int *)c.ws.get("mbr_pairs", 8 * (size_t)(K > 0 ? K : 1));
+  if (K > 0) {
+    {
+      LaunchTimer t(c, "mbr_strip_write");
+      hipLaunchKernelGGL(k_strip_query, dim3(grid_for(N, 256)), dim3(256), 0, c.stream, B, O, g,
+                         (const long long *)sOff, (const int *)list, (const uint64_t *)skeys, R,
+                         rank, (const StripRec *)qrec, Y, counts, (const long long *)offs,
+                         pairs, heavy, inl);
+      HIP_CHECK(hipGetLastError());
+    }
+    if (nHeavy > 0) {
+      LaunchTimer t(c, "mbr_sweep_write");
+      hipLaunchKernelGGL(k_sweep, dim3(grid_for(nHeavy, 4)), dim3(256), 0, c.stream, B, O, rank,
+                         rec_id, rec_ylo, rec_yhi, Y, counts, (const long long *)offs, pairs,
+                         (const int *)(heavy + 1), (long)nHeavy);
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+  *pairs_out = pairs;
+  return K;
+}
+
 long mbr_filter(Context &c, long B, long O, const uint64_t *bMBR, const uint64_t *oMBR,
                 int **counts_out, int **pairs_out, MbrSortView *sv) {
   const long N = B + O, n = 2 * N;
@@ -136,7 +502,7 @@ long mbr_filter(Context &c, long B, long O, const uint64_t *bMBR, const uint64_t
   uint64_t *keys = (uint64_t *)c.ws.get("mbr_keys", 8 * (size_t)n);
   int *idx = (int *)c.ws.get("mbr_idx", 4 * (size_t)n);
   uint64_t *Y = (uint64_t *)c.ws.get("mbr_Y", 8 * (size_t)n);
-  unsigned long long *mm = (unsigned long long *)c.ws.get("mbr_minmax", 16);
+  unsigned long long *mm = (unsigned long long *)c.ws.get("mbr_minmax", 32);
   uint64_t *skeys = (uint64_t *)c.ws.get("mbr_skeys", 8 * (size_t)n);
   int *perm = (int *)c.ws.get("mbr_perm", 4 * (size_t)n);
   int *rank = (int *)c.ws.get("mbr_rank", 4 * (size_t)n);
@@ -145,15 +511,15 @@ long mbr_filter(Context &c, long B, long O, const uint64_t *bMBR, const uint64_t
   uint64_t *rec_yhi = (uint64_t *)c.ws.get("mbr_rec_yhi", 8 * (size_t)n);
   int *counts = (int *)c.ws.get("mbr_counts", 4 * (size_t)N);
   long long *offs = (long long *)c.ws.get("mbr_offs", 8 * (size_t)(N + 1));
-  hipLaunchKernelGGL(k_init_minmax, dim3(1), dim3(64), 0, c.stream, mm, mm + 1);
+  hipLaunchKernelGGL(k_init_minmax, dim3(1), dim3(64), 0, c.stream, mm);
   {
     LaunchTimer t(c, "mbr_endpoints");
-    hipLaunchKernelGGL(k_endpoints, dim3(grid_for(N, 256)), dim3(256), 0, c.stream, B, O, bMBR,
-                       oMBR, keys, idx, Y, mm, mm + 1);
+    hipLaunchKernelGGL(k_endpoints, dim3(std::min(grid_for(N, 256), kEndpointBlocks)), dim3(256),
+                       0, c.stream, B, O, bMBR, oMBR, keys, idx, Y, mm);
     HIP_CHECK(hipGetLastError());
   }
-  unsigned long long h[2];
-  HIP_CHECK(hipMemcpyAsync(h, mm, 16, hipMemcpyDeviceToHost, c.stream));
+  unsigned long long h[4];
+  HIP_CHECK(hipMemcpyAsync(h, mm, 32, hipMemcpyDeviceToHost, c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   unsigned long long range = h[1] - h[0];
   int bits = 0;
@@ -167,20 +533,28 @@ long mbr_filter(Context &c, long B, long O, const uint64_t *bMBR, const uint64_t
                        rank, rec_id, rec_ylo, rec_yhi);
     HIP_CHECK(hipGetLastError());
   }
-  {
-    LaunchTimer t(c, "mbr_sweep_count");
-    hipLaunchKernelGGL(k_sweep, dim3(grid_for(N, 4)), dim3(256), 0, c.stream, B, O, rank, rec_id,
-                       rec_ylo, rec_yhi, Y, counts, (const long long *)nullptr, (int *)nullptr);
-    HIP_CHECK(hipGetLastError());
-  }
-  scan_i32_to_i64(c, counts, offs, N);
-  long K = (long)read_scalar(c, offs + N);
-  int *pairs = (int *)c.ws.get("mbr_pairs", 8 * (size_t)(K > 0 ? K : 1));
-  if (K > 0) {
-    LaunchTimer t(c, "mbr_sweep_write");
-    hipLaunchKernelGGL(k_sweep, dim3(grid_for(N, 4)), dim3(256), 0, c.stream, B, O, rank, rec_id,
-                       rec_ylo, rec_yhi, Y, counts, (const long long *)offs, pairs);
-    HIP_CHECK(hipGetLastError());
+  long K = -1;
+  int *pairs = nullptr;
+  if (N >= c.tune.stripMin)
+    K = strip_sweep(c, B, O, rank, rec_id, rec_ylo, rec_yhi, Y, h[2], h[3], counts, offs, &pairs);
+  if (K < 0) {
+    {
+      LaunchTimer t(c, "mbr_sweep_count");
+      hipLaunchKernelGGL(k_sweep, dim3(grid_for(N, 4)), dim3(256), 0, c.stream, B, O, rank,
+                         rec_id, rec_ylo, rec_yhi, Y, counts, (const long long *)nullptr,
+                         (int *)nullptr, (const int *)nullptr, 0L);
+      HIP_CHECK(hipGetLastError());
+    }
+    scan_i32_to_i64(c, counts, offs, N);
+    K = (long)read_scalar(c, offs + N);
+    pairs = (int *)c.ws.get("mbr_pairs", 8 * (size_t)(K > 0 ? K : 1));
+    if (K > 0) {
+      LaunchTimer t(c, "mbr_sweep_write");
+      hipLaunchKernelGGL(k_sweep, dim3(grid_for(N, 4)), dim3(256), 0, c.stream, B, O, rank,
+                         rec_id, rec_ylo, rec_yhi, Y, counts, (const long long *)offs, pairs,
+                         (const int *)nullptr, 0L);
+      HIP_CHECK(hipGetLastError());
+    }
   }
   if (counts_out) *counts_out = counts;
   if (pairs_out) *pairs_out = pairs;

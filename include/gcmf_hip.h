@@ -225,6 +225,8 @@ typedef struct {
   int grid_clean;       /* in such a pair, result components longer than this
                           are cleaned up by the whole GPU, the others by one
                           wave or thread each (49.., 16384) */
+  long long strip_min;  /* joins of at least this many polygons (both layers)
+                          run the MBR sweep over y strips (2^17; 0: always) */
 } gcmf_tuning;
 int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out);
 int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t);

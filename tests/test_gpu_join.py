@@ -78,10 +78,25 @@ def test_join_c2_full_size(gctx):
 
 
 def test_join_many_keys(gctx):
-    # 2 x 150k small polygons: 600k endpoint keys, multi-tile sort and scans
+    # 2 x 150k small polygons: 600k endpoint keys, multi-tile sort and scans;
+    # above strip_min, so the MBR sweep runs over y strips
     base, over = G.gen_join_layers(150000, 150000, seed=11, rmin=1e-4, rmax=1e-3, nmin=8,
                                    nmax=16)
+    assert base.npoly + over.npoly >= gctx.tuning()["strip_min"]
     check_join(gctx, base, over)
+
+
+def test_join_c2_strip_sweep(gctx):
+    """The join workload's layers with the y-strip MBR sweep forced
+    (strip_min 0): every stage's output still equals the oracle's."""
+    base, over = G.gen_join_layers(20000, 20000, seed=0)
+    default = gctx.tuning()
+    try:
+        gctx.set_tuning(**dict(default, strip_min=0))
+        st = check_join(gctx, base, over)
+    finally:
+        gctx.set_tuning(**default)
+    assert st["candidates"] > 10000
 
 
 def test_join_then_clip_pipeline(gctx):

@@ -72,6 +72,68 @@ def layers(gctx):
     do.free()
 
 
+def _adversarial_mbrs(rs, n):
+    """MBR keys with inverted x or y corners, zero heights, tall boxes (so
+    that some polygons touch many y strips) and repeated keys."""
+    x1, y1 = rs.randint(0, 10 ** 6, n), rs.randint(0, 10 ** 6, n)
+    m = np.stack([x1, y1, x1 + rs.randint(0, 3000, n), y1 + rs.randint(0, 3000, n)], 1)
+    m = m.astype(np.uint64)
+    flip = rs.rand(n) < 0.1
+    m[flip, 1], m[flip, 3] = m[flip, 3].copy(), m[flip, 1].copy()
+    flip = rs.rand(n) < 0.05
+    m[flip, 0], m[flip, 2] = m[flip, 2].copy(), m[flip, 0].copy()
+    m[rs.rand(n) < 0.05, 3] = m[0, 1]  # one shared y key
+    z = rs.rand(n) < 0.05
+    m[z, 3] = m[z, 1]
+    t = rs.rand(n) < 0.03
+    m[t, 3] = m[t, 1] + np.uint64(400000)
+    return m
+
+
+def _mbr_cases():
+    rs = np.random.RandomState(21)
+    base, over = G.gen_join_layers(6000, 5000, seed=23)
+    cl = [(rs.randint(0, 50, (2000, 4)) * 1000).astype(np.uint64) for _ in range(2)]
+    for m, w, h in ((cl[0], 1000, 1000), (cl[1], 500, 2000)):
+        m[:, 2] = m[:, 0] + np.uint64(w)
+        m[:, 3] = m[:, 1] + np.uint64(h)
+    return {"layers": (base.mbr.reshape(-1, 4), over.mbr.reshape(-1, 4)),
+            "adversarial": (_adversarial_mbrs(rs, 3000), _adversarial_mbrs(rs, 3500)),
+            "clustered": tuple(cl)}
+
+
+@pytest.mark.parametrize("case", ["layers", "adversarial", "clustered"])
+def test_mbr_strip_sweep_equals_sweep(gctx, case):
+    """SortBaseMBROverlap with the y-strip sweep forced (strip_min 0) and
+    with the rank-range sweep alone: both give the oracle sweep's counts and
+    ordered candidate list (orc_mbr_sweep), on generated layers, on keys
+    with inverted, zero-height and tall boxes (polygons the strips hand back
+    to the sweep), and on clustered boxes with repeated keys."""
+    bmbr, ombr = _mbr_cases()[case]
+    counts_ref, pairs_ref = O.mbr_sweep(bmbr, ombr)
+    assert len(pairs_ref) > 0
+    L = G.hip_lib()
+    ref = G.RefContext()
+    default = ref.tuning()
+    db, do = gctx.upload(np.ascontiguousarray(bmbr)), gctx.upload(np.ascontiguousarray(ombr))
+    try:
+        for strip_min in (0, 1 << 40):
+            ref.set_tuning(**dict(default, strip_min=strip_min))
+            dcnt, dvec = C.c_void_p(), C.c_void_p()
+            K = L.SortBaseMBROverlap(len(bmbr), len(ombr), db.ptr, do.ptr, C.byref(dcnt),
+                                     C.byref(dvec), b"\x01", b"\x00")
+            assert K == len(pairs_ref), strip_min
+            np.testing.assert_array_equal(dget(gctx, dcnt, np.int32, len(bmbr) + len(ombr)),
+                                          counts_ref)
+            np.testing.assert_array_equal(dget(gctx, dvec, np.int32, 2 * K),
+                                          pairs_ref.reshape(-1))
+            dfree(gctx, dcnt, dvec)
+    finally:
+        ref.set_tuning(**default)
+        db.free()
+        do.free()
+
+
 def test_join_operators_in_reference_order(gctx, layers):
     base, over, db, do = layers
     L = G.hip_lib()
