@@ -208,6 +208,26 @@ def parity(ctx, js, cs, cand, joined, results):
                        "sizes, vertex coordinates) bit-exact against the oracle"}
 
 
+def graph_replay(ctx, steps, capture, total_verts):
+    """The same step captured as one HIP graph (gcmf_*_capture) and replayed
+    `steps` times back to back; the last replay is waited for and its size
+    checks read.  Every kernel runs in every replay; the result vertices of
+    the last replay are compared with the eager run's, bit for bit."""
+    xy0 = ctx.clip_get("xy", 2 * total_verts)
+    capture()
+    ctx.graph_launch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps - 1):
+        ctx.graph_launch(wait=False)
+    valid = ctx.graph_launch()
+    el = time.perf_counter() - t0
+    same = bool(np.array_equal(ctx.clip_get("xy", 2 * total_verts), xy0))
+    ctx.graph_destroy()
+    return {"ms_per_step": round(1e3 * el / steps, 4), "replays": steps,
+            "sizes_match_capture": valid, "results_equal_eager": same}
+
+
 def copy_calibration():
     import torch
     a = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
@@ -316,6 +336,9 @@ def main():
                "gh_init_label", "gh_cpu_fill", "gh_cpu_sort", "gh_cpu_link", "gh_label_trace",
                "gh_gather_out", "fill_i32"]
     times = {k: ctx.timer(k) for k in kernels}
+    # the step as one captured graph, replayed (reported beside the eager figure)
+    graph = graph_replay(ctx, args.steps, lambda: ctx.join_clip_capture(db, do, flags),
+                         cs["total_verts"])
     ms_step = 1e3 * elapsed / args.steps
     pairs = cs["pairs"]
     value = pairs / (elapsed / args.steps)
@@ -340,7 +363,7 @@ def main():
             "latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 4),
             "kernel_ms_per_step": {k: round(v[0] / BREAKDOWN_STEPS, 4)
                                    for k, v in times.items() if v[1]},
-            "roofline": roofs[0], "roofline_next": roofs[1]}
+            "roofline": roofs[0], "roofline_next": roofs[1], "graph_replay": graph}
     if cpu:
         line["cpu_baseline"] = cpu[0]
         line["parity"] = check
@@ -408,6 +431,9 @@ def batched(args):
     # same pairs; each pair's result must match its result in the batch
     nverts_batch = ctx.clip_get("nverts", args.pairs)
     status_batch = ctx.clip_get("status", args.pairs)
+    # the batch as one captured graph, replayed
+    graph = graph_replay(ctx, args.steps,
+                         lambda: ctx.clip_capture(dP, dQ, d_pairs, args.pairs), cs["total_verts"])
     stride = max(1, args.pairs // max(1, args.per_pair_sample))
     sample = list(range(0, args.pairs, stride))[:args.per_pair_sample]
     mism = 0
@@ -452,7 +478,7 @@ def batched(args):
                               "batched_speedup": round(args.pairs / (elapsed / args.steps)
                                                        / per_pair_rate, 2)},
             "roofline": label_roofline(lab_ms, lab_n, cs, None),
-            "roofline_next": count_roofline(cnt_ms, cnt_n, cs, None)}
+            "roofline_next": count_roofline(cnt_ms, cnt_n, cs, None), "graph_replay": graph}
     d_pairs.free()
     dP.free()
     dQ.free()
@@ -611,9 +637,23 @@ def prims(args):
         out.append({"prim": "mbr_filter", "keys": 4 * npoly, "candidates": K,
                     "ms": round(ms, 4), "sweep_ms": round(sweep_ms, 4), "GB/s": round(gb, 1),
                     "frac": round(gb / HBM_PEAK_GBS, 4)})
+        # CMF edge filter (count + apply) over the same join's candidates that
+        # the PiP test did not settle: every vertex of both polygons read
+        # (16 B) and every surviving edge index written (4 B)
+        cmf_ms = ctx.timer("cmf_count")[0] + ctx.timer("cmf_apply")[0]
+        pairs = ctx.join_get("pairs", 2 * K).reshape(-1, 2)
+        pip = ctx.join_get("pip_flag", K)
+        todo = pairs[pip != 1]
+        verts = int(base.vnum[todo[:, 0]].sum()) + int(over.vnum[todo[:, 1]].sum())
+        cbytes = 16 * verts + 4 * (js["b_edges"] + js["o_edges"])
+        cgb = cbytes / (cmf_ms * 1e-3) / 1e9
+        out.append({"prim": "cmf_filter", "keys": 4 * npoly, "pairs": int(len(todo)),
+                    "vertices": verts, "surviving_edges": int(js["b_edges"] + js["o_edges"]),
+                    "ms": round(cmf_ms, 4), "GB/s": round(cgb, 1),
+                    "frac": round(cgb / HBM_PEAK_GBS, 4)})
         db.free()
         do.free()
-        print(json.dumps(out[-1]), file=sys.stderr, flush=True)
+        print(json.dumps(out[-2:]), file=sys.stderr, flush=True)
     ctx.close()
     print(json.dumps({"workload": "primitive HBM sweep", "peak_GBs": HBM_PEAK_GBS,
                       "results": out}))

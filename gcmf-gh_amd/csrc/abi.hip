@@ -2,6 +2,7 @@
 // contexts, device memory helpers, kernel timers, and the batched join / clip
 // entry points.  Library exceptions become negative status codes here.
 #include <cstring>
+#include <functional>
 #include <string>
 
 #include <vector>
@@ -29,6 +30,12 @@ int guarded(gcmf_ctx *ctx, F &&f) {
   }
 }
 
+void no_graph(gcmf_ctx *ctx) {
+  if (ctx->graph)
+    throw gcmf::Error{GCMF_ERR_ARG,
+                      "a captured graph holds this context's workspace (gcmf_graph_destroy)"};
+}
+
 gcmf::JoinInputs to_inputs(const gcmf_dlayer *b, const gcmf_dlayer *o) {
   gcmf::JoinInputs in;
   in.B = b->npoly;
@@ -49,6 +56,7 @@ gcmf::JoinInputs to_inputs(const gcmf_dlayer *b, const gcmf_dlayer *o) {
 void run_join(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
               gcmf_join_stats *st) {
   if (!base || !over) throw gcmf::Error{GCMF_ERR_ARG, "null layer"};
+  if (ctx->c.replay == 0) no_graph(ctx);
   ctx->join_valid = false;
   ctx->jin = to_inputs(base, over);
   ctx->K = gcmf::mbr_filter(ctx->c, base->npoly, over->npoly, base->d_mbr, over->d_mbr,
@@ -91,6 +99,7 @@ void fill_clip_stats(gcmf_ctx *ctx, gcmf_clip_stats *st) {
 
 void run_clip(gcmf_ctx *ctx, const gcmf::JoinInputs &in, const int *d_pairs, long M, int flags,
               gcmf_clip_stats *st) {
+  if (ctx->c.replay == 0) no_graph(ctx);
   ctx->clip_valid = false;
   gcmf::ClipOptions opt;
   // frame bits: 1 CMBR normalisation, 4 edge scale (unless the unit frame is asked for)
@@ -103,6 +112,58 @@ void run_clip(gcmf_ctx *ctx, const gcmf::JoinInputs &in, const int *d_pairs, lon
   ctx->clip_valid = true;
   ctx->clip_nested = false;
   fill_clip_stats(ctx, st);
+}
+
+// Runs `run` eagerly, recording what it reads back, then captures it on the
+// context's stream into one graph: every read-back is answered from the
+// record, and a kernel in the graph sets the context's graph flag when the
+// value on the device differs.  run(true) is the eager run.
+void capture(gcmf_ctx *ctx, const std::function<void(bool)> &run) {
+  no_graph(ctx);
+  gcmf::Context &c = ctx->c;
+  c.rec.clear();
+  c.rec_bytes.clear();
+  c.rec_pos = c.rec_bytes_pos = 0;
+  c.graph_flag = (int *)c.ws.get("graph_flag", 256);
+  c.replay = 1;
+  try {
+    run(true);
+  } catch (...) {
+    c.replay = 0;
+    throw;
+  }
+  const bool timing0 = c.timing;
+  c.timing = false;
+  c.replay = 2;
+  c.ws.no_alloc = true;
+  auto restore = [&] {
+    c.replay = 0;
+    c.ws.no_alloc = false;
+    c.timing = timing0;
+  };
+  HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+  try {
+    gcmf::fill_i32(c, c.graph_flag, 0, 1);
+    gcmf::lb_reset(c);
+    run(false);
+    if (c.rec_pos != c.rec.size() || c.rec_bytes_pos != c.rec_bytes.size())
+      throw gcmf::Error{GCMF_ERR_ARG, "capture: read-back sequence differs from the recorded run"};
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    restore();
+    throw;
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c.stream, &g);
+  restore();
+  HIP_CHECK(e);
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIP_CHECK(e2);
+  ctx->graph = ex;
 }
 
 void copy_out(gcmf::Context &c, void *dst, const void *src, size_t bytes) {
@@ -152,6 +213,7 @@ void gcmf_shutdown(gcmf_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->c.device);
   (void)hipStreamSynchronize(ctx->c.stream);
+  if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
   ctx->c.ws.release_all();
   for (auto &p : ctx->c.pending) {
     (void)hipEventDestroy(p.second.first);
@@ -197,6 +259,7 @@ int gcmf_radix_sort_u64(gcmf_ctx *ctx, const uint64_t *d_keys_in, const int *d_v
                                                       !d_vals_out)))
       throw gcmf::Error{GCMF_ERR_ARG, "bad radix sort arguments"};
     if (n >= 0x7fffffffL) throw gcmf::Error{GCMF_ERR_ARG, "radix sort of 2^31 keys or more"};
+    no_graph(ctx);
     gcmf::radix_sort_u64(ctx->c, d_keys_in, d_vals_in, d_keys_out, d_vals_out, n, bits);
     gcmf::check_lookback(ctx->c);
   });
@@ -215,6 +278,7 @@ int gcmf_radix_sort_u64_noalloc(gcmf_ctx *ctx, const uint64_t *d_keys_in, const 
 int gcmf_scan_i32_i64(gcmf_ctx *ctx, const int *d_in, long long *d_out, long n) {
   return guarded(ctx, [&] {
     if (n < 0 || !d_out || (n > 0 && !d_in)) throw gcmf::Error{GCMF_ERR_ARG, "bad scan arguments"};
+    no_graph(ctx);
     gcmf::scan_i32_to_i64(ctx->c, d_in, d_out, n);
     gcmf::check_lookback(ctx->c);
   });
@@ -225,6 +289,7 @@ int gcmf_compact_flags(gcmf_ctx *ctx, const int *d_flags, long n, int *d_idx, lo
     if (n < 0 || !count || (n > 0 && (!d_flags || !d_idx)))
       throw gcmf::Error{GCMF_ERR_ARG, "bad compaction arguments"};
     if (n >= 0x7fffffffL) throw gcmf::Error{GCMF_ERR_ARG, "compaction of 2^31 flags or more"};
+    no_graph(ctx);
     *count = gcmf::compact_flags(ctx->c, d_flags, n, d_idx, "api");
     gcmf::check_lookback(ctx->c);
   });
@@ -327,6 +392,7 @@ int gcmf_clip_complex(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer 
   return guarded(ctx, [&] {
     if (!base || !over || (M > 0 && !d_parts) || M < 0)
       throw gcmf::Error{GCMF_ERR_ARG, "bad clip arguments"};
+    no_graph(ctx);
     ctx->clip_valid = false;
     gcmf::ClipOptions opt;
     opt.normalize = (flags & GCMF_CLIP_NORMALIZE) ? ((flags & GCMF_CLIP_UNIT_FRAME) ? 1 : 5) : 0;
@@ -380,6 +446,7 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
   return guarded(ctx, [&] {
     if (!base || !over || (M > 0 && !d_pairs) || M < 0)
       throw gcmf::Error{GCMF_ERR_ARG, "bad tuning arguments"};
+    no_graph(ctx);
     gcmf::Context &c = ctx->c;
     const gcmf::JoinInputs in = to_inputs(base, over);
     const bool timing0 = c.timing;
@@ -467,6 +534,53 @@ int gcmf_join_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
   return guarded(ctx, [&] {
     run_join(ctx, base, over, jstats);
     run_clip(ctx, ctx->jin, ctx->jr.joinedPairs, ctx->jr.nJoined, flags, cstats);
+  });
+}
+
+int gcmf_clip_capture(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+                      const int *d_pairs, long M, int flags, gcmf_clip_stats *stats) {
+  return guarded(ctx, [&] {
+    if (!base || !over || (M > 0 && !d_pairs) || M < 0)
+      throw gcmf::Error{GCMF_ERR_ARG, "bad clip arguments"};
+    const gcmf::JoinInputs in = to_inputs(base, over);
+    capture(ctx, [&](bool eager) {
+      run_clip(ctx, in, d_pairs, M, flags, eager ? stats : nullptr);
+    });
+  });
+}
+
+int gcmf_join_clip_capture(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+                           int flags, gcmf_join_stats *jstats, gcmf_clip_stats *cstats) {
+  return guarded(ctx, [&] {
+    if (!base || !over) throw gcmf::Error{GCMF_ERR_ARG, "null layer"};
+    capture(ctx, [&](bool eager) {
+      run_join(ctx, base, over, eager ? jstats : nullptr);
+      run_clip(ctx, ctx->jin, ctx->jr.joinedPairs, ctx->jr.nJoined, flags,
+               eager ? cstats : nullptr);
+    });
+  });
+}
+
+int gcmf_graph_launch(gcmf_ctx *ctx, int *valid) {
+  return guarded(ctx, [&] {
+    if (!ctx->graph) throw gcmf::Error{GCMF_ERR_ARG, "no captured graph on this context"};
+    gcmf::Context &c = ctx->c;
+    HIP_CHECK(hipGraphLaunch(ctx->graph, c.stream));
+    if (valid) {
+      int f = 1;
+      HIP_CHECK(hipMemcpyAsync(&f, c.graph_flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+      HIP_CHECK(hipStreamSynchronize(c.stream));
+      *valid = f == 0 ? 1 : 0;
+    }
+  });
+}
+
+int gcmf_graph_destroy(gcmf_ctx *ctx) {
+  return guarded(ctx, [&] {
+    if (!ctx->graph) return;
+    HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    HIP_CHECK(hipGraphExecDestroy(ctx->graph));
+    ctx->graph = nullptr;
   });
 }
 

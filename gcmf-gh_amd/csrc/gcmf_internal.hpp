@@ -43,6 +43,7 @@ struct Workspace {
     size_t bytes = 0;
   };
   std::map<std::string, Slot> slots;
+  bool no_alloc = false;  // set while a launch sequence is captured: get() may not allocate
   void *get(const std::string &name, size_t bytes);
   void release_all();
 };
@@ -95,6 +96,17 @@ struct Context {
   // uses one and clears the other for the next launch
   unsigned long long *lb_desc = nullptr;
   int lb_half = 0;
+  // Capture of a launch sequence into a HIP graph: replay 0 runs eagerly;
+  // 1 runs eagerly and records every value read back (rec, rec_bytes) and a
+  // device copy of every buffer read back whole; 2 captures, answering each
+  // read-back from the record and adding a kernel that sets *graph_flag when
+  // the value on the device differs from the recorded one.
+  int replay = 0;
+  std::vector<long long> rec;
+  size_t rec_pos = 0;
+  std::vector<std::vector<char>> rec_bytes;
+  size_t rec_bytes_pos = 0;
+  int *graph_flag = nullptr;
 
   hipEvent_t take_event();
   void flush_timers();  // synchronises on pending events and accumulates
@@ -136,12 +148,17 @@ void fill_i32(Context &c, int *p, int v, long n);
 // raises an Error when a single-pass scan's look-back wait ran past its
 // bound since the last check (synchronises)
 void check_lookback(Context &c);
+// copies `bytes` of device memory back to the host (synchronises); under
+// capture, answered from the recorded run and checked on the device
+void host_read(Context &c, void *dst, const void *src, size_t bytes);
+// clears the single-pass scans' descriptors and error word (the first step
+// of a captured sequence, so that every replay starts as the capture did)
+void lb_reset(Context &c);
 // copies one value back to the host (synchronises)
 template <typename T>
 T read_scalar(Context &c, const T *dptr) {
   T v;
-  HIP_CHECK(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  host_read(c, &v, dptr, sizeof(T));
   return v;
 }
 

@@ -132,4 +132,8 @@ struct gcmf_ctx {
   bool clip_nested = false;  // the last clip was of complex polygons (or the union)
   gcmf::ClipResult cr;
   gcmf::Phase1View p1;
+  // a captured clip or join-and-clip (gcmf_*_capture), replayed by
+  // gcmf_graph_launch; while it exists the calls that use the workspace
+  // are refused, since the graph holds the workspace's buffers
+  hipGraphExec_t graph = nullptr;
 };

@@ -5205,10 +5205,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     if (nGrid > 0) {
       // the pairs of gridNodes nodes or more, one cooperative launch each over
       // as many blocks as the GPU holds at once, after the other labelling
+      if (c.replay == 2)
+        throw Error{GCMF_ERR_ARG, "pairs labelled by the whole GPU cannot be captured"};
       std::vector<int> gl((size_t)nGrid);
-      HIP_CHECK(hipMemcpyAsync(gl.data(), gridList, sizeof(int) * (size_t)nGrid,
-                               hipMemcpyDeviceToHost, c.stream));
-      HIP_CHECK(hipStreamSynchronize(c.stream));
+      host_read(c, gl.data(), gridList, sizeof(int) * (size_t)nGrid);
       int perCu = 0;
       HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_label_grid, kGridBlock,
                                                              0));

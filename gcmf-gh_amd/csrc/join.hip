@@ -403,8 +403,7 @@ static long strip_sweep(Context &c, long B, long O, const int *rank, const int *
                      samp);
   HIP_CHECK(hipGetLastError());
   std::vector<uint64_t> hs((size_t)ns);
-  HIP_CHECK(hipMemcpyAsync(hs.data(), samp, 8 * (size_t)ns, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  host_read(c, hs.data(), samp, 8 * (size_t)ns);
   std::nth_element(hs.begin(), hs.begin() + ns / 2, hs.end());
   const uint64_t range = y1 - y0, h2 = 2 * hs[(size_t)ns / 2];
   int shift = 0;
@@ -461,9 +460,7 @@ static long strip_sweep(Context &c, long B, long O, const int *rank, const int *
                        (int *)nullptr, heavy, inl);
     HIP_CHECK(hipGetLastError());
   }
-  int nHeavy = 0;
-  HIP_CHECK(hipMemcpyAsync(&nHeavy, heavy, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  const int nHeavy = read_scalar(c, (const int *)heavy);
   if (nHeavy > 0) {
     LaunchTimer t(c, "mbr_sweep_count");
     hipLaunchKernelGGL(k_sweep, dim3(grid_for(nHeavy, 4)), dim3(256), 0, c.stream, B, O, rank,
@@ -519,8 +516,7 @@ long mbr_filter(Context &c, long B, long O, const uint64_t *bMBR, const uint64_t
     HIP_CHECK(hipGetLastError());
   }
   unsigned long long h[4];
-  HIP_CHECK(hipMemcpyAsync(h, mm, 32, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  host_read(c, h, mm, 32);
   unsigned long long range = h[1] - h[0];
   int bits = 0;
   while (bits < 64 && (range >> bits) != 0) bits++;

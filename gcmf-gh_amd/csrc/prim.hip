@@ -14,6 +14,7 @@
 // every digit run's ends, run on one XCD (measured on MI355X at 2^27 keys:
 // 1.57 -> 0.75 ms per scatter pass).
 #include "gcmf_device.hpp"
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 
@@ -27,6 +28,8 @@ void *Workspace::get(const std::string &name, size_t bytes) {
   if (bytes == 0) bytes = 256;
   Slot &s = slots[name];
   if (s.bytes < bytes) {
+    if (no_alloc)
+      throw Error{GCMF_ERR_ARG, "capture would allocate workspace '" + name + "'"};
     if (s.ptr) HIP_CHECK(hipFree(s.ptr));
     s.ptr = nullptr;
     size_t want = bytes + bytes / 4;  // headroom against small growth
@@ -107,6 +110,51 @@ __global__ void k_gather_scalars(ScalarSrc s, int n, long long *dst) {
   if (i < n) dst[i] = s.p[i] ? *s.p[i] : 0;
 }
 
+struct ScalarVals {
+  long long v[kHostSlots];
+};
+
+// sets *flag when a source's value differs from the recorded one
+__global__ void k_check_scalars(ScalarSrc s, int n, ScalarVals want, int *flag) {
+  const int i = threadIdx.x;
+  if (i < n && (s.p[i] ? *s.p[i] : 0) != want.v[i]) *flag = 1;
+}
+
+// sets *flag when the n bytes at a differ from those at b
+__global__ void k_check_bytes(const unsigned char *a, const unsigned char *b, long n, int *flag) {
+  bool diff = false;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    diff |= a[i] != b[i];
+  if (diff) *flag = 1;
+}
+
+void host_read(Context &c, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return;
+  if (c.replay == 2) {
+    if (c.rec_bytes_pos >= c.rec_bytes.size() || c.rec_bytes[c.rec_bytes_pos].size() != bytes)
+      throw Error{GCMF_ERR_ARG, "capture: read-back sequence differs from the recorded run"};
+    const size_t k = c.rec_bytes_pos++;
+    std::memcpy(dst, c.rec_bytes[k].data(), bytes);
+    const void *snap = c.ws.get("replay_snap_" + std::to_string(k), bytes);
+    hipLaunchKernelGGL(k_check_bytes, dim3(std::min<unsigned>(grid_for((long)bytes, 256), 64)),
+                       dim3(256), 0, c.stream, (const unsigned char *)src,
+                       (const unsigned char *)snap, (long)bytes, c.graph_flag);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  if (c.replay == 1) {
+    void *snap = c.ws.get("replay_snap_" + std::to_string(c.rec_bytes.size()), bytes);
+    HIP_CHECK(hipMemcpyAsync(snap, src, bytes, hipMemcpyDeviceToDevice, c.stream));
+  }
+  HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (c.replay == 1) {
+    const char *p = (const char *)dst;
+    c.rec_bytes.emplace_back(p, p + bytes);
+  }
+}
+
 // the look-back error word rides along as one more slot
 void read_scalars(Context &c, std::initializer_list<const long long *> src, long long *out) {
   const int n = (int)src.size();
@@ -115,6 +163,17 @@ void read_scalars(Context &c, std::initializer_list<const long long *> src, long
   int i = 0;
   for (const long long *p : src) s.p[i++] = p;
   s.p[n] = c.lb_err;
+  if (c.replay == 2) {
+    if (c.rec_pos + n > c.rec.size())
+      throw Error{GCMF_ERR_ARG, "capture: read-back sequence differs from the recorded run"};
+    ScalarVals want{};
+    for (int j = 0; j < n; j++) want.v[j] = out[j] = c.rec[c.rec_pos++];
+    want.v[n] = 0;  // no look-back overrun
+    hipLaunchKernelGGL(k_check_scalars, dim3(1), dim3(64), 0, c.stream, s, n + 1, want,
+                       c.graph_flag);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
   long long *d = (long long *)c.ws.get("read_scalars", sizeof(long long) * kHostSlots);
   hipLaunchKernelGGL(k_gather_scalars, dim3(1), dim3(64), 0, c.stream, s, n + 1, d);
   HIP_CHECK(hipGetLastError());
@@ -123,6 +182,7 @@ void read_scalars(Context &c, std::initializer_list<const long long *> src, long
                            c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   for (int j = 0; j < n; j++) out[j] = c.hbuf[j];
+  if (c.replay == 1) c.rec.insert(c.rec.end(), out, out + n);
   if (c.hbuf[n] != 0) {
     fill_i32(c, (int *)c.lb_err, 0, 2);
     throw Error{GCMF_ERR_HIP, "a scan's look-back wait exceeded its bound"};
@@ -610,6 +670,14 @@ static unsigned long long *lb_halves(Context &c, unsigned long long **next) {
 }
 
 static void lb_flip(Context &c) { c.lb_half = 1 - c.lb_half; }
+
+void lb_reset(Context &c) {
+  if (c.lb_desc) {
+    hipLaunchKernelGGL(k_lb_clear_all, dim3(1), dim3(256), 0, c.stream, c.lb_desc);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (c.lb_err) fill_i32(c, (int *)c.lb_err, 0, 2);
+}
 
 static bool lb_fits(const void *a, const void *b, long n) {
   return n > 0 && n <= kLbMaxN && !((uintptr_t)a & 15) && !((uintptr_t)b & 15);

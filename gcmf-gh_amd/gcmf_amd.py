@@ -269,6 +269,12 @@ def hip_lib():
                                 C.POINTER(ClipStats)]
         L.gcmf_join_clip.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), C.c_int,
                                      C.POINTER(JoinStats), C.POINTER(ClipStats)]
+        L.gcmf_clip_capture.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), vp, C.c_long,
+                                        C.c_int, C.POINTER(ClipStats)]
+        L.gcmf_join_clip_capture.argtypes = [vp, C.POINTER(DLayer), C.POINTER(DLayer), C.c_int,
+                                             C.POINTER(JoinStats), C.POINTER(ClipStats)]
+        L.gcmf_graph_launch.argtypes = [vp, C.POINTER(C.c_int)]
+        L.gcmf_graph_destroy.argtypes = [vp]
         L.gcmf_clip_get.argtypes = [vp, C.c_int, vp, C.c_long]
         L.gcmf_clip_areas.argtypes = [vp, vp]
         L.gcmf_clip_write.argtypes = [vp, C.c_char_p, C.c_int]
@@ -525,6 +531,37 @@ class Context:
         self._check(hip_lib().gcmf_join_clip(self.h, C.byref(base.desc), C.byref(over.desc),
                                              flags, C.byref(js), C.byref(cs)), "gcmf_join_clip")
         return js.as_dict(), cs.as_dict()
+
+    # ------------------------------------------------------ graph capture
+    def clip_capture(self, base, over, d_pairs, M, flags=CLIP_NORMALIZE):
+        """Clips the batch eagerly (device pairs buffer d_pairs of M pairs)
+        and captures the launches as a graph replayed by graph_launch;
+        returns the eager run's stats."""
+        st = ClipStats()
+        self._check(hip_lib().gcmf_clip_capture(self.h, C.byref(base.desc), C.byref(over.desc),
+                                                d_pairs.ptr, int(M), flags, C.byref(st)),
+                    "gcmf_clip_capture")
+        return st.as_dict()
+
+    def join_clip_capture(self, base, over, flags=CLIP_NORMALIZE):
+        js, cs = JoinStats(), ClipStats()
+        self._check(hip_lib().gcmf_join_clip_capture(self.h, C.byref(base.desc),
+                                                     C.byref(over.desc), flags, C.byref(js),
+                                                     C.byref(cs)), "gcmf_join_clip_capture")
+        return js.as_dict(), cs.as_dict()
+
+    def graph_launch(self, wait=True):
+        """Replays the captured graph; with wait, returns whether every size
+        matched the captured one (the results are valid)."""
+        if not wait:
+            self._check(hip_lib().gcmf_graph_launch(self.h, None), "gcmf_graph_launch")
+            return None
+        v = C.c_int(0)
+        self._check(hip_lib().gcmf_graph_launch(self.h, C.byref(v)), "gcmf_graph_launch")
+        return bool(v.value)
+
+    def graph_destroy(self):
+        self._check(hip_lib().gcmf_graph_destroy(self.h), "gcmf_graph_destroy")
 
     def clip_get(self, field, count):
         fid, dt = C_FIELDS[field]

@@ -242,6 +242,31 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
 int gcmf_join_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
                    int flags, gcmf_join_stats *jstats, gcmf_clip_stats *cstats);
 
+/* HIP-graph capture of a batch, for callers that run the same batch again
+ * (the launch sequence of ghcuda.cpp:393-422's per-pair loop, captured once
+ * for the whole batch; no reference counterpart).  gcmf_clip_capture (and
+ * gcmf_join_clip_capture) first runs gcmf_clip (gcmf_join_clip) eagerly --
+ * stats and results are that run's -- then captures the same launches on
+ * the context's stream as one graph, each size the eager run read back to
+ * the host becoming a constant that a kernel of the graph checks against the
+ * device's value.  gcmf_graph_launch replays the graph: every kernel runs
+ * again on the current contents of the layers and pairs, and the results
+ * are read with gcmf_clip_get / gcmf_join_get as after the eager call.  With
+ * valid non-null it waits for the replay and sets *valid to 0 when some
+ * size differed from the captured one (the inputs' shapes changed: the
+ * results are then not to be used); with valid null it returns at once.
+ * Replays are for inputs whose sizes are those captured: the buffers are
+ * the capture's.  While a graph exists, the calls that use the context's
+ * workspace (join, clip, tuning, sort, scan, compaction) return
+ * GCMF_ERR_ARG; gcmf_graph_destroy releases it.  Pairs labelled by the
+ * whole GPU (tuning grid_nodes) cannot be captured (GCMF_ERR_ARG). */
+int gcmf_clip_capture(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+                      const int *d_pairs, long M, int flags, gcmf_clip_stats *stats);
+int gcmf_join_clip_capture(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
+                           int flags, gcmf_join_stats *jstats, gcmf_clip_stats *cstats);
+int gcmf_graph_launch(gcmf_ctx *ctx, int *valid);
+int gcmf_graph_destroy(gcmf_ctx *ctx);
+
 enum {
   GCMF_C_STATUS = 0,    /* i32 [M] 0 ok, 1 invalid, 2 disjoint, <0 failure */
   GCMF_C_SWAPPED = 1,   /* i32 [M] */
