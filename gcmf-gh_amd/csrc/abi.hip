@@ -60,7 +60,7 @@ void run_join(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *over,
   ctx->join_valid = false;
   ctx->jin = to_inputs(base, over);
   ctx->K = gcmf::mbr_filter(ctx->c, base->npoly, over->npoly, base->d_mbr, over->d_mbr,
-                            &ctx->counts, &ctx->pairs, &ctx->sv);
+                            &ctx->counts, &ctx->pairs, &ctx->sv, base->d_vnum, over->d_vnum);
   gcmf::join_refine(ctx->c, ctx->jin, ctx->K, ctx->pairs, ctx->jr);
   ctx->c.flush_timers();
   ctx->join_valid = true;
@@ -419,6 +419,7 @@ int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out) {
     out->grid_nodes = ctx->c.tune.gridNodes;
     out->grid_clean = ctx->c.tune.gridClean;
     out->strip_min = ctx->c.tune.stripMin;
+    out->cmf_group = ctx->c.tune.cmfGroup;
   });
 }
 
@@ -428,7 +429,8 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
         (t->count_block != 64 && t->count_block != 128 && t->count_block != 256) ||
         t->lds_max_q < 0 || t->lds_max_q > 1024 || t->label_lds < 0 || t->label_lds > 65536 ||
         t->label_lds % 16 != 0 || t->grid_nodes < 1 || t->grid_clean < 49 ||
-        t->strip_min < 0)
+        t->strip_min < 0 ||
+        (t->cmf_group != 0 && t->cmf_group != 16 && t->cmf_group != 64))
       throw gcmf::Error{GCMF_ERR_ARG, "bad tuning"};
     ctx->c.tune.cullMin = t->cull_min;
     ctx->c.tune.bigNodes = t->big_nodes;
@@ -438,6 +440,7 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
     ctx->c.tune.gridNodes = t->grid_nodes;
     ctx->c.tune.gridClean = t->grid_clean;
     ctx->c.tune.stripMin = t->strip_min;
+    ctx->c.tune.cmfGroup = t->cmf_group;
   });
 }
 
@@ -525,6 +528,7 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
       out->grid_nodes = best.gridNodes;
       out->grid_clean = best.gridClean;
       out->strip_min = best.stripMin;
+      out->cmf_group = best.cmfGroup;
     }
   });
 }

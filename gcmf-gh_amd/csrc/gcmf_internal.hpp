@@ -62,6 +62,7 @@ struct Tuning {
   long long gridNodes = 1LL << 20;  // ... and with this many by the whole GPU
   int gridClean = 16384;    // there, components longer than this are cleaned by the grid
   long long stripMin = 1LL << 17;  // joins of this many polygons sweep the MBRs in y strips
+  int cmfGroup = 0;         // lanes per pair in the CMF edge filter (16, 64; 0: by polygon size)
   int countBlock = 128;     // threads per block of the count kernel (64, 128, 256)
   int labelLds = 20480;     // LDS bytes per labelling wave for a pair's arrays (0: none)
   int ldsMaxQ = 1024;       // pairs with Q rings above cullMin and up to this size are
@@ -107,6 +108,8 @@ struct Context {
   std::vector<std::vector<char>> rec_bytes;
   size_t rec_bytes_pos = 0;
   int *graph_flag = nullptr;
+  // mean vertex count per polygon of the last join's layers (-1: unknown)
+  double avgVerts = -1.0;
 
   hipEvent_t take_event();
   void flush_timers();  // synchronises on pending events and accumulates

@@ -21,8 +21,10 @@ struct MbrSortView {
 };
 
 // Stage 1.  Returns K; *pairs_out = workspace array of 2K int32 (b, o).
+// With the vertex counts, their mean per polygon is kept in c.avgVerts.
 long mbr_filter(Context &c, long B, long O, const uint64_t *bMBR, const uint64_t *oMBR,
-                int **counts_out, int **pairs_out, MbrSortView *sv);
+                int **counts_out, int **pairs_out, MbrSortView *sv,
+                const int *bVNum = nullptr, const int *oVNum = nullptr);
 
 // Stages 2-5, all device arrays in the context workspace.
 struct JoinRefine {

@@ -27,14 +27,17 @@ namespace gcmf {
 constexpr uint64_t kKeyMod = 10000000000000000ULL;  // 16 decimal digits
 
 // ------------------------------------------------------------ stage 1 ----
-// mm: x key min, x key max, y key min, y key max (over both corners); a
-// grid of at most kEndpointBlocks blocks striding over the polygons
+// mm: x key min, x key max, y key min, y key max (over both corners) and,
+// when the vertex counts are given, their sum; a grid of at most
+// kEndpointBlocks blocks striding over the polygons
 constexpr unsigned kEndpointBlocks = 1024;
 __global__ void k_endpoints(long B, long O, const uint64_t *bMBR, const uint64_t *oMBR,
-                            uint64_t *keys, int *idx, uint64_t *Y, unsigned long long *mm) {
-  uint64_t lo = ~0ULL, hi = 0, ylo = ~0ULL, yhi = 0;
+                            const int *bVNum, const int *oVNum, uint64_t *keys, int *idx,
+                            uint64_t *Y, unsigned long long *mm) {
+  uint64_t lo = ~0ULL, hi = 0, ylo = ~0ULL, yhi = 0, vs = 0;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < B + O;
        e += (long)gridDim.x * blockDim.x) {
+    if (bVNum) vs += (uint64_t)(e < B ? bVNum[e] : oVNum[e - B]);
     const uint64_t *m = e < B ? bMBR + 4 * e : oMBR + 4 * (e - B);
     const uint64_t x1 = m[0] % kKeyMod, x2 = m[2] % kKeyMod, y1 = m[1], y2 = m[3];
     keys[2 * e] = x1;
@@ -51,7 +54,7 @@ __global__ void k_endpoints(long B, long O, const uint64_t *bMBR, const uint64_t
   // wave min/max, block min/max, then one atomic per block and value
   // (atomics on one cache line serialise across the chip, so the grid is
   // capped)
-  __shared__ uint64_t slo[4], shi[4], sylo[4], syhi[4];
+  __shared__ uint64_t slo[4], shi[4], sylo[4], syhi[4], svs[4];
   for (int off = 32; off > 0; off >>= 1) {
     uint64_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
     uint64_t yl2 = __shfl_xor(ylo, off, 64), yh2 = __shfl_xor(yhi, off, 64);
@@ -59,12 +62,14 @@ __global__ void k_endpoints(long B, long O, const uint64_t *bMBR, const uint64_t
     hi = h2 > hi ? h2 : hi;
     ylo = yl2 < ylo ? yl2 : ylo;
     yhi = yh2 > yhi ? yh2 : yhi;
+    vs += __shfl_xor(vs, off, 64);
   }
   if (__lane_id() == 0) {
     slo[threadIdx.x >> 6] = lo;
     shi[threadIdx.x >> 6] = hi;
     sylo[threadIdx.x >> 6] = ylo;
     syhi[threadIdx.x >> 6] = yhi;
+    svs[threadIdx.x >> 6] = vs;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -73,16 +78,18 @@ __global__ void k_endpoints(long B, long O, const uint64_t *bMBR, const uint64_t
       hi = shi[w] > hi ? shi[w] : hi;
       ylo = sylo[w] < ylo ? sylo[w] : ylo;
       yhi = syhi[w] > yhi ? syhi[w] : yhi;
+      vs += svs[w];
     }
     atomicMin(mm, (unsigned long long)lo);
     atomicMax(mm + 1, (unsigned long long)hi);
     atomicMin(mm + 2, (unsigned long long)ylo);
     atomicMax(mm + 3, (unsigned long long)yhi);
+    if (vs) atomicAdd(mm + 4, (unsigned long long)vs);
   }
 }
 
 __global__ void k_init_minmax(unsigned long long *mm) {
-  if (threadIdx.x < 4) mm[threadIdx.x] = (threadIdx.x & 1) ? 0ULL : ~0ULL;
+  if (threadIdx.x < 5) mm[threadIdx.x] = threadIdx.x == 4 || (threadIdx.x & 1) ? 0ULL : ~0ULL;
 }
 
 __global__ void k_sub_key(uint64_t *keys, long n, const unsigned long long *mn) {
@@ -493,13 +500,14 @@ static long strip_sweep(Context &c, long B, long O, const int *rank, const int *
 }
 
 long mbr_filter(Context &c, long B, long O, const uint64_t *bMBR, const uint64_t *oMBR,
-                int **counts_out, int **pairs_out, MbrSortView *sv) {
+                int **counts_out, int **pairs_out, MbrSortView *sv, const int *bVNum,
+                const int *oVNum) {
   const long N = B + O, n = 2 * N;
   if (N <= 0) return 0;
   uint64_t *keys = (uint64_t *)c.ws.get("mbr_keys", 8 * (size_t)n);
   int *idx = (int *)c.ws.get("mbr_idx", 4 * (size_t)n);
   uint64_t *Y = (uint64_t *)c.ws.get("mbr_Y", 8 * (size_t)n);
-  unsigned long long *mm = (unsigned long long *)c.ws.get("mbr_minmax", 32);
+  unsigned long long *mm = (unsigned long long *)c.ws.get("mbr_minmax", 40);
   uint64_t *skeys = (uint64_t *)c.ws.get("mbr_skeys", 8 * (size_t)n);
   int *perm = (int *)c.ws.get("mbr_perm", 4 * (size_t)n);
   int *rank = (int *)c.ws.get("mbr_rank", 4 * (size_t)n);
@@ -512,11 +520,13 @@ long mbr_filter(Context &c, long B, long O, const uint64_t *bMBR, const uint64_t
   {
     LaunchTimer t(c, "mbr_endpoints");
     hipLaunchKernelGGL(k_endpoints, dim3(std::min(grid_for(N, 256), kEndpointBlocks)), dim3(256),
-                       0, c.stream, B, O, bMBR, oMBR, keys, idx, Y, mm);
+                       0, c.stream, B, O, bMBR, oMBR, bVNum && oVNum ? bVNum : nullptr, oVNum,
+                       keys, idx, Y, mm);
     HIP_CHECK(hipGetLastError());
   }
-  unsigned long long h[4];
-  host_read(c, h, mm, 32);
+  unsigned long long h[5];
+  host_read(c, h, mm, 40);
+  c.avgVerts = bVNum && oVNum ? (double)h[4] / (double)N : -1.0;
   unsigned long long range = h[1] - h[0];
   int bits = 0;
   while (bits < 64 && (range >> bits) != 0) bits++;
@@ -651,19 +661,26 @@ __device__ __forceinline__ long long list_base(const long long *off, int k, bool
   return inclusive ? (k > 0 ? off[k - 1] : 0) : off[k];
 }
 
+// polygons of at most this many vertices on average take the 16-lane CMF
+constexpr double kCmfSmall = 32.0;
+
 // count mode (bEdge == nullptr): counts and clears joinFlag when a side has
 // no edge; apply mode: writes the ordered edge lists at the pair offsets
 // (exclusive offsets, or the reference's inclusive prefix when inclusive).
-template <typename EdgeT>
+// One group of G lanes per pair (G = 16: four pairs per wave, for layers of
+// small polygons; G = 64: one wave per pair); ballots are masked to the
+// group, whose edges are ranked in ring order.
+template <typename EdgeT, int G>
 __global__ void __launch_bounds__(256)
     k_cmf(long nList, const int *list, const int *pairs, const double *cmbr,
           const double *bCoords, const int *bVNum, const long *bVPSNum, const double *oCoords,
           const int *oVNum, const long *oVPSNum, int *joinFlag, int *bCount, int *oCount,
           const long long *bOff, const long long *oOff, EdgeT *bEdge, EdgeT *oEdge,
           bool inclusive, const long long *nDev) {
-  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long w = (long)blockIdx.x * (256 / G) + (threadIdx.x / G);
   if (w >= (nDev ? (long)*nDev : nList)) return;
-  const int lane = __lane_id();
+  const int lane = __lane_id(), g = lane & (G - 1);
+  const unsigned long long gm = G == 64 ? ~0ULL : (((1ULL << G) - 1ULL) << (lane & ~(G - 1)));
   const int k = list[w];
   const int b = pairs[2 * k], o = pairs[2 * k + 1];
   const double cX1 = cmbr[4 * k], cY1 = cmbr[4 * k + 1], cX2 = cmbr[4 * k + 2],
@@ -676,16 +693,16 @@ __global__ void __launch_bounds__(256)
     EdgeT *outE = side == 0 ? bEdge : oEdge;
     long long pos = bEdge ? list_base(side == 0 ? bOff : oOff, k, inclusive) : 0;
     int n = 0;
-    for (int i0 = 0; i0 < vNum; i0 += 64) {
-      int i = i0 + lane;
+    for (int i0 = 0; i0 < vNum; i0 += G) {
+      int i = i0 + g;
       bool ok = i < vNum && cmf_edge(c, vNum, i, cX1, cY1, cX2, cY2);
-      unsigned long long m = __ballot(ok);
+      unsigned long long m = __ballot(ok) & gm;
       if (outE && ok) outE[pos + n + __popcll(m & ((1ULL << lane) - 1ULL))] = (EdgeT)i;
       n += __popcll(m);
     }
     cnt[side] = n;
   }
-  if (!bEdge && lane == 0) {
+  if (!bEdge && g == 0) {
     if (cnt[0] == 0 || cnt[1] == 0) {
       joinFlag[k] = 0;
       bCount[k] = 0;
@@ -889,9 +906,14 @@ void join_refine(Context &c, const JoinInputs &in, long K, const int *pairs, Joi
   fill_i32(c, r.segFlag, 0, K);
   const long long *withinN = compact_flags_dev(c, r.pipFlag, K, r.list, "within");
   const long long *cmfN = compact_flags_dev(c, r.joinFlag, K, r.list, "cmf");
+  // four pairs per wave where the layers' polygons are small on average
+  const bool cmf16 = c.tune.cmfGroup == 16 ||
+                     (c.tune.cmfGroup == 0 && c.avgVerts >= 0 && c.avgVerts <= kCmfSmall);
+  const int cmfPer = cmf16 ? 16 : 4;
   {
     LaunchTimer t(c, "cmf_count");
-    hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(K, 4)), dim3(256), 0, c.stream, K, r.list,
+    hipLaunchKernelGGL((cmf16 ? k_cmf<int, 16> : k_cmf<int, 64>), dim3(grid_for(K, cmfPer)),
+                       dim3(256), 0, c.stream, K, r.list,
                        pairs, r.cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum,
                        in.oVPSNum, r.joinFlag, r.bCount, r.oCount, (const long long *)nullptr,
                        (const long long *)nullptr, (int *)nullptr, (int *)nullptr, false, cmfN);
@@ -908,7 +930,8 @@ void join_refine(Context &c, const JoinInputs &in, long K, const int *pairs, Joi
   const long long *EN = compact_flags_dev(c, r.joinFlag, K, r.list, "cmf2");
   {
     LaunchTimer t(c, "cmf_apply");
-    hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(K, 4)), dim3(256), 0, c.stream, K, r.list,
+    hipLaunchKernelGGL((cmf16 ? k_cmf<int, 16> : k_cmf<int, 64>), dim3(grid_for(K, cmfPer)),
+                       dim3(256), 0, c.stream, K, r.list,
                        pairs, r.cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum,
                        in.oVPSNum, r.joinFlag, r.bCount, r.oCount, (const long long *)r.bOff,
                        (const long long *)r.oOff, r.bEdge, r.oEdge, false, EN);
@@ -972,7 +995,8 @@ void launch_cmf_count(Context &c, long nList, const int *list, const int *pairs,
                       int *oCount) {
   if (nList <= 0) return;
   LaunchTimer t(c, "cmf_count");
-  hipLaunchKernelGGL(k_cmf<int>, dim3(grid_for(nList, 4)), dim3(256), 0, c.stream, nList, list,
+  hipLaunchKernelGGL((k_cmf<int, 64>), dim3(grid_for(nList, 4)), dim3(256), 0, c.stream, nList,
+                     list,
                      pairs, cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum,
                      in.oVPSNum, joinFlag, bCount, oCount, (const long long *)nullptr,
                      (const long long *)nullptr, (int *)nullptr, (int *)nullptr, false,
@@ -985,7 +1009,8 @@ void launch_cmf_apply_short(Context &c, long nList, const int *list, const int *
                             const long long *oIncl, short *bEdge, short *oEdge) {
   if (nList <= 0) return;
   LaunchTimer t(c, "cmf_apply");
-  hipLaunchKernelGGL(k_cmf<short>, dim3(grid_for(nList, 4)), dim3(256), 0, c.stream, nList, list,
+  hipLaunchKernelGGL((k_cmf<short, 64>), dim3(grid_for(nList, 4)), dim3(256), 0, c.stream,
+                     nList, list,
                      pairs, cmbr, in.bCoords, in.bVNum, in.bVPSNum, in.oCoords, in.oVNum,
                      in.oVPSNum, (int *)nullptr, (int *)nullptr, (int *)nullptr, bIncl, oIncl,
                      bEdge, oEdge, true, (const long long *)nullptr);

@@ -171,7 +171,8 @@ gcmf::JoinRefine &st_intersect(gcmf_ctx &x, long bPolNum, long oPolNum, coord_t 
   in.oMBR2 = (const double *)up("om2", seqOMBR2, 32 * (size_t)oPolNum);
   x.join_valid = false;
   x.jin = in;
-  x.K = gcmf::mbr_filter(c, bPolNum, oPolNum, in.bMBR, in.oMBR, &x.counts, &x.pairs, &x.sv);
+  x.K = gcmf::mbr_filter(c, bPolNum, oPolNum, in.bMBR, in.oMBR, &x.counts, &x.pairs, &x.sv,
+                         in.bVNum, in.oVNum);
   gcmf::join_refine(c, in, x.K, x.pairs, x.jr);
   x.join_valid = true;
   return x.jr;
@@ -596,7 +597,8 @@ int spatialJoin(int argc, char *argv[], int **pIDList, int **qIDList, long *tota
     in.oMBR = dk[1];
     x.join_valid = false;
     x.jin = in;
-    x.K = gcmf::mbr_filter(c, B, O, in.bMBR, in.oMBR, &x.counts, &x.pairs, &x.sv);
+    x.K = gcmf::mbr_filter(c, B, O, in.bMBR, in.oMBR, &x.counts, &x.pairs, &x.sv, in.bVNum,
+                           in.oVNum);
     gcmf::join_refine(c, in, x.K, x.pairs, x.jr);
     x.join_valid = true;
     // the joined pairs (within U intersecting; the reference hands over the

@@ -227,6 +227,9 @@ typedef struct {
                           wave or thread each (49.., 16384) */
   long long strip_min;  /* joins of at least this many polygons (both layers)
                           run the MBR sweep over y strips (2^17; 0: always) */
+  int cmf_group;        /* lanes per candidate pair in the CMF edge filter:
+                          16 (four pairs per wave), 64, or 0: 16 when the
+                          layers average at most 32 vertices per polygon (0) */
 } gcmf_tuning;
 int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out);
 int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t);

@@ -86,6 +86,23 @@ def test_join_many_keys(gctx):
     check_join(gctx, base, over)
 
 
+@pytest.mark.parametrize("group", [16, 64])
+def test_join_cmf_group_widths(gctx, group):
+    """The CMF edge filter with 16 lanes per candidate pair (four pairs per
+    wave, edge lists ranked by masked ballots) and with a wave per pair, on
+    small and on larger polygons: the same outputs as the oracle."""
+    default = gctx.tuning()
+    try:
+        gctx.set_tuning(**dict(default, cmf_group=group))
+        for base, over in (G.gen_join_layers(1500, 1200, seed=3, rmin=5e-3, rmax=5e-2, nmin=8,
+                                             nmax=64),
+                           G.gen_join_layers(3000, 3000, seed=5, rmin=2e-3, rmax=2e-2, nmin=16,
+                                             nmax=300)):
+            check_join(gctx, base, over)
+    finally:
+        gctx.set_tuning(**default)
+
+
 def test_join_c2_strip_sweep(gctx):
     """The join workload's layers with the y-strip MBR sweep forced
     (strip_min 0): every stage's output still equals the oracle's."""
