@@ -246,6 +246,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=20000, help="polygons per layer")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the captured-graph replay after the timed steps (for profiles)")
     ap.add_argument("--workload", choices=["join-clip", "batched", "large", "serve", "prims"],
                     default="join-clip",
                     help="join-clip: the two-layer join + clip (default); batched: "
@@ -337,8 +339,8 @@ def main():
                "gh_gather_out", "fill_i32"]
     times = {k: ctx.timer(k) for k in kernels}
     # the step as one captured graph, replayed (reported beside the eager figure)
-    graph = graph_replay(ctx, args.steps, lambda: ctx.join_clip_capture(db, do, flags),
-                         cs["total_verts"])
+    graph = None if args.no_graph else graph_replay(
+        ctx, args.steps, lambda: ctx.join_clip_capture(db, do, flags), cs["total_verts"])
     ms_step = 1e3 * elapsed / args.steps
     pairs = cs["pairs"]
     value = pairs / (elapsed / args.steps)
@@ -432,8 +434,8 @@ def batched(args):
     nverts_batch = ctx.clip_get("nverts", args.pairs)
     status_batch = ctx.clip_get("status", args.pairs)
     # the batch as one captured graph, replayed
-    graph = graph_replay(ctx, args.steps,
-                         lambda: ctx.clip_capture(dP, dQ, d_pairs, args.pairs), cs["total_verts"])
+    graph = None if args.no_graph else graph_replay(
+        ctx, args.steps, lambda: ctx.clip_capture(dP, dQ, d_pairs, args.pairs), cs["total_verts"])
     stride = max(1, args.pairs // max(1, args.per_pair_sample))
     sample = list(range(0, args.pairs, stride))[:args.per_pair_sample]
     mism = 0
