@@ -420,6 +420,7 @@ int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out) {
     out->grid_clean = ctx->c.tune.gridClean;
     out->strip_min = ctx->c.tune.stripMin;
     out->cmf_group = ctx->c.tune.cmfGroup;
+    out->label_steal = ctx->c.tune.labelSteal;
   });
 }
 
@@ -430,7 +431,8 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
         t->lds_max_q < 0 || t->lds_max_q > 1024 || t->label_lds < 0 || t->label_lds > 65536 ||
         t->label_lds % 16 != 0 || t->grid_nodes < 1 || t->grid_clean < 49 ||
         t->strip_min < 0 ||
-        (t->cmf_group != 0 && t->cmf_group != 16 && t->cmf_group != 64))
+        (t->cmf_group != 0 && t->cmf_group != 16 && t->cmf_group != 64) ||
+        (t->label_steal != 0 && t->label_steal != 1))
       throw gcmf::Error{GCMF_ERR_ARG, "bad tuning"};
     ctx->c.tune.cullMin = t->cull_min;
     ctx->c.tune.bigNodes = t->big_nodes;
@@ -441,6 +443,7 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
     ctx->c.tune.gridClean = t->grid_clean;
     ctx->c.tune.stripMin = t->strip_min;
     ctx->c.tune.cmfGroup = t->cmf_group;
+    ctx->c.tune.labelSteal = t->label_steal;
   });
 }
 
@@ -508,11 +511,20 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
     }
     {
       double bt = 1e300;
-      for (int v : {0, 16384, 24576, 32768, 49152, 65536}) {
+      for (int v : {0, 16384, 20480, 24576, 32768, 49152, 65536}) {
         c.tune = best;
         c.tune.labelLds = v;
         const double t = measure("gh_label_trace");
         if (t < bt) bt = t, best.labelLds = v;
+      }
+    }
+    {
+      double bt = 1e300;
+      for (int v : {0, 1}) {
+        c.tune = best;
+        c.tune.labelSteal = v;
+        const double t = measure("gh_label_trace");
+        if (t < bt) bt = t, best.labelSteal = v;
       }
     }
     c.tune = best;
@@ -529,6 +541,7 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
       out->grid_clean = best.gridClean;
       out->strip_min = best.stripMin;
       out->cmf_group = best.cmfGroup;
+      out->label_steal = best.labelSteal;
     }
   });
 }

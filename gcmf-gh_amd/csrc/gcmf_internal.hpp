@@ -63,6 +63,7 @@ struct Tuning {
   int gridClean = 16384;    // there, components longer than this are cleaned by the grid
   long long stripMin = 1LL << 17;  // joins of this many polygons sweep the MBRs in y strips
   int cmfGroup = 0;         // lanes per pair in the CMF edge filter (16, 64; 0: by polygon size)
+  int labelSteal = 0;       // 1: labelling waves are persistent and take pairs from a counter
   int countBlock = 128;     // threads per block of the count kernel (64, 128, 256)
   int labelLds = 20480;     // LDS bytes per labelling wave for a pair's arrays (0: none)
   int ldsMaxQ = 1024;       // pairs with Q rings above cullMin and up to this size are
@@ -110,6 +111,10 @@ struct Context {
   int *graph_flag = nullptr;
   // mean vertex count per polygon of the last join's layers (-1: unknown)
   double avgVerts = -1.0;
+  // the persistent labelling launch's grid (waves the GPU holds at once) for
+  // the LDS arena size it was measured with
+  long label_steal_blocks = 0;
+  int label_steal_lds = -1;
 
   hipEvent_t take_event();
   void flush_timers();  // synchronises on pending events and accumulates

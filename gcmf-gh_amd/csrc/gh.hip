@@ -4312,6 +4312,33 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// The same labelling as a persistent, work-stealing launch: as many waves as
+// the GPU holds at once, each taking the next pair of the order from a
+// device counter until the order is exhausted (every wave reaches the exit:
+// the counter only grows); the pairs of k_label_block are skipped as above.
+__global__ void __launch_bounds__(64)
+    k_label_wave_steal(long M, const int *order, unsigned long long *next, int *status,
+                       SlotArrays s, TraceArgs a, int *aux, int *lpath, long long *lticks,
+                       int arenaBytes) {
+  extern __shared__ __align__(16) char label_lds[];
+  for (;;) {
+    unsigned long long idx = 0;
+    if (threadIdx.x == 0) idx = atomicAdd(next, 1ULL);
+    idx = __shfl(idx, 0, 64);
+    if (idx >= (unsigned long long)M) return;
+    const long k = order ? order[idx] : (long)idx;
+    if (pair_nodes(a, k) >= a.bigNodes) continue;  // k_label_block's
+    const long long t0 = wall_clock64();
+    const int path = label_pair<64>(k, status, s, a, aux, arenaBytes > 0 ? label_lds : nullptr,
+                                    arenaBytes);
+    if (threadIdx.x == 0 && lpath) {
+      lpath[k] = path;
+      lticks[k] = wall_clock64() - t0;
+    }
+    __syncthreads();  // the arena is reused by the next pair
+  }
+}
+
 // the pairs k_label_block takes (bigNodes <= nodes < gridNodes) appended to
 // list, cnt[0] their number, and those k_label_grid takes (nodes >=
 // gridNodes) to glist, cnt[1] their number; by size alone, whatever their
@@ -5197,9 +5224,33 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       HIP_CHECK(hipGetLastError());
       HIP_CHECK(hipEventRecord(c.ev_join, c.side));
     }
-    hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), (size_t)c.tune.labelLds,
-                       c.stream, M, (const int *)order, pm.status, s, ta, aux, res.labelPath,
-                       res.labelTicks, c.tune.labelLds);
+    if (c.tune.labelSteal) {
+      // persistent waves taking the pairs in turn
+      if (c.label_steal_blocks <= 0) {
+        int perCu = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &perCu, k_label_wave_steal, 64, (size_t)c.tune.labelLds));
+        c.label_steal_blocks = (perCu > 0 ? perCu : 1) * c.cu_count;
+        c.label_steal_lds = c.tune.labelLds;
+      }
+      if (c.label_steal_lds != c.tune.labelLds) {
+        int perCu = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &perCu, k_label_wave_steal, 64, (size_t)c.tune.labelLds));
+        c.label_steal_blocks = (perCu > 0 ? perCu : 1) * c.cu_count;
+        c.label_steal_lds = c.tune.labelLds;
+      }
+      unsigned long long *next = wsT<unsigned long long>(c, "gh_label_next", 1);
+      fill_i32(c, (int *)next, 0, 2);
+      const long blocks = std::min<long>(M, c.label_steal_blocks);
+      hipLaunchKernelGGL(k_label_wave_steal, dim3((unsigned)blocks), dim3(64),
+                         (size_t)c.tune.labelLds, c.stream, M, (const int *)order, next,
+                         pm.status, s, ta, aux, res.labelPath, res.labelTicks, c.tune.labelLds);
+    } else {
+      hipLaunchKernelGGL(k_label_wave, dim3((unsigned)M), dim3(64), (size_t)c.tune.labelLds,
+                         c.stream, M, (const int *)order, pm.status, s, ta, aux, res.labelPath,
+                         res.labelTicks, c.tune.labelLds);
+    }
     HIP_CHECK(hipGetLastError());
     if (nBig > 0) HIP_CHECK(hipStreamWaitEvent(c.stream, c.ev_join, 0));
     if (nGrid > 0) {

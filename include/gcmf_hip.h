@@ -230,6 +230,9 @@ typedef struct {
   int cmf_group;        /* lanes per candidate pair in the CMF edge filter:
                           16 (four pairs per wave), 64, or 0: 16 when the
                           layers average at most 32 vertices per polygon (0) */
+  int label_steal;      /* 1: the labelling waves are a persistent launch that
+                          takes the pairs from a device counter in turn
+                          (work stealing); 0: one wave per pair (0) */
 } gcmf_tuning;
 int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out);
 int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t);

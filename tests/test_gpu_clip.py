@@ -587,6 +587,36 @@ def test_clip_grid_labelling_large_pair(gctx):
     assert hashlib.sha256(xy.tobytes()).hexdigest() == ref["xy_sha256"]
 
 
+def test_label_steal_keeps_results(gctx):
+    """The persistent, work-stealing labelling launch (waves taking pairs from
+    a device counter, label_steal 1) gives every pair the oracle's result and
+    the one-wave-per-pair launch's, on a batch with the block path's pairs,
+    the sequential fallback and the no-crossing path, in largest-first order
+    (1024 pairs or more) and in index order."""
+    z = np.load(GOLDEN + "/batched_pairs.npz")
+    cases = [(n, z[n + "_P"], z[n + "_Q"]) for n in ["p4596", "p156581", "p73779", "p176719"]]
+    cases += fig_cases() + random_cases(60, seed0=33, nmax=400) + snapped_cases(810, n=4)
+    cases += [("wavy%d" % s,) + tuple(G.gen_single_pair(1, 50 + s, 6000, 1500)) for s in range(2)]
+    P, Q = G.gen_pair_layers(1100, seed=7, nmax=2000)
+    big = cases + [("u%d" % k, P.ring(k), Q.ring(k)) for k in range(1100)]
+    default = gctx.tuning()
+    for batch in (cases, big):
+        st0 = run_batch(gctx, batch, G.CLIP_NORMALIZE)
+        s0, r0 = gctx.clip_results(st0)
+        try:
+            gctx.set_tuning(**dict(default, label_steal=1))
+            st = run_batch(gctx, batch, G.CLIP_NORMALIZE)
+            s1, r1 = gctx.clip_results(st)
+        finally:
+            gctx.set_tuning(**default)
+        np.testing.assert_array_equal(s1, s0)
+        for x, y in zip(r1, r0):
+            assert [len(c) for c in x] == [len(c) for c in y]
+            for p, q in zip(x, y):
+                np.testing.assert_array_equal(p, q)
+    check_results(gctx, st, big, NORM)
+
+
 def dense_square_cases():
     """A square with 1500 vertices on each side against a diamond and a
     7-point star crossing it: the result components carry runs of hundreds
