@@ -421,6 +421,7 @@ int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out) {
     out->strip_min = ctx->c.tune.stripMin;
     out->cmf_group = ctx->c.tune.cmfGroup;
     out->label_steal = ctx->c.tune.labelSteal;
+    out->block_cap = ctx->c.tune.blockCap;
   });
 }
 
@@ -432,7 +433,8 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
         t->label_lds % 16 != 0 || t->grid_nodes < 1 || t->grid_clean < 49 ||
         t->strip_min < 0 ||
         (t->cmf_group != 0 && t->cmf_group != 16 && t->cmf_group != 64) ||
-        (t->label_steal != 0 && t->label_steal != 1))
+        (t->label_steal != 0 && t->label_steal != 1) || t->block_cap < 1 ||
+        t->block_cap > 65536)
       throw gcmf::Error{GCMF_ERR_ARG, "bad tuning"};
     ctx->c.tune.cullMin = t->cull_min;
     ctx->c.tune.bigNodes = t->big_nodes;
@@ -444,6 +446,7 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
     ctx->c.tune.stripMin = t->strip_min;
     ctx->c.tune.cmfGroup = t->cmf_group;
     ctx->c.tune.labelSteal = t->label_steal;
+    ctx->c.tune.blockCap = t->block_cap;
   });
 }
 
@@ -542,6 +545,7 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
       out->strip_min = best.stripMin;
       out->cmf_group = best.cmfGroup;
       out->label_steal = best.labelSteal;
+      out->block_cap = best.blockCap;
     }
   });
 }

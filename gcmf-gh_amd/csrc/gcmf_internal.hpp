@@ -64,6 +64,7 @@ struct Tuning {
   long long stripMin = 1LL << 17;  // joins of this many polygons sweep the MBRs in y strips
   int cmfGroup = 0;         // lanes per pair in the CMF edge filter (16, 64; 0: by polygon size)
   int labelSteal = 0;       // 1: labelling waves are persistent and take pairs from a counter
+  int blockCap = 128;       // blocks of the block labelling (each takes pairs from a counter)
   int countBlock = 128;     // threads per block of the count kernel (64, 128, 256)
   int labelLds = 20480;     // LDS bytes per labelling wave for a pair's arrays (0: none)
   int ldsMaxQ = 1024;       // pairs with Q rings above cullMin and up to this size are

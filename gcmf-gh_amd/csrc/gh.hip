@@ -4381,10 +4381,18 @@ __global__ void __launch_bounds__(kGridBlock)
 // One block of kBigGroup threads per listed pair, the blocks taking the list
 // in turn
 __global__ void __launch_bounds__(kBigGroup)
-    k_label_block(const int *list, const unsigned long long *cnt, int *status, SlotArrays s,
-                  TraceArgs a, int *aux, int *lpath, long long *lticks) {
+    k_label_block(const int *list, const unsigned long long *cnt, unsigned long long *next,
+                  int *status, SlotArrays s, TraceArgs a, int *aux, int *lpath,
+                  long long *lticks) {
   const long n = (long)*cnt;
-  for (long i = blockIdx.x; i < n; i += gridDim.x) {
+  __shared__ long s_i;
+  for (;;) {
+    // the next listed pair from a device counter: a block held up by a slow
+    // pair (a sequential fallback) takes no further pairs meanwhile
+    if (threadIdx.x == 0) s_i = (long)atomicAdd(next, 1ULL);
+    __syncthreads();
+    const long i = s_i;
+    if (i >= n) return;
     const long k = list[i];
     const long long t0 = wall_clock64();
     const int path = label_pair<kBigGroup>(k, status, s, a, aux, nullptr, 0);
@@ -5215,11 +5223,14 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     // the large pairs on the side stream, concurrently with one wave per
     // remaining pair on the main one (disjoint pairs, disjoint regions)
     if (nBig > 0) {
+      unsigned long long *bigNext = wsT<unsigned long long>(c, "gh_big_next", 1);
+      fill_i32(c, (int *)bigNext, 0, 2);
       HIP_CHECK(hipEventRecord(c.ev_fork, c.stream));
       HIP_CHECK(hipStreamWaitEvent(c.side, c.ev_fork, 0));
-      hipLaunchKernelGGL(k_label_block, dim3((unsigned)(nBig < 128 ? nBig : 128)),
-                         dim3(kBigGroup), 0, c.side, (const int *)(order ? order : bigList),
-                         (const unsigned long long *)bigCnt, pm.status, s, ta, aux,
+      const long blocks = std::min<long>(nBig, c.tune.blockCap);
+      hipLaunchKernelGGL(k_label_block, dim3((unsigned)blocks), dim3(kBigGroup), 0, c.side,
+                         (const int *)(order ? order : bigList),
+                         (const unsigned long long *)bigCnt, bigNext, pm.status, s, ta, aux,
                          res.labelPath, res.labelTicks);
       HIP_CHECK(hipGetLastError());
       HIP_CHECK(hipEventRecord(c.ev_join, c.side));

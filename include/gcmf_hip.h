@@ -233,6 +233,9 @@ typedef struct {
   int label_steal;      /* 1: the labelling waves are a persistent launch that
                           takes the pairs from a device counter in turn
                           (work stealing); 0: one wave per pair (0) */
+  int block_cap;        /* blocks of the 1024-thread labelling launch, each
+                          taking the next listed pair from a device counter
+                          (1..65536, 128) */
 } gcmf_tuning;
 int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out);
 int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t);
