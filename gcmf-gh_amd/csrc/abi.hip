@@ -422,6 +422,7 @@ int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out) {
     out->cmf_group = ctx->c.tune.cmfGroup;
     out->label_steal = ctx->c.tune.labelSteal;
     out->block_cap = ctx->c.tune.blockCap;
+    out->count_stage = ctx->c.tune.countStage;
   });
 }
 
@@ -434,7 +435,7 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
         t->strip_min < 0 ||
         (t->cmf_group != 0 && t->cmf_group != 16 && t->cmf_group != 64) ||
         (t->label_steal != 0 && t->label_steal != 1) || t->block_cap < 1 ||
-        t->block_cap > 65536)
+        t->block_cap > 65536 || (t->count_stage != 0 && t->count_stage != 1))
       throw gcmf::Error{GCMF_ERR_ARG, "bad tuning"};
     ctx->c.tune.cullMin = t->cull_min;
     ctx->c.tune.bigNodes = t->big_nodes;
@@ -447,6 +448,7 @@ int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t) {
     ctx->c.tune.cmfGroup = t->cmf_group;
     ctx->c.tune.labelSteal = t->label_steal;
     ctx->c.tune.blockCap = t->block_cap;
+    ctx->c.tune.countStage = t->count_stage;
   });
 }
 
@@ -505,6 +507,15 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
     }
     {
       double bt = 1e300;
+      for (int v : {0, 1}) {
+        c.tune = best;
+        c.tune.countStage = v;
+        const double t = measure("gh_count");
+        if (t < bt) bt = t, best.countStage = v;
+      }
+    }
+    {
+      double bt = 1e300;
       for (long long v : {2048LL, 4096LL, 8192LL, 16384LL, 65536LL}) {
         c.tune = best;
         c.tune.bigNodes = v;
@@ -546,6 +557,7 @@ int gcmf_tune_clip(gcmf_ctx *ctx, const gcmf_dlayer *base, const gcmf_dlayer *ov
       out->cmf_group = best.cmfGroup;
       out->label_steal = best.labelSteal;
       out->block_cap = best.blockCap;
+      out->count_stage = best.countStage;
     }
   });
 }

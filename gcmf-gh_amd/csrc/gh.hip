@@ -1391,6 +1391,13 @@ __device__ __forceinline__ bool src_on_q(int t) {
 // intersect(P edge, Q edge) with the same argument order as the P view.
 // clip_pairs launches it over P's edges only (edgesQ = 0) and builds the Q
 // side by transposition (k_cpu_transpose).
+// A wave whose P edges all belong to one pair, with a Q ring of at most
+// kStageQ vertices walked in full, first copies that ring into its slice of
+// LDS (count_stage, kStageQ points per wave) and walks it there: the walk's
+// loads, which every lane shares, then come from LDS.
+constexpr int kStageQ = 512;
+extern __shared__ __align__(16) char count_stage[];
+
 template <bool CULL>
 __device__ __forceinline__ void count_edge_cpu(long g, long edgesP, long edgesQ, const int *mapP,
                                                const int *mapQ, const long long *offP,
@@ -1401,7 +1408,7 @@ __device__ __forceinline__ void count_edge_cpu(long g, long edgesP, long edgesQ,
                                                int *cnt2P, int *cnt1Q, int *cnt2Q, int *boolP,
                                                int *boolQ, int *lstP, int *lstQ,
                                                const RunBoxes &boxP, const RunBoxes &boxQ,
-                                               int ldsQ, unsigned long long &ev,
+                                               int ldsQ, int stage, unsigned long long &ev,
                                                unsigned long long &tv) {
   const bool sideP = g < edgesP;
   const long ga = sideP ? g : g - edgesP;
@@ -1447,6 +1454,9 @@ __device__ __forceinline__ void count_edge_cpu(long g, long edgesP, long edgesQ,
       const double G = f.G * (1.0 + 1e-12) + 1e-300;
       const EdgeBox wide{ab.x0 - G, ab.y0 - G, ab.x1 + G, ab.y1 + G};
       tv = culled_edges(Bv, bBase, bBase + bN, sideP ? boxQ : boxP, wide, f.la, f.T, visit);
+    } else if (sideP && stage >= 0) {
+      ring_edges((const double *)count_stage + 2 * kStageQ * stage, (int)bN, visit);
+      tv = bN;
     } else {
       ring_edges(Bv, (int)bN, visit);
       tv = bN;
@@ -1474,13 +1484,38 @@ __global__ void __launch_bounds__(256, CULL ? 1 : 5)
                 const double *ringQ, const double *lenP, const double *mrgP, const double *lenQ,
                 const double *mrgQ, const double *cf, int *cnt1P, int *cnt2P, int *cnt1Q,
                 int *cnt2Q, int *boolP, int *boolQ, unsigned long long *wstat, int *lstP,
-                int *lstQ, RunBoxes boxP, RunBoxes boxQ, int ldsQ) {
+                int *lstQ, RunBoxes boxP, RunBoxes boxQ, int ldsQ, int stageOk) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long ev = 0, tv = 0;
+  // stage the wave's Q ring when every P edge of the wave is of one pair
+  // whose ring is walked in full and fits (the same in every lane)
+  int stage = -1;
+  if (stageOk) {
+    const bool pv = g < edgesP;
+    const int k = pv ? mapP[g] : -1;
+    const unsigned long long vm = __ballot(pv);
+    if (vm) {
+      const int k0 = __shfl(k, __ffsll((long long)vm) - 1, 64);
+      if (__ballot(pv && k != k0) == 0) {
+        const long long b0 = offQ[k0], bn = offQ[k0 + 1] - b0;
+        const bool walked = !(bn > boxP.cullMin && (CULL || bn <= ldsQ));
+        if (bn > 0 && bn <= kStageQ && walked) {
+          double *st = (double *)count_stage + 2 * kStageQ * (threadIdx.x >> 6);
+          for (long j = __lane_id(); j < bn; j += 64) {
+            const pt v = ld(ringQ, b0 + j);
+            st[2 * j] = v.x;
+            st[2 * j + 1] = v.y;
+          }
+          stage = (int)(threadIdx.x >> 6);
+        }
+      }
+    }
+    __syncthreads();
+  }
   if (g < edgesP + edgesQ)
     count_edge_cpu<CULL>(g, edgesP, edgesQ, mapP, mapQ, offP, offQ, ringP, ringQ, lenP, mrgP,
                          lenQ, mrgQ, cf, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, lstP, lstQ,
-                         boxP, boxQ, ldsQ, ev, tv);
+                         boxP, boxQ, ldsQ, stage, ev, tv);
   // every lane of every wave reaches the statistics
   wave_stats(wstat, g, ev, tv);
 }
@@ -4670,13 +4705,15 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
         unsigned long long *wstat = wsT<unsigned long long>(c, "gh_wave_stats", 2 * waves);
         {
           LaunchTimer t(c, "gh_count");
+          // the Q-ring staging slices: kStageQ points per wave
+          const size_t stageBytes = c.tune.countStage ? (size_t)(cb / 64) * kStageQ * 16 : 0;
           hipLaunchKernelGGL(cull ? k_count_cpu<true> : k_count_cpu<false>, dim3(grid), dim3(cb),
-                             0, c.stream, edgesP, 0L, mapP, mapQ, (const long long *)offP,
-                             (const long long *)offQ, ringP, ringQ, (const double *)elenP,
-                             (const double *)emrgP, (const double *)elenQ,
-                             (const double *)emrgQ, (const double *)cf, cnt1P, cnt2P, cnt1Q,
-                             cnt2Q, boolP, boolQ, wstat, lstP, lstQ, boxP, boxQ,
-                             capQ > minQ ? capQ : 0);
+                             stageBytes, c.stream, edgesP, 0L, mapP, mapQ,
+                             (const long long *)offP, (const long long *)offQ, ringP, ringQ,
+                             (const double *)elenP, (const double *)emrgP,
+                             (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
+                             cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, wstat, lstP, lstQ, boxP,
+                             boxQ, capQ > minQ ? capQ : 0, c.tune.countStage);
           HIP_CHECK(hipGetLastError());
         }
         sum_wave_stats(c, wstat, waves, stats);

@@ -196,7 +196,7 @@ class Tuning(C.Structure):
     _fields_ = [("cull_min", C.c_long), ("big_nodes", C.c_longlong), ("count_block", C.c_int),
                 ("lds_max_q", C.c_int), ("label_lds", C.c_int), ("grid_nodes", C.c_longlong),
                 ("grid_clean", C.c_int), ("strip_min", C.c_longlong), ("cmf_group", C.c_int),
-                ("label_steal", C.c_int), ("block_cap", C.c_int)]
+                ("label_steal", C.c_int), ("block_cap", C.c_int), ("count_stage", C.c_int)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -434,7 +434,7 @@ class Context:
 
     def set_tuning(self, cull_min, big_nodes, count_block, lds_max_q=None, label_lds=None,
                    grid_nodes=None, grid_clean=None, strip_min=None, cmf_group=None,
-                   label_steal=None, block_cap=None):
+                   label_steal=None, block_cap=None, count_stage=None):
         cur = self.tuning()
         if lds_max_q is None:
             lds_max_q = cur["lds_max_q"]
@@ -452,8 +452,10 @@ class Context:
             label_steal = cur["label_steal"]
         if block_cap is None:
             block_cap = cur["block_cap"]
+        if count_stage is None:
+            count_stage = cur["count_stage"]
         t = Tuning(cull_min, big_nodes, count_block, lds_max_q, label_lds, grid_nodes,
-                   grid_clean, strip_min, cmf_group, label_steal, block_cap)
+                   grid_clean, strip_min, cmf_group, label_steal, block_cap, count_stage)
         self._check(hip_lib().gcmf_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def tune_clip(self, base, over, d_pairs, M, flags=CLIP_NORMALIZE):
