@@ -72,6 +72,43 @@ __device__ __forceinline__ long block_append(unsigned long long *cnt, bool pred,
   return r;
 }
 
+// block_append for four lists at once, one block scan for all four: the
+// per-wave counts travel as 16-bit fields of one word (a block holds at most
+// 1024 threads).  Returns in r[i] the slot in list i, or -1.
+__device__ __forceinline__ void block_append4(unsigned long long *const cnt[4], const bool pred[4],
+                                              unsigned long long *lds, long r[4]) {
+  const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  unsigned long long m[4], packed = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    m[i] = __ballot(pred[i]);
+    packed |= (unsigned long long)__popcll(m[i]) << (16 * i);
+  }
+  if (__lane_id() == 0) lds[wave] = packed;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < nw; w++) {
+      const unsigned long long c = lds[w];
+      lds[w] = t;
+      t += c;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const unsigned long long ti = (t >> (16 * i)) & 0xffffULL;
+      lds[16 + i] = ti ? atomicAdd(cnt[i], ti) : 0ULL;
+    }
+  }
+  __syncthreads();
+  const unsigned long long below = (1ULL << __lane_id()) - 1ULL, before = lds[wave];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    r[i] = pred[i] ? (long)(lds[16 + i] + ((before >> (16 * i)) & 0xffffULL)) +
+                         __popcll(m[i] & below)
+                   : -1;
+  __syncthreads();
+}
+
 // ------------------------------------------------------------- setup ----
 struct PairMeta {
   long *startP, *startQ;  // ring start (vertex index) inside the source layer
@@ -1030,23 +1067,25 @@ __global__ void __launch_bounds__(1024)
     k_list_long_runs(long edges, const int *map, const long long *ps2, const long long *ps1,
                      const int *status, int *listS, int *list, int *listL, int *lenL, int *listO,
                      unsigned long long *cnt) {
-  __shared__ unsigned long long lds[17];
+  __shared__ unsigned long long lds[20];
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool ok = g < edges && status[map[g]] == GH_OK;
   const long long m = ok ? ps2[g + 1] - ps2[g] - 1 : 0;
-  long i = block_append(cnt + 1, m >= 2 && m <= kSmallSort, lds);
-  if (i >= 0) listS[i] = (int)g;
-  i = block_append(cnt, m > kSmallSort && m <= kBlockSort, lds);
-  if (i >= 0) list[i] = (int)g;
-  i = block_append(cnt + 2, m > kBlockSort, lds);
-  if (i >= 0) {
-    listL[i] = (int)g;
-    lenL[i] = (int)m;
+  // short runs, block-sorted runs, long runs, and edges with more
+  // intersecting partners than the count pass recorded
+  unsigned long long *const cs[4] = {cnt + 1, cnt, cnt + 2, cnt + 4};
+  const bool pr[4] = {m >= 2 && m <= kSmallSort, m > kSmallSort && m <= kBlockSort,
+                      m > kBlockSort, ok && ps1[g + 1] - ps1[g] > kPartners};
+  long r[4];
+  block_append4(cs, pr, lds, r);
+  if (r[0] >= 0) listS[r[0]] = (int)g;
+  if (r[1] >= 0) list[r[1]] = (int)g;
+  if (r[2] >= 0) {
+    listL[r[2]] = (int)g;
+    lenL[r[2]] = (int)m;
     atomicAdd(cnt + 3, (unsigned long long)m);
   }
-  // edges with more intersecting partners than the count pass recorded
-  i = block_append(cnt + 4, ok && ps1[g + 1] - ps1[g] > kPartners, lds);
-  if (i >= 0) listO[i] = (int)g;
+  if (r[3] >= 0) listO[r[3]] = (int)g;
 }
 
 // OWN pass of the Q sort over listed runs, one block per run
