@@ -141,7 +141,11 @@ void capture(gcmf_ctx *ctx, const std::function<void(bool)> &run) {
     c.ws.no_alloc = false;
     c.timing = timing0;
   };
-  HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+  const hipError_t eb = hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed);
+  if (eb != hipSuccess) {
+    restore();
+    HIP_CHECK(eb);
+  }
   try {
     gcmf::fill_i32(c, c.graph_flag, 0, 1);
     gcmf::lb_reset(c);
