@@ -443,6 +443,12 @@ def test_clip_autotuner_keeps_results(gctx):
                 np.testing.assert_array_equal(x, y)
         with pytest.raises(G.GcmfError):
             gctx.set_tuning(512, 8192, 100)
+        for bad in ({"label_lds": 100}, {"grid_nodes": 0}, {"grid_clean": 48},
+                    {"strip_min": -1}, {"cmf_group": 32}, {"label_steal": 2},
+                    {"block_cap": 0}, {"count_stage": 2}):
+            with pytest.raises(G.GcmfError):
+                gctx.set_tuning(**dict(default, **bad))
+        assert gctx.tuning() == tuned
     finally:
         gctx.set_tuning(**default)
         d_pairs.free()
