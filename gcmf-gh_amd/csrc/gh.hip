@@ -197,12 +197,52 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The ring boxes of a few large pairs over `split` 1024-thread blocks per
+// pair: block b takes a strided part of pair b / split's rings and leaves
+// its (min x, min y, max x, max y) of P and of Q in part[8 b..8 b + 8), as
+// the pairs are oriented by k_pair_setup (a ring's vertex 0 seeds every
+// part, which leaves the minima and maxima unchanged).
+__global__ void __launch_bounds__(1024)
+    k_pair_box_parts(long M, int split, const int *pairs, const double *bCoords,
+                     const int *bVNum, const long *bVPSNum, const double *oCoords,
+                     const int *oVNum, const long *oVPSNum, int noswap, double *part) {
+  const long k = (long)(blockIdx.x / (unsigned)split);
+  const int j = (int)(blockIdx.x % (unsigned)split);
+  if (k >= M) return;
+  const int b = pairs[2 * k], o = pairs[2 * k + 1];
+  const int nb = bVNum[b] - 1, no = oVNum[o] - 1;
+  const long sb = bVPSNum[b] - bVNum[b], so = oVPSNum[o] - oVNum[o];
+  const int sw = noswap ? 0 : (nb < no);
+  for (int side = 0; side < 2; side++) {
+    const double *c = side == 0 ? (sw ? oCoords + 2 * so : bCoords + 2 * sb)
+                                : (sw ? bCoords + 2 * sb : oCoords + 2 * so);
+    const int n = side == 0 ? (sw ? no : nb) : (sw ? nb : no);
+    double mnx = c[0], mny = c[1], mxx = c[0], mxy = c[1];
+    for (long i = (long)j * 1024 + threadIdx.x; i < n; i += (long)split * 1024) {
+      const double x = c[2 * i], y = c[2 * i + 1];
+      mnx = fmin(mnx, x);
+      mny = fmin(mny, y);
+      mxx = fmax(mxx, x);
+      mxy = fmax(mxy, y);
+    }
+    pgrp_minmax<1024>(mnx, mxx);
+    pgrp_minmax<1024>(mny, mxy);
+    if (threadIdx.x == 0) {
+      double *d = part + 8 * (long)blockIdx.x + 4 * side;
+      d[0] = mnx;
+      d[1] = mny;
+      d[2] = mxx;
+      d[3] = mxy;
+    }
+  }
+}
+
 template <int W>
 __global__ void __launch_bounds__(W == 64 ? 256 : W)
     k_pair_setup(long M, const int *pairs, const double *bCoords, const int *bVNum,
                  const long *bVPSNum, const double *oCoords, const int *oVNum, const long *oVPSNum,
                  int normalize, int noswap, PairMeta pm, int *eP, int *eQ,
-                 unsigned long long *blockMax) {
+                 unsigned long long *blockMax, const double *part, int split) {
   const long k = pgrp_pair<W>();
   if (W == 64) {
     __shared__ int lmax[2][4];
@@ -256,12 +296,23 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
     const double *c = side == 0 ? cP + 2 * sP : cQ + 2 * sQ;
     const int n = side == 0 ? nP : nQ;
     double mnx = c[0], mny = c[1], mxx = c[0], mxy = c[1];
-    for (int i = lane; i < n; i += W) {
-      double x = c[2 * i], y = c[2 * i + 1];
-      mnx = fmin(mnx, x);
-      mny = fmin(mny, y);
-      mxx = fmax(mxx, x);
-      mxy = fmax(mxy, y);
+    if (part) {
+      // the parts k_pair_box_parts left (W = 1024)
+      for (int j = lane; j < split; j += W) {
+        const double *d = part + 8 * (k * split + j) + 4 * side;
+        mnx = fmin(mnx, d[0]);
+        mny = fmin(mny, d[1]);
+        mxx = fmax(mxx, d[2]);
+        mxy = fmax(mxy, d[3]);
+      }
+    } else {
+      for (int i = lane; i < n; i += W) {
+        double x = c[2 * i], y = c[2 * i + 1];
+        mnx = fmin(mnx, x);
+        mny = fmin(mny, y);
+        mxx = fmax(mxx, x);
+        mxy = fmax(mxy, y);
+      }
     }
     pgrp_minmax<W>(mnx, mxx);
     pgrp_minmax<W>(mny, mxy);
@@ -1293,6 +1344,40 @@ __global__ void __launch_bounds__(256)
 enum { CF_BOXP = 0, CF_BOXQ = 4, CF_MAXMP = 8, CF_MAXMQ = 9, CF_MINLP = 10, CF_MINLQ = 11,
        CF_TINY = 12, CF_N = 16 };
 
+// The filter's constants from the two rings' boxes (normalised frame):
+// D is the diagonal of their union, dl and eA bound the rounding of the
+// predicates, kA / kB / c0 build an edge's margin from its length.
+struct FilterK {
+  double kA, kB, c0, dl;
+};
+__device__ __forceinline__ FilterK filter_consts(const double bx[2][4]) {
+  const double w = fmax(bx[0][2], bx[1][2]) - fmin(bx[0][0], bx[1][0]);
+  const double h = fmax(bx[0][3], bx[1][3]) - fmin(bx[0][1], bx[1][1]);
+  const double D2 = w * w + h * h, D = sqrt(D2);
+  const double u = 1.1102230246251565e-16;
+  const double dl = 8.0 * u * D2, eA = 3e-8 * D2 + 1e-15;
+  return FilterK{1.5 * (kGhEps + eA), 1.5 * 2.2 * (kGhEps + 2.0 * dl), 16.0 * u * D, dl};
+}
+// an edge's length and margin (Q edges carry the 1/L term)
+__device__ __forceinline__ void edge_margin(const FilterK &K, pt p, pt q, bool sideQ, double &L,
+                                            double &m) {
+  const double dx = q.x - p.x, dy = q.y - p.y;
+  L = sqrt(dx * dx + dy * dy);
+  m = sideQ ? K.kA * L + K.kB / L + K.c0 : K.kA * L + K.c0;
+}
+__device__ __forceinline__ void write_cf(double *c, const double bx[2][4], const double minL[2],
+                                         const double maxM[2], const FilterK &K) {
+  for (int i = 0; i < 4; i++) {
+    c[CF_BOXP + i] = bx[0][i];
+    c[CF_BOXQ + i] = bx[1][i];
+  }
+  c[CF_MAXMP] = maxM[0];
+  c[CF_MAXMQ] = maxM[1];
+  c[CF_MINLP] = minL[0];
+  c[CF_MINLQ] = minL[1];
+  c[CF_TINY] = 4.5 * (kGhEps + 2.0 * K.dl);
+}
+
 // one group per pair: ring boxes, edge lengths and margins
 template <int W>
 __global__ void __launch_bounds__(W == 64 ? 256 : W)
@@ -1321,12 +1406,7 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
     bx[side][2] = x1;
     bx[side][3] = y1;
   }
-  const double w = fmax(bx[0][2], bx[1][2]) - fmin(bx[0][0], bx[1][0]);
-  const double h = fmax(bx[0][3], bx[1][3]) - fmin(bx[0][1], bx[1][1]);
-  const double D2 = w * w + h * h, D = sqrt(D2);
-  const double u = 1.1102230246251565e-16;
-  const double dl = 8.0 * u * D2, eA = 3e-8 * D2 + 1e-15;
-  const double kA = 1.5 * (kGhEps + eA), kB = 1.5 * 2.2 * (kGhEps + 2.0 * dl), c0 = 16.0 * u * D;
+  const FilterK K = filter_consts(bx);
   double minL[2], maxM[2];
   for (int side = 0; side < 2; side++) {
     const long long b = side ? offQ[k] : offP[k], n = (side ? offQ[k + 1] : offP[k + 1]) - b;
@@ -1334,10 +1414,8 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
     double *len = side ? lenQ : lenP, *mrg = side ? mrgQ : mrgP;
     double mn = INFINITY, mx = 0.0;
     for (long long i = lane; i < n; i += W) {
-      const pt p = ld(V, b + i), q = ld(V, b + (i + 1 == n ? 0 : i + 1));
-      const double dx = q.x - p.x, dy = q.y - p.y;
-      const double L = sqrt(dx * dx + dy * dy);
-      const double m = side ? kA * L + kB / L + c0 : kA * L + c0;
+      double L, m;
+      edge_margin(K, ld(V, b + i), ld(V, b + (i + 1 == n ? 0 : i + 1)), side != 0, L, m);
       len[b + i] = L;
       mrg[b + i] = m;
       mn = fmin(mn, L);
@@ -1347,18 +1425,134 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
     minL[side] = mn;
     maxM[side] = mx;
   }
-  if (lane == 0) {
-    double *c = cf + (long long)CF_N * k;
-    for (int i = 0; i < 4; i++) {
-      c[CF_BOXP + i] = bx[0][i];
-      c[CF_BOXQ + i] = bx[1][i];
+  if (lane == 0) write_cf(cf + (long long)CF_N * k, bx, minL, maxM, K);
+}
+
+// The same over `split` 1024-thread blocks per pair, for a few large pairs:
+// the ring boxes' parts (k_filter_box_parts), combined into the constants'
+// box fields (k_filter_box_combine); then each block writes its share of
+// the edges' lengths and margins with the parts of their minimum and
+// maximum (k_filter_edge_parts), which k_filter_finish combines into the
+// remaining constants.  Minima and maxima are exact in
+// any order, so the constants are those of k_cpu_filter_setup.
+__global__ void __launch_bounds__(1024)
+    k_filter_box_parts(long M, int split, const int *status, const long long *offP,
+                       const long long *offQ, const double *ringP, const double *ringQ,
+                       double *part) {
+  const long k = (long)(blockIdx.x / (unsigned)split);
+  const int j = (int)(blockIdx.x % (unsigned)split);
+  if (k >= M || status[k] != GH_OK) return;
+  for (int side = 0; side < 2; side++) {
+    const long long b = side ? offQ[k] : offP[k], n = (side ? offQ[k + 1] : offP[k + 1]) - b;
+    const double *V = side ? ringQ : ringP;
+    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    for (long long i = (long long)j * 1024 + threadIdx.x; i < n; i += (long long)split * 1024) {
+      const pt p = ld(V, b + i);
+      x0 = fmin(x0, p.x);
+      y0 = fmin(y0, p.y);
+      x1 = fmax(x1, p.x);
+      y1 = fmax(y1, p.y);
     }
-    c[CF_MAXMP] = maxM[0];
-    c[CF_MAXMQ] = maxM[1];
-    c[CF_MINLP] = minL[0];
-    c[CF_MINLQ] = minL[1];
-    c[CF_TINY] = 4.5 * (kGhEps + 2.0 * dl);
+    pgrp_minmax<1024>(x0, x1);
+    pgrp_minmax<1024>(y0, y1);
+    if (threadIdx.x == 0) {
+      double *d = part + 8 * (long)blockIdx.x + 4 * side;
+      d[0] = x0;
+      d[1] = y0;
+      d[2] = x1;
+      d[3] = y1;
+    }
   }
+}
+
+// part 1's boxes of pair blockIdx.x combined by one 256-thread block into
+// the pair's constants' box fields
+__global__ void __launch_bounds__(256)
+    k_filter_box_combine(long M, int split, const int *status, const double *part, double *cf) {
+  const long k = blockIdx.x;
+  if (k >= M || status[k] != GH_OK) return;
+  double *c = cf + (long long)CF_N * k;
+  for (int side = 0; side < 2; side++) {
+    double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+    for (int j = threadIdx.x; j < split; j += 256) {
+      const double *d = part + 8 * (k * split + j) + 4 * side;
+      x0 = fmin(x0, d[0]);
+      y0 = fmin(y0, d[1]);
+      x1 = fmax(x1, d[2]);
+      y1 = fmax(y1, d[3]);
+    }
+    pgrp_minmax<256>(x0, x1);
+    pgrp_minmax<256>(y0, y1);
+    if (threadIdx.x == 0) {
+      const int o = side ? CF_BOXQ : CF_BOXP;
+      c[o + 0] = x0;
+      c[o + 1] = y0;
+      c[o + 2] = x1;
+      c[o + 3] = y1;
+    }
+  }
+}
+
+__device__ __forceinline__ void cf_boxes(const double *c, double bx[2][4]) {
+  for (int i = 0; i < 4; i++) {
+    bx[0][i] = c[CF_BOXP + i];
+    bx[1][i] = c[CF_BOXQ + i];
+  }
+}
+
+__global__ void __launch_bounds__(1024)
+    k_filter_edge_parts(long M, int split, const int *status, const long long *offP,
+                        const long long *offQ, const double *ringP, const double *ringQ,
+                        const double *cf, double *lenP, double *mrgP, double *lenQ,
+                        double *mrgQ, double *lenPart) {
+  const long k = (long)(blockIdx.x / (unsigned)split);
+  const int j = (int)(blockIdx.x % (unsigned)split);
+  if (k >= M || status[k] != GH_OK) return;
+  double bx[2][4];
+  cf_boxes(cf + (long long)CF_N * k, bx);
+  const FilterK K = filter_consts(bx);
+  for (int side = 0; side < 2; side++) {
+    const long long b = side ? offQ[k] : offP[k], n = (side ? offQ[k + 1] : offP[k + 1]) - b;
+    const double *V = side ? ringQ : ringP;
+    double *len = side ? lenQ : lenP, *mrg = side ? mrgQ : mrgP;
+    double mn = INFINITY, mx = 0.0;
+    for (long long i = (long long)j * 1024 + threadIdx.x; i < n; i += (long long)split * 1024) {
+      double L, m;
+      edge_margin(K, ld(V, b + i), ld(V, b + (i + 1 == n ? 0 : i + 1)), side != 0, L, m);
+      len[b + i] = L;
+      mrg[b + i] = m;
+      mn = fmin(mn, L);
+      mx = fmax(mx, m);
+    }
+    pgrp_minmax<1024>(mn, mx);
+    if (threadIdx.x == 0) {
+      lenPart[4 * (long)blockIdx.x + 2 * side] = mn;
+      lenPart[4 * (long)blockIdx.x + 2 * side + 1] = mx;
+    }
+  }
+}
+
+// part 2's minima and maxima combined by one 256-thread block per pair
+__global__ void __launch_bounds__(256)
+    k_filter_finish(long M, int split, const int *status, const double *lenPart, double *cf) {
+  const long k = blockIdx.x;
+  if (k >= M || status[k] != GH_OK) return;
+  double *c = cf + (long long)CF_N * k;
+  double minL[2], maxM[2];
+  for (int side = 0; side < 2; side++) {
+    double mn = INFINITY, mx = 0.0;
+    for (int j = threadIdx.x; j < split; j += 256) {
+      mn = fmin(mn, lenPart[4 * (k * split + j) + 2 * side]);
+      mx = fmax(mx, lenPart[4 * (k * split + j) + 2 * side + 1]);
+    }
+    pgrp_minmax<256>(mn, mx);
+    minL[side] = mn;
+    maxM[side] = mx;
+  }
+  if (threadIdx.x != 0) return;
+  double bx[2][4];
+  cf_boxes(c, bx);
+  write_cf(c, bx, minL, maxM, filter_consts(bx));
 }
 
 // Chebyshev gap of two closed edge boxes (negative when they overlap);
@@ -1916,6 +2110,31 @@ __device__ __forceinline__ uint64_t dbits(double v) {
   return c.u;
 }
 
+// Edges with more than kSmallSort inserted vertices (the medium and long
+// runs' lists) are filled and linked by one wave each, 64 list entries or
+// slots a round (k_cpu_fill_runs, k_cpu_link_runs); the per-edge threads
+// leave them out.  A few such edges can hold most of a large pair's
+// intersections.
+__device__ __forceinline__ bool wave_run(long long nIns) { return nIns > kSmallSort; }
+
+// the wave-filled edges: wave w takes entry w of [medium P | long P |
+// medium Q | long Q]
+struct RunLists {
+  const int *l[4];
+  int n[4];
+  __device__ __forceinline__ bool pick(long w, bool &sideP, long &g) const {
+    for (int j = 0; j < 4; j++) {
+      if (w < n[j]) {
+        sideP = j < 2;
+        g = l[j][w];
+        return true;
+      }
+      w -= n[j];
+    }
+    return false;
+  }
+};
+
 // The inserted vertices of every active edge, in decreasing partner order
 // after the parent slot (so that a stable sort by alpha gives insertVertex's
 // order), and the source vertex's last link.  Coordinates as
@@ -1948,6 +2167,7 @@ __global__ void __launch_bounds__(256)
   const pt A1 = ld(ringA, g), A2 = ld(ringA, aBase + ia2);
   const long long *ps2 = sideP ? psP2 : psQ2, *ps1 = sideP ? psP1 : psQ1;
   const long long base = ps2[g], nIns = ps2[g + 1] - base - 1;
+  if (wave_run(nIns)) return;  // k_cpu_fill_runs
   double *i2 = sideP ? s.iP2 : s.iQ2, *o2 = sideP ? s.oP2 : s.oQ2;
   uint64_t *k2 = sideP ? cs.kP2 : cs.kQ2;
   int *r2 = sideP ? cs.rP2 : cs.rQ2;
@@ -2123,7 +2343,9 @@ __global__ void __launch_bounds__(256)
     const long long u = find_partner(rB2, ps2B[qg] + 1, ps2B[qg + 1], ia);
     return u >= 0 ? (long long)dB[u] : -1;
   };
-  const long long base = ps2A[g], hi = ps2A[g + 1];
+  const long long base = ps2A[g];
+  // the slots of wave-filled edges are linked by k_cpu_link_runs
+  const long long hi = wave_run(ps2A[g + 1] - base - 1) ? base + 1 : ps2A[g + 1];
   // kListStage slots a round: their records, then the partner edges' slot
   // ranges, are loaded before the round's searches and stores (which write
   // only nbA)
@@ -2157,6 +2379,118 @@ __global__ void __launch_bounds__(256)
     const long long nb = (t == 4 || t == 8) ? ps2B[bBase + q] : counterpart(q);
     if (nb < 0) bad = true;
     else nbA[base] = (int)(nb + 1);
+  }
+  if (bad) status[k] = GH_INVALID;
+}
+
+// k_cpu_fill for the wave-filled edges: lane l visits list entry i0 + l of
+// each round; ballots give an inserted vertex its rank among the edge's
+// inserts in list order (so its slot is the one the per-edge loop gives it)
+// and the source link its last entry in list order.
+__global__ void __launch_bounds__(256)
+    k_cpu_fill_runs(RunLists L, const int *mapP, const int *mapQ, const long long *offP,
+                    const long long *offQ, const double *ringP, const double *ringQ,
+                    const long long *psP1, const long long *psQ1, const long long *psP2,
+                    const long long *psQ2, const int *status, PairMeta pm,
+                    const double *bCoords, const double *oCoords, const int *lstP,
+                    const int *lstQ, SlotArrays s, CpuSlots cs) {
+  const long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  bool sideP;
+  long g;
+  if (!L.pick(w, sideP, g)) return;
+  const int k = sideP ? mapP[g] : mapQ[g];
+  if (status[k] != GH_OK) return;
+  const int lane = (int)__lane_id();
+  const int sw = pm.swapped[k];
+  const double *cP = sw ? oCoords : bCoords, *cQ = sw ? bCoords : oCoords;
+  const long long aBase = sideP ? offP[k] : offQ[k];
+  const long long aN = (sideP ? offP[k + 1] : offQ[k + 1]) - aBase;
+  const long long bBase = sideP ? offQ[k] : offP[k];
+  const long long bN = (sideP ? offQ[k + 1] : offP[k + 1]) - bBase;
+  const double *cA = sideP ? cP : cQ, *cB = sideP ? cQ : cP;
+  const long sA = sideP ? pm.startP[k] : pm.startQ[k], sB = sideP ? pm.startQ[k] : pm.startP[k];
+  const int ia = (int)(g - aBase);
+  const int ia2 = ia == aN - 1 ? 0 : ia + 1;
+  const double *ringA = sideP ? ringP : ringQ, *VB = (sideP ? ringQ : ringP) + 2 * bBase;
+  const pt A1 = ld(ringA, g), A2 = ld(ringA, aBase + ia2);
+  const long long *ps2 = sideP ? psP2 : psQ2, *ps1 = sideP ? psP1 : psQ1;
+  const long long base = ps2[g], nIns = ps2[g + 1] - base - 1;
+  double *i2 = sideP ? s.iP2 : s.iQ2, *o2 = sideP ? s.oP2 : s.oQ2;
+  uint64_t *k2 = sideP ? cs.kP2 : cs.kQ2;
+  int *r2 = sideP ? cs.rP2 : cs.rQ2;
+  const int *lst = (sideP ? lstP : lstQ) + ps1[g];
+  const int cN = (int)(ps1[g + 1] - ps1[g]);
+  long long r = 0;
+  int src = -1;
+  for (int i0 = 0; i0 < cN; i0 += 64) {
+    const int i = i0 + lane;
+    int q = 0, t = 0;
+    double al = 0.0, be = 0.0;
+    pt B1{0.0, 0.0}, B2{0.0, 0.0};
+    if (i < cN) {
+      q = lst[i];
+      B1 = ld(VB, q);
+      B2 = ld(VB, q + 1 == bN ? 0 : q + 1);
+      t = sideP ? itype(A1, A2, B1, B2, al, be) : itype(B1, B2, A1, A2, al, be);
+    }
+    const bool isSrc = t != 0 && (sideP ? src_on_p(t) : src_on_q(t));
+    const bool isIns = t != 0 && (sideP ? ins_on_p(t) : ins_on_q(t));
+    const unsigned long long sb = __ballot(isSrc), ib = __ballot(isIns);
+    if (sb) src = __shfl(isSrc ? ((q << 4) | t) : 0, 63 - __clzll(sb), 64);
+    if (isIns) {
+      const long long u = base + nIns - (r + __popcll(ib & ((1ULL << lane) - 1)));
+      pt v, vo;
+      if (t == 1) {
+        const int q2 = q + 1 == bN ? 0 : q + 1;
+        const pt P1 = sideP ? A1 : B1, P2 = sideP ? A2 : B2;
+        const pt P1o = sideP ? ld(cA, sA + ia) : ld(cB, sB + q);
+        const pt P2o = sideP ? ld(cA, sA + ia2) : ld(cB, sB + q2);
+        v = pt{(1.0 - al) * P1.x + al * P2.x, (1.0 - al) * P1.y + al * P2.y};
+        vo = pt{(1.0 - al) * P1o.x + al * P2o.x, (1.0 - al) * P1o.y + al * P2o.y};
+      } else {
+        v = B1;
+        vo = ld(cB, sB + q);
+      }
+      if (u > base && u <= base + nIns) {
+        st(i2, u, v);
+        st(o2, u, vo);
+        k2[u] = dbits(sideP ? al : be);
+        r2[u] = (q << 4) | t;
+      }
+    }
+    r += __popcll(ib);
+  }
+  if (lane == 0) (sideP ? cs.srcP : cs.srcQ)[g] = src;
+}
+
+// k_cpu_link's slot links for the wave-filled edges, one slot per lane
+__global__ void __launch_bounds__(256)
+    k_cpu_link_runs(RunLists L, const int *mapP, const int *mapQ, const long long *offP,
+                    const long long *offQ, const long long *psP2, const long long *psQ2,
+                    int *status, SlotArrays s, CpuSlots cs) {
+  const long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  bool sideP;
+  long g;
+  if (!L.pick(w, sideP, g)) return;
+  const int k = sideP ? mapP[g] : mapQ[g];
+  if (status[k] != GH_OK) return;
+  const long long aBase = sideP ? offP[k] : offQ[k], bBase = sideP ? offQ[k] : offP[k];
+  const int ia = (int)(g - aBase);
+  const long long *ps2A = sideP ? psP2 : psQ2, *ps2B = sideP ? psQ2 : psP2;
+  const int *rA = sideP ? cs.rP : cs.rQ, *rB2 = sideP ? cs.rQ2 : cs.rP2;
+  const int *dB = sideP ? cs.dQ : cs.dP;
+  int *nbA = sideP ? s.nbP : s.nbQ;
+  bool bad = false;
+  for (long long d = ps2A[g] + 1 + __lane_id(), hi = ps2A[g + 1]; d < hi; d += 64) {
+    const int rec = rA[d];
+    const long long qg = bBase + (rec >> 4);
+    long long nb = ps2B[qg];
+    if ((rec & 15) == 1) {
+      const long long u = find_partner(rB2, nb + 1, ps2B[qg + 1], ia);
+      nb = u >= 0 ? (long long)dB[u] : -1;
+    }
+    if (nb < 0) bad = true;
+    else nbA[d] = (int)(nb + 1);
   }
   if (bad) status[k] = GH_INVALID;
 }
@@ -4610,10 +4944,20 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     unsigned long long *blockMax = wsT<unsigned long long>(c, "gh_setup_bmax", 2 * nb);
     {
       LaunchTimer t(c, "gh_pair_setup");
+      // few large pairs: the ring boxes over several blocks per pair first
+      const int split = pgrp_split(M, pw);
+      double *part = nullptr;
+      if (split > 1) {
+        part = wsT<double>(c, "gh_setup_parts", 8 * M * (long)split);
+        hipLaunchKernelGGL(k_pair_box_parts, dim3((unsigned)(M * split)), dim3(1024), 0,
+                           c.stream, M, split, pairs, in.bCoords, in.bVNum, in.bVPSNum,
+                           in.oCoords, in.oVNum, in.oVPSNum, opt.noswap, part);
+        HIP_CHECK(hipGetLastError());
+      }
       hipLaunchKernelGGL(pw == 64 ? k_pair_setup<64> : k_pair_setup<1024>, dim3((unsigned)nb),
                          dim3(pgrp_block(pw)), 0, c.stream, M, pairs, in.bCoords, in.bVNum,
                          in.bVPSNum, in.oCoords, in.oVNum, in.oVPSNum, opt.normalize,
-                         opt.noswap, pm, eP, eQ, blockMax);
+                         opt.noswap, pm, eP, eQ, blockMax, (const double *)part, split);
       HIP_CHECK(hipGetLastError());
     }
     const long rblocks = std::min<long>(64, (nb + 255) / 256);
@@ -4653,11 +4997,33 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     elenQ = wsT<double>(c, "gh_elenQ", edgesQ);
     emrgQ = wsT<double>(c, "gh_emrgQ", edgesQ);
     cf = wsT<double>(c, "gh_cf", (long long)CF_N * M);
-    hipLaunchKernelGGL(pw == 64 ? k_cpu_filter_setup<64> : k_cpu_filter_setup<1024>, dim3(pgrp_grid(M, pw)),
-                       dim3(pgrp_block(pw)), 0, c.stream, M,
-                       (const int *)pm.status, (const long long *)offP, (const long long *)offQ,
-                       (const double *)ringP, (const double *)ringQ, elenP, emrgP, elenQ, emrgQ,
-                       cf);
+    const int split = pgrp_split(M, pw);
+    if (split > 1) {
+      // few large pairs: over several blocks per pair
+      double *boxPart = wsT<double>(c, "gh_filter_box_parts", 8 * M * (long)split);
+      double *lenPart = wsT<double>(c, "gh_filter_len_parts", 4 * M * (long)split);
+      hipLaunchKernelGGL(k_filter_box_parts, dim3((unsigned)(M * split)), dim3(1024), 0,
+                         c.stream, M, split, (const int *)pm.status, (const long long *)offP,
+                         (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
+                         boxPart);
+      HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(k_filter_box_combine, dim3((unsigned)M), dim3(256), 0, c.stream, M,
+                         split, (const int *)pm.status, (const double *)boxPart, cf);
+      HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(k_filter_edge_parts, dim3((unsigned)(M * split)), dim3(1024), 0,
+                         c.stream, M, split, (const int *)pm.status, (const long long *)offP,
+                         (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
+                         (const double *)cf, elenP, emrgP, elenQ, emrgQ, lenPart);
+      HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(k_filter_finish, dim3((unsigned)M), dim3(256), 0, c.stream, M, split,
+                         (const int *)pm.status, (const double *)lenPart, cf);
+    } else {
+      hipLaunchKernelGGL(pw == 64 ? k_cpu_filter_setup<64> : k_cpu_filter_setup<1024>,
+                         dim3(pgrp_grid(M, pw)), dim3(pgrp_block(pw)), 0, c.stream, M,
+                         (const int *)pm.status, (const long long *)offP,
+                         (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
+                         elenP, emrgP, elenQ, emrgQ, cf);
+    }
     HIP_CHECK(hipGetLastError());
   }
   int *cnt1P = wsT<int>(c, "gh_cnt1P", edgesP), *cnt2P = wsT<int>(c, "gh_cnt2P", edgesP);
@@ -5004,6 +5370,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          pm.status, pm, in.bCoords, in.oCoords, s);
     HIP_CHECK(hipGetLastError());
     const long nAct = nActP + nActQ;
+    RunLists runLists{{listMP, listP, listMQ, listQ}, {nMedP, nRunsP, nMedQ, nRunsQ}};
+    const long nWaveRuns = (long)nMedP + nRunsP + nMedQ + nRunsQ;
     if (nAct > 0) {
       LaunchTimer t(c, "gh_cpu_fill");
       hipLaunchKernelGGL(k_cpu_fill, dim3(grid_for(nAct, 256)), dim3(256), 0, c.stream, nActP,
@@ -5013,6 +5381,14 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          (const long long *)psP2, (const long long *)psQ2, pm.status, pm,
                          in.bCoords, in.oCoords, (const int *)lstAllP, (const int *)lstAllQ, s,
                          cs);
+      HIP_CHECK(hipGetLastError());
+      if (nWaveRuns > 0)
+        hipLaunchKernelGGL(k_cpu_fill_runs, dim3(grid_for(64 * nWaveRuns, 256)), dim3(256), 0,
+                           c.stream, runLists, mapP, mapQ, (const long long *)offP,
+                           (const long long *)offQ, ringP, ringQ, (const long long *)psP1,
+                           (const long long *)psQ1, (const long long *)psP2,
+                           (const long long *)psQ2, pm.status, pm, in.bCoords, in.oCoords,
+                           (const int *)lstAllP, (const int *)lstAllQ, s, cs);
       HIP_CHECK(hipGetLastError());
     }
     // per-edge sorts by exact alpha (beta), stable in slot order
@@ -5067,6 +5443,12 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          nActQ, (const int *)actP, (const int *)actQ, mapP, mapQ,
                          (const long long *)offP, (const long long *)offQ,
                          (const long long *)psP2, (const long long *)psQ2, pm.status, s, cs);
+      HIP_CHECK(hipGetLastError());
+      if (nWaveRuns > 0)
+        hipLaunchKernelGGL(k_cpu_link_runs, dim3(grid_for(64 * nWaveRuns, 256)), dim3(256), 0,
+                           c.stream, runLists, mapP, mapQ, (const long long *)offP,
+                           (const long long *)offQ, (const long long *)psP2,
+                           (const long long *)psQ2, pm.status, s, cs);
       HIP_CHECK(hipGetLastError());
     }
   } else {
