@@ -134,6 +134,14 @@ struct LaunchTimer {
 // ---------------------------------------------------------------- prims ----
 // Exclusive scan of n int32 values into out[0..n] (out[n] = total).
 void scan_i32_to_i64(Context &c, const int *in, long long *out, long n);
+// the same over up to four arrays, in one launch where they suit the
+// single-pass scan
+struct ScanJob {
+  const int *in;
+  long long *out;
+  long n;
+};
+void scan_i32_to_i64_multi(Context &c, std::initializer_list<ScanJob> jobs);
 void scan_i32_to_i32(Context &c, const int *in, int *out, long n);
 // out[k] = i for the k-th i with flag[i] != 0; returns the count (synchronises).
 long compact_flags(Context &c, const int *flag, long n, int *out_idx, const char *tag);

@@ -1109,34 +1109,105 @@ __device__ __forceinline__ void move_q(SlotArrays &s, const int *ownP, long long
 // from a copy of their keys in LDS; longer ones go through the radix sort.
 constexpr int kBlockSort = 2048;
 
-// edges by run length m: the short ones (2 <= m <= kSmallSort) in listS
-// with cnt[1] their number, the medium ones (up to kBlockSort) in list with
-// cnt[0] their number, the longer ones in listL / lenL with cnt[2] their
-// number and cnt[3] their total length; cnt[4] counts the edges in listO,
-// those with more than kPartners intersecting partners
+// One side's edges for k_list_long_runs: the short, block-sorted and long
+// runs, and the edges with more intersecting partners than the count pass
+// recorded, appended to listS / list / listL (+ lenL) / listO with the
+// counts at cnt[1], cnt[0], cnt[2] (+ the long runs' total at cnt[3]),
+// cnt[4].
+struct RunListSide {
+  long edges;
+  const int *map;
+  const long long *ps2, *ps1;
+  int *listS, *list, *listL, *lenL, *listO;
+  unsigned long long *cnt;
+};
+constexpr int kRunListEdges = 8;  // edges per thread of k_list_long_runs
+
+// Both sides in one launch (blocks [0, blocksA) take side A), each block
+// kRunListEdges x 1024 edges: one block-wide scan of the four counts packed
+// in 16-bit fields and at most four atomics per block (one per non-empty
+// list), so that the atomics on the shared counters stay few.  The lists'
+// order is not used: every listed edge is worked on its own.
 __global__ void __launch_bounds__(1024)
-    k_list_long_runs(long edges, const int *map, const long long *ps2, const long long *ps1,
-                     const int *status, int *listS, int *list, int *listL, int *lenL, int *listO,
-                     unsigned long long *cnt) {
+    k_list_long_runs(RunListSide A, RunListSide B, long blocksA, const int *status) {
   __shared__ unsigned long long lds[20];
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ok = g < edges && status[map[g]] == GH_OK;
-  const long long m = ok ? ps2[g + 1] - ps2[g] - 1 : 0;
-  // short runs, block-sorted runs, long runs, and edges with more
-  // intersecting partners than the count pass recorded
-  unsigned long long *const cs[4] = {cnt + 1, cnt, cnt + 2, cnt + 4};
-  const bool pr[4] = {m >= 2 && m <= kSmallSort, m > kSmallSort && m <= kBlockSort,
-                      m > kBlockSort, ok && ps1[g + 1] - ps1[g] > kPartners};
-  long r[4];
-  block_append4(cs, pr, lds, r);
-  if (r[0] >= 0) listS[r[0]] = (int)g;
-  if (r[1] >= 0) list[r[1]] = (int)g;
-  if (r[2] >= 0) {
-    listL[r[2]] = (int)g;
-    lenL[r[2]] = (int)m;
-    atomicAdd(cnt + 3, (unsigned long long)m);
+  const bool sa = (long)blockIdx.x < blocksA;
+  const RunListSide &S = sa ? A : B;
+  const long blk = sa ? (long)blockIdx.x : (long)blockIdx.x - blocksA;
+  const long g0 = blk * (1024L * kRunListEdges) + threadIdx.x;
+  unsigned bits = 0;
+  unsigned long long packed = 0;
+#pragma unroll
+  for (int r = 0; r < kRunListEdges; r++) {
+    const long g = g0 + 1024L * r;
+    const bool ok = g < S.edges && status[S.map[g]] == GH_OK;
+    const long long m = ok ? S.ps2[g + 1] - S.ps2[g] - 1 : 0;
+    const bool pr[4] = {m >= 2 && m <= kSmallSort, m > kSmallSort && m <= kBlockSort,
+                        m > kBlockSort, ok && S.ps1[g + 1] - S.ps1[g] > kPartners};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      bits |= (unsigned)pr[i] << (4 * r + i);
+      packed += (unsigned long long)pr[i] << (16 * i);
+    }
   }
-  if (r[3] >= 0) listO[r[3]] = (int)g;
+  // block-wide exclusive scan of the packed counts (a block's total per
+  // field is at most 8192)
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  unsigned long long inc = packed;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned long long v = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += v;
+  }
+  if (lane == 63) lds[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < 16; w++) {
+      const unsigned long long c = lds[w];
+      lds[w] = t;
+      t += c;
+    }
+    unsigned long long *const cs[4] = {S.cnt + 1, S.cnt, S.cnt + 2, S.cnt + 4};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const unsigned long long ti = (t >> (16 * i)) & 0xffffULL;
+      lds[16 + i] = ti ? atomicAdd(cs[i], ti) : 0ULL;
+    }
+  }
+  __syncthreads();
+  unsigned long long run = lds[wave] + inc - packed;
+  if (!bits) return;
+  // edges by run length m (inserted vertices): short runs have 2 <= m <=
+  // kSmallSort, block-sorted ones up to kBlockSort, long ones more
+  int *const out[4] = {S.listS, S.list, S.listL, S.listO};
+#pragma unroll
+  for (int r = 0; r < kRunListEdges; r++) {
+    const long g = g0 + 1024L * r;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (!((bits >> (4 * r + i)) & 1u)) continue;
+      const long long pos = (long long)lds[16 + i] + (long long)((run >> (16 * i)) & 0xffffULL);
+      run += 1ULL << (16 * i);
+      out[i][pos] = (int)g;
+      if (i == 2) {
+        const long long m = S.ps2[g + 1] - S.ps2[g] - 1;
+        S.lenL[pos] = (int)m;
+        atomicAdd(S.cnt + 3, (unsigned long long)m);
+      }
+    }
+  }
+}
+
+// k_list_long_runs over both sides (a side with no edges takes no block)
+static void list_runs(Context &c, const RunListSide &A, const RunListSide &B,
+                      const int *status) {
+  const long per = 1024L * kRunListEdges;
+  const long bA = (A.edges + per - 1) / per, bB = (B.edges + per - 1) / per;
+  if (bA + bB == 0) return;
+  hipLaunchKernelGGL(k_list_long_runs, dim3((unsigned)(bA + bB)), dim3(1024), 0, c.stream, A, B,
+                     bA, status);
+  HIP_CHECK(hipGetLastError());
 }
 
 // OWN pass of the Q sort over listed runs, one block per run
@@ -4965,8 +5036,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                        (const unsigned long long *)blockMax, nb, maxRing);
     HIP_CHECK(hipGetLastError());
   }
-  scan_i32_to_i64(c, eP, offP, M);
-  scan_i32_to_i64(c, eQ, offQ, M);
+  scan_i32_to_i64_multi(c, {{eP, offP, M}, {eQ, offQ, M}});
   long long rb[kHostSlots];
   read_scalars(c, {offP + M, offQ + M, (const long long *)maxRing,
                    (const long long *)maxRing + 1}, rb);
@@ -5124,14 +5194,14 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
         sum_wave_stats(c, wstat, waves, stats);
       }
     }
-    scan_i32_to_i64(c, cnt1P, psP1, edgesP);
-    scan_i32_to_i64(c, cnt2P, psP2, edgesP);
+    scan_i32_to_i64_multi(c, {{cnt1P, psP1, edgesP}, {cnt2P, psP2, edgesP}});
     fill_i32(c, (int *)runCnt, 0, 32);  // 8 counters per side
     if (edgesP > 0)
-      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesP, 1024)), dim3(1024), 0,
-                         c.stream, edgesP, mapP, (const long long *)psP2,
-                         (const long long *)psP1, pm.status, listSP, listMP, listP, lenP, listOP,
-                         runCnt);
+      list_runs(c, RunListSide{edgesP, mapP, (const long long *)psP2, (const long long *)psP1,
+                               listSP, listMP, listP, lenP, listOP, runCnt},
+                RunListSide{0, mapQ, nullptr, nullptr, listSQ, listMQ, listQ, lenQ, listOQ,
+                            runCnt + 8},
+                pm.status);
     HIP_CHECK(hipGetLastError());
     const long long *nActPd = compact_flags_dev(c, cnt1P, edgesP, actP, "actP");
     const long long *rc = (const long long *)runCnt;
@@ -5191,17 +5261,17 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       for (long x = edgesQ; x > 0; x >>= 1) bits++;
       radix_sort_u64(c, tk, tv, tk2, lstAllQ, H, bits);
     }
-    scan_i32_to_i64(c, cnt1Q, psQ1, edgesQ);
-    scan_i32_to_i64(c, cnt2Q, psQ2, edgesQ);
+    scan_i32_to_i64_multi(c, {{cnt1Q, psQ1, edgesQ}, {cnt2Q, psQ2, edgesQ}});
     hipLaunchKernelGGL(k_pair_check, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
                        (const long long *)offP, (const long long *)offQ,
                        (const long long *)psP1, (const long long *)psQ1,
                        (const long long *)psP2, (const long long *)psQ2, pm.status, stats + 1);
     if (edgesQ > 0)
-      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesQ, 1024)), dim3(1024), 0,
-                         c.stream, edgesQ, mapQ, (const long long *)psQ2,
-                         (const long long *)psQ1, pm.status, listSQ, listMQ, listQ, lenQ, listOQ,
-                         runCnt + 8);
+      list_runs(c, RunListSide{0, mapP, nullptr, nullptr, listSP, listMP, listP, lenP, listOP,
+                               runCnt},
+                RunListSide{edgesQ, mapQ, (const long long *)psQ2, (const long long *)psQ1,
+                            listSQ, listMQ, listQ, lenQ, listOQ, runCnt + 8},
+                pm.status);
     HIP_CHECK(hipGetLastError());
     const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
     fill_i32(c, (int *)bigCnt, 0, 4);
@@ -5239,10 +5309,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       }
       sum_wave_stats(c, wstat, waves, stats);
     }
-    scan_i32_to_i64(c, cnt1P, psP1, edgesP);
-    scan_i32_to_i64(c, cnt2P, psP2, edgesP);
-    scan_i32_to_i64(c, cnt1Q, psQ1, edgesQ);
-    scan_i32_to_i64(c, cnt2Q, psQ2, edgesQ);
+    scan_i32_to_i64_multi(c, {{cnt1P, psP1, edgesP},
+                              {cnt2P, psP2, edgesP},
+                              {cnt1Q, psQ1, edgesQ},
+                              {cnt2Q, psQ2, edgesQ}});
     hipLaunchKernelGGL(k_pair_check, dim3(grid_for(M, 256)), dim3(256), 0, c.stream, M,
                        (const long long *)offP, (const long long *)offQ, (const long long *)psP1,
                        (const long long *)psQ1, (const long long *)psP2, (const long long *)psQ2,
@@ -5251,14 +5321,11 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     // edges whose alpha runs go through the segmented sort (statuses are final
     // here), read back with the slot totals
     fill_i32(c, (int *)runCnt, 0, 32);  // 8 counters per side
-    if (edgesP > 0)
-      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesP, 1024)), dim3(1024), 0, c.stream,
-                         edgesP, mapP, (const long long *)psP2, (const long long *)psP1, pm.status,
-                         listSP, listMP, listP, lenP, listOP, runCnt);
-    if (edgesQ > 0)
-      hipLaunchKernelGGL(k_list_long_runs, dim3(grid_for(edgesQ, 1024)), dim3(1024), 0, c.stream,
-                         edgesQ, mapQ, (const long long *)psQ2, (const long long *)psQ1, pm.status,
-                         listSQ, listMQ, listQ, lenQ, listOQ, runCnt + 8);
+    list_runs(c, RunListSide{edgesP, mapP, (const long long *)psP2, (const long long *)psP1,
+                             listSP, listMP, listP, lenP, listOP, runCnt},
+              RunListSide{edgesQ, mapQ, (const long long *)psQ2, (const long long *)psQ1,
+                          listSQ, listMQ, listQ, lenQ, listOQ, runCnt + 8},
+              pm.status);
     HIP_CHECK(hipGetLastError());
     // edges with intersections (count1 > 0): the later passes run over these
     const long long *nActPd = compact_flags_dev(c, cnt1P, edgesP, actP, "actP");
@@ -5612,8 +5679,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                      (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
                      (const long long *)psQ2, pm.status, nodeCap, compCap);
   HIP_CHECK(hipGetLastError());
-  scan_i32_to_i64(c, nodeCap, nodeOff, M);
-  scan_i32_to_i64(c, compCap, compCapOff, M);
+  scan_i32_to_i64_multi(c, {{nodeCap, nodeOff, M}, {compCap, compCapOff, M}});
   // bounds of the capacity sums (k_capacity gives an OK pair 2 (cP + cQ) + 4
   // nodes and cP + cQ + 6 components), so no read-back is needed
   const long nodes = 2 * (slotsP + slotsQ) + 4 * M;
@@ -5746,8 +5812,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   }
   res.vertOff = wsT<long long>(c, "gh_vOff", M + 1);
   res.compOff = wsT<long long>(c, "gh_cOff", M + 1);
-  scan_i32_to_i64(c, ta.outNVerts, res.vertOff, M);
-  scan_i32_to_i64(c, ta.outNComp, res.compOff, M);
+  scan_i32_to_i64_multi(c, {{ta.outNVerts, res.vertOff, M}, {ta.outNComp, res.compOff, M}});
   // a pair's result fits its node / component region, so the region totals
   // bound the outputs
   res.xy = wsT<double>(c, "gh_out_xy", 2 * nodes);

@@ -919,8 +919,7 @@ void join_refine(Context &c, const JoinInputs &in, long K, const int *pairs, Joi
                        (const long long *)nullptr, (int *)nullptr, (int *)nullptr, false, cmfN);
     HIP_CHECK(hipGetLastError());
   }
-  scan_i32_to_i64(c, r.bCount, r.bOff, K);
-  scan_i32_to_i64(c, r.oCount, r.oOff, K);
+  scan_i32_to_i64_multi(c, {{r.bCount, r.bOff, K}, {r.oCount, r.oOff, K}});
   long long rb[kHostSlots];
   read_scalars(c, {r.bOff + K, r.oOff + K}, rb);
   r.nbEdges = (long)rb[0];

@@ -445,13 +445,15 @@ __global__ void __launch_bounds__(256)
 // before it have published theirs, so the chain of look-backs, not HBM,
 // bounds this path on long arrays: measured on MI355X, 0.67 ms for 2^27
 // elements against 0.44 ms for the reduce-then-scan path.  It is used up
-// to 2^20 elements (256 tiles), where it replaces that path's four
-// launches by two.
+// to 2^22 elements (1024 tiles), where it replaces that path's four or more
+// launches by one.  One launch may scan several arrays (segments, up to
+// kLbSegs): each segment's first tile publishes an inclusive prefix, so no
+// look-back crosses into the segment before it.
 constexpr unsigned long long kLbAgg = 1, kLbIncl = 2;
-constexpr long kLbMaxN = 1L << 20;
+constexpr long kLbMaxN = 1L << 22;
 constexpr int kLbSpinLimit = 1 << 22;
-// descriptors per half: the tiles of kLbMaxN elements and the tile counter
-constexpr long kLbSlots = 512;
+// descriptors per half: the tiles of one launch and the tile counter
+constexpr long kLbSlots = 4096;
 static_assert(kLbMaxN / 4096 + 1 <= kLbSlots, "descriptor half too small");
 
 // every launch clears the other half, which the next launch uses
@@ -555,9 +557,9 @@ __device__ __forceinline__ void ld_tile4(const int *in, long n, long long base, 
 // tiles before it (every thread).
 __device__ __forceinline__ long long lb_tile_prefix(unsigned long long *desc, long t,
                                                     long long tot, long long *err,
-                                                    long long *s_pre) {
-  if (threadIdx.x == 0) lb_publish(desc + t, tot, t == 0 ? kLbIncl : kLbAgg);
-  if (t > 0) {
+                                                    long long *s_pre, bool first) {
+  if (threadIdx.x == 0) lb_publish(desc + t, tot, first ? kLbIncl : kLbAgg);
+  if (!first) {
     if (threadIdx.x < 64) {
       const long long pre = lb_lookback(desc, t, err);
       if (threadIdx.x == 0) {
@@ -576,10 +578,22 @@ __global__ void k_lb_clear_all(unsigned long long *desc) {
   for (long j = threadIdx.x; j < 2 * kLbSlots; j += blockDim.x) desc[j] = 0;
 }
 
+// up to kLbSegs arrays scanned by one launch: segment j takes the tiles
+// [t0[j], t0[j + 1]) (at least one, so that out[0] = 0 of an empty array
+// is written too)
+constexpr int kLbSegs = 4;
+struct LbSegs {
+  const int *in[kLbSegs];
+  long long *out[kLbSegs];
+  long n[kLbSegs];
+  long t0[kLbSegs + 1];
+  int k;
+};
+
 // desc[tiles] holds the tile counter
 __global__ void __launch_bounds__(256)
-    k_lb_scan(const int *in, long n, long long *out, unsigned long long *desc,
-              unsigned long long *next, long tiles, long long *err) {
+    k_lb_scan(LbSegs S, unsigned long long *desc, unsigned long long *next, long tiles,
+              long long *err) {
   __shared__ long long lds[4][4];
   __shared__ long long s_pre;
   __shared__ long s_tile;
@@ -587,7 +601,12 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) s_tile = (long)atomicAdd(desc + tiles, 1ULL);
   __syncthreads();
   const long t = s_tile;
-  const long long base = (long long)t * kVecTile;
+  int j = 0;
+  while (j + 1 < S.k && t >= S.t0[j + 1]) j++;
+  const int *in = S.in[j];
+  long long *out = S.out[j];
+  const long n = S.n[j], lt = t - S.t0[j];
+  const long long base = (long long)lt * kVecTile;
   const bool full = base + kVecTile <= n;
   int4 v[4];
   ld_tile4(in, n, base, full, v);
@@ -595,7 +614,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int k = 0; k < 4; k++) s[k] = (long long)v[k].x + v[k].y + v[k].z + v[k].w;
   block_excl_scan4(s, e, tot, lds);
-  const long long pre = lb_tile_prefix(desc, t, tot, err, &s_pre);
+  const long long pre = lb_tile_prefix(desc, t, tot, err, &s_pre, lt == 0);
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const long long o0 = pre + e[k], o1 = o0 + v[k].x, o2 = o1 + v[k].y, o3 = o2 + v[k].z;
@@ -611,7 +630,7 @@ __global__ void __launch_bounds__(256)
       if (i + 3 < n) out[i + 3] = o3;
     }
   }
-  if (t == tiles - 1 && threadIdx.x == 0) out[n] = pre + tot;
+  if (t == S.t0[j + 1] - 1 && threadIdx.x == 0) out[n] = pre + tot;
 }
 
 __global__ void __launch_bounds__(256)
@@ -633,7 +652,7 @@ __global__ void __launch_bounds__(256)
   for (int k = 0; k < 4; k++)
     s[k] = (v[k].x != 0) + (v[k].y != 0) + (v[k].z != 0) + (v[k].w != 0);
   block_excl_scan4(s, e, tot, lds);
-  const long long pre = lb_tile_prefix(desc, t, tot, err, &s_pre);
+  const long long pre = lb_tile_prefix(desc, t, tot, err, &s_pre, t == 0);
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const long long i = base + k * 1024 + 4 * threadIdx.x;
@@ -683,18 +702,37 @@ static bool lb_fits(const void *a, const void *b, long n) {
   return n > 0 && n <= kLbMaxN && !((uintptr_t)a & 15) && !((uintptr_t)b & 15);
 }
 
-static bool lb_scan(Context &c, const int *in, long long *out, long n) {
-  if (!lb_fits(in, out, n)) return false;
-  const long tiles = (n + kVecTile - 1) / kVecTile;
+// one look-back launch over the given arrays, when every one suits it and
+// their tiles fit one descriptor half
+static bool lb_scan_segs(Context &c, std::initializer_list<ScanJob> jobs) {
+  LbSegs S{};
+  long tiles = 0;
+  if (jobs.size() == 0 || jobs.size() > (size_t)kLbSegs) return false;
+  for (const ScanJob &j : jobs) {
+    if (j.n < 0 || j.n > kLbMaxN || ((uintptr_t)j.in & 15) || ((uintptr_t)j.out & 15))
+      return false;
+    S.in[S.k] = j.in;
+    S.out[S.k] = j.out;
+    S.n[S.k] = j.n;
+    S.t0[S.k] = tiles;
+    tiles += j.n > 0 ? (j.n + kVecTile - 1) / kVecTile : 1;
+    S.k++;
+  }
+  S.t0[S.k] = tiles;
+  if (tiles + 1 > kLbSlots) return false;
   long long *err = lb_err_word(c);
   unsigned long long *next;
   unsigned long long *desc = lb_halves(c, &next);
   LaunchTimer t(c, "scan_lookback");
-  hipLaunchKernelGGL(k_lb_scan, dim3((unsigned)tiles), dim3(256), 0, c.stream, in, n, out, desc,
-                     next, tiles, err);
+  hipLaunchKernelGGL(k_lb_scan, dim3((unsigned)tiles), dim3(256), 0, c.stream, S, desc, next,
+                     tiles, err);
   HIP_CHECK(hipGetLastError());
   lb_flip(c);
   return true;
+}
+
+static bool lb_scan(Context &c, const int *in, long long *out, long n) {
+  return n > 0 && lb_scan_segs(c, {ScanJob{in, out, n}});
 }
 
 // returns the device address of the count, or nullptr when the arrays do
@@ -803,6 +841,11 @@ void scan_i32_to_i64(Context &c, const int *in, long long *out, long n) {
   if (lb_scan(c, in, out, n)) return;
   if (vscan(c, in, out, n)) return;
   scan_impl<int, long long>(c, in, out, n, 0);
+}
+
+void scan_i32_to_i64_multi(Context &c, std::initializer_list<ScanJob> jobs) {
+  if (lb_scan_segs(c, jobs)) return;
+  for (const ScanJob &j : jobs) scan_i32_to_i64(c, j.in, j.out, j.n);
 }
 
 void scan_i32_to_i32(Context &c, const int *in, int *out, long n) {
