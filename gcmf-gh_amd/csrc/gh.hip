@@ -2313,29 +2313,47 @@ __global__ void __launch_bounds__(256)
   if (l < m) cpu_move<QSIDE>(s, cs, lo + l, lo + d);
 }
 
-// runs of kSmallSort < m <= kBlockSort slots: one block per run from LDS
+// runs of kSmallSort < m <= kBlockSort slots: one block per run, a bitonic
+// sort in LDS of (key, slot) pairs padded to a power of two (pads take the
+// largest key and slots past the run, so they sort last); the pairs are
+// distinct, so the order is the stable one by key
 template <bool QSIDE>
 __global__ void __launch_bounds__(256)
     k_cpu_sort_block(int nRuns, const int *list, const long long *ps2, SlotArrays s,
                      CpuSlots cs) {
   __shared__ uint64_t key[kBlockSort];
+  __shared__ int slot[kBlockSort];
   const uint64_t *k2 = QSIDE ? cs.kQ2 : cs.kP2;
   for (int r = blockIdx.x; r < nRuns; r += gridDim.x) {
     const long g = list[r];
     const long long lo = ps2[g] + 1;
     const int m = (int)(ps2[g + 1] - lo);
+    int P = 64;
+    while (P < m) P <<= 1;
     __syncthreads();
-    for (int t = threadIdx.x; t < m; t += blockDim.x) key[t] = k2[lo + t];
-    __syncthreads();
-    for (int t = threadIdx.x; t < m; t += blockDim.x) {
-      const uint64_t kt = key[t];
-      int d = 0;
-      for (int i = 0; i < m; i++) {
-        const uint64_t ki = key[i];
-        d += (ki < kt) || (ki == kt && i < t);
-      }
-      cpu_move<QSIDE>(s, cs, lo + t, lo + d);
+    for (int t = threadIdx.x; t < P; t += blockDim.x) {
+      key[t] = t < m ? k2[lo + t] : ~0ULL;
+      slot[t] = t;
     }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int t = threadIdx.x; t < P / 2; t += blockDim.x) {
+          const int a = 2 * t - (t & (stride - 1)), b = a + stride;
+          const uint64_t ka = key[a], kb = key[b];
+          const int sa = slot[a], sb = slot[b];
+          const bool gt = ka > kb || (ka == kb && sa > sb);
+          if (gt == ((a & size) == 0)) {
+            key[a] = kb;
+            key[b] = ka;
+            slot[a] = sb;
+            slot[b] = sa;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int d = threadIdx.x; d < m; d += blockDim.x) cpu_move<QSIDE>(s, cs, lo + slot[d], lo + d);
   }
 }
 
@@ -5008,8 +5026,12 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   int *eP = wsT<int>(c, "gh_eP", M), *eQ = wsT<int>(c, "gh_eQ", M);
   long long *offP = wsT<long long>(c, "gh_offP", M + 1);
   long long *offQ = wsT<long long>(c, "gh_offQ", M + 1);
-  unsigned long long *maxRing = wsT<unsigned long long>(c, "gh_maxRing", 2);
-  fill_i32(c, (int *)maxRing, 0, 4);
+  // the call's counters, cleared by one fill: [0, 2) largest rings, [2, 5)
+  // count statistics, [8, 24) run lists, [24, 26) large pairs, [26] and
+  // [27] labelling work counters, [28, 32) status counts
+  unsigned long long *ctr = wsT<unsigned long long>(c, "gh_counters", 32);
+  fill_i32(c, (int *)ctr, 0, 64);
+  unsigned long long *maxRing = ctr;
   {
     const long nb = pgrp_grid(M, pw);
     unsigned long long *blockMax = wsT<unsigned long long>(c, "gh_setup_bmax", 2 * nb);
@@ -5102,8 +5124,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   int *lstP = wsT<int>(c, "gh_lstP", (long)kPartners * edgesP);
   int *lstQ = wsT<int>(c, "gh_lstQ", (long)kPartners * edgesQ);
   // [0] edge pairs classified, [1] intersection records, [2] edge pairs tested
-  unsigned long long *stats = wsT<unsigned long long>(c, "gh_stats", 3);
-  fill_i32(c, (int *)stats, 0, 6);
+  unsigned long long *stats = ctr + 2;
   // run boxes of a side, when some ring of it is long enough to be culled
   auto run_boxes = [&](bool qside, long edges, const int *map, const long long *off,
                        const double *ring) {
@@ -5140,7 +5161,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   long long *psP2 = wsT<long long>(c, "gh_psP2", edgesP + 1);
   long long *psQ1 = wsT<long long>(c, "gh_psQ1", edgesQ + 1);
   long long *psQ2 = wsT<long long>(c, "gh_psQ2", edgesQ + 1);
-  unsigned long long *runCnt = wsT<unsigned long long>(c, "gh_runCnt", 16);
+  unsigned long long *runCnt = ctr + 8;  // 8 counters per side
   int *listMP = wsT<int>(c, "gh_runListMP", edgesP), *listMQ = wsT<int>(c, "gh_runListMQ", edgesQ);
   int *listSP = wsT<int>(c, "gh_runListSP", edgesP), *listSQ = wsT<int>(c, "gh_runListSQ", edgesQ);
   int *listOP = wsT<int>(c, "gh_runListOP", edgesP), *listOQ = wsT<int>(c, "gh_runListOQ", edgesQ);
@@ -5148,7 +5169,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   int *listQ = wsT<int>(c, "gh_runListQ", edgesQ), *lenQ = wsT<int>(c, "gh_runLenQ", edgesQ);
   int *actP = wsT<int>(c, "gh_actP", edgesP), *actQ = wsT<int>(c, "gh_actQ", edgesQ);
   int *bigList = wsT<int>(c, "gh_bigList", M);
-  unsigned long long *bigCnt = wsT<unsigned long long>(c, "gh_bigCnt", 2);
+  unsigned long long *bigCnt = ctr + 24;
   int *gridList = wsT<int>(c, "gh_gridList", M);
   int *lstAllP = nullptr, *lstAllQ = nullptr;
   if (cpuRoute) {
@@ -5195,7 +5216,6 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       }
     }
     scan_i32_to_i64_multi(c, {{cnt1P, psP1, edgesP}, {cnt2P, psP2, edgesP}});
-    fill_i32(c, (int *)runCnt, 0, 32);  // 8 counters per side
     if (edgesP > 0)
       list_runs(c, RunListSide{edgesP, mapP, (const long long *)psP2, (const long long *)psP1,
                                listSP, listMP, listP, lenP, listOP, runCnt},
@@ -5274,7 +5294,6 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                 pm.status);
     HIP_CHECK(hipGetLastError());
     const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
-    fill_i32(c, (int *)bigCnt, 0, 4);
     hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M,
                        (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
                        (const long long *)psQ2, (long long)c.tune.bigNodes,
@@ -5320,7 +5339,6 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     HIP_CHECK(hipGetLastError());
     // edges whose alpha runs go through the segmented sort (statuses are final
     // here), read back with the slot totals
-    fill_i32(c, (int *)runCnt, 0, 32);  // 8 counters per side
     list_runs(c, RunListSide{edgesP, mapP, (const long long *)psP2, (const long long *)psP1,
                              listSP, listMP, listP, lenP, listOP, runCnt},
               RunListSide{edgesQ, mapQ, (const long long *)psQ2, (const long long *)psQ1,
@@ -5331,7 +5349,6 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     const long long *nActPd = compact_flags_dev(c, cnt1P, edgesP, actP, "actP");
     const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
     // the pairs labelled by a block (their number comes back with the totals)
-    fill_i32(c, (int *)bigCnt, 0, 4);
     hipLaunchKernelGGL(k_big_pairs, dim3(grid_for(M, 1024)), dim3(1024), 0, c.stream, M,
                        (const long long *)offP, (const long long *)offQ, (const long long *)psP2,
                        (const long long *)psQ2, (long long)c.tune.bigNodes,
@@ -5747,8 +5764,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     // the large pairs on the side stream, concurrently with one wave per
     // remaining pair on the main one (disjoint pairs, disjoint regions)
     if (nBig > 0) {
-      unsigned long long *bigNext = wsT<unsigned long long>(c, "gh_big_next", 1);
-      fill_i32(c, (int *)bigNext, 0, 2);
+      unsigned long long *bigNext = ctr + 26;
       HIP_CHECK(hipEventRecord(c.ev_fork, c.stream));
       HIP_CHECK(hipStreamWaitEvent(c.side, c.ev_fork, 0));
       const long blocks = std::min<long>(nBig, c.tune.blockCap);
@@ -5775,8 +5791,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
         c.label_steal_blocks = (perCu > 0 ? perCu : 1) * c.cu_count;
         c.label_steal_lds = c.tune.labelLds;
       }
-      unsigned long long *next = wsT<unsigned long long>(c, "gh_label_next", 1);
-      fill_i32(c, (int *)next, 0, 2);
+      unsigned long long *next = ctr + 27;
       const long blocks = std::min<long>(M, c.label_steal_blocks);
       hipLaunchKernelGGL(k_label_wave_steal, dim3((unsigned)blocks), dim3(64),
                          (size_t)c.tune.labelLds, c.stream, M, (const int *)order, next,
@@ -5831,8 +5846,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   res.swapped = pm.swapped;
   res.ncomp = ta.outNComp;
   res.nverts = ta.outNVerts;
-  unsigned long long *sc = wsT<unsigned long long>(c, "gh_status_counts", 4);
-  fill_i32(c, (int *)sc, 0, 8);
+  unsigned long long *sc = ctr + 28;
   hipLaunchKernelGGL(k_status_counts, dim3(std::min<unsigned>(kStatBlocks, grid_for(M, 256))),
                      dim3(256), 0, c.stream, M, (const int *)pm.status, sc);
   HIP_CHECK(hipGetLastError());

@@ -573,10 +573,18 @@ long mbr_filter(Context &c, long B, long O, const uint64_t *bMBR, const uint64_t
 }
 
 // ------------------------------------------------------------ stage 2 ----
+// zero0..zero2 (optional): per-candidate arrays of the later passes,
+// cleared here rather than by fills of their own
 __global__ void k_get_cmbr(long K, const int *pairs, const double *bMBR2, const double *oMBR2,
-                           double *cmbr, int *pipFlag, signed char *pipType, int *joinFlag) {
+                           double *cmbr, int *pipFlag, signed char *pipType, int *joinFlag,
+                           int *zero0, int *zero1, int *zero2) {
   long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
+  if (zero0) {
+    zero0[k] = 0;
+    zero1[k] = 0;
+    zero2[k] = 0;
+  }
   int b = pairs[2 * k], o = pairs[2 * k + 1];
   double x1 = bMBR2[4 * b], x2 = bMBR2[4 * b + 2], y1 = bMBR2[4 * b + 1], y2 = bMBR2[4 * b + 3];
   double a1 = oMBR2[4 * o], a2 = oMBR2[4 * o + 2], b1 = oMBR2[4 * o + 1], b2 = oMBR2[4 * o + 3];
@@ -887,7 +895,8 @@ void join_refine(Context &c, const JoinInputs &in, long K, const int *pairs, Joi
   {
     LaunchTimer t(c, "get_cmbr");
     hipLaunchKernelGGL(k_get_cmbr, dim3(grid_for(K, 256)), dim3(256), 0, c.stream, K, pairs,
-                       in.bMBR2, in.oMBR2, r.cmbr, r.pipFlag, r.pipType, r.joinFlag);
+                       in.bMBR2, in.oMBR2, r.cmbr, r.pipFlag, r.pipType, r.joinFlag, r.bCount,
+                       r.oCount, r.segFlag);
     HIP_CHECK(hipGetLastError());
   }
   // list sizes stay on the device (grids are sized by K); they come back in
@@ -900,10 +909,8 @@ void join_refine(Context &c, const JoinInputs &in, long K, const int *pairs, Joi
                        r.joinFlag, pipN);
     HIP_CHECK(hipGetLastError());
   }
-  // within count = number of pipFlag==1 after the test
-  fill_i32(c, r.bCount, 0, K);
-  fill_i32(c, r.oCount, 0, K);
-  fill_i32(c, r.segFlag, 0, K);
+  // within count = number of pipFlag==1 after the test (bCount, oCount and
+  // segFlag were cleared by k_get_cmbr)
   const long long *withinN = compact_flags_dev(c, r.pipFlag, K, r.list, "within");
   const long long *cmfN = compact_flags_dev(c, r.joinFlag, K, r.list, "cmf");
   // four pairs per wave where the layers' polygons are small on average
@@ -974,7 +981,8 @@ void launch_get_cmbr(Context &c, long K, const int *pairs, const double *bMBR2,
   if (K <= 0) return;
   LaunchTimer t(c, "get_cmbr");
   hipLaunchKernelGGL(k_get_cmbr, dim3(grid_for(K, 256)), dim3(256), 0, c.stream, K, pairs, bMBR2,
-                     oMBR2, cmbr, pipFlag, pipType, joinFlag);
+                     oMBR2, cmbr, pipFlag, pipType, joinFlag, (int *)nullptr, (int *)nullptr,
+                     (int *)nullptr);
   HIP_CHECK(hipGetLastError());
 }
 
