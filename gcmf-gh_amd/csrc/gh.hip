@@ -452,16 +452,24 @@ __global__ void k_run_boxes(long edges, const int *map, const long long *off, co
   const long e0 = G * kRun1;
   if (e0 >= edges) return;
   double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, lmin = INFINITY;
+  bool nan = false;
   for (long e = e0; e < e0 + kRun1 && e < edges; e++) {
     const int k = map[e];
     const long long b = off[k], n = off[k + 1] - b;
     const pt p = ld(ring, e), q = ld(ring, e - b == n - 1 ? b : e + 1);
+    nan |= (p.x != p.x) | (p.y != p.y) | (q.x != q.x) | (q.y != q.y);
     x0 = fmin(x0, fmin(p.x, q.x));
     y0 = fmin(y0, fmin(p.y, q.y));
     x1 = fmax(x1, fmax(p.x, q.x));
     y1 = fmax(y1, fmax(p.y, q.y));
     const double dx = q.x - p.x, dy = q.y - p.y;
     lmin = fmin(lmin, sqrt(dx * dx + dy * dy));
+  }
+  if (nan) {
+    // a run with a NaN coordinate is never left out
+    x0 = y0 = -INFINITY;
+    x1 = y1 = INFINITY;
+    lmin = 0.0;
   }
   box1[4 * G] = x0;
   box1[4 * G + 1] = y0;
@@ -1702,44 +1710,159 @@ __device__ __forceinline__ bool src_on_q(int t) {
 constexpr int kStageQ = 512;
 extern __shared__ __align__(16) char count_stage[];
 
-template <bool CULL>
-__device__ __forceinline__ void count_edge_cpu(long g, long edgesP, long edgesQ, const int *mapP,
-                                               const int *mapQ, const long long *offP,
-                                               const long long *offQ, const double *ringP,
-                                               const double *ringQ, const double *lenP,
-                                               const double *mrgP, const double *lenQ,
-                                               const double *mrgQ, const double *cf, int *cnt1P,
-                                               int *cnt2P, int *cnt1Q, int *cnt2Q, int *boolP,
-                                               int *boolQ, int *lstP, int *lstQ,
-                                               const RunBoxes &boxP, const RunBoxes &boxQ,
-                                               int ldsQ, int stage, unsigned long long &ev,
-                                               unsigned long long &tv) {
-  const bool sideP = g < edgesP;
+// One lane's edge of the count pass: its pair, its ring neighbours, the
+// filter and whether the edge is walked at all (flag).  live is false for
+// lanes past the edges and for P edges of pairs the LDS pass takes.
+struct CountLane {
+  bool live, sideP, flag;
+  long ga;
+  int k;
+  long long bBase, bN;
+  pt A1, A2;
+  EdgeBox ab;
+  CpuFilter f;
+};
+
+__device__ __forceinline__ CountLane count_prep(long g, long edgesP, long edgesQ,
+                                                const int *mapP, const int *mapQ,
+                                                const long long *offP, const long long *offQ,
+                                                const double *ringP, const double *ringQ,
+                                                const double *lenP, const double *mrgP,
+                                                const double *lenQ, const double *mrgQ,
+                                                const double *cf, long cullMin, int ldsQ) {
+  CountLane L;
+  L.live = false;
+  L.flag = false;
+  L.sideP = g < edgesP;
+  if (g >= edgesP + edgesQ) return L;
+  const bool sideP = L.sideP;
   const long ga = sideP ? g : g - edgesP;
   const int k = sideP ? mapP[ga] : mapQ[ga];
   const long long aBase = sideP ? offP[k] : offQ[k];
   const long long aN = (sideP ? offP[k + 1] : offQ[k + 1]) - aBase;
   const long long bBase = sideP ? offQ[k] : offP[k];
   const long long bN = (sideP ? offQ[k + 1] : offP[k + 1]) - bBase;
+  L.ga = ga;
+  L.k = k;
+  L.bBase = bBase;
+  L.bN = bN;
   // P edges of pairs whose Q ring the LDS pass (k_count_cpu_lds) takes
-  if (sideP && bN > boxP.cullMin && bN <= ldsQ) return;
+  if (sideP && bN > cullMin && bN <= ldsQ) return L;
+  L.live = true;
   const double *A = sideP ? ringP : ringQ;
-  const double *Bv = (sideP ? ringQ : ringP) + 2 * bBase;
   const long long i = ga - aBase;
-  const pt A1 = ld(A, ga), A2 = ld(A, i == aN - 1 ? aBase : ga + 1);
-  CpuFilter f = cpu_filter(cf, k, sideP, ga, bBase, lenP, mrgP, lenQ, mrgQ);
-  const double *ob = cf + (long long)CF_N * k + (sideP ? CF_BOXQ : CF_BOXP);
-  const EdgeBox ab = edge_box(A1, A2);
+  L.A1 = ld(A, ga);
+  L.A2 = ld(A, i == aN - 1 ? aBase : ga + 1);
+  L.f = cpu_filter(cf, k, sideP, ga, bBase, lenP, mrgP, lenQ, mrgQ);
+  L.ab = edge_box(L.A1, L.A2);
   {
     // G widened by a relative 1e-12 so that rounding of the box sums cannot
     // turn a gap of exactly G into a reject
-    const double Gw = f.G * (1.0 + 1e-12) + 1e-300;
-    f.wide = EdgeBox{ab.x0 - Gw, ab.y0 - Gw, ab.x1 + Gw, ab.y1 + Gw};
+    const double Gw = L.f.G * (1.0 + 1e-12) + 1e-300;
+    L.f.wide = EdgeBox{L.ab.x0 - Gw, L.ab.y0 - Gw, L.ab.x1 + Gw, L.ab.y1 + Gw};
   }
+  const double *ob = cf + (long long)CF_N * k + (sideP ? CF_BOXQ : CF_BOXP);
+  const EdgeBox &ab = L.ab;
   const double ogap = fmax(fmax(ob[0] - ab.x1, ab.x0 - ob[2]), fmax(ob[1] - ab.y1, ab.y0 - ob[3]));
-  const bool flag = f.tiny || !(ogap > f.G);
+  L.flag = L.f.tiny || !(ogap > L.f.G);
+  return L;
+}
+
+// a value every lane of the wave holds, moved to scalar registers
+__device__ __forceinline__ double wave_uniform(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// U for the lanes of a wave that walk a ring, when they are all P edges of
+// one pair walking its Q ring in full with no possible tiny pair (then
+// true, with that ring's first global edge e0 and length n); every lane of
+// the wave calls it
+__device__ __forceinline__ bool wave_walk_box(const CountLane &L, bool walkedFull, EdgeBox &U,
+                                              long long &e0, int &n) {
+  const bool walks = L.live && L.flag;
+  const unsigned long long wm = __ballot(walks);
+  if (!wm) return false;
+  const int first = __ffsll((long long)wm) - 1;
+  const int k0 = __shfl(L.k, first, 64);
+  const bool bad = walks && (!L.sideP || L.k != k0 || L.f.tiny || !walkedFull);
+  if (__ballot(bad)) return false;
+  double x0 = walks ? L.f.wide.x0 : INFINITY, y0 = walks ? L.f.wide.y0 : INFINITY;
+  double x1 = walks ? L.f.wide.x1 : -INFINITY, y1 = walks ? L.f.wide.y1 : -INFINITY;
+  for (int off = 32; off > 0; off >>= 1) {
+    x0 = fmin(x0, __shfl_xor(x0, off, 64));
+    y0 = fmin(y0, __shfl_xor(y0, off, 64));
+    x1 = fmax(x1, __shfl_xor(x1, off, 64));
+    y1 = fmax(y1, __shfl_xor(y1, off, 64));
+  }
+  // the same in every lane: held in scalar registers
+  U = EdgeBox{wave_uniform(x0), wave_uniform(y0), wave_uniform(x1), wave_uniform(y1)};
+  const long long b0 = __shfl(L.bBase, first, 64);
+  const int n0 = (int)__shfl(L.bN, first, 64);
+  e0 = (long long)__builtin_amdgcn_readfirstlane((unsigned)b0) |
+       ((long long)__builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)b0 >> 32)) << 32);
+  n = __builtin_amdgcn_readfirstlane(n0);
+  return true;
+}
+
+// lane l's double, read by every lane (l the same in every lane)
+__device__ __forceinline__ double lane_d(double v, int l) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The count pass of a wave whose walking lanes share one Q ring (global
+// edges [e0, e0 + n)), walked by every lane of the wave in step: the ring's
+// runs of kRun1 edges (boxes b1, global edge order) whose box misses U are
+// left out, since U holds the widened boxes of every walking lane, none with
+// a possible tiny pair, so each walking lane's filter skips every edge of
+// such a run (its endpoints lie beyond one side of the lane's widened box);
+// of each other run, lanes 0..kRun1 load its vertices once and every lane
+// reads them from there (in scalar registers), in index order.  Lanes that
+// do not walk take part without visiting.
+template <class F>
+__device__ __forceinline__ int wave_ring_edges(const double *V, long long e0, int n,
+                                               const double *b1, const EdgeBox &U, F &&f) {
+  // run r covers the ring's edges [r kRun1 - o, (r + 1) kRun1 - o)
+  const double *b = b1 + 4 * (e0 / kRun1);
+  const int o = (int)(e0 % kRun1);
+  const int lane = (int)__lane_id();
+  int visited = 0;
+  for (int qa = 0, qz = kRun1 - o; qa < n; qa = qz, qz += kRun1, b += 4) {
+    if (U.x0 > b[2] || U.x1 < b[0] || U.y0 > b[3] || U.y1 < b[1]) continue;
+    const int z = min(qz, n);
+    visited += z - qa;
+    // vertices qa .. z (the last one wraps to 0 at the ring's end)
+    pt v{0.0, 0.0};
+    if (lane <= z - qa) v = ld(V, qa + lane == n ? 0 : qa + lane);
+    for (int q = qa; q < z; q++) {
+      const int j = q - qa;
+      f(q, pt{lane_d(v.x, j), lane_d(v.y, j)}, pt{lane_d(v.x, j + 1), lane_d(v.y, j + 1)});
+    }
+  }
+  return visited;
+}
+
+template <bool CULL>
+__device__ __forceinline__ void count_edge_cpu(const CountLane &L, const double *ringP,
+                                               const double *ringQ, int *cnt1P, int *cnt2P,
+                                               int *cnt1Q, int *cnt2Q, int *boolP, int *boolQ,
+                                               int *lstP, int *lstQ, const RunBoxes &boxP,
+                                               const RunBoxes &boxQ, int stage,
+                                               unsigned long long &ev, unsigned long long &tv) {
+  const bool sideP = L.sideP;
+  const long ga = L.ga;
+  const long long bBase = L.bBase, bN = L.bN;
+  const double *Bv = (sideP ? ringQ : ringP) + 2 * bBase;
+  const pt A1 = L.A1, A2 = L.A2;
+  const CpuFilter &f = L.f;
+  const EdgeBox &ab = L.ab;
   int c1 = 0, c2 = 0;
-  if (flag) {
+  if (L.flag) {
     int *lst = (sideP ? lstP : lstQ) + (long long)ga * kPartners;
     auto visit = [&](int q, pt B1, pt B2) {
       if (f.skip(q, A1, A2, B1, B2)) return;
@@ -1752,28 +1875,59 @@ __device__ __forceinline__ void count_edge_cpu(long g, long edgesP, long edgesQ,
       }
       c2 += (sideP ? ins_on_p(t) : ins_on_q(t)) ? 1 : 0;
     };
+    const double *Vs =
+        sideP && stage >= 0 ? (const double *)count_stage + 2 * kStageQ * stage : Bv;
     if (CULL && bN > boxP.cullMin) {
       // runs apart from the box widened by G and without a possible tiny
       // pair hold no pair the filter keeps
       const double G = f.G * (1.0 + 1e-12) + 1e-300;
       const EdgeBox wide{ab.x0 - G, ab.y0 - G, ab.x1 + G, ab.y1 + G};
       tv = culled_edges(Bv, bBase, bBase + bN, sideP ? boxQ : boxP, wide, f.la, f.T, visit);
-    } else if (sideP && stage >= 0) {
-      ring_edges((const double *)count_stage + 2 * kStageQ * stage, (int)bN, visit);
-      tv = bN;
     } else {
-      ring_edges(Bv, (int)bN, visit);
+      ring_edges(Vs, (int)bN, visit);
       tv = bN;
     }
   }
   if (sideP) {
     cnt1P[ga] = c1;
     cnt2P[ga] = c2 + 1;
-    boolP[ga] = flag;
+    boolP[ga] = L.flag;
   } else {
     cnt1Q[ga] = c1;
     cnt2Q[ga] = c2 + 1;
-    boolQ[ga] = flag;
+    boolQ[ga] = L.flag;
+  }
+}
+
+// count_edge_cpu for a wave that walks one Q ring together
+// (wave_ring_edges); every lane of the wave calls it
+__device__ __forceinline__ void count_edge_wave(const CountLane &L, const double *ringQ,
+                                                int *cnt1P, int *cnt2P, int *boolP, int *lstP,
+                                                long long e0, int n, const double *b1,
+                                                const EdgeBox &U, unsigned long long &ev,
+                                                unsigned long long &tv) {
+  const bool walks = L.live && L.flag;
+  const pt A1 = L.A1, A2 = L.A2;
+  const CpuFilter &f = L.f;
+  int c1 = 0, c2 = 0;
+  int *lst = lstP + (long long)L.ga * kPartners;
+  auto visit = [&](int q, pt B1, pt B2) {
+    if (!walks || f.skip(q, A1, A2, B1, B2)) return;
+    double al, be;
+    ev++;
+    const int t = itype(A1, A2, B1, B2, al, be);
+    if (t != 0) {
+      if (c1 < kPartners) lst[c1] = q;
+      c1++;
+    }
+    c2 += ins_on_p(t) ? 1 : 0;
+  };
+  const int vis = wave_ring_edges(ringQ + 2 * e0, e0, n, b1, U, visit);
+  if (walks) tv = vis;
+  if (L.live) {
+    cnt1P[L.ga] = c1;
+    cnt2P[L.ga] = c2 + 1;
+    boolP[L.ga] = L.flag;
   }
 }
 
@@ -1781,6 +1935,9 @@ __device__ __forceinline__ void count_edge_cpu(long g, long edgesP, long edgesQ,
 // is bound by latency, and 5 waves run it 6 % faster than the 4 that its
 // register count would otherwise allow (0.42 vs 0.45 ms on the join
 // workload); the culled walk keeps the registers it needs to run without spills
+// skipQ (optional): the Q edges' kRun1-edge run boxes, with which a wave
+// whose walking lanes share one pair walks its Q ring together and leaves
+// out the runs none of them can pair with (count_edge_wave)
 template <bool CULL>
 __global__ void __launch_bounds__(256, CULL ? 1 : 5)
     k_count_cpu(long edgesP, long edgesQ, const int *mapP, const int *mapQ,
@@ -1788,7 +1945,8 @@ __global__ void __launch_bounds__(256, CULL ? 1 : 5)
                 const double *ringQ, const double *lenP, const double *mrgP, const double *lenQ,
                 const double *mrgQ, const double *cf, int *cnt1P, int *cnt2P, int *cnt1Q,
                 int *cnt2Q, int *boolP, int *boolQ, unsigned long long *wstat, int *lstP,
-                int *lstQ, RunBoxes boxP, RunBoxes boxQ, int ldsQ, int stageOk) {
+                int *lstQ, RunBoxes boxP, RunBoxes boxQ, int ldsQ, int stageOk,
+                const double *skipQ) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long ev = 0, tv = 0;
   // stage the wave's Q ring when every P edge of the wave is of one pair
@@ -1816,10 +1974,16 @@ __global__ void __launch_bounds__(256, CULL ? 1 : 5)
     }
     __syncthreads();
   }
-  if (g < edgesP + edgesQ)
-    count_edge_cpu<CULL>(g, edgesP, edgesQ, mapP, mapQ, offP, offQ, ringP, ringQ, lenP, mrgP,
-                         lenQ, mrgQ, cf, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, lstP, lstQ,
-                         boxP, boxQ, ldsQ, stage, ev, tv);
+  const CountLane L = count_prep(g, edgesP, edgesQ, mapP, mapQ, offP, offQ, ringP, ringQ, lenP,
+                                 mrgP, lenQ, mrgQ, cf, boxP.cullMin, ldsQ);
+  EdgeBox U{0.0, 0.0, 0.0, 0.0};
+  long long e0 = 0;
+  int n = 0;
+  if (skipQ && wave_walk_box(L, !(CULL && L.bN > boxP.cullMin), U, e0, n))
+    count_edge_wave(L, ringQ, cnt1P, cnt2P, boolP, lstP, e0, n, skipQ, U, ev, tv);
+  else if (L.live)
+    count_edge_cpu<CULL>(L, ringP, ringQ, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, lstP, lstQ,
+                         boxP, boxQ, stage, ev, tv);
   // every lane of every wave reaches the statistics
   wave_stats(wstat, g, ev, tv);
 }
@@ -5195,6 +5359,18 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       }
       {  // the ring walk: every pair the LDS pass does not take
         const bool cull = boxQ.b1 != nullptr;
+        // the Q edges' kRun1-edge run boxes for the waves' run skipping
+        const double *skipQ = boxQ.b1;
+        if (!skipQ && edgesQ > 0) {
+          const long r1 = (edgesQ + kRun1 - 1) / kRun1;
+          double *b1 = wsT<double>(c, "gh_rb1Q", 4 * r1);
+          double *l1 = wsT<double>(c, "gh_rl1Q", r1);
+          hipLaunchKernelGGL(k_run_boxes, dim3(grid_for(r1, 256)), dim3(256), 0, c.stream,
+                             edgesQ, mapQ, (const long long *)offQ, (const double *)ringQ, b1,
+                             l1);
+          HIP_CHECK(hipGetLastError());
+          skipQ = b1;
+        }
         const int cb = c.tune.countBlock;
         const unsigned grid = grid_for(edgesP, cb);
         const long waves = (long)grid * cb / 64;
@@ -5209,7 +5385,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                              (const double *)elenP, (const double *)emrgP,
                              (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
                              cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, wstat, lstP, lstQ, boxP,
-                             boxQ, capQ > minQ ? capQ : 0, c.tune.countStage);
+                             boxQ, capQ > minQ ? capQ : 0, c.tune.countStage, skipQ);
           HIP_CHECK(hipGetLastError());
         }
         sum_wave_stats(c, wstat, waves, stats);
