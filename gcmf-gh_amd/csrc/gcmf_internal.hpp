@@ -65,7 +65,8 @@ struct Tuning {
   int cmfGroup = 0;         // lanes per pair in the CMF edge filter (16, 64; 0: by polygon size)
   int labelSteal = 0;       // 1: labelling waves are persistent and take pairs from a counter
   int blockCap = 128;       // blocks of the block labelling (each takes pairs from a counter)
-  int countStage = 0;       // 1: the count walk reads a wave's Q ring from an LDS copy
+  int countStage = 0;       // 1: each count lane walks the whole Q ring (a wave's copy in
+                            // LDS); 0: a wave's lanes of one pair walk it together by runs
   int countBlock = 128;     // threads per block of the count kernel (64, 128, 256)
   int labelLds = 20480;     // LDS bytes per labelling wave for a pair's arrays (0: none)
   int ldsMaxQ = 1024;       // pairs with Q rings above cullMin and up to this size are

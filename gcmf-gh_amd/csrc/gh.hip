@@ -1776,19 +1776,21 @@ __device__ __forceinline__ double wave_uniform(double v) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// U for the lanes of a wave that walk a ring, when they are all P edges of
-// one pair walking its Q ring in full with no possible tiny pair (then
-// true, with that ring's first global edge e0 and length n); every lane of
-// the wave calls it
-__device__ __forceinline__ bool wave_walk_box(const CountLane &L, bool walkedFull, EdgeBox &U,
-                                              long long &e0, int &n) {
-  const bool walks = L.live && L.flag;
-  const unsigned long long wm = __ballot(walks);
-  if (!wm) return false;
-  const int first = __ffsll((long long)wm) - 1;
+// The next group of a wave's lanes for the wave walk: the pending lanes
+// (P edges walked over their whole Q ring, not yet counted) of the pair of
+// the first pending lane, with that ring's first global edge e0 and length
+// n, and U, the union of the group's widened boxes over the lanes that walk
+// (all of the plane when one of them may form a tiny pair).  False when no
+// lane is pending.  Every lane of the wave calls it.
+__device__ __forceinline__ bool wave_group(const CountLane &L, bool pending, bool &group,
+                                           EdgeBox &U, long long &e0, int &n) {
+  const unsigned long long pm = __ballot(pending);
+  if (!pm) return false;
+  const int first = __ffsll((long long)pm) - 1;
   const int k0 = __shfl(L.k, first, 64);
-  const bool bad = walks && (!L.sideP || L.k != k0 || L.f.tiny || !walkedFull);
-  if (__ballot(bad)) return false;
+  group = pending && L.k == k0;
+  const bool walks = group && L.flag;
+  const bool anyTiny = __ballot(walks && L.f.tiny) != 0;
   double x0 = walks ? L.f.wide.x0 : INFINITY, y0 = walks ? L.f.wide.y0 : INFINITY;
   double x1 = walks ? L.f.wide.x1 : -INFINITY, y1 = walks ? L.f.wide.y1 : -INFINITY;
   for (int off = 32; off > 0; off >>= 1) {
@@ -1796,6 +1798,10 @@ __device__ __forceinline__ bool wave_walk_box(const CountLane &L, bool walkedFul
     y0 = fmin(y0, __shfl_xor(y0, off, 64));
     x1 = fmax(x1, __shfl_xor(x1, off, 64));
     y1 = fmax(y1, __shfl_xor(y1, off, 64));
+  }
+  if (anyTiny) {
+    x0 = y0 = -INFINITY;
+    x1 = y1 = INFINITY;
   }
   // the same in every lane: held in scalar registers
   U = EdgeBox{wave_uniform(x0), wave_uniform(y0), wave_uniform(x1), wave_uniform(y1)};
@@ -1815,15 +1821,16 @@ __device__ __forceinline__ double lane_d(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// The count pass of a wave whose walking lanes share one Q ring (global
+// The count pass of a group of a wave's lanes that share one Q ring (global
 // edges [e0, e0 + n)), walked by every lane of the wave in step: the ring's
 // runs of kRun1 edges (boxes b1, global edge order) whose box misses U are
-// left out, since U holds the widened boxes of every walking lane, none with
-// a possible tiny pair, so each walking lane's filter skips every edge of
-// such a run (its endpoints lie beyond one side of the lane's widened box);
-// of each other run, lanes 0..kRun1 load its vertices once and every lane
-// reads them from there (in scalar registers), in index order.  Lanes that
-// do not walk take part without visiting.
+// left out, since U holds the widened boxes of every walking lane of the
+// group, none with a possible tiny pair (U is the whole plane otherwise), so
+// each walking lane's filter skips every edge of such a run (its endpoints
+// lie beyond one side of the lane's widened box); of each other run, lanes
+// 0..kRun1 load its vertices once and every lane reads them from there (in
+// scalar registers), in index order.  Lanes that do not walk take part
+// without visiting.
 template <class F>
 __device__ __forceinline__ int wave_ring_edges(const double *V, long long e0, int n,
                                                const double *b1, const EdgeBox &U, F &&f) {
@@ -1899,14 +1906,14 @@ __device__ __forceinline__ void count_edge_cpu(const CountLane &L, const double 
   }
 }
 
-// count_edge_cpu for a wave that walks one Q ring together
-// (wave_ring_edges); every lane of the wave calls it
-__device__ __forceinline__ void count_edge_wave(const CountLane &L, const double *ringQ,
-                                                int *cnt1P, int *cnt2P, int *boolP, int *lstP,
-                                                long long e0, int n, const double *b1,
-                                                const EdgeBox &U, unsigned long long &ev,
-                                                unsigned long long &tv) {
-  const bool walks = L.live && L.flag;
+// count_edge_cpu for a group of a wave's lanes that walks one Q ring
+// together (wave_ring_edges); every lane of the wave calls it
+__device__ __forceinline__ void count_edge_wave(const CountLane &L, bool group,
+                                                const double *ringQ, int *cnt1P, int *cnt2P,
+                                                int *boolP, int *lstP, long long e0, int n,
+                                                const double *b1, const EdgeBox &U,
+                                                unsigned long long &ev, unsigned long long &tv) {
+  const bool walks = group && L.flag;
   const pt A1 = L.A1, A2 = L.A2;
   const CpuFilter &f = L.f;
   int c1 = 0, c2 = 0;
@@ -1922,9 +1929,11 @@ __device__ __forceinline__ void count_edge_wave(const CountLane &L, const double
     }
     c2 += ins_on_p(t) ? 1 : 0;
   };
-  const int vis = wave_ring_edges(ringQ + 2 * e0, e0, n, b1, U, visit);
-  if (walks) tv = vis;
-  if (L.live) {
+  if (__ballot(walks)) {
+    const int vis = wave_ring_edges(ringQ + 2 * e0, e0, n, b1, U, visit);
+    if (walks) tv = vis;
+  }
+  if (group) {
     cnt1P[L.ga] = c1;
     cnt2P[L.ga] = c2 + 1;
     boolP[L.ga] = L.flag;
@@ -1935,9 +1944,10 @@ __device__ __forceinline__ void count_edge_wave(const CountLane &L, const double
 // is bound by latency, and 5 waves run it 6 % faster than the 4 that its
 // register count would otherwise allow (0.42 vs 0.45 ms on the join
 // workload); the culled walk keeps the registers it needs to run without spills
-// skipQ (optional): the Q edges' kRun1-edge run boxes, with which a wave
-// whose walking lanes share one pair walks its Q ring together and leaves
-// out the runs none of them can pair with (count_edge_wave)
+// skipQ (optional): the Q edges' kRun1-edge run boxes, with which the P
+// edges of a wave walked over their whole Q ring walk it together, one
+// group per pair, leaving out the runs no lane of the group can pair with
+// (count_edge_wave)
 template <bool CULL>
 __global__ void __launch_bounds__(256, CULL ? 1 : 5)
     k_count_cpu(long edgesP, long edgesQ, const int *mapP, const int *mapQ,
@@ -1976,12 +1986,19 @@ __global__ void __launch_bounds__(256, CULL ? 1 : 5)
   }
   const CountLane L = count_prep(g, edgesP, edgesQ, mapP, mapQ, offP, offQ, ringP, ringQ, lenP,
                                  mrgP, lenQ, mrgQ, cf, boxP.cullMin, ldsQ);
-  EdgeBox U{0.0, 0.0, 0.0, 0.0};
-  long long e0 = 0;
-  int n = 0;
-  if (skipQ && wave_walk_box(L, !(CULL && L.bN > boxP.cullMin), U, e0, n))
-    count_edge_wave(L, ringQ, cnt1P, cnt2P, boolP, lstP, e0, n, skipQ, U, ev, tv);
-  else if (L.live)
+  // P edges walked over their whole Q ring: one group per pair of the wave
+  bool pending = skipQ && L.live && L.sideP && !(CULL && L.bN > boxP.cullMin);
+  if (skipQ) {
+    EdgeBox U;
+    long long e0;
+    int n;
+    bool group;
+    while (wave_group(L, pending, group, U, e0, n)) {
+      count_edge_wave(L, group, ringQ, cnt1P, cnt2P, boolP, lstP, e0, n, skipQ, U, ev, tv);
+      if (group) pending = false;
+    }
+  }
+  if (L.live && !(skipQ && L.sideP && !(CULL && L.bN > boxP.cullMin)))
     count_edge_cpu<CULL>(L, ringP, ringQ, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, lstP, lstQ,
                          boxP, boxQ, stage, ev, tv);
   // every lane of every wave reaches the statistics
@@ -5359,9 +5376,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       }
       {  // the ring walk: every pair the LDS pass does not take
         const bool cull = boxQ.b1 != nullptr;
-        // the Q edges' kRun1-edge run boxes for the waves' run skipping
-        const double *skipQ = boxQ.b1;
-        if (!skipQ && edgesQ > 0) {
+        // the Q edges' kRun1-edge run boxes for the waves' walks (the
+        // staged walk, Tuning::countStage, walks lane by lane instead)
+        const double *skipQ = c.tune.countStage ? nullptr : boxQ.b1;
+        if (!skipQ && !c.tune.countStage && edgesQ > 0) {
           const long r1 = (edgesQ + kRun1 - 1) / kRun1;
           double *b1 = wsT<double>(c, "gh_rb1Q", 4 * r1);
           double *l1 = wsT<double>(c, "gh_rl1Q", r1);

@@ -236,9 +236,12 @@ typedef struct {
   int block_cap;        /* blocks of the 1024-thread labelling launch, each
                           taking the next listed pair from a device counter
                           (1..65536, 128) */
-  int count_stage;      /* 1: a count wave whose edges are of one pair copies
-                          that pair's other ring (up to 512 vertices) into LDS
-                          and walks it there; 0: walks it through the caches (0) */
+  int count_stage;      /* 0: the edges of one pair in a count wave walk the
+                          other ring together, 16-edge runs at a time, leaving
+                          out the runs none of them can pair with; 1: every
+                          edge walks the whole ring, which a wave whose edges
+                          are of one pair first copies into LDS (up to 512
+                          vertices) (0) */
 } gcmf_tuning;
 int gcmf_get_tuning(gcmf_ctx *ctx, gcmf_tuning *out);
 int gcmf_set_tuning(gcmf_ctx *ctx, const gcmf_tuning *t);

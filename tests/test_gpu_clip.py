@@ -540,6 +540,43 @@ def test_count_lds_pass_matches_ring_walk(gctx):
         np.testing.assert_array_equal(a2[f], a0[f], err_msg=f)
 
 
+def test_count_wave_walk_equals_lane_walk(gctx):
+    """The count pass's wave walk (the edges of one pair in a wave walk the
+    other ring together by 16-edge runs, leaving out runs none of them can
+    pair with; count_stage 0) gives the lane-by-lane walk's (count_stage 1)
+    phase-1 arrays and results bit for bit, and visits fewer edge pairs: the
+    Fig fixtures, random pairs of the join workload's sizes, grid-snapped
+    pairs (tiny and collinear edges), the long-run cases and an uneven batch
+    whose longest rings take the culled walk, with the culling threshold at
+    its default and at 64."""
+    cases = fig_cases() + random_cases(64, seed0=950, nmax=520) + snapped_cases(200) + \
+        long_run_cases()
+    P, Q = G.gen_pair_layers(600, seed=13, nmax=3000)
+    cases += [("u%d" % k, P.ring(k), Q.ring(k)) for k in range(600)]
+    default = gctx.tuning()
+    try:
+        for cull in (default["cull_min"], 64):
+            out = []
+            for stage in (0, 1):
+                gctx.set_tuning(cull, default["big_nodes"], default["count_block"], 0,
+                                count_stage=stage)
+                st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
+                n = {"psP1": st["edges_p"] + 1, "psP2": st["edges_p"] + 1,
+                     "psQ1": st["edges_q"] + 1, "psQ2": st["edges_q"] + 1,
+                     "boolP": st["edges_p"], "boolQ": st["edges_q"], "iP": 2 * st["slots_p"],
+                     "iQ": 2 * st["slots_q"], "nbP": st["slots_p"], "nbQ": st["slots_q"],
+                     "status": len(cases), "comp_sizes": st["total_comps"],
+                     "xy": 2 * st["total_verts"]}
+                out.append((st, {f: gctx.clip_get(f, c) for f, c in n.items()}))
+            (s0, a0), (s1, a1) = out
+            assert s0["n_eval"] == s1["n_eval"] and s0["n_intersections"] == s1["n_intersections"]
+            assert s0["n_tested"] < s1["n_tested"], (cull, s0["n_tested"], s1["n_tested"])
+            for f in a0:
+                np.testing.assert_array_equal(a0[f], a1[f], err_msg="%s cull %d" % (f, cull))
+    finally:
+        gctx.set_tuning(**default)
+
+
 @pytest.mark.parametrize("label_lds", [0, 2048, 65536])
 def test_label_lds_arena_keeps_results(gctx, label_lds):
     """The wave labelling kernel keeps a pair's node arrays, segment scratch
