@@ -446,25 +446,35 @@ struct RunBoxes {
   long cullMin;                // rings longer than this are walked through the runs
 };
 
-__global__ void k_run_boxes(long edges, const int *map, const long long *off, const double *ring,
-                            double *box1, double *len1) {
-  const long G = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long e0 = G * kRun1;
-  if (e0 >= edges) return;
+// one thread per edge; the kRun1 = 16 lanes of a run combine their boxes
+__global__ void __launch_bounds__(256)
+    k_run_boxes(long edges, const int *map, const long long *off, const double *ring,
+                double *box1, double *len1) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY, lmin = INFINITY;
   bool nan = false;
-  for (long e = e0; e < e0 + kRun1 && e < edges; e++) {
+  if (e < edges) {
     const int k = map[e];
     const long long b = off[k], n = off[k + 1] - b;
     const pt p = ld(ring, e), q = ld(ring, e - b == n - 1 ? b : e + 1);
-    nan |= (p.x != p.x) | (p.y != p.y) | (q.x != q.x) | (q.y != q.y);
-    x0 = fmin(x0, fmin(p.x, q.x));
-    y0 = fmin(y0, fmin(p.y, q.y));
-    x1 = fmax(x1, fmax(p.x, q.x));
-    y1 = fmax(y1, fmax(p.y, q.y));
+    nan = (p.x != p.x) | (p.y != p.y) | (q.x != q.x) | (q.y != q.y);
+    x0 = fmin(p.x, q.x);
+    y0 = fmin(p.y, q.y);
+    x1 = fmax(p.x, q.x);
+    y1 = fmax(p.y, q.y);
     const double dx = q.x - p.x, dy = q.y - p.y;
-    lmin = fmin(lmin, sqrt(dx * dx + dy * dy));
+    lmin = sqrt(dx * dx + dy * dy);
   }
+  for (int off2 = 1; off2 < kRun1; off2 <<= 1) {
+    x0 = fmin(x0, __shfl_xor(x0, off2, 64));
+    y0 = fmin(y0, __shfl_xor(y0, off2, 64));
+    x1 = fmax(x1, __shfl_xor(x1, off2, 64));
+    y1 = fmax(y1, __shfl_xor(y1, off2, 64));
+    lmin = fmin(lmin, __shfl_xor(lmin, off2, 64));
+    nan |= __shfl_xor((int)nan, off2, 64) != 0;
+  }
+  const long G = e / kRun1;
+  if ((e & (kRun1 - 1)) != 0 || e >= edges) return;
   if (nan) {
     // a run with a NaN coordinate is never left out
     x0 = y0 = -INFINITY;
@@ -1125,7 +1135,7 @@ constexpr int kBlockSort = 2048;
 struct RunListSide {
   long edges;
   const int *map;
-  const long long *ps2, *ps1;
+  const int *cnt2, *cnt1;  // the count pass's count2 (1 + inserts) and count1
   int *listS, *list, *listL, *lenL, *listO;
   unsigned long long *cnt;
 };
@@ -1149,9 +1159,9 @@ __global__ void __launch_bounds__(1024)
   for (int r = 0; r < kRunListEdges; r++) {
     const long g = g0 + 1024L * r;
     const bool ok = g < S.edges && status[S.map[g]] == GH_OK;
-    const long long m = ok ? S.ps2[g + 1] - S.ps2[g] - 1 : 0;
+    const long long m = ok ? S.cnt2[g] - 1 : 0;
     const bool pr[4] = {m >= 2 && m <= kSmallSort, m > kSmallSort && m <= kBlockSort,
-                        m > kBlockSort, ok && S.ps1[g + 1] - S.ps1[g] > kPartners};
+                        m > kBlockSort, ok && S.cnt1[g] > kPartners};
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       bits |= (unsigned)pr[i] << (4 * r + i);
@@ -1199,7 +1209,7 @@ __global__ void __launch_bounds__(1024)
       run += 1ULL << (16 * i);
       out[i][pos] = (int)g;
       if (i == 2) {
-        const long long m = S.ps2[g + 1] - S.ps2[g] - 1;
+        const long long m = S.cnt2[g] - 1;
         S.lenL[pos] = (int)m;
         atomicAdd(S.cnt + 3, (unsigned long long)m);
       }
@@ -1947,8 +1957,12 @@ __device__ __forceinline__ void count_edge_wave(const CountLane &L, bool group,
 // skipQ (optional): the Q edges' kRun1-edge run boxes, with which the P
 // edges of a wave walked over their whole Q ring walk it together, one
 // group per pair, leaving out the runs no lane of the group can pair with
-// (count_edge_wave)
-template <bool CULL>
+// (count_edge_wave).  WAVE: skipQ is given and every edge is a P edge
+// (edgesQ = 0), so that without culling no lane walks on its own and the
+// lane walk is left out of the kernel.  (At five waves per SIMD it spills
+// 12 bytes per lane; four waves without spills measured slower on the join
+// workload, 0.398 against 0.38 ms.)
+template <bool CULL, bool WAVE>
 __global__ void __launch_bounds__(256, CULL ? 1 : 5)
     k_count_cpu(long edgesP, long edgesQ, const int *mapP, const int *mapQ,
                 const long long *offP, const long long *offQ, const double *ringP,
@@ -1962,7 +1976,7 @@ __global__ void __launch_bounds__(256, CULL ? 1 : 5)
   // stage the wave's Q ring when every P edge of the wave is of one pair
   // whose ring is walked in full and fits (the same in every lane)
   int stage = -1;
-  if (stageOk) {
+  if (!WAVE && stageOk) {
     const bool pv = g < edgesP;
     const int k = pv ? mapP[g] : -1;
     const unsigned long long vm = __ballot(pv);
@@ -1987,8 +2001,8 @@ __global__ void __launch_bounds__(256, CULL ? 1 : 5)
   const CountLane L = count_prep(g, edgesP, edgesQ, mapP, mapQ, offP, offQ, ringP, ringQ, lenP,
                                  mrgP, lenQ, mrgQ, cf, boxP.cullMin, ldsQ);
   // P edges walked over their whole Q ring: one group per pair of the wave
-  bool pending = skipQ && L.live && L.sideP && !(CULL && L.bN > boxP.cullMin);
-  if (skipQ) {
+  bool pending = WAVE && L.live && L.sideP && !(CULL && L.bN > boxP.cullMin);
+  if (WAVE) {
     EdgeBox U;
     long long e0;
     int n;
@@ -1998,7 +2012,7 @@ __global__ void __launch_bounds__(256, CULL ? 1 : 5)
       if (group) pending = false;
     }
   }
-  if (L.live && !(skipQ && L.sideP && !(CULL && L.bN > boxP.cullMin)))
+  if ((CULL || !WAVE) && L.live && !(WAVE && L.sideP && !(CULL && L.bN > boxP.cullMin)))
     count_edge_cpu<CULL>(L, ringP, ringQ, cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, lstP, lstQ,
                          boxP, boxQ, stage, ev, tv);
   // every lane of every wave reaches the statistics
@@ -5317,8 +5331,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     double *l1 = wsT<double>(c, qside ? "gh_rl1Q" : "gh_rl1P", r1);
     double *l2 = wsT<double>(c, qside ? "gh_rl2Q" : "gh_rl2P", r2);
     double *l3 = wsT<double>(c, qside ? "gh_rl3Q" : "gh_rl3P", r3);
-    hipLaunchKernelGGL(k_run_boxes, dim3(grid_for(r1, 256)), dim3(256), 0, c.stream, edges, map,
-                       off, ring, b1, l1);
+    hipLaunchKernelGGL(k_run_boxes, dim3(grid_for(edges, 256)), dim3(256), 0, c.stream, edges,
+                       map, off, ring, b1, l1);
     hipLaunchKernelGGL(k_run_boxes_merge, dim3(grid_for(r2, 256)), dim3(256), 0, c.stream, r1,
                        (const double *)b1, (const double *)l1, b2, l2);
     hipLaunchKernelGGL(k_run_boxes_merge, dim3(grid_for(r3, 256)), dim3(256), 0, c.stream, r2,
@@ -5383,7 +5397,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
           const long r1 = (edgesQ + kRun1 - 1) / kRun1;
           double *b1 = wsT<double>(c, "gh_rb1Q", 4 * r1);
           double *l1 = wsT<double>(c, "gh_rl1Q", r1);
-          hipLaunchKernelGGL(k_run_boxes, dim3(grid_for(r1, 256)), dim3(256), 0, c.stream,
+          hipLaunchKernelGGL(k_run_boxes, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
                              edgesQ, mapQ, (const long long *)offQ, (const double *)ringQ, b1,
                              l1);
           HIP_CHECK(hipGetLastError());
@@ -5397,7 +5411,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
           LaunchTimer t(c, "gh_count");
           // the Q-ring staging slices: kStageQ points per wave
           const size_t stageBytes = c.tune.countStage ? (size_t)(cb / 64) * kStageQ * 16 : 0;
-          hipLaunchKernelGGL(cull ? k_count_cpu<true> : k_count_cpu<false>, dim3(grid), dim3(cb),
+          // (P edges only, so the wave walk's form applies whenever skipQ is given)
+          auto kern = cull ? (skipQ ? k_count_cpu<true, true> : k_count_cpu<true, false>)
+                           : (skipQ ? k_count_cpu<false, true> : k_count_cpu<false, false>);
+          hipLaunchKernelGGL(kern, dim3(grid), dim3(cb),
                              stageBytes, c.stream, edgesP, 0L, mapP, mapQ,
                              (const long long *)offP, (const long long *)offQ, ringP, ringQ,
                              (const double *)elenP, (const double *)emrgP,
@@ -5411,8 +5428,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     }
     scan_i32_to_i64_multi(c, {{cnt1P, psP1, edgesP}, {cnt2P, psP2, edgesP}});
     if (edgesP > 0)
-      list_runs(c, RunListSide{edgesP, mapP, (const long long *)psP2, (const long long *)psP1,
-                               listSP, listMP, listP, lenP, listOP, runCnt},
+      list_runs(c, RunListSide{edgesP, mapP, cnt2P, cnt1P, listSP, listMP, listP, lenP, listOP,
+                               runCnt},
                 RunListSide{0, mapQ, nullptr, nullptr, listSQ, listMQ, listQ, lenQ, listOQ,
                             runCnt + 8},
                 pm.status);
@@ -5483,8 +5500,8 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     if (edgesQ > 0)
       list_runs(c, RunListSide{0, mapP, nullptr, nullptr, listSP, listMP, listP, lenP, listOP,
                                runCnt},
-                RunListSide{edgesQ, mapQ, (const long long *)psQ2, (const long long *)psQ1,
-                            listSQ, listMQ, listQ, lenQ, listOQ, runCnt + 8},
+                RunListSide{edgesQ, mapQ, cnt2Q, cnt1Q, listSQ, listMQ, listQ, lenQ, listOQ,
+                            runCnt + 8},
                 pm.status);
     HIP_CHECK(hipGetLastError());
     const long long *nActQd = compact_flags_dev(c, cnt1Q, edgesQ, actQ, "actQ");
@@ -5533,10 +5550,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     HIP_CHECK(hipGetLastError());
     // edges whose alpha runs go through the segmented sort (statuses are final
     // here), read back with the slot totals
-    list_runs(c, RunListSide{edgesP, mapP, (const long long *)psP2, (const long long *)psP1,
-                             listSP, listMP, listP, lenP, listOP, runCnt},
-              RunListSide{edgesQ, mapQ, (const long long *)psQ2, (const long long *)psQ1,
-                          listSQ, listMQ, listQ, lenQ, listOQ, runCnt + 8},
+    list_runs(c, RunListSide{edgesP, mapP, cnt2P, cnt1P, listSP, listMP, listP, lenP, listOP,
+                             runCnt},
+              RunListSide{edgesQ, mapQ, cnt2Q, cnt1Q, listSQ, listMQ, listQ, lenQ, listOQ,
+                          runCnt + 8},
               pm.status);
     HIP_CHECK(hipGetLastError());
     // edges with intersections (count1 > 0): the later passes run over these
