@@ -1431,7 +1431,7 @@ __global__ void __launch_bounds__(256)
 
 // per-pair filter constants (cf[16 k + i])
 enum { CF_BOXP = 0, CF_BOXQ = 4, CF_MAXMP = 8, CF_MAXMQ = 9, CF_MINLP = 10, CF_MINLQ = 11,
-       CF_TINY = 12, CF_N = 16 };
+       CF_TINY = 12, CF_KA = 13, CF_KB = 14, CF_C0 = 15, CF_N = 16 };
 
 // The filter's constants from the two rings' boxes (normalised frame):
 // D is the diagonal of their union, dl and eA bound the rounding of the
@@ -1465,14 +1465,18 @@ __device__ __forceinline__ void write_cf(double *c, const double bx[2][4], const
   c[CF_MINLP] = minL[0];
   c[CF_MINLQ] = minL[1];
   c[CF_TINY] = 4.5 * (kGhEps + 2.0 * K.dl);
+  c[CF_KA] = K.kA;
+  c[CF_KB] = K.kB;
+  c[CF_C0] = K.c0;
 }
 
-// one group per pair: ring boxes, edge lengths and margins
+// one group per pair: ring boxes, then the shortest edge and the largest
+// margin of each ring (the edges' own lengths and margins are recomputed
+// where they are needed)
 template <int W>
 __global__ void __launch_bounds__(W == 64 ? 256 : W)
     k_cpu_filter_setup(long M, const int *status, const long long *offP, const long long *offQ,
-                       const double *ringP, const double *ringQ, double *lenP, double *mrgP,
-                       double *lenQ, double *mrgQ, double *cf) {
+                       const double *ringP, const double *ringQ, double *cf) {
   const long k = pgrp_pair<W>();
   if (k >= M || status[k] != GH_OK) return;
   const int lane = pgrp_lane<W>();
@@ -1500,13 +1504,10 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
   for (int side = 0; side < 2; side++) {
     const long long b = side ? offQ[k] : offP[k], n = (side ? offQ[k + 1] : offP[k + 1]) - b;
     const double *V = side ? ringQ : ringP;
-    double *len = side ? lenQ : lenP, *mrg = side ? mrgQ : mrgP;
     double mn = INFINITY, mx = 0.0;
     for (long long i = lane; i < n; i += W) {
       double L, m;
       edge_margin(K, ld(V, b + i), ld(V, b + (i + 1 == n ? 0 : i + 1)), side != 0, L, m);
-      len[b + i] = L;
-      mrg[b + i] = m;
       mn = fmin(mn, L);
       mx = fmax(mx, m);
     }
@@ -1519,11 +1520,11 @@ __global__ void __launch_bounds__(W == 64 ? 256 : W)
 
 // The same over `split` 1024-thread blocks per pair, for a few large pairs:
 // the ring boxes' parts (k_filter_box_parts), combined into the constants'
-// box fields (k_filter_box_combine); then each block writes its share of
-// the edges' lengths and margins with the parts of their minimum and
-// maximum (k_filter_edge_parts), which k_filter_finish combines into the
-// remaining constants.  Minima and maxima are exact in
-// any order, so the constants are those of k_cpu_filter_setup.
+// box fields (k_filter_box_combine); then each block takes its share of
+// the edges' lengths and margins to parts of their minimum and maximum
+// (k_filter_edge_parts), which k_filter_finish combines into the remaining
+// constants.  Minima and maxima are exact in any order, so the constants
+// are those of k_cpu_filter_setup.
 __global__ void __launch_bounds__(1024)
     k_filter_box_parts(long M, int split, const int *status, const long long *offP,
                        const long long *offQ, const double *ringP, const double *ringQ,
@@ -1592,8 +1593,7 @@ __device__ __forceinline__ void cf_boxes(const double *c, double bx[2][4]) {
 __global__ void __launch_bounds__(1024)
     k_filter_edge_parts(long M, int split, const int *status, const long long *offP,
                         const long long *offQ, const double *ringP, const double *ringQ,
-                        const double *cf, double *lenP, double *mrgP, double *lenQ,
-                        double *mrgQ, double *lenPart) {
+                        const double *cf, double *lenPart) {
   const long k = (long)(blockIdx.x / (unsigned)split);
   const int j = (int)(blockIdx.x % (unsigned)split);
   if (k >= M || status[k] != GH_OK) return;
@@ -1603,13 +1603,10 @@ __global__ void __launch_bounds__(1024)
   for (int side = 0; side < 2; side++) {
     const long long b = side ? offQ[k] : offP[k], n = (side ? offQ[k + 1] : offP[k + 1]) - b;
     const double *V = side ? ringQ : ringP;
-    double *len = side ? lenQ : lenP, *mrg = side ? mrgQ : mrgP;
     double mn = INFINITY, mx = 0.0;
     for (long long i = (long long)j * 1024 + threadIdx.x; i < n; i += (long long)split * 1024) {
       double L, m;
       edge_margin(K, ld(V, b + i), ld(V, b + (i + 1 == n ? 0 : i + 1)), side != 0, L, m);
-      len[b + i] = L;
-      mrg[b + i] = m;
       mn = fmin(mn, L);
       mx = fmax(mx, m);
     }
@@ -1654,8 +1651,10 @@ __device__ __forceinline__ double edge_gap(pt a1, pt a2, pt b1, pt b2) {
 
 // Per-edge view of the filter: an edge A of one ring against the edges of
 // the other ring (B).  skip() is the pair decision; the ring-level bound G
-// (A's margin plus the largest margin of B) rejects without loading B's
-// length and margin when no pair of A can be tiny.  A tiny pair whose boxes
+// (A's margin plus the largest margin of B) rejects without computing B's
+// length and margin when no pair of A can be tiny.  An edge's length and
+// margin are computed from its vertices where they are needed, with the
+// same operations as the set-up kernels that bound them per ring.  A tiny pair whose boxes
 // are apart is still skipped when |A(P1, Q1, Q2)| >= eps: intersect() then
 // cannot take its collinear branch (the value is computed as intersect()
 // computes it), and the other types are bounded by the margins.
@@ -1664,7 +1663,7 @@ struct CpuFilter {
   bool tiny;   // some pair of A may have L_A L_B < T
   bool sideP;  // A is a P edge
   EdgeBox wide;  // A's box widened by G (outward), for the plain-compare reject
-  const double *lenB, *mrgB;
+  const double *c;  // the pair's constants
   __device__ __forceinline__ bool skip(int q, pt A1, pt A2, pt B1, pt B2) const {
     // both of B's endpoints beyond one side of the widened box: the gap
     // exceeds G, and no pair of A is tiny
@@ -1672,28 +1671,27 @@ struct CpuFilter {
     const double gap = edge_gap(A1, A2, B1, B2);
     if (!(gap > 0.0)) return false;
     if (!tiny && gap > G) return true;
-    if (!(gap > ma + mrgB[q])) return false;
-    if (!(la * lenB[q] < T)) return true;
+    double lb, mb;
+    edge_margin(FilterK{c[CF_KA], c[CF_KB], c[CF_C0], 0.0}, B1, B2, sideP, lb, mb);
+    if (!(gap > ma + mb)) return false;
+    if (!(la * lb < T)) return true;
     const double AP1 = sideP ? triA(A1, B1, B2) : triA(B1, A1, A2);
     return !(fabs(AP1) < kGhEps);
   }
 };
 
-__device__ __forceinline__ CpuFilter cpu_filter(const double *cf, int k, bool sideP, long long ga,
-                                                long long bBase, const double *lenP,
-                                                const double *mrgP, const double *lenQ,
-                                                const double *mrgQ) {
+// the filter of edge A1 A2 of pair k (a P edge when sideP)
+__device__ __forceinline__ CpuFilter cpu_filter(const double *cf, int k, bool sideP, pt A1,
+                                                pt A2) {
   const double *c = cf + (long long)CF_N * k;
   CpuFilter f;
-  f.la = sideP ? lenP[ga] : lenQ[ga];
-  f.ma = sideP ? mrgP[ga] : mrgQ[ga];
+  edge_margin(FilterK{c[CF_KA], c[CF_KB], c[CF_C0], 0.0}, A1, A2, !sideP, f.la, f.ma);
   f.T = c[CF_TINY];
   f.G = f.ma + (sideP ? c[CF_MAXMQ] : c[CF_MAXMP]);
   f.tiny = f.la * (sideP ? c[CF_MINLQ] : c[CF_MINLP]) < f.T;
   f.sideP = sideP;
   f.wide = EdgeBox{0.0, 0.0, 0.0, 0.0};
-  f.lenB = (sideP ? lenQ : lenP) + bBase;
-  f.mrgB = (sideP ? mrgQ : mrgP) + bBase;
+  f.c = c;
   return f;
 }
 
@@ -1737,8 +1735,6 @@ __device__ __forceinline__ CountLane count_prep(long g, long edgesP, long edgesQ
                                                 const int *mapP, const int *mapQ,
                                                 const long long *offP, const long long *offQ,
                                                 const double *ringP, const double *ringQ,
-                                                const double *lenP, const double *mrgP,
-                                                const double *lenQ, const double *mrgQ,
                                                 const double *cf, long cullMin, int ldsQ) {
   CountLane L;
   L.live = false;
@@ -1763,7 +1759,7 @@ __device__ __forceinline__ CountLane count_prep(long g, long edgesP, long edgesQ
   const long long i = ga - aBase;
   L.A1 = ld(A, ga);
   L.A2 = ld(A, i == aN - 1 ? aBase : ga + 1);
-  L.f = cpu_filter(cf, k, sideP, ga, bBase, lenP, mrgP, lenQ, mrgQ);
+  L.f = cpu_filter(cf, k, sideP, L.A1, L.A2);
   L.ab = edge_box(L.A1, L.A2);
   {
     // G widened by a relative 1e-12 so that rounding of the box sums cannot
@@ -1959,15 +1955,12 @@ __device__ __forceinline__ void count_edge_wave(const CountLane &L, bool group,
 // group per pair, leaving out the runs no lane of the group can pair with
 // (count_edge_wave).  WAVE: skipQ is given and every edge is a P edge
 // (edgesQ = 0), so that without culling no lane walks on its own and the
-// lane walk is left out of the kernel.  (At five waves per SIMD it spills
-// 12 bytes per lane; four waves without spills measured slower on the join
-// workload, 0.398 against 0.38 ms.)
+// lane walk is left out of the kernel.
 template <bool CULL, bool WAVE>
 __global__ void __launch_bounds__(256, CULL ? 1 : 5)
     k_count_cpu(long edgesP, long edgesQ, const int *mapP, const int *mapQ,
                 const long long *offP, const long long *offQ, const double *ringP,
-                const double *ringQ, const double *lenP, const double *mrgP, const double *lenQ,
-                const double *mrgQ, const double *cf, int *cnt1P, int *cnt2P, int *cnt1Q,
+                const double *ringQ, const double *cf, int *cnt1P, int *cnt2P, int *cnt1Q,
                 int *cnt2Q, int *boolP, int *boolQ, unsigned long long *wstat, int *lstP,
                 int *lstQ, RunBoxes boxP, RunBoxes boxQ, int ldsQ, int stageOk,
                 const double *skipQ) {
@@ -1998,8 +1991,8 @@ __global__ void __launch_bounds__(256, CULL ? 1 : 5)
     }
     __syncthreads();
   }
-  const CountLane L = count_prep(g, edgesP, edgesQ, mapP, mapQ, offP, offQ, ringP, ringQ, lenP,
-                                 mrgP, lenQ, mrgQ, cf, boxP.cullMin, ldsQ);
+  const CountLane L = count_prep(g, edgesP, edgesQ, mapP, mapQ, offP, offQ, ringP, ringQ, cf,
+                                 boxP.cullMin, ldsQ);
   // P edges walked over their whole Q ring: one group per pair of the wave
   bool pending = WAVE && L.live && L.sideP && !(CULL && L.bN > boxP.cullMin);
   if (WAVE) {
@@ -2071,8 +2064,7 @@ __host__ __device__ constexpr size_t lds_count_bytes(int capQ) {
 
 __global__ void __launch_bounds__(kLdsBlock)
     k_count_cpu_lds(long M, int minQ, int capQ, const long long *offP, const long long *offQ,
-                    const double *ringP, const double *ringQ, const double *lenP,
-                    const double *mrgP, const double *lenQ, const double *mrgQ, const double *cf,
+                    const double *ringP, const double *ringQ, const double *cf,
                     int *cnt1P, int *cnt2P, int *boolP, int *lstP, unsigned long long *bstat) {
   extern __shared__ double lds_count[];
   pt *sq = (pt *)lds_count;
@@ -2153,7 +2145,7 @@ __global__ void __launch_bounds__(kLdsBlock)
     for (long long i = threadIdx.x; i < nP; i += kLdsBlock) {
       const long long ga = pBase + i;
       const pt A1 = ld(ringP, ga), A2 = ld(ringP, i == nP - 1 ? pBase : ga + 1);
-      CpuFilter f = cpu_filter(cf, (int)k, true, ga, qBase, lenP, mrgP, lenQ, mrgQ);
+      CpuFilter f = cpu_filter(cf, (int)k, true, A1, A2);
       const EdgeBox ab = edge_box(A1, A2);
       {
         const double Gw = f.G * (1.0 + 1e-12) + 1e-300;
@@ -2249,8 +2241,7 @@ __global__ void __launch_bounds__(64)
     k_partner_rescan_cpu(long nOver, const int *listO, bool sideP, const int *mapA,
                          const long long *offA, const long long *offB, const double *ringA,
                          const double *ringB, const long long *psA1, int *lst,
-                         const double *lenP, const double *mrgP, const double *lenQ,
-                         const double *mrgQ, const double *cf, RunBoxes rb) {
+                         const double *cf, RunBoxes rb) {
   for (long w = blockIdx.x; w < nOver; w += gridDim.x) {
     const long g = listO[w];
     const int k = mapA[g];
@@ -2260,7 +2251,7 @@ __global__ void __launch_bounds__(64)
     const long long bBase = offB[k], bN = offB[k + 1] - bBase;
     const long long i = g - aBase;
     const pt A1 = ld(ringA, g), A2 = ld(ringA, i == aN - 1 ? aBase : g + 1);
-    CpuFilter f = cpu_filter(cf, k, sideP, g, bBase, lenP, mrgP, lenQ, mrgQ);
+    CpuFilter f = cpu_filter(cf, k, sideP, A1, A2);
     {
       const EdgeBox ab = edge_box(A1, A2);
       const double Gw = f.G * (1.0 + 1e-12) + 1e-300;
@@ -2345,14 +2336,13 @@ __global__ void __launch_bounds__(256)
 // the P ring's box), for a batch whose Q side is built by transposition.
 __global__ void __launch_bounds__(256)
     k_cpu_flags_q(long edgesQ, const int *mapQ, const long long *offP, const long long *offQ,
-                  const double *ringQ, const double *lenP, const double *mrgP,
-                  const double *lenQ, const double *mrgQ, const double *cf, int *boolQ) {
+                  const double *ringQ, const double *cf, int *boolQ) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= edgesQ) return;
   const int k = mapQ[g];
   const long long aBase = offQ[k], aN = offQ[k + 1] - aBase;
   const pt A1 = ld(ringQ, g), A2 = ld(ringQ, g - aBase == aN - 1 ? aBase : g + 1);
-  const CpuFilter f = cpu_filter(cf, k, false, g, offP[k], lenP, mrgP, lenQ, mrgQ);
+  const CpuFilter f = cpu_filter(cf, k, false, A1, A2);
   const double *ob = cf + (long long)CF_N * k + CF_BOXP;
   const EdgeBox ab = edge_box(A1, A2);
   const double ogap = fmax(fmax(ob[0] - ab.x1, ab.x0 - ob[2]), fmax(ob[1] - ab.y1, ab.y0 - ob[3]));
@@ -5275,14 +5265,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   // phase 1 of the CPU route (computeIntersections) or of the GPU route
   // (calculateIntersections, whose phase-1 arrays the view then holds)
   const bool cpuRoute = opt.route != 0;
-  double *elenP = nullptr, *emrgP = nullptr, *elenQ = nullptr, *emrgQ = nullptr, *cf = nullptr;
+  double *cf = nullptr;
   if (cpuRoute) {
     if (maxRingP >= (1L << 27) || maxRingQ >= (1L << 27))
       throw Error{GCMF_ERR_OVERFLOW, "CPU-route clip takes rings of fewer than 2^27 vertices"};
-    elenP = wsT<double>(c, "gh_elenP", edgesP);
-    emrgP = wsT<double>(c, "gh_emrgP", edgesP);
-    elenQ = wsT<double>(c, "gh_elenQ", edgesQ);
-    emrgQ = wsT<double>(c, "gh_emrgQ", edgesQ);
     cf = wsT<double>(c, "gh_cf", (long long)CF_N * M);
     const int split = pgrp_split(M, pw);
     if (split > 1) {
@@ -5300,7 +5286,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
       hipLaunchKernelGGL(k_filter_edge_parts, dim3((unsigned)(M * split)), dim3(1024), 0,
                          c.stream, M, split, (const int *)pm.status, (const long long *)offP,
                          (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
-                         (const double *)cf, elenP, emrgP, elenQ, emrgQ, lenPart);
+                         (const double *)cf, lenPart);
       HIP_CHECK(hipGetLastError());
       hipLaunchKernelGGL(k_filter_finish, dim3((unsigned)M), dim3(256), 0, c.stream, M, split,
                          (const int *)pm.status, (const double *)lenPart, cf);
@@ -5309,7 +5295,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          dim3(pgrp_grid(M, pw)), dim3(pgrp_block(pw)), 0, c.stream, M,
                          (const int *)pm.status, (const long long *)offP,
                          (const long long *)offQ, (const double *)ringP, (const double *)ringQ,
-                         elenP, emrgP, elenQ, emrgQ, cf);
+                         cf);
     }
     HIP_CHECK(hipGetLastError());
   }
@@ -5381,8 +5367,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                              lds_count_bytes(capQ), c.stream, M, minQ, capQ,
                              (const long long *)offP,
                              (const long long *)offQ, (const double *)ringP,
-                             (const double *)ringQ, (const double *)elenP, (const double *)emrgP,
-                             (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
+                             (const double *)ringQ, (const double *)cf,
                              cnt1P, cnt2P, boolP, lstP, bstat);
           HIP_CHECK(hipGetLastError());
         }
@@ -5417,8 +5402,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
           hipLaunchKernelGGL(kern, dim3(grid), dim3(cb),
                              stageBytes, c.stream, edgesP, 0L, mapP, mapQ,
                              (const long long *)offP, (const long long *)offQ, ringP, ringQ,
-                             (const double *)elenP, (const double *)emrgP,
-                             (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
+                             (const double *)cf,
                              cnt1P, cnt2P, cnt1Q, cnt2Q, boolP, boolQ, wstat, lstP, lstQ, boxP,
                              boxQ, capQ > minQ ? capQ : 0, c.tune.countStage, skipQ);
           HIP_CHECK(hipGetLastError());
@@ -5463,8 +5447,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          dim3(64), 0, c.stream, nOverP0, (const int *)listOP, true, mapP,
                          (const long long *)offP, (const long long *)offQ,
                          (const double *)ringP, (const double *)ringQ, (const long long *)psP1,
-                         lstAllP, (const double *)elenP, (const double *)emrgP,
-                         (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
+                         lstAllP, (const double *)cf,
                          boxQ);
     HIP_CHECK(hipGetLastError());
     // -- the Q side by transposition
@@ -5473,8 +5456,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     if (edgesQ > 0)
       hipLaunchKernelGGL(k_cpu_flags_q, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
                          edgesQ, mapQ, (const long long *)offP, (const long long *)offQ,
-                         (const double *)ringQ, (const double *)elenP, (const double *)emrgP,
-                         (const double *)elenQ, (const double *)emrgQ, (const double *)cf,
+                         (const double *)ringQ, (const double *)cf,
                          boolQ);
     uint64_t *tk = wsT<uint64_t>(c, "gh_trK", H), *tk2 = wsT<uint64_t>(c, "gh_trK2", H);
     int *tv = wsT<int>(c, "gh_trV", H);
@@ -5588,8 +5570,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          c.stream, nOverP, (const int *)listOP, true, mapP,
                          (const long long *)offP, (const long long *)offQ, (const double *)ringP,
                          (const double *)ringQ, (const long long *)psP1, lstAllP,
-                         (const double *)elenP, (const double *)emrgP, (const double *)elenQ,
-                         (const double *)emrgQ, (const double *)cf, boxQ);
+                         (const double *)cf, boxQ);
     else if (nOverP > 0)
       hipLaunchKernelGGL(k_partner_rescan, dim3(nOverP < 65536 ? nOverP : 65536), dim3(64), 0,
                          c.stream, nOverP, (const int *)listOP, mapP, (const long long *)offP,
@@ -5600,8 +5581,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          c.stream, nOverQ, (const int *)listOQ, false, mapQ,
                          (const long long *)offQ, (const long long *)offP, (const double *)ringQ,
                          (const double *)ringP, (const long long *)psQ1, lstAllQ,
-                         (const double *)elenP, (const double *)emrgP, (const double *)elenQ,
-                         (const double *)emrgQ, (const double *)cf, boxP);
+                         (const double *)cf, boxP);
     else if (nOverQ > 0)
       hipLaunchKernelGGL(k_partner_rescan, dim3(nOverQ < 65536 ? nOverQ : 65536), dim3(64), 0,
                          c.stream, nOverQ, (const int *)listOQ, mapQ, (const long long *)offQ,
