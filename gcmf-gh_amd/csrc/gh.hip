@@ -2498,10 +2498,11 @@ __global__ void __launch_bounds__(256)
   if (l < m) cpu_move<QSIDE>(s, cs, lo + l, lo + d);
 }
 
-// runs of kSmallSort < m <= kBlockSort slots: one block per run, a bitonic
-// sort in LDS of (key, slot) pairs padded to a power of two (pads take the
+// runs of kSmallSort < m <= kBlockSort slots: one block per run from LDS.
+// Up to 128 slots each slot's rank is counted directly; longer runs take a
+// bitonic sort of (key, slot) pairs padded to a power of two (pads take the
 // largest key and slots past the run, so they sort last); the pairs are
-// distinct, so the order is the stable one by key
+// distinct, so the order is the stable one by key either way
 template <bool QSIDE>
 __global__ void __launch_bounds__(256)
     k_cpu_sort_block(int nRuns, const int *list, const long long *ps2, SlotArrays s,
@@ -2513,9 +2514,23 @@ __global__ void __launch_bounds__(256)
     const long g = list[r];
     const long long lo = ps2[g] + 1;
     const int m = (int)(ps2[g + 1] - lo);
-    int P = 64;
-    while (P < m) P <<= 1;
     __syncthreads();
+    if (m <= 128) {
+      for (int t = threadIdx.x; t < m; t += blockDim.x) key[t] = k2[lo + t];
+      __syncthreads();
+      for (int t = threadIdx.x; t < m; t += blockDim.x) {
+        const uint64_t kt = key[t];
+        int d = 0;
+        for (int i = 0; i < m; i++) {
+          const uint64_t ki = key[i];
+          d += (ki < kt) || (ki == kt && i < t);
+        }
+        cpu_move<QSIDE>(s, cs, lo + t, lo + d);
+      }
+      continue;
+    }
+    int P = 256;
+    while (P < m) P <<= 1;
     for (int t = threadIdx.x; t < P; t += blockDim.x) {
       key[t] = t < m ? k2[lo + t] : ~0ULL;
       slot[t] = t;
