@@ -1043,15 +1043,35 @@ __global__ void __launch_bounds__(kSortThreads)
 
 // Stable scatter: wave w ranks its 1024 keys in 16 rounds of 64 in order;
 // position = tile digit offset + earlier waves' counts + rank within wave.
+// The tile digit offsets come from off (the exclusive scan of the
+// digit-major histograms) or, for sorts of at most kSortFuseTiles tiles,
+// from the histograms themselves (hist): every block adds up its digit's
+// row and scans the digits' totals, which saves the scan's launch.
+constexpr long kSortFuseTiles = 64;
 __global__ void __launch_bounds__(kSortThreads)
     k_sort_scatter(const uint64_t *keys, const int *vals, long n, int shift,
-                   const long long *off, long tiles, uint64_t *keys_out, int *vals_out) {
+                   const long long *off, const int *hist, long tiles, uint64_t *keys_out,
+                   int *vals_out) {
   __shared__ int wcnt[kSortThreads / 64][256];
+  __shared__ long long lds_hist[kSortThreads / 64];
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const long T = xcd_tile(blockIdx.x, tiles);
   // this tile's output start for digit threadIdx.x (256 threads, 256
   // digits), fetched now so that its latency hides behind the ranking
-  const long long dOff = off[(long)threadIdx.x * tiles + T];
+  long long dOff;
+  if (hist) {
+    const int *row = hist + (long)threadIdx.x * tiles;
+    long long tot = 0, pre = 0;
+    for (long t = 0; t < tiles; t++) {
+      const int h = row[t];
+      tot += h;
+      pre += t < T ? h : 0;
+    }
+    long long all;
+    dOff = block_excl_scan<long long>(tot, lds_hist, all) + pre;
+  } else {
+    dOff = off[(long)threadIdx.x * tiles + T];
+  }
   for (int d = threadIdx.x; d < 256 * (kSortThreads / 64); d += kSortThreads)
     (&wcnt[0][0])[d] = 0;
   const long wbase = T * kSortTile + (long)wave * kWaveKeys;
@@ -1164,11 +1184,13 @@ void radix_sort_u64(Context &c, const uint64_t *keys_in, const int *vals_in, uin
                          src_k, n, 8 * p, hist, tiles);
       HIP_CHECK(hipGetLastError());
     }
-    scan_i32_to_i64(c, hist, off, 256 * tiles);
+    const bool fuse = tiles <= kSortFuseTiles;
+    if (!fuse) scan_i32_to_i64(c, hist, off, 256 * tiles);
     {
       LaunchTimer t(c, "sort_scatter");
       hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)tiles), dim3(kSortThreads), 0, c.stream,
-                         src_k, src_v, n, 8 * p, off, tiles, dk, dv);
+                         src_k, src_v, n, 8 * p, (const long long *)off,
+                         fuse ? (const int *)hist : nullptr, tiles, dk, dv);
       HIP_CHECK(hipGetLastError());
     }
     src_k = dk;
@@ -1264,7 +1286,8 @@ void radix_sort_u64_noalloc(Context &c, const uint64_t *keys_in, const int *vals
                          (const long long *)parts, st);
     }
     hipLaunchKernelGGL(k_sort_scatter, dim3((unsigned)tiles), dim3(kSortThreads), 0, c.stream,
-                       src_k, src_v, n, 8 * p, (const long long *)off, tiles, dk, dv);
+                       src_k, src_v, n, 8 * p, (const long long *)off, (const int *)nullptr,
+                       tiles, dk, dv);
     HIP_CHECK(hipGetLastError());
     src_k = dk;
     src_v = dv;
