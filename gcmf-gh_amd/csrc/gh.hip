@@ -2336,9 +2336,12 @@ __global__ void __launch_bounds__(256)
 // the P ring's box), for a batch whose Q side is built by transposition.
 __global__ void __launch_bounds__(256)
     k_cpu_flags_q(long edgesQ, const int *mapQ, const long long *offP, const long long *offQ,
-                  const double *ringQ, const double *cf, int *boolQ) {
+                  const double *ringQ, const double *cf, int *boolQ, int *cnt1Q, int *cnt2Q) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= edgesQ) return;
+  // the transposition's counters start here: count1 0, count2 1 (the parent)
+  cnt1Q[g] = 0;
+  cnt2Q[g] = 1;
   const int k = mapQ[g];
   const long long aBase = offQ[k], aN = offQ[k + 1] - aBase;
   const pt A1 = ld(ringQ, g), A2 = ld(ringQ, g - aBase == aN - 1 ? aBase : g + 1);
@@ -5466,13 +5469,10 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
                          boxQ);
     HIP_CHECK(hipGetLastError());
     // -- the Q side by transposition
-    fill_i32(c, cnt1Q, 0, edgesQ);
-    fill_i32(c, cnt2Q, 1, edgesQ);
     if (edgesQ > 0)
       hipLaunchKernelGGL(k_cpu_flags_q, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
                          edgesQ, mapQ, (const long long *)offP, (const long long *)offQ,
-                         (const double *)ringQ, (const double *)cf,
-                         boolQ);
+                         (const double *)ringQ, (const double *)cf, boolQ, cnt1Q, cnt2Q);
     uint64_t *tk = wsT<uint64_t>(c, "gh_trK", H), *tk2 = wsT<uint64_t>(c, "gh_trK2", H);
     int *tv = wsT<int>(c, "gh_trV", H);
     lstAllQ = wsT<int>(c, "gh_lstAllQ", H);
