@@ -896,6 +896,27 @@ __global__ void k_q_parents(long edgesQ, const int *mapQ, const long long *offQ,
   st(s.oQ, sl, ld(cQ, pm.startQ[k] + i));
 }
 
+// k_p_parents and k_q_parents in one launch: threads [0, edgesP) take the P
+// edges, the rest the Q edges
+__global__ void k_parents(long edgesP, long edgesQ, const int *mapP, const int *mapQ,
+                          const long long *offP, const long long *offQ, const double *ringP,
+                          const double *ringQ, const long long *psP2, const long long *psQ2,
+                          const int *status, PairMeta pm, const double *bCoords,
+                          const double *oCoords, SlotArrays s) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= edgesP + edgesQ) return;
+  const bool sideP = t < edgesP;
+  const long g = sideP ? t : t - edgesP;
+  const int k = (sideP ? mapP : mapQ)[g];
+  if (status[k] != GH_OK) return;
+  const long long i = g - (sideP ? offP : offQ)[k];
+  // P's coordinates are the overlay's when the pair is swapped, Q's the base's
+  const double *cA = (pm.swapped[k] != 0) == sideP ? oCoords : bCoords;
+  const long long sl = (sideP ? psP2 : psQ2)[g];
+  st(sideP ? s.iP : s.iQ, sl, ld(sideP ? ringP : ringQ, g));
+  st(sideP ? s.oP : s.oQ, sl, ld(cA, (sideP ? pm.startP[k] : pm.startQ[k]) + i));
+}
+
 // Stable order of keys[lo..hi) by value: dest[j] = #{i : k_i < k_j} +
 // #{i < j : k_i == k_j}, which equals the reference's 7-digit decimal LSD
 // counting sort (:263-305) on these non-negative keys.
@@ -5650,14 +5671,11 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
     cs.dQ = wsT<int>(c, "gh_dQ", slotsQ);
     cs.srcP = wsT<int>(c, "gh_srcP", edgesP);
     cs.srcQ = wsT<int>(c, "gh_srcQ", edgesQ);
-    if (edgesQ > 0)
-      hipLaunchKernelGGL(k_q_parents, dim3(grid_for(edgesQ, 256)), dim3(256), 0, c.stream,
-                         edgesQ, mapQ, (const long long *)offQ, ringQ, (const long long *)psQ2,
-                         pm.status, pm, in.bCoords, in.oCoords, s);
-    if (edgesP > 0)
-      hipLaunchKernelGGL(k_p_parents, dim3(grid_for(edgesP, 256)), dim3(256), 0, c.stream,
-                         edgesP, mapP, (const long long *)offP, ringP, (const long long *)psP2,
-                         pm.status, pm, in.bCoords, in.oCoords, s);
+    if (edgesP + edgesQ > 0)
+      hipLaunchKernelGGL(k_parents, dim3(grid_for(edgesP + edgesQ, 256)), dim3(256), 0,
+                         c.stream, edgesP, edgesQ, mapP, mapQ, (const long long *)offP,
+                         (const long long *)offQ, ringP, ringQ, (const long long *)psP2,
+                         (const long long *)psQ2, pm.status, pm, in.bCoords, in.oCoords, s);
     HIP_CHECK(hipGetLastError());
     const long nAct = nActP + nActQ;
     RunLists runLists{{listMP, listP, listMQ, listQ}, {nMedP, nRunsP, nMedQ, nRunsQ}};
