@@ -2475,23 +2475,23 @@ __global__ void __launch_bounds__(256)
       vo = ld(cB, sB + q);
     }
     if (u > base && u <= base + nIns) {
-      st(i2, u, v);
-      st(o2, u, vo);
-      k2[u] = dbits(sideP ? al : be);
       r2[u] = (q << 4) | t;
+      if (nIns == 1) {
+        // a run of one is in place: no sort reads its key or staged copy
+        st(sideP ? s.iP : s.iQ, u, v);
+        st(sideP ? s.oP : s.oQ, u, vo);
+        (sideP ? cs.rP : cs.rQ)[u] = (q << 4) | t;
+        (sideP ? cs.dP : cs.dQ)[u] = (int)u;
+      } else {
+        st(i2, u, v);
+        st(o2, u, vo);
+        k2[u] = dbits(sideP ? al : be);
+      }
     }
   };
   listed_edges(ringB + 2 * bBase, (int)bN, (sideP ? lstP : lstQ) + ps1[g],
                (int)(ps1[g + 1] - ps1[g]), visit);
   (sideP ? cs.srcP : cs.srcQ)[g] = src;
-  if (nIns == 1) {
-    // a run of one is in place
-    const long long u = base + 1;
-    st(sideP ? s.iP : s.iQ, u, ld(i2, u));
-    st(sideP ? s.oP : s.oQ, u, ld(o2, u));
-    (sideP ? cs.rP : cs.rQ)[u] = r2[u];
-    (sideP ? cs.dP : cs.dQ)[u] = (int)u;
-  }
 }
 
 template <bool QSIDE>
