@@ -860,22 +860,29 @@ struct SlotArrays {
   int *lP, *lQ;                   // initial labels
 };
 
-__global__ void k_init_slots(long slotsP, long slotsQ, SlotArrays s, int *nmap) {
+// gpuRoute: also the arrays only the GPU route reads (its sort sources aP2,
+// aQ2, nbP2, nbQ2; the neighbour map is exported with the phase-1 arrays,
+// so it is cleared on both routes)
+__global__ void k_init_slots(long slotsP, long slotsQ, SlotArrays s, int *nmap, int gpuRoute) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < slotsP) {
     s.aP[i] = kNone;
-    s.aP2[i] = kNone;
     s.nbP[i] = 0;
-    s.nbP2[i] = 0;
     s.lP[i] = kNone;
+    if (gpuRoute) {
+      s.aP2[i] = kNone;
+      s.nbP2[i] = 0;
+    }
   }
   if (i < slotsQ) {
     s.aQ[i] = kNone;
-    s.aQ2[i] = kNone;
     s.nbQ[i] = 0;
-    s.nbQ2[i] = 0;
     s.lQ[i] = kNone;
     nmap[i] = kNone;
+    if (gpuRoute) {
+      s.aQ2[i] = kNone;
+      s.nbQ2[i] = 0;
+    }
   }
 }
 
@@ -5656,7 +5663,7 @@ void clip_pairs(Context &c, const JoinInputs &in, const int *pairs, long M,
   const long slotsMax = slotsP > slotsQ ? slotsP : slotsQ;
   if (slotsMax > 0) {
     hipLaunchKernelGGL(k_init_slots, dim3(grid_for(slotsMax, 256)), dim3(256), 0, c.stream,
-                       slotsP, slotsQ, s, nmap);
+                       slotsP, slotsQ, s, nmap, cpuRoute ? 0 : 1);
     HIP_CHECK(hipGetLastError());
   }
   if (cpuRoute) {
