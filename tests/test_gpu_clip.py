@@ -309,8 +309,13 @@ def test_clip_area_oracle(gctx):
 
 
 def test_clip_batch_independence(gctx):
-    """Each pair's result is the same alone and inside a mixed batch."""
+    """Each pair's result is the same alone and inside a mixed batch.  Calls
+    of at most 32 pairs run the per-pair set-up with 1024 threads per pair,
+    split over several blocks (512 for one pair, 17 for 30); 40 pairs run it
+    one wave per pair: the results and the number of edge pairs the
+    prefilter lets through (which follows the filter constants) agree."""
     cases = random_cases(30, seed0=500, nmax=300)
+    extra = random_cases(10, seed0=700, nmax=300)
     st = run_batch(gctx, cases, G.CLIP_NORMALIZE)
     _, mixed = gctx.clip_results(st)
     for k in (0, 7, 29):
@@ -318,6 +323,15 @@ def test_clip_batch_independence(gctx):
         _, alone = gctx.clip_results(st1)
         assert len(alone[0]) == len(mixed[k])
         for a, b in zip(alone[0], mixed[k]):
+            np.testing.assert_array_equal(a, b)
+    st_x = run_batch(gctx, extra, G.CLIP_NORMALIZE)
+    st40 = run_batch(gctx, cases + extra, G.CLIP_NORMALIZE)
+    _, wide = gctx.clip_results(st40)
+    assert st40["n_eval"] == st["n_eval"] + st_x["n_eval"]
+    assert st40["n_intersections"] == st["n_intersections"] + st_x["n_intersections"]
+    for k in range(30):
+        assert len(wide[k]) == len(mixed[k])
+        for a, b in zip(wide[k], mixed[k]):
             np.testing.assert_array_equal(a, b)
 
 
