@@ -86,6 +86,7 @@ struct Context {
   std::string last_error;
   int cu_count = 256;
   long long *hbuf = nullptr;  // pinned host staging for small read-backs (kHostSlots)
+  long long *hbuf_dev = nullptr;  // its device address (mapped, coherent)
   // device address of the allocation-free sort's static scratch on this
   // context's device (noalloc_scratch_address, looked up by gcmf_init)
   void *scratch = nullptr;

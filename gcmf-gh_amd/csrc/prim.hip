@@ -114,6 +114,22 @@ struct ScalarVals {
   long long v[kHostSlots];
 };
 
+// the pinned staging buffer for small read-backs, mapped into the device's
+// address space and coherent, so that a kernel stores the values straight
+// into host memory (no copy launch); returns its device address
+static long long *host_slots(Context &c) {
+  if (!c.hbuf) {
+    HIP_CHECK(hipHostMalloc((void **)&c.hbuf, sizeof(long long) * kHostSlots,
+                            hipHostMallocMapped | hipHostMallocCoherent));
+    HIP_CHECK(hipHostGetDevicePointer((void **)&c.hbuf_dev, c.hbuf, 0));
+  }
+  return c.hbuf_dev;
+}
+
+__global__ void k_copy_small(const unsigned char *src, int bytes, unsigned char *dst) {
+  for (int i = threadIdx.x; i < bytes; i += blockDim.x) dst[i] = src[i];
+}
+
 // sets *flag when a source's value differs from the recorded one
 __global__ void k_check_scalars(ScalarSrc s, int n, ScalarVals want, int *flag) {
   const int i = threadIdx.x;
@@ -147,8 +163,17 @@ void host_read(Context &c, void *dst, const void *src, size_t bytes) {
     void *snap = c.ws.get("replay_snap_" + std::to_string(c.rec_bytes.size()), bytes);
     HIP_CHECK(hipMemcpyAsync(snap, src, bytes, hipMemcpyDeviceToDevice, c.stream));
   }
-  HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipStreamSynchronize(c.stream));
+  if (bytes <= sizeof(long long) * kHostSlots) {
+    // a few bytes: stored into the mapped staging buffer by a kernel
+    hipLaunchKernelGGL(k_copy_small, dim3(1), dim3(64), 0, c.stream,
+                       (const unsigned char *)src, (int)bytes, (unsigned char *)host_slots(c));
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    std::memcpy(dst, c.hbuf, bytes);
+  } else {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+  }
   if (c.replay == 1) {
     const char *p = (const char *)dst;
     c.rec_bytes.emplace_back(p, p + bytes);
@@ -174,12 +199,9 @@ void read_scalars(Context &c, std::initializer_list<const long long *> src, long
     HIP_CHECK(hipGetLastError());
     return;
   }
-  long long *d = (long long *)c.ws.get("read_scalars", sizeof(long long) * kHostSlots);
-  hipLaunchKernelGGL(k_gather_scalars, dim3(1), dim3(64), 0, c.stream, s, n + 1, d);
+  // gathered straight into the mapped staging buffer
+  hipLaunchKernelGGL(k_gather_scalars, dim3(1), dim3(64), 0, c.stream, s, n + 1, host_slots(c));
   HIP_CHECK(hipGetLastError());
-  if (!c.hbuf) HIP_CHECK(hipHostMalloc((void **)&c.hbuf, sizeof(long long) * kHostSlots));
-  HIP_CHECK(hipMemcpyAsync(c.hbuf, d, sizeof(long long) * (size_t)(n + 1), hipMemcpyDeviceToHost,
-                           c.stream));
   HIP_CHECK(hipStreamSynchronize(c.stream));
   for (int j = 0; j < n; j++) out[j] = c.hbuf[j];
   if (c.replay == 1) c.rec.insert(c.rec.end(), out, out + n);
